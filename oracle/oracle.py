@@ -1,0 +1,317 @@
+"""ctypes wrappers for the checkers: the C restatement (liboracle.so) and, where it
+was built, the compiled reference (libraysht_ref.so).
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module, and only to check or time the reference
+path; the product never does.
+
+Layouts are the reference's (see raysht_oracle.h):
+  spectral  packed over mp in m_values order, block mp = data(m:lmax, nr, 2, nf), l fastest
+  hybrid    p2[mp][c][theta] with c = r + imi*nr + 2*nr*f
+  physical  p3[f][theta][r][nphi+2]
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+ORACLE_SO = os.path.join(REF_DIR, "liboracle.so")
+REF_SO = os.path.join(REF_DIR, "libraysht_ref.so")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_L = ctypes.c_long
+_D = ctypes.c_double
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_P)
+
+
+def build_oracle(with_reference: bool = False) -> None:
+    """Compile the C restatement (and the compiled reference when asked and its sources exist)."""
+    targets = ["oracle"]
+    if with_reference and os.path.isdir("/root/reference/src"):
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle_lib():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        lib = ctypes.CDLL(ORACLE_SO)
+        lib.orc_gauss_legendre.argtypes = [_I, _P, _P, _P, _P]
+        lib.orc_table_size_even.restype = _L
+        lib.orc_table_size_even.argtypes = [_I, _I, _P, _I]
+        lib.orc_table_size_odd.restype = _L
+        lib.orc_table_size_odd.argtypes = [_I, _I, _P, _I]
+        lib.orc_legendre_tables.restype = _I
+        lib.orc_legendre_tables.argtypes = [_I, _I, _P, _I, _P, _P, _P, _P]
+        lib.orc_stp.argtypes = [_I, _I, _P, _I, _I, _I, _P, _P, _P, _P]
+        lib.orc_pts.argtypes = [_I, _I, _P, _I, _I, _I, _P, _P, _P, _P]
+        lib.orc_fft_c2r.argtypes = [_I, _L, _P]
+        lib.orc_fft_r2c.argtypes = [_I, _L, _P, _I]
+        lib.orc_permute_2a3a.argtypes = [_I, _I, _P, _I, _I, _I, _I, _P, _P]
+        lib.orc_permute_3b2b.argtypes = [_I, _I, _P, _I, _I, _I, _I, _P, _P]
+        lib.orc_cheby_grid.argtypes = [_I, _D, _D, _P]
+        lib.orc_cheby_tables.argtypes = [_I, _P, _P]
+        lib.orc_cheby_to_spectral.argtypes = [_I, _L, _P, _P, _P, _P]
+        lib.orc_cheby_from_spectral.argtypes = [_I, _L, _P, _P, _P, _P]
+        lib.orc_cheby_dealias.argtypes = [_I, _L, _P]
+        lib.orc_m_order_balanced.argtypes = [_I, _P]
+        _oracle = lib
+    return _oracle
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        lib = ctypes.CDLL(REF_SO)
+        lib.ref_legendre_init.argtypes = [_I, _I, _I, _P]
+        lib.ref_legendre_nodes.argtypes = [_P, _P, _P, _P]
+        lib.ref_legendre_tables.argtypes = [_P, _P, _P, _P]
+        lib.ref_stp.argtypes = [_I, _I, _P, _P]
+        lib.ref_pts.argtypes = [_I, _I, _P, _P]
+        lib.ref_cheby_init.argtypes = [_I, _D, _D, _P, _P]
+        lib.ref_cheby_tables.argtypes = [_P, _P]
+        lib.ref_cheby_to_spectral.argtypes = [_I, _P, _P]
+        lib.ref_cheby_from_spectral.argtypes = [_I, _P, _P]
+        _ref = lib
+    return _ref
+
+
+# ---------------------------------------------------------------------------
+# shapes and synthetic inputs
+
+
+def n_even(lmax: int, m: int) -> int:
+    return (lmax - m + 2) // 2
+
+
+def n_odd(lmax: int, m: int) -> int:
+    return (lmax - m + 1) // 2
+
+
+def spectral_size(lmax: int, mvals, nr: int, nf: int) -> int:
+    return int(sum((lmax - int(m) + 1) for m in mvals)) * nr * 2 * nf
+
+
+def spectral_offsets(lmax: int, mvals, nr: int, nf: int) -> np.ndarray:
+    sizes = [(lmax - int(m) + 1) * nr * 2 * nf for m in mvals]
+    return np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+
+
+def m_order_balanced(lmax: int) -> np.ndarray:
+    mv = np.zeros(lmax + 1, dtype=np.int32)
+    oracle_lib().orc_m_order_balanced(lmax, _ptr(mv))
+    return mv
+
+
+def synthetic_spectral(lmax: int, mvals, nr: int, nf: int, seed: int) -> np.ndarray:
+    """Coefficients a_lm(r) = (N(0,1) + i N(0,1)) (l+1)^-2 sin(pi r_i / nr), i = 1..nr,
+    split into Re/Im planes; the m = 0 imaginary plane is zero.  Packed layout."""
+    rng = np.random.default_rng(seed)
+    rfac = np.sin(np.pi * np.arange(1, nr + 1) / nr)
+    out = np.empty(spectral_size(lmax, mvals, nr, nf))
+    off = 0
+    for m in mvals:
+        m = int(m)
+        nl = lmax - m + 1
+        l = np.arange(m, lmax + 1)
+        blk = rng.standard_normal((nf, 2, nr, nl))
+        blk *= ((l + 1.0) ** -2)[None, None, None, :]
+        blk *= rfac[None, None, :, None]
+        if m == 0:
+            blk[:, 1] = 0.0
+        out[off:off + blk.size] = blk.ravel()
+        off += blk.size
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the restatement
+
+
+def gauss_legendre(n: int):
+    xh, xl, wh, wl = (np.zeros(n) for _ in range(4))
+    oracle_lib().orc_gauss_legendre(n, _ptr(xh), _ptr(xl), _ptr(wh), _ptr(wl))
+    return xh, xl, wh, wl
+
+
+def legendre_tables(nt: int, lmax: int, mvals):
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    lib = oracle_lib()
+    ne = lib.orc_table_size_even(nt, lmax, _ptr(mv), len(mv))
+    no = lib.orc_table_size_odd(nt, lmax, _ptr(mv), len(mv))
+    pe, ie = np.zeros(ne), np.zeros(ne)
+    po, io = np.zeros(max(no, 1)), np.zeros(max(no, 1))
+    rc = lib.orc_legendre_tables(nt, lmax, _ptr(mv), len(mv), _ptr(pe), _ptr(po), _ptr(ie), _ptr(io))
+    if rc != 0:
+        raise ValueError(f"orc_legendre_tables failed ({rc})")
+    return pe, po[:no], ie, io[:no]
+
+
+def stp(nt, lmax, mvals, nr, nf, tables, s_in):
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    pe, po, _, _ = tables
+    po = po if po.size else np.zeros(1)
+    out = np.zeros(nt * 2 * nr * nf * len(mv))
+    oracle_lib().orc_stp(nt, lmax, _ptr(mv), len(mv), nr, nf, _ptr(pe), _ptr(po),
+                         _ptr(np.ascontiguousarray(s_in)), _ptr(out))
+    return out
+
+
+def pts(nt, lmax, mvals, nr, nf, tables, p_in):
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    _, _, ie, io = tables
+    io = io if io.size else np.zeros(1)
+    out = np.zeros(spectral_size(lmax, mv, nr, nf))
+    oracle_lib().orc_pts(nt, lmax, _ptr(mv), len(mv), nr, nf, _ptr(ie), _ptr(io),
+                         _ptr(np.ascontiguousarray(p_in)), _ptr(out))
+    return out
+
+
+def fft_c2r(nphi: int, x: np.ndarray) -> np.ndarray:
+    y = np.array(x, dtype=np.float64, copy=True)
+    oracle_lib().orc_fft_c2r(nphi, y.size // (nphi + 2), _ptr(y))
+    return y
+
+
+def fft_r2c(nphi: int, x: np.ndarray, rescale: bool = False) -> np.ndarray:
+    y = np.array(x, dtype=np.float64, copy=True)
+    oracle_lib().orc_fft_r2c(nphi, y.size // (nphi + 2), _ptr(y), int(rescale))
+    return y
+
+
+def permute_2a3a(nt, lmax, mvals, nr, nf, nphi, p2a):
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    out = np.empty((nphi + 2) * nr * nt * nf)
+    oracle_lib().orc_permute_2a3a(nt, lmax, _ptr(mv), len(mv), nr, nf, nphi,
+                                  _ptr(np.ascontiguousarray(p2a)), _ptr(out))
+    return out
+
+
+def permute_3b2b(nt, lmax, mvals, nr, nf, nphi, p3b):
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    out = np.zeros(nt * 2 * nr * nf * len(mv))
+    oracle_lib().orc_permute_3b2b(nt, lmax, _ptr(mv), len(mv), nr, nf, nphi,
+                                  _ptr(np.ascontiguousarray(p3b)), _ptr(out))
+    return out
+
+
+def cheby_grid(n: int, rmin: float, rmax: float) -> np.ndarray:
+    g = np.zeros(n)
+    oracle_lib().orc_cheby_grid(n, rmin, rmax, _ptr(g))
+    return g
+
+
+def cheby_tables(n: int):
+    nx = n - n // 2
+    ce, co = np.zeros(nx * nx), np.zeros(max(nx * (n // 2), 1))
+    oracle_lib().orc_cheby_tables(n, _ptr(ce), _ptr(co))
+    return ce, co
+
+
+def cheby_to_spectral(n: int, f: np.ndarray) -> np.ndarray:
+    ce, co = cheby_tables(n)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    c = np.zeros_like(f)
+    oracle_lib().orc_cheby_to_spectral(n, f.size // n, _ptr(ce), _ptr(co), _ptr(f), _ptr(c))
+    return c
+
+
+def cheby_from_spectral(n: int, c: np.ndarray) -> np.ndarray:
+    ce, co = cheby_tables(n)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    f = np.zeros_like(c)
+    oracle_lib().orc_cheby_from_spectral(n, c.size // n, _ptr(ce), _ptr(co), _ptr(c), _ptr(f))
+    return f
+
+
+def cheby_dealias(n: int, c: np.ndarray) -> np.ndarray:
+    y = np.array(c, dtype=np.float64, copy=True)
+    oracle_lib().orc_cheby_dealias(n, y.size // n, _ptr(y))
+    return y
+
+
+# ---------------------------------------------------------------------------
+# the compiled reference (present only where oracle/_ref was built from the sources)
+
+
+class Reference:
+    """The reference's own Math_Layer modules behind the bind(C) harness."""
+
+    def __init__(self):
+        self.lib = ref_lib()
+        self.nt = self.lmax = None
+        self.mvals = None
+
+    def legendre_init(self, nt: int, lmax: int, mvals):
+        self.mvals = np.ascontiguousarray(mvals, dtype=np.int32)
+        self.nt, self.lmax = nt, lmax
+        self.lib.ref_legendre_init(nt, lmax, len(self.mvals), _ptr(self.mvals))
+
+    def nodes(self):
+        xh, xl, wh, wl = (np.zeros(self.nt) for _ in range(4))
+        self.lib.ref_legendre_nodes(_ptr(xh), _ptr(xl), _ptr(wh), _ptr(wl))
+        return xh, xl, wh, wl
+
+    def tables(self):
+        lib = oracle_lib()
+        mv = self.mvals
+        ne = lib.orc_table_size_even(self.nt, self.lmax, _ptr(mv), len(mv))
+        no = lib.orc_table_size_odd(self.nt, self.lmax, _ptr(mv), len(mv))
+        pe, ie = np.zeros(ne), np.zeros(ne)
+        po, io = np.zeros(max(no, 1)), np.zeros(max(no, 1))
+        self.lib.ref_legendre_tables(_ptr(pe), _ptr(po), _ptr(ie), _ptr(io))
+        return pe, po[:no], ie, io[:no]
+
+    def stp(self, nr: int, nf: int, s_in: np.ndarray) -> np.ndarray:
+        out = np.zeros(self.nt * 2 * nr * nf * len(self.mvals))
+        self.lib.ref_stp(nr, nf, _ptr(np.ascontiguousarray(s_in)), _ptr(out))
+        return out
+
+    def pts(self, nr: int, nf: int, p_in: np.ndarray) -> np.ndarray:
+        out = np.zeros(spectral_size(self.lmax, self.mvals, nr, nf))
+        self.lib.ref_pts(nr, nf, _ptr(np.ascontiguousarray(p_in)), _ptr(out))
+        return out
+
+    def cheby_init(self, n: int, rmin: float, rmax: float):
+        self.ncheb = n
+        g, w = np.zeros(n), np.zeros(n)
+        self.lib.ref_cheby_init(n, rmin, rmax, _ptr(g), _ptr(w))
+        return g, w
+
+    def cheby_tables(self):
+        n = self.ncheb
+        nx = n - n // 2
+        ce, co = np.zeros(nx * nx), np.zeros(max(nx * (n // 2), 1))
+        self.lib.ref_cheby_tables(_ptr(ce), _ptr(co))
+        return ce, co
+
+    def cheby_to_spectral(self, f: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        c = np.zeros_like(f)
+        self.lib.ref_cheby_to_spectral(f.size // self.ncheb, _ptr(f), _ptr(c))
+        return c
+
+    def cheby_from_spectral(self, c: np.ndarray) -> np.ndarray:
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        f = np.zeros_like(c)
+        self.lib.ref_cheby_from_spectral(c.size // self.ncheb, _ptr(c), _ptr(f))
+        return f

@@ -1,0 +1,560 @@
+/*
+ * raysht oracle -- CPU restatement of Rayleigh's spectral-transform path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker that tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg compare the HIP library against.  It is
+ * never linked into, loaded by or called from the product.
+ *
+ * Each function below restates one reference routine (file:line under the
+ * Rayleigh tree) step by step:
+ *   - Gauss-Legendre nodes/weights and the P_lm tables run in __float128,
+ *     as the reference runs them in Real*16, with the same operation order,
+ *     so the stored doubles are the reference's quad-then-round values.
+ *   - StP / PtS / Chebyshev contractions run as plain loops in the order the
+ *     reference's DGEMM calls define (inner sum over k ascending).
+ *   - The FFT (FFTW is not part of the reference tree) is an exact-as-possible
+ *     restatement of FFTW's unnormalised r2c / c2r: a mixed-radix DFT in
+ *     long double, rounded to double once.
+ *
+ * What pins each function (see tests/test_oracle_*.py):
+ *   orc_gauss_legendre   compiled reference (bit-exact), numpy leggauss, sum w = 2, symmetry
+ *   orc_legendre_tables  compiled reference (bit-exact), closed-form Y_lm, discrete orthogonality
+ *   orc_stp / orc_pts    compiled reference + fixtures, Test_SHT known answers, round-trip identity
+ *   orc_fft_*            numpy.fft (pocketfft), brute-force long-double DFT, Parseval
+ *   orc_permute_*        exhaustive index maps on small grids, zero-fill / truncation invariants
+ *   orc_cheby_*          compiled reference (bit-exact tables), scipy DCT-II/III, round trip
+ */
+#include "raysht_oracle.h"
+
+#include <math.h>
+#include <quadmath.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef __float128 quad;
+
+/* compiler runtime integer power, the routine the reference's `tmp**(mv/2)` lowers to */
+extern quad __powitf2(quad a, int b);
+
+static const quad PI_Q = 3.1415926535897932384626433832795028841972Q;
+
+static void split_quad(quad q, double *hi, double *lo)
+{
+    double h = (double)q;
+    *hi = h;
+    *lo = (double)(q - (quad)h);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Legendre_Polynomials.F90:375-393  nth_legendre                            */
+static void nth_legendre(quad x, int n, quad *pn_out, quad *dpn_out)
+{
+    quad pn = 1.0Q, pn1 = 0.0Q, pn2;
+    for (int j = 1; j <= n; ++j) {
+        pn2 = pn1;
+        pn1 = pn;
+        pn = ((2.0Q * (quad)j - 1.0Q) * x * pn1 - ((quad)j - 1.0Q) * pn2) / (quad)j;
+    }
+    *pn_out = pn;
+    *dpn_out = (quad)n * (x * pn - pn1) / (x * x - 1.0Q);
+}
+
+/* Legendre_Polynomials.F90:332-373  Find_Colocation on [-1, 1]:
+ * Newton from cos(pi(i-1/4)/(n+1/2)) until |step| <= 3e-14; the weight of the
+ * final iterate uses the derivative evaluated at the previous iterate, as the
+ * reference does.  Nodes ascend from -1. */
+static void gauss_legendre_q(int n, quad *x, quad *w)
+{
+    const quad x1 = -1.0Q, x2 = 1.0Q;
+    const quad mid = 0.5Q * (x1 + x2), scl = 0.5Q * (x2 - x1);
+    const quad eps = 3.0e-14Q;
+    int nroots = (n + 1) / 2;
+    for (int i = 1; i <= nroots; ++i) {
+        quad r = cosq(PI_Q * ((quad)i - 0.25Q) / ((quad)n + 0.5Q));
+        int converged = 0;
+        while (!converged) {
+            quad pn, dpn;
+            nth_legendre(r, n, &pn, &dpn);
+            quad nr = r - pn / dpn;
+            quad del = fabsq(r - nr);
+            r = nr;
+            if (del <= eps) converged = 1;
+            x[i - 1] = mid - scl * r;
+            x[n - i] = mid + scl * r;
+            w[i - 1] = 2.0Q * scl / ((1.0Q - r * r) * dpn * dpn);
+            w[n - i] = w[i - 1];
+        }
+    }
+}
+
+void orc_gauss_legendre(int n, double *xhi, double *xlo, double *whi, double *wlo)
+{
+    quad *x = malloc(sizeof(quad) * (size_t)n), *w = malloc(sizeof(quad) * (size_t)n);
+    gauss_legendre_q(n, x, w);
+    for (int i = 0; i < n; ++i) {
+        split_quad(x[i], &xhi[i], &xlo[i]);
+        split_quad(w[i], &whi[i], &wlo[i]);
+    }
+    free(x);
+    free(w);
+}
+
+static int n_even_of(int lmax, int m) { return (lmax - m + 2) / 2; }
+static int n_odd_of(int lmax, int m) { return (lmax - m + 1) / 2; }
+
+long orc_table_size_even(int nt, int lmax, const int *mvals, int nm)
+{
+    long s = 0;
+    for (int i = 0; i < nm; ++i) s += (long)n_even_of(lmax, mvals[i]) * (nt / 2);
+    return s;
+}
+
+long orc_table_size_odd(int nt, int lmax, const int *mvals, int nm)
+{
+    long s = 0;
+    for (int i = 0; i < nm; ++i) s += (long)n_odd_of(lmax, mvals[i]) * (nt / 2);
+    return s;
+}
+
+/* Legendre_Polynomials.F90:146-258 Compute_Plms (parity branch) followed by
+ * :260-331 Parity_Resort.  P_lm carries the orthonormal spherical-harmonic
+ * normalisation with the Condon-Shortley sign; only the nt/2 southern nodes are
+ * evaluated.  The per-l recursion coefficients depend on (l, m) only, so they
+ * are computed once per l here instead of once per node: the quad operations
+ * and therefore the values are the same. */
+int orc_legendre_tables(int nt, int lmax, const int *mvals, int nm,
+                        double *p_even, double *p_odd, double *ip_even, double *ip_odd)
+{
+    if (nt <= 0 || (nt % 2) != 0 || lmax < 0) return -1;
+    const int nt2 = nt / 2;
+    quad *x = malloc(sizeof(quad) * (size_t)nt), *w = malloc(sizeof(quad) * (size_t)nt);
+    gauss_legendre_q(nt, x, w);
+    quad *plm = malloc(sizeof(quad) * (size_t)nt2 * (size_t)(lmax + 1));
+    long oe = 0, oo = 0;
+    for (int mp = 0; mp < nm; ++mp) {
+        const int mv = mvals[mp];
+        if (mv < 0 || mv > lmax) { free(x); free(w); free(plm); return -2; }
+        /* compute_factorial_ratio, :406-415 */
+        quad ratio = 1.0Q;
+        for (int i = 1; i <= mv; ++i) ratio = ratio * powq(((quad)i - 0.5Q) / (quad)i, 0.5Q);
+        quad amp = powq(((quad)mv + 0.5Q) / (2.0Q * PI_Q), 0.5Q);
+        amp = amp * ratio;
+        /* plm[(l-mv)*nt2 + i] */
+        for (int i = 0; i < nt2; ++i) {
+            quad xi = x[i];
+            quad tmp = 1.0Q - xi * xi;
+            quad v;
+            if (mv % 2 == 1) v = -amp * powq(tmp, (quad)(mv / 2) + 0.5Q);
+            else v = amp * __powitf2(tmp, mv / 2);
+            plm[i] = v;
+            if (mv < lmax) plm[(size_t)nt2 + i] = v * xi * powq(2.0Q * (quad)mv + 3.0Q, 0.5Q);
+        }
+        for (int l = mv + 2; l <= lmax; ++l) {
+            quad a = (quad)((l - 1) * (l - 1) - mv * mv);
+            a = a / (4.0Q * (quad)((l - 1) * (l - 1)) - 1.0Q);
+            a = powq(a, 0.5Q);
+            quad b = (4.0Q * (quad)l * (quad)l - 1.0Q) / (quad)(l * l - mv * mv);
+            b = powq(b, 0.5Q);
+            quad *pl = plm + (size_t)(l - mv) * nt2;
+            const quad *p1 = pl - nt2, *p2 = pl - 2 * nt2;
+            for (int i = 0; i < nt2; ++i) {
+                quad t = p1[i] * x[i] - a * p2[i];
+                pl[i] = t * b;
+            }
+        }
+        /* Parity_Resort: FFT normalisation folded into the tables (:268-276) */
+        const quad pts_norm = (mv == 0) ? (quad)(1.0 / (2.0 * (double)nt)) : (quad)(1.0 / (double)nt);
+        const quad stp_norm = (mv == 0) ? 1.0Q : 0.5Q;
+        const int ne = n_even_of(lmax, mv), no = n_odd_of(lmax, mv);
+        for (int k = 0; k < ne; ++k) {
+            const quad *pl = plm + (size_t)(2 * k) * nt2;
+            for (int i = 0; i < nt2; ++i) {
+                quad renorm = 2.0Q * PI_Q * w[i];
+                quad t = pl[i] * renorm;
+                ip_even[oe + (long)k * nt2 + i] = (double)(t * pts_norm);
+                p_even[oe + (long)i * ne + k] = (double)(pl[i] * stp_norm);
+            }
+        }
+        for (int k = 0; k < no; ++k) {
+            const quad *pl = plm + (size_t)(2 * k + 1) * nt2;
+            for (int i = 0; i < nt2; ++i) {
+                quad renorm = 2.0Q * PI_Q * w[i];
+                quad t = pl[i] * renorm;
+                ip_odd[oo + (long)k * nt2 + i] = (double)(t * pts_norm);
+                p_odd[oo + (long)i * no + k] = (double)(pl[i] * stp_norm);
+            }
+        }
+        oe += (long)ne * nt2;
+        oo += (long)no * nt2;
+    }
+    free(plm);
+    free(x);
+    free(w);
+    return 0;
+}
+
+static long spec_block(int lmax, int m, int nr, int nf) { return (long)(lmax - m + 1) * nr * 2 * nf; }
+
+/* Legendre_Transforms.F90:968-1074  StP_4d_dgp2 (parity branch):
+ * E = p_lm_even^T . s_even, O = p_lm_odd^T . s_odd (each nt2 x nrhs);
+ * out(j) = E + O, out(nt+1-j) = E - O; out zeroed first. */
+void orc_stp(int nt, int lmax, const int *mvals, int nm, int nr, int nf,
+             const double *p_even, const double *p_odd, const double *s_in, double *p_out)
+{
+    const int nt2 = nt / 2, nrhs = 2 * nr * nf;
+    long so = 0, oe = 0, oo = 0;
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp], nl = lmax - m + 1;
+        const int ne = n_even_of(lmax, m), no = n_odd_of(lmax, m);
+        const double *pe = p_even + oe, *po = p_odd + oo;
+        double *out = p_out + (long)mp * nrhs * nt;
+        for (int c = 0; c < nrhs; ++c) {
+            const double *col = s_in + so + (long)c * nl;
+            double *oc = out + (long)c * nt;
+            for (int j = 0; j < nt2; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < ne; ++k) acc += pe[(long)j * ne + k] * col[2 * k];
+                oc[j] = acc;
+                oc[nt - 1 - j] = acc;
+            }
+            if (no > 0) {
+                for (int j = 0; j < nt2; ++j) {
+                    double acc = 0.0;
+                    for (int k = 0; k < no; ++k) acc += po[(long)j * no + k] * col[2 * k + 1];
+                    oc[j] = oc[j] + acc;
+                    oc[nt - 1 - j] = oc[nt - 1 - j] - acc;
+                }
+            }
+        }
+        so += spec_block(lmax, m, nr, nf);
+        oe += (long)ne * nt2;
+        oo += (long)no * nt2;
+    }
+}
+
+/* Legendre_Transforms.F90:1077-1175  PtS_4d_dgpv3 (parity branch):
+ * feven(j) = f(j) + f(nt+1-j), fodd(j) = f(j) - f(nt+1-j), j <= nt2;
+ * s_even = ip_lm_even^T . feven, s_odd = ip_lm_odd^T . fodd, scattered to l. */
+void orc_pts(int nt, int lmax, const int *mvals, int nm, int nr, int nf,
+             const double *ip_even, const double *ip_odd, const double *p_in, double *s_out)
+{
+    const int nt2 = nt / 2, nrhs = 2 * nr * nf;
+    long so = 0, oe = 0, oo = 0;
+    double *fe = malloc(sizeof(double) * (size_t)nt2), *fo = malloc(sizeof(double) * (size_t)nt2);
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp], nl = lmax - m + 1;
+        const int ne = n_even_of(lmax, m), no = n_odd_of(lmax, m);
+        const double *ie = ip_even + oe, *io = ip_odd + oo;
+        const double *in = p_in + (long)mp * nrhs * nt;
+        for (int c = 0; c < nrhs; ++c) {
+            const double *ic = in + (long)c * nt;
+            double *col = s_out + so + (long)c * nl;
+            for (int j = 0; j < nt2; ++j) {
+                fe[j] = ic[j] + ic[nt - 1 - j];
+                fo[j] = ic[j] - ic[nt - 1 - j];
+            }
+            for (int k = 0; k < ne; ++k) {
+                double acc = 0.0;
+                for (int j = 0; j < nt2; ++j) acc += ie[(long)k * nt2 + j] * fe[j];
+                col[2 * k] = acc;
+            }
+            for (int k = 0; k < no; ++k) {
+                double acc = 0.0;
+                for (int j = 0; j < nt2; ++j) acc += io[(long)k * nt2 + j] * fo[j];
+                col[2 * k + 1] = acc;
+            }
+        }
+        so += spec_block(lmax, m, nr, nf);
+        oe += (long)ne * nt2;
+        oo += (long)no * nt2;
+    }
+    free(fe);
+    free(fo);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FFT.  Fourier_Transform.F90:100-183 call FFTW's r2c (e^{-i}) and c2r (e^{+i}),
+ * unnormalised, in place on lines of nphi+2 doubles holding Re/Im of
+ * m = 0..nphi/2.  FFTW itself is outside the reference tree; this restates the
+ * transform it is documented to compute, in long double. */
+typedef struct { long double re, im; } cld;
+
+static void dft_rec(const cld *in, cld *out, int n, int stride, int sign, const cld *tw, int ntw)
+{
+    /* out[k] = sum_j in[j*stride] * exp(sign * 2 pi i j k / n); tw[t] = exp(sign 2 pi i t / ntw) */
+    if (n == 1) { out[0] = in[0]; return; }
+    int p = 0;
+    if (n % 2 == 0) p = 2;
+    else if (n % 3 == 0) p = 3;
+    else {
+        for (int k = 0; k < n; ++k) {
+            long double sr = 0, si = 0;
+            for (int j = 0; j < n; ++j) {
+                long t = ((long)j * k % n) * (ntw / n);
+                long double c = tw[t].re, s = tw[t].im;
+                sr += in[(long)j * stride].re * c - in[(long)j * stride].im * s;
+                si += in[(long)j * stride].re * s + in[(long)j * stride].im * c;
+            }
+            out[k].re = sr;
+            out[k].im = si;
+        }
+        return;
+    }
+    const int q = n / p;
+    /* sub-transforms of the p decimated sequences, stored consecutively */
+    for (int r = 0; r < p; ++r) dft_rec(in + (long)r * stride, out + (long)r * q, q, stride * p, sign, tw, ntw);
+    cld *tmp = malloc(sizeof(cld) * (size_t)n);
+    memcpy(tmp, out, sizeof(cld) * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        long double sr = 0, si = 0;
+        for (int r = 0; r < p; ++r) {
+            long t = ((long)r * k % n) * (ntw / n);
+            const cld a = tmp[(long)r * q + (k % q)];
+            sr += a.re * tw[t].re - a.im * tw[t].im;
+            si += a.re * tw[t].im + a.im * tw[t].re;
+        }
+        out[k].re = sr;
+        out[k].im = si;
+    }
+    free(tmp);
+}
+
+static cld *twiddles(int n, int sign)
+{
+    cld *tw = malloc(sizeof(cld) * (size_t)n);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int t = 0; t < n; ++t) {
+        long double a = two_pi * (long double)t / (long double)n;
+        tw[t].re = cosl(a);
+        tw[t].im = (long double)sign * sinl(a);
+    }
+    return tw;
+}
+
+void orc_fft_c2r(int nphi, long howmany, double *x)
+{
+    const int n = nphi, h = nphi / 2;
+    cld *tw = twiddles(n, +1);
+    cld *z = malloc(sizeof(cld) * (size_t)n), *y = malloc(sizeof(cld) * (size_t)n);
+    for (long b = 0; b < howmany; ++b) {
+        double *line = x + b * (long)(nphi + 2);
+        /* Hermitian extension; Im of m = 0 and m = n/2 do not enter a c2r */
+        z[0].re = line[0];
+        z[0].im = 0;
+        for (int m = 1; m < h; ++m) {
+            z[m].re = line[2 * m];
+            z[m].im = line[2 * m + 1];
+            z[n - m].re = line[2 * m];
+            z[n - m].im = -(long double)line[2 * m + 1];
+        }
+        if (n % 2 == 0) { z[h].re = line[2 * h]; z[h].im = 0; }
+        dft_rec(z, y, n, 1, +1, tw, n);
+        for (int k = 0; k < n; ++k) line[k] = (double)y[k].re;
+        line[n] = 0.0;
+        line[n + 1] = 0.0;
+    }
+    free(z);
+    free(y);
+    free(tw);
+}
+
+/* Fourier_Transform.F90:100-140 (rescale = 0) and :56-98 (rescale != 0: the
+ * _RSC variant divides x(3:) by n and x(1) by 2n, leaving x(2) untouched). */
+void orc_fft_r2c(int nphi, long howmany, double *x, int rescale)
+{
+    const int n = nphi, h = nphi / 2;
+    cld *tw = twiddles(n, -1);
+    cld *z = malloc(sizeof(cld) * (size_t)n), *y = malloc(sizeof(cld) * (size_t)n);
+    for (long b = 0; b < howmany; ++b) {
+        double *line = x + b * (long)(nphi + 2);
+        for (int k = 0; k < n; ++k) { z[k].re = line[k]; z[k].im = 0; }
+        dft_rec(z, y, n, 1, -1, tw, n);
+        for (int m = 0; m <= h; ++m) {
+            line[2 * m] = (double)y[m].re;
+            line[2 * m + 1] = (double)y[m].im;
+        }
+        if (rescale) {
+            for (int i = 2; i < n + 2; ++i) line[i] = line[i] / (double)n;
+            line[0] = line[0] / (double)(2 * n);
+        }
+    }
+    free(z);
+    free(y);
+    free(tw);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Spherical_Buffer.F90:318-459 Transpose_2a3a_v0 on one rank: p3a is zeroed,
+ * then p3a(2m+1, r, theta, f) = p2a(theta, r + 2 nr f, mp) (Re) and
+ * p3a(2m+2, ...) = p2a(theta, r + nr + 2 nr f, mp) (Im). */
+void orc_permute_2a3a(int nt, int lmax, const int *mvals, int nm, int nr, int nf, int nphi,
+                      const double *p2a, double *p3a)
+{
+    (void)lmax;
+    const long ld = nphi + 2, nrhs = 2L * nr * nf;
+    memset(p3a, 0, sizeof(double) * (size_t)(ld * nr * nt * nf));
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp];
+        for (int f = 0; f < nf; ++f)
+            for (int t = 0; t < nt; ++t)
+                for (int r = 0; r < nr; ++r) {
+                    double *line = p3a + (((long)f * nt + t) * nr + r) * ld;
+                    line[2 * m] = p2a[((long)mp * nrhs + r + 2L * nr * f) * nt + t];
+                    line[2 * m + 1] = p2a[((long)mp * nrhs + r + nr + 2L * nr * f) * nt + t];
+                }
+    }
+}
+
+/* Spherical_Buffer.F90:583-703 Transpose_3b2b_v0 on one rank: the inverse map,
+ * reading back only the m in m_values (truncation of m > lmax). */
+void orc_permute_3b2b(int nt, int lmax, const int *mvals, int nm, int nr, int nf, int nphi,
+                      const double *p3b, double *p2b)
+{
+    (void)lmax;
+    const long ld = nphi + 2, nrhs = 2L * nr * nf;
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp];
+        for (int f = 0; f < nf; ++f)
+            for (int t = 0; t < nt; ++t)
+                for (int r = 0; r < nr; ++r) {
+                    const double *line = p3b + (((long)f * nt + t) * nr + r) * ld;
+                    p2b[((long)mp * nrhs + r + 2L * nr * f) * nt + t] = line[2 * m];
+                    p2b[((long)mp * nrhs + r + nr + 2L * nr * f) * nt + t] = line[2 * m + 1];
+                }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Chebyshev_Polynomials.F90:210-248 Gen_Colocation_Points (zeros branch: the
+ * angle accumulates by repeated addition) and :65-207 Initialize_Cheby_Grid's
+ * affine map of one domain onto [rmin, rmax] (grid descends from rmax). */
+static void cheby_theta(int n, double *theta)
+{
+    const double pi = 3.1415926535897932384626433832795028841972;
+    double dtheta = pi / n;
+    double arg = dtheta * 0.5;
+    for (int i = 0; i < n; ++i) {
+        theta[i] = arg;
+        arg = arg + dtheta;
+    }
+}
+
+void orc_cheby_grid(int n, double rmin, double rmax, double *grid)
+{
+    double *th = malloc(sizeof(double) * (size_t)n), *x = malloc(sizeof(double) * (size_t)n);
+    cheby_theta(n, th);
+    for (int i = 0; i < n; ++i) x[i] = cos(th[i]);
+    double upperb = rmax, lowerb = rmin, xmin = x[n - 1], xmax = x[0];
+    double scaling = (upperb - lowerb) / (xmax - xmin);
+    for (int i = 0; i < n; ++i) grid[i] = (x[i] - xmin) * scaling + lowerb;
+    free(th);
+    free(x);
+}
+
+/* Chebyshev_Polynomials.F90:251-297 Gen_Tn: T_k(x_r) = cos(k theta_r);
+ * ceven[i][r] (r fastest, 1..n_x) = T_{2i}, codd = T_{2i+1}; the x = 0 row is
+ * halved when n is odd. */
+void orc_cheby_tables(int n, double *ceven, double *codd)
+{
+    double *th = malloc(sizeof(double) * (size_t)n);
+    cheby_theta(n, th);
+    const int n_odd = n / 2, n_even = n_odd + n % 2, n_x = n_even;
+    for (int i = 0; i < n_even; ++i)
+        for (int r = 0; r < n_x; ++r) ceven[(long)i * n_x + r] = cos((double)(2 * i) * th[r]);
+    for (int i = 0; i < n_odd; ++i)
+        for (int r = 0; r < n_x; ++r) codd[(long)i * n_x + r] = cos((double)(2 * i + 1) * th[r]);
+    if (n_x != n_odd) {
+        for (int i = 0; i < n_even; ++i) ceven[(long)i * n_x + n_x - 1] *= 0.5;
+        for (int i = 0; i < n_odd; ++i) codd[(long)i * n_x + n_x - 1] *= 0.5;
+    }
+    free(th);
+}
+
+/* Chebyshev_Polynomials.F90:375-448 To_Spectral4D (one domain):
+ * fold f(i) +- f(n+1-i), contract with alpha = 2/n, interleave even -> c(2i-1),
+ * odd -> c(2i). */
+void orc_cheby_to_spectral(int n, long ncols, const double *ceven, const double *codd,
+                           const double *f, double *c)
+{
+    const int n_odd = n / 2, n_even = n_odd + n % 2, n_x = n_even;
+    const double alpha = 2.0 / n;
+    double *fe = malloc(sizeof(double) * (size_t)n_x), *fo = malloc(sizeof(double) * (size_t)n_x);
+    for (long col = 0; col < ncols; ++col) {
+        const double *fc = f + col * n;
+        double *cc = c + col * n;
+        for (int i = 0; i < n_x; ++i) {
+            fe[i] = fc[i] + fc[n - 1 - i];
+            fo[i] = fc[i] - fc[n - 1 - i];
+        }
+        for (int k = 0; k < n_even; ++k) {
+            double acc = 0.0;
+            for (int i = 0; i < n_x; ++i) acc += ceven[(long)k * n_x + i] * fe[i];
+            cc[2 * k] = alpha * acc;
+        }
+        for (int k = 0; k < n_odd; ++k) {
+            double acc = 0.0;
+            for (int i = 0; i < n_x; ++i) acc += codd[(long)k * n_x + i] * fo[i];
+            cc[2 * k + 1] = alpha * acc;
+        }
+    }
+    free(fe);
+    free(fo);
+}
+
+/* Chebyshev_Polynomials.F90:450-558 From_Spectral4D (one domain):
+ * f = sum_n c_n T_n - c_0/2; the even part is mirrored, the odd part added with
+ * +/-.  For odd n the x = 0 row (halved in the tables) is doubled back, and the
+ * odd product at that row is added twice as the reference does (:537-539; that
+ * product vanishes to rounding because T_odd(0) = 0). */
+void orc_cheby_from_spectral(int n, long ncols, const double *ceven, const double *codd,
+                             const double *c, double *f)
+{
+    const int n_odd = n / 2, n_even = n_odd + n % 2, n_x = n_even;
+    double *fo = malloc(sizeof(double) * (size_t)n_x);
+    for (long col = 0; col < ncols; ++col) {
+        const double *cc = c + col * n;
+        double *fc = f + col * n;
+        for (int i = 0; i < n_x; ++i) {
+            double acc = 0.0;
+            for (int k = 0; k < n_even; ++k) acc += ceven[(long)k * n_x + i] * cc[2 * k];
+            fc[i] = acc;
+            fc[n - 1 - i] = acc;
+        }
+        if (n_even != n_odd) fc[n_x - 1] = fc[n_x - 1] * 2.0;
+        for (int i = 0; i < n_x; ++i) {
+            double acc = 0.0;
+            for (int k = 0; k < n_odd; ++k) acc += codd[(long)k * n_x + i] * cc[2 * k + 1];
+            fo[i] = acc;
+        }
+        for (int i = 0; i < n_odd; ++i) {
+            fc[i] = fc[i] + fo[i];
+            fc[n - 1 - i] = fc[n - 1 - i] - fo[i];
+        }
+        if (n_even != n_odd) fc[n_x - 1] = fc[n_x - 1] + fo[n_x - 1] * 2.0;
+        const double c0h = cc[0] / 2.0;
+        for (int i = 0; i < n; ++i) fc[i] = fc[i] - c0h;
+    }
+    free(fo);
+}
+
+/* Chebyshev_Polynomials.F90:353-374 dealias_buffer4d: zero n >= rda-1 (0-based),
+ * rda = (2n)/3 + 1. */
+void orc_cheby_dealias(int n, long ncols, double *c)
+{
+    const int rda = (2 * n) / 3 + 1;
+    for (long col = 0; col < ncols; ++col)
+        for (int i = rda - 1; i < n; ++i) c[col * n + i] = 0.0;
+}
+
+/* Load_Balance.F90:126-226 M_Balance_v0 for one rank: pairs (i, n_m-1-i), then
+ * the middle m when n_m is odd. */
+void orc_m_order_balanced(int lmax, int *mvals)
+{
+    const int nmm = lmax + 1, npairs = nmm / 2;
+    int k = 0;
+    for (int i = 0; i < npairs; ++i) {
+        mvals[k++] = i;
+        mvals[k++] = nmm - 1 - i;
+    }
+    if (nmm % 2 == 1) mvals[k++] = nmm / 2;
+}

@@ -1,0 +1,191 @@
+!
+! Checker-only harness: drives the reference's own Math_Layer modules
+! (Legendre_Polynomials, Legendre_Transforms, Chebyshev_Polynomials, Structures),
+! compiled from their sources under the reference tree, through bind(C) entry
+! points that Python (ctypes) calls when it validates the C restatement in
+! oracle/raysht_oracle.c and writes the golden fixtures under tests/golden/.
+!
+! This is test infrastructure.  Nothing in the product links or calls it.
+!
+! Layouts (all column-major, as the reference stores them):
+!   spectral, packed over mp = 1..n_m in m_values order:
+!       s(mp)%data(m:l_max, 1:nr, 1:2, 1:nf)      l fastest
+!   hybrid p2(1:n_theta, 1:2*nr*nf, 1:n_m)        theta fastest
+!   Chebyshev buffers f(1:n, 1:ncols)             radius fastest
+!
+Module raysht_ref_harness
+    Use ISO_C_BINDING
+    Use Structures
+    Use Legendre_Polynomials
+    Use Legendre_Transforms
+    Use Chebyshev_Polynomials, Only : Cheby_Grid
+    Implicit None
+    Type(Cheby_Grid), Save, Target :: h_grid
+    Logical, Save :: h_grid_ready = .false.
+    Integer, Save :: h_ncheb = 0
+
+Contains
+
+    Subroutine release_legendre()
+        Integer :: i
+        If (.not. allocated(coloc)) Return
+        ! Finalize_Legendre (Legendre_Polynomials.F90:58-63) leaves the
+        ! integration-weighted parity tables allocated; release them here so
+        ! that Initialize_Legendre can run again in the same process.
+        If (allocated(ip_lm_even)) Then
+            Do i = 1, size(ip_lm_even)
+                If (allocated(ip_lm_even(i)%data)) DeAllocate(ip_lm_even(i)%data)
+            Enddo
+            DeAllocate(ip_lm_even)
+        Endif
+        If (allocated(ip_lm_odd)) Then
+            Do i = 1, size(ip_lm_odd)
+                If (allocated(ip_lm_odd(i)%data)) DeAllocate(ip_lm_odd(i)%data)
+            Enddo
+            DeAllocate(ip_lm_odd)
+        Endif
+        Call Finalize_Legendre()
+    End Subroutine release_legendre
+
+    Subroutine ref_legendre_init(nt, lmax, nm, mvals) bind(C, name='ref_legendre_init')
+        Integer(C_INT), Value :: nt, lmax, nm
+        Integer(C_INT), Intent(In) :: mvals(nm)
+        Integer :: mv(nm)
+        Call release_legendre()
+        mv(:) = mvals(:)
+        Call Initialize_Legendre(nt, lmax, mv, .true.)
+    End Subroutine ref_legendre_init
+
+    ! Gauss-Legendre nodes and weights (Real*16) exported as head/tail double pairs.
+    Subroutine ref_legendre_nodes(xhi, xlo, whi, wlo) bind(C, name='ref_legendre_nodes')
+        Real(C_DOUBLE), Intent(Out) :: xhi(n_theta), xlo(n_theta), whi(n_theta), wlo(n_theta)
+        Integer :: i
+        Real*16 :: q
+        Do i = 1, n_theta
+            xhi(i) = Real(coloc(i), 8)
+            q = coloc(i) - Real(xhi(i), 16)
+            xlo(i) = Real(q, 8)
+            whi(i) = Real(gl_weights(i), 8)
+            q = gl_weights(i) - Real(whi(i), 16)
+            wlo(i) = Real(q, 8)
+        Enddo
+    End Subroutine ref_legendre_nodes
+
+    ! The four parity tables of every mp, concatenated in mp order, each in its
+    ! stored orientation: p_lm_even(mp)%data(1:n_e,1:nt2) (l fastest) and
+    ! ip_lm_even(mp)%data(1:nt2,1:n_e) (theta fastest); odd twins likewise.
+    Subroutine ref_legendre_tables(pe, po, ipe, ipo) bind(C, name='ref_legendre_tables')
+        Real(C_DOUBLE), Intent(Out) :: pe(*), po(*), ipe(*), ipo(*)
+        Integer :: mp, ne, no, nt2, oe, oo, k
+        nt2 = n_theta/2
+        oe = 0
+        oo = 0
+        Do mp = 1, n_m
+            ne = n_l_even(mp)
+            no = n_l_odd(mp)
+            k = ne*nt2
+            If (ne > 0) Then
+                pe(oe+1:oe+k) = Reshape(p_lm_even(mp)%data, (/ k /))
+                ipe(oe+1:oe+k) = Reshape(ip_lm_even(mp)%data, (/ k /))
+            Endif
+            oe = oe + k
+            k = no*nt2
+            If (no > 0) Then
+                po(oo+1:oo+k) = Reshape(p_lm_odd(mp)%data, (/ k /))
+                ipo(oo+1:oo+k) = Reshape(ip_lm_odd(mp)%data, (/ k /))
+            Endif
+            oo = oo + k
+        Enddo
+    End Subroutine ref_legendre_tables
+
+    ! StP through the reference's generic Legendre_Transform (resolves to StP_4d_dgp2).
+    Subroutine ref_stp(nr, nf, s_in, p_out) bind(C, name='ref_stp')
+        Integer(C_INT), Value :: nr, nf
+        Real(C_DOUBLE), Intent(In) :: s_in(*)
+        Real(C_DOUBLE), Intent(Out) :: p_out(n_theta, 2*nr*nf, n_m)
+        Type(rmcontainer4d), Allocatable :: s(:)
+        Integer :: mp, m, n, off
+        Allocate(s(1:n_m))
+        off = 0
+        Do mp = 1, n_m
+            m = m_values(mp)
+            Allocate(s(mp)%data(m:l_max, 1:nr, 1:2, 1:nf))
+            n = (l_max-m+1)*nr*2*nf
+            s(mp)%data = Reshape(s_in(off+1:off+n), (/ l_max-m+1, nr, 2, nf /))
+            off = off + n
+        Enddo
+        Call Legendre_Transform(s, p_out)
+        Do mp = 1, n_m
+            DeAllocate(s(mp)%data)
+        Enddo
+        DeAllocate(s)
+    End Subroutine ref_stp
+
+    ! PtS through the generic Legendre_Transform (resolves to PtS_4d_dgpv3).
+    Subroutine ref_pts(nr, nf, p_in, s_out) bind(C, name='ref_pts')
+        Integer(C_INT), Value :: nr, nf
+        Real(C_DOUBLE), Intent(In) :: p_in(n_theta, 2*nr*nf, n_m)
+        Real(C_DOUBLE), Intent(Out) :: s_out(*)
+        Type(rmcontainer4d), Allocatable :: s(:)
+        Integer :: mp, m, n, off
+        Allocate(s(1:n_m))
+        Do mp = 1, n_m
+            m = m_values(mp)
+            Allocate(s(mp)%data(m:l_max, 1:nr, 1:2, 1:nf))
+            s(mp)%data = 0.0d0
+        Enddo
+        Call Legendre_Transform(p_in, s)
+        off = 0
+        Do mp = 1, n_m
+            m = m_values(mp)
+            n = (l_max-m+1)*nr*2*nf
+            s_out(off+1:off+n) = Reshape(s(mp)%data, (/ n /))
+            off = off + n
+            DeAllocate(s(mp)%data)
+        Enddo
+        DeAllocate(s)
+    End Subroutine ref_pts
+
+    ! One-domain Chebyshev grid on [rmin, rmax] with n points (zeros grid).
+    Subroutine ref_cheby_init(n, rmin, rmax, grid, weights) bind(C, name='ref_cheby_init')
+        Integer(C_INT), Value :: n
+        Real(C_DOUBLE), Value :: rmin, rmax
+        Real(C_DOUBLE), Intent(Out) :: grid(n), weights(n)
+        Integer :: npoly(1)
+        Real*8 :: bounds(2)
+        Type(Cheby_Grid) :: fresh
+        npoly(1) = n
+        bounds(1) = rmin
+        bounds(2) = rmax
+        h_grid = fresh
+        Call h_grid%init(grid, weights, 1, npoly, bounds)
+        h_grid_ready = .true.
+        h_ncheb = n
+    End Subroutine ref_cheby_init
+
+    Subroutine ref_cheby_tables(ceven, codd) bind(C, name='ref_cheby_tables')
+        Real(C_DOUBLE), Intent(Out) :: ceven(*), codd(*)
+        Integer :: ke, ko
+        ke = size(h_grid%cheby_even(1)%data)
+        ko = size(h_grid%cheby_odd(1)%data)
+        ceven(1:ke) = Reshape(h_grid%cheby_even(1)%data, (/ ke /))
+        codd(1:ko) = Reshape(h_grid%cheby_odd(1)%data, (/ ko /))
+    End Subroutine ref_cheby_tables
+
+    ! To_Spectral4D on a (n, ncols, 1, 1) buffer.
+    Subroutine ref_cheby_to_spectral(ncols, f_in, c_out) bind(C, name='ref_cheby_to_spectral')
+        Integer(C_INT), Value :: ncols
+        Real(C_DOUBLE), Intent(In) :: f_in(h_ncheb, ncols, 1, 1)
+        Real(C_DOUBLE), Intent(Out) :: c_out(h_ncheb, ncols, 1, 1)
+        Call h_grid%to_spectral(f_in, c_out)
+    End Subroutine ref_cheby_to_spectral
+
+    ! From_Spectral4D on a (n, ncols, 1, 1) buffer.
+    Subroutine ref_cheby_from_spectral(ncols, c_in, f_out) bind(C, name='ref_cheby_from_spectral')
+        Integer(C_INT), Value :: ncols
+        Real(C_DOUBLE), Intent(In) :: c_in(h_ncheb, ncols, 1, 1)
+        Real(C_DOUBLE), Intent(Out) :: f_out(h_ncheb, ncols, 1, 1)
+        Call h_grid%from_spectral(c_in, f_out)
+    End Subroutine ref_cheby_from_spectral
+
+End Module raysht_ref_harness

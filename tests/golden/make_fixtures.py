@@ -1,0 +1,76 @@
+"""Writes the golden fixtures in this folder from the COMPILED REFERENCE.
+
+Run in a container that holds the reference tree, after `make -C oracle ref`
+has compiled the reference's own Legendre_Polynomials, Legendre_Transforms and
+Chebyshev_Polynomials modules (with the bind(C) harness oracle/ref_harness.F90):
+
+    python tests/golden/make_fixtures.py
+
+Every array in the .npz files is an input chosen here or an output of the
+reference's routines on that input; nothing is computed by this build's code
+except the seeded synthetic inputs (oracle.synthetic_spectral, the config-0
+distribution) and the balanced m order (M_Balance_v0 on one rank).
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from oracle import oracle as O  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def legendre_case(name, nt, lmax, mvals, nr, nf, seed, sample_tables=None):
+    R = O.Reference()
+    R.legendre_init(nt, lmax, mvals)
+    xh, xl, wh, wl = R.nodes()
+    pe, po, ie, io = R.tables()
+    rec = dict(nt=nt, lmax=lmax, mvals=np.asarray(mvals, np.int32), nr=nr, nf=nf, seed=seed,
+               x_hi=xh, x_lo=xl, w_hi=wh, w_lo=wl)
+    if sample_tables is None:
+        rec.update(p_even=pe, p_odd=po, ip_even=ie, ip_odd=io)
+    else:
+        rng = np.random.default_rng(1000 + seed)
+        for key, arr in (("p_even", pe), ("p_odd", po), ("ip_even", ie), ("ip_odd", io)):
+            idx = np.sort(rng.choice(arr.size, size=min(sample_tables, arr.size), replace=False))
+            rec[key + "_idx"] = idx.astype(np.int64)
+            rec[key + "_val"] = arr[idx]
+    if nr > 0:
+        s = O.synthetic_spectral(lmax, mvals, nr, nf, seed)
+        p = R.stp(nr, nf, s)
+        s2 = R.pts(nr, nf, p)
+        rec.update(s_in=s, p_out=p, s_rt=s2)
+    np.savez_compressed(os.path.join(OUT, name), **rec)
+    print("wrote", name)
+
+
+def cheby_case(name, n, ncols, seed):
+    R = O.Reference()
+    g, w = R.cheby_init(n, 0.35, 1.0)
+    ce, co = R.cheby_tables()
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal(n * ncols)
+    c = R.cheby_to_spectral(f)
+    f2 = R.cheby_from_spectral(c)
+    np.savez_compressed(os.path.join(OUT, name), n=n, ncols=ncols, seed=seed, rmin=0.35, rmax=1.0,
+                        grid=g, ceven=ce, codd=co, f_in=f, c_out=c, f_rt=f2)
+    print("wrote", name)
+
+
+if __name__ == "__main__":
+    if not O.ref_available():
+        sys.exit("compile the reference checker first: make -C oracle ref")
+    # the reference's own unit-test grid (tests/unit_tests/SHT/main_input: N_theta=64 -> l_max=41)
+    legendre_case("legendre_L41_nt64.npz", 64, 41, O.m_order_balanced(41), 2, 1, seed=41)
+    # config-0 grid (l_max=63, N_theta=96), balanced m order, reduced radial/field count
+    legendre_case("legendre_L63_nt96.npz", 96, 63, O.m_order_balanced(63), 2, 1, seed=63, sample_tables=4000)
+    # an arbitrary partial m list (one rank of a row: m values not contiguous)
+    legendre_case("legendre_L63_partial.npz", 96, 63, np.array([17, 0, 63, 5, 40], np.int32), 3, 2, seed=5)
+    # high-l tables where P_mm underflows double at the polar nodes: sampled entries
+    legendre_case("legendre_L511_nt768_sampled.npz", 768, 511, np.arange(512, dtype=np.int32), 0, 0,
+                  seed=511, sample_tables=6000)
+    cheby_case("cheby_n48.npz", 48, 6, seed=48)
+    cheby_case("cheby_n33.npz", 33, 4, seed=33)
