@@ -1,0 +1,222 @@
+"""Headline benchmark: Legendre round trip (StP then PtS) at l_max = 511.
+
+Workload: l_max = 511, N_theta = 768 Gauss-Legendre nodes, N_r = 192, 8 fields,
+all 512 m in Rayleigh's balanced one-rank order; spectral inputs with the
+config-0 distribution (N(0,1) (l+1)^-2 sin(pi r_i / N_r), m = 0 real), synthetic,
+resident in HBM before the timed region.  One step = StP_4d_dgp2 + PtS_4d_dgpv3
+over every m (the HIP library's grouped launches).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,bf16x6]
+
+Prints one JSON line.  value = FP64-equivalent algorithmic TFLOP/s of the round
+trip, 2 * F_LT / step time with F_LT = N_theta N_r N_f (L+1)(L+2).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+LMAX, NTHETA, NR, NF = 511, 768, 192, 8
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 matrix, dense
+PEAK_BF16_TFLOPS = 2500.0    # MI355X BF16 matrix, dense
+METRIC = "SHT round-trip ms + TFLOP/s at l_max=511: FP64 vs split-BF16 at 1e-12 rel err"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def flops_per_direction(lmax=LMAX, nt=NTHETA, nr=NR, nf=NF) -> float:
+    return float(nt) * nr * nf * (lmax + 1) * (lmax + 2)
+
+
+def synthetic_spectral_device(plan, mvals, seed: int):
+    """The config-0 coefficient distribution, generated on the device."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    s = torch.empty(plan.spectral_size, dtype=torch.float64, device="cuda")
+    rfac = torch.sin(torch.pi * torch.arange(1, NR + 1, dtype=torch.float64, device="cuda") / NR)
+    off = 0
+    for m in mvals:
+        m = int(m)
+        nl = LMAX - m + 1
+        lfac = (torch.arange(m, LMAX + 1, dtype=torch.float64, device="cuda") + 1.0) ** -2
+        blk = torch.randn((NF, 2, NR, nl), dtype=torch.float64, device="cuda", generator=g)
+        blk *= lfac.view(1, 1, 1, nl)
+        blk *= rfac.view(1, 1, NR, 1)
+        if m == 0:
+            blk[:, 1] = 0.0
+        s[off: off + blk.numel()] = blk.reshape(-1)
+        off += blk.numel()
+    return s
+
+
+def time_form(ra, form: str, mvals, steps: int, warmup: int):
+    import torch
+    log(f"form {form}: creating plan (l_max={LMAX}, N_theta={NTHETA}, N_r={NR}, N_f={NF})")
+    t0 = time.time()
+    plan = ra.LegendrePlan(NTHETA, LMAX, mvals, NR, NF, precision=form)
+    torch.cuda.synchronize()
+    setup_s = time.time() - t0
+    s = synthetic_spectral_device(plan, mvals, seed=2024)
+    p2 = torch.empty(plan.hybrid_size, dtype=torch.float64, device="cuda")
+    s2 = torch.empty_like(s)
+    log(f"form {form}: plan ready in {setup_s:.2f} s; warmup {warmup}")
+    for _ in range(warmup):
+        plan.stp(s, p2)
+        plan.pts(p2, s2)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    log(f"form {form}: timing {steps} steps")
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record()
+        plan.stp(s, p2)
+        ev[k][1].record()
+        plan.pts(p2, s2)
+        ev[k][2].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    stp_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    pts_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    ms_per_step = wall * 1e3 / steps
+    # the round-trip identity on the full problem: N_phi * PtS(StP(a)) = a
+    rt = float(((s2 * (2 * NTHETA) - s).abs().max() / s.abs().max()).item())
+    log(f"form {form}: {ms_per_step:.3f} ms/step (StP {stp_ms:.3f} ms, PtS {pts_ms:.3f} ms), round trip {rt:.2e}")
+    return dict(plan=plan, s=s, p2=p2, s2=s2, setup_s=setup_s, ms_per_step=ms_per_step, stp_ms=stp_ms,
+                pts_ms=pts_ms, round_trip_rel_err=rt)
+
+
+def reference_check_and_baseline(res, mvals, sample_m: int):
+    """Runs the reference's own FP64 StP/PtS (compiled from its sources into
+    oracle/_ref, when present; else the C restatement) on a sample of the m-blocks:
+    the error of the device results on those blocks, and the CPU timing."""
+    import torch
+    from oracle import oracle as O
+    plan = res["plan"]
+    pos = {int(m): i for i, m in enumerate(mvals)}
+    sample = np.unique(np.linspace(0, LMAX, sample_m).round().astype(np.int32))
+    mps = [pos[int(m)] for m in sample]
+    off = O.spectral_offsets(LMAX, mvals, NR, NF)
+    hyb = NTHETA * 2 * NR * NF
+    s_all = res["s"]
+    s_in = np.concatenate([s_all[off[mp]: off[mp + 1]].cpu().numpy() for mp in mps])
+    p_dev = np.concatenate([res["p2"][mp * hyb: (mp + 1) * hyb].cpu().numpy() for mp in mps])
+    s_dev = np.concatenate([res["s2"][off[mp]: off[mp + 1]].cpu().numpy() for mp in mps])
+    flops = sum(2.0 * 2 * (NTHETA // 2) * 2 * NR * NF * (LMAX - int(m) + 1) for m in sample)
+    threads = min(16, len(os.sched_getaffinity(0)))
+    if O.ref_available():
+        kind = "reference"
+        from threadpoolctl import threadpool_limits
+        R = O.Reference()
+        log(f"cpu baseline: compiled reference, {len(sample)} m-blocks, {threads} threads (setup)")
+        R.legendre_init(NTHETA, LMAX, sample)
+        times = []
+        with threadpool_limits(limits=threads):
+            for _ in range(3):
+                t0 = time.perf_counter()
+                p_ref = R.stp(NR, NF, s_in)
+                s_ref = R.pts(NR, NF, p_dev)
+                times.append(time.perf_counter() - t0)
+    else:
+        kind = "port"
+        threads = 1
+        sample = sample[:: max(1, len(sample) // 6)]
+        mps = [pos[int(m)] for m in sample]
+        s_in = np.concatenate([s_all[off[mp]: off[mp + 1]].cpu().numpy() for mp in mps])
+        p_dev = np.concatenate([res["p2"][mp * hyb: (mp + 1) * hyb].cpu().numpy() for mp in mps])
+        s_dev = np.concatenate([res["s2"][off[mp]: off[mp + 1]].cpu().numpy() for mp in mps])
+        flops = sum(2.0 * 2 * (NTHETA // 2) * 2 * NR * NF * (LMAX - int(m) + 1) for m in sample)
+        log(f"cpu baseline: C restatement, {len(sample)} m-blocks, 1 thread")
+        tabs = O.legendre_tables(NTHETA, LMAX, sample)
+        t0 = time.perf_counter()
+        p_ref = O.stp(NTHETA, LMAX, sample, NR, NF, tabs, s_in)
+        s_ref = O.pts(NTHETA, LMAX, sample, NR, NF, tabs, p_dev)
+        times = [time.perf_counter() - t0]
+    t = float(np.median(times))
+    err_stp = float(np.abs(p_dev - p_ref).max() / np.abs(p_ref).max())
+    err_pts = float(np.abs(s_dev - s_ref).max() / np.abs(s_ref).max())
+    del torch
+    cpu = {"value": flops / t / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": kind,
+           "sample": f"StP+PtS on {len(sample)} of 512 m-blocks (m = {int(sample[0])}..{int(sample[-1])}) at "
+                     f"N_theta={NTHETA}, N_r={NR}, 8 fields; {flops / 1e9:.1f} GFLOP in {t:.3f} s; "
+                     f"full round trip at this rate = {2 * flops_per_direction() / (flops / t) * 1e3:.0f} ms"}
+    log(f"cpu baseline: {cpu['value']:.4f} TFLOP/s; device vs {kind}: StP {err_stp:.2e}, PtS {err_pts:.2e}")
+    return cpu, err_stp, err_pts
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--forms", default="fp64")
+    ap.add_argument("--cpu-sample-m", type=int, default=48)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import rayleigh_amd as ra
+    from oracle import oracle as O
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a GPU")
+    mvals = O.m_order_balanced(LMAX)
+    F = flops_per_direction()
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    results = {}
+    for form in forms:
+        results[form] = time_form(ra, form, mvals, args.steps, args.warmup)
+    head = forms[0]
+    r = results[head]
+    cpu, err_stp, err_pts = (None, None, None)
+    if not args.no_cpu_baseline:
+        cpu, err_stp, err_pts = reference_check_and_baseline(r, mvals, args.cpu_sample_m)
+    dom_ms = max(r["stp_ms"], r["pts_ms"])
+    dom = "stp" if r["stp_ms"] >= r["pts_ms"] else "pts"
+    peak = PEAK_FP64_TFLOPS
+    achieved = F / (dom_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": 2 * F / (r["ms_per_step"] * 1e-3) / 1e12,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "bf16",
+        "data": "synthetic",
+        "config": {"workload": "Legendre round trip StP+PtS, l_max=511, N_theta=768, N_r=192, 8 fields, "
+                               "all m (balanced order)", "l_max": LMAX, "n_theta": NTHETA, "n_r": NR,
+                   "n_fields": NF, "form": head},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None, "kernel": f"{dom}_fp64_kernel",
+                     "kernel_ms": dom_ms},
+        "cpu_baseline": cpu,
+        "rel_err_vs_reference": {"stp": err_stp, "pts": err_pts},
+        "round_trip_rel_err": r["round_trip_rel_err"],
+        "forms": {f: {"ms_per_step": v["ms_per_step"], "stp_ms": v["stp_ms"], "pts_ms": v["pts_ms"],
+                      "tflops": 2 * F / (v["ms_per_step"] * 1e-3) / 1e12,
+                      "round_trip_rel_err": v["round_trip_rel_err"], "setup_s": v["setup_s"]}
+                  for f, v in results.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,128 @@
+/*
+ * raysht -- MI355X-native spectral-transform engine for Rayleigh's per-timestep
+ * transform stack.  Flat C ABI: plain pointers, sizes and int status codes.
+ *
+ * Every entry point names the reference interface (file:line under the
+ * Rayleigh tree) whose body it replaces.  Data layouts are the reference's own
+ * storage, so a caller hands over its buffers unchanged:
+ *
+ *   spectral  packed over mp = 0..n_m-1 in m_values order; block mp holds
+ *             data(m:lmax, 1:nr, 1:2, 1:nf) with l fastest
+ *             (rmcontainer4d, Structures.F90:37-39; Spherical_Buffer.F90:1459-1484)
+ *   hybrid    p2[mp][c][theta], c = r + imi*nr + 2*nr*f, theta = 0..ntheta-1
+ *             ascending cos(theta) from the south pole (Spherical_Buffer.F90:1391-1406)
+ *   physical  p3[f][theta][r][nphi+2]: one in-place real-FFT line per (r, theta, f)
+ *             (Spherical_Buffer.F90:1424-1440)
+ *   radial    cheb buffer [col][n], radius fastest (Chebyshev_Polynomials.F90:375-558)
+ *
+ * Device pointers are HIP device memory on the current device; `stream` is a
+ * hipStream_t (NULL = the default stream).  Calls are asynchronous on the
+ * stream, validate their arguments before launching anything, and return
+ * RAYSHT_OK or an error code (never abort).  A plan is used from one host
+ * thread at a time.
+ */
+#ifndef RAYSHT_H
+#define RAYSHT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct raysht_plan raysht_plan;
+typedef struct raysht_cheb raysht_cheb;
+
+enum raysht_status {
+    RAYSHT_OK = 0,
+    RAYSHT_EINVAL = 1,      /* bad size, layout or argument */
+    RAYSHT_EHIP = 2,        /* the HIP runtime reported an error */
+    RAYSHT_ENOMEM = 3,      /* device or host allocation failed */
+    RAYSHT_EUNSUPPORTED = 4 /* valid request this build does not implement */
+};
+
+/* Arithmetic used by the Legendre contractions. */
+enum raysht_precision {
+    RAYSHT_FP64 = 0,    /* FP64 MFMA */
+    RAYSHT_BF16X3 = 1,  /* 3-slice BF16 split, FP32 accumulation (about 1e-6) */
+    RAYSHT_BF16X6 = 2   /* 6-slice BF16 split, exact FP32 sums, FP64 recombination (about 1e-12) */
+};
+
+const char *raysht_strerror(int status);
+/* Library version as major*10000 + minor*100 + patch. */
+int raysht_version(void);
+
+/* Plan = Initialize_Legendre(nt, lmax, mval, parity=.true.)
+ * (Legendre_Polynomials.F90:123-143): Gauss-Legendre nodes/weights and the
+ * four parity P_lm tables (Compute_Plms :146-258, Parity_Resort :260-331) for
+ * the given m list, resident on the device, for transforms of nr radii and
+ * nfields fields.  ntheta must be even; m_values (host array) may be in any
+ * order (Load_Balance.F90:126-286) but each m in 0..lmax at most once. */
+int raysht_plan_create(raysht_plan **plan, int lmax, int ntheta, const int *m_values, int n_m,
+                       int nr, int nfields, int precision);
+void raysht_plan_destroy(raysht_plan *plan);
+
+/* Sizes (in doubles) of the buffers a plan transforms. */
+size_t raysht_spectral_size(const raysht_plan *plan);
+size_t raysht_hybrid_size(const raysht_plan *plan);
+
+/* Legendre_Transform(s2a, p2a) -> StP_4d_dgp2 (Legendre_Transforms.F90:968-1074).
+ * s: spectral (device), p2: hybrid (device), fully overwritten. */
+int raysht_legendre_stp(raysht_plan *plan, const double *s, double *p2, void *stream);
+
+/* Legendre_Transform(p2b, s2b) -> PtS_4d_dgpv3 (Legendre_Transforms.F90:1077-1175).
+ * p2: hybrid (device), s: spectral (device), every l in m..lmax written. */
+int raysht_legendre_pts(raysht_plan *plan, const double *p2, double *s, void *stream);
+
+/* Host copies of the setup data, for checking.  Nodes/weights as head+tail
+ * doubles of the quad values (ntheta each).  Tables in the reference's stored
+ * orientation, concatenated in mp order: which = 0 p_lm_even (n_e x nt/2, l
+ * fastest), 1 p_lm_odd, 2 ip_lm_even (nt/2 x n_e, theta fastest), 3 ip_lm_odd. */
+int raysht_plan_nodes(const raysht_plan *plan, double *x_hi, double *x_lo, double *w_hi, double *w_lo);
+size_t raysht_plan_table_size(const raysht_plan *plan, int which);
+int raysht_plan_table(const raysht_plan *plan, int which, double *host_out);
+
+/* FFT_To_Physical = c2r_ip4D_fftw (Fourier_Transform.F90:142-183): in-place,
+ * unnormalised complex-to-real transform of `howmany` lines of nphi+2 doubles
+ * (Re/Im of m = 0..nphi/2 in, nphi reals out; Im of m=0 and m=nphi/2 ignored). */
+int raysht_fft_to_physical(int nphi, long howmany, double *x, void *stream);
+
+/* FFT_To_Spectral = r2c_ip4D_fftw (Fourier_Transform.F90:100-140), rescale = 0;
+ * FFT_To_Spectral_RSC = r2c_ip4D_fftw_rsc (:56-98), rescale != 0. */
+int raysht_fft_to_spectral(int nphi, long howmany, double *x, int rescale, void *stream);
+
+/* SphericalBuffer%reform p2a -> p3a = Transpose_2a3a (Spherical_Buffer.F90:196-459)
+ * on one rank: p3 zero-filled, then Re/Im of each m in m_values placed at
+ * 2m, 2m+1 of every line.  nphi >= 2*(lmax+1). */
+int raysht_permute_2a3a(raysht_plan *plan, int nphi, const double *p2, double *p3, void *stream);
+
+/* SphericalBuffer%reform p3b -> p2b = Transpose_3b2b (Spherical_Buffer.F90:461-703)
+ * on one rank (the kill-signal cargo is not carried). */
+int raysht_permute_3b2b(raysht_plan *plan, int nphi, const double *p3, double *p2, void *stream);
+
+/* Fused forms of the same steps (results identical to the separate calls):
+ * StP then 2a->3a then c2r, and r2c then 3b->2b then PtS.  `work` is a hybrid
+ * buffer the call may overwrite. */
+int raysht_spectral_to_physical(raysht_plan *plan, int nphi, const double *s, double *work,
+                                double *p3, void *stream);
+int raysht_physical_to_spectral(raysht_plan *plan, int nphi, double *p3, double *work,
+                                double *s, void *stream);
+
+/* Cheby_Grid%init (Chebyshev_Polynomials.F90:65-207) for one domain of n
+ * zeros-grid points on [rmin, rmax]; grid (host, n) receives the radii. */
+int raysht_cheb_create(raysht_cheb **cheb, int n, double rmin, double rmax, double *grid);
+void raysht_cheb_destroy(raysht_cheb *cheb);
+/* Gen_Tn tables (:251-297) as stored: which = 0 cheby_even (n_x x n_even), 1 cheby_odd. */
+int raysht_cheb_table(const raysht_cheb *cheb, int which, double *host_out);
+
+/* Cheby_Grid%to_spectral = To_Spectral4D (:375-448) over ncols columns of n. */
+int raysht_cheb_to_spectral(raysht_cheb *cheb, const double *f, double *c, long ncols, void *stream);
+/* Cheby_Grid%from_spectral = From_Spectral4D (:450-558). */
+int raysht_cheb_from_spectral(raysht_cheb *cheb, const double *c, double *f, long ncols, void *stream);
+/* Cheby_Grid%dealias_buffer = dealias_buffer4d (:353-374): zero n >= 2n/3 (0-based). */
+int raysht_cheb_dealias(raysht_cheb *cheb, double *c, long ncols, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

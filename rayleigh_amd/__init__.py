@@ -1,0 +1,9 @@
+"""rayleigh_amd -- MI355X-native engine for Rayleigh's spectral-transform stack.
+
+The compute lives in the HIP library libraysht.so (C ABI in include/raysht.h);
+this package loads it and mirrors the reference's Math_Layer interface names.
+"""
+from .transforms import (ChebyGrid, LegendrePlan, fft_to_physical, fft_to_spectral,  # noqa: F401
+                         fft_to_spectral_rsc)
+
+__all__ = ["LegendrePlan", "ChebyGrid", "fft_to_physical", "fft_to_spectral", "fft_to_spectral_rsc"]

@@ -1,0 +1,64 @@
+"""Builds the HIP library rayleigh_amd/libraysht.so for gfx950, in tree.
+
+    python -m rayleigh_amd.build          (or __graft_entry__.build())
+
+Every .hip file under csrc/ is compiled with hipcc for gfx950 only; the small
+quad-precision host part (csrc/host_quad.cpp) with g++.  Objects go to
+rayleigh_amd/_build/ (git-ignored); the shared library sits next to this file.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import glob
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libraysht.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# per-file additions: the double-double table builder must not have products fused into its sums
+FILE_FLAGS = {"legendre_tables.hip": ["-ffp-contract=off"]}
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
+    # packed FP32 instructions stay off in every kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(os.path.dirname(HERE), "include"),
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = []
+    for src in sorted(glob.glob(os.path.join(CSRC, "*.hip"))):
+        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+        extra = FILE_FLAGS.get(os.path.basename(src), [])
+        jobs.append([HIPCC] + HIP_FLAGS + extra + ["-c", src, "-o", obj])
+    host = os.path.join(CSRC, "host_quad.cpp")
+    jobs.append(["g++", "-O2", "-std=gnu++17", "-fPIC", "-Wall", "-c", host,
+                 "-o", os.path.join(BUILD, "host_quad.cpp.o")])
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for r in ex.map(_run, jobs):
+            if verbose and (r.stdout or r.stderr):
+                print(r.stdout + r.stderr, file=sys.stderr)
+    objs = [j[-1] for j in jobs]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lquadmath"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,80 @@
+// Internal declarations shared by the raysht translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/raysht.h"
+
+#define RAYSHT_HIP_TRY(expr)                                  \
+    do {                                                      \
+        hipError_t raysht_e_ = (expr);                        \
+        if (raysht_e_ != hipSuccess) {                        \
+            raysht::set_last_hip_error(raysht_e_, #expr);     \
+            return RAYSHT_EHIP;                               \
+        }                                                     \
+    } while (0)
+
+namespace raysht {
+
+void set_last_hip_error(hipError_t e, const char *what);
+
+// Double-double value of a quad computed on the host (head + tail).
+struct dd_pair {
+    double hi, lo;
+};
+
+// One unit of grouped-GEMM work: m-block mp, row tile start, column tile start.
+struct work_item {
+    int mp;
+    int r0;
+    int c0;
+    int pad;
+};
+
+}  // namespace raysht
+
+struct raysht_plan {
+    int device = 0;
+    int lmax = 0, nt = 0, nt2 = 0, nm = 0, nr = 0, nf = 0, nrhs = 0, precision = 0;
+    std::vector<int> mvals;
+    std::vector<long> s_off;   // spectral offset of block mp (doubles), nm + 1 entries
+    std::vector<long> te_off;  // even-table offset of block mp (doubles), nm + 1 entries
+    std::vector<long> to_off;  // odd-table offset of block mp (doubles), nm + 1 entries
+    std::vector<double> x_hi, x_lo, w_hi, w_lo;
+    // device-resident setup data
+    double *d_pt_even = nullptr, *d_pt_odd = nullptr;  // StP tables P*STP_norm, [k][j] j fastest
+    double *d_ip_even = nullptr, *d_ip_odd = nullptr;  // PtS tables 2 pi w P PTS_norm, [k][j]
+    long *d_s_off = nullptr, *d_te_off = nullptr, *d_to_off = nullptr;
+    int *d_mvals = nullptr;
+    int *d_mp_of_m = nullptr;  // m -> mp, -1 where the plan has no such m
+    raysht::work_item *d_work_stp = nullptr, *d_work_pts = nullptr;
+    int n_work_stp = 0, n_work_pts = 0;
+};
+
+struct raysht_cheb {
+    int device = 0;
+    int n = 0, n_even = 0, n_odd = 0, n_x = 0;
+    std::vector<double> ceven, codd;  // [k][i] i fastest (n_x)
+    double *d_ceven = nullptr, *d_codd = nullptr;
+};
+
+namespace raysht {
+
+// Host-side quad-precision setup (host_quad.cpp): Gauss-Legendre nodes and
+// weights as head/tail pairs, 2*pi*w_j, and the l = m amplitude of each m.
+void gauss_legendre_dd(int n, double *x_hi, double *x_lo, double *w_hi, double *w_lo,
+                       double *renorm_hi, double *renorm_lo);
+void plm_amplitudes_dd(int lmax, double *amp_hi, double *amp_lo);
+void fft_twiddles(int n, double *cos_out, double *sin_out);
+
+// Device launches (each returns RAYSHT_OK or an error code).
+int launch_plm_tables(raysht_plan *p, const double *d_x, const double *d_renorm, const double *d_amp,
+                      hipStream_t st);
+int launch_stp_fp64(raysht_plan *p, const double *s, double *p2, hipStream_t st);
+int launch_pts_fp64(raysht_plan *p, const double *p2, double *s, hipStream_t st);
+
+}  // namespace raysht

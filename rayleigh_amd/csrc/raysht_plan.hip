@@ -1,0 +1,277 @@
+// Plan life cycle and the Legendre entry points of the C ABI.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "raysht_internal.h"
+
+namespace raysht {
+
+int stp_tile_rows();
+int stp_tile_cols();
+int pts_tile_rows();
+int pts_tile_cols();
+int split_plan_init(raysht_plan *p);
+void split_plan_free(raysht_plan *p);
+int launch_stp_split(raysht_plan *p, const double *s, double *p2, hipStream_t st);
+int launch_pts_split(raysht_plan *p, const double *p2, double *s, hipStream_t st);
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename T>
+int device_upload(T **dst, const T *src, size_t n)
+{
+    *dst = nullptr;
+    if (n == 0) return RAYSHT_OK;
+    if (hipMalloc(reinterpret_cast<void **>(dst), n * sizeof(T)) != hipSuccess) {
+        *dst = nullptr;
+        return RAYSHT_ENOMEM;
+    }
+    if (src) RAYSHT_HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return RAYSHT_OK;
+}
+
+template <typename T>
+void device_free(T *&ptr)
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+}
+
+int n_even(int lmax, int m) { return (lmax - m + 2) / 2; }
+int n_odd(int lmax, int m) { return (lmax - m + 1) / 2; }
+
+// Work items of one grouped launch, the largest contractions first so the tail of
+// the grid is made of the short m-blocks.
+std::vector<work_item> make_work(const raysht_plan *p, bool stp)
+{
+    std::vector<int> order(p->nm);
+    for (int i = 0; i < p->nm; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return p->mvals[a] < p->mvals[b]; });
+    std::vector<work_item> w;
+    const int tc = stp ? stp_tile_cols() : pts_tile_cols();
+    for (int mp : order) {
+        const int m = p->mvals[mp];
+        const int rows = stp ? p->nt2 : n_even(p->lmax, m);
+        const int tr = stp ? stp_tile_rows() : pts_tile_rows();
+        for (int r0 = 0; r0 < rows; r0 += tr)
+            for (int c0 = 0; c0 < p->nrhs; c0 += tc) w.push_back(work_item{mp, r0, c0, 0});
+    }
+    return w;
+}
+
+}  // namespace
+
+void set_last_hip_error(hipError_t e, const char *what)
+{
+    g_last_error = std::string(hipGetErrorString(e)) + " in " + what;
+}
+
+}  // namespace raysht
+
+using namespace raysht;
+
+extern "C" {
+
+const char *raysht_strerror(int status)
+{
+    switch (status) {
+    case RAYSHT_OK: return "ok";
+    case RAYSHT_EINVAL: return "invalid argument";
+    case RAYSHT_EHIP: return g_last_error.empty() ? "HIP runtime error" : g_last_error.c_str();
+    case RAYSHT_ENOMEM: return "out of memory";
+    case RAYSHT_EUNSUPPORTED: return "not supported";
+    default: return "unknown status";
+    }
+}
+
+int raysht_version(void) { return 100; }
+
+void raysht_plan_destroy(raysht_plan *p)
+{
+    if (!p) return;
+    device_free(p->d_pt_even);
+    device_free(p->d_pt_odd);
+    device_free(p->d_ip_even);
+    device_free(p->d_ip_odd);
+    device_free(p->d_s_off);
+    device_free(p->d_te_off);
+    device_free(p->d_to_off);
+    device_free(p->d_mvals);
+    device_free(p->d_mp_of_m);
+    device_free(p->d_work_stp);
+    device_free(p->d_work_pts);
+    split_plan_free(p);
+    delete p;
+}
+
+int raysht_plan_create(raysht_plan **out, int lmax, int ntheta, const int *m_values, int n_m, int nr, int nfields,
+                       int precision)
+{
+    if (!out) return RAYSHT_EINVAL;
+    *out = nullptr;
+    if (lmax < 0 || lmax > 8191 || ntheta < 2 || (ntheta & 1) || !m_values || n_m < 1 || n_m > lmax + 1 || nr < 1 ||
+        nfields < 1)
+        return RAYSHT_EINVAL;
+    if (precision != RAYSHT_FP64 && precision != RAYSHT_BF16X3 && precision != RAYSHT_BF16X6) return RAYSHT_EINVAL;
+    if ((long)2 * nr * nfields > (1L << 30)) return RAYSHT_EINVAL;
+    std::vector<int> seen(lmax + 1, -1);
+    for (int i = 0; i < n_m; ++i) {
+        const int m = m_values[i];
+        if (m < 0 || m > lmax || seen[m] >= 0) return RAYSHT_EINVAL;
+        seen[m] = i;
+    }
+
+    int device = 0;
+    RAYSHT_HIP_TRY(hipGetDevice(&device));
+    raysht_plan *p = new (std::nothrow) raysht_plan();
+    if (!p) return RAYSHT_ENOMEM;
+    int rc = RAYSHT_OK;
+    p->device = device;
+    p->lmax = lmax;
+    p->nt = ntheta;
+    p->nt2 = ntheta / 2;
+    p->nm = n_m;
+    p->nr = nr;
+    p->nf = nfields;
+    p->nrhs = 2 * nr * nfields;
+    p->precision = precision;
+    p->mvals.assign(m_values, m_values + n_m);
+    p->s_off.assign(n_m + 1, 0);
+    p->te_off.assign(n_m + 1, 0);
+    p->to_off.assign(n_m + 1, 0);
+    for (int i = 0; i < n_m; ++i) {
+        const int m = m_values[i];
+        p->s_off[i + 1] = p->s_off[i] + (long)(lmax - m + 1) * p->nrhs;
+        p->te_off[i + 1] = p->te_off[i] + (long)n_even(lmax, m) * p->nt2;
+        p->to_off[i + 1] = p->to_off[i] + (long)n_odd(lmax, m) * p->nt2;
+    }
+
+    // quad-precision setup on the host
+    p->x_hi.resize(ntheta);
+    p->x_lo.resize(ntheta);
+    p->w_hi.resize(ntheta);
+    p->w_lo.resize(ntheta);
+    std::vector<double> rn_hi(ntheta), rn_lo(ntheta), amp_hi(lmax + 1), amp_lo(lmax + 1);
+    gauss_legendre_dd(ntheta, p->x_hi.data(), p->x_lo.data(), p->w_hi.data(), p->w_lo.data(), rn_hi.data(),
+                      rn_lo.data());
+    plm_amplitudes_dd(lmax, amp_hi.data(), amp_lo.data());
+    std::vector<double> x_dd(2 * p->nt2), rn_dd(2 * p->nt2), amp_dd(2 * (lmax + 1));
+    for (int j = 0; j < p->nt2; ++j) {
+        x_dd[2 * j] = p->x_hi[j];
+        x_dd[2 * j + 1] = p->x_lo[j];
+        rn_dd[2 * j] = rn_hi[j];
+        rn_dd[2 * j + 1] = rn_lo[j];
+    }
+    for (int m = 0; m <= lmax; ++m) {
+        amp_dd[2 * m] = amp_hi[m];
+        amp_dd[2 * m + 1] = amp_lo[m];
+    }
+
+    std::vector<work_item> ws = make_work(p, true), wp = make_work(p, false);
+    p->n_work_stp = (int)ws.size();
+    p->n_work_pts = (int)wp.size();
+
+    double *d_x = nullptr, *d_rn = nullptr, *d_amp = nullptr;
+    const size_t ne_tot = p->te_off[n_m], no_tot = p->to_off[n_m];
+    if ((rc = device_upload(&p->d_s_off, p->s_off.data(), p->s_off.size())) ||
+        (rc = device_upload(&p->d_te_off, p->te_off.data(), p->te_off.size())) ||
+        (rc = device_upload(&p->d_to_off, p->to_off.data(), p->to_off.size())) ||
+        (rc = device_upload(&p->d_mvals, p->mvals.data(), p->mvals.size())) ||
+        (rc = device_upload(&p->d_mp_of_m, seen.data(), seen.size())) ||
+        (rc = device_upload(&p->d_work_stp, ws.data(), ws.size())) ||
+        (rc = device_upload(&p->d_work_pts, wp.data(), wp.size())) ||
+        (rc = device_upload(&p->d_pt_even, (const double *)nullptr, ne_tot)) ||
+        (rc = device_upload(&p->d_ip_even, (const double *)nullptr, ne_tot)) ||
+        (rc = device_upload(&p->d_pt_odd, (const double *)nullptr, std::max<size_t>(no_tot, 1))) ||
+        (rc = device_upload(&p->d_ip_odd, (const double *)nullptr, std::max<size_t>(no_tot, 1))) ||
+        (rc = device_upload(&d_x, x_dd.data(), x_dd.size())) ||
+        (rc = device_upload(&d_rn, rn_dd.data(), rn_dd.size())) ||
+        (rc = device_upload(&d_amp, amp_dd.data(), amp_dd.size()))) {
+        device_free(d_x);
+        device_free(d_rn);
+        device_free(d_amp);
+        raysht_plan_destroy(p);
+        return rc;
+    }
+    rc = launch_plm_tables(p, d_x, d_rn, d_amp, nullptr);
+    if (rc == RAYSHT_OK && hipDeviceSynchronize() != hipSuccess) rc = RAYSHT_EHIP;
+    device_free(d_x);
+    device_free(d_rn);
+    device_free(d_amp);
+    if (rc == RAYSHT_OK && precision != RAYSHT_FP64) rc = split_plan_init(p);
+    if (rc != RAYSHT_OK) {
+        raysht_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return RAYSHT_OK;
+}
+
+size_t raysht_spectral_size(const raysht_plan *p) { return p ? (size_t)p->s_off[p->nm] : 0; }
+
+size_t raysht_hybrid_size(const raysht_plan *p) { return p ? (size_t)p->nt * p->nrhs * p->nm : 0; }
+
+int raysht_legendre_stp(raysht_plan *p, const double *s, double *p2, void *stream)
+{
+    if (!p || !s || !p2) return RAYSHT_EINVAL;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (p->precision == RAYSHT_FP64) return launch_stp_fp64(p, s, p2, st);
+    return launch_stp_split(p, s, p2, st);
+}
+
+int raysht_legendre_pts(raysht_plan *p, const double *p2, double *s, void *stream)
+{
+    if (!p || !s || !p2) return RAYSHT_EINVAL;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (p->precision == RAYSHT_FP64) return launch_pts_fp64(p, p2, s, st);
+    return launch_pts_split(p, p2, s, st);
+}
+
+int raysht_plan_nodes(const raysht_plan *p, double *x_hi, double *x_lo, double *w_hi, double *w_lo)
+{
+    if (!p || !x_hi || !x_lo || !w_hi || !w_lo) return RAYSHT_EINVAL;
+    std::memcpy(x_hi, p->x_hi.data(), sizeof(double) * p->nt);
+    std::memcpy(x_lo, p->x_lo.data(), sizeof(double) * p->nt);
+    std::memcpy(w_hi, p->w_hi.data(), sizeof(double) * p->nt);
+    std::memcpy(w_lo, p->w_lo.data(), sizeof(double) * p->nt);
+    return RAYSHT_OK;
+}
+
+size_t raysht_plan_table_size(const raysht_plan *p, int which)
+{
+    if (!p || which < 0 || which > 3) return 0;
+    return (which % 2 == 0) ? (size_t)p->te_off[p->nm] : (size_t)p->to_off[p->nm];
+}
+
+int raysht_plan_table(const raysht_plan *p, int which, double *host_out)
+{
+    if (!p || !host_out || which < 0 || which > 3) return RAYSHT_EINVAL;
+    const bool even = (which % 2 == 0);
+    const size_t n = raysht_plan_table_size(p, which);
+    if (n == 0) return RAYSHT_OK;
+    const double *src = which == 0 ? p->d_pt_even : which == 1 ? p->d_pt_odd : which == 2 ? p->d_ip_even : p->d_ip_odd;
+    std::vector<double> tmp(n);
+    RAYSHT_HIP_TRY(hipMemcpy(tmp.data(), src, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (which >= 2) {
+        std::memcpy(host_out, tmp.data(), n * sizeof(double));  // [k][j] = ip_lm(1:nt2, 1:n)
+        return RAYSHT_OK;
+    }
+    // device StP tables are [k][j]; the reference stores p_lm(1:n, 1:nt2), l fastest
+    const std::vector<long> &off = even ? p->te_off : p->to_off;
+    for (int mp = 0; mp < p->nm; ++mp) {
+        const int m = p->mvals[mp];
+        const int nk = even ? n_even(p->lmax, m) : n_odd(p->lmax, m);
+        const double *b = tmp.data() + off[mp];
+        double *o = host_out + off[mp];
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < p->nt2; ++j) o[(long)j * nk + k] = b[(long)k * p->nt2 + j];
+    }
+    return RAYSHT_OK;
+}
+
+}  // extern "C"

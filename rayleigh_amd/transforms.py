@@ -1,0 +1,208 @@
+"""Python mirror of Rayleigh's Math_Layer transform interface over the HIP library.
+
+The names follow the reference's Fortran interface so that code (and tests)
+written against it read the same:
+
+  LegendrePlan(n_theta, l_max, m_values, n_r, n_fields)   Initialize_Legendre (Legendre_Polynomials.F90:123)
+  plan.legendre_transform(s2a, p2a)                        StP_4d_dgp2  (Legendre_Transforms.F90:968)
+  plan.legendre_transform(p2b, s2b)                        PtS_4d_dgpv3 (Legendre_Transforms.F90:1077)
+  fft_to_physical(x, n_phi) / fft_to_spectral(x, n_phi)    Fourier_Transform.F90:25-35
+  fft_to_spectral_rsc(x, n_phi)                            Fourier_Transform.F90:56-98
+  plan.reform_2a3a(p2a, p3a) / plan.reform_3b2b(p3b, p2b)  Spherical_Buffer.F90:196-703 (one rank)
+  ChebyGrid(n, r_min, r_max).to_spectral / from_spectral / dealias_buffer
+                                                           Chebyshev_Polynomials.F90:65-558
+
+All data are float64 tensors on a HIP device in the reference's layouts (see
+include/raysht.h); every call is asynchronous on torch's current stream.  There
+is no CPU path: the HIP library must be built and loadable.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+PRECISIONS = {"fp64": 0, "bf16x3": 1, "bf16x6": 2}
+
+
+def _stream(stream=None) -> ctypes.c_void_p:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _dptr(t, name: str, numel: int | None = None) -> ctypes.c_void_p:
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a device tensor")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name} must be float64")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name} has {t.numel()} elements, expected {numel}")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class LegendrePlan:
+    """Initialize_Legendre(n_theta, l_max, m_values, parity=.true.) plus the transform
+    sizes (n_r radii, n_fields fields) the plan's transforms take."""
+
+    def __init__(self, n_theta: int, l_max: int, m_values, n_r: int, n_fields: int, precision: str = "fp64"):
+        lib = _lib.lib()
+        self.n_theta, self.l_max, self.n_r, self.n_fields = int(n_theta), int(l_max), int(n_r), int(n_fields)
+        self.m_values = np.ascontiguousarray(m_values, dtype=np.int32)
+        self.precision = precision
+        self.nrhs = 2 * self.n_r * self.n_fields
+        h = ctypes.c_void_p()
+        st = lib.raysht_plan_create(ctypes.byref(h), self.l_max, self.n_theta,
+                                    self.m_values.ctypes.data_as(ctypes.c_void_p), len(self.m_values),
+                                    self.n_r, self.n_fields, PRECISIONS[precision])
+        _lib.check(st, "raysht_plan_create")
+        self._h = h
+        self.spectral_size = int(lib.raysht_spectral_size(h))
+        self.hybrid_size = int(lib.raysht_hybrid_size(h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().raysht_plan_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    # --- transforms -------------------------------------------------------
+    def stp(self, s, p2, stream=None):
+        _lib.check(_lib.lib().raysht_legendre_stp(self._h, _dptr(s, "s", self.spectral_size),
+                                                  _dptr(p2, "p2", self.hybrid_size), _stream(stream)),
+                   "raysht_legendre_stp")
+
+    def pts(self, p2, s, stream=None):
+        _lib.check(_lib.lib().raysht_legendre_pts(self._h, _dptr(p2, "p2", self.hybrid_size),
+                                                  _dptr(s, "s", self.spectral_size), _stream(stream)),
+                   "raysht_legendre_pts")
+
+    def legendre_transform(self, data_in, data_out, stream=None):
+        """The generic of Legendre_Transforms.F90:37-40, resolved by which argument is spectral."""
+        if data_in.numel() == self.spectral_size and data_out.numel() == self.hybrid_size:
+            return self.stp(data_in, data_out, stream)
+        if data_in.numel() == self.hybrid_size and data_out.numel() == self.spectral_size:
+            return self.pts(data_in, data_out, stream)
+        raise ValueError("legendre_transform: arguments are neither (spectral, hybrid) nor (hybrid, spectral)")
+
+    def reform_2a3a(self, p2a, p3a, n_phi: int, stream=None):
+        n = (n_phi + 2) * self.n_r * self.n_theta * self.n_fields
+        _lib.check(_lib.lib().raysht_permute_2a3a(self._h, int(n_phi), _dptr(p2a, "p2a", self.hybrid_size),
+                                                  _dptr(p3a, "p3a", n), _stream(stream)), "raysht_permute_2a3a")
+
+    def reform_3b2b(self, p3b, p2b, n_phi: int, stream=None):
+        n = (n_phi + 2) * self.n_r * self.n_theta * self.n_fields
+        _lib.check(_lib.lib().raysht_permute_3b2b(self._h, int(n_phi), _dptr(p3b, "p3b", n),
+                                                  _dptr(p2b, "p2b", self.hybrid_size), _stream(stream)),
+                   "raysht_permute_3b2b")
+
+    def spectral_to_physical(self, s, work, p3, n_phi: int, stream=None):
+        n = (n_phi + 2) * self.n_r * self.n_theta * self.n_fields
+        _lib.check(_lib.lib().raysht_spectral_to_physical(self._h, int(n_phi), _dptr(s, "s", self.spectral_size),
+                                                          _dptr(work, "work", self.hybrid_size),
+                                                          _dptr(p3, "p3", n), _stream(stream)),
+                   "raysht_spectral_to_physical")
+
+    def physical_to_spectral(self, p3, work, s, n_phi: int, stream=None):
+        n = (n_phi + 2) * self.n_r * self.n_theta * self.n_fields
+        _lib.check(_lib.lib().raysht_physical_to_spectral(self._h, int(n_phi), _dptr(p3, "p3", n),
+                                                          _dptr(work, "work", self.hybrid_size),
+                                                          _dptr(s, "s", self.spectral_size), _stream(stream)),
+                   "raysht_physical_to_spectral")
+
+    # --- setup data, for checking ------------------------------------------
+    def nodes(self):
+        out = [np.zeros(self.n_theta) for _ in range(4)]
+        _lib.check(_lib.lib().raysht_plan_nodes(self._h, *[o.ctypes.data_as(ctypes.c_void_p) for o in out]),
+                   "raysht_plan_nodes")
+        return tuple(out)
+
+    def table(self, which: int) -> np.ndarray:
+        n = int(_lib.lib().raysht_plan_table_size(self._h, which))
+        out = np.zeros(max(n, 1))
+        _lib.check(_lib.lib().raysht_plan_table(self._h, which, out.ctypes.data_as(ctypes.c_void_p)),
+                   "raysht_plan_table")
+        return out[:n]
+
+    def tables(self):
+        """(p_lm_even, p_lm_odd, ip_lm_even, ip_lm_odd) in the reference's stored orientation."""
+        return tuple(self.table(w) for w in range(4))
+
+
+def fft_to_physical(x, n_phi: int, stream=None):
+    """c2r_ip4D_fftw: in place on lines of n_phi+2 doubles, unnormalised."""
+    if x.numel() % (n_phi + 2):
+        raise ValueError("x is not a whole number of lines of n_phi+2")
+    _lib.check(_lib.lib().raysht_fft_to_physical(int(n_phi), x.numel() // (n_phi + 2), _dptr(x, "x"),
+                                                 _stream(stream)), "raysht_fft_to_physical")
+
+
+def fft_to_spectral(x, n_phi: int, stream=None, rsc: bool = False):
+    """r2c_ip4D_fftw (rsc is accepted and ignored, as in the reference)."""
+    del rsc
+    if x.numel() % (n_phi + 2):
+        raise ValueError("x is not a whole number of lines of n_phi+2")
+    _lib.check(_lib.lib().raysht_fft_to_spectral(int(n_phi), x.numel() // (n_phi + 2), _dptr(x, "x"), 0,
+                                                 _stream(stream)), "raysht_fft_to_spectral")
+
+
+def fft_to_spectral_rsc(x, n_phi: int, stream=None):
+    """r2c_ip4D_fftw_rsc: r2c, then x(3:) /= n and x(1) /= 2n."""
+    if x.numel() % (n_phi + 2):
+        raise ValueError("x is not a whole number of lines of n_phi+2")
+    _lib.check(_lib.lib().raysht_fft_to_spectral(int(n_phi), x.numel() // (n_phi + 2), _dptr(x, "x"), 1,
+                                                 _stream(stream)), "raysht_fft_to_spectral")
+
+
+class ChebyGrid:
+    """Cheby_Grid with one domain of n zeros-grid points on [r_min, r_max]."""
+
+    def __init__(self, n: int, r_min: float, r_max: float):
+        self.n = int(n)
+        self.grid = np.zeros(self.n)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().raysht_cheb_create(ctypes.byref(h), self.n, float(r_min), float(r_max),
+                                                 self.grid.ctypes.data_as(ctypes.c_void_p)), "raysht_cheb_create")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().raysht_cheb_destroy(h)
+            self._h = None
+
+    def tables(self):
+        nx = self.n - self.n // 2
+        ce, co = np.zeros(nx * nx), np.zeros(max(nx * (self.n // 2), 1))
+        _lib.check(_lib.lib().raysht_cheb_table(self._h, 0, ce.ctypes.data_as(ctypes.c_void_p)), "cheb_table")
+        _lib.check(_lib.lib().raysht_cheb_table(self._h, 1, co.ctypes.data_as(ctypes.c_void_p)), "cheb_table")
+        return ce, co[: nx * (self.n // 2)]
+
+    def to_spectral(self, f_in, c_out, stream=None):
+        if f_in.numel() % self.n or f_in.numel() != c_out.numel():
+            raise ValueError("buffers must hold whole columns of n and match")
+        _lib.check(_lib.lib().raysht_cheb_to_spectral(self._h, _dptr(f_in, "f_in"), _dptr(c_out, "c_out"),
+                                                      f_in.numel() // self.n, _stream(stream)),
+                   "raysht_cheb_to_spectral")
+
+    def from_spectral(self, c_in, f_out, stream=None):
+        if c_in.numel() % self.n or c_in.numel() != f_out.numel():
+            raise ValueError("buffers must hold whole columns of n and match")
+        _lib.check(_lib.lib().raysht_cheb_from_spectral(self._h, _dptr(c_in, "c_in"), _dptr(f_out, "f_out"),
+                                                        c_in.numel() // self.n, _stream(stream)),
+                   "raysht_cheb_from_spectral")
+
+    def dealias_buffer(self, c, stream=None):
+        if c.numel() % self.n:
+            raise ValueError("buffer must hold whole columns of n")
+        _lib.check(_lib.lib().raysht_cheb_dealias(self._h, _dptr(c, "c"), c.numel() // self.n, _stream(stream)),
+                   "raysht_cheb_dealias")

@@ -1,0 +1,151 @@
+"""GPU parity of the phi FFTs, the row permutes and the Chebyshev transforms."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_helpers import dev, empty, host, rel_max
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ra():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import rayleigh_amd
+    return rayleigh_amd
+
+
+@pytest.mark.parametrize("nphi", [6, 12, 96, 192, 768, 1536, 3072])
+def test_fft_c2r_r2c_vs_oracle(ra, nphi):
+    rng = np.random.default_rng(nphi)
+    nl = 37
+    x = rng.standard_normal((nl, nphi + 2))
+    want = O.fft_c2r(nphi, x.ravel())
+    t = dev(x)
+    ra.fft_to_physical(t, nphi)
+    got = host(t).ravel()
+    assert rel_max(got, want) < 1e-14 * np.log2(nphi) * 4
+    y = np.zeros((nl, nphi + 2))
+    y[:, :nphi] = rng.standard_normal((nl, nphi))
+    want = O.fft_r2c(nphi, y.ravel())
+    t = dev(y)
+    ra.fft_to_spectral(t, nphi)
+    assert rel_max(host(t).ravel(), want) < 1e-14 * np.log2(nphi) * 4
+    want = O.fft_r2c(nphi, y.ravel(), rescale=True)
+    t = dev(y)
+    ra.fft_to_spectral_rsc(t, nphi)
+    assert rel_max(host(t).ravel(), want) < 1e-14 * np.log2(nphi) * 4
+
+
+def test_fft_round_trip_large_batch(ra):
+    import torch
+    nphi, nlines = 768, 384 * 16
+    g = torch.Generator(device="cuda").manual_seed(1)
+    x = torch.zeros(nlines, nphi + 2, dtype=torch.float64, device="cuda")
+    x[:, :nphi] = torch.randn(nlines, nphi, dtype=torch.float64, device="cuda", generator=g)
+    x0 = x.clone()
+    ra.fft_to_spectral(x.view(-1), nphi)
+    ra.fft_to_physical(x.view(-1), nphi)
+    torch.cuda.synchronize()
+    err = ((x[:, :nphi] / nphi - x0[:, :nphi]).abs().max() / x0.abs().max()).item()
+    assert err < 1e-14
+
+
+@pytest.mark.parametrize("nt,lmax,nr,nf", [(12, 7, 3, 2), (96, 63, 4, 2)])
+def test_permutes_vs_oracle(ra, nt, lmax, nr, nf):
+    nphi = 2 * nt
+    mv = O.m_order_balanced(lmax)
+    plan = ra.LegendrePlan(nt, lmax, mv, nr, nf)
+    rng = np.random.default_rng(nt)
+    p2 = rng.standard_normal(plan.hybrid_size)
+    want = O.permute_2a3a(nt, lmax, mv, nr, nf, nphi, p2)
+    p3 = empty(want.size)
+    plan.reform_2a3a(dev(p2), p3, nphi)
+    assert np.array_equal(host(p3), want)
+    p3b = rng.standard_normal(want.size)
+    want2 = O.permute_3b2b(nt, lmax, mv, nr, nf, nphi, p3b)
+    p2b = empty(plan.hybrid_size)
+    plan.reform_3b2b(dev(p3b), p2b, nphi)
+    assert np.array_equal(host(p2b), want2)
+
+
+def test_full_stack_loop(ra):
+    """Test_SHT.F90:449-575 (test_LT_FFT): StP -> 2a3a -> c2r -> r2c -> 3b2b -> PtS is the
+    identity on band-limited data, and the fused entry points equal the separate steps."""
+    nt, L, nr, nf = 96, 63, 4, 5
+    nphi = 2 * nt
+    mv = O.m_order_balanced(L)
+    plan = ra.LegendrePlan(nt, L, mv, nr, nf)
+    off = O.spectral_offsets(L, mv, nr, nf)
+    s = np.zeros(off[-1])
+    for mp, m in enumerate(mv):
+        blk = s[off[mp]: off[mp + 1]].reshape(nf, 2, nr, L - m + 1)
+        for f in range(1, nf + 1):
+            for r in range(1, nr + 1):
+                blk[f - 1, 0, r - 1, : L - m] = (np.arange(m, L) + 1.0) * (2 * f + 3) * r
+                blk[f - 1, 1, r - 1, : L - m] = (m + 1.0) * (4 * f + 5) * r * r
+        # per-m normalisation; the m = l_max block is all zero (only l = l_max, which is zeroed)
+        n1, n2 = np.abs(blk[:, 0]).max(), np.abs(blk[:, 1]).max()
+        if n1 > 0:
+            blk[:, 0] /= n1
+        if n2 > 0:
+            blk[:, 1] /= n2
+        if m == 0:
+            blk[:, 1] = 0.0
+    n3 = (nphi + 2) * nr * nt * nf
+    w = empty(plan.hybrid_size)
+    p3 = empty(n3)
+    plan.spectral_to_physical(dev(s), w, p3, nphi)
+    # the same through the separate entry points
+    p2a = empty(plan.hybrid_size)
+    plan.stp(dev(s), p2a)
+    p3s = empty(n3)
+    plan.reform_2a3a(p2a, p3s, nphi)
+    ra.fft_to_physical(p3s, nphi)
+    assert np.array_equal(host(p3), host(p3s))
+    # against the oracle's stack
+    tabs = O.legendre_tables(nt, L, mv)
+    p3o = O.fft_c2r(nphi, O.permute_2a3a(nt, L, mv, nr, nf, nphi, O.stp(nt, L, mv, nr, nf, tabs, s)))
+    assert rel_max(host(p3), p3o) < 1e-13
+    s2 = empty(plan.spectral_size)
+    plan.physical_to_spectral(p3, w, s2, nphi)
+    s2o = O.pts(nt, L, mv, nr, nf, tabs, O.permute_3b2b(nt, L, mv, nr, nf, nphi, O.fft_r2c(nphi, p3o)))
+    assert rel_max(host(s2), s2o) < 1e-13
+    assert np.abs(host(s2) - s).max() < 1e-10  # Test_SHT.F90:34 passing tolerance
+
+
+@pytest.mark.parametrize("n", [48, 33, 128, 256])
+def test_cheby_vs_oracle(ra, n):
+    grid = ra.ChebyGrid(n, 0.35, 1.0)
+    assert np.array_equal(grid.grid, O.cheby_grid(n, 0.35, 1.0))
+    ce, co = grid.tables()
+    oe, oo = O.cheby_tables(n)
+    assert np.array_equal(ce, oe) and np.array_equal(co, oo[: co.size])
+    rng = np.random.default_rng(n)
+    ncols = 1000
+    f = rng.standard_normal(n * ncols)
+    c = empty(f.size)
+    grid.to_spectral(dev(f), c)
+    want = O.cheby_to_spectral(n, f)
+    assert rel_max(host(c), want) < 1e-14
+    f2 = empty(f.size)
+    grid.from_spectral(dev(want), f2)
+    assert rel_max(host(f2), O.cheby_from_spectral(n, want)) < 1e-14
+    d = dev(want)
+    grid.dealias_buffer(d)
+    assert np.array_equal(host(d), O.cheby_dealias(n, want))
+
+
+def test_cheby_vs_reference_fixture(ra, golden):
+    for name in ("cheby_n48.npz", "cheby_n33.npz"):
+        g = golden(name)
+        n = int(g["n"])
+        grid = ra.ChebyGrid(n, float(g["rmin"]), float(g["rmax"]))
+        c = empty(g["f_in"].size)
+        grid.to_spectral(dev(g["f_in"]), c)
+        assert rel_max(host(c), g["c_out"]) < 1e-14
+        f = empty(g["c_out"].size)
+        grid.from_spectral(dev(g["c_out"]), f)
+        assert rel_max(host(f), g["f_rt"]) < 1e-14
