@@ -162,6 +162,8 @@ def main():
     ap.add_argument("--forms", default="fp64")
     ap.add_argument("--cpu-sample-m", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain 1 GiB device-to-device copy (scales profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -179,6 +181,14 @@ def main():
     results = {}
     for form in forms:
         results[form] = time_form(ra, form, mvals, args.steps, args.warmup)
+    if args.copy_calibration:
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written by one copy kernel")
+        del a, b
     head = forms[0]
     r = results[head]
     cpu, err_stp, err_pts = (None, None, None)

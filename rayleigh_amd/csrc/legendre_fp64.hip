@@ -33,22 +33,31 @@ __device__ __forceinline__ double4_t mfma_f64(double a, double b, double4_t c)
 constexpr int kThreads = 256;
 
 // ---------------------------------------------------------------- StP
-// tile: 64 columns (c) x 64 southern nodes (j); K chunk = 16 parity-l values
-constexpr int STP_TC = 64;
+// Workgroup tile: 128 columns (c) x 64 southern nodes (j), both parities.
+// Wave w owns columns [32w, 32w+32): 2 column blocks x 4 node blocks x 2 parities
+// = 16 accumulators of 16x16.  K runs over chunks of 16 parity-l values (32 l);
+// the next chunk is fetched into registers while the current one is multiplied
+// out of LDS.
+constexpr int STP_TC = 128;
 constexpr int STP_TJ = 64;
 constexpr int STP_KC = 16;
+constexpr int STP_BS_LD = 2 * STP_KC + 1;  // Bs row stride (doubles)
+constexpr int STP_A_LD = STP_TJ + 16;      // Ae/Ao row stride (doubles)
+constexpr int STP_B_PER_T = STP_TC * 2 * STP_KC / kThreads;  // 16
+constexpr int STP_A_PER_T = STP_KC * STP_TJ / kThreads;      // 4 (per parity)
 
-__global__ __launch_bounds__(kThreads) void stp_fp64_kernel(
+__global__ __launch_bounds__(kThreads, 1) void stp_fp64_kernel(
     const double *__restrict__ s, double *__restrict__ p2, const double *__restrict__ pte,
     const double *__restrict__ pto, const long *__restrict__ s_off, const long *__restrict__ te_off,
     const long *__restrict__ to_off, const int *__restrict__ mvals, const work_item *__restrict__ work, int lmax,
     int nt, int nt2, int nrhs)
 {
-    __shared__ double Ae[STP_KC][STP_TJ];
-    __shared__ double Ao[STP_KC][STP_TJ];
-    __shared__ double Bs[STP_TC][2 * STP_KC + 1];
+    __shared__ double Ae[STP_KC * STP_A_LD];
+    __shared__ double Ao[STP_KC * STP_A_LD];
+    __shared__ double Bs[STP_TC * STP_BS_LD];
 
     const work_item w = work[blockIdx.x];
+    if (w.mp < 0) return;  // padding item of the XCD-balanced schedule
     const int mp = w.mp, j0 = w.r0, c0 = w.c0;
     const int m = mvals[mp];
     const int nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
@@ -58,35 +67,71 @@ __global__ __launch_bounds__(kThreads) void stp_fp64_kernel(
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
 
-    double4_t acc_e[4], acc_o[4];
+    double rb[STP_B_PER_T], rae[STP_A_PER_T], rao[STP_A_PER_T];
+    auto fetch = [&](int k0) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        acc_e[b] = double4_t{0.0, 0.0, 0.0, 0.0};
-        acc_o[b] = double4_t{0.0, 0.0, 0.0, 0.0};
-    }
-
-    for (int k0 = 0; k0 < ne; k0 += STP_KC) {
-        for (int idx = tid; idx < STP_KC * STP_TJ; idx += kThreads) {
+        for (int i = 0; i < STP_B_PER_T; ++i) {
+            const int idx = tid + i * kThreads;
+            const int cc = idx / (2 * STP_KC), q = idx % (2 * STP_KC);
+            const int c = c0 + cc, l = 2 * k0 + q;
+            rb[i] = (c < nrhs && l < nl) ? sb[(long)c * nl + l] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < STP_A_PER_T; ++i) {
+            const int idx = tid + i * kThreads;
             const int kk = idx / STP_TJ, jj = idx % STP_TJ;
             const int k = k0 + kk, j = j0 + jj;
             const bool jin = j < nt2;
-            Ae[kk][jj] = (jin && k < ne) ? pe[(long)k * nt2 + j] : 0.0;
-            Ao[kk][jj] = (jin && k < no) ? po[(long)k * nt2 + j] : 0.0;
+            rae[i] = (jin && k < ne) ? pe[(long)k * nt2 + j] : 0.0;
+            rao[i] = (jin && k < no) ? po[(long)k * nt2 + j] : 0.0;
         }
-        for (int idx = tid; idx < STP_TC * 2 * STP_KC; idx += kThreads) {
-            const int cc = idx / (2 * STP_KC), q = idx % (2 * STP_KC);
-            const int c = c0 + cc, l = 2 * k0 + q;
-            Bs[cc][q] = (c < nrhs && l < nl) ? sb[(long)c * nl + l] : 0.0;
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int i = 0; i < STP_B_PER_T; ++i) {
+            const int idx = tid + i * kThreads;
+            Bs[(idx / (2 * STP_KC)) * STP_BS_LD + idx % (2 * STP_KC)] = rb[i];
         }
+#pragma unroll
+        for (int i = 0; i < STP_A_PER_T; ++i) {
+            const int idx = tid + i * kThreads;
+            Ae[(idx / STP_TJ) * STP_A_LD + idx % STP_TJ] = rae[i];
+            Ao[(idx / STP_TJ) * STP_A_LD + idx % STP_TJ] = rao[i];
+        }
+    };
+
+    double4_t acc_e[2][4], acc_o[2][4];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            acc_e[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+            acc_o[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+        }
+
+    fetch(0);
+    for (int k0 = 0; k0 < ne; k0 += STP_KC) {
+        stash();
         __syncthreads();
+        if (k0 + STP_KC < ne) fetch(k0 + STP_KC);
 #pragma unroll
         for (int kk = 0; kk < STP_KC; kk += 4) {
-            const double a_e = Bs[16 * wave + li][2 * (kk + lg)];
-            const double a_o = Bs[16 * wave + li][2 * (kk + lg) + 1];
+            double be[4], bo[4];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                acc_e[b] = mfma_f64(a_e, Ae[kk + lg][16 * b + li], acc_e[b]);
-                acc_o[b] = mfma_f64(a_o, Ao[kk + lg][16 * b + li], acc_o[b]);
+                be[b] = Ae[(kk + lg) * STP_A_LD + 16 * b + li];
+                bo[b] = Ao[(kk + lg) * STP_A_LD + 16 * b + li];
+            }
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                const int row = (32 * wave + 16 * cb + li) * STP_BS_LD + 2 * (kk + lg);
+                const double a_e = Bs[row];
+                const double a_o = Bs[row + 1];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    acc_e[cb][b] = mfma_f64(a_e, be[b], acc_e[cb][b]);
+                    acc_o[cb][b] = mfma_f64(a_o, bo[b], acc_o[cb][b]);
+                }
             }
         }
         __syncthreads();
@@ -94,38 +139,48 @@ __global__ __launch_bounds__(kThreads) void stp_fp64_kernel(
 
     double *ob = p2 + (long)mp * nrhs * nt;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int j = j0 + 16 * b + li;
+    for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int c = c0 + 16 * wave + lg + 4 * v;
-            if (c < nrhs && j < nt2) {
-                const double e = acc_e[b][v], o = acc_o[b][v];
-                ob[(long)c * nt + j] = e + o;
-                ob[(long)c * nt + (nt - 1 - j)] = e - o;
+        for (int b = 0; b < 4; ++b) {
+            const int j = j0 + 16 * b + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = c0 + 32 * wave + 16 * cb + lg + 4 * v;
+                if (c < nrhs && j < nt2) {
+                    const double e = acc_e[cb][b][v], o = acc_o[cb][b][v];
+                    ob[(long)c * nt + j] = e + o;
+                    ob[(long)c * nt + (nt - 1 - j)] = e - o;
+                }
             }
         }
-    }
 }
 
 // ---------------------------------------------------------------- PtS
-// tile: 64 columns (c) x 32 parity-l values of each parity; K chunk = 16 nodes
-constexpr int PTS_TC = 64;
-constexpr int PTS_TK = 32;
+// Workgroup tile: 128 columns (c) x 64 parity-l values of each parity (128 l).
+// Wave w owns columns [32w, 32w+32): 2 column blocks x 4 l blocks x 2 parities.
+// K runs over chunks of 16 southern nodes; the fold f(j) +/- f(nt+1-j) is formed
+// on the way into LDS, and the next chunk is fetched while this one is multiplied.
+constexpr int PTS_TC = 128;
+constexpr int PTS_TK = 64;
 constexpr int PTS_KJ = 16;
+constexpr int PTS_F_LD = PTS_KJ + 1;   // Fe/Fo row stride (doubles)
+constexpr int PTS_I_LD = PTS_TK + 4;   // Ie/Io row stride (doubles)
+constexpr int PTS_F_PER_T = PTS_TC * PTS_KJ / kThreads;  // 8
+constexpr int PTS_I_PER_T = PTS_KJ * PTS_TK / kThreads;  // 4 (per parity)
 
-__global__ __launch_bounds__(kThreads) void pts_fp64_kernel(
+__global__ __launch_bounds__(kThreads, 1) void pts_fp64_kernel(
     const double *__restrict__ p2, double *__restrict__ s, const double *__restrict__ ipe,
     const double *__restrict__ ipo, const long *__restrict__ s_off, const long *__restrict__ te_off,
     const long *__restrict__ to_off, const int *__restrict__ mvals, const work_item *__restrict__ work, int lmax,
     int nt, int nt2, int nrhs)
 {
-    __shared__ double Fe[PTS_TC][PTS_KJ + 1];
-    __shared__ double Fo[PTS_TC][PTS_KJ + 1];
-    __shared__ double Ie[PTS_KJ][PTS_TK + 1];
-    __shared__ double Io[PTS_KJ][PTS_TK + 1];
+    __shared__ double Fe[PTS_TC * PTS_F_LD];
+    __shared__ double Fo[PTS_TC * PTS_F_LD];
+    __shared__ double Ie[PTS_KJ * PTS_I_LD];
+    __shared__ double Io[PTS_KJ * PTS_I_LD];
 
     const work_item w = work[blockIdx.x];
+    if (w.mp < 0) return;  // padding item of the XCD-balanced schedule
     const int mp = w.mp, k0 = w.r0, c0 = w.c0;
     const int m = mvals[mp];
     const int nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
@@ -135,43 +190,76 @@ __global__ __launch_bounds__(kThreads) void pts_fp64_kernel(
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
 
-    double4_t acc_e[2], acc_o[2];
+    double rfa[PTS_F_PER_T], rfb[PTS_F_PER_T], rie[PTS_I_PER_T], rio[PTS_I_PER_T];
+    auto fetch = [&](int j0) {
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        acc_e[b] = double4_t{0.0, 0.0, 0.0, 0.0};
-        acc_o[b] = double4_t{0.0, 0.0, 0.0, 0.0};
-    }
-
-    for (int j0 = 0; j0 < nt2; j0 += PTS_KJ) {
-        for (int idx = tid; idx < PTS_TC * PTS_KJ; idx += kThreads) {
+        for (int i = 0; i < PTS_F_PER_T; ++i) {
+            const int idx = tid + i * kThreads;
             const int cc = idx / PTS_KJ, jj = idx % PTS_KJ;
             const int c = c0 + cc, j = j0 + jj;
-            double fe = 0.0, fo = 0.0;
-            if (c < nrhs && j < nt2) {
-                const double a = ib[(long)c * nt + j];
-                const double b = ib[(long)c * nt + (nt - 1 - j)];
-                fe = a + b;
-                fo = a - b;
-            }
-            Fe[cc][jj] = fe;
-            Fo[cc][jj] = fo;
+            const bool in = c < nrhs && j < nt2;
+            rfa[i] = in ? ib[(long)c * nt + j] : 0.0;
+            rfb[i] = in ? ib[(long)c * nt + (nt - 1 - j)] : 0.0;
         }
-        for (int idx = tid; idx < PTS_KJ * PTS_TK; idx += kThreads) {
+#pragma unroll
+        for (int i = 0; i < PTS_I_PER_T; ++i) {
+            const int idx = tid + i * kThreads;
             const int kk = idx / PTS_KJ, jj = idx % PTS_KJ;
             const int k = k0 + kk, j = j0 + jj;
             const bool jin = j < nt2;
-            Ie[jj][kk] = (jin && k < ne) ? ie[(long)k * nt2 + j] : 0.0;
-            Io[jj][kk] = (jin && k < no) ? io[(long)k * nt2 + j] : 0.0;
+            rie[i] = (jin && k < ne) ? ie[(long)k * nt2 + j] : 0.0;
+            rio[i] = (jin && k < no) ? io[(long)k * nt2 + j] : 0.0;
         }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int i = 0; i < PTS_F_PER_T; ++i) {
+            const int idx = tid + i * kThreads;
+            const int at = (idx / PTS_KJ) * PTS_F_LD + idx % PTS_KJ;
+            Fe[at] = rfa[i] + rfb[i];
+            Fo[at] = rfa[i] - rfb[i];
+        }
+#pragma unroll
+        for (int i = 0; i < PTS_I_PER_T; ++i) {
+            const int idx = tid + i * kThreads;
+            const int at = (idx % PTS_KJ) * PTS_I_LD + idx / PTS_KJ;
+            Ie[at] = rie[i];
+            Io[at] = rio[i];
+        }
+    };
+
+    double4_t acc_e[2][4], acc_o[2][4];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            acc_e[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+            acc_o[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+        }
+
+    fetch(0);
+    for (int j0 = 0; j0 < nt2; j0 += PTS_KJ) {
+        stash();
         __syncthreads();
+        if (j0 + PTS_KJ < nt2) fetch(j0 + PTS_KJ);
 #pragma unroll
         for (int jj = 0; jj < PTS_KJ; jj += 4) {
-            const double a_e = Fe[16 * wave + li][jj + lg];
-            const double a_o = Fo[16 * wave + li][jj + lg];
+            double be[4], bo[4];
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                acc_e[b] = mfma_f64(a_e, Ie[jj + lg][16 * b + li], acc_e[b]);
-                acc_o[b] = mfma_f64(a_o, Io[jj + lg][16 * b + li], acc_o[b]);
+            for (int b = 0; b < 4; ++b) {
+                be[b] = Ie[(jj + lg) * PTS_I_LD + 16 * b + li];
+                bo[b] = Io[(jj + lg) * PTS_I_LD + 16 * b + li];
+            }
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                const int row = (32 * wave + 16 * cb + li) * PTS_F_LD + jj + lg;
+                const double a_e = Fe[row];
+                const double a_o = Fo[row];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    acc_e[cb][b] = mfma_f64(a_e, be[b], acc_e[cb][b]);
+                    acc_o[cb][b] = mfma_f64(a_o, bo[b], acc_o[cb][b]);
+                }
             }
         }
         __syncthreads();
@@ -179,17 +267,19 @@ __global__ __launch_bounds__(kThreads) void pts_fp64_kernel(
 
     double *sb = s + s_off[mp];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        const int k = k0 + 16 * b + li;
+    for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int c = c0 + 16 * wave + lg + 4 * v;
-            if (c < nrhs) {
-                if (k < ne) sb[(long)c * nl + 2 * k] = acc_e[b][v];
-                if (k < no) sb[(long)c * nl + 2 * k + 1] = acc_o[b][v];
+        for (int b = 0; b < 4; ++b) {
+            const int k = k0 + 16 * b + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = c0 + 32 * wave + 16 * cb + lg + 4 * v;
+                if (c < nrhs) {
+                    if (k < ne) sb[(long)c * nl + 2 * k] = acc_e[cb][b][v];
+                    if (k < no) sb[(long)c * nl + 2 * k + 1] = acc_o[cb][b][v];
+                }
             }
         }
-    }
 }
 
 }  // namespace

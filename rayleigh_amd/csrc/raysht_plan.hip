@@ -45,22 +45,42 @@ void device_free(T *&ptr)
 int n_even(int lmax, int m) { return (lmax - m + 2) / 2; }
 int n_odd(int lmax, int m) { return (lmax - m + 1) / 2; }
 
-// Work items of one grouped launch, the largest contractions first so the tail of
-// the grid is made of the short m-blocks.
+// Work items of one grouped launch.  Workgroups are dispatched to the 8 XCDs of the
+// chip round-robin by workgroup index, each XCD with its own L2.  Whole m-blocks are
+// assigned to XCDs (longest first, each to the least-loaded XCD) so that the tiles
+// of one m-block, which share its P_lm table and its input columns, run on one XCD
+// and hit in its L2; within an XCD the largest contractions come first.  The list
+// is interleaved so that workgroup b runs item b/8 of XCD b%8, padded with empty
+// items (mp = -1) where XCD queues differ in length.
+constexpr int kXcds = 8;
+
 std::vector<work_item> make_work(const raysht_plan *p, bool stp)
 {
     std::vector<int> order(p->nm);
     for (int i = 0; i < p->nm; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return p->mvals[a] < p->mvals[b]; });
-    std::vector<work_item> w;
     const int tc = stp ? stp_tile_cols() : pts_tile_cols();
+    const int tr = stp ? stp_tile_rows() : pts_tile_rows();
+    std::vector<std::vector<work_item>> q(kXcds);
+    std::vector<double> load(kXcds, 0.0);
     for (int mp : order) {
         const int m = p->mvals[mp];
-        const int rows = stp ? p->nt2 : n_even(p->lmax, m);
-        const int tr = stp ? stp_tile_rows() : pts_tile_rows();
-        for (int r0 = 0; r0 < rows; r0 += tr)
-            for (int c0 = 0; c0 < p->nrhs; c0 += tc) w.push_back(work_item{mp, r0, c0, 0});
+        const int ne = n_even(p->lmax, m);
+        const int rows = stp ? p->nt2 : ne;
+        const int nrt = (rows + tr - 1) / tr, nct = (p->nrhs + tc - 1) / tc;
+        // cost of one item ~ its K loop: parity-l chunks for StP, node chunks for PtS
+        const double item = stp ? (double)ne : (double)p->nt2;
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        for (int r = 0; r < nrt; ++r)
+            for (int c = 0; c < nct; ++c) q[x].push_back(work_item{mp, r * tr, c * tc, 0});
+        load[x] += item * nrt * nct;
     }
+    size_t len = 0;
+    for (auto &v : q) len = std::max(len, v.size());
+    std::vector<work_item> w(len * kXcds, work_item{-1, 0, 0, 0});
+    for (size_t i = 0; i < len; ++i)
+        for (int x = 0; x < kXcds; ++x)
+            if (i < q[x].size()) w[i * kXcds + x] = q[x][i];
     return w;
 }
 
