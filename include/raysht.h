@@ -60,6 +60,12 @@ int raysht_version(void);
  * order (Load_Balance.F90:126-286) but each m in 0..lmax at most once. */
 int raysht_plan_create(raysht_plan **plan, int lmax, int ntheta, const int *m_values, int n_m,
                        int nr, int nfields, int precision);
+/* The same plan over `ncols` anonymous columns per m instead of 2*nr*nfields: the
+ * rmcontainer (2-D) pair of the generic, PtS_2d_dgpv2 / StP_2d_dgp
+ * (Legendre_Transforms.F90:506-668), whose blocks are data(m:lmax, 1:ncols).
+ * Such a plan has no radial/field structure, so the permutes reject it. */
+int raysht_plan_create_cols(raysht_plan **plan, int lmax, int ntheta, const int *m_values, int n_m, int ncols,
+                            int precision);
 void raysht_plan_destroy(raysht_plan *plan);
 
 /* Sizes (in doubles) of the buffers a plan transforms. */
@@ -73,6 +79,14 @@ int raysht_legendre_stp(raysht_plan *plan, const double *s, double *p2, void *st
 /* Legendre_Transform(p2b, s2b) -> PtS_4d_dgpv3 (Legendre_Transforms.F90:1077-1175).
  * p2: hybrid (device), s: spectral (device), every l in m..lmax written. */
 int raysht_legendre_pts(raysht_plan *plan, const double *p2, double *s, void *stream);
+
+/* Host-array forms of the two transforms for callers whose buffers are host
+ * memory (the Fortran rmcontainer4d arrays): s_m[mp] points at block mp
+ * (data(m:lmax, ...) of that m, contiguous), p2 at the host hybrid array.  The
+ * call stages through device memory held by the plan and returns when the
+ * result is in the caller's array. */
+int raysht_legendre_stp_host(raysht_plan *plan, const double *const *s_m, double *p2);
+int raysht_legendre_pts_host(raysht_plan *plan, const double *p2, double *const *s_m);
 
 /* Host copies of the setup data, for checking.  Nodes/weights as head+tail
  * doubles of the quad values (ntheta each).  Tables in the reference's stored
@@ -90,6 +104,10 @@ int raysht_fft_to_physical(int nphi, long howmany, double *x, void *stream);
 /* FFT_To_Spectral = r2c_ip4D_fftw (Fourier_Transform.F90:100-140), rescale = 0;
  * FFT_To_Spectral_RSC = r2c_ip4D_fftw_rsc (:56-98), rescale != 0. */
 int raysht_fft_to_spectral(int nphi, long howmany, double *x, int rescale, void *stream);
+
+/* Host-array forms of the two FFTs (staged through device memory; synchronous). */
+int raysht_fft_to_physical_host(int nphi, long howmany, double *x);
+int raysht_fft_to_spectral_host(int nphi, long howmany, double *x, int rescale);
 
 /* SphericalBuffer%reform p2a -> p3a = Transpose_2a3a (Spherical_Buffer.F90:196-459)
  * on one rank: p3 zero-filled, then Re/Im of each m in m_values placed at
@@ -119,6 +137,9 @@ int raysht_cheb_table(const raysht_cheb *cheb, int which, double *host_out);
 int raysht_cheb_to_spectral(raysht_cheb *cheb, const double *f, double *c, long ncols, void *stream);
 /* Cheby_Grid%from_spectral = From_Spectral4D (:450-558). */
 int raysht_cheb_from_spectral(raysht_cheb *cheb, const double *c, double *f, long ncols, void *stream);
+/* Host-array forms of the two Chebyshev transforms (synchronous). */
+int raysht_cheb_to_spectral_host(raysht_cheb *cheb, const double *f, double *c, long ncols);
+int raysht_cheb_from_spectral_host(raysht_cheb *cheb, const double *c, double *f, long ncols);
 /* Cheby_Grid%dealias_buffer = dealias_buffer4d (:353-374): zero n >= 2n/3 (0-based). */
 int raysht_cheb_dealias(raysht_cheb *cheb, double *c, long ncols, void *stream);
 

@@ -60,6 +60,8 @@ def oracle_lib():
         lib.orc_legendre_tables.argtypes = [_I, _I, _P, _I, _P, _P, _P, _P]
         lib.orc_stp.argtypes = [_I, _I, _P, _I, _I, _I, _P, _P, _P, _P]
         lib.orc_pts.argtypes = [_I, _I, _P, _I, _I, _I, _P, _P, _P, _P]
+        lib.orc_stp_cols.argtypes = [_I, _I, _P, _I, _I, _P, _P, _P, _P]
+        lib.orc_pts_cols.argtypes = [_I, _I, _P, _I, _I, _P, _P, _P, _P]
         lib.orc_fft_c2r.argtypes = [_I, _L, _P]
         lib.orc_fft_r2c.argtypes = [_I, _L, _P, _I]
         lib.orc_permute_2a3a.argtypes = [_I, _I, _P, _I, _I, _I, _I, _P, _P]
@@ -183,6 +185,26 @@ def pts(nt, lmax, mvals, nr, nf, tables, p_in):
     out = np.zeros(spectral_size(lmax, mv, nr, nf))
     oracle_lib().orc_pts(nt, lmax, _ptr(mv), len(mv), nr, nf, _ptr(ie), _ptr(io),
                          _ptr(np.ascontiguousarray(p_in)), _ptr(out))
+    return out
+
+
+def stp_cols(nt, lmax, mvals, ncols, tables, s_in):
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    pe, po, _, _ = tables
+    po = po if po.size else np.zeros(1)
+    out = np.zeros(nt * ncols * len(mv))
+    oracle_lib().orc_stp_cols(nt, lmax, _ptr(mv), len(mv), ncols, _ptr(pe), _ptr(po),
+                              _ptr(np.ascontiguousarray(s_in)), _ptr(out))
+    return out
+
+
+def pts_cols(nt, lmax, mvals, ncols, tables, p_in):
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    _, _, ie, io = tables
+    io = io if io.size else np.zeros(1)
+    out = np.zeros(int(sum(lmax - int(m) + 1 for m in mv)) * ncols)
+    oracle_lib().orc_pts_cols(nt, lmax, _ptr(mv), len(mv), ncols, _ptr(ie), _ptr(io),
+                              _ptr(np.ascontiguousarray(p_in)), _ptr(out))
     return out
 
 

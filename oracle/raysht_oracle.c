@@ -193,15 +193,20 @@ int orc_legendre_tables(int nt, int lmax, const int *mvals, int nm,
     return 0;
 }
 
-static long spec_block(int lmax, int m, int nr, int nf) { return (long)(lmax - m + 1) * nr * 2 * nf; }
-
 /* Legendre_Transforms.F90:968-1074  StP_4d_dgp2 (parity branch):
  * E = p_lm_even^T . s_even, O = p_lm_odd^T . s_odd (each nt2 x nrhs);
  * out(j) = E + O, out(nt+1-j) = E - O; out zeroed first. */
 void orc_stp(int nt, int lmax, const int *mvals, int nm, int nr, int nf,
              const double *p_even, const double *p_odd, const double *s_in, double *p_out)
 {
-    const int nt2 = nt / 2, nrhs = 2 * nr * nf;
+    orc_stp_cols(nt, lmax, mvals, nm, 2 * nr * nf, p_even, p_odd, s_in, p_out);
+}
+
+/* The same over ncols anonymous columns: StP_2d_dgp (Legendre_Transforms.F90:588-668). */
+void orc_stp_cols(int nt, int lmax, const int *mvals, int nm, int ncols,
+                  const double *p_even, const double *p_odd, const double *s_in, double *p_out)
+{
+    const int nt2 = nt / 2, nrhs = ncols;
     long so = 0, oe = 0, oo = 0;
     for (int mp = 0; mp < nm; ++mp) {
         const int m = mvals[mp], nl = lmax - m + 1;
@@ -226,7 +231,7 @@ void orc_stp(int nt, int lmax, const int *mvals, int nm, int nr, int nf,
                 }
             }
         }
-        so += spec_block(lmax, m, nr, nf);
+        so += (long)nl * nrhs;
         oe += (long)ne * nt2;
         oo += (long)no * nt2;
     }
@@ -238,7 +243,14 @@ void orc_stp(int nt, int lmax, const int *mvals, int nm, int nr, int nf,
 void orc_pts(int nt, int lmax, const int *mvals, int nm, int nr, int nf,
              const double *ip_even, const double *ip_odd, const double *p_in, double *s_out)
 {
-    const int nt2 = nt / 2, nrhs = 2 * nr * nf;
+    orc_pts_cols(nt, lmax, mvals, nm, 2 * nr * nf, ip_even, ip_odd, p_in, s_out);
+}
+
+/* The same over ncols anonymous columns: PtS_2d_dgpv2 (Legendre_Transforms.F90:506-585). */
+void orc_pts_cols(int nt, int lmax, const int *mvals, int nm, int ncols,
+                  const double *ip_even, const double *ip_odd, const double *p_in, double *s_out)
+{
+    const int nt2 = nt / 2, nrhs = ncols;
     long so = 0, oe = 0, oo = 0;
     double *fe = malloc(sizeof(double) * (size_t)nt2), *fo = malloc(sizeof(double) * (size_t)nt2);
     for (int mp = 0; mp < nm; ++mp) {
@@ -264,7 +276,7 @@ void orc_pts(int nt, int lmax, const int *mvals, int nm, int nr, int nf,
                 col[2 * k + 1] = acc;
             }
         }
-        so += spec_block(lmax, m, nr, nf);
+        so += (long)nl * nrhs;
         oe += (long)ne * nt2;
         oo += (long)no * nt2;
     }

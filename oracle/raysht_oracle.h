@@ -41,9 +41,17 @@ int orc_legendre_tables(int nt, int lmax, const int *mvals, int nm,
 void orc_stp(int nt, int lmax, const int *mvals, int nm, int nr, int nf,
              const double *p_even, const double *p_odd, const double *s_in, double *p_out);
 
+/* The same over ncols anonymous columns (StP_2d_dgp). */
+void orc_stp_cols(int nt, int lmax, const int *mvals, int nm, int ncols,
+                  const double *p_even, const double *p_odd, const double *s_in, double *p_out);
+
 /* Hybrid -> spectral (PtS_4d_dgpv3). */
 void orc_pts(int nt, int lmax, const int *mvals, int nm, int nr, int nf,
              const double *ip_even, const double *ip_odd, const double *p_in, double *s_out);
+
+/* The same over ncols anonymous columns (PtS_2d_dgpv2). */
+void orc_pts_cols(int nt, int lmax, const int *mvals, int nm, int ncols,
+                  const double *ip_even, const double *ip_odd, const double *p_in, double *s_out);
 
 /* In-place real FFTs along lines of length nphi+2 (FFTW half-complex layout). */
 void orc_fft_c2r(int nphi, long howmany, double *x);

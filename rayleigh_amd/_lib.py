@@ -24,7 +24,14 @@ SIGNATURES = {
     "raysht_strerror": (ctypes.c_char_p, [_I]),
     "raysht_version": (_I, []),
     "raysht_plan_create": (_I, [ctypes.POINTER(_P), _I, _I, _P, _I, _I, _I, _I]),
+    "raysht_plan_create_cols": (_I, [ctypes.POINTER(_P), _I, _I, _P, _I, _I, _I]),
     "raysht_plan_destroy": (None, [_P]),
+    "raysht_legendre_stp_host": (_I, [_P, _P, _P]),
+    "raysht_legendre_pts_host": (_I, [_P, _P, _P]),
+    "raysht_fft_to_physical_host": (_I, [_I, _L, _P]),
+    "raysht_fft_to_spectral_host": (_I, [_I, _L, _P, _I]),
+    "raysht_cheb_to_spectral_host": (_I, [_P, _P, _P, _L]),
+    "raysht_cheb_from_spectral_host": (_I, [_P, _P, _P, _L]),
     "raysht_spectral_size": (_S, [_P]),
     "raysht_hybrid_size": (_S, [_P]),
     "raysht_legendre_stp": (_I, [_P, _P, _P, _P]),

@@ -57,7 +57,35 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lquadmath"])
     os.replace(tmp, LIB)
+    build_fortran()
     return LIB
+
+
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_BUILD = os.path.join(FORTRAN_DIR, "_build")
+FORTRAN_TEST = os.path.join(FORTRAN_BUILD, "test_sht_raysht")
+FC = os.environ.get("RAYSHT_FC", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran() -> str | None:
+    """The ISO_C_BINDING module (RaySHT_Math_Layer) and the Fortran test program that
+    drives it; skipped (None) where no Fortran compiler is installed."""
+    if not os.path.exists(FC):
+        return None
+    os.makedirs(FORTRAN_BUILD, exist_ok=True)
+    srcs = [os.path.join(FORTRAN_DIR, "raysht_structures.F90"), os.path.join(FORTRAN_DIR, "raysht_math_layer.F90")]
+    _run_in(FORTRAN_BUILD, [FC, "-O2", "-fPIC", "-c"] + srcs)
+    test_src = os.path.join(os.path.dirname(HERE), "tests", "fortran", "test_sht_raysht.F90")
+    _run_in(FORTRAN_BUILD, [FC, "-O2", "-o", FORTRAN_TEST, test_src, "raysht_structures.o", "raysht_math_layer.o",
+                            "-L" + HERE, "-lraysht", "-Wl,-rpath,$ORIGIN/../.."])
+    return FORTRAN_TEST
+
+
+def _run_in(cwd, cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=cwd)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
 
 
 if __name__ == "__main__":

@@ -321,7 +321,7 @@ int raysht_fft_to_spectral(int nphi, long howmany, double *x, int rescale, void 
 
 int raysht_permute_2a3a(raysht_plan *p, int nphi, const double *p2, double *p3, void *stream)
 {
-    if (!p || !p2 || !p3 || nphi < 2 * (p->lmax + 1) || (nphi & 1)) return RAYSHT_EINVAL;
+    if (!p || !p2 || !p3 || p->nr < 1 || nphi < 2 * (p->lmax + 1) || (nphi & 1)) return RAYSHT_EINVAL;
     const long total = (long)(nphi + 2) * p->nr * p->nt * p->nf;
     const long blocks = std::min<long>((total + 255) / 256, 1L << 20);
     hipLaunchKernelGGL(permute_2a3a_kernel, dim3((unsigned)blocks), dim3(256), 0,
@@ -333,7 +333,7 @@ int raysht_permute_2a3a(raysht_plan *p, int nphi, const double *p2, double *p3, 
 
 int raysht_permute_3b2b(raysht_plan *p, int nphi, const double *p3, double *p2, void *stream)
 {
-    if (!p || !p2 || !p3 || nphi < 2 * (p->lmax + 1) || (nphi & 1)) return RAYSHT_EINVAL;
+    if (!p || !p2 || !p3 || p->nr < 1 || nphi < 2 * (p->lmax + 1) || (nphi & 1)) return RAYSHT_EINVAL;
     const long total = (long)p->nt * p->nrhs * p->nm;
     const long blocks = std::min<long>((total + 255) / 256, 1L << 20);
     hipLaunchKernelGGL(permute_3b2b_kernel, dim3((unsigned)blocks), dim3(256), 0,

@@ -1,0 +1,141 @@
+// Host-array entry points: the drop-in form for callers whose buffers live in host
+// memory (Rayleigh's own rmcontainer4d arrays, reached through the Fortran
+// ISO_C_BINDING module).  Each call stages its operands through device memory,
+// runs the device transform and copies the result back; it returns when the
+// result is in the caller's array.  Device-resident callers use the device-pointer
+// entry points instead and pay no transfer.
+#include <cstring>
+#include <vector>
+
+#include "raysht_internal.h"
+
+using namespace raysht;
+
+namespace {
+
+int ensure_stage(raysht_plan *p)
+{
+    if (p->d_stage_s && p->d_stage_p) return RAYSHT_OK;
+    const size_t ns = (size_t)p->s_off[p->nm] * sizeof(double);
+    const size_t nh = (size_t)p->nt * p->nrhs * p->nm * sizeof(double);
+    if (!p->d_stage_s && hipMalloc(&p->d_stage_s, ns) != hipSuccess) {
+        p->d_stage_s = nullptr;
+        return RAYSHT_ENOMEM;
+    }
+    if (!p->d_stage_p && hipMalloc(&p->d_stage_p, nh) != hipSuccess) {
+        p->d_stage_p = nullptr;
+        return RAYSHT_ENOMEM;
+    }
+    return RAYSHT_OK;
+}
+
+struct scoped_device_buffer {
+    double *ptr = nullptr;
+    ~scoped_device_buffer()
+    {
+        if (ptr) (void)hipFree(ptr);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int raysht_legendre_stp_host(raysht_plan *p, const double *const *s_m, double *p2)
+{
+    if (!p || !s_m || !p2) return RAYSHT_EINVAL;
+    for (int mp = 0; mp < p->nm; ++mp)
+        if (!s_m[mp]) return RAYSHT_EINVAL;
+    int rc = ensure_stage(p);
+    if (rc != RAYSHT_OK) return rc;
+    for (int mp = 0; mp < p->nm; ++mp) {
+        const size_t n = (size_t)(p->s_off[mp + 1] - p->s_off[mp]) * sizeof(double);
+        RAYSHT_HIP_TRY(hipMemcpy(p->d_stage_s + p->s_off[mp], s_m[mp], n, hipMemcpyHostToDevice));
+    }
+    if ((rc = raysht_legendre_stp(p, p->d_stage_s, p->d_stage_p, nullptr)) != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipMemcpy(p2, p->d_stage_p, (size_t)p->nt * p->nrhs * p->nm * sizeof(double),
+                             hipMemcpyDeviceToHost));
+    return RAYSHT_OK;
+}
+
+int raysht_legendre_pts_host(raysht_plan *p, const double *p2, double *const *s_m)
+{
+    if (!p || !s_m || !p2) return RAYSHT_EINVAL;
+    for (int mp = 0; mp < p->nm; ++mp)
+        if (!s_m[mp]) return RAYSHT_EINVAL;
+    int rc = ensure_stage(p);
+    if (rc != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipMemcpy(p->d_stage_p, p2, (size_t)p->nt * p->nrhs * p->nm * sizeof(double),
+                             hipMemcpyHostToDevice));
+    if ((rc = raysht_legendre_pts(p, p->d_stage_p, p->d_stage_s, nullptr)) != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipDeviceSynchronize());
+    for (int mp = 0; mp < p->nm; ++mp) {
+        const size_t n = (size_t)(p->s_off[mp + 1] - p->s_off[mp]) * sizeof(double);
+        RAYSHT_HIP_TRY(hipMemcpy(s_m[mp], p->d_stage_s + p->s_off[mp], n, hipMemcpyDeviceToHost));
+    }
+    return RAYSHT_OK;
+}
+
+int raysht_fft_to_physical_host(int nphi, long howmany, double *x)
+{
+    if (nphi < 2 || (nphi & 1) || howmany < 0 || (!x && howmany > 0)) return RAYSHT_EINVAL;
+    if (howmany == 0) return RAYSHT_OK;
+    const size_t bytes = (size_t)howmany * (nphi + 2) * sizeof(double);
+    scoped_device_buffer d;
+    if (hipMalloc(&d.ptr, bytes) != hipSuccess) {
+        d.ptr = nullptr;
+        return RAYSHT_ENOMEM;
+    }
+    RAYSHT_HIP_TRY(hipMemcpy(d.ptr, x, bytes, hipMemcpyHostToDevice));
+    int rc = raysht_fft_to_physical(nphi, howmany, d.ptr, nullptr);
+    if (rc != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipMemcpy(x, d.ptr, bytes, hipMemcpyDeviceToHost));
+    return RAYSHT_OK;
+}
+
+int raysht_fft_to_spectral_host(int nphi, long howmany, double *x, int rescale)
+{
+    if (nphi < 2 || (nphi & 1) || howmany < 0 || (!x && howmany > 0)) return RAYSHT_EINVAL;
+    if (howmany == 0) return RAYSHT_OK;
+    const size_t bytes = (size_t)howmany * (nphi + 2) * sizeof(double);
+    scoped_device_buffer d;
+    if (hipMalloc(&d.ptr, bytes) != hipSuccess) {
+        d.ptr = nullptr;
+        return RAYSHT_ENOMEM;
+    }
+    RAYSHT_HIP_TRY(hipMemcpy(d.ptr, x, bytes, hipMemcpyHostToDevice));
+    int rc = raysht_fft_to_spectral(nphi, howmany, d.ptr, rescale, nullptr);
+    if (rc != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipMemcpy(x, d.ptr, bytes, hipMemcpyDeviceToHost));
+    return RAYSHT_OK;
+}
+
+int raysht_cheb_to_spectral_host(raysht_cheb *cb, const double *f, double *c, long ncols)
+{
+    if (!cb || ncols < 0 || (ncols > 0 && (!f || !c))) return RAYSHT_EINVAL;
+    if (ncols == 0) return RAYSHT_OK;
+    const size_t bytes = (size_t)ncols * cb->n * sizeof(double);
+    scoped_device_buffer din, dout;
+    if (hipMalloc(&din.ptr, bytes) != hipSuccess || hipMalloc(&dout.ptr, bytes) != hipSuccess) return RAYSHT_ENOMEM;
+    RAYSHT_HIP_TRY(hipMemcpy(din.ptr, f, bytes, hipMemcpyHostToDevice));
+    int rc = raysht_cheb_to_spectral(cb, din.ptr, dout.ptr, ncols, nullptr);
+    if (rc != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipMemcpy(c, dout.ptr, bytes, hipMemcpyDeviceToHost));
+    return RAYSHT_OK;
+}
+
+int raysht_cheb_from_spectral_host(raysht_cheb *cb, const double *c, double *f, long ncols)
+{
+    if (!cb || ncols < 0 || (ncols > 0 && (!f || !c))) return RAYSHT_EINVAL;
+    if (ncols == 0) return RAYSHT_OK;
+    const size_t bytes = (size_t)ncols * cb->n * sizeof(double);
+    scoped_device_buffer din, dout;
+    if (hipMalloc(&din.ptr, bytes) != hipSuccess || hipMalloc(&dout.ptr, bytes) != hipSuccess) return RAYSHT_ENOMEM;
+    RAYSHT_HIP_TRY(hipMemcpy(din.ptr, c, bytes, hipMemcpyHostToDevice));
+    int rc = raysht_cheb_from_spectral(cb, din.ptr, dout.ptr, ncols, nullptr);
+    if (rc != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipMemcpy(f, dout.ptr, bytes, hipMemcpyDeviceToHost));
+    return RAYSHT_OK;
+}
+
+}  // extern "C"

@@ -30,256 +30,368 @@ __device__ __forceinline__ double4_t mfma_f64(double a, double b, double4_t c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-constexpr int kThreads = 256;
+// Tile geometry.  The defaults are the measured best on MI355X for the l_max = 511
+// workload; they can be overridden at build time (-DRAYSHT_STP_THREADS=... etc.)
+// when retuning.  Node / parity-l tiles are fixed at 64 (4 MFMA blocks).
+#ifndef RAYSHT_STP_THREADS
+#define RAYSHT_STP_THREADS 512
+#endif
+#ifndef RAYSHT_STP_CB
+#define RAYSHT_STP_CB 1
+#endif
+#ifndef RAYSHT_STP_KC
+#define RAYSHT_STP_KC 16
+#endif
+#ifndef RAYSHT_PTS_THREADS
+#define RAYSHT_PTS_THREADS 512
+#endif
+#ifndef RAYSHT_PTS_CB
+#define RAYSHT_PTS_CB 1
+#endif
+#ifndef RAYSHT_PTS_KJ
+#define RAYSHT_PTS_KJ 16
+#endif
+// minimum resident waves per SIMD the register allocation must allow
+#ifndef RAYSHT_FP64_MIN_WAVES
+#define RAYSHT_FP64_MIN_WAVES 2
+#endif
+
+// Raw buffer access: loads whose byte offset falls outside [0, bytes) return 0
+// without touching memory, so tile edges need no branches.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const double *base, long bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ double load_f64(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
 
 // ---------------------------------------------------------------- StP
-// Workgroup tile: 128 columns (c) x 64 southern nodes (j), both parities.
-// Wave w owns columns [32w, 32w+32): 2 column blocks x 4 node blocks x 2 parities
-// = 16 accumulators of 16x16.  K runs over chunks of 16 parity-l values (32 l);
+// Workgroup tile: (waves x 16 x STP_CB) columns (c) x 64 southern nodes (j), both
+// parities.  Wave w owns STP_CB blocks of 16 columns: STP_CB x 4 node blocks x 2
+// parities of 16x16 accumulators.  K runs over chunks of STP_KC parity-l values;
 // the next chunk is fetched into registers while the current one is multiplied
 // out of LDS.
-constexpr int STP_TC = 128;
+constexpr int STP_THREADS = RAYSHT_STP_THREADS;
+constexpr int STP_CB = RAYSHT_STP_CB;  // 16-column blocks per wave
+constexpr int STP_TC = (STP_THREADS / 64) * 16 * STP_CB;
 constexpr int STP_TJ = 64;
-constexpr int STP_KC = 16;
+constexpr int STP_KC = RAYSHT_STP_KC;
 constexpr int STP_BS_LD = 2 * STP_KC + 1;  // Bs row stride (doubles)
 constexpr int STP_A_LD = STP_TJ + 16;      // Ae/Ao row stride (doubles)
-constexpr int STP_B_PER_T = STP_TC * 2 * STP_KC / kThreads;  // 16
-constexpr int STP_A_PER_T = STP_KC * STP_TJ / kThreads;      // 4 (per parity)
+constexpr int STP_B_PER_T = STP_TC * 2 * STP_KC / STP_THREADS;
+static_assert(STP_THREADS % 64 == 0 && STP_THREADS % (2 * STP_KC) == 0 && (STP_KC * STP_TJ) % STP_THREADS == 0 &&
+                  STP_KC % 4 == 0,
+              "StP tile geometry");
+constexpr int STP_A_PER_T = STP_KC * STP_TJ / STP_THREADS;      // per parity
 
-__global__ __launch_bounds__(kThreads, 1) void stp_fp64_kernel(
+// One workgroup per work item; the hardware dispatcher balances the uneven items
+// (a persistent variant that prefetched across items measured slower).
+struct stp_item {
+    int mp, j0, c0, nl, ne, no;
+    __amdgpu_buffer_rsrc_t rs, rpe, rpo;
+};
+
+__device__ __forceinline__ bool stp_next_item(const work_item *__restrict__ work, int n_work, int &idx, int stride,
+                                              const double *s, const double *pte, const double *pto,
+                                              const long *s_off, const long *te_off, const long *to_off,
+                                              const int *mvals, int lmax, int nt2, int nrhs, stp_item &it)
+{
+    for (; idx < n_work; idx += stride) {
+        const work_item w = work[idx];
+        if (w.mp < 0) continue;  // padding item of the XCD-balanced schedule
+        it.mp = w.mp;
+        it.j0 = w.r0;
+        it.c0 = w.c0;
+        it.nl = lmax - mvals[w.mp] + 1;
+        it.ne = (it.nl + 1) >> 1;
+        it.no = it.nl >> 1;
+        it.rs = make_rsrc(s + s_off[w.mp], (long)it.nl * nrhs * 8);
+        it.rpe = make_rsrc(pte + te_off[w.mp], (long)it.ne * nt2 * 8);
+        it.rpo = make_rsrc(pto + to_off[w.mp], (long)it.no * nt2 * 8);
+        return true;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(STP_THREADS, RAYSHT_FP64_MIN_WAVES) void stp_fp64_kernel(
     const double *__restrict__ s, double *__restrict__ p2, const double *__restrict__ pte,
     const double *__restrict__ pto, const long *__restrict__ s_off, const long *__restrict__ te_off,
-    const long *__restrict__ to_off, const int *__restrict__ mvals, const work_item *__restrict__ work, int lmax,
-    int nt, int nt2, int nrhs)
+    const long *__restrict__ to_off, const int *__restrict__ mvals, const work_item *__restrict__ work, int n_work,
+    int lmax, int nt, int nt2, int nrhs)
 {
     __shared__ double Ae[STP_KC * STP_A_LD];
     __shared__ double Ao[STP_KC * STP_A_LD];
     __shared__ double Bs[STP_TC * STP_BS_LD];
 
-    const work_item w = work[blockIdx.x];
-    if (w.mp < 0) return;  // padding item of the XCD-balanced schedule
-    const int mp = w.mp, j0 = w.r0, c0 = w.c0;
-    const int m = mvals[mp];
-    const int nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
-    const double *sb = s + s_off[mp];
-    const double *pe = pte + te_off[mp];
-    const double *po = pto + to_off[mp];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
+    const int stride = gridDim.x;
+    // Per-thread load geometry.  B: column bcc0 + (STP_THREADS / 2KC) i, l = 2 k0 + bq;
+    // A: parity-l row akk0 + (STP_THREADS / 64) i, node ajj.
+    const int bq = tid % (2 * STP_KC), bcc0 = tid / (2 * STP_KC);
+    const int ajj = tid % STP_TJ, akk0 = tid / STP_TJ;
+
+    int idx = blockIdx.x;
+    stp_item cur;
+    if (!stp_next_item(work, n_work, idx, n_work, s, pte, pto, s_off, te_off, to_off, mvals, lmax, nt2, nrhs, cur))
+        return;
+    const int b_voff = ((cur.c0 + bcc0) * cur.nl + bq) * 8;
+    const int b_sstep = (STP_THREADS / (2 * STP_KC)) * cur.nl * 8;
+    const bool a_jin = cur.j0 + ajj < nt2;
+    const int a_voff = (akk0 * nt2 + cur.j0 + ajj) * 8;
+    const int a_sstep = (STP_THREADS / STP_TJ) * nt2 * 8;
 
     double rb[STP_B_PER_T], rae[STP_A_PER_T], rao[STP_A_PER_T];
-    auto fetch = [&](int k0) {
+    auto fetch = [&](const stp_item &it, int k0) {
+        const bool lin = 2 * k0 + bq < it.nl;  // rows past a column's last l belong to the next column
 #pragma unroll
         for (int i = 0; i < STP_B_PER_T; ++i) {
-            const int idx = tid + i * kThreads;
-            const int cc = idx / (2 * STP_KC), q = idx % (2 * STP_KC);
-            const int c = c0 + cc, l = 2 * k0 + q;
-            rb[i] = (c < nrhs && l < nl) ? sb[(long)c * nl + l] : 0.0;
+            const double v = load_f64(it.rs, b_voff + 2 * k0 * 8, i * b_sstep);
+            rb[i] = lin ? v : 0.0;
         }
+        const int a_s0 = k0 * nt2 * 8;
 #pragma unroll
         for (int i = 0; i < STP_A_PER_T; ++i) {
-            const int idx = tid + i * kThreads;
-            const int kk = idx / STP_TJ, jj = idx % STP_TJ;
-            const int k = k0 + kk, j = j0 + jj;
-            const bool jin = j < nt2;
-            rae[i] = (jin && k < ne) ? pe[(long)k * nt2 + j] : 0.0;
-            rao[i] = (jin && k < no) ? po[(long)k * nt2 + j] : 0.0;
+            const double ve = load_f64(it.rpe, a_voff, a_s0 + i * a_sstep);
+            const double vo = load_f64(it.rpo, a_voff, a_s0 + i * a_sstep);
+            rae[i] = a_jin ? ve : 0.0;
+            rao[i] = a_jin ? vo : 0.0;
         }
     };
     auto stash = [&]() {
 #pragma unroll
         for (int i = 0; i < STP_B_PER_T; ++i) {
-            const int idx = tid + i * kThreads;
+            const int idx = tid + i * STP_THREADS;
             Bs[(idx / (2 * STP_KC)) * STP_BS_LD + idx % (2 * STP_KC)] = rb[i];
         }
 #pragma unroll
         for (int i = 0; i < STP_A_PER_T; ++i) {
-            const int idx = tid + i * kThreads;
+            const int idx = tid + i * STP_THREADS;
             Ae[(idx / STP_TJ) * STP_A_LD + idx % STP_TJ] = rae[i];
             Ao[(idx / STP_TJ) * STP_A_LD + idx % STP_TJ] = rao[i];
         }
     };
 
-    double4_t acc_e[2][4], acc_o[2][4];
+    fetch(cur, 0);
+    {
+        double4_t acc_e[STP_CB][4], acc_o[STP_CB][4];
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            acc_e[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
-            acc_o[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
-        }
-
-    fetch(0);
-    for (int k0 = 0; k0 < ne; k0 += STP_KC) {
-        stash();
-        __syncthreads();
-        if (k0 + STP_KC < ne) fetch(k0 + STP_KC);
-#pragma unroll
-        for (int kk = 0; kk < STP_KC; kk += 4) {
-            double be[4], bo[4];
+        for (int cb = 0; cb < STP_CB; ++cb)
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                be[b] = Ae[(kk + lg) * STP_A_LD + 16 * b + li];
-                bo[b] = Ao[(kk + lg) * STP_A_LD + 16 * b + li];
+                acc_e[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+                acc_o[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
             }
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
-                const int row = (32 * wave + 16 * cb + li) * STP_BS_LD + 2 * (kk + lg);
-                const double a_e = Bs[row];
-                const double a_o = Bs[row + 1];
+        for (int k0 = 0; k0 < cur.ne; k0 += STP_KC) {
+            stash();
+            __syncthreads();
+            if (k0 + STP_KC < cur.ne) fetch(cur, k0 + STP_KC);
+#pragma unroll 1
+            for (int kk = 0; kk < STP_KC; kk += 4) {
+                double be[4], bo[4];
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    acc_e[cb][b] = mfma_f64(a_e, be[b], acc_e[cb][b]);
-                    acc_o[cb][b] = mfma_f64(a_o, bo[b], acc_o[cb][b]);
+                    be[b] = Ae[(kk + lg) * STP_A_LD + 16 * b + li];
+                    bo[b] = Ao[(kk + lg) * STP_A_LD + 16 * b + li];
+                }
+#pragma unroll
+                for (int cb = 0; cb < STP_CB; ++cb) {
+                    const int row = (16 * STP_CB * wave + 16 * cb + li) * STP_BS_LD + 2 * (kk + lg);
+                    const double a_e = Bs[row];
+                    const double a_o = Bs[row + 1];
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        acc_e[cb][b] = mfma_f64(a_e, be[b], acc_e[cb][b]);
+                        acc_o[cb][b] = mfma_f64(a_o, bo[b], acc_o[cb][b]);
+                    }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
-    }
 
-    double *ob = p2 + (long)mp * nrhs * nt;
+        double *ob = p2 + (long)cur.mp * nrhs * nt;
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
+        for (int cb = 0; cb < STP_CB; ++cb)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int j = j0 + 16 * b + li;
+            for (int b = 0; b < 4; ++b) {
+                const int j = cur.j0 + 16 * b + li;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int c = c0 + 32 * wave + 16 * cb + lg + 4 * v;
-                if (c < nrhs && j < nt2) {
-                    const double e = acc_e[cb][b][v], o = acc_o[cb][b][v];
-                    ob[(long)c * nt + j] = e + o;
-                    ob[(long)c * nt + (nt - 1 - j)] = e - o;
+                for (int v = 0; v < 4; ++v) {
+                    const int c = cur.c0 + 16 * STP_CB * wave + 16 * cb + lg + 4 * v;
+                    if (c < nrhs && j < nt2) {
+                        const double e = acc_e[cb][b][v], o = acc_o[cb][b][v];
+                        ob[(long)c * nt + j] = e + o;
+                        ob[(long)c * nt + (nt - 1 - j)] = e - o;
+                    }
                 }
             }
-        }
+    }
 }
 
 // ---------------------------------------------------------------- PtS
-// Workgroup tile: 128 columns (c) x 64 parity-l values of each parity (128 l).
-// Wave w owns columns [32w, 32w+32): 2 column blocks x 4 l blocks x 2 parities.
-// K runs over chunks of 16 southern nodes; the fold f(j) +/- f(nt+1-j) is formed
+// Workgroup tile: (waves x 16 x PTS_CB) columns (c) x 64 parity-l values of each
+// parity (128 l).  Wave w owns PTS_CB blocks of 16 columns: PTS_CB x 4 l blocks x 2
+// parities.  K runs over chunks of PTS_KJ southern nodes; the fold f(j) +/- f(nt+1-j) is formed
 // on the way into LDS, and the next chunk is fetched while this one is multiplied.
-constexpr int PTS_TC = 128;
+constexpr int PTS_THREADS = RAYSHT_PTS_THREADS;
+constexpr int PTS_CB = RAYSHT_PTS_CB;  // 16-column blocks per wave
+constexpr int PTS_TC = (PTS_THREADS / 64) * 16 * PTS_CB;
 constexpr int PTS_TK = 64;
-constexpr int PTS_KJ = 16;
+constexpr int PTS_KJ = RAYSHT_PTS_KJ;
 constexpr int PTS_F_LD = PTS_KJ + 1;   // Fe/Fo row stride (doubles)
 constexpr int PTS_I_LD = PTS_TK + 4;   // Ie/Io row stride (doubles)
-constexpr int PTS_F_PER_T = PTS_TC * PTS_KJ / kThreads;  // 8
-constexpr int PTS_I_PER_T = PTS_KJ * PTS_TK / kThreads;  // 4 (per parity)
+constexpr int PTS_F_PER_T = PTS_TC * PTS_KJ / PTS_THREADS;
+constexpr int PTS_I_PER_T = PTS_KJ * PTS_TK / PTS_THREADS;  // per parity
+static_assert(PTS_THREADS % 64 == 0 && PTS_THREADS % PTS_KJ == 0 && (PTS_KJ * PTS_TK) % PTS_THREADS == 0 &&
+                  PTS_KJ % 4 == 0,
+              "PtS tile geometry");
 
-__global__ __launch_bounds__(kThreads, 1) void pts_fp64_kernel(
+struct pts_item {
+    int mp, k0, c0, nl, ne, no;
+    __amdgpu_buffer_rsrc_t rp, rie, rio;
+};
+
+__device__ __forceinline__ bool pts_next_item(const work_item *__restrict__ work, int n_work, int &idx, int stride,
+                                              const double *p2, const double *ipe, const double *ipo,
+                                              const long *te_off, const long *to_off, const int *mvals, int lmax,
+                                              int nt, int nt2, int nrhs, pts_item &it)
+{
+    for (; idx < n_work; idx += stride) {
+        const work_item w = work[idx];
+        if (w.mp < 0) continue;  // padding item of the XCD-balanced schedule
+        it.mp = w.mp;
+        it.k0 = w.r0;
+        it.c0 = w.c0;
+        it.nl = lmax - mvals[w.mp] + 1;
+        it.ne = (it.nl + 1) >> 1;
+        it.no = it.nl >> 1;
+        it.rp = make_rsrc(p2 + (long)w.mp * nrhs * nt, (long)nrhs * nt * 8);
+        it.rie = make_rsrc(ipe + te_off[w.mp], (long)it.ne * nt2 * 8);
+        it.rio = make_rsrc(ipo + to_off[w.mp], (long)it.no * nt2 * 8);
+        return true;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_kernel(
     const double *__restrict__ p2, double *__restrict__ s, const double *__restrict__ ipe,
     const double *__restrict__ ipo, const long *__restrict__ s_off, const long *__restrict__ te_off,
-    const long *__restrict__ to_off, const int *__restrict__ mvals, const work_item *__restrict__ work, int lmax,
-    int nt, int nt2, int nrhs)
+    const long *__restrict__ to_off, const int *__restrict__ mvals, const work_item *__restrict__ work, int n_work,
+    int lmax, int nt, int nt2, int nrhs)
 {
     __shared__ double Fe[PTS_TC * PTS_F_LD];
     __shared__ double Fo[PTS_TC * PTS_F_LD];
     __shared__ double Ie[PTS_KJ * PTS_I_LD];
     __shared__ double Io[PTS_KJ * PTS_I_LD];
 
-    const work_item w = work[blockIdx.x];
-    if (w.mp < 0) return;  // padding item of the XCD-balanced schedule
-    const int mp = w.mp, k0 = w.r0, c0 = w.c0;
-    const int m = mvals[mp];
-    const int nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
-    const double *ib = p2 + (long)mp * nrhs * nt;
-    const double *ie = ipe + te_off[mp];
-    const double *io = ipo + to_off[mp];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
+    const int stride = gridDim.x;
+    // Per-thread load geometry.  F: column fcc0 + (PTS_THREADS / KJ) i, node j0 + fjj and
+    // its mirror; I: parity-l row fcc0 + (PTS_THREADS / KJ) i, node j0 + fjj.
+    const int fjj = tid % PTS_KJ, fcc0 = tid / PTS_KJ;
+    const int f_sstep = (PTS_THREADS / PTS_KJ) * nt * 8;
+    const int i_sstep = (PTS_THREADS / PTS_KJ) * nt2 * 8;
 
     double rfa[PTS_F_PER_T], rfb[PTS_F_PER_T], rie[PTS_I_PER_T], rio[PTS_I_PER_T];
-    auto fetch = [&](int j0) {
+    auto fetch = [&](const pts_item &it, int j0) {
+        const int j = j0 + fjj;
+        const bool jin = j < nt2;
+        const int va = ((it.c0 + fcc0) * nt + j) * 8;
+        const int vb = ((it.c0 + fcc0) * nt + (nt - 1 - j)) * 8;
 #pragma unroll
         for (int i = 0; i < PTS_F_PER_T; ++i) {
-            const int idx = tid + i * kThreads;
-            const int cc = idx / PTS_KJ, jj = idx % PTS_KJ;
-            const int c = c0 + cc, j = j0 + jj;
-            const bool in = c < nrhs && j < nt2;
-            rfa[i] = in ? ib[(long)c * nt + j] : 0.0;
-            rfb[i] = in ? ib[(long)c * nt + (nt - 1 - j)] : 0.0;
+            const double a = load_f64(it.rp, va, i * f_sstep);
+            const double b = load_f64(it.rp, vb, i * f_sstep);
+            rfa[i] = jin ? a : 0.0;
+            rfb[i] = jin ? b : 0.0;
         }
+        const int vi = ((it.k0 + fcc0) * nt2 + j) * 8;
 #pragma unroll
         for (int i = 0; i < PTS_I_PER_T; ++i) {
-            const int idx = tid + i * kThreads;
-            const int kk = idx / PTS_KJ, jj = idx % PTS_KJ;
-            const int k = k0 + kk, j = j0 + jj;
-            const bool jin = j < nt2;
-            rie[i] = (jin && k < ne) ? ie[(long)k * nt2 + j] : 0.0;
-            rio[i] = (jin && k < no) ? io[(long)k * nt2 + j] : 0.0;
+            const double e = load_f64(it.rie, vi, i * i_sstep);
+            const double o = load_f64(it.rio, vi, i * i_sstep);
+            rie[i] = jin ? e : 0.0;
+            rio[i] = jin ? o : 0.0;
         }
     };
     auto stash = [&]() {
 #pragma unroll
         for (int i = 0; i < PTS_F_PER_T; ++i) {
-            const int idx = tid + i * kThreads;
+            const int idx = tid + i * PTS_THREADS;
             const int at = (idx / PTS_KJ) * PTS_F_LD + idx % PTS_KJ;
             Fe[at] = rfa[i] + rfb[i];
             Fo[at] = rfa[i] - rfb[i];
         }
 #pragma unroll
         for (int i = 0; i < PTS_I_PER_T; ++i) {
-            const int idx = tid + i * kThreads;
+            const int idx = tid + i * PTS_THREADS;
             const int at = (idx % PTS_KJ) * PTS_I_LD + idx / PTS_KJ;
             Ie[at] = rie[i];
             Io[at] = rio[i];
         }
     };
 
-    double4_t acc_e[2][4], acc_o[2][4];
+    int idx = blockIdx.x;
+    pts_item cur;
+    if (!pts_next_item(work, n_work, idx, n_work, p2, ipe, ipo, te_off, to_off, mvals, lmax, nt, nt2, nrhs, cur))
+        return;
+    fetch(cur, 0);
+    {
+        double4_t acc_e[PTS_CB][4], acc_o[PTS_CB][4];
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            acc_e[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
-            acc_o[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
-        }
-
-    fetch(0);
-    for (int j0 = 0; j0 < nt2; j0 += PTS_KJ) {
-        stash();
-        __syncthreads();
-        if (j0 + PTS_KJ < nt2) fetch(j0 + PTS_KJ);
-#pragma unroll
-        for (int jj = 0; jj < PTS_KJ; jj += 4) {
-            double be[4], bo[4];
+        for (int cb = 0; cb < PTS_CB; ++cb)
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                be[b] = Ie[(jj + lg) * PTS_I_LD + 16 * b + li];
-                bo[b] = Io[(jj + lg) * PTS_I_LD + 16 * b + li];
+                acc_e[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+                acc_o[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
             }
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
-                const int row = (32 * wave + 16 * cb + li) * PTS_F_LD + jj + lg;
-                const double a_e = Fe[row];
-                const double a_o = Fo[row];
+        for (int j0 = 0; j0 < nt2; j0 += PTS_KJ) {
+            stash();
+            __syncthreads();
+            if (j0 + PTS_KJ < nt2) fetch(cur, j0 + PTS_KJ);
+#pragma unroll 1
+            for (int jj = 0; jj < PTS_KJ; jj += 4) {
+                double be[4], bo[4];
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
-                    acc_e[cb][b] = mfma_f64(a_e, be[b], acc_e[cb][b]);
-                    acc_o[cb][b] = mfma_f64(a_o, bo[b], acc_o[cb][b]);
+                    be[b] = Ie[(jj + lg) * PTS_I_LD + 16 * b + li];
+                    bo[b] = Io[(jj + lg) * PTS_I_LD + 16 * b + li];
+                }
+#pragma unroll
+                for (int cb = 0; cb < PTS_CB; ++cb) {
+                    const int row = (16 * PTS_CB * wave + 16 * cb + li) * PTS_F_LD + jj + lg;
+                    const double a_e = Fe[row];
+                    const double a_o = Fo[row];
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        acc_e[cb][b] = mfma_f64(a_e, be[b], acc_e[cb][b]);
+                        acc_o[cb][b] = mfma_f64(a_o, bo[b], acc_o[cb][b]);
+                    }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
-    }
 
-    double *sb = s + s_off[mp];
+        double *sb = s + s_off[cur.mp];
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
+        for (int cb = 0; cb < PTS_CB; ++cb)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int k = k0 + 16 * b + li;
+            for (int b = 0; b < 4; ++b) {
+                const int k = cur.k0 + 16 * b + li;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int c = c0 + 32 * wave + 16 * cb + lg + 4 * v;
-                if (c < nrhs) {
-                    if (k < ne) sb[(long)c * nl + 2 * k] = acc_e[cb][b][v];
-                    if (k < no) sb[(long)c * nl + 2 * k + 1] = acc_o[cb][b][v];
+                for (int v = 0; v < 4; ++v) {
+                    const int c = cur.c0 + 16 * PTS_CB * wave + 16 * cb + lg + 4 * v;
+                    if (c < nrhs) {
+                        if (k < cur.ne) sb[(long)c * cur.nl + 2 * k] = acc_e[cb][b][v];
+                        if (k < cur.no) sb[(long)c * cur.nl + 2 * k + 1] = acc_o[cb][b][v];
+                    }
                 }
             }
-        }
+    }
 }
 
 }  // namespace
@@ -287,9 +399,9 @@ __global__ __launch_bounds__(kThreads, 1) void pts_fp64_kernel(
 int launch_stp_fp64(raysht_plan *p, const double *s, double *p2, hipStream_t st)
 {
     if (p->n_work_stp == 0) return RAYSHT_OK;
-    hipLaunchKernelGGL(stp_fp64_kernel, dim3(p->n_work_stp), dim3(kThreads), 0, st, s, p2, p->d_pt_even,
-                       p->d_pt_odd, p->d_s_off, p->d_te_off, p->d_to_off, p->d_mvals, p->d_work_stp, p->lmax, p->nt,
-                       p->nt2, p->nrhs);
+    hipLaunchKernelGGL(stp_fp64_kernel, dim3(p->n_work_stp), dim3(STP_THREADS), 0, st, s, p2, p->d_pt_even,
+                       p->d_pt_odd, p->d_s_off, p->d_te_off, p->d_to_off, p->d_mvals, p->d_work_stp, p->n_work_stp,
+                       p->lmax, p->nt, p->nt2, p->nrhs);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
@@ -297,9 +409,9 @@ int launch_stp_fp64(raysht_plan *p, const double *s, double *p2, hipStream_t st)
 int launch_pts_fp64(raysht_plan *p, const double *p2, double *s, hipStream_t st)
 {
     if (p->n_work_pts == 0) return RAYSHT_OK;
-    hipLaunchKernelGGL(pts_fp64_kernel, dim3(p->n_work_pts), dim3(kThreads), 0, st, p2, s, p->d_ip_even,
-                       p->d_ip_odd, p->d_s_off, p->d_te_off, p->d_to_off, p->d_mvals, p->d_work_pts, p->lmax, p->nt,
-                       p->nt2, p->nrhs);
+    hipLaunchKernelGGL(pts_fp64_kernel, dim3(p->n_work_pts), dim3(PTS_THREADS), 0, st, p2, s, p->d_ip_even,
+                       p->d_ip_odd, p->d_s_off, p->d_te_off, p->d_to_off, p->d_mvals, p->d_work_pts, p->n_work_pts,
+                       p->lmax, p->nt, p->nt2, p->nrhs);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }

@@ -53,6 +53,8 @@ struct raysht_plan {
     int *d_mp_of_m = nullptr;  // m -> mp, -1 where the plan has no such m
     raysht::work_item *d_work_stp = nullptr, *d_work_pts = nullptr;
     int n_work_stp = 0, n_work_pts = 0;
+    // staging buffers of the host-array entry points, allocated on first use
+    double *d_stage_s = nullptr, *d_stage_p = nullptr;
 };
 
 struct raysht_cheb {

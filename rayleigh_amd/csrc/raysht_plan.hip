@@ -125,20 +125,29 @@ void raysht_plan_destroy(raysht_plan *p)
     device_free(p->d_mp_of_m);
     device_free(p->d_work_stp);
     device_free(p->d_work_pts);
+    device_free(p->d_stage_s);
+    device_free(p->d_stage_p);
     split_plan_free(p);
     delete p;
 }
 
-int raysht_plan_create(raysht_plan **out, int lmax, int ntheta, const int *m_values, int n_m, int nr, int nfields,
-                       int precision)
+}  // extern "C"
+
+namespace {
+
+// nr = nfields = 0 makes a plan over `ncols` anonymous columns (the rmcontainer
+// 2-D transforms); otherwise ncols = 2 nr nfields.
+int plan_create(raysht_plan **out, int lmax, int ntheta, const int *m_values, int n_m, int nr, int nfields,
+                long ncols, int precision)
 {
     if (!out) return RAYSHT_EINVAL;
     *out = nullptr;
-    if (lmax < 0 || lmax > 8191 || ntheta < 2 || (ntheta & 1) || !m_values || n_m < 1 || n_m > lmax + 1 || nr < 1 ||
-        nfields < 1)
+    if (lmax < 0 || lmax > 8191 || ntheta < 2 || (ntheta & 1) || !m_values || n_m < 1 || n_m > lmax + 1 ||
+        ncols < 1 || nr < 0 || nfields < 0)
         return RAYSHT_EINVAL;
     if (precision != RAYSHT_FP64 && precision != RAYSHT_BF16X3 && precision != RAYSHT_BF16X6) return RAYSHT_EINVAL;
-    if ((long)2 * nr * nfields > (1L << 30)) return RAYSHT_EINVAL;
+    // every m-block of the hybrid and spectral layouts is addressed with 32-bit byte offsets
+    if (ncols * (long)ntheta * 8 >= (1L << 31) || ncols * (long)(lmax + 1) * 8 >= (1L << 31)) return RAYSHT_EINVAL;
     std::vector<int> seen(lmax + 1, -1);
     for (int i = 0; i < n_m; ++i) {
         const int m = m_values[i];
@@ -158,7 +167,7 @@ int raysht_plan_create(raysht_plan **out, int lmax, int ntheta, const int *m_val
     p->nm = n_m;
     p->nr = nr;
     p->nf = nfields;
-    p->nrhs = 2 * nr * nfields;
+    p->nrhs = (int)ncols;
     p->precision = precision;
     p->mvals.assign(m_values, m_values + n_m);
     p->s_off.assign(n_m + 1, 0);
@@ -230,6 +239,26 @@ int raysht_plan_create(raysht_plan **out, int lmax, int ntheta, const int *m_val
     }
     *out = p;
     return RAYSHT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int raysht_plan_create(raysht_plan **out, int lmax, int ntheta, const int *m_values, int n_m, int nr, int nfields,
+                       int precision)
+{
+    if (nr < 1 || nfields < 1) {
+        if (out) *out = nullptr;
+        return RAYSHT_EINVAL;
+    }
+    return plan_create(out, lmax, ntheta, m_values, n_m, nr, nfields, 2L * nr * nfields, precision);
+}
+
+int raysht_plan_create_cols(raysht_plan **out, int lmax, int ntheta, const int *m_values, int n_m, int ncols,
+                            int precision)
+{
+    return plan_create(out, lmax, ntheta, m_values, n_m, 0, 0, ncols, precision);
 }
 
 size_t raysht_spectral_size(const raysht_plan *p) { return p ? (size_t)p->s_off[p->nm] : 0; }

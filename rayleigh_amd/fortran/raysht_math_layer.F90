@@ -1,0 +1,373 @@
+!
+! RaySHT_Math_Layer -- Fortran ISO_C_BINDING layer over libraysht (include/raysht.h).
+!
+! Gives Rayleigh's Math_Layer callers the interfaces they already use, with the
+! same argument types, kinds and ranks (TKR), so that swapping the module is the
+! whole integration:
+!
+!   Initialize_Legendre(nt, lmax, mval, parity)      Legendre_Polynomials.F90:123
+!   Legendre_Transform(rmcontainer4d(:), Real*8(:,:,:))   StP_4d_dgp2  (Legendre_Transforms.F90:968)
+!   Legendre_Transform(Real*8(:,:,:), rmcontainer4d(:))   PtS_4d_dgpv3 (Legendre_Transforms.F90:1077)
+!   Legendre_Transform(rmcontainer(:), Real*8(:,:,:))     StP_2d_dgp   (Legendre_Transforms.F90:588)
+!   Legendre_Transform(Real*8(:,:,:), rmcontainer(:))     PtS_2d_dgpv2 (Legendre_Transforms.F90:506)
+!   FFT_To_Physical(x, plan, rsc) / FFT_To_Spectral(x, plan, rsc) / FFT_To_Spectral_RSC(x, plan)
+!                                                     Fourier_Transform.F90:25-35
+!   Cheby_To_Spectral(f_in, c_out) / Cheby_From_Spectral(c_in, f_out) after Cheby_Init(n, rmin, rmax, grid)
+!                                                     Chebyshev_Polynomials.F90:65-558 (one domain)
+!
+! The data stay in the caller's host arrays: each call hands their addresses to
+! the library's host-array entry points, which stage through device memory.
+! As in the reference, sizes come from the shapes of the arguments; a device plan
+! is built (once) for each (n_r, n_fields) or column count the transforms meet.
+! Errors stop the program with the library's message (the reference has no
+! error returns either).
+!
+Module RaySHT_Math_Layer
+    Use ISO_C_BINDING
+    Use Structures, Only : rmcontainer, rmcontainer4d
+    Implicit None
+    Private
+
+    Public :: Initialize_Legendre, Finalize_Legendre, Legendre_Transform
+    Public :: FFT_To_Physical, FFT_To_Spectral, FFT_To_Spectral_RSC
+    Public :: Cheby_Init, Cheby_To_Spectral, Cheby_From_Spectral, Cheby_Finalize
+    Public :: coloc, gl_weights
+
+    ! the Gauss-Legendre grid, as the reference exposes it (Legendre_Polynomials.F90:28)
+    Real*8, Allocatable, Save :: coloc(:), gl_weights(:)
+
+    Integer, Parameter :: max_plans = 8
+    Type(C_PTR), Save :: plans(max_plans) = C_NULL_PTR
+    Integer, Save :: plan_key(2, max_plans) = -1       ! (n_r, n_fields), or (-ncols, 0) for 2-D
+    Integer, Save :: leg_ntheta = 0, leg_lmax = -1
+    Integer(C_INT), Allocatable, Save :: leg_mvals(:)
+    Type(C_PTR), Save :: cheb = C_NULL_PTR
+    Integer, Save :: cheb_n = 0
+
+    Interface
+        Function c_plan_create(plan, lmax, ntheta, m_values, n_m, nr, nfields, precision) &
+                 Bind(C, name='raysht_plan_create') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Intent(Out) :: plan
+            Integer(C_INT), Value :: lmax, ntheta, n_m, nr, nfields, precision
+            Integer(C_INT), Intent(In) :: m_values(*)
+            Integer(C_INT) :: st
+        End Function
+        Function c_plan_create_cols(plan, lmax, ntheta, m_values, n_m, ncols, precision) &
+                 Bind(C, name='raysht_plan_create_cols') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Intent(Out) :: plan
+            Integer(C_INT), Value :: lmax, ntheta, n_m, ncols, precision
+            Integer(C_INT), Intent(In) :: m_values(*)
+            Integer(C_INT) :: st
+        End Function
+        Subroutine c_plan_destroy(plan) Bind(C, name='raysht_plan_destroy')
+            Import :: C_PTR
+            Type(C_PTR), Value :: plan
+        End Subroutine
+        Function c_plan_nodes(plan, xh, xl, wh, wl) Bind(C, name='raysht_plan_nodes') Result(st)
+            Import :: C_PTR, C_INT, C_DOUBLE
+            Type(C_PTR), Value :: plan
+            Real(C_DOUBLE), Intent(Out) :: xh(*), xl(*), wh(*), wl(*)
+            Integer(C_INT) :: st
+        End Function
+        Function c_stp_host(plan, s_m, p2) Bind(C, name='raysht_legendre_stp_host') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, p2
+            Type(C_PTR), Intent(In) :: s_m(*)
+            Integer(C_INT) :: st
+        End Function
+        Function c_pts_host(plan, p2, s_m) Bind(C, name='raysht_legendre_pts_host') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, p2
+            Type(C_PTR), Intent(In) :: s_m(*)
+            Integer(C_INT) :: st
+        End Function
+        Function c_fft_phys(nphi, howmany, x) Bind(C, name='raysht_fft_to_physical_host') Result(st)
+            Import :: C_PTR, C_INT, C_LONG
+            Integer(C_INT), Value :: nphi
+            Integer(C_LONG), Value :: howmany
+            Type(C_PTR), Value :: x
+            Integer(C_INT) :: st
+        End Function
+        Function c_fft_spec(nphi, howmany, x, rescale) Bind(C, name='raysht_fft_to_spectral_host') Result(st)
+            Import :: C_PTR, C_INT, C_LONG
+            Integer(C_INT), Value :: nphi, rescale
+            Integer(C_LONG), Value :: howmany
+            Type(C_PTR), Value :: x
+            Integer(C_INT) :: st
+        End Function
+        Function c_cheb_create(cb, n, rmin, rmax, grid) Bind(C, name='raysht_cheb_create') Result(st)
+            Import :: C_PTR, C_INT, C_DOUBLE
+            Type(C_PTR), Intent(Out) :: cb
+            Integer(C_INT), Value :: n
+            Real(C_DOUBLE), Value :: rmin, rmax
+            Real(C_DOUBLE), Intent(Out) :: grid(*)
+            Integer(C_INT) :: st
+        End Function
+        Subroutine c_cheb_destroy(cb) Bind(C, name='raysht_cheb_destroy')
+            Import :: C_PTR
+            Type(C_PTR), Value :: cb
+        End Subroutine
+        Function c_cheb_to_spec(cb, f, c, ncols) Bind(C, name='raysht_cheb_to_spectral_host') Result(st)
+            Import :: C_PTR, C_INT, C_LONG
+            Type(C_PTR), Value :: cb, f, c
+            Integer(C_LONG), Value :: ncols
+            Integer(C_INT) :: st
+        End Function
+        Function c_cheb_from_spec(cb, c, f, ncols) Bind(C, name='raysht_cheb_from_spectral_host') Result(st)
+            Import :: C_PTR, C_INT, C_LONG
+            Type(C_PTR), Value :: cb, c, f
+            Integer(C_LONG), Value :: ncols
+            Integer(C_INT) :: st
+        End Function
+        Function c_strerror(status) Bind(C, name='raysht_strerror') Result(msg)
+            Import :: C_PTR, C_INT
+            Integer(C_INT), Value :: status
+            Type(C_PTR) :: msg
+        End Function
+    End Interface
+
+    Interface Legendre_Transform
+        Module Procedure PtS_2d_raysht, StP_2d_raysht
+        Module Procedure PtS_4d_raysht, StP_4d_raysht
+    End Interface
+
+    Interface FFT_To_Spectral
+        Module Procedure r2c_ip4d_raysht
+    End Interface
+
+    Interface FFT_To_Spectral_RSC
+        Module Procedure r2c_ip4d_rsc_raysht
+    End Interface
+
+    Interface FFT_To_Physical
+        Module Procedure c2r_ip4d_raysht
+    End Interface
+
+Contains
+
+    Subroutine check(st, what)
+        Integer(C_INT), Intent(In) :: st
+        Character(len=*), Intent(In) :: what
+        Character(kind=C_CHAR), Pointer :: msg(:)
+        Integer :: n
+        If (st == 0) Return
+        Call c_f_pointer(c_strerror(st), msg, [256])
+        n = 0
+        Do While (n < 256)
+            If (msg(n+1) == C_NULL_CHAR) Exit
+            n = n + 1
+        Enddo
+        Write(6, '(A, A, A, I0, A, 256A)') 'RaySHT: ', what, ' failed (status ', st, '): ', msg(1:n)
+        Error Stop 1
+    End Subroutine check
+
+    ! Initialize_Legendre (Legendre_Polynomials.F90:123): records the grid and m list;
+    ! parity is always on (the non-parity branch is not reproduced).
+    Subroutine Initialize_Legendre(nt, lmax, mval, parity_in)
+        Integer, Intent(In) :: nt, lmax, mval(:)
+        Logical, Intent(In) :: parity_in
+        Type(C_PTR) :: probe
+        Real(C_DOUBLE), Allocatable :: xh(:), xl(:), wh(:), wl(:)
+        If (.not. parity_in) Then
+            Write(6, *) 'RaySHT: only the parity-split transforms are provided'
+            Error Stop 1
+        Endif
+        Call Finalize_Legendre()
+        leg_ntheta = nt
+        leg_lmax = lmax
+        Allocate(leg_mvals(size(mval)))
+        leg_mvals(:) = mval(:)
+        ! the grid, from a one-column plan (the nodes do not depend on the columns)
+        Call check(c_plan_create_cols(probe, leg_lmax, leg_ntheta, leg_mvals, size(leg_mvals), 1, 0), &
+                   'Initialize_Legendre')
+        Allocate(xh(nt), xl(nt), wh(nt), wl(nt), coloc(nt), gl_weights(nt))
+        Call check(c_plan_nodes(probe, xh, xl, wh, wl), 'Initialize_Legendre')
+        coloc(:) = xh(:)
+        gl_weights(:) = wh(:)
+        plans(1) = probe
+        plan_key(:, 1) = (/ -1, 0 /)
+    End Subroutine Initialize_Legendre
+
+    Subroutine Finalize_Legendre()
+        Integer :: i
+        Do i = 1, max_plans
+            If (c_associated(plans(i))) Call c_plan_destroy(plans(i))
+            plans(i) = C_NULL_PTR
+            plan_key(:, i) = -1
+        Enddo
+        If (allocated(leg_mvals)) DeAllocate(leg_mvals)
+        If (allocated(coloc)) DeAllocate(coloc)
+        If (allocated(gl_weights)) DeAllocate(gl_weights)
+    End Subroutine Finalize_Legendre
+
+    ! The device plan for (nr, nf) (nf = 0: a 2-D plan over -nr columns), built on first use.
+    Function plan_for(nr, nf) Result(plan)
+        Integer, Intent(In) :: nr, nf
+        Type(C_PTR) :: plan
+        Integer :: i
+        If (leg_lmax < 0) Then
+            Write(6, *) 'RaySHT: Legendre_Transform called before Initialize_Legendre'
+            Error Stop 1
+        Endif
+        Do i = 1, max_plans
+            If (plan_key(1, i) == nr .and. plan_key(2, i) == nf .and. c_associated(plans(i))) Then
+                plan = plans(i)
+                Return
+            Endif
+        Enddo
+        Do i = 1, max_plans
+            If (.not. c_associated(plans(i))) Exit
+        Enddo
+        If (i > max_plans) Then
+            i = max_plans
+            Call c_plan_destroy(plans(i))
+        Endif
+        If (nf == 0) Then
+            Call check(c_plan_create_cols(plans(i), leg_lmax, leg_ntheta, leg_mvals, size(leg_mvals), -nr, 0), &
+                       'plan (2-D)')
+        Else
+            Call check(c_plan_create(plans(i), leg_lmax, leg_ntheta, leg_mvals, size(leg_mvals), nr, nf, 0), &
+                       'plan')
+        Endif
+        plan_key(:, i) = (/ nr, nf /)
+        plan = plans(i)
+    End Function plan_for
+
+    ! StP_4d_dgp2 (Legendre_Transforms.F90:968-1074): data_in(mp)%data(m:lmax, rmin:rmax, 1:2, 1:nf)
+    ! -> data_out(1:ntheta, 1:2*nr*nf, 1:n_m)
+    Subroutine StP_4d_raysht(data_in, data_out)
+        Type(rmcontainer4d), Intent(In), Target :: data_in(:)
+        Real*8, Intent(InOut), Target, Contiguous :: data_out(:,:,:)
+        Type(C_PTR) :: ptrs(size(data_in))
+        Integer :: mp, nr, nf
+        nr = size(data_in(1)%data, 2)
+        nf = size(data_in(1)%data, 4)
+        If (size(data_out, 2) /= 2*nr*nf .or. size(data_out, 3) /= size(leg_mvals) .or. &
+            size(data_out, 1) /= leg_ntheta) Then
+            Write(6, *) 'RaySHT: StP argument shapes do not match the Legendre setup'
+            Error Stop 1
+        Endif
+        Do mp = 1, size(data_in)
+            ptrs(mp) = c_loc(data_in(mp)%data)
+        Enddo
+        Call check(c_stp_host(plan_for(nr, nf), ptrs, c_loc(data_out)), 'StP')
+    End Subroutine StP_4d_raysht
+
+    ! PtS_4d_dgpv3 (Legendre_Transforms.F90:1077-1175)
+    Subroutine PtS_4d_raysht(data_in, data_out)
+        Real*8, Intent(In), Target, Contiguous :: data_in(:,:,:)
+        Type(rmcontainer4d), Intent(InOut), Target :: data_out(1:)
+        Type(C_PTR) :: ptrs(size(data_out))
+        Integer :: mp, nr, nf
+        nr = size(data_out(1)%data, 2)
+        nf = size(data_out(1)%data, 4)
+        If (size(data_in, 2) /= 2*nr*nf .or. size(data_in, 3) /= size(leg_mvals) .or. &
+            size(data_in, 1) /= leg_ntheta) Then
+            Write(6, *) 'RaySHT: PtS argument shapes do not match the Legendre setup'
+            Error Stop 1
+        Endif
+        Do mp = 1, size(data_out)
+            ptrs(mp) = c_loc(data_out(mp)%data)
+        Enddo
+        Call check(c_pts_host(plan_for(nr, nf), c_loc(data_in), ptrs), 'PtS')
+    End Subroutine PtS_4d_raysht
+
+    ! StP_2d_dgp (Legendre_Transforms.F90:588-668): data_in(mp)%data(m:lmax, 1:nrhs)
+    Subroutine StP_2d_raysht(data_in, data_out)
+        Type(rmcontainer), Intent(In), Target :: data_in(:)
+        Real*8, Intent(InOut), Target, Contiguous :: data_out(:,:,:)
+        Type(C_PTR) :: ptrs(size(data_in))
+        Integer :: mp
+        Do mp = 1, size(data_in)
+            ptrs(mp) = c_loc(data_in(mp)%data)
+        Enddo
+        Call check(c_stp_host(plan_for(-size(data_out, 2), 0), ptrs, c_loc(data_out)), 'StP (2-D)')
+    End Subroutine StP_2d_raysht
+
+    ! PtS_2d_dgpv2 (Legendre_Transforms.F90:506-585)
+    Subroutine PtS_2d_raysht(data_in, data_out)
+        Real*8, Intent(In), Target, Contiguous :: data_in(:,:,:)
+        Type(rmcontainer), Intent(InOut), Target :: data_out(1:)
+        Type(C_PTR) :: ptrs(size(data_out))
+        Integer :: mp
+        Do mp = 1, size(data_out)
+            ptrs(mp) = c_loc(data_out(mp)%data)
+        Enddo
+        Call check(c_pts_host(plan_for(-size(data_in, 2), 0), c_loc(data_in), ptrs), 'PtS (2-D)')
+    End Subroutine PtS_2d_raysht
+
+    ! c2r_ip4D_fftw (Fourier_Transform.F90:142-183); plan and rsc are accepted and unused,
+    ! as in the reference
+    Subroutine c2r_ip4d_raysht(x, plan, rsc)
+        Real*8, Intent(InOut), Target, Contiguous :: x(:,:,:,:)
+        Integer*8, Intent(In), Optional :: plan
+        Logical, Intent(In), Optional :: rsc
+        Integer(C_LONG) :: howmany
+        howmany = int(size(x, 2), C_LONG) * size(x, 3) * size(x, 4)
+        Call check(c_fft_phys(int(size(x, 1) - 2, C_INT), howmany, c_loc(x)), 'FFT_To_Physical')
+    End Subroutine c2r_ip4d_raysht
+
+    ! r2c_ip4D_fftw (Fourier_Transform.F90:100-140)
+    Subroutine r2c_ip4d_raysht(x, plan, rsc)
+        Real*8, Intent(InOut), Target, Contiguous :: x(:,:,:,:)
+        Integer*8, Intent(In), Optional :: plan
+        Logical, Intent(In), Optional :: rsc
+        Integer(C_LONG) :: howmany
+        howmany = int(size(x, 2), C_LONG) * size(x, 3) * size(x, 4)
+        Call check(c_fft_spec(int(size(x, 1) - 2, C_INT), howmany, c_loc(x), 0), 'FFT_To_Spectral')
+    End Subroutine r2c_ip4d_raysht
+
+    ! r2c_ip4D_fftw_rsc (Fourier_Transform.F90:56-98)
+    Subroutine r2c_ip4d_rsc_raysht(x, plan)
+        Real*8, Intent(InOut), Target, Contiguous :: x(:,:,:,:)
+        Integer*8, Intent(In), Optional :: plan
+        Integer(C_LONG) :: howmany
+        howmany = int(size(x, 2), C_LONG) * size(x, 3) * size(x, 4)
+        Call check(c_fft_spec(int(size(x, 1) - 2, C_INT), howmany, c_loc(x), 1), 'FFT_To_Spectral_RSC')
+    End Subroutine r2c_ip4d_rsc_raysht
+
+    ! Cheby_Grid%init for one domain (Chebyshev_Polynomials.F90:65-207); grid(1:n) receives r.
+    Subroutine Cheby_Init(n, rmin, rmax, grid)
+        Integer, Intent(In) :: n
+        Real*8, Intent(In) :: rmin, rmax
+        Real*8, Intent(Out) :: grid(:)
+        Call Cheby_Finalize()
+        Call check(c_cheb_create(cheb, n, rmin, rmax, grid), 'Cheby_Init')
+        cheb_n = n
+    End Subroutine Cheby_Init
+
+    Subroutine Cheby_Finalize()
+        If (c_associated(cheb)) Call c_cheb_destroy(cheb)
+        cheb = C_NULL_PTR
+        cheb_n = 0
+    End Subroutine Cheby_Finalize
+
+    ! To_Spectral4D (Chebyshev_Polynomials.F90:375-448)
+    Subroutine Cheby_To_Spectral(f_in, c_out)
+        Real*8, Intent(In), Target, Contiguous :: f_in(:,:,:,:)
+        Real*8, Intent(InOut), Target, Contiguous :: c_out(:,:,:,:)
+        Integer(C_LONG) :: ncols
+        ncols = int(size(f_in, 2), C_LONG) * size(f_in, 3) * size(f_in, 4)
+        If (size(f_in, 1) /= cheb_n) Then
+            Write(6, *) 'RaySHT: Cheby_To_Spectral radial size does not match Cheby_Init'
+            Error Stop 1
+        Endif
+        Call check(c_cheb_to_spec(cheb, c_loc(f_in), c_loc(c_out), ncols), 'Cheby_To_Spectral')
+    End Subroutine Cheby_To_Spectral
+
+    ! From_Spectral4D (Chebyshev_Polynomials.F90:450-558)
+    Subroutine Cheby_From_Spectral(c_in, f_out)
+        Real*8, Intent(In), Target, Contiguous :: c_in(:,:,:,:)
+        Real*8, Intent(InOut), Target, Contiguous :: f_out(:,:,:,:)
+        Integer(C_LONG) :: ncols
+        ncols = int(size(c_in, 2), C_LONG) * size(c_in, 3) * size(c_in, 4)
+        If (size(c_in, 1) /= cheb_n) Then
+            Write(6, *) 'RaySHT: Cheby_From_Spectral radial size does not match Cheby_Init'
+            Error Stop 1
+        Endif
+        Call check(c_cheb_from_spec(cheb, c_loc(c_in), c_loc(f_out), ncols), 'Cheby_From_Spectral')
+    End Subroutine Cheby_From_Spectral
+
+End Module RaySHT_Math_Layer

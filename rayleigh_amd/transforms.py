@@ -50,16 +50,24 @@ class LegendrePlan:
     """Initialize_Legendre(n_theta, l_max, m_values, parity=.true.) plus the transform
     sizes (n_r radii, n_fields fields) the plan's transforms take."""
 
-    def __init__(self, n_theta: int, l_max: int, m_values, n_r: int, n_fields: int, precision: str = "fp64"):
+    def __init__(self, n_theta: int, l_max: int, m_values, n_r: int, n_fields: int, precision: str = "fp64",
+                 n_cols: int | None = None):
+        """n_cols instead of (n_r, n_fields) gives the rmcontainer (2-D) pair: blocks
+        data(m:l_max, 1:n_cols) (pass n_r = n_fields = 0)."""
         lib = _lib.lib()
         self.n_theta, self.l_max, self.n_r, self.n_fields = int(n_theta), int(l_max), int(n_r), int(n_fields)
         self.m_values = np.ascontiguousarray(m_values, dtype=np.int32)
         self.precision = precision
-        self.nrhs = 2 * self.n_r * self.n_fields
         h = ctypes.c_void_p()
-        st = lib.raysht_plan_create(ctypes.byref(h), self.l_max, self.n_theta,
-                                    self.m_values.ctypes.data_as(ctypes.c_void_p), len(self.m_values),
-                                    self.n_r, self.n_fields, PRECISIONS[precision])
+        mptr = self.m_values.ctypes.data_as(ctypes.c_void_p)
+        if n_cols is None:
+            self.nrhs = 2 * self.n_r * self.n_fields
+            st = lib.raysht_plan_create(ctypes.byref(h), self.l_max, self.n_theta, mptr, len(self.m_values),
+                                        self.n_r, self.n_fields, PRECISIONS[precision])
+        else:
+            self.nrhs = int(n_cols)
+            st = lib.raysht_plan_create_cols(ctypes.byref(h), self.l_max, self.n_theta, mptr, len(self.m_values),
+                                             self.nrhs, PRECISIONS[precision])
         _lib.check(st, "raysht_plan_create")
         self._h = h
         self.spectral_size = int(lib.raysht_spectral_size(h))

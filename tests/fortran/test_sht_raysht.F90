@@ -1,0 +1,284 @@
+!
+! Test program for the RaySHT Fortran binding, written the way Rayleigh's own
+! Test_SHT (src/Test_Suite/Test_SHT.F90) exercises its transforms, on one rank:
+! single (l,m) = (2,1) mode against the analytic field, 10 white-noise round trips,
+! the Legendre + transpose + FFT loop, the 2-D Legendre pair and a Chebyshev round
+! trip.  Prints one PASSING / FAIL line per check (tolerance 1e-10, as Test_SHT.F90:34)
+! and exits non-zero on any failure.
+!
+Program test_sht_raysht
+    Use Structures
+    Use RaySHT_Math_Layer
+    Implicit None
+    Integer, Parameter :: lmax = 41, nt = 64, nphi = 128, nr = 4
+    Real*8, Parameter :: tol = 1.0d-10, pi = 3.1415926535897932384626433832795d0
+    Integer :: mvals(lmax+1), nfail
+    Integer :: i
+
+    ! balanced one-rank m order (Load_Balance.F90:126-226): 0, L, 1, L-1, ...
+    Do i = 0, (lmax+1)/2 - 1
+        mvals(2*i+1) = i
+        mvals(2*i+2) = lmax - i
+    Enddo
+    nfail = 0
+    Call Initialize_Legendre(nt, lmax, mvals, .true.)
+    Call single_mode(nfail)
+    Call white_noise(nfail)
+    Call lt_fft_loop(nfail)
+    Call two_d_pair(nfail)
+    Call cheby_round_trip(nfail)
+    Call Finalize_Legendre()
+    If (nfail > 0) Then
+        Write(6, '(A, I0, A)') 'RaySHT Fortran tests: ', nfail, ' FAILED'
+        Error Stop 1
+    Endif
+    Write(6, '(A)') 'RaySHT Fortran tests: all PASSING'
+
+Contains
+
+    Subroutine report(name, err, nfail)
+        Character(len=*), Intent(In) :: name
+        Real*8, Intent(In) :: err
+        Integer, Intent(InOut) :: nfail
+        If (err < tol) Then
+            Write(6, '(A, A, ES10.3)') name, ' result = PASSING, max error ', err
+        Else
+            Write(6, '(A, A, ES10.3)') name, ' result = FAIL, max error ', err
+            nfail = nfail + 1
+        Endif
+    End Subroutine report
+
+    Subroutine alloc_spec(s, nf)
+        Type(rmcontainer4d), Allocatable, Intent(InOut) :: s(:)
+        Integer, Intent(In) :: nf
+        Integer :: mp
+        Allocate(s(lmax+1))
+        Do mp = 1, lmax+1
+            Allocate(s(mp)%data(mvals(mp):lmax, 1:nr, 1:2, 1:nf))
+            s(mp)%data = 0.0d0
+        Enddo
+    End Subroutine alloc_spec
+
+    ! Test_SHT.F90:47-200
+    Subroutine single_mode(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: nf = 3
+        Type(rmcontainer4d), Allocatable :: s(:)
+        Real*8, Allocatable :: p(:,:,:), truth(:)
+        Real*8 :: radius(nr), d_phys, d_spec, amp, norm
+        Integer :: mp, f, r, ind
+        Call alloc_spec(s, nf)
+        Allocate(p(nt, 2*nr*nf, lmax+1), truth(nt))
+        Do r = 1, nr
+            radius(r) = 0.5d0 + (r-1)*1.0d0/(nr-1)
+        Enddo
+        truth(:) = sqrt(5.0d0/(4.0d0*pi*6.0d0))*(-3.0d0*coloc(:)*sqrt(1.0d0-coloc(:)**2))
+        Do mp = 1, lmax+1
+            If (mvals(mp) /= 1) Cycle
+            Do f = 1, nf
+                s(mp)%data(2, :, 1, f) = (2.0d0*f+1.0d0)*radius(:)*2.0d0
+                s(mp)%data(2, :, 2, f) = -(3.0d0*f*f+2.0d0)*radius(:)**2*2.0d0
+            Enddo
+        Enddo
+        Call Legendre_Transform(s, p)
+        d_phys = 0.0d0
+        Do mp = 1, lmax+1
+            If (mvals(mp) == 1) Then
+                Do f = 1, nf
+                    Do r = 1, nr
+                        ind = r + (f-1)*nr*2
+                        amp = (2.0d0*f+1.0d0)*radius(r)
+                        d_phys = max(d_phys, maxval(abs(amp*truth(:) - p(:, ind, mp))))
+                        ind = r + nr + (f-1)*nr*2
+                        amp = -(3.0d0*f*f+2.0d0)*radius(r)**2
+                        d_phys = max(d_phys, maxval(abs(amp*truth(:) - p(:, ind, mp))))
+                    Enddo
+                Enddo
+            Else
+                d_phys = max(d_phys, maxval(abs(p(:, :, mp))))
+            Endif
+        Enddo
+        Call report('Spec -> Phys', d_phys, nfail)
+        Call Legendre_Transform(p, s)
+        d_spec = 0.0d0
+        Do mp = 1, lmax+1
+            norm = nt
+            s(mp)%data(lmax, :, :, :) = 0.0d0
+            s(mp)%data = s(mp)%data*norm
+            If (mvals(mp) == 1) Then
+                Do f = 1, nf
+                    d_spec = max(d_spec, maxval(abs(s(mp)%data(2, :, 1, f) - (2.0d0*f+1.0d0)*radius(:))))
+                    d_spec = max(d_spec, maxval(abs(s(mp)%data(2, :, 2, f) + (3.0d0*f*f+2.0d0)*radius(:)**2)))
+                    s(mp)%data(2, :, :, f) = 0.0d0
+                Enddo
+            Endif
+            d_spec = max(d_spec, maxval(abs(s(mp)%data)))
+        Enddo
+        Call report('Spec -> Phys -> Spec', d_spec, nfail)
+    End Subroutine single_mode
+
+    ! Test_SHT.F90:328-447
+    Subroutine white_noise(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: nf = 3, nloops = 10
+        Type(rmcontainer4d), Allocatable :: s(:)
+        Real*8, Allocatable :: p(:,:,:)
+        Real*8 :: d, to_phys, to_spec
+        Integer :: mp, m, it
+        Call alloc_spec(s, nf)
+        Allocate(p(nt, 2*nr*nf, lmax+1))
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            If (m < lmax) s(mp)%data(m:lmax-1, :, :, :) = 1.0d0
+            If (m == 0) s(mp)%data(:, :, 2, :) = 0.0d0
+        Enddo
+        Do it = 1, nloops
+            Do mp = 1, lmax+1
+                to_phys = 0.5d0
+                If (mvals(mp) == 0) to_phys = 1.0d0
+                s(mp)%data = s(mp)%data/to_phys
+            Enddo
+            Call Legendre_Transform(s, p)
+            Call Legendre_Transform(p, s)
+            Do mp = 1, lmax+1
+                to_spec = 1.0d0/nt
+                If (mvals(mp) == 0) to_spec = 1.0d0/nphi
+                s(mp)%data(lmax, :, :, :) = 0.0d0
+                s(mp)%data = s(mp)%data/to_spec
+            Enddo
+        Enddo
+        d = 0.0d0
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            If (m < lmax) Then
+                If (m > 0) Then
+                    d = max(d, maxval(abs(1.0d0 - s(mp)%data(m:lmax-1, :, :, :))))
+                Else
+                    d = max(d, maxval(abs(1.0d0 - s(mp)%data(m:lmax-1, :, 1, :))))
+                    d = max(d, maxval(abs(s(mp)%data(m:lmax-1, :, 2, :))))
+                Endif
+            Endif
+            d = max(d, maxval(abs(s(mp)%data(lmax, :, :, :))))
+        Enddo
+        Call report('Spec -> Phys -> Spec -> ... (10 loops)', d/nloops, nfail)
+    End Subroutine white_noise
+
+    ! Test_SHT.F90:449-575, the one-rank transposes written out as index maps
+    Subroutine lt_fft_loop(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: nf = 5
+        Type(rmcontainer4d), Allocatable :: s(:)
+        Real*8, Allocatable :: p(:,:,:), x(:,:,:,:), norm(:,:)
+        Real*8 :: d, ans
+        Integer :: mp, m, l, f, r, t
+        Call alloc_spec(s, nf)
+        Allocate(p(nt, 2*nr*nf, lmax+1), x(nphi+2, nr, nt, nf), norm(lmax+1, 2))
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Do l = m, lmax-1
+                Do r = 1, nr
+                    Do f = 1, nf
+                        s(mp)%data(l, r, 1, f) = (l+1)*1.0d0*(2*f+3)*r
+                        s(mp)%data(l, r, 2, f) = (m+1)*1.0d0*(4*f+5)*r*r
+                    Enddo
+                Enddo
+            Enddo
+            norm(mp, 1) = max(maxval(abs(s(mp)%data(:, :, 1, :))), 1.0d-300)
+            norm(mp, 2) = max(maxval(abs(s(mp)%data(:, :, 2, :))), 1.0d-300)
+            s(mp)%data(:, :, 1, :) = s(mp)%data(:, :, 1, :)/norm(mp, 1)
+            s(mp)%data(:, :, 2, :) = s(mp)%data(:, :, 2, :)/norm(mp, 2)
+            s(mp)%data(lmax, :, :, :) = 0.0d0
+            If (m == 0) s(mp)%data(:, :, 2, :) = 0.0d0
+        Enddo
+        Call Legendre_Transform(s, p)
+        x(:, :, :, :) = 0.0d0
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Do f = 1, nf
+                Do t = 1, nt
+                    Do r = 1, nr
+                        x(2*m+1, r, t, f) = p(t, r + (f-1)*2*nr, mp)
+                        x(2*m+2, r, t, f) = p(t, r + nr + (f-1)*2*nr, mp)
+                    Enddo
+                Enddo
+            Enddo
+        Enddo
+        Call FFT_To_Physical(x, rsc=.true.)
+        Call FFT_To_Spectral(x, rsc=.true.)
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Do f = 1, nf
+                Do t = 1, nt
+                    Do r = 1, nr
+                        p(t, r + (f-1)*2*nr, mp) = x(2*m+1, r, t, f)
+                        p(t, r + nr + (f-1)*2*nr, mp) = x(2*m+2, r, t, f)
+                    Enddo
+                Enddo
+            Enddo
+        Enddo
+        Call Legendre_Transform(p, s)
+        d = 0.0d0
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Do l = m, lmax-1
+                Do r = 1, nr
+                    Do f = 1, nf
+                        ans = (l+1)*1.0d0*(2*f+3)*r/norm(mp, 1)
+                        d = max(d, abs(ans - s(mp)%data(l, r, 1, f)))
+                        ans = 0.0d0
+                        If (m > 0) ans = (m+1)*1.0d0*(4*f+5)*r*r/norm(mp, 2)
+                        d = max(d, abs(ans - s(mp)%data(l, r, 2, f)))
+                    Enddo
+                Enddo
+            Enddo
+        Enddo
+        Call report('LT/FFT/transpose', d, nfail)
+    End Subroutine lt_fft_loop
+
+    ! StP_2d_dgp / PtS_2d_dgpv2 with 7 anonymous columns: the round trip is the identity / N_phi
+    Subroutine two_d_pair(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: ncol = 7
+        Type(rmcontainer), Allocatable :: s(:), s0(:)
+        Real*8, Allocatable :: p(:,:,:)
+        Real*8 :: d
+        Integer :: mp, m, l, c
+        Allocate(s(lmax+1), s0(lmax+1), p(nt, ncol, lmax+1))
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Allocate(s(mp)%data(m:lmax, 1:ncol))
+            Do c = 1, ncol
+                Do l = m, lmax
+                    s(mp)%data(l, c) = cos(0.37d0*l + 1.3d0*c + 0.11d0*m)/(l+1)
+                Enddo
+            Enddo
+            s0(mp) = s(mp)
+        Enddo
+        Call Legendre_Transform(s, p)
+        Call Legendre_Transform(p, s)
+        d = 0.0d0
+        Do mp = 1, lmax+1
+            d = max(d, maxval(abs(s(mp)%data*nphi - s0(mp)%data)))
+        Enddo
+        Call report('2-D Legendre pair', d, nfail)
+    End Subroutine two_d_pair
+
+    ! To_Spectral4D then From_Spectral4D (Chebyshev_Polynomials.F90:375-558)
+    Subroutine cheby_round_trip(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: n = 48
+        Real*8 :: grid(n), f(n, 2, 3, 2), c(n, 2, 3, 2), g(n, 2, 3, 2), d
+        Integer :: i
+        Call Cheby_Init(n, 0.35d0, 1.0d0, grid)
+        Do i = 1, n
+            f(i, :, :, :) = exp(-grid(i))
+            f(i, 2, :, :) = grid(i)**3
+        Enddo
+        Call Cheby_To_Spectral(f, c)
+        Call Cheby_From_Spectral(c, g)
+        d = maxval(abs(g - f))
+        Call Cheby_Finalize()
+        Call report('Chebyshev round trip', d, nfail)
+    End Subroutine cheby_round_trip
+
+End Program test_sht_raysht

@@ -171,3 +171,45 @@ def test_round_trip_identity_full_size(ra):
     # unit-variance coefficients at every l (no decay): the identity holds to the accuracy of
     # the reference's Gauss weights (about 1e-14 relative each); the reference's own tolerance is 1e-10
     assert err < 1e-10
+
+
+@pytest.mark.parametrize("ncols", [7, 1, 130])
+def test_two_d_column_plans(ra, ncols):
+    """StP_2d_dgp / PtS_2d_dgpv2: data(m:l_max, 1:ncols) blocks with any column count."""
+    nt, L = 96, 63
+    mv = O.m_order_balanced(L)
+    plan = ra.LegendrePlan(nt, L, mv, 0, 0, n_cols=ncols)
+    tabs = O.legendre_tables(nt, L, mv)
+    rng = np.random.default_rng(ncols)
+    s = rng.standard_normal(plan.spectral_size)
+    p = empty(plan.hybrid_size)
+    plan.stp(dev(s), p)
+    pr = O.stp_cols(nt, L, mv, ncols, tabs, s)
+    assert rel_max(host(p), pr) < TOL_FP64
+    s2 = empty(plan.spectral_size)
+    plan.pts(dev(pr), s2)
+    assert rel_max(host(s2), O.pts_cols(nt, L, mv, ncols, tabs, pr)) < TOL_FP64
+    with pytest.raises(Exception):
+        plan.reform_2a3a(p, empty((2 * nt + 2) * nt), 2 * nt)  # no radial structure to permute
+
+
+def test_host_array_entry_points(ra):
+    """raysht_legendre_stp_host / _pts_host: per-m host pointers, as the Fortran layer passes them."""
+    import ctypes
+    from rayleigh_amd import _lib
+    nt, L, nr, nf = 64, 41, 3, 2
+    mv = O.m_order_balanced(L)
+    plan = ra.LegendrePlan(nt, L, mv, nr, nf)
+    tabs = O.legendre_tables(nt, L, mv)
+    s = O.synthetic_spectral(L, mv, nr, nf, seed=4)
+    off = O.spectral_offsets(L, mv, nr, nf)
+    blocks = [np.ascontiguousarray(s[off[i]:off[i + 1]]) for i in range(len(mv))]
+    ptrs = (ctypes.c_void_p * len(mv))(*[b.ctypes.data for b in blocks])
+    p = np.zeros(plan.hybrid_size)
+    _lib.check(_lib.lib().raysht_legendre_stp_host(plan.handle, ptrs, p.ctypes.data_as(ctypes.c_void_p)), "stp_host")
+    pr = O.stp(nt, L, mv, nr, nf, tabs, s)
+    assert rel_max(p, pr) < TOL_FP64
+    outs = [np.zeros_like(b) for b in blocks]
+    optrs = (ctypes.c_void_p * len(mv))(*[b.ctypes.data for b in outs])
+    _lib.check(_lib.lib().raysht_legendre_pts_host(plan.handle, pr.ctypes.data_as(ctypes.c_void_p), optrs), "pts_host")
+    assert rel_max(np.concatenate(outs), O.pts(nt, L, mv, nr, nf, tabs, pr)) < TOL_FP64
