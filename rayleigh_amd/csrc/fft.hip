@@ -30,7 +30,8 @@ __device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im
 __device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
 
 constexpr int kFftThreads = 256;
-constexpr int kMaxHalf = 2048;  // largest complex length (nphi/2) held in LDS
+constexpr int kMaxHalf = 4096;       // largest complex length (nphi/2) held in LDS
+constexpr int kFftLdsBudget = 65536;  // bytes of LDS per workgroup (two workgroups per CU)
 
 // e^{sign 2 pi i t / n2}, t in [0, n2), from the table of e^{+2 pi i u / nphi}, nphi = 2 n2
 __device__ __forceinline__ cplx twiddle(const double2 *__restrict__ tw, int nphi, long t, int sign)
@@ -39,51 +40,81 @@ __device__ __forceinline__ cplx twiddle(const double2 *__restrict__ tw, int nphi
     return {v.x, sign * v.y};
 }
 
-// Length-n2 complex DFT of buf (in LDS) with radix schedule `radices` (product n2),
-// sign = +1 (inverse, unnormalised) or -1 (forward).  Result is returned in the
-// buffer the function returns (buf or tmp).
-__device__ cplx *stockham(cplx *buf, cplx *tmp, int n2, int nphi, const int *radices, int nrad,
+// One radix-R Stockham butterfly (R = 2, 3 or 4, fixed at compile time) of one line.
+template <int R>
+__device__ __forceinline__ void stockham_butterfly(const cplx *sl, cplx *dl, int j, int nb, int ns, int nphi,
+                                                   const double2 *__restrict__ tw, int sign)
+{
+    const int jm = j % ns;
+    // table step for angle 2 pi jm / (ns R) on a length-nphi table
+    const long step = (long)jm * (nphi / (ns * R));
+    cplx v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        cplx a = sl[j + r * nb];
+        if (r > 0 && jm > 0) a = cmul(a, twiddle(tw, nphi, step * r, sign));
+        v[r] = a;
+    }
+    cplx y[R];
+    if constexpr (R == 2) {
+        y[0] = cadd(v[0], v[1]);
+        y[1] = csub(v[0], v[1]);
+    } else if constexpr (R == 4) {
+        const cplx a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+        const cplx c = cadd(v[1], v[3]), d0 = csub(v[1], v[3]);
+        const cplx d = {-sign * d0.im, sign * d0.re};  // (sign i) * (v1 - v3)
+        y[0] = cadd(a, c);
+        y[1] = cadd(b, d);
+        y[2] = csub(a, c);
+        y[3] = csub(b, d);
+    } else {
+        const double h = 0.8660254037844386467637231707529361834714;  // sqrt(3)/2
+        const cplx t1 = cadd(v[1], v[2]);
+        const cplx t2 = {v[0].re - 0.5 * t1.re, v[0].im - 0.5 * t1.im};
+        const cplx d0 = csub(v[1], v[2]);
+        const cplx t3 = {-sign * h * d0.im, sign * h * d0.re};
+        y[0] = cadd(v[0], t1);
+        y[1] = cadd(t2, t3);
+        y[2] = csub(t2, t3);
+    }
+    const int base = (j / ns) * ns * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dl[base + r * ns] = y[r];
+}
+
+template <int R>
+__device__ __forceinline__ void stockham_stage(const cplx *src, cplx *dst, int n2, int lines, int ns, int nphi,
+                                               const double2 *__restrict__ tw, int sign)
+{
+    const int nb = n2 / R;
+    const int total = nb * lines;
+    for (int jj = threadIdx.x; jj < total; jj += blockDim.x) {
+        const int line = jj / nb, j = jj - line * nb;
+        stockham_butterfly<R>(src + line * n2, dst + line * n2, j, nb, ns, nphi, tw, sign);
+    }
+}
+
+// `lines` independent length-n2 complex DFTs of buf[line][n2] (LDS); the radix of stage
+// s is 2 bits of `code` (0: 2, 1: 4, 2: 3), product n2; sign = +1 (inverse,
+// unnormalised) or -1 (forward).  Stockham auto-sort, ping-ponging with tmp.  Returns
+// the buffer that holds the result.
+__device__ cplx *stockham(cplx *buf, cplx *tmp, int n2, int lines, int nphi, int code, int nrad,
                           const double2 *__restrict__ tw, int sign)
 {
     cplx *src = buf, *dst = tmp;
     int ns = 1;
     for (int s = 0; s < nrad; ++s) {
-        const int R = radices[s];
-        const int nb = n2 / R;
-        for (int j = threadIdx.x; j < nb; j += blockDim.x) {
-            const int jm = j % ns;
-            // table step for angle 2 pi jm / (ns R) on a length-nphi table
-            const long step = (long)jm * (nphi / (ns * R));
-            cplx v[4];
-            for (int r = 0; r < R; ++r) {
-                cplx a = src[j + r * nb];
-                if (r > 0 && jm > 0) a = cmul(a, twiddle(tw, nphi, step * r, sign));
-                v[r] = a;
-            }
-            cplx y[4];
-            if (R == 2) {
-                y[0] = cadd(v[0], v[1]);
-                y[1] = csub(v[0], v[1]);
-            } else if (R == 4) {
-                const cplx a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
-                const cplx c = cadd(v[1], v[3]), d0 = csub(v[1], v[3]);
-                const cplx d = {-sign * d0.im, sign * d0.re};  // (sign i) * (v1 - v3)
-                y[0] = cadd(a, c);
-                y[1] = cadd(b, d);
-                y[2] = csub(a, c);
-                y[3] = csub(b, d);
-            } else {  // R == 3
-                const double h = 0.8660254037844386467637231707529361834714;  // sqrt(3)/2
-                const cplx t1 = cadd(v[1], v[2]);
-                const cplx t2 = {v[0].re - 0.5 * t1.re, v[0].im - 0.5 * t1.im};
-                const cplx d0 = csub(v[1], v[2]);
-                const cplx t3 = {-sign * h * d0.im, sign * h * d0.re};
-                y[0] = cadd(v[0], t1);
-                y[1] = cadd(t2, t3);
-                y[2] = csub(t2, t3);
-            }
-            const int base = (j / ns) * ns * R + jm;
-            for (int r = 0; r < R; ++r) dst[base + r * ns] = y[r];
+        const int c = (code >> (2 * s)) & 3;
+        int R;
+        if (c == 1) {
+            stockham_stage<4>(src, dst, n2, lines, ns, nphi, tw, sign);
+            R = 4;
+        } else if (c == 0) {
+            stockham_stage<2>(src, dst, n2, lines, ns, nphi, tw, sign);
+            R = 2;
+        } else {
+            stockham_stage<3>(src, dst, n2, lines, ns, nphi, tw, sign);
+            R = 3;
         }
         __syncthreads();
         cplx *t = src;
@@ -94,76 +125,69 @@ __device__ cplx *stockham(cplx *buf, cplx *tmp, int n2, int nphi, const int *rad
     return src;
 }
 
-__global__ __launch_bounds__(kFftThreads) void fft_c2r_kernel(int nphi, long howmany, double *__restrict__ x,
-                                                             const double2 *__restrict__ tw, int4 sched, int nrad)
+// dynamic LDS: 2 x lines x n2 complex
+__global__ __launch_bounds__(kFftThreads) void fft_c2r_kernel(int nphi, long howmany, int lines, double *__restrict__ x,
+                                                             const double2 *__restrict__ tw, int code, int nrad)
 {
-    __shared__ cplx bufa[kMaxHalf];
-    __shared__ cplx bufb[kMaxHalf];
+    extern __shared__ cplx fft_lds[];
     const int n2 = nphi / 2;
     const long ld = nphi + 2;
-    // radix schedule decoded from the packed int4 (up to 16 stages of 2 bits + the radix-3 flag)
-    int rad[16];
-    int nr = 0;
-    {
-        int code = sched.x;
-        for (int s = 0; s < nrad; ++s) {
-            const int c = (code >> (2 * s)) & 3;
-            rad[nr++] = c == 0 ? 2 : c == 1 ? 4 : 3;
-        }
-    }
-    for (long line = blockIdx.x; line < howmany; line += gridDim.x) {
-        double *xl = x + line * ld;
+    cplx *bufa = fft_lds, *bufb = fft_lds + (long)lines * n2;
+    for (long l0 = (long)blockIdx.x * lines; l0 < howmany; l0 += (long)gridDim.x * lines) {
+        const int nl = (int)((howmany - l0) < lines ? (howmany - l0) : lines);
         // Z_k = A_k + i B_k, A = X_k + conj(X_{n2-k}), B = (X_k - conj(X_{n2-k})) e^{+2 pi i k / nphi};
         // Im X_0 and Im X_{n2} do not enter a c2r.
-        for (int k = threadIdx.x; k < n2; k += blockDim.x) {
+        for (int kk = threadIdx.x; kk < nl * n2; kk += blockDim.x) {
+            const int line = kk / n2, k = kk - line * n2;
+            const double *xl = x + (l0 + line) * ld;
             cplx a = {xl[2 * k], k == 0 ? 0.0 : xl[2 * k + 1]};
             const int km = n2 - k;
             cplx b = {xl[2 * km], km == n2 ? 0.0 : xl[2 * km + 1]};
             const cplx bc = cconj(b);
             const cplx A = cadd(a, bc);
             const cplx B = cmul(csub(a, bc), twiddle(tw, nphi, k, +1));
-            bufa[k] = {A.re - B.im, A.im + B.re};
+            bufa[line * n2 + k] = {A.re - B.im, A.im + B.re};
         }
         __syncthreads();
-        cplx *res = stockham(bufa, bufb, n2, nphi, rad, nr, tw, +1);
-        for (int k = threadIdx.x; k < n2; k += blockDim.x) {
-            xl[2 * k] = res[k].re;
-            xl[2 * k + 1] = res[k].im;
+        cplx *res = stockham(bufa, bufb, n2, nl, nphi, code, nrad, tw, +1);
+        for (int kk = threadIdx.x; kk < nl * n2; kk += blockDim.x) {
+            const int line = kk / n2, k = kk - line * n2;
+            double *xl = x + (l0 + line) * ld;
+            const cplx z = res[line * n2 + k];
+            xl[2 * k] = z.re;
+            xl[2 * k + 1] = z.im;
         }
-        if (threadIdx.x == 0) {
-            xl[nphi] = 0.0;
-            xl[nphi + 1] = 0.0;
+        for (int line = threadIdx.x; line < nl; line += blockDim.x) {
+            x[(l0 + line) * ld + nphi] = 0.0;
+            x[(l0 + line) * ld + nphi + 1] = 0.0;
         }
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kFftThreads) void fft_r2c_kernel(int nphi, long howmany, double *__restrict__ x,
-                                                             const double2 *__restrict__ tw, int4 sched, int nrad,
+__global__ __launch_bounds__(kFftThreads) void fft_r2c_kernel(int nphi, long howmany, int lines, double *__restrict__ x,
+                                                             const double2 *__restrict__ tw, int code, int nrad,
                                                              int rescale)
 {
-    __shared__ cplx bufa[kMaxHalf];
-    __shared__ cplx bufb[kMaxHalf];
+    extern __shared__ cplx fft_lds[];
     const int n2 = nphi / 2;
     const long ld = nphi + 2;
-    int rad[16];
-    int nr = 0;
-    {
-        int code = sched.x;
-        for (int s = 0; s < nrad; ++s) {
-            const int c = (code >> (2 * s)) & 3;
-            rad[nr++] = c == 0 ? 2 : c == 1 ? 4 : 3;
+    cplx *bufa = fft_lds, *bufb = fft_lds + (long)lines * n2;
+    for (long l0 = (long)blockIdx.x * lines; l0 < howmany; l0 += (long)gridDim.x * lines) {
+        const int nl = (int)((howmany - l0) < lines ? (howmany - l0) : lines);
+        for (int kk = threadIdx.x; kk < nl * n2; kk += blockDim.x) {
+            const int line = kk / n2, k = kk - line * n2;
+            const double *xl = x + (l0 + line) * ld;
+            bufa[line * n2 + k] = {xl[2 * k], xl[2 * k + 1]};
         }
-    }
-    for (long line = blockIdx.x; line < howmany; line += gridDim.x) {
-        double *xl = x + line * ld;
-        for (int k = threadIdx.x; k < n2; k += blockDim.x) bufa[k] = {xl[2 * k], xl[2 * k + 1]};
         __syncthreads();
-        cplx *res = stockham(bufa, bufb, n2, nphi, rad, nr, tw, -1);
+        cplx *res = stockham(bufa, bufb, n2, nl, nphi, code, nrad, tw, -1);
         // X_k = E_k + e^{-2 pi i k / nphi} O_k, E = (Z_k + conj Z_{n2-k}) / 2, O = (Z_k - conj Z_{n2-k}) / (2i)
-        for (int k = threadIdx.x; k <= n2; k += blockDim.x) {
-            const cplx zk = res[k % n2];
-            const cplx zc = cconj(res[(n2 - k) % n2]);
+        for (int kk = threadIdx.x; kk < nl * (n2 + 1); kk += blockDim.x) {
+            const int line = kk / (n2 + 1), k = kk - line * (n2 + 1);
+            const cplx *rl = res + line * n2;
+            const cplx zk = rl[k % n2];
+            const cplx zc = cconj(rl[(n2 - k) % n2]);
             const cplx E = {0.5 * (zk.re + zc.re), 0.5 * (zk.im + zc.im)};
             const cplx D = csub(zk, zc);
             const cplx O = {0.5 * D.im, -0.5 * D.re};
@@ -173,6 +197,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_r2c_kernel(int nphi, long how
                 re = (k == 0) ? re / (2.0 * (double)nphi) : re / (double)nphi;
                 if (k > 0) im = im / (double)nphi;
             }
+            double *xl = x + (l0 + line) * ld;
             xl[2 * k] = re;
             xl[2 * k + 1] = im;
         }
@@ -180,43 +205,62 @@ __global__ __launch_bounds__(kFftThreads) void fft_r2c_kernel(int nphi, long how
     }
 }
 
-// 2a -> 3a: each thread writes one element of a physical line.
-__global__ void permute_2a3a_kernel(int nt, int nr, int nf, int nphi, int lmax, const int *__restrict__ mp_of_m,
-                                    const double *__restrict__ p2, double *__restrict__ p3, long total)
+// The one-rank row transposes are 2-D transposes per (r, f): the hybrid side is
+// (mode q = 2m+imi) x theta with theta contiguous, the physical side theta x q with q
+// contiguous.  A 32 x 32 tile goes through LDS so both sides are read and written in
+// 256-byte rows.
+constexpr int kTile = 32;
+
+// 2a -> 3a: p3 line (f, theta, r) gets Re/Im of m at 2m, 2m+1 and zeros elsewhere.
+__global__ __launch_bounds__(256) void permute_2a3a_kernel(int nt, int nr, int nf, int nphi, int lmax,
+                                                           const int *__restrict__ mp_of_m,
+                                                           const double *__restrict__ p2, double *__restrict__ p3)
 {
-    const long ld = nphi + 2;
-    const long nrhs = 2L * nr * nf;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-        const int q = (int)(i % ld);
-        const long line = i / ld;
-        const int r = (int)(line % nr);
-        const long rest = line / nr;
-        const int t = (int)(rest % nt);
-        const int f = (int)(rest / nt);
-        const int m = q >> 1, imi = q & 1;
+    __shared__ double tile[kTile][kTile + 1];
+    const int t0 = blockIdx.x * kTile, q0 = blockIdx.y * kTile;
+    const int rf = blockIdx.z, r = rf % nr, f = rf / nr;
+    const long ld = nphi + 2, nrhs = 2L * nr * nf;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int i = ty; i < kTile; i += 8) {
+        const int q = q0 + i, t = t0 + tx;
         double v = 0.0;
-        if (m <= lmax) {
+        const int m = q >> 1;
+        if (t < nt && m <= lmax) {
             const int mp = mp_of_m[m];
-            if (mp >= 0) v = p2[((long)mp * nrhs + r + (long)imi * nr + 2L * nr * f) * nt + t];
+            if (mp >= 0) v = p2[((long)mp * nrhs + r + (long)(q & 1) * nr + 2L * nr * f) * nt + t];
         }
-        p3[i] = v;
+        tile[i][tx] = v;
+    }
+    __syncthreads();
+    for (int i = ty; i < kTile; i += 8) {
+        const int t = t0 + i, q = q0 + tx;
+        if (t < nt && q < ld) p3[(((long)f * nt + t) * nr + r) * ld + q] = tile[tx][i];
     }
 }
 
-// 3b -> 2b: each thread writes one hybrid element.
-__global__ void permute_3b2b_kernel(int nt, int nr, int nf, int nphi, const int *__restrict__ mvals,
-                                    const double *__restrict__ p3, double *__restrict__ p2, long total)
+// 3b -> 2b: the inverse map, reading back only the m of the plan (m <= lmax).
+__global__ __launch_bounds__(256) void permute_3b2b_kernel(int nt, int nr, int nf, int nphi, int lmax,
+                                                           const int *__restrict__ mp_of_m,
+                                                           const double *__restrict__ p3, double *__restrict__ p2)
 {
-    const long ld = nphi + 2;
-    const long nrhs = 2L * nr * nf;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-        const int t = (int)(i % nt);
-        const long col = i / nt;
-        const int c = (int)(col % nrhs);
-        const int mp = (int)(col / nrhs);
-        const int r = c % nr, imi = (c / nr) & 1, f = c / (2 * nr);
-        const int m = mvals[mp];
-        p2[i] = p3[(((long)f * nt + t) * nr + r) * ld + 2 * m + imi];
+    __shared__ double tile[kTile][kTile + 1];
+    const int t0 = blockIdx.x * kTile, q0 = blockIdx.y * kTile;
+    const int rf = blockIdx.z, r = rf % nr, f = rf / nr;
+    const long ld = nphi + 2, nrhs = 2L * nr * nf;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int qmax = 2 * (lmax + 1);
+    for (int i = ty; i < kTile; i += 8) {
+        const int t = t0 + i, q = q0 + tx;
+        tile[i][tx] = (t < nt && q < qmax) ? p3[(((long)f * nt + t) * nr + r) * ld + q] : 0.0;
+    }
+    __syncthreads();
+    for (int i = ty; i < kTile; i += 8) {
+        const int q = q0 + i, t = t0 + tx;
+        const int m = q >> 1;
+        if (t < nt && q < qmax) {
+            const int mp = mp_of_m[m];
+            if (mp >= 0) p2[((long)mp * nrhs + r + (long)(q & 1) * nr + 2L * nr * f) * nt + t] = tile[tx][i];
+        }
     }
 }
 
@@ -272,6 +316,14 @@ bool fft_schedule(int n2, int *code, int *nrad)
     return true;
 }
 
+// lines transformed together by one workgroup: as many as fit the LDS budget (at least one)
+int fft_lines_per_block(int n2)
+{
+    const int per_line = 2 * 16 * n2;
+    int lines = kFftLdsBudget / per_line;
+    return lines < 1 ? 1 : (lines > 16 ? 16 : lines);
+}
+
 int check_fft_args(int nphi, long howmany, const double *x)
 {
     if (nphi < 2 || (nphi & 1) || howmany < 0 || (!x && howmany > 0)) return RAYSHT_EINVAL;
@@ -296,9 +348,11 @@ int raysht_fft_to_physical(int nphi, long howmany, double *x, void *stream)
     if ((rc = get_twiddles(nphi, &tw)) != RAYSHT_OK) return rc;
     int code = 0, nrad = 0;
     fft_schedule(nphi / 2, &code, &nrad);
-    const long grid = howmany < 65536 * 8 ? howmany : 65536 * 8;
-    hipLaunchKernelGGL(fft_c2r_kernel, dim3((unsigned)grid), dim3(kFftThreads), 0,
-                       reinterpret_cast<hipStream_t>(stream), nphi, howmany, x, tw, make_int4(code, 0, 0, 0), nrad);
+    const int lines = fft_lines_per_block(nphi / 2);
+    const long blocks = std::min<long>((howmany + lines - 1) / lines, 1L << 20);
+    const size_t lds = 2 * sizeof(double2) * (size_t)lines * (nphi / 2);
+    hipLaunchKernelGGL(fft_c2r_kernel, dim3((unsigned)blocks), dim3(kFftThreads), lds,
+                       reinterpret_cast<hipStream_t>(stream), nphi, howmany, lines, x, tw, code, nrad);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
@@ -311,9 +365,11 @@ int raysht_fft_to_spectral(int nphi, long howmany, double *x, int rescale, void 
     if ((rc = get_twiddles(nphi, &tw)) != RAYSHT_OK) return rc;
     int code = 0, nrad = 0;
     fft_schedule(nphi / 2, &code, &nrad);
-    const long grid = howmany < 65536 * 8 ? howmany : 65536 * 8;
-    hipLaunchKernelGGL(fft_r2c_kernel, dim3((unsigned)grid), dim3(kFftThreads), 0,
-                       reinterpret_cast<hipStream_t>(stream), nphi, howmany, x, tw, make_int4(code, 0, 0, 0), nrad,
+    const int lines = fft_lines_per_block(nphi / 2);
+    const long blocks = std::min<long>((howmany + lines - 1) / lines, 1L << 20);
+    const size_t lds = 2 * sizeof(double2) * (size_t)lines * (nphi / 2);
+    hipLaunchKernelGGL(fft_r2c_kernel, dim3((unsigned)blocks), dim3(kFftThreads), lds,
+                       reinterpret_cast<hipStream_t>(stream), nphi, howmany, lines, x, tw, code, nrad,
                        rescale ? 1 : 0);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
@@ -322,11 +378,10 @@ int raysht_fft_to_spectral(int nphi, long howmany, double *x, int rescale, void 
 int raysht_permute_2a3a(raysht_plan *p, int nphi, const double *p2, double *p3, void *stream)
 {
     if (!p || !p2 || !p3 || p->nr < 1 || nphi < 2 * (p->lmax + 1) || (nphi & 1)) return RAYSHT_EINVAL;
-    const long total = (long)(nphi + 2) * p->nr * p->nt * p->nf;
-    const long blocks = std::min<long>((total + 255) / 256, 1L << 20);
-    hipLaunchKernelGGL(permute_2a3a_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                       reinterpret_cast<hipStream_t>(stream), p->nt, p->nr, p->nf, nphi, p->lmax, p->d_mp_of_m, p2,
-                       p3, total);
+    if ((long)p->nr * p->nf > 65535) return RAYSHT_EUNSUPPORTED;
+    dim3 grid((p->nt + kTile - 1) / kTile, (nphi + 2 + kTile - 1) / kTile, p->nr * p->nf);
+    hipLaunchKernelGGL(permute_2a3a_kernel, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), p->nt, p->nr,
+                       p->nf, nphi, p->lmax, p->d_mp_of_m, p2, p3);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
@@ -334,10 +389,10 @@ int raysht_permute_2a3a(raysht_plan *p, int nphi, const double *p2, double *p3, 
 int raysht_permute_3b2b(raysht_plan *p, int nphi, const double *p3, double *p2, void *stream)
 {
     if (!p || !p2 || !p3 || p->nr < 1 || nphi < 2 * (p->lmax + 1) || (nphi & 1)) return RAYSHT_EINVAL;
-    const long total = (long)p->nt * p->nrhs * p->nm;
-    const long blocks = std::min<long>((total + 255) / 256, 1L << 20);
-    hipLaunchKernelGGL(permute_3b2b_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                       reinterpret_cast<hipStream_t>(stream), p->nt, p->nr, p->nf, nphi, p->d_mvals, p3, p2, total);
+    if ((long)p->nr * p->nf > 65535) return RAYSHT_EUNSUPPORTED;
+    dim3 grid((p->nt + kTile - 1) / kTile, (2 * (p->lmax + 1) + kTile - 1) / kTile, p->nr * p->nf);
+    hipLaunchKernelGGL(permute_3b2b_kernel, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), p->nt, p->nr,
+                       p->nf, nphi, p->lmax, p->d_mp_of_m, p3, p2);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
