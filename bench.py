@@ -154,6 +154,79 @@ def reference_check_and_baseline(res, mvals, sample_m: int):
     return cpu, err_stp, err_pts
 
 
+def stack_bench(ra, steps: int, warmup: int):
+    """The transform stack at l_max = 255, N_theta = 384, N_phi = 768, N_r = 128, 8 fields:
+    Chebyshev From_Spectral -> StP -> 2a->3a -> c2r, then r2c -> 3b->2b -> PtS -> To_Spectral,
+    each stage timed with HIP events; GB/s of the bandwidth-bound stages.  The radial
+    buffer is transformed in place of Rayleigh's p1 layout (the 1a<->2a radial transposes
+    between the Chebyshev and Legendre stages are not part of this build)."""
+    import torch
+    from oracle import oracle as O
+    L, nt, nphi, nr, nf = 255, 384, 768, 128, 8
+    mv = O.m_order_balanced(L)
+    plan = ra.LegendrePlan(nt, L, mv, nr, nf)
+    cheb = ra.ChebyGrid(nr, 0.35, 1.0)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    nlm = (L + 1) * (L + 2) // 2
+    rad = torch.randn(nr * 2 * nlm * nf, dtype=torch.float64, device="cuda", generator=g)
+    rad2 = torch.empty_like(rad)
+    s = torch.randn(plan.spectral_size, dtype=torch.float64, device="cuda", generator=g)
+    p2 = torch.empty(plan.hybrid_size, dtype=torch.float64, device="cuda")
+    n3 = (nphi + 2) * nr * nt * nf
+    p3 = torch.empty(n3, dtype=torch.float64, device="cuda")
+    s2 = torch.empty_like(s)
+    stages = [
+        ("cheb_from_spectral", lambda: cheb.from_spectral(rad, rad2), 16.0 * nr * 2 * nlm * nf),
+        ("stp", lambda: plan.stp(s, p2), None),
+        ("permute_2a3a", lambda: plan.reform_2a3a(p2, p3, nphi), 8.0 * (plan.hybrid_size + n3)),
+        ("fft_c2r", lambda: ra.fft_to_physical(p3, nphi), 16.0 * n3),
+        ("fft_r2c", lambda: ra.fft_to_spectral(p3, nphi), 16.0 * n3),
+        ("permute_3b2b", lambda: plan.reform_3b2b(p3, p2, nphi), 8.0 * 2 * plan.hybrid_size),
+        ("pts", lambda: plan.pts(p2, s2), None),
+        ("cheb_to_spectral", lambda: cheb.to_spectral(rad2, rad), 16.0 * nr * 2 * nlm * nf),
+    ]
+    for _ in range(warmup):
+        for _, fn, _ in stages:
+            fn()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record()
+        for i, (_, fn, _) in enumerate(stages):
+            fn()
+            ev[k][i + 1].record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    F = flops_per_direction(L, nt, nr, nf)
+    out = {"workload": "stack", "l_max": L, "n_theta": nt, "n_phi": nphi, "n_r": nr, "n_fields": nf,
+           "ms_per_round_trip": wall * 1e3, "stages": {}}
+    for i, (name, _, nbytes) in enumerate(stages):
+        ms = float(np.mean([e[i].elapsed_time(e[i + 1]) for e in ev]))
+        rec = {"ms": ms}
+        if nbytes is None:
+            rec["tflops"] = F / (ms * 1e-3) / 1e12
+            rec["frac_fp64_peak"] = rec["tflops"] / PEAK_FP64_TFLOPS
+        else:
+            rec["gbps"] = nbytes / (ms * 1e-3) / 1e9
+            rec["frac_hbm_peak"] = rec["gbps"] / 8000.0
+        out["stages"][name] = rec
+    print(json.dumps(out), flush=True)
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (profiles/legendre_fp64_lmax511.traffic.json: FETCH_SIZE and WRITE_SIZE in KiB
+    per dispatch, each scaled by the same counter on a 1 GiB device copy), or None."""
+    path = os.path.join(ROOT, "profiles", "legendre_fp64_lmax511.traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return (rec["fetch_kib"] * rec["fetch_factor"] + rec["write_kib"] * rec["write_factor"]) * 1024.0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +235,8 @@ def main():
     ap.add_argument("--forms", default="fp64")
     ap.add_argument("--cpu-sample-m", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="legendre", choices=["legendre", "stack"],
+                    help="legendre: the l_max=511 round trip (headline); stack: l_max=255 full stack per stage")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy (scales profiler byte counters)")
     args = ap.parse_args()
@@ -175,6 +250,9 @@ def main():
     from oracle import oracle as O
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a GPU")
+    if args.workload == "stack":
+        stack_bench(ra, args.steps, args.warmup)
+        return
     mvals = O.m_order_balanced(LMAX)
     F = flops_per_direction()
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
@@ -215,8 +293,8 @@ def main():
                                "all m (balanced order)", "l_max": LMAX, "n_theta": NTHETA, "n_r": NR,
                    "n_fields": NF, "form": head},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None, "kernel": f"{dom}_fp64_kernel",
-                     "kernel_ms": dom_ms},
+                     "frac": achieved / peak, "traffic": measured_traffic(f"{dom}_fp64_kernel"),
+                     "kernel": f"{dom}_fp64_kernel", "kernel_ms": dom_ms},
         "cpu_baseline": cpu,
         "rel_err_vs_reference": {"stp": err_stp, "pts": err_pts},
         "round_trip_rel_err": r["round_trip_rel_err"],

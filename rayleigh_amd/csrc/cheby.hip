@@ -18,78 +18,157 @@
 namespace raysht {
 namespace {
 
-constexpr int kChebThreads = 256;
-constexpr int kChebMaxN = 512;
-constexpr int kChebCols = 8;  // columns per workgroup
+typedef double double4_t __attribute__((ext_vector_type(4)));
 
-// dynamic LDS: kChebCols columns of n doubles for each folded half
-__global__ __launch_bounds__(kChebThreads) void cheb_to_spectral_kernel(int n, int n_x, int n_even, int n_odd,
-                                                                       long ncols, const double *__restrict__ ce,
-                                                                       const double *__restrict__ co,
-                                                                       const double *__restrict__ f,
-                                                                       double *__restrict__ c)
+__device__ __forceinline__ double4_t mfma_f64(double a, double b, double4_t c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int kChebMaxN = 512;
+constexpr int kChebKB = 4;  // 16-row output blocks per accumulator group
+
+// Both contractions are small dense products per column (K <= n/2) over a huge
+// number of columns, so a workgroup stages a block of columns in LDS (folded or
+// de-interleaved on the way in) and multiplies it with the parity tables on FP64
+// MFMA (v_mfma_f64_16x16x4_f64; result rows interleaved by 4: row = lane/16 + 4v).
+// The products are taken transposed (columns on the MFMA row axis) so each store
+// instruction writes 16 consecutive radii/coefficients of a column.
+// Each wave owns 16 columns; tc = 16 x waves columns per workgroup.
+
+// To_Spectral4D: c(2k) = (2/n) sum_i T_2k(x_i) fe(i), c(2k+1) = (2/n) sum_i T_2k+1(x_i) fo(i).
+// Tables tt_e/tt_o: [i][k] (k fastest), i < ix4 (n_x rounded up to 4), k < kp (rounded up to 16).
+__global__ __launch_bounds__(256) void cheb_to_spectral_kernel(int n, int n_x, int n_even, int n_odd, int ix4, int kp,
+                                                               long ncols, const double *__restrict__ tt_e,
+                                                               const double *__restrict__ tt_o,
+                                                               const double *__restrict__ f, double *__restrict__ c)
 {
     extern __shared__ double lds[];
-    double *fe = lds;                   // [kChebCols][n_x]
-    double *fo = lds + kChebCols * n_x;  // [kChebCols][n_x]
-    const long col0 = (long)blockIdx.x * kChebCols;
-    const int ncol = (int)((ncols - col0) < kChebCols ? (ncols - col0) : kChebCols);
-    for (int idx = threadIdx.x; idx < ncol * n_x; idx += blockDim.x) {
-        const int cc = idx / n_x, i = idx % n_x;
-        const double *fc = f + (col0 + cc) * n;
-        const double a = fc[i], b = fc[n - 1 - i];
-        fe[cc * n_x + i] = a + b;
-        fo[cc * n_x + i] = a - b;
+    const int tc = blockDim.x / 4;  // 16 columns per wave
+    const int ld = ix4 + 1;
+    double *fe = lds, *fo = lds + tc * ld;
+    const long col0 = (long)blockIdx.x * tc;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+    for (int idx = tid; idx < tc * ix4; idx += blockDim.x) {
+        const int cc = idx / ix4, i = idx % ix4;
+        const long col = col0 + cc;
+        double e = 0.0, o = 0.0;
+        if (col < ncols && i < n_x) {
+            const double a = f[col * n + i], b = f[col * n + (n - 1 - i)];
+            e = a + b;
+            o = a - b;
+        }
+        fe[cc * ld + i] = e;
+        fo[cc * ld + i] = o;
     }
     __syncthreads();
     const double alpha = 2.0 / (double)n;
-    for (int idx = threadIdx.x; idx < ncol * n; idx += blockDim.x) {
-        const int cc = idx / n, q = idx % n;
-        const int k = q >> 1;
-        double acc = 0.0;
-        if ((q & 1) == 0) {
-            const double *t = ce + (long)k * n_x;
-            const double *v = fe + cc * n_x;
-            for (int i = 0; i < n_x; ++i) acc = __builtin_fma(t[i], v[i], acc);
-        } else {
-            const double *t = co + (long)k * n_x;
-            const double *v = fo + cc * n_x;
-            for (int i = 0; i < n_x; ++i) acc = __builtin_fma(t[i], v[i], acc);
+    for (int kb0 = 0; kb0 < kp / 16; kb0 += kChebKB) {
+        double4_t acc_e[kChebKB], acc_o[kChebKB];
+#pragma unroll
+        for (int b = 0; b < kChebKB; ++b) {
+            acc_e[b] = double4_t{0.0, 0.0, 0.0, 0.0};
+            acc_o[b] = double4_t{0.0, 0.0, 0.0, 0.0};
         }
-        c[(col0 + cc) * n + q] = alpha * acc;
+        const int nb = (kp / 16 - kb0) < kChebKB ? (kp / 16 - kb0) : kChebKB;
+        for (int ii = 0; ii < ix4; ii += 4) {
+            const double a_e = fe[(16 * wave + li) * ld + ii + lg];
+            const double a_o = fo[(16 * wave + li) * ld + ii + lg];
+            const double *te = tt_e + (long)(ii + lg) * kp + 16 * kb0 + li;
+            const double *to = tt_o + (long)(ii + lg) * kp + 16 * kb0 + li;
+#pragma unroll
+            for (int b = 0; b < kChebKB; ++b) {
+                if (b < nb) {
+                    acc_e[b] = mfma_f64(a_e, te[16 * b], acc_e[b]);
+                    acc_o[b] = mfma_f64(a_o, to[16 * b], acc_o[b]);
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kChebKB; ++b) {
+            if (b >= nb) break;
+            const int k = 16 * (kb0 + b) + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long col = col0 + 16 * wave + lg + 4 * v;
+                if (col < ncols) {
+                    if (k < n_even) c[col * n + 2 * k] = alpha * acc_e[b][v];
+                    if (k < n_odd) c[col * n + 2 * k + 1] = alpha * acc_o[b][v];
+                }
+            }
+        }
     }
 }
 
-__global__ __launch_bounds__(kChebThreads) void cheb_from_spectral_kernel(int n, int n_x, int n_even, int n_odd,
-                                                                         long ncols, const double *__restrict__ ce,
-                                                                         const double *__restrict__ co,
-                                                                         const double *__restrict__ c,
-                                                                         double *__restrict__ f)
+// From_Spectral4D: E(i) = sum_k T_2k(x_i) c(2k), O(i) = sum_k T_2k+1(x_i) c(2k+1);
+// f(i) = E + O, f(n-1-i) = E - O, the x = 0 row of odd n doubled back, then - c(0)/2.
+// Tables t_e/t_o: [k][i] (i fastest), k < k4 (rounded up to 4), i < ip (n_x rounded up to 16).
+__global__ __launch_bounds__(256) void cheb_from_spectral_kernel(int n, int n_x, int n_even, int n_odd, int k4, int ip,
+                                                                 long ncols, const double *__restrict__ t_e,
+                                                                 const double *__restrict__ t_o,
+                                                                 const double *__restrict__ c, double *__restrict__ f)
 {
     extern __shared__ double lds[];
-    double *cs = lds;  // [kChebCols][n] coefficient columns
-    const long col0 = (long)blockIdx.x * kChebCols;
-    const int ncol = (int)((ncols - col0) < kChebCols ? (ncols - col0) : kChebCols);
-    for (int idx = threadIdx.x; idx < ncol * n; idx += blockDim.x) cs[idx] = c[col0 * n + idx];
+    const int tc = blockDim.x / 4;
+    const int ld = k4 + 1;
+    double *ce = lds, *co = lds + tc * ld;
+    const long col0 = (long)blockIdx.x * tc;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lg = lane >> 4;
+    for (int idx = tid; idx < tc * 2 * k4; idx += blockDim.x) {
+        const int cc = idx / (2 * k4), q = idx % (2 * k4);
+        const long col = col0 + cc;
+        const int k = q >> 1;
+        double v = 0.0;
+        if (col < ncols && q < n) v = c[col * n + q];
+        if (q & 1)
+            co[cc * ld + k] = v;
+        else
+            ce[cc * ld + k] = v;
+    }
     __syncthreads();
-    const bool odd_n = (n_even != n_odd);
-    for (int idx = threadIdx.x; idx < ncol * n; idx += blockDim.x) {
-        const int cc = idx / n, r = idx % n;
-        const double *cv = cs + cc * n;
-        // node i of the half grid this output row mirrors, and the sign of the odd part
-        const bool upper = r >= n_x;
-        const int i = upper ? n - 1 - r : r;
-        double ev = 0.0, od = 0.0;
-        for (int k = 0; k < n_even; ++k) ev = __builtin_fma(ce[(long)k * n_x + i], cv[2 * k], ev);
-        for (int k = 0; k < n_odd; ++k) od = __builtin_fma(co[(long)k * n_x + i], cv[2 * k + 1], od);
-        double v;
-        if (odd_n && i == n_x - 1) {
-            // the x = 0 row: halved in the tables, doubled back; the reference adds the odd product twice
-            v = ev * 2.0 + od * 2.0;
-        } else {
-            v = upper ? ev - od : ev + od;
+    const bool odd_n = n_even != n_odd;
+    for (int ib0 = 0; ib0 < ip / 16; ib0 += kChebKB) {
+        double4_t acc_e[kChebKB], acc_o[kChebKB];
+#pragma unroll
+        for (int b = 0; b < kChebKB; ++b) {
+            acc_e[b] = double4_t{0.0, 0.0, 0.0, 0.0};
+            acc_o[b] = double4_t{0.0, 0.0, 0.0, 0.0};
         }
-        f[(col0 + cc) * n + r] = v - cv[0] / 2.0;
+        const int nb = (ip / 16 - ib0) < kChebKB ? (ip / 16 - ib0) : kChebKB;
+        for (int kk = 0; kk < k4; kk += 4) {
+            const double a_e = ce[(16 * wave + li) * ld + kk + lg];
+            const double a_o = co[(16 * wave + li) * ld + kk + lg];
+            const double *te = t_e + (long)(kk + lg) * ip + 16 * ib0 + li;
+            const double *to = t_o + (long)(kk + lg) * ip + 16 * ib0 + li;
+#pragma unroll
+            for (int b = 0; b < kChebKB; ++b) {
+                if (b < nb) {
+                    acc_e[b] = mfma_f64(a_e, te[16 * b], acc_e[b]);
+                    acc_o[b] = mfma_f64(a_o, to[16 * b], acc_o[b]);
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kChebKB; ++b) {
+            if (b >= nb) break;
+            const int i = 16 * (ib0 + b) + li;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int cc = 16 * wave + lg + 4 * v;
+                const long col = col0 + cc;
+                if (col < ncols && i < n_x) {
+                    const double e = acc_e[b][v], o = acc_o[b][v];
+                    const double c0h = ce[cc * ld] / 2.0;
+                    double *fc = f + col * n;
+                    if (odd_n && i == n_x - 1) {
+                        fc[i] = (e * 2.0 + o * 2.0) - c0h;
+                    } else {
+                        fc[i] = (e + o) - c0h;
+                        fc[n - 1 - i] = (e - o) - c0h;
+                    }
+                }
+            }
+        }
     }
 }
 
@@ -152,17 +231,38 @@ int raysht_cheb_create(raysht_cheb **out, int n, double rmin, double rmax, doubl
         for (int k = 0; k < cb->n_even; ++k) cb->ceven[(size_t)k * cb->n_x + cb->n_x - 1] *= 0.5;
         for (int k = 0; k < cb->n_odd; ++k) cb->codd[(size_t)k * cb->n_x + cb->n_x - 1] *= 0.5;
     }
-    if (hipMalloc(&cb->d_ceven, cb->ceven.size() * sizeof(double)) != hipSuccess ||
-        hipMalloc(&cb->d_codd, cb->codd.size() * sizeof(double)) != hipSuccess) {
-        raysht_cheb_destroy(cb);
-        return RAYSHT_ENOMEM;
-    }
-    if (hipMemcpy(cb->d_ceven, cb->ceven.data(), cb->ceven.size() * sizeof(double), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(cb->d_codd, cb->codd.data(), cb->codd.size() * sizeof(double), hipMemcpyHostToDevice) !=
-            hipSuccess) {
-        raysht_cheb_destroy(cb);
-        return RAYSHT_EHIP;
+    auto up = [](int v, int m) { return (v + m - 1) / m * m; };
+    cb->ix4 = up(cb->n_x, 4);
+    cb->kp = up(cb->n_even, 16);
+    cb->k4 = up(cb->n_even, 4);
+    cb->ip = up(cb->n_x, 16);
+    std::vector<double> tt_e((size_t)cb->ix4 * cb->kp, 0.0), tt_o((size_t)cb->ix4 * cb->kp, 0.0);
+    std::vector<double> t_e((size_t)cb->k4 * cb->ip, 0.0), t_o((size_t)cb->k4 * cb->ip, 0.0);
+    for (int k = 0; k < cb->n_even; ++k)
+        for (int i = 0; i < cb->n_x; ++i) {
+            tt_e[(size_t)i * cb->kp + k] = cb->ceven[(size_t)k * cb->n_x + i];
+            t_e[(size_t)k * cb->ip + i] = cb->ceven[(size_t)k * cb->n_x + i];
+        }
+    for (int k = 0; k < cb->n_odd; ++k)
+        for (int i = 0; i < cb->n_x; ++i) {
+            tt_o[(size_t)i * cb->kp + k] = cb->codd[(size_t)k * cb->n_x + i];
+            t_o[(size_t)k * cb->ip + i] = cb->codd[(size_t)k * cb->n_x + i];
+        }
+    struct up_t {
+        double **dst;
+        const std::vector<double> *src;
+    } ups[4] = {{&cb->d_tt_e, &tt_e}, {&cb->d_tt_o, &tt_o}, {&cb->d_t_e, &t_e}, {&cb->d_t_o, &t_o}};
+    for (auto &u : ups) {
+        const size_t bytes = u.src->size() * sizeof(double);
+        if (hipMalloc(u.dst, bytes) != hipSuccess) {
+            *u.dst = nullptr;
+            raysht_cheb_destroy(cb);
+            return RAYSHT_ENOMEM;
+        }
+        if (hipMemcpy(*u.dst, u.src->data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            raysht_cheb_destroy(cb);
+            return RAYSHT_EHIP;
+        }
     }
     *out = cb;
     return RAYSHT_OK;
@@ -171,8 +271,8 @@ int raysht_cheb_create(raysht_cheb **out, int n, double rmin, double rmax, doubl
 void raysht_cheb_destroy(raysht_cheb *cb)
 {
     if (!cb) return;
-    if (cb->d_ceven) (void)hipFree(cb->d_ceven);
-    if (cb->d_codd) (void)hipFree(cb->d_codd);
+    for (double *d : {cb->d_tt_e, cb->d_tt_o, cb->d_t_e, cb->d_t_o})
+        if (d) (void)hipFree(d);
     delete cb;
 }
 
@@ -185,16 +285,22 @@ int raysht_cheb_table(const raysht_cheb *cb, int which, double *host_out)
     return RAYSHT_OK;
 }
 
+namespace {
+// 16 columns per wave; fewer waves per workgroup for long columns to bound the LDS tile
+int cheb_waves(int n) { return n <= 128 ? 4 : (n <= 256 ? 2 : 1); }
+}  // namespace
+
 int raysht_cheb_to_spectral(raysht_cheb *cb, const double *f, double *c, long ncols, void *stream)
 {
     if (!cb || ncols < 0 || (ncols > 0 && (!f || !c))) return RAYSHT_EINVAL;
     if (ncols == 0) return RAYSHT_OK;
-    const long blocks = (ncols + kChebCols - 1) / kChebCols;
+    const int waves = cheb_waves(cb->n), tc = 16 * waves;
+    const long blocks = (ncols + tc - 1) / tc;
     if (blocks > 0x7fffffffL) return RAYSHT_EINVAL;
-    const size_t lds = sizeof(double) * 2 * kChebCols * cb->n_x;
-    hipLaunchKernelGGL(cheb_to_spectral_kernel, dim3((unsigned)blocks), dim3(kChebThreads), lds,
-                       reinterpret_cast<hipStream_t>(stream), cb->n, cb->n_x, cb->n_even, cb->n_odd, ncols,
-                       cb->d_ceven, cb->d_codd, f, c);
+    const size_t lds = sizeof(double) * 2 * tc * (cb->ix4 + 1);
+    hipLaunchKernelGGL(cheb_to_spectral_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds,
+                       reinterpret_cast<hipStream_t>(stream), cb->n, cb->n_x, cb->n_even, cb->n_odd, cb->ix4, cb->kp,
+                       ncols, cb->d_tt_e, cb->d_tt_o, f, c);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
@@ -203,12 +309,13 @@ int raysht_cheb_from_spectral(raysht_cheb *cb, const double *c, double *f, long 
 {
     if (!cb || ncols < 0 || (ncols > 0 && (!f || !c))) return RAYSHT_EINVAL;
     if (ncols == 0) return RAYSHT_OK;
-    const long blocks = (ncols + kChebCols - 1) / kChebCols;
+    const int waves = cheb_waves(cb->n), tc = 16 * waves;
+    const long blocks = (ncols + tc - 1) / tc;
     if (blocks > 0x7fffffffL) return RAYSHT_EINVAL;
-    const size_t lds = sizeof(double) * kChebCols * cb->n;
-    hipLaunchKernelGGL(cheb_from_spectral_kernel, dim3((unsigned)blocks), dim3(kChebThreads), lds,
-                       reinterpret_cast<hipStream_t>(stream), cb->n, cb->n_x, cb->n_even, cb->n_odd, ncols,
-                       cb->d_ceven, cb->d_codd, c, f);
+    const size_t lds = sizeof(double) * 2 * tc * (cb->k4 + 1);
+    hipLaunchKernelGGL(cheb_from_spectral_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds,
+                       reinterpret_cast<hipStream_t>(stream), cb->n, cb->n_x, cb->n_even, cb->n_odd, cb->k4, cb->ip,
+                       ncols, cb->d_t_e, cb->d_t_o, c, f);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }

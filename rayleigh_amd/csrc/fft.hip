@@ -59,6 +59,41 @@ __device__ __forceinline__ void stockham_butterfly(const cplx *sl, cplx *dl, int
     if constexpr (R == 2) {
         y[0] = cadd(v[0], v[1]);
         y[1] = csub(v[0], v[1]);
+    } else if constexpr (R == 8) {
+        // two radix-4 DFTs on the even / odd inputs, combined with W8^k = e^{sign 2 pi i k / 8}
+        cplx e[4], o[4];
+        {
+            const cplx a = cadd(v[0], v[4]), b = csub(v[0], v[4]);
+            const cplx c = cadd(v[2], v[6]), d0 = csub(v[2], v[6]);
+            const cplx d = {-sign * d0.im, sign * d0.re};
+            e[0] = cadd(a, c);
+            e[1] = cadd(b, d);
+            e[2] = csub(a, c);
+            e[3] = csub(b, d);
+        }
+        {
+            const cplx a = cadd(v[1], v[5]), b = csub(v[1], v[5]);
+            const cplx c = cadd(v[3], v[7]), d0 = csub(v[3], v[7]);
+            const cplx d = {-sign * d0.im, sign * d0.re};
+            o[0] = cadd(a, c);
+            o[1] = cadd(b, d);
+            o[2] = csub(a, c);
+            o[3] = csub(b, d);
+        }
+        const double r2 = 0.7071067811865475244008443621048490392848;  // 1/sqrt(2)
+        const cplx w1 = {r2, sign * r2}, w3 = {-r2, sign * r2};
+        const cplx t0 = o[0];
+        const cplx t1 = cmul(o[1], w1);
+        const cplx t2 = {-sign * o[2].im, sign * o[2].re};
+        const cplx t3 = cmul(o[3], w3);
+        y[0] = cadd(e[0], t0);
+        y[1] = cadd(e[1], t1);
+        y[2] = cadd(e[2], t2);
+        y[3] = cadd(e[3], t3);
+        y[4] = csub(e[0], t0);
+        y[5] = csub(e[1], t1);
+        y[6] = csub(e[2], t2);
+        y[7] = csub(e[3], t3);
     } else if constexpr (R == 4) {
         const cplx a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
         const cplx c = cadd(v[1], v[3]), d0 = csub(v[1], v[3]);
@@ -95,7 +130,7 @@ __device__ __forceinline__ void stockham_stage(const cplx *src, cplx *dst, int n
 }
 
 // `lines` independent length-n2 complex DFTs of buf[line][n2] (LDS); the radix of stage
-// s is 2 bits of `code` (0: 2, 1: 4, 2: 3), product n2; sign = +1 (inverse,
+// s is 2 bits of `code` (0: 2, 1: 4, 2: 3, 3: 8), product n2; sign = +1 (inverse,
 // unnormalised) or -1 (forward).  Stockham auto-sort, ping-ponging with tmp.  Returns
 // the buffer that holds the result.
 __device__ cplx *stockham(cplx *buf, cplx *tmp, int n2, int lines, int nphi, int code, int nrad,
@@ -106,7 +141,10 @@ __device__ cplx *stockham(cplx *buf, cplx *tmp, int n2, int lines, int nphi, int
     for (int s = 0; s < nrad; ++s) {
         const int c = (code >> (2 * s)) & 3;
         int R;
-        if (c == 1) {
+        if (c == 3) {
+            stockham_stage<8>(src, dst, n2, lines, ns, nphi, tw, sign);
+            R = 8;
+        } else if (c == 1) {
             stockham_stage<4>(src, dst, n2, lines, ns, nphi, tw, sign);
             R = 4;
         } else if (c == 0) {
@@ -139,10 +177,11 @@ __global__ __launch_bounds__(kFftThreads) void fft_c2r_kernel(int nphi, long how
         // Im X_0 and Im X_{n2} do not enter a c2r.
         for (int kk = threadIdx.x; kk < nl * n2; kk += blockDim.x) {
             const int line = kk / n2, k = kk - line * n2;
-            const double *xl = x + (l0 + line) * ld;
-            cplx a = {xl[2 * k], k == 0 ? 0.0 : xl[2 * k + 1]};
+            const double2 *xl = reinterpret_cast<const double2 *>(x + (l0 + line) * ld);
+            const double2 xa = xl[k], xb = xl[n2 - k];
             const int km = n2 - k;
-            cplx b = {xl[2 * km], km == n2 ? 0.0 : xl[2 * km + 1]};
+            cplx a = {xa.x, k == 0 ? 0.0 : xa.y};
+            cplx b = {xb.x, km == n2 ? 0.0 : xb.y};
             const cplx bc = cconj(b);
             const cplx A = cadd(a, bc);
             const cplx B = cmul(csub(a, bc), twiddle(tw, nphi, k, +1));
@@ -152,10 +191,9 @@ __global__ __launch_bounds__(kFftThreads) void fft_c2r_kernel(int nphi, long how
         cplx *res = stockham(bufa, bufb, n2, nl, nphi, code, nrad, tw, +1);
         for (int kk = threadIdx.x; kk < nl * n2; kk += blockDim.x) {
             const int line = kk / n2, k = kk - line * n2;
-            double *xl = x + (l0 + line) * ld;
+            double2 *xl = reinterpret_cast<double2 *>(x + (l0 + line) * ld);
             const cplx z = res[line * n2 + k];
-            xl[2 * k] = z.re;
-            xl[2 * k + 1] = z.im;
+            xl[k] = make_double2(z.re, z.im);
         }
         for (int line = threadIdx.x; line < nl; line += blockDim.x) {
             x[(l0 + line) * ld + nphi] = 0.0;
@@ -177,8 +215,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_r2c_kernel(int nphi, long how
         const int nl = (int)((howmany - l0) < lines ? (howmany - l0) : lines);
         for (int kk = threadIdx.x; kk < nl * n2; kk += blockDim.x) {
             const int line = kk / n2, k = kk - line * n2;
-            const double *xl = x + (l0 + line) * ld;
-            bufa[line * n2 + k] = {xl[2 * k], xl[2 * k + 1]};
+            const double2 v = reinterpret_cast<const double2 *>(x + (l0 + line) * ld)[k];
+            bufa[line * n2 + k] = {v.x, v.y};
         }
         __syncthreads();
         cplx *res = stockham(bufa, bufb, n2, nl, nphi, code, nrad, tw, -1);
@@ -197,9 +235,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_r2c_kernel(int nphi, long how
                 re = (k == 0) ? re / (2.0 * (double)nphi) : re / (double)nphi;
                 if (k > 0) im = im / (double)nphi;
             }
-            double *xl = x + (l0 + line) * ld;
-            xl[2 * k] = re;
-            xl[2 * k + 1] = im;
+            reinterpret_cast<double2 *>(x + (l0 + line) * ld)[k] = make_double2(re, im);
         }
         __syncthreads();
     }
@@ -292,8 +328,9 @@ int get_twiddles(int nphi, const double2 **out)
     return RAYSHT_OK;
 }
 
-// radix schedule for n2 = 2^a 3^b: radix-4 stages, one radix-2 if a is odd, then
-// the radix-3 stages; packed 2 bits per stage (0: 2, 1: 4, 2: 3).
+// radix schedule for n2 = 2^a 3^b: radix-8 stages, then one radix-4 or radix-2 for
+// the remaining power of two, then the radix-3 stages; packed 2 bits per stage
+// (0: 2, 1: 4, 2: 3, 3: 8).
 bool fft_schedule(int n2, int *code, int *nrad)
 {
     int a = 0, b = 0, n = n2;
@@ -307,10 +344,11 @@ bool fft_schedule(int n2, int *code, int *nrad)
     }
     if (n != 1) return false;
     int c = 0, s = 0;
-    for (int i = 0; i < a / 2; ++i) c |= 1 << (2 * s++);
-    if (a & 1) c |= 0 << (2 * s++);
+    for (int i = 0; i < a / 3; ++i) c |= 3 << (2 * s++);
+    if (a % 3 == 2) c |= 1 << (2 * s++);
+    if (a % 3 == 1) c |= 0 << (2 * s++);
     for (int i = 0; i < b; ++i) c |= 2 << (2 * s++);
-    if (s > 16) return false;
+    if (s > 15) return false;
     *code = c;
     *nrad = s;
     return true;

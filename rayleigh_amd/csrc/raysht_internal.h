@@ -60,8 +60,11 @@ struct raysht_plan {
 struct raysht_cheb {
     int device = 0;
     int n = 0, n_even = 0, n_odd = 0, n_x = 0;
-    std::vector<double> ceven, codd;  // [k][i] i fastest (n_x)
-    double *d_ceven = nullptr, *d_codd = nullptr;
+    std::vector<double> ceven, codd;  // [k][i] i fastest (n_x), as Gen_Tn stores them
+    // device copies, zero-padded for the MFMA tiles: [i][k] for the analysis
+    // (ix4 x kp) and [k][i] for the synthesis (k4 x ip)
+    int ix4 = 0, kp = 0, k4 = 0, ip = 0;
+    double *d_tt_e = nullptr, *d_tt_o = nullptr, *d_t_e = nullptr, *d_t_o = nullptr;
 };
 
 namespace raysht {

@@ -51,7 +51,8 @@ int n_odd(int lmax, int m) { return (lmax - m + 1) / 2; }
 // of one m-block, which share its P_lm table and its input columns, run on one XCD
 // and hit in its L2; within an XCD the largest contractions come first.  The list
 // is interleaved so that workgroup b runs item b/8 of XCD b%8, padded with empty
-// items (mp = -1) where XCD queues differ in length.
+// items (mp = -1) where XCD queues differ in length.  Inside an m-block the row tiles
+// of one column tile are consecutive, so they find the column tile's input in L2.
 constexpr int kXcds = 8;
 
 std::vector<work_item> make_work(const raysht_plan *p, bool stp)
@@ -71,8 +72,10 @@ std::vector<work_item> make_work(const raysht_plan *p, bool stp)
         // cost of one item ~ its K loop: parity-l chunks for StP, node chunks for PtS
         const double item = stp ? (double)ne : (double)p->nt2;
         const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        for (int r = 0; r < nrt; ++r)
-            for (int c = 0; c < nct; ++c) q[x].push_back(work_item{mp, r * tr, c * tc, 0});
+        // column tile outer, row tile inner: consecutive items share the column tile's
+        // input (spectral columns for StP, hybrid columns for PtS) while it is in L2
+        for (int c = 0; c < nct; ++c)
+            for (int r = 0; r < nrt; ++r) q[x].push_back(work_item{mp, r * tr, c * tc, 0});
         load[x] += item * nrt * nct;
     }
     size_t len = 0;
