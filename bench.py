@@ -6,10 +6,12 @@ config-0 distribution (N(0,1) (l+1)^-2 sin(pi r_i / N_r), m = 0 real), synthetic
 resident in HBM before the timed region.  One step = StP_4d_dgp2 + PtS_4d_dgpv3
 over every m (the HIP library's grouped launches).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,bf16x6]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--forms fp64,bf16x6,bf16x3]
 
 Prints one JSON line.  value = FP64-equivalent algorithmic TFLOP/s of the round
-trip, 2 * F_LT / step time with F_LT = N_theta N_r N_f (L+1)(L+2).
+trip of the first form, 2 * F_LT / step time with F_LT = N_theta N_r N_f (L+1)(L+2).
+Every split form is also timed and its max / RMS / per-field error against the FP64
+path on the same input is reported under "forms".
 """
 from __future__ import annotations
 
@@ -28,6 +30,8 @@ LMAX, NTHETA, NR, NF = 511, 768, 192, 8
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 matrix, dense
 PEAK_BF16_TFLOPS = 2500.0    # MI355X BF16 matrix, dense
 METRIC = "SHT round-trip ms + TFLOP/s at l_max=511: FP64 vs split-BF16 at 1e-12 rel err"
+# BF16 slice pairs (one MFMA product each) per direction of the split forms
+SPLIT_PAIRS = {"bf16x6": {"stp": 27, "pts": 21}, "bf16x3": {"stp": 6, "pts": 6}}
 
 
 def log(msg: str) -> None:
@@ -94,6 +98,45 @@ def time_form(ra, form: str, mvals, steps: int, warmup: int):
     log(f"form {form}: {ms_per_step:.3f} ms/step (StP {stp_ms:.3f} ms, PtS {pts_ms:.3f} ms), round trip {rt:.2e}")
     return dict(plan=plan, s=s, p2=p2, s2=s2, setup_s=setup_s, ms_per_step=ms_per_step, stp_ms=stp_ms,
                 pts_ms=pts_ms, round_trip_rel_err=rt)
+
+
+def errors_vs_fp64(ra, res, ref, mvals):
+    """Max and RMS relative error of a split form against the FP64 path on the same
+    input, for StP (same spectral input) and PtS (run on the FP64 hybrid output), over
+    the whole output and per field."""
+    import torch
+    plan = res["plan"]
+    p_form = res["p2"]
+    s_form = torch.empty_like(res["s2"])
+    plan.pts(ref["p2"], s_form)
+    torch.cuda.synchronize()
+
+    def rel(a, b):
+        d = (a - b)
+        return float(d.abs().max() / b.abs().max()), float(torch.sqrt((d * d).mean() / (b * b).mean()))
+
+    stp_max, stp_rms = rel(p_form, ref["p2"])
+    pts_max, pts_rms = rel(s_form, ref["s2"])
+    # per field: hybrid block mp holds columns c = r + imi*N_r + 2*N_r*f; spectral block
+    # mp is data(m:l_max, N_r, 2, N_f)
+    nm = len(mvals)
+    dp = (p_form - ref["p2"]).view(nm, NF, 2 * NR * NTHETA).abs().amax(dim=(0, 2))
+    rp = ref["p2"].view(nm, NF, 2 * NR * NTHETA).abs().amax(dim=(0, 2))
+    ds_f = torch.zeros(NF, dtype=torch.float64, device="cuda")
+    rs_f = torch.zeros(NF, dtype=torch.float64, device="cuda")
+    off = 0
+    for m in mvals:
+        n = (LMAX - int(m) + 1) * 2 * NR * NF
+        blk_d = (s_form[off:off + n] - ref["s2"][off:off + n]).view(NF, -1).abs().amax(dim=1)
+        blk_r = ref["s2"][off:off + n].view(NF, -1).abs().amax(dim=1)
+        ds_f = torch.maximum(ds_f, blk_d)
+        rs_f = torch.maximum(rs_f, blk_r)
+        off += n
+    field_stp = (dp / rp).cpu().numpy().tolist()
+    field_pts = (ds_f / rs_f).cpu().numpy().tolist()
+    del s_form
+    return {"stp_max": stp_max, "stp_rms": stp_rms, "pts_max": pts_max, "pts_rms": pts_rms,
+            "stp_max_per_field": field_stp, "pts_max_per_field": field_pts}
 
 
 def reference_check_and_baseline(res, mvals, sample_m: int):
@@ -232,7 +275,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--forms", default="fp64")
+    ap.add_argument("--forms", default="fp64,bf16x6,bf16x3",
+                    help="Legendre forms to run; the first is the headline, fp64 is always run as the error reference")
     ap.add_argument("--cpu-sample-m", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="legendre", choices=["legendre", "stack"],
@@ -256,9 +300,19 @@ def main():
     mvals = O.m_order_balanced(LMAX)
     F = flops_per_direction()
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    run_order = ["fp64"] + [f for f in forms if f != "fp64"]
     results = {}
-    for form in forms:
+    for form in run_order:
         results[form] = time_form(ra, form, mvals, args.steps, args.warmup)
+        if form != "fp64":
+            results[form]["err_vs_fp64"] = errors_vs_fp64(ra, results[form], results["fp64"], mvals)
+            e = results[form]["err_vs_fp64"]
+            log(f"form {form}: vs FP64 StP max {e['stp_max']:.2e} rms {e['stp_rms']:.2e}, "
+                f"PtS max {e['pts_max']:.2e} rms {e['pts_rms']:.2e}")
+            # keep only what the report needs
+            for key in ("plan", "s", "p2", "s2"):
+                results[form].pop(key, None)
+            torch.cuda.empty_cache()
     if args.copy_calibration:
         a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
         b = torch.empty_like(a)
@@ -271,11 +325,15 @@ def main():
     r = results[head]
     cpu, err_stp, err_pts = (None, None, None)
     if not args.no_cpu_baseline:
-        cpu, err_stp, err_pts = reference_check_and_baseline(r, mvals, args.cpu_sample_m)
+        cpu, err_stp, err_pts = reference_check_and_baseline(results["fp64"], mvals, args.cpu_sample_m)
     dom_ms = max(r["stp_ms"], r["pts_ms"])
     dom = "stp" if r["stp_ms"] >= r["pts_ms"] else "pts"
-    peak = PEAK_FP64_TFLOPS
-    achieved = F / (dom_ms * 1e-3) / 1e12
+    if head == "fp64":
+        peak, kernel, achieved = PEAK_FP64_TFLOPS, f"{dom}_fp64_kernel", F / (dom_ms * 1e-3) / 1e12
+    else:
+        # BF16 MFMA work actually issued: one F-sized product per kept slice pair
+        pairs = SPLIT_PAIRS[head][dom]
+        peak, kernel, achieved = PEAK_BF16_TFLOPS, f"{dom}_split_kernel", pairs * F / (dom_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
         "value": 2 * F / (r["ms_per_step"] * 1e-3) / 1e12,
@@ -293,14 +351,15 @@ def main():
                                "all m (balanced order)", "l_max": LMAX, "n_theta": NTHETA, "n_r": NR,
                    "n_fields": NF, "form": head},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": measured_traffic(f"{dom}_fp64_kernel"),
-                     "kernel": f"{dom}_fp64_kernel", "kernel_ms": dom_ms},
+                     "frac": achieved / peak, "traffic": measured_traffic(kernel),
+                     "kernel": kernel, "kernel_ms": dom_ms},
         "cpu_baseline": cpu,
         "rel_err_vs_reference": {"stp": err_stp, "pts": err_pts},
         "round_trip_rel_err": r["round_trip_rel_err"],
         "forms": {f: {"ms_per_step": v["ms_per_step"], "stp_ms": v["stp_ms"], "pts_ms": v["pts_ms"],
                       "tflops": 2 * F / (v["ms_per_step"] * 1e-3) / 1e12,
-                      "round_trip_rel_err": v["round_trip_rel_err"], "setup_s": v["setup_s"]}
+                      "round_trip_rel_err": v["round_trip_rel_err"], "setup_s": v["setup_s"],
+                      "err_vs_fp64": v.get("err_vs_fp64")}
                   for f, v in results.items()},
     }
     print(json.dumps(out), flush=True)

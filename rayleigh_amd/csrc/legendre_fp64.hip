@@ -128,7 +128,6 @@ __global__ __launch_bounds__(STP_THREADS, RAYSHT_FP64_MIN_WAVES) void stp_fp64_k
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
-    const int stride = gridDim.x;
     // Per-thread load geometry.  B: column bcc0 + (STP_THREADS / 2KC) i, l = 2 k0 + bq;
     // A: parity-l row akk0 + (STP_THREADS / 64) i, node ajj.
     const int bq = tid % (2 * STP_KC), bcc0 = tid / (2 * STP_KC);
@@ -289,7 +288,6 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
-    const int stride = gridDim.x;
     // Per-thread load geometry.  F: column fcc0 + (PTS_THREADS / KJ) i, node j0 + fjj and
     // its mirror; I: parity-l row fcc0 + (PTS_THREADS / KJ) i, node j0 + fjj.
     const int fjj = tid % PTS_KJ, fcc0 = tid / PTS_KJ;

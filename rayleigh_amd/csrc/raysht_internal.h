@@ -35,6 +35,20 @@ struct work_item {
     int pad;
 };
 
+// Device state of the split-precision (BF16 slice) forms: the tables pre-split into
+// 1 KiB MFMA B-operand fragments (64 lanes x 8 BF16), the power-of-two scale exponent
+// of every table row, and the work lists of the 64 x 128 tiles.
+struct raysht_split {
+    long *d_frag_stp_off = nullptr;  // first StP fragment of block mp (even, then odd region)
+    long *d_frag_pts_off = nullptr;  // first PtS fragment of block mp (even, then odd region)
+    long *d_exp_pts_off = nullptr;   // PtS row exponents of block mp: even rows, then odd rows
+    work_item *d_work_stp = nullptr, *d_work_pts = nullptr;
+    int n_work_stp = 0, n_work_pts = 0;
+    uint4 *d_tdig_stp = nullptr, *d_tdig_pts = nullptr;
+    int *d_texp_stp = nullptr;  // [mp][parity][j]
+    int *d_texp_pts = nullptr;
+};
+
 }  // namespace raysht
 
 struct raysht_plan {
@@ -55,6 +69,7 @@ struct raysht_plan {
     int n_work_stp = 0, n_work_pts = 0;
     // staging buffers of the host-array entry points, allocated on first use
     double *d_stage_s = nullptr, *d_stage_p = nullptr;
+    raysht::raysht_split split;  // used by the BF16X3 / BF16X6 forms only
 };
 
 struct raysht_cheb {
