@@ -6,16 +6,19 @@
 // i.e. 8-bit "slices", each exactly representable in BF16.  A slice product p_s q_t is
 // exact in FP32 and every product of one level L = s + t is an integer multiple of
 // 2^(-14-8L), so FP32 sums of one level are exact while the integer count stays below
-// 2^24: with level sums moved into FP64 every 256 terms of K (kChunkK), levels 0, 1
-// and 2 (1, 2 and 3 slice pairs) are summed exactly; all levels >= 3 (weights
-// <= 2^-24 of level 0) share a fourth FP32 sum.  The FP64 recombination adds the four
-// sums and scales by 2^(e_row + e_col).
+// 2^24: with level sums moved into FP64 every 256 terms of K (kChunkK), every level
+// with at most 4 slice pairs (levels 0-3) is summed exactly.  The remaining levels share
+// one more FP32 sum, which rounds once per MFMA: at level >= 4 (weights <= 2^-32 of
+// level 0) that costs ~1e-14; sharing from level 3 on costs ~3e-12 on a decaying
+// spectrum.  The FP64 recombination adds the sums and scales by 2^(e_row + e_col).
 //
 // Forms (slices of the data operand x slices of the table, pairs kept):
-//   BF16X6, StP: 7 x 6, pairs with s + t <= 6 (27 MFMAs per 32x32x16 block).  Spectral
-//     columns decay with l, so the data operand needs the seventh slice: the 6 x 6
-//     triangle leaves ~2e-12 of max|result| on an (l+1)^-2 spectrum, 7 x 6 ~4e-14.
-//   BF16X6, PtS: 6 x 6, s + t <= 5 (21 MFMAs); the folded hybrid data does not decay.
+//   BF16X6, StP: 7 x 6, pairs with s + t <= 6 (27 MFMAs per 32x32x16 block), five FP32
+//     sums (levels 0, 1, 2, 3, >= 4).  Spectral columns decay with l, so the data operand
+//     needs the seventh slice: the 6 x 6 triangle leaves ~2e-12 of max|result| on an
+//     (l+1)^-2 spectrum, 7 x 6 ~5e-14.
+//   BF16X6, PtS: 6 x 6, s + t <= 5 (21 MFMAs), four FP32 sums (levels 0, 1, 2, >= 3);
+//     the folded hybrid data does not decay (~2e-14).
 //   BF16X3, both: 3 x 3, s + t <= 2 (6 MFMAs), one FP32 sum: ~1e-6.
 //
 // Tables are split once at plan creation into "fragment-major" planes, so that a wave's
@@ -60,7 +63,7 @@ struct form {
     static constexpr int NA = NA_;  // data slices
     static constexpr int NB = NB_;  // table slices
     static constexpr int LEV = LEV_;
-    static constexpr int NG = NG_;  // FP32 sums: 1, or 4 (levels 0, 1, 2, >= 3)
+    static constexpr int NG = NG_;  // FP32 sums: one per level below NG - 1, the rest share the last
     static constexpr int pair(int q, bool want_s)
     {
         int n = 0;
@@ -83,10 +86,10 @@ struct form {
     static constexpr int group(int q)
     {
         const int L = pair(q, true) + pair(q, false);
-        return NG == 1 ? 0 : (L < 3 ? L : 3);
+        return L < NG - 1 ? L : NG - 1;
     }
 };
-using stp_x6 = form<7, 6, 6, 4>;
+using stp_x6 = form<7, 6, 6, 5>;
 using pts_x6 = form<6, 6, 5, 4>;
 using any_x3 = form<3, 3, 2, 1>;
 static_assert(stp_x6::npairs() == 27 && pts_x6::npairs() == 21 && any_x3::npairs() == 6, "pair counts");
@@ -222,9 +225,10 @@ __device__ __forceinline__ void mfma_pairs(f32x16 (&acc)[F::NG], const u32x4 (&a
     }
 }
 
-// adds the FP32 level sums into the FP64 sums and clears them
+// adds the FP32 level sums into this thread's FP64 sums in LDS (dsum[cb*16+v][tid]) and
+// clears them
 template <class F>
-__device__ __forceinline__ void flush(f32x16 (&acc)[2][F::NG], double (&dacc)[2][16])
+__device__ __forceinline__ void flush(f32x16 (&acc)[2][F::NG], double (*dsum)[kThreads])
 {
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb)
@@ -233,12 +237,19 @@ __device__ __forceinline__ void flush(f32x16 (&acc)[2][F::NG], double (&dacc)[2]
             double sum = 0.0;
 #pragma unroll
             for (int g = F::NG - 1; g >= 0; --g) sum += (double)acc[cb][g][v];
-            dacc[cb][v] += sum;
+            dsum[cb * 16 + v][threadIdx.x] += sum;
         }
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
         for (int g = 0; g < F::NG; ++g) acc[cb][g] = f32x16{};
+}
+
+// this thread's FP64 sums in LDS, cleared
+__device__ __forceinline__ void clear_sums(double (*dsum)[kThreads])
+{
+#pragma unroll
+    for (int i = 0; i < 32; ++i) dsum[i][threadIdx.x] = 0.0;
 }
 
 // Writes the NA slices of 4 consecutive K values of data column ca (0..63) into the A
@@ -282,12 +293,12 @@ __device__ __forceinline__ void get_b(const u32x4 *__restrict__ frag, int lane, 
 
 // The K loop of one contraction: nks steps of 16, with the data of step ks delivered
 // by load_a(ks, v) (4 values of this thread's column quarter) and the table fragments
-// of step ks at tb + ks * NB fragments.  Adds the result into dacc.  nks is uniform
-// over the workgroup; every step ends with one barrier.
+// of step ks at tb + ks * NB fragments.  Adds the result into this thread's FP64 sums
+// in LDS.  nks is uniform over the workgroup; every step ends with one barrier.
 template <class F, class LoadA>
 __device__ __forceinline__ void k_loop(const LoadA &load_a, int nks, const u32x4 *__restrict__ tb, bool active,
                                        u32x4 (*abuf)[2 * F::NA * 64], int ca, int q, int ea, int lane,
-                                       double (&dacc)[2][16])
+                                       double (*dsum)[kThreads])
 {
     if (nks <= 0) return;
     {
@@ -328,14 +339,15 @@ __device__ __forceinline__ void k_loop(const LoadA &load_a, int nks, const u32x4
             }
             __syncthreads();
         }
-        flush<F>(acc, dacc);
+        flush<F>(acc, dsum);
     }
 }
 
 // --------------------------------------------------------------- StP
-// Work item (mp, j0 = r0, c0): 64 columns x 128 theta rows of both parities.  Wave w
-// owns theta block jb = j0/32 + w.  The spectral tile is read as 16-byte pairs
-// (l, l+1) and each parity pass keeps its half.
+// Work item (mp, c0): 64 columns, every theta row, in tiles of 128 rows (wave w owns
+// theta block jb = j0/32 + w) with both parities each.  The spectral tile is read as
+// 16-byte pairs (l, l+1) and each parity pass keeps its half; the column exponents come
+// from one pass over the tile.
 template <class F>
 __global__ __launch_bounds__(kThreads, 1) void stp_split_kernel(
     const work_item *__restrict__ work, int lmax, int nt, int nt2, int nrhs, const int *__restrict__ mvals,
@@ -343,7 +355,7 @@ __global__ __launch_bounds__(kThreads, 1) void stp_split_kernel(
     const u32x4 *__restrict__ tdig, const int *__restrict__ texp, double *__restrict__ p2)
 {
     __shared__ u32x4 abuf[2][2 * F::NA * 64];
-    __shared__ double ebuf[2 * 16][kThreads];  // the scaled even-parity sums of each lane
+    __shared__ double sums[2][2 * 16][kThreads];  // FP64 sums of each thread, per parity
     __shared__ int cexp[kTileC];
     const work_item w = work[blockIdx.x];
     if (w.mp < 0) return;
@@ -352,68 +364,81 @@ __global__ __launch_bounds__(kThreads, 1) void stp_split_kernel(
     const double *sb = s + s_off[mp];
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(sb, (long)nl * nrhs * 8);
 
-    for (int cc = wave; cc < kTileC; cc += 4) {
-        const int c = w.c0 + cc;
+    // column exponents: the four threads of a column read it in the main loop's
+    // pattern (l = 32 i + 8 q .. +7), four steps of loads in flight at a time
+    const int ca = threadIdx.x >> 2, q = threadIdx.x & 3;
+    const unsigned a_off = (unsigned)min(w.c0 + ca, nrhs) * nl * 8;  // column nrhs and beyond read as 0
+    {
         double mx = 0.0;
-        if (c < nrhs)
-            for (int l = lane; l < nl; l += 64) mx = fmax(mx, fabs(sb[(long)c * nl + l]));
-        mx = wave_max(mx);
-        if (lane == 0) cexp[cc] = scale_exponent(mx);
+        const int n32 = cdiv(nl, 32);
+        for (int i0 = 0; i0 < n32; i0 += 4) {
+            double2 d[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    d[u][e] = load_d2(rs, a_off + (unsigned)(32 * (i0 + u) + 8 * q + 2 * e) * 8);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int l = 32 * (i0 + u) + 8 * q + 2 * e;
+                    if (l < nl) mx = fmax(mx, fabs(d[u][e].x));
+                    if (l + 1 < nl) mx = fmax(mx, fabs(d[u][e].y));
+                }
+        }
+        mx = fmax(mx, __shfl_xor(mx, 1, 64));
+        mx = fmax(mx, __shfl_xor(mx, 2, 64));
+        if (q == 0) cexp[ca] = scale_exponent(mx);
     }
     __syncthreads();
-
-    const int ca = threadIdx.x >> 2, q = threadIdx.x & 3;
     const int ea = cexp[ca];
-    const unsigned a_off = (unsigned)min(w.c0 + ca, nrhs) * nl * 8;  // column nrhs and beyond read as 0
-    const int njb = cdiv(nt2, 32), jb = w.r0 / 32 + wave;
-    const bool active = jb < njb;
-    const int nks_e = cdiv(ne, 16);
-    const int j = jb * 32 + (lane & 31);
-
+    const int njb = cdiv(nt2, 32), nks_e = cdiv(ne, 16);
+    double *ob = p2 + (long)mp * nrhs * nt;
+    for (int r0 = 0; r0 < nt2; r0 += kTileR) {
+        const int jb = r0 / 32 + wave;
+        const bool active = jb < njb;
+        const int j = jb * 32 + (lane & 31);
 #pragma unroll
-    for (int par = 0; par < 2; ++par) {
-        double dacc[2][16];
+        for (int par = 0; par < 2; ++par) {
+            clear_sums(sums[par]);
+            const int K = par ? no : ne, nks = cdiv(K, 16);
+            const u32x4 *tb =
+                tdig + ((frag_off[mp] + (par ? (long)njb * nks_e * F::NB : 0) + (long)jb * nks * F::NB) << 6);
+            // raw data of one K step: l = 32 ks + 8 q .. +7 as four (even, odd) pairs
+            auto load_a = [&](int ks, double (&v)[4]) {
+                const int l0 = 32 * ks + 8 * q;
 #pragma unroll
-        for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) dacc[cb][v] = 0.0;
-        const int K = par ? no : ne, nks = cdiv(K, 16);
-        const u32x4 *tb =
-            tdig + ((frag_off[mp] + (par ? (long)njb * nks_e * F::NB : 0) + (long)jb * nks * F::NB) << 6);
-        // raw data of one K step: l = 32 ks + 8 q .. +7 as four (even, odd) pairs
-        auto load_a = [&](int ks, double (&v)[4]) {
-            const int l0 = 32 * ks + 8 * q;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const double2 d = load_d2(rs, a_off + (unsigned)(l0 + 2 * i) * 8);
-                v[i] = l0 + 2 * i + par < nl ? (par ? d.y : d.x) : 0.0;
-            }
-        };
-        k_loop<F>(load_a, nks, tb, active, abuf, ca, q, ea, lane, dacc);
-        const bool store = active && j < nt2;
-        const int ej = store ? texp[((long)mp * 2 + par) * nt2 + j] : 0;
-        double *ob = p2 + (long)mp * nrhs * nt;
-#pragma unroll
-        for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = w.c0 + cc;
-                const double r = __builtin_ldexp(dacc[cb][v], cexp[cc] + ej);
-                if (par == 0) {
-                    ebuf[cb * 16 + v][threadIdx.x] = r;
-                } else if (store && c < nrhs) {
-                    const double e = ebuf[cb * 16 + v][threadIdx.x];
-                    ob[(long)c * nt + j] = e + r;
-                    ob[(long)c * nt + (nt - 1 - j)] = e - r;
+                for (int i = 0; i < 4; ++i) {
+                    const double2 d = load_d2(rs, a_off + (unsigned)(l0 + 2 * i) * 8);
+                    v[i] = l0 + 2 * i + par < nl ? (par ? d.y : d.x) : 0.0;
                 }
-            }
+            };
+            k_loop<F>(load_a, nks, tb, active, abuf, ca, q, ea, lane, sums[par]);
+        }
+        if (active && j < nt2) {
+            const int ej = texp[((long)mp * 2) * nt2 + j], oj = texp[((long)mp * 2 + 1) * nt2 + j];
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = w.c0 + cc;
+                    if (c < nrhs) {
+                        const double e = __builtin_ldexp(sums[0][cb * 16 + v][threadIdx.x], cexp[cc] + ej);
+                        const double o = __builtin_ldexp(sums[1][cb * 16 + v][threadIdx.x], cexp[cc] + oj);
+                        ob[(long)c * nt + j] = e + o;
+                        ob[(long)c * nt + (nt - 1 - j)] = e - o;
+                    }
+                }
+        }
     }
 }
 
 // --------------------------------------------------------------- PtS
-// Work item (mp, k0 = r0, c0, parity = pad): 64 columns x 128 parity-l rows of one
-// parity.  Wave w owns row block kb = k0/32 + w.  The data operand is the fold
-// f(j) + f(nt-1-j) (even) or f(j) - f(nt-1-j) (odd) of the hybrid columns.
+// Work item (mp, c0): 64 columns, every parity-l row of both parities, in row tiles of
+// 128 (wave w owns row block kb = k0/32 + w).  The data operand is the fold
+// f(j) + f(nt-1-j) (even) or f(j) - f(nt-1-j) (odd) of the hybrid columns; both folds'
+// column exponents come from one pass over the tile.
 template <class F>
 __global__ __launch_bounds__(kThreads, 1) void pts_split_kernel(
     const work_item *__restrict__ work, int lmax, int nt, int nt2, int nrhs, const int *__restrict__ mvals,
@@ -422,73 +447,96 @@ __global__ __launch_bounds__(kThreads, 1) void pts_split_kernel(
     double *__restrict__ s)
 {
     __shared__ u32x4 abuf[2][2 * F::NA * 64];
-    __shared__ int cexp[kTileC];
+    __shared__ double sums[2 * 16][kThreads];  // FP64 sums of each thread
+    __shared__ int cexp[2][kTileC];
     const work_item w = work[blockIdx.x];
     if (w.mp < 0) return;
-    const int mp = w.mp, par = w.pad, m = mvals[mp], nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
-    const int K_rows = par ? no : ne;
+    const int mp = w.mp, m = mvals[mp], nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const double *pb = p2 + (long)mp * nrhs * nt;
     const __amdgpu_buffer_rsrc_t rp = make_rsrc(pb, (long)nrhs * nt * 8);
-    const double sgn = par ? -1.0 : 1.0;
-
-    for (int cc = wave; cc < kTileC; cc += 4) {
-        const int c = w.c0 + cc;
-        double mx = 0.0;
-        if (c < nrhs) {
-            const double *pc = pb + (long)c * nt;
-            for (int jj = lane; jj < nt2; jj += 64) mx = fmax(mx, fabs(pc[jj] + sgn * pc[nt - 1 - jj]));
-        }
-        mx = wave_max(mx);
-        if (lane == 0) cexp[cc] = scale_exponent(mx);
-    }
-    __syncthreads();
-
     const int ca = threadIdx.x >> 2, q = threadIdx.x & 3;
-    const int ea = cexp[ca];
     const unsigned a_off = (unsigned)min(w.c0 + ca, nrhs) * nt * 8;
-    const int njs = cdiv(nt2, 16), kb = w.r0 / 32 + wave;
-    const bool active = kb * 32 < K_rows;
-    const u32x4 *tb =
-        tdig + ((frag_off[mp] + (par ? (long)cdiv(ne, 32) * njs * F::NB : 0) + (long)kb * njs * F::NB) << 6);
+    const int njs = cdiv(nt2, 16);
 
-    // folded data of one K step: j = 16 js + 4 q .. +3 against the mirrored nt-1-j
-    auto load_a = [&](int js, double (&v)[4]) {
+    // raw data of one K step: f(j) and f(nt-1-j) for j = 16 js + 4 q .. +3 (nt-4-j0 < 0
+    // only past the last valid j, where the offset wraps past the end or into a column
+    // whose values are masked)
+    auto load_raw = [&](int js, double (&a)[4], double (&b)[4]) {
         const int j0 = 16 * js + 4 * q;
         const double2 f0 = load_d2(rp, a_off + j0 * 8u), f1 = load_d2(rp, a_off + (j0 + 2) * 8u);
-        // g0 = (f(nt-2-j0), f(nt-1-j0)), g1 = (f(nt-4-j0), f(nt-3-j0))
-        // (nt-4-j0 < 0 only past the last step's valid j: wraps past the end, masked)
-        const double2 g0 = load_d2(rp, a_off + (unsigned)(nt - 2 - j0) * 8u);
-        const double2 g1 = load_d2(rp, a_off + (unsigned)(nt - 4 - j0) * 8u);
-        const double a[4] = {f0.x, f0.y, f1.x, f1.y}, b[4] = {g0.y, g0.x, g1.y, g1.x};
+        const double2 g0 = load_d2(rp, a_off + (unsigned)(nt - 2 - j0) * 8u);  // f(nt-2-j0), f(nt-1-j0)
+        const double2 g1 = load_d2(rp, a_off + (unsigned)(nt - 4 - j0) * 8u);  // f(nt-4-j0), f(nt-3-j0)
+        a[0] = f0.x, a[1] = f0.y, a[2] = f1.x, a[3] = f1.y;
+        b[0] = g0.y, b[1] = g0.x, b[2] = g1.y, b[3] = g1.x;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = j0 + i < nt2 ? a[i] + sgn * b[i] : 0.0;
+        for (int i = 0; i < 4; ++i)
+            if (j0 + i >= nt2) a[i] = b[i] = 0.0;
     };
-
-    double dacc[2][16];
+    // exponents of both folds, four steps of loads in flight at a time
+    {
+        double me = 0.0, mo = 0.0;
+        for (int j0 = 0; j0 < njs; j0 += 4) {
+            double a[4][4], b[4][4];
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
+            for (int u = 0; u < 4; ++u) load_raw(j0 + u, a[u], b[u]);  // steps past njs read zeros
 #pragma unroll
-        for (int v = 0; v < 16; ++v) dacc[cb][v] = 0.0;
-    k_loop<F>(load_a, njs, tb, active, abuf, ca, q, ea, lane, dacc);
-
-    const int k = kb * 32 + (lane & 31);
-    if (!active || k >= K_rows) return;
-    const int ek = texp[exp_off[mp] + (par ? ne : 0) + k];
-    double *sb = s + s_off[mp];
+            for (int u = 0; u < 4; ++u)
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = w.c0 + cc;
-            if (c < nrhs) sb[(long)c * nl + 2 * k + par] = __builtin_ldexp(dacc[cb][v], cexp[cc] + ek);
+                for (int i = 0; i < 4; ++i) {
+                    me = fmax(me, fabs(a[u][i] + b[u][i]));
+                    mo = fmax(mo, fabs(a[u][i] - b[u][i]));
+                }
         }
+        me = fmax(me, __shfl_xor(me, 1, 64));
+        me = fmax(me, __shfl_xor(me, 2, 64));
+        mo = fmax(mo, __shfl_xor(mo, 1, 64));
+        mo = fmax(mo, __shfl_xor(mo, 2, 64));
+        if (q == 0) {
+            cexp[0][ca] = scale_exponent(me);
+            cexp[1][ca] = scale_exponent(mo);
+        }
+    }
+    __syncthreads();
+    double *sb = s + s_off[mp];
+
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+        const int K_rows = par ? no : ne, ea = cexp[par][ca];
+        auto load_a = [&](int js, double (&v)[4]) {
+            double a[4], b[4];
+            load_raw(js, a, b);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] = par ? a[i] - b[i] : a[i] + b[i];
+        };
+        const long region = frag_off[mp] + (par ? (long)cdiv(ne, 32) * njs * F::NB : 0);
+        for (int k0 = 0; k0 < K_rows; k0 += kTileR) {
+            const int kb = k0 / 32 + wave;
+            const bool active = kb * 32 < K_rows;
+            const u32x4 *tb = tdig + ((region + (long)kb * njs * F::NB) << 6);
+            clear_sums(sums);
+            k_loop<F>(load_a, njs, tb, active, abuf, ca, q, ea, lane, sums);
+
+            const int k = kb * 32 + (lane & 31);
+            if (active && k < K_rows) {
+                const int ek = texp[exp_off[mp] + (par ? ne : 0) + k];
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) {
+                        const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = w.c0 + cc;
+                        if (c < nrhs)
+                            sb[(long)c * nl + 2 * k + par] =
+                                __builtin_ldexp(sums[cb * 16 + v][threadIdx.x], cexp[par][cc] + ek);
+                    }
+            }
+        }
+    }
 }
 
 // --------------------------------------------------------------- host side
-// XCD-balanced work lists on the 64 x 128 tiles: whole m-blocks dealt to the eight
-// XCDs by longest-processing-time, column tile outer, interleaved so that item i runs
-// on XCD i % 8.  PtS items carry their parity in `pad`.
+// XCD-balanced work lists of (m-block, column tile) items: whole m-blocks dealt to the
+// eight XCDs by longest-processing-time, interleaved so that item i runs on XCD i % 8.
 std::vector<work_item> split_work(const raysht_plan *p, bool stp)
 {
     constexpr int kXcds = 8;
@@ -497,26 +545,15 @@ std::vector<work_item> split_work(const raysht_plan *p, bool stp)
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return p->mvals[a] < p->mvals[b]; });
     std::vector<std::vector<work_item>> q(kXcds);
     std::vector<double> load(kXcds, 0.0);
-    const int nct = cdiv(p->nrhs, kTileC);
+    const int nct = cdiv(p->nrhs, kTileC), nrt = cdiv(p->nt2, kTileR), njs = cdiv(p->nt2, 16);
     for (int mp : order) {
         const int m = p->mvals[mp], nl = p->lmax - m + 1, ne = (nl + 1) / 2, no = nl / 2;
+        // K steps per item (plus the exponent pass)
+        const double steps = stp ? (double)nrt * (cdiv(ne, 16) + cdiv(no, 16)) + cdiv(nl, 32)
+                                 : (double)(cdiv(ne, kTileR) + cdiv(no, kTileR)) * njs + njs;
         const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        double cost = 0.0;
-        for (int c = 0; c < nct; ++c) {
-            if (stp) {
-                for (int r = 0; r < p->nt2; r += kTileR) {
-                    q[x].push_back(work_item{mp, r, c * kTileC, 0});
-                    cost += 16.0 * (cdiv(ne, 16) + cdiv(no, 16)) + 8.0;
-                }
-            } else {
-                for (int par = 0; par < 2; ++par)
-                    for (int r = 0; r < (par ? no : ne); r += kTileR) {
-                        q[x].push_back(work_item{mp, r, c * kTileC, par});
-                        cost += (double)p->nt2 * std::min(kTileR, (par ? no : ne) - r) / kTileR + 8.0;
-                    }
-            }
-        }
-        load[x] += cost;
+        for (int c = 0; c < nct; ++c) q[x].push_back(work_item{mp, 0, c * kTileC, 0});
+        load[x] += steps * nct;
     }
     size_t len = 0;
     for (auto &v : q) len = std::max(len, v.size());
