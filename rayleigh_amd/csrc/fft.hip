@@ -2,9 +2,10 @@
 //
 // FFT_To_Physical / FFT_To_Spectral (Fourier_Transform.F90:56-183) run FFTW's
 // unnormalised c2r (e^{+i}) / r2c (e^{-i}) in place on lines of nphi+2 doubles
-// holding Re/Im of m = 0..nphi/2.  Rayleigh's nphi = 3(l_max+1) is 3 * 2^k, so a
-// length-nphi real transform is done as a length-nphi/2 complex transform (radix
-// 4/2/3 Stockham stages in LDS) plus the usual even/odd split pass.
+// holding Re/Im of m = 0..nphi/2.  A length-nphi real transform is done as a
+// length-nphi/2 complex transform plus the usual even/odd split pass.  Rayleigh's
+// nphi = 3 (l_max + 1) is 3 * 2^k: those lengths run register-resident (below); other
+// even lengths 2^a 3^b run as radix 8/4/2/3 Stockham stages in LDS.
 //
 // Transpose_2a3a / 3b2b (Spherical_Buffer.F90:196-703) reduce on one rank to
 // pure index maps between the hybrid layout p2[mp][c][theta] and the physical
@@ -24,10 +25,10 @@ struct cplx {
     double re, im;
 };
 
-__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
+__host__ __device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__host__ __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__host__ __device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+__host__ __device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
 
 constexpr int kFftThreads = 256;
 constexpr int kMaxHalf = 4096;       // largest complex length (nphi/2) held in LDS
@@ -241,6 +242,293 @@ __global__ __launch_bounds__(kFftThreads) void fft_r2c_kernel(int nphi, long how
     }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident real FFTs for Rayleigh's line lengths, nphi = 3 (l_max + 1)
+// with n2 = nphi / 2 = 3 * 2^k (24 .. 3072).  The complex length-n2 transform runs
+// as radix-8 Stockham passes followed by one radix-3, -6 or -12 pass, all sizes and
+// strides compile-time: every thread holds 24 points (kP) of one line in registers,
+// n2 / 24 threads share a line, and the passes exchange through LDS with a padded
+// layout (one 16-byte gap per 16 points: conflict-free ds_read/write_b128 for every
+// pass's pattern).  The real-to-complex packing rides on the first pass (c2r) or a
+// final LDS pass (r2c), so each line is read and written in HBM exactly once.
+
+constexpr int kP = 24;
+
+__host__ __device__ constexpr int fft3_n8(int n)  // radix-8 passes before the last
+{
+    int a = 0;
+    while (n % 8 == 0 && n / 8 >= 3 && (n / 8 == 3 || n / 8 == 6 || n / 8 == 12 || n / 8 >= 24)) {
+        n /= 8;
+        ++a;
+    }
+    return a;
+}
+__host__ __device__ constexpr int fft3_last(int n)
+{
+    for (int a = fft3_n8(n); a > 0; --a) n /= 8;
+    return n;
+}
+__host__ __device__ constexpr int pow8(int a) { return a == 0 ? 1 : 8 * pow8(a - 1); }
+__host__ __device__ constexpr bool fft3_ok(int n)
+{
+    return n >= kP && n % kP == 0 && (fft3_last(n) == 3 || fft3_last(n) == 6 || fft3_last(n) == 12) &&
+           pow8(fft3_n8(n)) * fft3_last(n) == n;
+}
+__host__ __device__ constexpr int fft3_pad(int i) { return i + (i >> 4); }
+
+// in-place small DFTs, y_k = sum_n v_n e^{sign 2 pi i n k / R}
+__host__ __device__ __forceinline__ cplx mul_i(cplx a, int sign) { return {-sign * a.im, sign * a.re}; }
+
+__host__ __device__ __forceinline__ void dft3(cplx &a0, cplx &a1, cplx &a2, int sign)
+{
+    const double h = 0.8660254037844386467637231707529361834714;  // sqrt(3)/2
+    const cplx t1 = cadd(a1, a2);
+    const cplx t2 = {a0.re - 0.5 * t1.re, a0.im - 0.5 * t1.im};
+    const cplx d0 = csub(a1, a2);
+    const cplx t3 = {-sign * h * d0.im, sign * h * d0.re};
+    a0 = cadd(a0, t1);
+    a1 = cadd(t2, t3);
+    a2 = csub(t2, t3);
+}
+
+__host__ __device__ __forceinline__ void dft4(cplx &a0, cplx &a1, cplx &a2, cplx &a3, int sign)
+{
+    const cplx a = cadd(a0, a2), b = csub(a0, a2);
+    const cplx c = cadd(a1, a3), d = mul_i(csub(a1, a3), sign);
+    a0 = cadd(a, c);
+    a1 = cadd(b, d);
+    a2 = csub(a, c);
+    a3 = csub(b, d);
+}
+
+__host__ __device__ __forceinline__ void dft8(cplx (&v)[8], int sign)
+{
+    cplx e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4(e0, e1, e2, e3, sign);
+    dft4(o0, o1, o2, o3, sign);
+    const double r2 = 0.7071067811865475244008443621048490392848;  // 1/sqrt(2)
+    const cplx t1 = cmul(o1, cplx{r2, sign * r2});
+    const cplx t2 = mul_i(o2, sign);
+    const cplx t3 = cmul(o3, cplx{-r2, sign * r2});
+    v[0] = cadd(e0, o0), v[4] = csub(e0, o0);
+    v[1] = cadd(e1, t1), v[5] = csub(e1, t1);
+    v[2] = cadd(e2, t2), v[6] = csub(e2, t2);
+    v[3] = cadd(e3, t3), v[7] = csub(e3, t3);
+}
+
+__host__ __device__ __forceinline__ void dft6(cplx (&v)[6], int sign)
+{
+    cplx e0 = v[0], e1 = v[2], e2 = v[4], o0 = v[1], o1 = v[3], o2 = v[5];
+    dft3(e0, e1, e2, sign);
+    dft3(o0, o1, o2, sign);
+    const double h = 0.8660254037844386467637231707529361834714;
+    const cplx t1 = cmul(o1, cplx{0.5, sign * h});   // e^{sign 2 pi i / 6}
+    const cplx t2 = cmul(o2, cplx{-0.5, sign * h});  // e^{sign 4 pi i / 6}
+    v[0] = cadd(e0, o0), v[3] = csub(e0, o0);
+    v[1] = cadd(e1, t1), v[4] = csub(e1, t1);
+    v[2] = cadd(e2, t2), v[5] = csub(e2, t2);
+}
+
+__host__ __device__ __forceinline__ void dft12(cplx (&v)[12], int sign)
+{
+    cplx e[6] = {v[0], v[2], v[4], v[6], v[8], v[10]}, o[6] = {v[1], v[3], v[5], v[7], v[9], v[11]};
+    dft6(e, sign);
+    dft6(o, sign);
+    const double h = 0.8660254037844386467637231707529361834714;
+    const cplx w[6] = {{1.0, 0.0},  {h, sign * 0.5},  {0.5, sign * h},
+                       {0.0, 1.0 * sign}, {-0.5, sign * h}, {-h, sign * 0.5}};  // e^{sign 2 pi i k / 12}
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const cplx t = k == 0 ? o[0] : cmul(o[k], w[k]);
+        v[k] = cadd(e[k], t);
+        v[k + 6] = csub(e[k], t);
+    }
+}
+
+template <int R>
+__host__ __device__ __forceinline__ void dft_r(cplx (&v)[R], int sign)
+{
+    if constexpr (R == 8) dft8(v, sign);
+    else if constexpr (R == 12) dft12(v, sign);
+    else if constexpr (R == 6) dft6(v, sign);
+    else dft3(v[0], v[1], v[2], sign);
+}
+
+// One Stockham pass of radix R with input stride NS over the thread's kP points:
+// butterfly u (b = t + T u) takes in[u*R + r] = element b + r N/R, twiddles input r by
+// e^{sign 2 pi i r (b mod NS) / (NS R)} (table of e^{2 pi i v / 2N}), and leaves
+// out element (b / NS) NS R + b mod NS + r NS in v[u*R + r].
+template <int N, int R, int NS>
+__device__ __forceinline__ void fft3_pass(cplx (&v)[kP], int t, const double2 *__restrict__ tw, int sign)
+{
+    constexpr int T = N / kP;
+#pragma unroll
+    for (int u = 0; u < kP / R; ++u) {
+        const int b = t + T * u, k = b & (NS - 1);
+        cplx y[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            y[r] = v[u * R + r];
+            if (NS > 1 && r > 0) {
+                const double2 w = tw[2 * r * k * (N / (NS * R))];
+                y[r] = cmul(y[r], cplx{w.x, sign * w.y});
+            }
+        }
+        dft_r<R>(y, sign);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[u * R + r] = y[r];
+    }
+}
+
+template <int N, int R, int NS>
+__device__ __forceinline__ void fft3_store_lds(cplx *__restrict__ buf, const cplx (&v)[kP], int t)
+{
+    constexpr int T = N / kP;
+#pragma unroll
+    for (int u = 0; u < kP / R; ++u) {
+        const int b = t + T * u;
+        const int o = (b / NS) * NS * R + (b & (NS - 1));
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[fft3_pad(o + r * NS)] = v[u * R + r];
+    }
+}
+
+template <int N, int R>
+__device__ __forceinline__ void fft3_load_lds(const cplx *__restrict__ buf, cplx (&v)[kP], int t)
+{
+    constexpr int T = N / kP;
+#pragma unroll
+    for (int u = 0; u < kP / R; ++u)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[u * R + r] = buf[fft3_pad(t + T * u + r * (N / R))];
+}
+
+// the radix-8 passes 1 .. A-1 and the last pass's LDS exchange; pass 0 is done by the caller
+template <int N, int P8, int A>
+__device__ __forceinline__ void fft3_middle(cplx *__restrict__ buf, cplx (&v)[kP], int t,
+                                            const double2 *__restrict__ tw, int sign)
+{
+    if constexpr (P8 < A) {
+        constexpr int NS = pow8(P8);
+        fft3_store_lds<N, 8, NS / 8>(buf, v, t);
+        __syncthreads();
+        fft3_load_lds<N, 8>(buf, v, t);
+        __syncthreads();
+        fft3_pass<N, 8, NS>(v, t, tw, sign);
+        fft3_middle<N, P8 + 1, A>(buf, v, t, tw, sign);
+    } else {
+        constexpr int F = fft3_last(N), NS = pow8(A);
+        if constexpr (A > 0) {
+            fft3_store_lds<N, 8, NS / 8>(buf, v, t);
+            __syncthreads();
+            fft3_load_lds<N, F>(buf, v, t);
+            __syncthreads();
+        }
+        fft3_pass<N, F, NS>(v, t, tw, sign);
+    }
+}
+
+template <int N>
+constexpr int fft3_threads()
+{
+    return N / kP > 64 ? N / kP : 64;
+}
+
+// c2r: z_k = A_k + i B_k (A = X_k + conj X_{N-k}, B = (X_k - conj X_{N-k}) e^{2 pi i k / 2N}),
+// inverse length-N FFT, x_{2n} + i x_{2n+1} = z_n.
+template <int N>
+__global__ __launch_bounds__(fft3_threads<N>()) void fft3_c2r_kernel(long howmany, double *__restrict__ x,
+                                                                      const double2 *__restrict__ tw)
+{
+    constexpr int T = N / kP, LPB = fft3_threads<N>() / T, A = fft3_n8(N), F = fft3_last(N);
+    constexpr int R0 = A > 0 ? 8 : F;
+    constexpr int STRIDE = fft3_pad(N - 1) + 1;
+    __shared__ cplx lds[LPB * STRIDE];
+    const int t = threadIdx.x % T, li = threadIdx.x / T;
+    const long line = (long)blockIdx.x * LPB + li;
+    if (line >= howmany) return;  // whole lines: the barriers below see only complete lines
+    double *xl = x + line * (2L * N + 2);
+    cplx *buf = lds + li * STRIDE;
+    cplx v[kP];
+#pragma unroll
+    for (int u = 0; u < kP / R0; ++u)
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const int k = t + T * u + r * (N / R0);
+            const double2 xa = reinterpret_cast<const double2 *>(xl)[k];
+            const double2 xb = reinterpret_cast<const double2 *>(xl)[N - k];
+            const cplx a = {xa.x, k == 0 ? 0.0 : xa.y};
+            const cplx bc = {xb.x, k == 0 ? 0.0 : -xb.y};  // conj X_{N-k}; Im X_N does not enter
+            const cplx Ak = cadd(a, bc);
+            const double2 w = tw[k];
+            const cplx Bk = cmul(csub(a, bc), cplx{w.x, w.y});
+            v[u * R0 + r] = {Ak.re - Bk.im, Ak.im + Bk.re};
+        }
+    fft3_pass<N, R0, 1>(v, t, tw, +1);
+    fft3_middle<N, 1, A>(buf, v, t, tw, +1);
+    constexpr int NSL = N / F;
+#pragma unroll
+    for (int u = 0; u < kP / F; ++u)
+#pragma unroll
+        for (int r = 0; r < F; ++r)
+            reinterpret_cast<double2 *>(xl)[t + T * u + r * NSL] = make_double2(v[u * F + r].re, v[u * F + r].im);
+    if (t == 0) reinterpret_cast<double2 *>(xl)[N] = make_double2(0.0, 0.0);
+}
+
+// r2c: z_n = x_{2n} + i x_{2n+1}, forward length-N FFT, then
+// X_k = (Z_k + conj Z_{N-k}) / 2 - i e^{-2 pi i k / 2N} (Z_k - conj Z_{N-k}) / 2, k = 0..N.
+template <int N>
+__global__ __launch_bounds__(fft3_threads<N>()) void fft3_r2c_kernel(long howmany, double *__restrict__ x,
+                                                                      const double2 *__restrict__ tw, int rescale)
+{
+    constexpr int T = N / kP, LPB = fft3_threads<N>() / T, A = fft3_n8(N), F = fft3_last(N);
+    constexpr int R0 = A > 0 ? 8 : F;
+    constexpr int STRIDE = fft3_pad(N - 1) + 1;
+    __shared__ cplx lds[LPB * STRIDE];
+    const int t = threadIdx.x % T, li = threadIdx.x / T;
+    const long line = (long)blockIdx.x * LPB + li;
+    if (line >= howmany) return;
+    double *xl = x + line * (2L * N + 2);
+    cplx *buf = lds + li * STRIDE;
+    cplx v[kP];
+#pragma unroll
+    for (int u = 0; u < kP / R0; ++u)
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+            const double2 z = reinterpret_cast<const double2 *>(xl)[t + T * u + r * (N / R0)];
+            v[u * R0 + r] = {z.x, z.y};
+        }
+    fft3_pass<N, R0, 1>(v, t, tw, -1);
+    fft3_middle<N, 1, A>(buf, v, t, tw, -1);
+    constexpr int NSL = N / F;
+    // Z_o for o = t + T u + r N/F, unpadded order through LDS for the mirrored pairs
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kP / F; ++u)
+#pragma unroll
+        for (int r = 0; r < F; ++r) buf[fft3_pad(t + T * u + r * NSL)] = v[u * F + r];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u <= kP; ++u) {
+        const int k = t + T * u;
+        if (u == kP && k != N) continue;  // k = N once, by thread 0
+        const cplx zk = buf[fft3_pad(k % N)];
+        const cplx zc = cconj(buf[fft3_pad((N - k) % N)]);
+        const cplx E = {0.5 * (zk.re + zc.re), 0.5 * (zk.im + zc.im)};
+        const cplx D = csub(zk, zc);
+        const cplx O = {0.5 * D.im, -0.5 * D.re};
+        const double2 w = tw[k % (2 * N)];
+        const cplx X = cadd(E, cmul(O, cplx{w.x, -w.y}));
+        double re = X.re, im = X.im;
+        if (rescale) {
+            // r2c_ip4D_fftw_rsc: X(k > 0) / nphi, Re X_0 / (2 nphi), Im X_0 untouched
+            re = k == 0 ? re / (4.0 * N) : re / (2.0 * N);
+            if (k > 0) im = im / (2.0 * N);
+        }
+        reinterpret_cast<double2 *>(xl)[k] = make_double2(re, im);
+    }
+}
+
 // The one-rank row transposes are 2-D transposes per (r, f): the hybrid side is
 // (mode q = 2m+imi) x theta with theta contiguous, the physical side theta x q with q
 // contiguous.  A 32 x 32 tile goes through LDS so both sides are read and written in
@@ -328,6 +616,40 @@ int get_twiddles(int nphi, const double2 **out)
     return RAYSHT_OK;
 }
 
+// the register-resident kernels for n2 = 3 * 2^k, 24 <= n2 <= 3072; returns false for
+// other lengths (handled by the generic LDS Stockham kernels)
+template <int N>
+int launch_fft3(bool c2r, long howmany, double *x, const double2 *tw, int rescale, hipStream_t st)
+{
+    static_assert(fft3_ok(N), "length without a register-resident schedule");
+    constexpr int threads = fft3_threads<N>(), lpb = threads / (N / kP);
+    const long blocks = (howmany + lpb - 1) / lpb;
+    if (blocks > 0x7fffffffL) return RAYSHT_EUNSUPPORTED;
+    if (c2r)
+        hipLaunchKernelGGL(fft3_c2r_kernel<N>, dim3((unsigned)blocks), dim3(threads), 0, st, howmany, x, tw);
+    else
+        hipLaunchKernelGGL(fft3_r2c_kernel<N>, dim3((unsigned)blocks), dim3(threads), 0, st, howmany, x, tw,
+                           rescale);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+bool fft3_dispatch(int n2, bool c2r, long howmany, double *x, const double2 *tw, int rescale, hipStream_t st,
+                   int *rc)
+{
+    switch (n2) {
+    case 24: *rc = launch_fft3<24>(c2r, howmany, x, tw, rescale, st); return true;
+    case 48: *rc = launch_fft3<48>(c2r, howmany, x, tw, rescale, st); return true;
+    case 96: *rc = launch_fft3<96>(c2r, howmany, x, tw, rescale, st); return true;
+    case 192: *rc = launch_fft3<192>(c2r, howmany, x, tw, rescale, st); return true;
+    case 384: *rc = launch_fft3<384>(c2r, howmany, x, tw, rescale, st); return true;
+    case 768: *rc = launch_fft3<768>(c2r, howmany, x, tw, rescale, st); return true;
+    case 1536: *rc = launch_fft3<1536>(c2r, howmany, x, tw, rescale, st); return true;
+    case 3072: *rc = launch_fft3<3072>(c2r, howmany, x, tw, rescale, st); return true;
+    default: return false;
+    }
+}
+
 // radix schedule for n2 = 2^a 3^b: radix-8 stages, then one radix-4 or radix-2 for
 // the remaining power of two, then the radix-3 stages; packed 2 bits per stage
 // (0: 2, 1: 4, 2: 3, 3: 8).
@@ -384,6 +706,7 @@ int raysht_fft_to_physical(int nphi, long howmany, double *x, void *stream)
     if (rc != RAYSHT_OK || howmany == 0) return rc;
     const double2 *tw = nullptr;
     if ((rc = get_twiddles(nphi, &tw)) != RAYSHT_OK) return rc;
+    if (fft3_dispatch(nphi / 2, true, howmany, x, tw, 0, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
     int code = 0, nrad = 0;
     fft_schedule(nphi / 2, &code, &nrad);
     const int lines = fft_lines_per_block(nphi / 2);
@@ -401,6 +724,8 @@ int raysht_fft_to_spectral(int nphi, long howmany, double *x, int rescale, void 
     if (rc != RAYSHT_OK || howmany == 0) return rc;
     const double2 *tw = nullptr;
     if ((rc = get_twiddles(nphi, &tw)) != RAYSHT_OK) return rc;
+    if (fft3_dispatch(nphi / 2, false, howmany, x, tw, rescale ? 1 : 0, reinterpret_cast<hipStream_t>(stream), &rc))
+        return rc;
     int code = 0, nrad = 0;
     fft_schedule(nphi / 2, &code, &nrad);
     const int lines = fft_lines_per_block(nphi / 2);

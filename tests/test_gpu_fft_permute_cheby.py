@@ -17,7 +17,7 @@ def ra():
     return rayleigh_amd
 
 
-@pytest.mark.parametrize("nphi", [6, 12, 96, 192, 768, 1536, 3072])
+@pytest.mark.parametrize("nphi", [6, 12, 18, 36, 48, 72, 96, 192, 384, 768, 1536, 3072, 6144])
 def test_fft_c2r_r2c_vs_oracle(ra, nphi):
     rng = np.random.default_rng(nphi)
     nl = 37
@@ -37,6 +37,14 @@ def test_fft_c2r_r2c_vs_oracle(ra, nphi):
     t = dev(y)
     ra.fft_to_spectral_rsc(t, nphi)
     assert rel_max(host(t).ravel(), want) < 1e-14 * np.log2(nphi) * 4
+
+
+def test_fft_unsupported_length_is_an_error(ra):
+    """Lengths with a prime factor other than 2 and 3 are refused, not computed wrongly."""
+    from rayleigh_amd._lib import RayshtError
+    t = dev(np.zeros((2, 12)))
+    with pytest.raises(RayshtError):
+        ra.fft_to_physical(t, 10)
 
 
 def test_fft_round_trip_large_batch(ra):
