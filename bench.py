@@ -197,15 +197,23 @@ def reference_check_and_baseline(res, mvals, sample_m: int):
     return cpu, err_stp, err_pts
 
 
-def stack_bench(ra, steps: int, warmup: int):
-    """The transform stack at l_max = 255, N_theta = 384, N_phi = 768, N_r = 128, 8 fields:
+STACKS = {
+    # l_max, N_theta, N_phi, N_r, N_f
+    "stack": (255, 384, 768, 128, 8),     # full FP64 stack at l_max = 255
+    "large": (1023, 1536, 3072, 256, 5),  # large one-GPU load at l_max = 1023
+}
+
+
+def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
+    """The transform stack (default l_max = 255, N_theta = 384, N_phi = 768, N_r = 128,
+    8 fields; "large": l_max = 1023, N_theta = 1536, N_phi = 3072, N_r = 256, 5 fields):
     Chebyshev From_Spectral -> StP -> 2a->3a -> c2r, then r2c -> 3b->2b -> PtS -> To_Spectral,
     each stage timed with HIP events; GB/s of the bandwidth-bound stages.  The radial
     buffer is transformed in place of Rayleigh's p1 layout (the 1a<->2a radial transposes
     between the Chebyshev and Legendre stages are not part of this build)."""
     import torch
     from oracle import oracle as O
-    L, nt, nphi, nr, nf = 255, 384, 768, 128, 8
+    L, nt, nphi, nr, nf = STACKS[workload]
     mv = O.m_order_balanced(L)
     plan = ra.LegendrePlan(nt, L, mv, nr, nf)
     cheb = ra.ChebyGrid(nr, 0.35, 1.0)
@@ -242,7 +250,7 @@ def stack_bench(ra, steps: int, warmup: int):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / steps
     F = flops_per_direction(L, nt, nr, nf)
-    out = {"workload": "stack", "l_max": L, "n_theta": nt, "n_phi": nphi, "n_r": nr, "n_fields": nf,
+    out = {"workload": workload, "l_max": L, "n_theta": nt, "n_phi": nphi, "n_r": nr, "n_fields": nf,
            "ms_per_round_trip": wall * 1e3, "stages": {}}
     for i, (name, _, nbytes) in enumerate(stages):
         ms = float(np.mean([e[i].elapsed_time(e[i + 1]) for e in ev]))
@@ -254,6 +262,20 @@ def stack_bench(ra, steps: int, warmup: int):
             rec["gbps"] = nbytes / (ms * 1e-3) / 1e9
             rec["frac_hbm_peak"] = rec["gbps"] / 8000.0
         out["stages"][name] = rec
+    # the Legendre + FFT round trip replayed from one hipGraph (no Chebyshev stages)
+    g_rt = plan.round_trip_graph(nphi, s, p2, p3, s2)
+    for _ in range(warmup):
+        g_rt.launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        g_rt.launch()
+    e1.record()
+    torch.cuda.synchronize()
+    out["graph_round_trip_ms"] = e0.elapsed_time(e1) / steps
+    out["eager_round_trip_ms"] = sum(out["stages"][k]["ms"] for k in
+                                     ("stp", "permute_2a3a", "fft_c2r", "fft_r2c", "permute_3b2b", "pts"))
     print(json.dumps(out), flush=True)
 
 
@@ -279,8 +301,9 @@ def main():
                     help="Legendre forms to run; the first is the headline, fp64 is always run as the error reference")
     ap.add_argument("--cpu-sample-m", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="legendre", choices=["legendre", "stack"],
-                    help="legendre: the l_max=511 round trip (headline); stack: l_max=255 full stack per stage")
+    ap.add_argument("--workload", default="legendre", choices=["legendre", "stack", "large"],
+                    help="legendre: the l_max=511 round trip (headline); stack: l_max=255 full stack per "
+                         "stage; large: l_max=1023, N_phi=3072, N_r=256, 5 fields full stack per stage")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy (scales profiler byte counters)")
     args = ap.parse_args()
@@ -294,8 +317,8 @@ def main():
     from oracle import oracle as O
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a GPU")
-    if args.workload == "stack":
-        stack_bench(ra, args.steps, args.warmup)
+    if args.workload in STACKS:
+        stack_bench(ra, args.steps, args.warmup, args.workload)
         return
     mvals = O.m_order_balanced(LMAX)
     F = flops_per_direction()

@@ -126,6 +126,18 @@ int raysht_spectral_to_physical(raysht_plan *plan, int nphi, const double *s, do
 int raysht_physical_to_spectral(raysht_plan *plan, int nphi, double *p3, double *work,
                                 double *s, void *stream);
 
+/* The spectral -> physical -> spectral round trip (StP, 2a->3a, c2r, r2c, 3b->2b,
+ * PtS: Sphere_Hybrid_Space.F90:111,143 / Sphere_Physical_Space.F90:115,215 around an
+ * identity physical step) recorded once into a hipGraph on the given device buffers;
+ * each launch replays all six kernels with one call.  The plan and buffers must
+ * outlive the graph.  No reference counterpart: the reference launches each step
+ * from the host every time step. */
+typedef struct raysht_graph raysht_graph;
+int raysht_graph_create_round_trip(raysht_graph **graph, raysht_plan *plan, int nphi, const double *s_in,
+                                   double *work, double *p3, double *s_out);
+int raysht_graph_launch(raysht_graph *graph, void *stream);
+void raysht_graph_destroy(raysht_graph *graph);
+
 /* Cheby_Grid%init (Chebyshev_Polynomials.F90:65-207) for one domain of n
  * zeros-grid points on [rmin, rmax]; grid (host, n) receives the radii. */
 int raysht_cheb_create(raysht_cheb **cheb, int n, double rmin, double rmax, double *grid);

@@ -45,6 +45,9 @@ SIGNATURES = {
     "raysht_permute_3b2b": (_I, [_P, _I, _P, _P, _P]),
     "raysht_spectral_to_physical": (_I, [_P, _I, _P, _P, _P, _P]),
     "raysht_physical_to_spectral": (_I, [_P, _I, _P, _P, _P, _P]),
+    "raysht_graph_create_round_trip": (_I, [ctypes.POINTER(_P), _P, _I, _P, _P, _P, _P]),
+    "raysht_graph_launch": (_I, [_P, _P]),
+    "raysht_graph_destroy": (None, [_P]),
     "raysht_cheb_create": (_I, [ctypes.POINTER(_P), _I, _D, _D, _P]),
     "raysht_cheb_destroy": (None, [_P]),
     "raysht_cheb_table": (_I, [_P, _I, _P]),

@@ -694,6 +694,13 @@ int check_fft_args(int nphi, long howmany, const double *x)
 }
 
 }  // namespace
+
+int fft_prepare(int nphi)
+{
+    const double2 *tw = nullptr;
+    return get_twiddles(nphi, &tw);
+}
+
 }  // namespace raysht
 
 using namespace raysht;

@@ -96,5 +96,8 @@ int launch_plm_tables(raysht_plan *p, const double *d_x, const double *d_renorm,
                       hipStream_t st);
 int launch_stp_fp64(raysht_plan *p, const double *s, double *p2, hipStream_t st);
 int launch_pts_fp64(raysht_plan *p, const double *p2, double *s, hipStream_t st);
+// allocates the device twiddle table of length-nphi FFTs (done by the first FFT call
+// otherwise; needed before a stream capture, which may not allocate)
+int fft_prepare(int nphi);
 
 }  // namespace raysht

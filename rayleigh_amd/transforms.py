@@ -127,6 +127,11 @@ class LegendrePlan:
                                                           _dptr(s, "s", self.spectral_size), _stream(stream)),
                    "raysht_physical_to_spectral")
 
+    def round_trip_graph(self, n_phi: int, s_in, work, p3, s_out):
+        """The StP -> 2a3a -> c2r -> r2c -> 3b2b -> PtS round trip recorded into a hipGraph
+        on these buffers (kept alive by the returned object)."""
+        return RoundTripGraph(self, n_phi, s_in, work, p3, s_out)
+
     # --- setup data, for checking ------------------------------------------
     def nodes(self):
         out = [np.zeros(self.n_theta) for _ in range(4)]
@@ -144,6 +149,29 @@ class LegendrePlan:
     def tables(self):
         """(p_lm_even, p_lm_odd, ip_lm_even, ip_lm_odd) in the reference's stored orientation."""
         return tuple(self.table(w) for w in range(4))
+
+
+class RoundTripGraph:
+    """raysht_graph_create_round_trip / raysht_graph_launch."""
+
+    def __init__(self, plan: LegendrePlan, n_phi: int, s_in, work, p3, s_out):
+        n3 = (n_phi + 2) * plan.n_r * plan.n_theta * plan.n_fields
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().raysht_graph_create_round_trip(
+            ctypes.byref(h), plan.handle, int(n_phi), _dptr(s_in, "s_in", plan.spectral_size),
+            _dptr(work, "work", plan.hybrid_size), _dptr(p3, "p3", n3), _dptr(s_out, "s_out", plan.spectral_size)),
+            "raysht_graph_create_round_trip")
+        self._h = h
+        self._keep = (plan, s_in, work, p3, s_out)
+
+    def launch(self, stream=None):
+        _lib.check(_lib.lib().raysht_graph_launch(self._h, _stream(stream)), "raysht_graph_launch")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().raysht_graph_destroy(h)
+            self._h = None
 
 
 def fft_to_physical(x, n_phi: int, stream=None):

@@ -157,3 +157,34 @@ def test_cheby_vs_reference_fixture(ra, golden):
         f = empty(g["c_out"].size)
         grid.from_spectral(dev(g["c_out"]), f)
         assert rel_max(host(f), g["f_rt"]) < 1e-14
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x6"])
+def test_round_trip_graph_replays(ra, precision):
+    """The round trip recorded into a hipGraph: every replay (with eager work and
+    synchronisations between replays) equals the eager round trip bit for bit, and the
+    round trip is the identity N_phi-scaled on band-limited data."""
+    import torch
+    nt, L, nr, nf = 96, 63, 6, 3
+    nphi = 2 * nt
+    mv = O.m_order_balanced(L)
+    plan = ra.LegendrePlan(nt, L, mv, nr, nf, precision=precision)
+    s_np = O.synthetic_spectral(L, mv, nr, nf, seed=21)
+    n3 = (nphi + 2) * nr * nt * nf
+    s_in, w, p3 = dev(s_np), empty(plan.hybrid_size), empty(n3)
+    eager = empty(plan.spectral_size)
+    plan.spectral_to_physical(s_in, w, p3, nphi)
+    plan.physical_to_spectral(p3, w, eager, nphi)
+    want = host(eager)
+    # the round trip: StP's 1/2 (m > 0) and PtS's 1/N_theta-type norms give the identity
+    assert rel_max(want, s_np) < 1e-11
+    s_out = empty(plan.spectral_size)
+    g = plan.round_trip_graph(nphi, s_in, w, p3, s_out)
+    for rep in range(4):
+        s_out.fill_(float("nan"))
+        g.launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(host(s_out), want), f"replay {rep}"
+        # eager work between the replays, on the same buffers
+        plan.stp(s_in, w)
+        torch.cuda.synchronize()
