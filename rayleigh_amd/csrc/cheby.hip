@@ -172,6 +172,199 @@ __global__ __launch_bounds__(256) void cheb_from_spectral_kernel(int n, int n_x,
     }
 }
 
+// ---------------------------------------------------------------------------
+// n <= 256: the parity tables live in registers.  One wave per 16 output rows holds
+// its rows' table entries for every K (KB steps of 4 per parity: the B operand of
+// v_mfma_f64_16x16x4_f64, lane (k = 4 kk + lane/16, row = lane%16)), so a persistent
+// workgroup reads the tables once and then only streams columns: tiles of 32 columns
+// (32, or 16 for n > 128) are loaded into registers one tile ahead, staged in LDS
+// (folded / de-interleaved) and multiplied as 16-column A operands.  Output rows ride the MFMA column axis,
+// so every store walks 16 consecutive radii or coefficients of a column.
+
+
+// floor(a / b) for 0 <= a < 2^15, 1 <= b <= 512 through float: the rounding error of
+// (a + 0.5) * (1/b) stays below the 0.5/b distance to the next integer
+__device__ __forceinline__ int small_div(int a, float inv_b) { return (int)(((float)a + 0.5f) * inv_b); }
+
+// this lane's B operand entries of a [k][row] table (row stride ld): k = 4 kk + lane/16,
+// row = row0 + lane%16; zero past kmax rows or ld columns
+template <int KB>
+__device__ __forceinline__ void load_table_regs(const double *__restrict__ t, int ld, int kmax, int row0, int lane,
+                                                double (&reg)[KB])
+{
+    const int row = row0 + (lane & 15);
+#pragma unroll
+    for (int kk = 0; kk < KB; ++kk) {
+        const int k = 4 * kk + (lane >> 4);
+        reg[kk] = (k < kmax && row < ld) ? t[(long)k * ld + row] : 0.0;
+    }
+}
+
+// From_Spectral4D with K = coefficients (KB * 4 >= k4), rows = radii i < n_x.
+template <int KB, int NW, int TC>
+__global__ __launch_bounds__(64 * NW) void cheb_from_spectral_reg_kernel(int n, int n_x, int n_even, int k4, int ip,
+                                                                     long ncols, const double *__restrict__ t_e,
+                                                                     const double *__restrict__ t_o,
+                                                                     const double *__restrict__ c, double *__restrict__ f)
+{
+    extern __shared__ double lds[];
+    constexpr int LD = 4 * KB + 1;  // odd row stride: the A-operand reads hit 16 banks
+    double *ce = lds, *co = lds + TC * LD;
+    constexpr int nthr = 64 * NW;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lg = lane >> 4;
+    double te[KB], to[KB];
+    load_table_regs<KB>(t_e, ip, k4, 16 * wave, lane, te);
+    load_table_regs<KB>(t_o, ip, k4, 16 * wave, lane, to);
+    const bool odd_n = (n & 1) != 0;
+    const float inv_n = 1.0f / (float)n;
+    const long ntiles = (ncols + TC - 1) / TC;
+    const int per = TC * n;  // doubles per tile
+    constexpr int kMaxPer = TC * 2 * 4 * KB;  // per = 32 n <= 32 * 8 KB
+    constexpr int kU = (kMaxPer + nthr - 1) / nthr;
+    double pre[kU];
+    auto fetch = [&](long tile) {
+        const double *src = c + tile * TC * (long)n;
+        const long left = (ncols - tile * TC) * (long)n;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int idx = tid + u * nthr;
+            pre[u] = (idx < per && idx < left) ? src[idx] : 0.0;
+        }
+    };
+    long tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    for (; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();  // the previous tile's A reads are done
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int idx = tid + u * nthr;
+            if (idx < per) {
+                const int col = small_div(idx, inv_n), q = idx - col * n;
+                (q & 1 ? co : ce)[col * LD + (q >> 1)] = pre[u];
+            }
+        }
+        // zero K padding (coefficients past n in the last parity row of the table)
+        for (int idx = tid; idx < TC * (4 * KB); idx += nthr) {
+            const int col = idx / (4 * KB), k = idx % (4 * KB);
+            if (2 * k >= n) ce[col * LD + k] = 0.0;
+            if (2 * k + 1 >= n) co[col * LD + k] = 0.0;
+        }
+        __syncthreads();
+        if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);
+        double4_t acc_e[TC / 16], acc_o[TC / 16];
+#pragma unroll
+        for (int cb = 0; cb < TC / 16; ++cb) acc_e[cb] = acc_o[cb] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < KB; ++kk)
+#pragma unroll
+            for (int cb = 0; cb < TC / 16; ++cb) {
+                const int at = (16 * cb + li) * LD + 4 * kk + lg;
+                acc_e[cb] = mfma_f64(ce[at], te[kk], acc_e[cb]);
+                acc_o[cb] = mfma_f64(co[at], to[kk], acc_o[cb]);
+            }
+        const int i = 16 * wave + li;
+        if (i < n_x) {
+#pragma unroll
+            for (int cb = 0; cb < TC / 16; ++cb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int cl = 16 * cb + lg + 4 * v;
+                    const long col = tile * TC + cl;
+                    if (col < ncols) {
+                        const double e = acc_e[cb][v], o = acc_o[cb][v], c0h = ce[cl * LD] / 2.0;
+                        double *fc = f + col * n;
+                        if (odd_n && i == n_x - 1) {
+                            fc[i] = (e * 2.0 + o * 2.0) - c0h;
+                        } else {
+                            fc[i] = (e + o) - c0h;
+                            fc[n - 1 - i] = (e - o) - c0h;
+                        }
+                    }
+                }
+        }
+    }
+}
+
+// To_Spectral4D with K = radii i < n_x (KB * 4 >= ix4), rows = coefficient pairs k.
+template <int KB, int NW, int TC>
+__global__ __launch_bounds__(64 * NW) void cheb_to_spectral_reg_kernel(int n, int n_x, int n_even, int n_odd, int ix4,
+                                                                   int kp, long ncols, const double *__restrict__ tt_e,
+                                                                   const double *__restrict__ tt_o,
+                                                                   const double *__restrict__ f, double *__restrict__ c)
+{
+    extern __shared__ double lds[];
+    constexpr int LD = 4 * KB + 1;
+    double *fe = lds, *fo = lds + TC * LD;
+    constexpr int nthr = 64 * NW;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lg = lane >> 4;
+    double te[KB], to[KB];
+    load_table_regs<KB>(tt_e, kp, ix4, 16 * wave, lane, te);
+    load_table_regs<KB>(tt_o, kp, ix4, 16 * wave, lane, to);
+    const float inv_nx = 1.0f / (float)n_x;
+    const long ntiles = (ncols + TC - 1) / TC;
+    const int per = TC * n_x;  // folded pairs per tile
+    constexpr int kMaxPer = TC * 4 * KB;  // per = 32 n_x <= 32 * 4 KB
+    constexpr int kU = (kMaxPer + nthr - 1) / nthr;
+    double pa[kU], pb[kU];
+    auto fetch = [&](long tile) {
+        const double *src = f + tile * TC * (long)n;
+        const long cols_left = ncols - tile * TC;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int idx = tid + u * nthr;
+            const int col = small_div(idx, inv_nx), i = idx - col * n_x;
+            const bool ok = idx < per && col < cols_left;
+            pa[u] = ok ? src[(long)col * n + i] : 0.0;
+            pb[u] = ok ? src[(long)col * n + (n - 1 - i)] : 0.0;
+        }
+    };
+    const double alpha = 2.0 / (double)n;
+    long tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    for (; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int idx = tid + u * nthr;
+            if (idx < per) {
+                const int col = small_div(idx, inv_nx), i = idx - col * n_x;
+                fe[col * LD + i] = pa[u] + pb[u];
+                fo[col * LD + i] = pa[u] - pb[u];
+            }
+        }
+        for (int idx = tid; idx < TC * (4 * KB); idx += nthr) {
+            const int col = idx / (4 * KB), i = idx % (4 * KB);
+            if (i >= n_x) fe[col * LD + i] = fo[col * LD + i] = 0.0;
+        }
+        __syncthreads();
+        if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);
+        double4_t acc_e[TC / 16], acc_o[TC / 16];
+#pragma unroll
+        for (int cb = 0; cb < TC / 16; ++cb) acc_e[cb] = acc_o[cb] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < KB; ++kk)
+#pragma unroll
+            for (int cb = 0; cb < TC / 16; ++cb) {
+                const int at = (16 * cb + li) * LD + 4 * kk + lg;
+                acc_e[cb] = mfma_f64(fe[at], te[kk], acc_e[cb]);
+                acc_o[cb] = mfma_f64(fo[at], to[kk], acc_o[cb]);
+            }
+        const int k = 16 * wave + li;
+#pragma unroll
+        for (int cb = 0; cb < TC / 16; ++cb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const long col = tile * TC + 16 * cb + lg + 4 * v;
+                if (col < ncols) {
+                    if (k < n_even) c[col * n + 2 * k] = alpha * acc_e[cb][v];
+                    if (k < n_odd) c[col * n + 2 * k + 1] = alpha * acc_o[cb][v];
+                }
+            }
+    }
+}
+
 __global__ void cheb_dealias_kernel(int n, int first, long ncols, double *__restrict__ c)
 {
     const int width = n - first;
@@ -186,6 +379,45 @@ __global__ void cheb_dealias_kernel(int n, int first, long ncols, double *__rest
 }  // namespace raysht
 
 using namespace raysht;
+
+namespace raysht {
+namespace {
+constexpr int kChebRegBlocks = 1024;  // persistent workgroups (about four per CU)
+
+// register-table kernels: (KB, NW) = K steps of 4 held per parity, waves (16 rows each)
+template <int KB, int NW, int TC>
+int launch_cheb_reg(raysht_cheb *cb, bool to, const double *in, double *out, long ncols, hipStream_t st)
+{
+    const long ntiles = (ncols + TC - 1) / TC;
+    const unsigned blocks = (unsigned)std::min<long>(ntiles, kChebRegBlocks);
+    const size_t lds = sizeof(double) * 2 * TC * (4 * KB + 1);
+    if (to)
+        hipLaunchKernelGGL((cheb_to_spectral_reg_kernel<KB, NW, TC>), dim3(blocks), dim3(64 * NW), lds, st, cb->n,
+                           cb->n_x, cb->n_even, cb->n_odd, cb->ix4, cb->kp, ncols, cb->d_tt_e, cb->d_tt_o, in, out);
+    else
+        hipLaunchKernelGGL((cheb_from_spectral_reg_kernel<KB, NW, TC>), dim3(blocks), dim3(64 * NW), lds, st, cb->n,
+                           cb->n_x, cb->n_even, cb->k4, cb->ip, ncols, cb->d_t_e, cb->d_t_o, in, out);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+// returns false when n is too long for the register tables (> 256)
+bool cheb_reg_dispatch(raysht_cheb *cb, bool to, const double *in, double *out, long ncols, hipStream_t st, int *rc)
+{
+    // K steps: to = radii (ix4), from = coefficients (k4); rows: to = kp, from = ip
+    const int ksteps = (std::max(cb->ix4, cb->k4) + 3) / 4, rows = std::max(cb->kp, cb->ip);
+    if (ksteps <= 8 && rows <= 32)
+        *rc = launch_cheb_reg<8, 2, 32>(cb, to, in, out, ncols, st);
+    else if (ksteps <= 16 && rows <= 64)
+        *rc = launch_cheb_reg<16, 4, 32>(cb, to, in, out, ncols, st);
+    else if (ksteps <= 32 && rows <= 128)
+        *rc = launch_cheb_reg<32, 8, 16>(cb, to, in, out, ncols, st);
+    else
+        return false;
+    return true;
+}
+}  // namespace
+}  // namespace raysht
 
 extern "C" {
 
@@ -294,6 +526,8 @@ int raysht_cheb_to_spectral(raysht_cheb *cb, const double *f, double *c, long nc
 {
     if (!cb || ncols < 0 || (ncols > 0 && (!f || !c))) return RAYSHT_EINVAL;
     if (ncols == 0) return RAYSHT_OK;
+    int rc = RAYSHT_OK;
+    if (cheb_reg_dispatch(cb, true, f, c, ncols, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
     const int waves = cheb_waves(cb->n), tc = 16 * waves;
     const long blocks = (ncols + tc - 1) / tc;
     if (blocks > 0x7fffffffL) return RAYSHT_EINVAL;
@@ -309,6 +543,8 @@ int raysht_cheb_from_spectral(raysht_cheb *cb, const double *c, double *f, long 
 {
     if (!cb || ncols < 0 || (ncols > 0 && (!f || !c))) return RAYSHT_EINVAL;
     if (ncols == 0) return RAYSHT_OK;
+    int rc = RAYSHT_OK;
+    if (cheb_reg_dispatch(cb, false, c, f, ncols, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
     const int waves = cheb_waves(cb->n), tc = 16 * waves;
     const long blocks = (ncols + tc - 1) / tc;
     if (blocks > 0x7fffffffL) return RAYSHT_EINVAL;

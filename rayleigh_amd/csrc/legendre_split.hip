@@ -50,10 +50,12 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(4))) unsigned u32x4;  // one operand fragment, kept as 4 dwords
 
-constexpr int kThreads = 256;  // 4 waves
-constexpr int kTileC = 64;     // data columns per workgroup: two 32-column MFMA blocks per wave
-constexpr int kTileR = 128;    // output rows per workgroup: one 32-row block per wave
-constexpr int kChunkK = 256;   // K terms per FP32 level sum: 3 * 256 * 2^14 < 2^24
+constexpr int kThreads = 512;  // 8 waves, each one 32 x 32 output block
+constexpr int kTileC = 64;     // data columns per workgroup (two 32-column blocks)
+constexpr int kTileR = 128;    // StP theta rows per workgroup tile (four 32-row blocks)
+constexpr int kTileRP = 64;    // PtS parity-l rows per workgroup tile and parity (two blocks)
+constexpr int kABuf = 4 * 7 * 64;  // LDS A fragments of one step: up to 4 blocks x 7 slices
+constexpr int kChunkK = 256;   // K terms per FP32 level sum: 4 * 256 * 2^14 <= 2^24
 constexpr int kChunkSteps = kChunkK / 16;
 
 // Slice counts and kept pairs of one form; pair q enumerates levels in increasing
@@ -212,8 +214,8 @@ __global__ __launch_bounds__(64) void split_pts_table_kernel(int lmax, int nt2, 
 }
 
 // --------------------------------------------------------------- contraction core
-// One K step of 16 for one 32-column block of a wave: acc[g] += sum over the form's
-// pairs of A_s (data) x B_t (table block of this wave).
+// One K step of 16 for a wave's 32 x 32 block: acc[g] += sum over the form's pairs of
+// A_s (data) x B_t (table).
 template <class F, int Q = 0>
 __device__ __forceinline__ void mfma_pairs(f32x16 (&acc)[F::NG], const u32x4 (&a)[F::NA], const u32x4 (&b)[F::NB])
 {
@@ -225,58 +227,60 @@ __device__ __forceinline__ void mfma_pairs(f32x16 (&acc)[F::NG], const u32x4 (&a
     }
 }
 
-// adds the FP32 level sums into this thread's FP64 sums in LDS (dsum[cb*16+v][tid]) and
-// clears them
+// adds the FP32 level sums into the FP64 sums and clears them
 template <class F>
-__device__ __forceinline__ void flush(f32x16 (&acc)[2][F::NG], double (*dsum)[kThreads])
+__device__ __forceinline__ void flush(f32x16 (&acc)[F::NG], double (&dacc)[16])
 {
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
+    for (int v = 0; v < 16; ++v) {
+        double sum = 0.0;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            double sum = 0.0;
+        for (int g = F::NG - 1; g >= 0; --g) sum += (double)acc[g][v];
+        dacc[v] += sum;
+    }
 #pragma unroll
-            for (int g = F::NG - 1; g >= 0; --g) sum += (double)acc[cb][g][v];
-            dsum[cb * 16 + v][threadIdx.x] += sum;
-        }
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-        for (int g = 0; g < F::NG; ++g) acc[cb][g] = f32x16{};
+    for (int g = 0; g < F::NG; ++g) acc[g] = f32x16{};
 }
 
-// this thread's FP64 sums in LDS, cleared
-__device__ __forceinline__ void clear_sums(double (*dsum)[kThreads])
+// Slice s of x, both relative to the row/column scale (|x| < 2^-8s): the nearest
+// multiple of 2^(-7-8s), by the add-and-subtract of 1.5 * 2^(45-8s), is removed from x
+// and returned as its BF16 bit pattern (at most 8 significant bits: exact).
+__device__ __forceinline__ unsigned grid_slice(double &x, int s)
 {
-#pragma unroll
-    for (int i = 0; i < 32; ++i) dsum[i][threadIdx.x] = 0.0;
+    const double c = __builtin_ldexp(1.5, 45 - 8 * s);
+    const double v = (x + c) - c;
+    x -= v;
+    return __builtin_bit_cast(unsigned, (float)v) >> 16;
 }
 
-// Writes the NA slices of 4 consecutive K values of data column ca (0..63) into the A
-// fragments of one LDS buffer: column block ca/32, fragment lane ca%32 + 32 (q/2),
-// elements 4 (q%2) .. +3, where q (0..3) is the quarter of the K step.
+// LDS slot of MFMA fragment lane L: lanes 32..63 are swapped in groups of 8, so that
+// the two halves of a 16-K step (same rows, L and L + 32) land in different banks when
+// one thread writes a pair of K values; the 16-lane groups of a 16-byte read still
+// cover distinct banks.
+__device__ __forceinline__ int frag_slot(int L) { return L ^ ((L >> 5) << 3); }
+
+// Writes the NA slices of two consecutive K values of one data column into the A
+// fragments of an LDS buffer laid out [block][slice][slot][8 BF16]: fragment lane
+// L = row % 32 + 32 * (k0 / 8), elements k0 % 8 and k0 % 8 + 1 (k0 even, within the step).
 template <int NA>
-__device__ __forceinline__ void put_slices(u32x4 *__restrict__ buf, int ca, int q, const double (&v)[4], int e)
+__device__ __forceinline__ void put_pair(u32x4 *__restrict__ buf, int blk, int row, int k0, double v0, double v1,
+                                         int e)
 {
-    double x[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) x[i] = __builtin_ldexp(v[i], -e);
-    const int cb = ca >> 5, L = (ca & 31) + 32 * (q >> 1);
-    uint2 *b2 = reinterpret_cast<uint2 *>(buf);
+    double x0 = __builtin_ldexp(v0, -e), x1 = __builtin_ldexp(v1, -e);
+    unsigned *b32 = reinterpret_cast<unsigned *>(buf);
+    const int at = ((blk * NA) * 64 + frag_slot((row & 31) + 32 * (k0 >> 3))) * 4 + ((k0 & 7) >> 1);
 #pragma unroll
     for (int s = 0; s < NA; ++s) {
-        unsigned h[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) h[i] = next_slice(x[i], s);
-        b2[(((cb * NA + s) << 6) + L) * 2 + (q & 1)] = uint2{h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
+        const unsigned h0 = grid_slice(x0, s), h1 = grid_slice(x1, s);
+        b32[at + s * 256] = h0 | (h1 << 16);
     }
 }
 
 template <int NA>
-__device__ __forceinline__ void get_a(const u32x4 *__restrict__ buf, int cb, int lane, u32x4 (&a)[NA])
+__device__ __forceinline__ void get_a(const u32x4 *__restrict__ buf, int blk, int lane, u32x4 (&a)[NA])
 {
 #pragma unroll
-    for (int s = 0; s < NA; ++s) a[s] = buf[((cb * NA + s) << 6) + lane];
+    for (int s = 0; s < NA; ++s) a[s] = buf[((blk * NA + s) << 6) + frag_slot(lane)];
 }
 
 template <int NB>
@@ -291,154 +295,161 @@ __device__ __forceinline__ void get_b(const u32x4 *__restrict__ frag, int lane, 
     }
 }
 
-// The K loop of one contraction: nks steps of 16, with the data of step ks delivered
-// by load_a(ks, v) (4 values of this thread's column quarter) and the table fragments
-// of step ks at tb + ks * NB fragments.  Adds the result into this thread's FP64 sums
-// in LDS.  nks is uniform over the workgroup; every step ends with one barrier.
-template <class F, class LoadA>
-__device__ __forceinline__ void k_loop(const LoadA &load_a, int nks, const u32x4 *__restrict__ tb, bool active,
-                                       u32x4 (*abuf)[2 * F::NA * 64], int ca, int q, int ea, int lane,
-                                       double (*dsum)[kThreads])
+// Raw FP64 data of one K step for one thread (loaded two steps ahead of its slicing).
+struct raw_step {
+    double2 a, b;
+};
+
+// The K loop of one contraction: nks steps of 16.  fetch(ks) issues the loads of step
+// ks's raw data (steps past the end read masked zeros) and slice(raw, buffer) writes
+// its A fragments into an LDS buffer; step ks's fragments are in buffer ks % 2 before
+// its MFMAs.  Raw data run two steps ahead of the MFMAs and table fragments (at tb +
+// ks * NB fragments) one step, so their latency hides behind a step of MFMAs.  Wave w
+// multiplies A block ablk of the buffer by its table fragments; the result is added
+// into dacc (FP32 level sums moved there every kChunkSteps steps).  nks is uniform
+// over the workgroup; every step ends with one barrier.
+template <class F, class Fetch, class Slice>
+__device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, int nks, const u32x4 *__restrict__ tb,
+                                       bool active, u32x4 (*abuf)[kABuf], int ablk, int lane, double (&dacc)[16])
 {
     if (nks <= 0) return;
-    {
-        double v[4];
-        load_a(0, v);
-        put_slices<F::NA>(abuf[0], ca, q, v, ea);
-    }
+    slice(fetch(0), abuf[0]);
+    raw_step r1 = fetch(1);
     u32x4 b[F::NB];
     get_b<F::NB>(tb, lane, active, b);
     __syncthreads();
-    f32x16 acc[2][F::NG];
+    f32x16 acc[F::NG];
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-        for (int g = 0; g < F::NG; ++g) acc[cb][g] = f32x16{};
+    for (int g = 0; g < F::NG; ++g) acc[g] = f32x16{};
     for (int kc = 0; kc < nks; kc += kChunkSteps) {
         const int kend = min(nks, kc + kChunkSteps);
         for (int ks = kc; ks < kend; ++ks) {
-            double vn[4];
-            u32x4 bn[F::NB];
             const bool more = ks + 1 < nks;
-            if (more) {
-                load_a(ks + 1, vn);
-                get_b<F::NB>(tb + (((long)(ks + 1) * F::NB) << 6), lane, active, bn);
-            }
-            // unconditional: an inactive wave multiplies zero table fragments (a branch
-            // here would make the accumulators leave the MFMA registers every step)
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
+            const raw_step r2 = fetch(ks + 2);
+            u32x4 bn[F::NB];
+            if (more) get_b<F::NB>(tb + (((long)(ks + 1) * F::NB) << 6), lane, active, bn);
+            {
+                // unconditional: an inactive wave multiplies zero table fragments (a branch
+                // here would make the accumulators leave the MFMA registers every step)
                 u32x4 a[F::NA];
-                get_a<F::NA>(abuf[ks & 1], cb, lane, a);
-                mfma_pairs<F>(acc[cb], a, b);
+                get_a<F::NA>(abuf[ks & 1], ablk, lane, a);
+                mfma_pairs<F>(acc, a, b);
             }
             if (more) {
-                put_slices<F::NA>(abuf[(ks + 1) & 1], ca, q, vn, ea);
+                slice(r1, abuf[(ks + 1) & 1]);
 #pragma unroll
                 for (int t = 0; t < F::NB; ++t) b[t] = bn[t];
             }
+            r1 = r2;
             __syncthreads();
         }
-        flush<F>(acc, dsum);
+        flush<F>(acc, dacc);
     }
 }
 
 // --------------------------------------------------------------- StP
-// Work item (mp, c0): 64 columns, every theta row, in tiles of 128 rows (wave w owns
-// theta block jb = j0/32 + w) with both parities each.  The spectral tile is read as
-// 16-byte pairs (l, l+1) and each parity pass keeps its half; the column exponents come
-// from one pass over the tile.
+// Work item (mp, c0): 64 columns, every theta row, in tiles of 128 rows with both
+// parities each.  Eight waves: wave w multiplies column block w % 2 by theta block
+// j0/32 + w/2.  The spectral tile is read as 16-byte pairs (l, l+1); each parity pass
+// slices its half, two values per thread and step.  The column exponents come from one
+// pass over the tile; the scaled even-parity sums wait in LDS for the odd pass.
 template <class F>
 __global__ __launch_bounds__(kThreads, 1) void stp_split_kernel(
     const work_item *__restrict__ work, int lmax, int nt, int nt2, int nrhs, const int *__restrict__ mvals,
     const long *__restrict__ s_off, const double *__restrict__ s, const long *__restrict__ frag_off,
     const u32x4 *__restrict__ tdig, const int *__restrict__ texp, double *__restrict__ p2)
 {
-    __shared__ u32x4 abuf[2][2 * F::NA * 64];
-    __shared__ double sums[2][2 * 16][kThreads];  // FP64 sums of each thread, per parity
+    __shared__ u32x4 abuf[2][kABuf];
+    __shared__ double ebuf[16][kThreads];
     __shared__ int cexp[kTileC];
     const work_item w = work[blockIdx.x];
     if (w.mp < 0) return;
     const int mp = w.mp, m = mvals[mp], nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const double *sb = s + s_off[mp];
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(sb, (long)nl * nrhs * 8);
 
-    // column exponents: the four threads of a column read it in the main loop's
-    // pattern (l = 32 i + 8 q .. +7), four steps of loads in flight at a time
-    const int ca = threadIdx.x >> 2, q = threadIdx.x & 3;
+    // producer role of the thread: column ca, l = 32 i + 4 q .. +3 of step i
+    const int ca = tid >> 3, q = tid & 7;
     const unsigned a_off = (unsigned)min(w.c0 + ca, nrhs) * nl * 8;  // column nrhs and beyond read as 0
     {
         double mx = 0.0;
         const int n32 = cdiv(nl, 32);
         for (int i0 = 0; i0 < n32; i0 += 4) {
-            double2 d[4][4];
+            double2 d[4][2];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    d[u][e] = load_d2(rs, a_off + (unsigned)(32 * (i0 + u) + 8 * q + 2 * e) * 8);
+                for (int e = 0; e < 2; ++e) d[u][e] = load_d2(rs, a_off + (unsigned)(32 * (i0 + u) + 4 * q + 2 * e) * 8);
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int l = 32 * (i0 + u) + 8 * q + 2 * e;
+                for (int e = 0; e < 2; ++e) {
+                    const int l = 32 * (i0 + u) + 4 * q + 2 * e;
                     if (l < nl) mx = fmax(mx, fabs(d[u][e].x));
                     if (l + 1 < nl) mx = fmax(mx, fabs(d[u][e].y));
                 }
         }
-        mx = fmax(mx, __shfl_xor(mx, 1, 64));
-        mx = fmax(mx, __shfl_xor(mx, 2, 64));
+#pragma unroll
+        for (int off = 1; off < 8; off <<= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
         if (q == 0) cexp[ca] = scale_exponent(mx);
     }
     __syncthreads();
     const int ea = cexp[ca];
-    const int njb = cdiv(nt2, 32), nks_e = cdiv(ne, 16);
+    const int cb = wave & 1, njb = cdiv(nt2, 32), nks_e = cdiv(ne, 16);
     double *ob = p2 + (long)mp * nrhs * nt;
+
     for (int r0 = 0; r0 < nt2; r0 += kTileR) {
-        const int jb = r0 / 32 + wave;
+        const int jb = r0 / 32 + (wave >> 1);
         const bool active = jb < njb;
         const int j = jb * 32 + (lane & 31);
 #pragma unroll
         for (int par = 0; par < 2; ++par) {
-            clear_sums(sums[par]);
+            double dacc[16];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
             const int K = par ? no : ne, nks = cdiv(K, 16);
             const u32x4 *tb =
                 tdig + ((frag_off[mp] + (par ? (long)njb * nks_e * F::NB : 0) + (long)jb * nks * F::NB) << 6);
-            // raw data of one K step: l = 32 ks + 8 q .. +7 as four (even, odd) pairs
-            auto load_a = [&](int ks, double (&v)[4]) {
-                const int l0 = 32 * ks + 8 * q;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const double2 d = load_d2(rs, a_off + (unsigned)(l0 + 2 * i) * 8);
-                    v[i] = l0 + 2 * i + par < nl ? (par ? d.y : d.x) : 0.0;
-                }
+            // step ks: parity-k = 16 ks + 2 q + {0, 1}, i.e. l = 32 ks + 4 q + {0, 2} + par
+            // (l >= nl is the next column or past the block: masked to zero)
+            auto fetch = [&](int ks) {
+                const int l0 = 32 * ks + 4 * q;
+                raw_step r{load_d2(rs, a_off + (unsigned)l0 * 8), load_d2(rs, a_off + (unsigned)(l0 + 2) * 8)};
+                if (l0 >= nl) r.a.x = 0.0;
+                if (l0 + 1 >= nl) r.a.y = 0.0;
+                if (l0 + 2 >= nl) r.b.x = 0.0;
+                if (l0 + 3 >= nl) r.b.y = 0.0;
+                return r;
             };
-            k_loop<F>(load_a, nks, tb, active, abuf, ca, q, ea, lane, sums[par]);
-        }
-        if (active && j < nt2) {
-            const int ej = texp[((long)mp * 2) * nt2 + j], oj = texp[((long)mp * 2 + 1) * nt2 + j];
+            auto slice = [&](const raw_step &r, u32x4 *buf) {
+                put_pair<F::NA>(buf, ca >> 5, ca, 2 * q, par ? r.a.y : r.a.x, par ? r.b.y : r.b.x, ea);
+            };
+            k_loop<F>(fetch, slice, nks, tb, active, abuf, cb, lane, dacc);
+            const bool store = active && j < nt2;
+            const int ej = store ? texp[((long)mp * 2 + par) * nt2 + j] : 0;
 #pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = w.c0 + cc;
-                    if (c < nrhs) {
-                        const double e = __builtin_ldexp(sums[0][cb * 16 + v][threadIdx.x], cexp[cc] + ej);
-                        const double o = __builtin_ldexp(sums[1][cb * 16 + v][threadIdx.x], cexp[cc] + oj);
-                        ob[(long)c * nt + j] = e + o;
-                        ob[(long)c * nt + (nt - 1 - j)] = e - o;
-                    }
+            for (int v = 0; v < 16; ++v) {
+                const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = w.c0 + cc;
+                const double r = __builtin_ldexp(dacc[v], cexp[cc] + ej);
+                if (par == 0) {
+                    ebuf[v][tid] = r;
+                } else if (store && c < nrhs) {
+                    const double e = ebuf[v][tid];
+                    ob[(long)c * nt + j] = e + r;
+                    ob[(long)c * nt + (nt - 1 - j)] = e - r;
                 }
+            }
         }
     }
 }
 
 // --------------------------------------------------------------- PtS
 // Work item (mp, c0): 64 columns, every parity-l row of both parities, in row tiles of
-// 128 (wave w owns row block kb = k0/32 + w).  The data operand is the fold
-// f(j) + f(nt-1-j) (even) or f(j) - f(nt-1-j) (odd) of the hybrid columns; both folds'
-// column exponents come from one pass over the tile.
+// 64.  Eight waves: wave w multiplies column block w % 2 of parity w / 4 by row block
+// k0/32 + (w / 2) % 2.  The data operand is the fold f(j) + f(nt-1-j) (even) or
+// f(j) - f(nt-1-j) (odd) of the hybrid columns; each thread slices both folds of two
+// nodes per step, and both folds' column exponents come from one pass over the tile.
 template <class F>
 __global__ __launch_bounds__(kThreads, 1) void pts_split_kernel(
     const work_item *__restrict__ work, int lmax, int nt, int nt2, int nrhs, const int *__restrict__ mvals,
@@ -446,89 +457,79 @@ __global__ __launch_bounds__(kThreads, 1) void pts_split_kernel(
     const long *__restrict__ exp_off, const u32x4 *__restrict__ tdig, const int *__restrict__ texp,
     double *__restrict__ s)
 {
-    __shared__ u32x4 abuf[2][2 * F::NA * 64];
-    __shared__ double sums[2 * 16][kThreads];  // FP64 sums of each thread
+    __shared__ u32x4 abuf[2][kABuf];
     __shared__ int cexp[2][kTileC];
     const work_item w = work[blockIdx.x];
     if (w.mp < 0) return;
     const int mp = w.mp, m = mvals[mp], nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const double *pb = p2 + (long)mp * nrhs * nt;
     const __amdgpu_buffer_rsrc_t rp = make_rsrc(pb, (long)nrhs * nt * 8);
-    const int ca = threadIdx.x >> 2, q = threadIdx.x & 3;
+    const int ca = tid >> 3, q = tid & 7;
     const unsigned a_off = (unsigned)min(w.c0 + ca, nrhs) * nt * 8;
     const int njs = cdiv(nt2, 16);
 
-    // raw data of one K step: f(j) and f(nt-1-j) for j = 16 js + 4 q .. +3 (nt-4-j0 < 0
-    // only past the last valid j, where the offset wraps past the end or into a column
-    // whose values are masked)
-    auto load_raw = [&](int js, double (&a)[4], double (&b)[4]) {
-        const int j0 = 16 * js + 4 * q;
-        const double2 f0 = load_d2(rp, a_off + j0 * 8u), f1 = load_d2(rp, a_off + (j0 + 2) * 8u);
-        const double2 g0 = load_d2(rp, a_off + (unsigned)(nt - 2 - j0) * 8u);  // f(nt-2-j0), f(nt-1-j0)
-        const double2 g1 = load_d2(rp, a_off + (unsigned)(nt - 4 - j0) * 8u);  // f(nt-4-j0), f(nt-3-j0)
-        a[0] = f0.x, a[1] = f0.y, a[2] = f1.x, a[3] = f1.y;
-        b[0] = g0.y, b[1] = g0.x, b[2] = g1.y, b[3] = g1.x;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if (j0 + i >= nt2) a[i] = b[i] = 0.0;
+    // raw data of step js: a = (f(j), f(j+1)) and b = (f(nt-1-j), f(nt-2-j)) for
+    // j = 16 js + 2 q, zero for j >= nt2 (the mirrored offset is negative only past the
+    // last valid j, where it wraps past the end or into a column whose values are masked)
+    auto fetch = [&](int js) {
+        const int j0 = 16 * js + 2 * q;
+        const double2 f = load_d2(rp, a_off + j0 * 8u);
+        const double2 g = load_d2(rp, a_off + (unsigned)(nt - 2 - j0) * 8u);  // f(nt-2-j0), f(nt-1-j0)
+        raw_step r{f, make_double2(g.y, g.x)};
+        if (j0 >= nt2) r.a.x = r.b.x = 0.0;
+        if (j0 + 1 >= nt2) r.a.y = r.b.y = 0.0;
+        return r;
     };
-    // exponents of both folds, four steps of loads in flight at a time
     {
         double me = 0.0, mo = 0.0;
         for (int j0 = 0; j0 < njs; j0 += 4) {
-            double a[4][4], b[4][4];
+            raw_step r[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) load_raw(j0 + u, a[u], b[u]);  // steps past njs read zeros
+            for (int u = 0; u < 4; ++u) r[u] = fetch(j0 + u);  // steps past njs read zeros
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    me = fmax(me, fabs(a[u][i] + b[u][i]));
-                    mo = fmax(mo, fabs(a[u][i] - b[u][i]));
-                }
+            for (int u = 0; u < 4; ++u) {
+                me = fmax(me, fmax(fabs(r[u].a.x + r[u].b.x), fabs(r[u].a.y + r[u].b.y)));
+                mo = fmax(mo, fmax(fabs(r[u].a.x - r[u].b.x), fabs(r[u].a.y - r[u].b.y)));
+            }
         }
-        me = fmax(me, __shfl_xor(me, 1, 64));
-        me = fmax(me, __shfl_xor(me, 2, 64));
-        mo = fmax(mo, __shfl_xor(mo, 1, 64));
-        mo = fmax(mo, __shfl_xor(mo, 2, 64));
+#pragma unroll
+        for (int off = 1; off < 8; off <<= 1) {
+            me = fmax(me, __shfl_xor(me, off, 64));
+            mo = fmax(mo, __shfl_xor(mo, off, 64));
+        }
         if (q == 0) {
             cexp[0][ca] = scale_exponent(me);
             cexp[1][ca] = scale_exponent(mo);
         }
     }
     __syncthreads();
+    const int ee = cexp[0][ca], eo = cexp[1][ca];
+    const int par = wave >> 2, cb = wave & 1, K_rows = par ? no : ne;
+    const long region = frag_off[mp] + (par ? (long)cdiv(ne, 32) * njs * F::NB : 0);
     double *sb = s + s_off[mp];
+    // A blocks of a buffer: [parity][column block]
+    auto slice = [&](const raw_step &r, u32x4 *buf) {
+        put_pair<F::NA>(buf, ca >> 5, ca, 2 * q, r.a.x + r.b.x, r.a.y + r.b.y, ee);
+        put_pair<F::NA>(buf, 2 + (ca >> 5), ca, 2 * q, r.a.x - r.b.x, r.a.y - r.b.y, eo);
+    };
 
+    for (int k0 = 0; k0 < ne; k0 += kTileRP) {
+        const int kb = k0 / 32 + ((wave >> 1) & 1);
+        const bool active = kb * 32 < K_rows;
+        const u32x4 *tb = tdig + ((region + (long)kb * njs * F::NB) << 6);
+        double dacc[16];
 #pragma unroll
-    for (int par = 0; par < 2; ++par) {
-        const int K_rows = par ? no : ne, ea = cexp[par][ca];
-        auto load_a = [&](int js, double (&v)[4]) {
-            double a[4], b[4];
-            load_raw(js, a, b);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = par ? a[i] - b[i] : a[i] + b[i];
-        };
-        const long region = frag_off[mp] + (par ? (long)cdiv(ne, 32) * njs * F::NB : 0);
-        for (int k0 = 0; k0 < K_rows; k0 += kTileR) {
-            const int kb = k0 / 32 + wave;
-            const bool active = kb * 32 < K_rows;
-            const u32x4 *tb = tdig + ((region + (long)kb * njs * F::NB) << 6);
-            clear_sums(sums);
-            k_loop<F>(load_a, njs, tb, active, abuf, ca, q, ea, lane, sums);
+        for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
+        k_loop<F>(fetch, slice, njs, tb, active, abuf, 2 * par + cb, lane, dacc);
 
-            const int k = kb * 32 + (lane & 31);
-            if (active && k < K_rows) {
-                const int ek = texp[exp_off[mp] + (par ? ne : 0) + k];
+        const int k = kb * 32 + (lane & 31);
+        if (active && k < K_rows) {
+            const int ek = texp[exp_off[mp] + (par ? ne : 0) + k];
 #pragma unroll
-                for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-                    for (int v = 0; v < 16; ++v) {
-                        const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = w.c0 + cc;
-                        if (c < nrhs)
-                            sb[(long)c * nl + 2 * k + par] =
-                                __builtin_ldexp(sums[cb * 16 + v][threadIdx.x], cexp[par][cc] + ek);
-                    }
+            for (int v = 0; v < 16; ++v) {
+                const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = w.c0 + cc;
+                if (c < nrhs) sb[(long)c * nl + 2 * k + par] = __builtin_ldexp(dacc[v], cexp[par][cc] + ek);
             }
         }
     }
@@ -550,7 +551,7 @@ std::vector<work_item> split_work(const raysht_plan *p, bool stp)
         const int m = p->mvals[mp], nl = p->lmax - m + 1, ne = (nl + 1) / 2, no = nl / 2;
         // K steps per item (plus the exponent pass)
         const double steps = stp ? (double)nrt * (cdiv(ne, 16) + cdiv(no, 16)) + cdiv(nl, 32)
-                                 : (double)(cdiv(ne, kTileR) + cdiv(no, kTileR)) * njs + njs;
+                                 : (double)cdiv(ne, kTileRP) * njs + njs;
         const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         for (int c = 0; c < nct; ++c) q[x].push_back(work_item{mp, 0, c * kTileC, 0});
         load[x] += steps * nct;
