@@ -227,6 +227,29 @@ __device__ __forceinline__ void mfma_pairs(f32x16 (&acc)[F::NG], const u32x4 (&a
     }
 }
 
+// mfma_pairs with the NA slicing steps of the next K step spread evenly between the
+// MFMAs (slice s after pair s * npairs / NA), in source order
+template <class F>
+constexpr int slice_at(int q)
+{
+    for (int s = 0; s < F::NA; ++s)
+        if (s * F::npairs() / F::NA == q) return s;
+    return -1;
+}
+
+template <class F, class SL, int Q = 0>
+__device__ __forceinline__ void mfma_pairs_sliced(f32x16 (&acc)[F::NG], const u32x4 (&a)[F::NA],
+                                                  const u32x4 (&b)[F::NB], SL &sl)
+{
+    if constexpr (Q < F::npairs()) {
+        constexpr int s = F::pair(Q, true), t = F::pair(Q, false), g = F::group(Q), sc = slice_at<F>(Q);
+        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[s]), __builtin_bit_cast(bf16x8, b[t]),
+                                                         acc[g], 0, 0, 0);
+        if constexpr (sc >= 0) sl.step(sc);
+        mfma_pairs_sliced<F, SL, Q + 1>(acc, a, b, sl);
+    }
+}
+
 // adds the FP32 level sums into the FP64 sums and clears them
 template <class F>
 __device__ __forceinline__ void flush(f32x16 (&acc)[F::NG], double (&dacc)[16])
@@ -276,6 +299,37 @@ __device__ __forceinline__ void put_pair(u32x4 *__restrict__ buf, int blk, int r
     }
 }
 
+// The slicing of two consecutive K values into NA slices, one slice per step() call,
+// so that the calls can be spread between the MFMAs of the previous step.
+template <int NA>
+struct pair_slicer {
+    double x0, x1;
+    unsigned *p;
+    __device__ __forceinline__ void init(u32x4 *__restrict__ buf, int blk, int row, int k0, double v0, double v1,
+                                         int e)
+    {
+        x0 = __builtin_ldexp(v0, -e);
+        x1 = __builtin_ldexp(v1, -e);
+        p = reinterpret_cast<unsigned *>(buf) + ((blk * NA) * 64 + frag_slot((row & 31) + 32 * (k0 >> 3))) * 4 +
+            ((k0 & 7) >> 1);
+    }
+    __device__ __forceinline__ void step(int s)
+    {
+        const unsigned h0 = grid_slice(x0, s), h1 = grid_slice(x1, s);
+        p[s * 256] = h0 | (h1 << 16);
+    }
+};
+
+template <int NA, int N>
+struct slicers {
+    pair_slicer<NA> sl[N];
+    __device__ __forceinline__ void step(int s)
+    {
+#pragma unroll
+        for (int i = 0; i < N; ++i) sl[i].step(s);
+    }
+};
+
 template <int NA>
 __device__ __forceinline__ void get_a(const u32x4 *__restrict__ buf, int blk, int lane, u32x4 (&a)[NA])
 {
@@ -283,40 +337,62 @@ __device__ __forceinline__ void get_a(const u32x4 *__restrict__ buf, int blk, in
     for (int s = 0; s < NA; ++s) a[s] = buf[((blk * NA + s) << 6) + frag_slot(lane)];
 }
 
+// table fragments f .. f + NB - 1 of an m-block's region (buffer resource over the
+// region: fragments past its end read as zeros)
 template <int NB>
-__device__ __forceinline__ void get_b(const u32x4 *__restrict__ frag, int lane, bool active, u32x4 (&b)[NB])
+__device__ __forceinline__ void get_b(__amdgpu_buffer_rsrc_t rt, int f, int lane, u32x4 (&b)[NB])
 {
-    if (active) {
 #pragma unroll
-        for (int t = 0; t < NB; ++t) b[t] = frag[(t << 6) + lane];
-    } else {
-#pragma unroll
-        for (int t = 0; t < NB; ++t) b[t] = u32x4{0u, 0u, 0u, 0u};
-    }
+    for (int t = 0; t < NB; ++t)
+        b[t] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rt, (((f + t) << 6) + lane) << 4, 0, 0));
 }
 
-// Raw FP64 data of one K step for one thread (loaded two steps ahead of its slicing).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_frag_rsrc(const u32x4 *base, long frags)
+{
+    const long n = frags * 1024 < 0x7fffffffL ? frags * 1024 : 0x7fffffffL;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<u32x4 *>(base), (short)0, (int)n, 0x00020000);
+}
+
+// Raw FP64 data of one K step for one thread (loaded two steps ahead of its slicing),
+// with the bit mask of the values that exist (applied when the data are sliced, so the
+// loads are not waited for until then).
 struct raw_step {
     double2 a, b;
+    unsigned valid;
+    __device__ __forceinline__ double2 va() const
+    {
+        return make_double2(valid & 1 ? a.x : 0.0, valid & 2 ? a.y : 0.0);
+    }
+    __device__ __forceinline__ double2 vb() const
+    {
+        return make_double2(valid & 4 ? b.x : 0.0, valid & 8 ? b.y : 0.0);
+    }
 };
 
 // The K loop of one contraction: nks steps of 16.  fetch(ks) issues the loads of step
-// ks's raw data (steps past the end read masked zeros) and slice(raw, buffer) writes
-// its A fragments into an LDS buffer; step ks's fragments are in buffer ks % 2 before
-// its MFMAs.  Raw data run two steps ahead of the MFMAs and table fragments (at tb +
-// ks * NB fragments) one step, so their latency hides behind a step of MFMAs.  Wave w
-// multiplies A block ablk of the buffer by its table fragments; the result is added
-// into dacc (FP32 level sums moved there every kChunkSteps steps).  nks is uniform
+// ks's raw data (steps past the end read masked zeros) and slice(raw, buffer) returns
+// the slicers that write its A fragments into an LDS buffer, one slice per step() call
+// (spread between the MFMAs of the step before); step ks's fragments are in buffer
+// ks % 2 before its MFMAs.  Raw data run two steps ahead of the MFMAs and table fragments (fragment
+// f0 + ks * NB of resource rt) one step, so their latency hides behind a step of
+// MFMAs.  Wave w multiplies A block ablk of the buffer by its table fragments; the
+// result is added into dacc (FP32 level sums moved there every kChunkSteps steps).
+// The step body has no branches (the last step slices and loads harmlessly past the
+// end), so one wave keeps its matrix pipe and its vector ALU busy at once.  nks is uniform
 // over the workgroup; every step ends with one barrier.
 template <class F, class Fetch, class Slice>
-__device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, int nks, const u32x4 *__restrict__ tb,
-                                       bool active, u32x4 (*abuf)[kABuf], int ablk, int lane, double (&dacc)[16])
+__device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, int nks, __amdgpu_buffer_rsrc_t rt,
+                                       int f0, u32x4 (*abuf)[kABuf], int ablk, int lane, double (&dacc)[16])
 {
     if (nks <= 0) return;
-    slice(fetch(0), abuf[0]);
+    {
+        auto sl = slice(fetch(0), abuf[0]);
+#pragma unroll
+        for (int sc = 0; sc < F::NA; ++sc) sl.step(sc);
+    }
     raw_step r1 = fetch(1);
     u32x4 b[F::NB];
-    get_b<F::NB>(tb, lane, active, b);
+    get_b<F::NB>(rt, f0, lane, b);
     __syncthreads();
     f32x16 acc[F::NG];
 #pragma unroll
@@ -324,22 +400,15 @@ __device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, i
     for (int kc = 0; kc < nks; kc += kChunkSteps) {
         const int kend = min(nks, kc + kChunkSteps);
         for (int ks = kc; ks < kend; ++ks) {
-            const bool more = ks + 1 < nks;
             const raw_step r2 = fetch(ks + 2);
             u32x4 bn[F::NB];
-            if (more) get_b<F::NB>(tb + (((long)(ks + 1) * F::NB) << 6), lane, active, bn);
-            {
-                // unconditional: an inactive wave multiplies zero table fragments (a branch
-                // here would make the accumulators leave the MFMA registers every step)
-                u32x4 a[F::NA];
-                get_a<F::NA>(abuf[ks & 1], ablk, lane, a);
-                mfma_pairs<F>(acc, a, b);
-            }
-            if (more) {
-                slice(r1, abuf[(ks + 1) & 1]);
+            get_b<F::NB>(rt, f0 + (ks + 1) * F::NB, lane, bn);
+            u32x4 a[F::NA];
+            get_a<F::NA>(abuf[ks & 1], ablk, lane, a);
+            auto sl = slice(r1, abuf[(ks + 1) & 1]);
+            mfma_pairs_sliced<F>(acc, a, b, sl);
 #pragma unroll
-                for (int t = 0; t < F::NB; ++t) b[t] = bn[t];
-            }
+            for (int t = 0; t < F::NB; ++t) b[t] = bn[t];
             r1 = r2;
             __syncthreads();
         }
@@ -368,6 +437,7 @@ __global__ __launch_bounds__(kThreads, 1) void stp_split_kernel(
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const double *sb = s + s_off[mp];
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(sb, (long)nl * nrhs * 8);
+    const __amdgpu_buffer_rsrc_t rt = make_frag_rsrc(tdig + (frag_off[mp] << 6), frag_off[mp + 1] - frag_off[mp]);
 
     // producer role of the thread: column ca, l = 32 i + 4 q .. +3 of step i
     const int ca = tid >> 3, q = tid & 7;
@@ -409,23 +479,22 @@ __global__ __launch_bounds__(kThreads, 1) void stp_split_kernel(
 #pragma unroll
             for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
             const int K = par ? no : ne, nks = cdiv(K, 16);
-            const u32x4 *tb =
-                tdig + ((frag_off[mp] + (par ? (long)njb * nks_e * F::NB : 0) + (long)jb * nks * F::NB) << 6);
+            const int f0 = (par ? njb * nks_e * F::NB : 0) + jb * nks * F::NB;
             // step ks: parity-k = 16 ks + 2 q + {0, 1}, i.e. l = 32 ks + 4 q + {0, 2} + par
             // (l >= nl is the next column or past the block: masked to zero)
             auto fetch = [&](int ks) {
                 const int l0 = 32 * ks + 4 * q;
-                raw_step r{load_d2(rs, a_off + (unsigned)l0 * 8), load_d2(rs, a_off + (unsigned)(l0 + 2) * 8)};
-                if (l0 >= nl) r.a.x = 0.0;
-                if (l0 + 1 >= nl) r.a.y = 0.0;
-                if (l0 + 2 >= nl) r.b.x = 0.0;
-                if (l0 + 3 >= nl) r.b.y = 0.0;
-                return r;
+                const unsigned valid = (l0 < nl) | (l0 + 1 < nl) << 1 | (l0 + 2 < nl) << 2 | (l0 + 3 < nl) << 3;
+                return raw_step{load_d2(rs, a_off + (unsigned)l0 * 8), load_d2(rs, a_off + (unsigned)(l0 + 2) * 8),
+                                valid};
             };
             auto slice = [&](const raw_step &r, u32x4 *buf) {
-                put_pair<F::NA>(buf, ca >> 5, ca, 2 * q, par ? r.a.y : r.a.x, par ? r.b.y : r.b.x, ea);
+                const double2 a = r.va(), b = r.vb();
+                slicers<F::NA, 1> sl;
+                sl.sl[0].init(buf, ca >> 5, ca, 2 * q, par ? a.y : a.x, par ? b.y : b.x, ea);
+                return sl;
             };
-            k_loop<F>(fetch, slice, nks, tb, active, abuf, cb, lane, dacc);
+            k_loop<F>(fetch, slice, nks, rt, f0, abuf, cb, lane, dacc);
             const bool store = active && j < nt2;
             const int ej = store ? texp[((long)mp * 2 + par) * nt2 + j] : 0;
 #pragma unroll
@@ -474,23 +543,22 @@ __global__ __launch_bounds__(kThreads, 1) void pts_split_kernel(
     // last valid j, where it wraps past the end or into a column whose values are masked)
     auto fetch = [&](int js) {
         const int j0 = 16 * js + 2 * q;
+        const unsigned valid = (j0 < nt2 ? 5u : 0u) | (j0 + 1 < nt2 ? 10u : 0u);
         const double2 f = load_d2(rp, a_off + j0 * 8u);
         const double2 g = load_d2(rp, a_off + (unsigned)(nt - 2 - j0) * 8u);  // f(nt-2-j0), f(nt-1-j0)
-        raw_step r{f, make_double2(g.y, g.x)};
-        if (j0 >= nt2) r.a.x = r.b.x = 0.0;
-        if (j0 + 1 >= nt2) r.a.y = r.b.y = 0.0;
-        return r;
+        return raw_step{f, make_double2(g.y, g.x), valid};
     };
     {
         double me = 0.0, mo = 0.0;
         for (int j0 = 0; j0 < njs; j0 += 4) {
             raw_step r[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) r[u] = fetch(j0 + u);  // steps past njs read zeros
+            for (int u = 0; u < 4; ++u) r[u] = fetch(j0 + u);  // steps past njs are masked
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                me = fmax(me, fmax(fabs(r[u].a.x + r[u].b.x), fabs(r[u].a.y + r[u].b.y)));
-                mo = fmax(mo, fmax(fabs(r[u].a.x - r[u].b.x), fabs(r[u].a.y - r[u].b.y)));
+                const double2 a = r[u].va(), b = r[u].vb();
+                me = fmax(me, fmax(fabs(a.x + b.x), fabs(a.y + b.y)));
+                mo = fmax(mo, fmax(fabs(a.x - b.x), fabs(a.y - b.y)));
             }
         }
 #pragma unroll
@@ -506,22 +574,25 @@ __global__ __launch_bounds__(kThreads, 1) void pts_split_kernel(
     __syncthreads();
     const int ee = cexp[0][ca], eo = cexp[1][ca];
     const int par = wave >> 2, cb = wave & 1, K_rows = par ? no : ne;
-    const long region = frag_off[mp] + (par ? (long)cdiv(ne, 32) * njs * F::NB : 0);
+    const int region = par ? cdiv(ne, 32) * njs * F::NB : 0;
+    const __amdgpu_buffer_rsrc_t rt = make_frag_rsrc(tdig + (frag_off[mp] << 6), frag_off[mp + 1] - frag_off[mp]);
     double *sb = s + s_off[mp];
     // A blocks of a buffer: [parity][column block]
     auto slice = [&](const raw_step &r, u32x4 *buf) {
-        put_pair<F::NA>(buf, ca >> 5, ca, 2 * q, r.a.x + r.b.x, r.a.y + r.b.y, ee);
-        put_pair<F::NA>(buf, 2 + (ca >> 5), ca, 2 * q, r.a.x - r.b.x, r.a.y - r.b.y, eo);
+        const double2 a = r.va(), b = r.vb();
+        slicers<F::NA, 2> sl;
+        sl.sl[0].init(buf, ca >> 5, ca, 2 * q, a.x + b.x, a.y + b.y, ee);
+        sl.sl[1].init(buf, 2 + (ca >> 5), ca, 2 * q, a.x - b.x, a.y - b.y, eo);
+        return sl;
     };
 
     for (int k0 = 0; k0 < ne; k0 += kTileRP) {
         const int kb = k0 / 32 + ((wave >> 1) & 1);
         const bool active = kb * 32 < K_rows;
-        const u32x4 *tb = tdig + ((region + (long)kb * njs * F::NB) << 6);
         double dacc[16];
 #pragma unroll
         for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
-        k_loop<F>(fetch, slice, njs, tb, active, abuf, 2 * par + cb, lane, dacc);
+        k_loop<F>(fetch, slice, njs, rt, region + kb * njs * F::NB, abuf, 2 * par + cb, lane, dacc);
 
         const int k = kb * 32 + (lane & 31);
         if (active && k < K_rows) {
