@@ -279,6 +279,160 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
     print(json.dumps(out), flush=True)
 
 
+def batch_problems(seed: int = 3, n: int = 64):
+    """64 independent transforms: l_max log-uniform in [31, 1023] with l_max + 1 a
+    multiple of 4 (so that N_theta = 3 (l_max + 1) / 2 is even), N_r from {32, 64, 96},
+    1-8 fields; every m of each problem."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(np.log(32), np.log(1024), n)
+    l1 = np.clip(4 * np.round(np.exp(u) / 4), 32, 1024).astype(int)
+    nr = rng.choice([32, 64, 96], n)
+    nf = rng.integers(1, 9, n)
+    return [(int(a - 1), int(3 * a // 2), int(b), int(c)) for a, b, c in zip(l1, nr, nf)]
+
+
+def spectral_device(plan, lmax, nr, nf, seed: int):
+    """The config-0 coefficient distribution for all m in ascending order, on the device."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    s = torch.empty(plan.spectral_size, dtype=torch.float64, device="cuda")
+    rfac = torch.sin(torch.pi * torch.arange(1, nr + 1, dtype=torch.float64, device="cuda") / nr)
+    lf = (torch.arange(0, lmax + 1, dtype=torch.float64, device="cuda") + 1.0) ** -2
+    off = 0
+    for m in range(lmax + 1):
+        nl = lmax - m + 1
+        blk = torch.randn((nf, 2, nr, nl), dtype=torch.float64, device="cuda", generator=g)
+        blk *= lf[m:].view(1, 1, 1, nl)
+        blk *= rfac.view(1, 1, nr, 1)
+        if m == 0:
+            blk[:, 1] = 0.0
+        s[off: off + blk.numel()] = blk.reshape(-1)
+        off += blk.numel()
+    return s
+
+
+def batch_bench(ra, steps: int, warmup: int, forms):
+    """64 problems of different l_max in one grouped launch per direction, FP64 and the
+    split forms side by side: eager grouped launches (HIP events per direction), the
+    grouped round trip replayed from a hipGraph, and the same 64 problems launched one
+    plan at a time.  The FP64 batch must equal the serial FP64 runs bit for bit; each
+    split form's per-problem max / RMS error is taken against them (StP on the same
+    spectral input, PtS on the FP64 hybrid output)."""
+    import torch
+    probs = batch_problems()
+    F = sum(flops_per_direction(L, nt, nr, nf) for L, nt, nr, nf in probs)
+    log(f"batch: {len(probs)} problems, l_max {min(p[0] for p in probs)}..{max(p[0] for p in probs)}, "
+        f"F_LT {F:.3e} per direction")
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    def make_plans(form):
+        t0 = time.time()
+        plans = [ra.LegendrePlan(nt, L, np.arange(L + 1, dtype=np.int32), nr, nf, precision=form)
+                 for L, nt, nr, nf in probs]
+        torch.cuda.synchronize()
+        return plans, time.time() - t0
+
+    def run(form, plans, s_in, p_in_pts=None):
+        batch = ra.LegendreBatch(plans)
+        p2 = [torch.empty(p.hybrid_size, dtype=torch.float64, device="cuda") for p in plans]
+        s2 = [torch.empty(p.spectral_size, dtype=torch.float64, device="cuda") for p in plans]
+        for _ in range(warmup):
+            batch.stp(s_in, p2)
+            batch.pts(p2, s2)
+        torch.cuda.synchronize()
+        e = [[ev(), ev(), ev()] for _ in range(steps)]
+        for k in range(steps):
+            e[k][0].record()
+            batch.stp(s_in, p2)
+            e[k][1].record()
+            batch.pts(p2, s2)
+            e[k][2].record()
+        torch.cuda.synchronize()
+        stp_ms = float(np.mean([x[0].elapsed_time(x[1]) for x in e]))
+        pts_ms = float(np.mean([x[1].elapsed_time(x[2]) for x in e]))
+        g = batch.round_trip_graph(s_in, p2, s2)
+        for _ in range(warmup):
+            g.launch()
+        torch.cuda.synchronize()
+        a, b = ev(), ev()
+        a.record()
+        for _ in range(steps):
+            g.launch()
+        b.record()
+        torch.cuda.synchronize()
+        graph_ms = a.elapsed_time(b) / steps
+        # one plan at a time
+        a.record()
+        for _ in range(steps):
+            for i, p in enumerate(plans):
+                p.stp(s_in[i], p2[i])
+                p.pts(p2[i], s2[i])
+        b.record()
+        torch.cuda.synchronize()
+        serial_ms = a.elapsed_time(b) / steps
+        del g
+        rec = {"stp_ms": stp_ms, "pts_ms": pts_ms, "graph_round_trip_ms": graph_ms,
+               "serial_round_trip_ms": serial_ms, "tflops_graph": 2 * F / (graph_ms * 1e-3) / 1e12}
+        if form == "fp64":
+            rec["frac_fp64_peak"] = rec["tflops_graph"] / PEAK_FP64_TFLOPS
+        log(f"batch {form}: StP {stp_ms:.2f} ms, PtS {pts_ms:.2f} ms, graph round trip {graph_ms:.2f} ms, "
+            f"serial plans {serial_ms:.2f} ms")
+        # the batch's own final outputs (from the serial pass, identical by construction
+        # for FP64: checked below) and PtS of the FP64 hybrid input for the error table
+        batch.stp(s_in, p2)
+        if p_in_pts is not None:
+            batch.pts(p_in_pts, s2)
+        else:
+            batch.pts(p2, s2)
+        torch.cuda.synchronize()
+        return rec, p2, s2
+
+    log("batch fp64: creating plans")
+    plans, setup_s = make_plans("fp64")
+    s_in = [spectral_device(p, L, nr, nf, seed=500 + i) for i, (p, (L, nt, nr, nf)) in enumerate(zip(plans, probs))]
+    rec64, p64, s64 = run("fp64", plans, s_in)
+    rec64["setup_s"] = setup_s
+    # grouped FP64 == one plan at a time, bit for bit
+    exact = True
+    for i, p in enumerate(plans):
+        t = torch.empty(p.hybrid_size, dtype=torch.float64, device="cuda")
+        p.stp(s_in[i], t)
+        exact &= bool(torch.equal(t, p64[i]))
+        u = torch.empty(p.spectral_size, dtype=torch.float64, device="cuda")
+        p.pts(p64[i], u)
+        exact &= bool(torch.equal(u, s64[i]))
+        del t, u
+    rec64["bit_exact_vs_serial"] = exact
+    out = {"workload": "batch", "n_problems": len(probs), "problems": probs, "flops_per_direction": F,
+           "forms": {"fp64": rec64}}
+    del plans
+    torch.cuda.empty_cache()
+    for form in forms:
+        if form == "fp64":
+            continue
+        log(f"batch {form}: creating plans")
+        plans, setup_s = make_plans(form)
+        rec, p2, s2 = run(form, plans, s_in, p_in_pts=p64)
+        rec["setup_s"] = setup_s
+        errs = []
+        for i in range(len(probs)):
+            def rel(x, y):
+                d = x - y
+                return float(d.abs().max() / y.abs().max()), float(torch.sqrt((d * d).mean() / (y * y).mean()))
+            sm, sr = rel(p2[i], p64[i])
+            pm, pr = rel(s2[i], s64[i])
+            errs.append({"stp_max": sm, "stp_rms": sr, "pts_max": pm, "pts_rms": pr})
+        rec["max_err"] = max(max(e["stp_max"], e["pts_max"]) for e in errs)
+        rec["per_problem_err"] = errs
+        log(f"batch {form}: worst per-problem max rel error vs FP64 {rec['max_err']:.2e}")
+        out["forms"][form] = rec
+        del plans, p2, s2
+        torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
     (profiles/legendre_fp64_lmax511.traffic.json: FETCH_SIZE and WRITE_SIZE in KiB
@@ -301,9 +455,10 @@ def main():
                     help="Legendre forms to run; the first is the headline, fp64 is always run as the error reference")
     ap.add_argument("--cpu-sample-m", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="legendre", choices=["legendre", "stack", "large"],
+    ap.add_argument("--workload", default="legendre", choices=["legendre", "stack", "large", "batch"],
                     help="legendre: the l_max=511 round trip (headline); stack: l_max=255 full stack per "
-                         "stage; large: l_max=1023, N_phi=3072, N_r=256, 5 fields full stack per stage")
+                         "stage; large: l_max=1023, N_phi=3072, N_r=256, 5 fields full stack per stage; "
+                         "batch: 64 problems of l_max 31..1023 in grouped launches")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy (scales profiler byte counters)")
     args = ap.parse_args()
@@ -319,6 +474,9 @@ def main():
         sys.exit("bench.py needs a GPU")
     if args.workload in STACKS:
         stack_bench(ra, args.steps, args.warmup, args.workload)
+        return
+    if args.workload == "batch":
+        batch_bench(ra, args.steps, args.warmup, [f.strip() for f in args.forms.split(",") if f.strip()])
         return
     mvals = O.m_order_balanced(LMAX)
     F = flops_per_direction()

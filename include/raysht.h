@@ -138,6 +138,26 @@ int raysht_graph_create_round_trip(raysht_graph **graph, raysht_plan *plan, int 
 int raysht_graph_launch(raysht_graph *graph, void *stream);
 void raysht_graph_destroy(raysht_graph *graph);
 
+/* Many independent Legendre problems in one grouped launch per direction: up to 64
+ * plans of one precision (any l_max, N_theta, m set and column count) whose m-blocks
+ * are scheduled together, longest contraction first, over the XCDs of the device.
+ * raysht_batch_stp(b, s, p2, stream) does what raysht_legendre_stp(plans[i], s[i],
+ * p2[i], stream) does for every i, in one kernel launch (results identical); likewise
+ * raysht_batch_pts.  s / p2 are host arrays of n_plans device pointers.  The plans
+ * must outlive the batch.  No reference counterpart: the reference transforms one
+ * problem per call (Legendre_Transforms.F90:968,1077). */
+typedef struct raysht_batch raysht_batch;
+int raysht_batch_create(raysht_batch **batch, raysht_plan *const *plans, int n_plans);
+void raysht_batch_destroy(raysht_batch *batch);
+int raysht_batch_size(const raysht_batch *batch);
+int raysht_batch_stp(raysht_batch *batch, const double *const *s, double *const *p2, void *stream);
+int raysht_batch_pts(raysht_batch *batch, const double *const *p2, double *const *s, void *stream);
+/* The batch's Legendre round trip (grouped StP into p2[i], grouped PtS into s_out[i])
+ * recorded into a hipGraph, launched with raysht_graph_launch.  No reference
+ * counterpart. */
+int raysht_graph_create_batch(raysht_graph **graph, raysht_batch *batch, const double *const *s_in,
+                              double *const *p2, double *const *s_out);
+
 /* Cheby_Grid%init (Chebyshev_Polynomials.F90:65-207) for one domain of n
  * zeros-grid points on [rmin, rmax]; grid (host, n) receives the radii. */
 int raysht_cheb_create(raysht_cheb **cheb, int n, double rmin, double rmax, double *grid);

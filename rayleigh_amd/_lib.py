@@ -46,7 +46,13 @@ SIGNATURES = {
     "raysht_spectral_to_physical": (_I, [_P, _I, _P, _P, _P, _P]),
     "raysht_physical_to_spectral": (_I, [_P, _I, _P, _P, _P, _P]),
     "raysht_graph_create_round_trip": (_I, [ctypes.POINTER(_P), _P, _I, _P, _P, _P, _P]),
+    "raysht_graph_create_batch": (_I, [ctypes.POINTER(_P), _P, _P, _P, _P]),
     "raysht_graph_launch": (_I, [_P, _P]),
+    "raysht_batch_create": (_I, [ctypes.POINTER(_P), _P, _I]),
+    "raysht_batch_destroy": (None, [_P]),
+    "raysht_batch_size": (_I, [_P]),
+    "raysht_batch_stp": (_I, [_P, _P, _P, _P]),
+    "raysht_batch_pts": (_I, [_P, _P, _P, _P]),
     "raysht_graph_destroy": (None, [_P]),
     "raysht_cheb_create": (_I, [ctypes.POINTER(_P), _I, _D, _D, _P]),
     "raysht_cheb_destroy": (None, [_P]),

@@ -19,6 +19,7 @@
 // the i axis) so that the 16 consecutive lanes of a store walk the unit-stride
 // axis of the output (theta for StP, l for PtS).
 #include "raysht_internal.h"
+#include "uniform.h"
 
 namespace raysht {
 namespace {
@@ -94,33 +95,8 @@ struct stp_item {
     __amdgpu_buffer_rsrc_t rs, rpe, rpo;
 };
 
-__device__ __forceinline__ bool stp_next_item(const work_item *__restrict__ work, int n_work, int &idx, int stride,
-                                              const double *s, const double *pte, const double *pto,
-                                              const long *s_off, const long *te_off, const long *to_off,
-                                              const int *mvals, int lmax, int nt2, int nrhs, stp_item &it)
-{
-    for (; idx < n_work; idx += stride) {
-        const work_item w = work[idx];
-        if (w.mp < 0) continue;  // padding item of the XCD-balanced schedule
-        it.mp = w.mp;
-        it.j0 = w.r0;
-        it.c0 = w.c0;
-        it.nl = lmax - mvals[w.mp] + 1;
-        it.ne = (it.nl + 1) >> 1;
-        it.no = it.nl >> 1;
-        it.rs = make_rsrc(s + s_off[w.mp], (long)it.nl * nrhs * 8);
-        it.rpe = make_rsrc(pte + te_off[w.mp], (long)it.ne * nt2 * 8);
-        it.rpo = make_rsrc(pto + to_off[w.mp], (long)it.no * nt2 * 8);
-        return true;
-    }
-    return false;
-}
-
 __global__ __launch_bounds__(STP_THREADS, RAYSHT_FP64_MIN_WAVES) void stp_fp64_kernel(
-    const double *__restrict__ s, double *__restrict__ p2, const double *__restrict__ pte,
-    const double *__restrict__ pto, const long *__restrict__ s_off, const long *__restrict__ te_off,
-    const long *__restrict__ to_off, const int *__restrict__ mvals, const work_item *__restrict__ work, int n_work,
-    int lmax, int nt, int nt2, int nrhs)
+    const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
 {
     __shared__ double Ae[STP_KC * STP_A_LD];
     __shared__ double Ao[STP_KC * STP_A_LD];
@@ -133,10 +109,22 @@ __global__ __launch_bounds__(STP_THREADS, RAYSHT_FP64_MIN_WAVES) void stp_fp64_k
     const int bq = tid % (2 * STP_KC), bcc0 = tid / (2 * STP_KC);
     const int ajj = tid % STP_TJ, akk0 = tid / STP_TJ;
 
-    int idx = blockIdx.x;
+    const int mp = uniform(work[blockIdx.x].mp);
+    if (mp < 0) return;  // padding item of the XCD-balanced schedule
+    const int pi = uniform(work[blockIdx.x].pad);
+    const prob_desc &P = probs[pi];
+    const int lmax = uniform(P.lmax), nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
+    double *p2 = uniform(io.out[pi]);
     stp_item cur;
-    if (!stp_next_item(work, n_work, idx, n_work, s, pte, pto, s_off, te_off, to_off, mvals, lmax, nt2, nrhs, cur))
-        return;
+    cur.mp = mp;
+    cur.j0 = uniform(work[blockIdx.x].r0);
+    cur.c0 = uniform(work[blockIdx.x].c0);
+    cur.nl = lmax - uniform(uniform(P.mvals)[mp]) + 1;
+    cur.ne = (cur.nl + 1) >> 1;
+    cur.no = cur.nl >> 1;
+    cur.rs = make_rsrc(uniform(io.in[pi]) + uniform(uniform(P.s_off)[mp]), (long)cur.nl * nrhs * 8);
+    cur.rpe = make_rsrc(uniform(P.pt_even) + uniform(uniform(P.te_off)[mp]), (long)cur.ne * nt2 * 8);
+    cur.rpo = make_rsrc(uniform(P.pt_odd) + uniform(uniform(P.to_off)[mp]), (long)cur.no * nt2 * 8);
     const int b_voff = ((cur.c0 + bcc0) * cur.nl + bq) * 8;
     const int b_sstep = (STP_THREADS / (2 * STP_KC)) * cur.nl * 8;
     const bool a_jin = cur.j0 + ajj < nt2;
@@ -253,38 +241,31 @@ struct pts_item {
     __amdgpu_buffer_rsrc_t rp, rie, rio;
 };
 
-__device__ __forceinline__ bool pts_next_item(const work_item *__restrict__ work, int n_work, int &idx, int stride,
-                                              const double *p2, const double *ipe, const double *ipo,
-                                              const long *te_off, const long *to_off, const int *mvals, int lmax,
-                                              int nt, int nt2, int nrhs, pts_item &it)
-{
-    for (; idx < n_work; idx += stride) {
-        const work_item w = work[idx];
-        if (w.mp < 0) continue;  // padding item of the XCD-balanced schedule
-        it.mp = w.mp;
-        it.k0 = w.r0;
-        it.c0 = w.c0;
-        it.nl = lmax - mvals[w.mp] + 1;
-        it.ne = (it.nl + 1) >> 1;
-        it.no = it.nl >> 1;
-        it.rp = make_rsrc(p2 + (long)w.mp * nrhs * nt, (long)nrhs * nt * 8);
-        it.rie = make_rsrc(ipe + te_off[w.mp], (long)it.ne * nt2 * 8);
-        it.rio = make_rsrc(ipo + to_off[w.mp], (long)it.no * nt2 * 8);
-        return true;
-    }
-    return false;
-}
-
 __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_kernel(
-    const double *__restrict__ p2, double *__restrict__ s, const double *__restrict__ ipe,
-    const double *__restrict__ ipo, const long *__restrict__ s_off, const long *__restrict__ te_off,
-    const long *__restrict__ to_off, const int *__restrict__ mvals, const work_item *__restrict__ work, int n_work,
-    int lmax, int nt, int nt2, int nrhs)
+    const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
 {
     __shared__ double Fe[PTS_TC * PTS_F_LD];
     __shared__ double Fo[PTS_TC * PTS_F_LD];
     __shared__ double Ie[PTS_KJ * PTS_I_LD];
     __shared__ double Io[PTS_KJ * PTS_I_LD];
+
+    const int mp = uniform(work[blockIdx.x].mp);
+    if (mp < 0) return;  // padding item of the XCD-balanced schedule
+    const int pi = uniform(work[blockIdx.x].pad);
+    const prob_desc &P = probs[pi];
+    const int lmax = uniform(P.lmax), nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
+    double *s = uniform(io.out[pi]);
+    const long s_base = uniform(uniform(P.s_off)[mp]);
+    pts_item cur;
+    cur.mp = mp;
+    cur.k0 = uniform(work[blockIdx.x].r0);
+    cur.c0 = uniform(work[blockIdx.x].c0);
+    cur.nl = lmax - uniform(uniform(P.mvals)[mp]) + 1;
+    cur.ne = (cur.nl + 1) >> 1;
+    cur.no = cur.nl >> 1;
+    cur.rp = make_rsrc(uniform(io.in[pi]) + (long)mp * nrhs * nt, (long)nrhs * nt * 8);
+    cur.rie = make_rsrc(uniform(P.ip_even) + uniform(uniform(P.te_off)[mp]), (long)cur.ne * nt2 * 8);
+    cur.rio = make_rsrc(uniform(P.ip_odd) + uniform(uniform(P.to_off)[mp]), (long)cur.no * nt2 * 8);
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
@@ -333,10 +314,6 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
         }
     };
 
-    int idx = blockIdx.x;
-    pts_item cur;
-    if (!pts_next_item(work, n_work, idx, n_work, p2, ipe, ipo, te_off, to_off, mvals, lmax, nt, nt2, nrhs, cur))
-        return;
     fetch(cur, 0);
     {
         double4_t acc_e[PTS_CB][4], acc_o[PTS_CB][4];
@@ -374,7 +351,7 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
             __syncthreads();
         }
 
-        double *sb = s + s_off[cur.mp];
+        double *sb = s + s_base;
 #pragma unroll
         for (int cb = 0; cb < PTS_CB; ++cb)
 #pragma unroll
@@ -394,24 +371,38 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
 
 }  // namespace
 
-int launch_stp_fp64(raysht_plan *p, const double *s, double *p2, hipStream_t st)
+int launch_stp_fp64_batch(const prob_desc *d_desc, const batch_io &io, const work_item *work, int n_work,
+                          hipStream_t st)
 {
-    if (p->n_work_stp == 0) return RAYSHT_OK;
-    hipLaunchKernelGGL(stp_fp64_kernel, dim3(p->n_work_stp), dim3(STP_THREADS), 0, st, s, p2, p->d_pt_even,
-                       p->d_pt_odd, p->d_s_off, p->d_te_off, p->d_to_off, p->d_mvals, p->d_work_stp, p->n_work_stp,
-                       p->lmax, p->nt, p->nt2, p->nrhs);
+    if (n_work == 0) return RAYSHT_OK;
+    hipLaunchKernelGGL(stp_fp64_kernel, dim3(n_work), dim3(STP_THREADS), 0, st, d_desc, io, work);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
 
-int launch_pts_fp64(raysht_plan *p, const double *p2, double *s, hipStream_t st)
+int launch_pts_fp64_batch(const prob_desc *d_desc, const batch_io &io, const work_item *work, int n_work,
+                          hipStream_t st)
 {
-    if (p->n_work_pts == 0) return RAYSHT_OK;
-    hipLaunchKernelGGL(pts_fp64_kernel, dim3(p->n_work_pts), dim3(PTS_THREADS), 0, st, p2, s, p->d_ip_even,
-                       p->d_ip_odd, p->d_s_off, p->d_te_off, p->d_to_off, p->d_mvals, p->d_work_pts, p->n_work_pts,
-                       p->lmax, p->nt, p->nt2, p->nrhs);
+    if (n_work == 0) return RAYSHT_OK;
+    hipLaunchKernelGGL(pts_fp64_kernel, dim3(n_work), dim3(PTS_THREADS), 0, st, d_desc, io, work);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
+}
+
+int launch_stp_fp64(raysht_plan *p, const double *s, double *p2, hipStream_t st)
+{
+    batch_io io{};
+    io.in[0] = s;
+    io.out[0] = p2;
+    return launch_stp_fp64_batch(p->d_desc, io, p->d_work_stp, p->n_work_stp, st);
+}
+
+int launch_pts_fp64(raysht_plan *p, const double *p2, double *s, hipStream_t st)
+{
+    batch_io io{};
+    io.in[0] = p2;
+    io.out[0] = s;
+    return launch_pts_fp64_batch(p->d_desc, io, p->d_work_pts, p->n_work_pts, st);
 }
 
 // Tile geometry the host work lists are built with.

@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "raysht_internal.h"
+#include "uniform.h"
 
 namespace raysht {
 
@@ -50,11 +51,26 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(4))) unsigned u32x4;  // one operand fragment, kept as 4 dwords
 
-constexpr int kThreads = 512;  // 8 waves, each one 32 x 32 output block
-constexpr int kTileC = 64;     // data columns per workgroup (two 32-column blocks)
-constexpr int kTileR = 128;    // StP theta rows per workgroup tile (four 32-row blocks)
-constexpr int kTileRP = 64;    // PtS parity-l rows per workgroup tile and parity (two blocks)
-constexpr int kABuf = 4 * 7 * 64;  // LDS A fragments of one step: up to 4 blocks x 7 slices
+// Workgroup layout: RAYSHT_SPLIT_WAVES waves, each one 32 x 32 output block, over
+// RAYSHT_SPLIT_CB 32-column blocks; StP tiles are 4 theta blocks tall, PtS tiles 2
+// parity-l blocks of each parity.
+#ifndef RAYSHT_SPLIT_WAVES
+#define RAYSHT_SPLIT_WAVES 4
+#endif
+#ifndef RAYSHT_SPLIT_CB
+#define RAYSHT_SPLIT_CB 1
+#endif
+constexpr int kWaves = RAYSHT_SPLIT_WAVES;
+constexpr int kCB = RAYSHT_SPLIT_CB;
+constexpr int kThreads = 64 * kWaves;
+constexpr int kTileC = 32 * kCB;                 // data columns per workgroup
+constexpr int kJB = kWaves / kCB;                // StP theta blocks per tile
+constexpr int kKB = kWaves / (2 * kCB);          // PtS parity-l blocks per tile and parity
+constexpr int kTileR = 32 * kJB;                 // StP theta rows per tile
+constexpr int kTileRP = 32 * kKB;                // PtS parity-l rows per tile and parity
+constexpr int kMinWaves = kWaves >= 8 ? 1 : 2;   // workgroups per CU the registers must allow
+constexpr int kABuf = 2 * kCB * 7 * 64;          // LDS A fragments of one step
+static_assert(kThreads / 8 == kTileC && kJB * kCB == kWaves && 2 * kKB * kCB == kWaves, "split layout");
 constexpr int kChunkK = 256;   // K terms per FP32 level sum: 4 * 256 * 2^14 <= 2^24
 constexpr int kChunkSteps = kChunkK / 16;
 
@@ -417,31 +433,36 @@ __device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, i
 }
 
 // --------------------------------------------------------------- StP
-// Work item (mp, c0): 64 columns, every theta row, in tiles of 128 rows with both
-// parities each.  Eight waves: wave w multiplies column block w % 2 by theta block
-// j0/32 + w/2.  The spectral tile is read as 16-byte pairs (l, l+1); each parity pass
+// Work item (mp, c0): kTileC columns, every theta row, in tiles of kTileR rows with
+// both parities each.  Wave w multiplies column block w % kCB by theta block
+// j0/32 + w/kCB.  The spectral tile is read as 16-byte pairs (l, l+1); each parity pass
 // slices its half, two values per thread and step.  The column exponents come from one
 // pass over the tile; the scaled even-parity sums wait in LDS for the odd pass.
 template <class F>
-__global__ __launch_bounds__(kThreads, 1) void stp_split_kernel(
-    const work_item *__restrict__ work, int lmax, int nt, int nt2, int nrhs, const int *__restrict__ mvals,
-    const long *__restrict__ s_off, const double *__restrict__ s, const long *__restrict__ frag_off,
-    const u32x4 *__restrict__ tdig, const int *__restrict__ texp, double *__restrict__ p2)
+__global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
+    const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
 {
     __shared__ u32x4 abuf[2][kABuf];
     __shared__ double ebuf[16][kThreads];
     __shared__ int cexp[kTileC];
-    const work_item w = work[blockIdx.x];
-    if (w.mp < 0) return;
-    const int mp = w.mp, m = mvals[mp], nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
+    const int mp = uniform(work[blockIdx.x].mp);
+    if (mp < 0) return;
+    const int pi = uniform(work[blockIdx.x].pad), c0 = uniform(work[blockIdx.x].c0);
+    const prob_desc &P = probs[pi];
+    const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
+    const long *frag_off = uniform(P.frag_stp_off);
+    const int *texp = uniform(P.texp_stp);
+    const int m = uniform(uniform(P.mvals)[mp]), nl = uniform(P.lmax) - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const double *sb = s + s_off[mp];
+    const double *sb = uniform(io.in[pi]) + uniform(uniform(P.s_off)[mp]);
+    const long f_lo = uniform(frag_off[mp]), f_hi = uniform(frag_off[mp + 1]);
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(sb, (long)nl * nrhs * 8);
-    const __amdgpu_buffer_rsrc_t rt = make_frag_rsrc(tdig + (frag_off[mp] << 6), frag_off[mp + 1] - frag_off[mp]);
+    const __amdgpu_buffer_rsrc_t rt =
+        make_frag_rsrc(reinterpret_cast<const u32x4 *>(uniform(P.tdig_stp)) + (f_lo << 6), f_hi - f_lo);
 
     // producer role of the thread: column ca, l = 32 i + 4 q .. +3 of step i
     const int ca = tid >> 3, q = tid & 7;
-    const unsigned a_off = (unsigned)min(w.c0 + ca, nrhs) * nl * 8;  // column nrhs and beyond read as 0
+    const unsigned a_off = (unsigned)min(c0 + ca, nrhs) * nl * 8;  // column nrhs and beyond read as 0
     {
         double mx = 0.0;
         const int n32 = cdiv(nl, 32);
@@ -466,11 +487,11 @@ __global__ __launch_bounds__(kThreads, 1) void stp_split_kernel(
     }
     __syncthreads();
     const int ea = cexp[ca];
-    const int cb = wave & 1, njb = cdiv(nt2, 32), nks_e = cdiv(ne, 16);
-    double *ob = p2 + (long)mp * nrhs * nt;
+    const int cb = wave % kCB, njb = cdiv(nt2, 32), nks_e = cdiv(ne, 16);
+    double *ob = uniform(io.out[pi]) + (long)mp * nrhs * nt;
 
     for (int r0 = 0; r0 < nt2; r0 += kTileR) {
-        const int jb = r0 / 32 + (wave >> 1);
+        const int jb = r0 / 32 + wave / kCB;
         const bool active = jb < njb;
         const int j = jb * 32 + (lane & 31);
 #pragma unroll
@@ -499,7 +520,7 @@ __global__ __launch_bounds__(kThreads, 1) void stp_split_kernel(
             const int ej = store ? texp[((long)mp * 2 + par) * nt2 + j] : 0;
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = w.c0 + cc;
+                const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = c0 + cc;
                 const double r = __builtin_ldexp(dacc[v], cexp[cc] + ej);
                 if (par == 0) {
                     ebuf[v][tid] = r;
@@ -514,28 +535,30 @@ __global__ __launch_bounds__(kThreads, 1) void stp_split_kernel(
 }
 
 // --------------------------------------------------------------- PtS
-// Work item (mp, c0): 64 columns, every parity-l row of both parities, in row tiles of
-// 64.  Eight waves: wave w multiplies column block w % 2 of parity w / 4 by row block
-// k0/32 + (w / 2) % 2.  The data operand is the fold f(j) + f(nt-1-j) (even) or
+// Work item (mp, c0): kTileC columns, every parity-l row of both parities, in row
+// tiles of kTileRP.  Wave w multiplies column block w % kCB of parity w / (kCB kKB) by
+// row block k0/32 + (w / kCB) % kKB.  The data operand is the fold f(j) + f(nt-1-j) (even) or
 // f(j) - f(nt-1-j) (odd) of the hybrid columns; each thread slices both folds of two
 // nodes per step, and both folds' column exponents come from one pass over the tile.
 template <class F>
-__global__ __launch_bounds__(kThreads, 1) void pts_split_kernel(
-    const work_item *__restrict__ work, int lmax, int nt, int nt2, int nrhs, const int *__restrict__ mvals,
-    const long *__restrict__ s_off, const double *__restrict__ p2, const long *__restrict__ frag_off,
-    const long *__restrict__ exp_off, const u32x4 *__restrict__ tdig, const int *__restrict__ texp,
-    double *__restrict__ s)
+__global__ __launch_bounds__(kThreads, kMinWaves) void pts_split_kernel(
+    const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
 {
     __shared__ u32x4 abuf[2][kABuf];
     __shared__ int cexp[2][kTileC];
-    const work_item w = work[blockIdx.x];
-    if (w.mp < 0) return;
-    const int mp = w.mp, m = mvals[mp], nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
+    const int mp = uniform(work[blockIdx.x].mp);
+    if (mp < 0) return;
+    const int pi = uniform(work[blockIdx.x].pad), c0 = uniform(work[blockIdx.x].c0);
+    const prob_desc &P = probs[pi];
+    const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
+    const long *frag_off = uniform(P.frag_pts_off);
+    const int *texp = uniform(P.texp_pts);
+    const int m = uniform(uniform(P.mvals)[mp]), nl = uniform(P.lmax) - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const double *pb = p2 + (long)mp * nrhs * nt;
+    const double *pb = uniform(io.in[pi]) + (long)mp * nrhs * nt;
     const __amdgpu_buffer_rsrc_t rp = make_rsrc(pb, (long)nrhs * nt * 8);
     const int ca = tid >> 3, q = tid & 7;
-    const unsigned a_off = (unsigned)min(w.c0 + ca, nrhs) * nt * 8;
+    const unsigned a_off = (unsigned)min(c0 + ca, nrhs) * nt * 8;
     const int njs = cdiv(nt2, 16);
 
     // raw data of step js: a = (f(j), f(j+1)) and b = (f(nt-1-j), f(nt-2-j)) for
@@ -573,33 +596,36 @@ __global__ __launch_bounds__(kThreads, 1) void pts_split_kernel(
     }
     __syncthreads();
     const int ee = cexp[0][ca], eo = cexp[1][ca];
-    const int par = wave >> 2, cb = wave & 1, K_rows = par ? no : ne;
+    const int par = wave / (kCB * kKB), cb = wave % kCB, K_rows = par ? no : ne;
     const int region = par ? cdiv(ne, 32) * njs * F::NB : 0;
-    const __amdgpu_buffer_rsrc_t rt = make_frag_rsrc(tdig + (frag_off[mp] << 6), frag_off[mp + 1] - frag_off[mp]);
-    double *sb = s + s_off[mp];
+    const long f_lo = uniform(frag_off[mp]), f_hi = uniform(frag_off[mp + 1]);
+    const __amdgpu_buffer_rsrc_t rt =
+        make_frag_rsrc(reinterpret_cast<const u32x4 *>(uniform(P.tdig_pts)) + (f_lo << 6), f_hi - f_lo);
+    double *sb = uniform(io.out[pi]) + uniform(uniform(P.s_off)[mp]);
+    const long e_base = uniform(uniform(P.exp_pts_off)[mp]);
     // A blocks of a buffer: [parity][column block]
     auto slice = [&](const raw_step &r, u32x4 *buf) {
         const double2 a = r.va(), b = r.vb();
         slicers<F::NA, 2> sl;
         sl.sl[0].init(buf, ca >> 5, ca, 2 * q, a.x + b.x, a.y + b.y, ee);
-        sl.sl[1].init(buf, 2 + (ca >> 5), ca, 2 * q, a.x - b.x, a.y - b.y, eo);
+        sl.sl[1].init(buf, kCB + (ca >> 5), ca, 2 * q, a.x - b.x, a.y - b.y, eo);
         return sl;
     };
 
     for (int k0 = 0; k0 < ne; k0 += kTileRP) {
-        const int kb = k0 / 32 + ((wave >> 1) & 1);
+        const int kb = k0 / 32 + (wave / kCB) % kKB;
         const bool active = kb * 32 < K_rows;
         double dacc[16];
 #pragma unroll
         for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
-        k_loop<F>(fetch, slice, njs, rt, region + kb * njs * F::NB, abuf, 2 * par + cb, lane, dacc);
+        k_loop<F>(fetch, slice, njs, rt, region + kb * njs * F::NB, abuf, kCB * par + cb, lane, dacc);
 
         const int k = kb * 32 + (lane & 31);
         if (active && k < K_rows) {
-            const int ek = texp[exp_off[mp] + (par ? ne : 0) + k];
+            const int ek = texp[e_base + (par ? ne : 0) + k];
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = w.c0 + cc;
+                const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = c0 + cc;
                 if (c < nrhs) sb[(long)c * nl + 2 * k + par] = __builtin_ldexp(dacc[v], cexp[par][cc] + ek);
             }
         }
@@ -607,25 +633,37 @@ __global__ __launch_bounds__(kThreads, 1) void pts_split_kernel(
 }
 
 // --------------------------------------------------------------- host side
-// XCD-balanced work lists of (m-block, column tile) items: whole m-blocks dealt to the
-// eight XCDs by longest-processing-time, interleaved so that item i runs on XCD i % 8.
-std::vector<work_item> split_work(const raysht_plan *p, bool stp)
+}  // namespace
+
+// XCD-balanced work lists of (problem, m-block, column tile) items: whole m-blocks dealt
+// to the eight XCDs by longest-processing-time, interleaved so that item i runs on
+// XCD i % 8.  The problem index travels in the `pad` field.
+std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans, bool stp)
 {
     constexpr int kXcds = 8;
-    std::vector<int> order(p->nm);
-    for (int i = 0; i < p->nm; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return p->mvals[a] < p->mvals[b]; });
+    struct blk {
+        int prob, mp;
+        double item;  // K steps of one item (plus the exponent pass)
+    };
+    std::vector<blk> order;
+    for (int pi = 0; pi < (int)plans.size(); ++pi) {
+        const raysht_plan *p = plans[pi];
+        const int nrt = cdiv(p->nt2, kTileR), njs = cdiv(p->nt2, 16);
+        for (int mp = 0; mp < p->nm; ++mp) {
+            const int nl = p->lmax - p->mvals[mp] + 1, ne = (nl + 1) / 2, no = nl / 2;
+            const double steps = stp ? (double)nrt * (cdiv(ne, 16) + cdiv(no, 16)) + cdiv(nl, 32)
+                                     : (double)cdiv(ne, kTileRP) * njs + njs;
+            order.push_back(blk{pi, mp, steps});
+        }
+    }
+    std::stable_sort(order.begin(), order.end(), [](const blk &a, const blk &b) { return a.item > b.item; });
     std::vector<std::vector<work_item>> q(kXcds);
     std::vector<double> load(kXcds, 0.0);
-    const int nct = cdiv(p->nrhs, kTileC), nrt = cdiv(p->nt2, kTileR), njs = cdiv(p->nt2, 16);
-    for (int mp : order) {
-        const int m = p->mvals[mp], nl = p->lmax - m + 1, ne = (nl + 1) / 2, no = nl / 2;
-        // K steps per item (plus the exponent pass)
-        const double steps = stp ? (double)nrt * (cdiv(ne, 16) + cdiv(no, 16)) + cdiv(nl, 32)
-                                 : (double)cdiv(ne, kTileRP) * njs + njs;
+    for (const blk &b : order) {
+        const int nct = cdiv(plans[b.prob]->nrhs, kTileC);
         const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        for (int c = 0; c < nct; ++c) q[x].push_back(work_item{mp, 0, c * kTileC, 0});
-        load[x] += steps * nct;
+        for (int c = 0; c < nct; ++c) q[x].push_back(work_item{b.mp, 0, c * kTileC, b.prob});
+        load[x] += b.item * nct;
     }
     size_t len = 0;
     for (auto &v : q) len = std::max(len, v.size());
@@ -635,6 +673,8 @@ std::vector<work_item> split_work(const raysht_plan *p, bool stp)
             if (i < q[x].size()) w[i * kXcds + x] = q[x][i];
     return w;
 }
+
+namespace {
 
 template <typename T>
 int dalloc(T **p, size_t n)
@@ -673,7 +713,7 @@ int split_plan_init(raysht_plan *p)
         fp[mp + 1] = fp[mp] + (long)(cdiv(ne, 32) + cdiv(no, 32)) * njs * nb_pts;
         ep[mp + 1] = ep[mp] + ne + no;
     }
-    const std::vector<work_item> ws = split_work(p, true), wp = split_work(p, false);
+    const std::vector<work_item> ws = split_work({p}, true), wp = split_work({p}, false);
     sp.n_work_stp = (int)ws.size();
     sp.n_work_pts = (int)wp.size();
     int rc;
@@ -708,36 +748,44 @@ void split_plan_free(raysht_plan *p)
     sp = raysht_split();
 }
 
-int launch_stp_split(raysht_plan *p, const double *s, double *p2, hipStream_t st)
+int launch_stp_split_batch(int precision, const prob_desc *d_desc, const batch_io &io, const work_item *work,
+                           int n_work, hipStream_t st)
 {
-    raysht_split &sp = p->split;
-    if (sp.n_work_stp == 0) return RAYSHT_OK;
-    if (x6(p))
-        hipLaunchKernelGGL(stp_split_kernel<stp_x6>, dim3(sp.n_work_stp), dim3(kThreads), 0, st, sp.d_work_stp,
-                           p->lmax, p->nt, p->nt2, p->nrhs, p->d_mvals, p->d_s_off, s, sp.d_frag_stp_off,
-                           reinterpret_cast<const u32x4 *>(sp.d_tdig_stp), sp.d_texp_stp, p2);
+    if (n_work == 0) return RAYSHT_OK;
+    if (precision == RAYSHT_BF16X6)
+        hipLaunchKernelGGL(stp_split_kernel<stp_x6>, dim3(n_work), dim3(kThreads), 0, st, d_desc, io, work);
     else
-        hipLaunchKernelGGL(stp_split_kernel<any_x3>, dim3(sp.n_work_stp), dim3(kThreads), 0, st, sp.d_work_stp,
-                           p->lmax, p->nt, p->nt2, p->nrhs, p->d_mvals, p->d_s_off, s, sp.d_frag_stp_off,
-                           reinterpret_cast<const u32x4 *>(sp.d_tdig_stp), sp.d_texp_stp, p2);
+        hipLaunchKernelGGL(stp_split_kernel<any_x3>, dim3(n_work), dim3(kThreads), 0, st, d_desc, io, work);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
 
-int launch_pts_split(raysht_plan *p, const double *p2, double *s, hipStream_t st)
+int launch_pts_split_batch(int precision, const prob_desc *d_desc, const batch_io &io, const work_item *work,
+                           int n_work, hipStream_t st)
 {
-    raysht_split &sp = p->split;
-    if (sp.n_work_pts == 0) return RAYSHT_OK;
-    if (x6(p))
-        hipLaunchKernelGGL(pts_split_kernel<pts_x6>, dim3(sp.n_work_pts), dim3(kThreads), 0, st, sp.d_work_pts,
-                           p->lmax, p->nt, p->nt2, p->nrhs, p->d_mvals, p->d_s_off, p2, sp.d_frag_pts_off,
-                           sp.d_exp_pts_off, reinterpret_cast<const u32x4 *>(sp.d_tdig_pts), sp.d_texp_pts, s);
+    if (n_work == 0) return RAYSHT_OK;
+    if (precision == RAYSHT_BF16X6)
+        hipLaunchKernelGGL(pts_split_kernel<pts_x6>, dim3(n_work), dim3(kThreads), 0, st, d_desc, io, work);
     else
-        hipLaunchKernelGGL(pts_split_kernel<any_x3>, dim3(sp.n_work_pts), dim3(kThreads), 0, st, sp.d_work_pts,
-                           p->lmax, p->nt, p->nt2, p->nrhs, p->d_mvals, p->d_s_off, p2, sp.d_frag_pts_off,
-                           sp.d_exp_pts_off, reinterpret_cast<const u32x4 *>(sp.d_tdig_pts), sp.d_texp_pts, s);
+        hipLaunchKernelGGL(pts_split_kernel<any_x3>, dim3(n_work), dim3(kThreads), 0, st, d_desc, io, work);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
+}
+
+int launch_stp_split(raysht_plan *p, const double *s, double *p2, hipStream_t st)
+{
+    batch_io io{};
+    io.in[0] = s;
+    io.out[0] = p2;
+    return launch_stp_split_batch(p->precision, p->d_desc, io, p->split.d_work_stp, p->split.n_work_stp, st);
+}
+
+int launch_pts_split(raysht_plan *p, const double *p2, double *s, hipStream_t st)
+{
+    batch_io io{};
+    io.in[0] = p2;
+    io.out[0] = s;
+    return launch_pts_split_batch(p->precision, p->d_desc, io, p->split.d_work_pts, p->split.n_work_pts, st);
 }
 
 }  // namespace raysht

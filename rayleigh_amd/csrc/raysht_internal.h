@@ -49,6 +49,28 @@ struct raysht_split {
     int *d_texp_pts = nullptr;
 };
 
+// Everything a contraction kernel reads about one problem (one plan), in device
+// memory, so that one grouped launch can cover the m-blocks of several problems: the
+// `pad` field of a work item selects the problem.
+struct prob_desc {
+    const double *pt_even, *pt_odd, *ip_even, *ip_odd;  // FP64 tables
+    const long *s_off, *te_off, *to_off;
+    const int *mvals;
+    // split forms (null for FP64 plans)
+    const long *frag_stp_off, *frag_pts_off, *exp_pts_off;
+    const uint4 *tdig_stp, *tdig_pts;
+    const int *texp_stp, *texp_pts;
+    int lmax, nt, nt2, nrhs;
+};
+
+// The per-call operands of a grouped launch, passed by value: input and output of each
+// problem (StP: spectral in, hybrid out; PtS: hybrid in, spectral out).
+constexpr int kMaxBatch = 64;
+struct batch_io {
+    const double *in[kMaxBatch];
+    double *out[kMaxBatch];
+};
+
 }  // namespace raysht
 
 struct raysht_plan {
@@ -70,6 +92,8 @@ struct raysht_plan {
     // staging buffers of the host-array entry points, allocated on first use
     double *d_stage_s = nullptr, *d_stage_p = nullptr;
     raysht::raysht_split split;  // used by the BF16X3 / BF16X6 forms only
+    raysht::prob_desc desc{};           // this plan as one problem of a grouped launch
+    raysht::prob_desc *d_desc = nullptr;
 };
 
 struct raysht_cheb {
@@ -96,6 +120,19 @@ int launch_plm_tables(raysht_plan *p, const double *d_x, const double *d_renorm,
                       hipStream_t st);
 int launch_stp_fp64(raysht_plan *p, const double *s, double *p2, hipStream_t st);
 int launch_pts_fp64(raysht_plan *p, const double *p2, double *s, hipStream_t st);
+// grouped launches over the work items of several problems (descriptors in device memory)
+int launch_stp_fp64_batch(const prob_desc *d_desc, const batch_io &io, const work_item *work, int n_work,
+                          hipStream_t st);
+int launch_pts_fp64_batch(const prob_desc *d_desc, const batch_io &io, const work_item *work, int n_work,
+                          hipStream_t st);
+int launch_stp_split_batch(int precision, const prob_desc *d_desc, const batch_io &io, const work_item *work,
+                           int n_work, hipStream_t st);
+int launch_pts_split_batch(int precision, const prob_desc *d_desc, const batch_io &io, const work_item *work,
+                           int n_work, hipStream_t st);
+// XCD-balanced work lists over the m-blocks of several problems (work_item.pad = index
+// into `plans`), for the FP64 and the split kernels
+std::vector<work_item> fp64_work(const std::vector<const raysht_plan *> &plans, bool stp);
+std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans, bool stp);
 // allocates the device twiddle table of length-nphi FFTs (done by the first FFT call
 // otherwise; needed before a stream capture, which may not allocate)
 int fft_prepare(int nphi);

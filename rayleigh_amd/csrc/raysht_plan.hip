@@ -45,6 +45,8 @@ void device_free(T *&ptr)
 int n_even(int lmax, int m) { return (lmax - m + 2) / 2; }
 int n_odd(int lmax, int m) { return (lmax - m + 1) / 2; }
 
+}  // namespace
+
 // Work items of one grouped launch.  Workgroups are dispatched to the 8 XCDs of the
 // chip round-robin by workgroup index, each XCD with its own L2.  Whole m-blocks are
 // assigned to XCDs (longest first, each to the least-loaded XCD) so that the tiles
@@ -53,30 +55,40 @@ int n_odd(int lmax, int m) { return (lmax - m + 1) / 2; }
 // is interleaved so that workgroup b runs item b/8 of XCD b%8, padded with empty
 // items (mp = -1) where XCD queues differ in length.  Inside an m-block the row tiles
 // of one column tile are consecutive, so they find the column tile's input in L2.
-constexpr int kXcds = 8;
-
-std::vector<work_item> make_work(const raysht_plan *p, bool stp)
+// The m-blocks of several problems share one list; `pad` holds the problem index.
+std::vector<work_item> fp64_work(const std::vector<const raysht_plan *> &plans, bool stp)
 {
-    std::vector<int> order(p->nm);
-    for (int i = 0; i < p->nm; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return p->mvals[a] < p->mvals[b]; });
+    constexpr int kXcds = 8;
     const int tc = stp ? stp_tile_cols() : pts_tile_cols();
     const int tr = stp ? stp_tile_rows() : pts_tile_rows();
+    struct blk {
+        int prob, mp;
+        double item;  // cost of one item ~ its K loop: parity-l chunks (StP), node chunks (PtS)
+    };
+    std::vector<blk> order;
+    for (int pi = 0; pi < (int)plans.size(); ++pi) {
+        const raysht_plan *p = plans[pi];
+        for (int mp = 0; mp < p->nm; ++mp)
+            order.push_back(blk{pi, mp, stp ? (double)n_even(p->lmax, p->mvals[mp]) : (double)p->nt2});
+    }
+    // longest first; one problem's blocks keep ascending m among equal costs
+    std::stable_sort(order.begin(), order.end(), [&](const blk &a, const blk &b) {
+        if (a.item != b.item) return a.item > b.item;
+        if (a.prob != b.prob) return a.prob < b.prob;
+        return plans[a.prob]->mvals[a.mp] < plans[b.prob]->mvals[b.mp];
+    });
     std::vector<std::vector<work_item>> q(kXcds);
     std::vector<double> load(kXcds, 0.0);
-    for (int mp : order) {
-        const int m = p->mvals[mp];
-        const int ne = n_even(p->lmax, m);
-        const int rows = stp ? p->nt2 : ne;
+    for (const blk &b : order) {
+        const raysht_plan *p = plans[b.prob];
+        const int rows = stp ? p->nt2 : n_even(p->lmax, p->mvals[b.mp]);
         const int nrt = (rows + tr - 1) / tr, nct = (p->nrhs + tc - 1) / tc;
-        // cost of one item ~ its K loop: parity-l chunks for StP, node chunks for PtS
-        const double item = stp ? (double)ne : (double)p->nt2;
         const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         // column tile outer, row tile inner: consecutive items share the column tile's
         // input (spectral columns for StP, hybrid columns for PtS) while it is in L2
         for (int c = 0; c < nct; ++c)
-            for (int r = 0; r < nrt; ++r) q[x].push_back(work_item{mp, r * tr, c * tc, 0});
-        load[x] += item * nrt * nct;
+            for (int r = 0; r < nrt; ++r) q[x].push_back(work_item{b.mp, r * tr, c * tc, b.prob});
+        load[x] += b.item * nrt * nct;
     }
     size_t len = 0;
     for (auto &v : q) len = std::max(len, v.size());
@@ -86,8 +98,6 @@ std::vector<work_item> make_work(const raysht_plan *p, bool stp)
             if (i < q[x].size()) w[i * kXcds + x] = q[x][i];
     return w;
 }
-
-}  // namespace
 
 void set_last_hip_error(hipError_t e, const char *what)
 {
@@ -130,6 +140,7 @@ void raysht_plan_destroy(raysht_plan *p)
     device_free(p->d_work_pts);
     device_free(p->d_stage_s);
     device_free(p->d_stage_p);
+    device_free(p->d_desc);
     split_plan_free(p);
     delete p;
 }
@@ -137,6 +148,31 @@ void raysht_plan_destroy(raysht_plan *p)
 }  // extern "C"
 
 namespace {
+
+prob_desc describe(const raysht_plan *p)
+{
+    prob_desc d{};
+    d.pt_even = p->d_pt_even;
+    d.pt_odd = p->d_pt_odd;
+    d.ip_even = p->d_ip_even;
+    d.ip_odd = p->d_ip_odd;
+    d.s_off = p->d_s_off;
+    d.te_off = p->d_te_off;
+    d.to_off = p->d_to_off;
+    d.mvals = p->d_mvals;
+    d.frag_stp_off = p->split.d_frag_stp_off;
+    d.frag_pts_off = p->split.d_frag_pts_off;
+    d.exp_pts_off = p->split.d_exp_pts_off;
+    d.tdig_stp = p->split.d_tdig_stp;
+    d.tdig_pts = p->split.d_tdig_pts;
+    d.texp_stp = p->split.d_texp_stp;
+    d.texp_pts = p->split.d_texp_pts;
+    d.lmax = p->lmax;
+    d.nt = p->nt;
+    d.nt2 = p->nt2;
+    d.nrhs = p->nrhs;
+    return d;
+}
 
 // nr = nfields = 0 makes a plan over `ncols` anonymous columns (the rmcontainer
 // 2-D transforms); otherwise ncols = 2 nr nfields.
@@ -204,7 +240,7 @@ int plan_create(raysht_plan **out, int lmax, int ntheta, const int *m_values, in
         amp_dd[2 * m + 1] = amp_lo[m];
     }
 
-    std::vector<work_item> ws = make_work(p, true), wp = make_work(p, false);
+    std::vector<work_item> ws = fp64_work({p}, true), wp = fp64_work({p}, false);
     p->n_work_stp = (int)ws.size();
     p->n_work_pts = (int)wp.size();
 
@@ -236,6 +272,10 @@ int plan_create(raysht_plan **out, int lmax, int ntheta, const int *m_values, in
     device_free(d_rn);
     device_free(d_amp);
     if (rc == RAYSHT_OK && precision != RAYSHT_FP64) rc = split_plan_init(p);
+    if (rc == RAYSHT_OK) {
+        p->desc = describe(p);
+        rc = device_upload(&p->d_desc, &p->desc, 1);
+    }
     if (rc != RAYSHT_OK) {
         raysht_plan_destroy(p);
         return rc;

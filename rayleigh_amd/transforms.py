@@ -151,6 +151,80 @@ class LegendrePlan:
         return tuple(self.table(w) for w in range(4))
 
 
+def _ptr_array(tensors, name: str, sizes):
+    return (ctypes.c_void_p * len(tensors))(*[_dptr(t, f"{name}[{i}]", n).value
+                                               for i, (t, n) in enumerate(zip(tensors, sizes))])
+
+
+class LegendreBatch:
+    """Up to 64 LegendrePlans of one precision transformed by one grouped launch per
+    direction (raysht_batch_*): batch.stp(s_list, p2_list) equals plans[i].stp(s[i],
+    p2[i]) for every i."""
+
+    def __init__(self, plans):
+        plans = list(plans)
+        h = ctypes.c_void_p()
+        arr = (ctypes.c_void_p * len(plans))(*[p.handle.value for p in plans])
+        _lib.check(_lib.lib().raysht_batch_create(ctypes.byref(h), arr, len(plans)), "raysht_batch_create")
+        self._h = h
+        self.plans = plans
+
+    def __len__(self):
+        return len(self.plans)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().raysht_batch_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _io(self, src, dst, src_sizes, dst_sizes, names):
+        if len(src) != len(self.plans) or len(dst) != len(self.plans):
+            raise ValueError("one input and one output per plan")
+        return _ptr_array(src, names[0], src_sizes), _ptr_array(dst, names[1], dst_sizes)
+
+    def stp(self, s, p2, stream=None):
+        a, b = self._io(s, p2, [p.spectral_size for p in self.plans], [p.hybrid_size for p in self.plans],
+                        ("s", "p2"))
+        _lib.check(_lib.lib().raysht_batch_stp(self._h, a, b, _stream(stream)), "raysht_batch_stp")
+
+    def pts(self, p2, s, stream=None):
+        a, b = self._io(p2, s, [p.hybrid_size for p in self.plans], [p.spectral_size for p in self.plans],
+                        ("p2", "s"))
+        _lib.check(_lib.lib().raysht_batch_pts(self._h, a, b, _stream(stream)), "raysht_batch_pts")
+
+    def round_trip_graph(self, s_in, p2, s_out):
+        """The grouped StP -> PtS round trip recorded into a hipGraph."""
+        return BatchGraph(self, s_in, p2, s_out)
+
+
+class BatchGraph:
+    """raysht_graph_create_batch / raysht_graph_launch."""
+
+    def __init__(self, batch: LegendreBatch, s_in, p2, s_out):
+        ss = [p.spectral_size for p in batch.plans]
+        hs = [p.hybrid_size for p in batch.plans]
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().raysht_graph_create_batch(
+            ctypes.byref(h), batch.handle, _ptr_array(s_in, "s_in", ss), _ptr_array(p2, "p2", hs),
+            _ptr_array(s_out, "s_out", ss)), "raysht_graph_create_batch")
+        self._h = h
+        self._keep = (batch, list(s_in), list(p2), list(s_out))
+
+    def launch(self, stream=None):
+        _lib.check(_lib.lib().raysht_graph_launch(self._h, _stream(stream)), "raysht_graph_launch")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.lib().raysht_graph_destroy(h)
+            self._h = None
+
+
 class RoundTripGraph:
     """raysht_graph_create_round_trip / raysht_graph_launch."""
 
