@@ -2,7 +2,7 @@
 
 Workload: l_max = 511, N_theta = 768 Gauss-Legendre nodes, N_r = 192, 8 fields,
 all 512 m in Rayleigh's balanced one-rank order; spectral inputs with the
-config-0 distribution (N(0,1) (l+1)^-2 sin(pi r_i / N_r), m = 0 real), synthetic,
+coefficient distribution N(0,1) (l+1)^-2 sin(pi r_i / N_r) with m = 0 real, synthetic,
 resident in HBM before the timed region.  One step = StP_4d_dgp2 + PtS_4d_dgpv3
 over every m (the HIP library's grouped launches).
 
@@ -43,7 +43,7 @@ def flops_per_direction(lmax=LMAX, nt=NTHETA, nr=NR, nf=NF) -> float:
 
 
 def synthetic_spectral_device(plan, mvals, seed: int):
-    """The config-0 coefficient distribution, generated on the device."""
+    """Coefficients N(0,1) (l+1)^-2 sin(pi r_i / N_r) (m = 0 real), generated on the device."""
     import torch
     g = torch.Generator(device="cuda").manual_seed(seed)
     s = torch.empty(plan.spectral_size, dtype=torch.float64, device="cuda")
@@ -210,7 +210,7 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
     Chebyshev From_Spectral -> StP -> 2a->3a -> c2r, then r2c -> 3b->2b -> PtS -> To_Spectral,
     each stage timed with HIP events; GB/s of the bandwidth-bound stages.  The radial
     buffer is transformed in place of Rayleigh's p1 layout (the 1a<->2a radial transposes
-    between the Chebyshev and Legendre stages are not part of this build)."""
+    between the Chebyshev and Legendre stages are not implemented)."""
     import torch
     from oracle import oracle as O
     L, nt, nphi, nr, nf = STACKS[workload]
@@ -292,7 +292,8 @@ def batch_problems(seed: int = 3, n: int = 64):
 
 
 def spectral_device(plan, lmax, nr, nf, seed: int):
-    """The config-0 coefficient distribution for all m in ascending order, on the device."""
+    """Coefficients N(0,1) (l+1)^-2 sin(pi r_i / N_r) (m = 0 real) for all m in ascending
+    order, on the device."""
     import torch
     g = torch.Generator(device="cuda").manual_seed(seed)
     s = torch.empty(plan.spectral_size, dtype=torch.float64, device="cuda")
@@ -338,6 +339,8 @@ def batch_bench(ra, steps: int, warmup: int, forms):
         batch = ra.LegendreBatch(plans)
         p2 = [torch.empty(p.hybrid_size, dtype=torch.float64, device="cuda") for p in plans]
         s2 = [torch.empty(p.spectral_size, dtype=torch.float64, device="cuda") for p in plans]
+        tune = batch.autotune(s_in, p2, s2, reps=2)
+        log(f"batch {form}: schedule candidates (ms) {tune}, chosen {batch.schedule()}")
         for _ in range(warmup):
             batch.stp(s_in, p2)
             batch.pts(p2, s2)
@@ -373,7 +376,8 @@ def batch_bench(ra, steps: int, warmup: int, forms):
         torch.cuda.synchronize()
         serial_ms = a.elapsed_time(b) / steps
         del g
-        rec = {"stp_ms": stp_ms, "pts_ms": pts_ms, "graph_round_trip_ms": graph_ms,
+        rec = {"autotune_ms": tune, "schedule": batch.schedule(),
+               "stp_ms": stp_ms, "pts_ms": pts_ms, "graph_round_trip_ms": graph_ms,
                "serial_round_trip_ms": serial_ms, "tflops_graph": 2 * F / (graph_ms * 1e-3) / 1e12}
         if form == "fp64":
             rec["frac_fp64_peak"] = rec["tflops_graph"] / PEAK_FP64_TFLOPS
@@ -463,7 +467,7 @@ def main():
                     help="also run one plain 1 GiB device-to-device copy (scales profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
 

@@ -38,7 +38,7 @@ enum raysht_status {
     RAYSHT_EINVAL = 1,      /* bad size, layout or argument */
     RAYSHT_EHIP = 2,        /* the HIP runtime reported an error */
     RAYSHT_ENOMEM = 3,      /* device or host allocation failed */
-    RAYSHT_EUNSUPPORTED = 4 /* valid request this build does not implement */
+    RAYSHT_EUNSUPPORTED = 4 /* valid request the library does not implement */
 };
 
 /* Arithmetic used by the Legendre contractions. */
@@ -152,6 +152,18 @@ void raysht_batch_destroy(raysht_batch *batch);
 int raysht_batch_size(const raysht_batch *batch);
 int raysht_batch_stp(raysht_batch *batch, const double *const *s, double *const *p2, void *stream);
 int raysht_batch_pts(raysht_batch *batch, const double *const *p2, double *const *s, void *stream);
+/* Schedule autotuner of a batch.  Candidates (index bits): bit 0 = no half-height last
+ * row tiles in the FP64 PtS kernel, bit 1 = no XCD grouping of m-blocks (items go out
+ * longest first and the dispatcher spreads them over the XCDs); 0 is the default.
+ * raysht_batch_autotune times every distinct candidate of each direction on the given
+ * buffers (one warm-up and `reps` timed grouped launches on `stream`; outputs are
+ * overwritten) and keeps the fastest; ms_out (optional, 8 doubles: [direction][candidate],
+ * direction 0 = StP, 1 = PtS) receives the mean times, -1 where not timed.  Results of a
+ * batch do not depend on its schedule.  No reference counterpart. */
+int raysht_batch_autotune(raysht_batch *batch, const double *const *s, double *const *p2,
+                          double *const *s_out, int reps, void *stream, double *ms_out);
+int raysht_batch_schedule(const raysht_batch *batch, int direction);
+int raysht_batch_set_schedule(raysht_batch *batch, int direction, int candidate);
 /* The batch's Legendre round trip (grouped StP into p2[i], grouped PtS into s_out[i])
  * recorded into a hipGraph, launched with raysht_graph_launch.  No reference
  * counterpart. */

@@ -53,6 +53,9 @@ SIGNATURES = {
     "raysht_batch_size": (_I, [_P]),
     "raysht_batch_stp": (_I, [_P, _P, _P, _P]),
     "raysht_batch_pts": (_I, [_P, _P, _P, _P]),
+    "raysht_batch_autotune": (_I, [_P, _P, _P, _P, _I, _P, _P]),
+    "raysht_batch_schedule": (_I, [_P, _I]),
+    "raysht_batch_set_schedule": (_I, [_P, _I, _I]),
     "raysht_graph_destroy": (None, [_P]),
     "raysht_cheb_create": (_I, [ctypes.POINTER(_P), _I, _D, _D, _P]),
     "raysht_cheb_destroy": (None, [_P]),

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(STP_THREADS, RAYSHT_FP64_MIN_WAVES) void stp_fp64_k
 
     const int mp = uniform(work[blockIdx.x].mp);
     if (mp < 0) return;  // padding item of the XCD-balanced schedule
-    const int pi = uniform(work[blockIdx.x].pad);
+    const int pi = uniform(work[blockIdx.x].pad) & 0xffff;
     const prob_desc &P = probs[pi];
     const int lmax = uniform(P.lmax), nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
     double *p2 = uniform(io.out[pi]);
@@ -241,32 +241,14 @@ struct pts_item {
     __amdgpu_buffer_rsrc_t rp, rie, rio;
 };
 
-__global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_kernel(
-    const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
+// One PtS tile of NBR 16-row blocks of each parity (NBR = 4: the full 64-row tile; 2: a
+// 32-row tile for the last rows of an m-block, which would leave half of a full tile
+// idle).  LDS buffers come from the kernel so that both shapes share them.
+template <int NBR>
+__device__ __forceinline__ void pts_tile(const pts_item &cur, int nt, int nt2, int nrhs, double *__restrict__ sb,
+                                         double *__restrict__ Fe, double *__restrict__ Fo, double *__restrict__ Ie,
+                                         double *__restrict__ Io)
 {
-    __shared__ double Fe[PTS_TC * PTS_F_LD];
-    __shared__ double Fo[PTS_TC * PTS_F_LD];
-    __shared__ double Ie[PTS_KJ * PTS_I_LD];
-    __shared__ double Io[PTS_KJ * PTS_I_LD];
-
-    const int mp = uniform(work[blockIdx.x].mp);
-    if (mp < 0) return;  // padding item of the XCD-balanced schedule
-    const int pi = uniform(work[blockIdx.x].pad);
-    const prob_desc &P = probs[pi];
-    const int lmax = uniform(P.lmax), nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
-    double *s = uniform(io.out[pi]);
-    const long s_base = uniform(uniform(P.s_off)[mp]);
-    pts_item cur;
-    cur.mp = mp;
-    cur.k0 = uniform(work[blockIdx.x].r0);
-    cur.c0 = uniform(work[blockIdx.x].c0);
-    cur.nl = lmax - uniform(uniform(P.mvals)[mp]) + 1;
-    cur.ne = (cur.nl + 1) >> 1;
-    cur.no = cur.nl >> 1;
-    cur.rp = make_rsrc(uniform(io.in[pi]) + (long)mp * nrhs * nt, (long)nrhs * nt * 8);
-    cur.rie = make_rsrc(uniform(P.ip_even) + uniform(uniform(P.te_off)[mp]), (long)cur.ne * nt2 * 8);
-    cur.rio = make_rsrc(uniform(P.ip_odd) + uniform(uniform(P.to_off)[mp]), (long)cur.no * nt2 * 8);
-
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
     // Per-thread load geometry.  F: column fcc0 + (PTS_THREADS / KJ) i, node j0 + fjj and
@@ -290,7 +272,7 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
         }
         const int vi = ((it.k0 + fcc0) * nt2 + j) * 8;
 #pragma unroll
-        for (int i = 0; i < PTS_I_PER_T; ++i) {
+        for (int i = 0; i < PTS_I_PER_T * NBR / 4; ++i) {
             const double e = load_f64(it.rie, vi, i * i_sstep);
             const double o = load_f64(it.rio, vi, i * i_sstep);
             rie[i] = jin ? e : 0.0;
@@ -306,7 +288,7 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
             Fo[at] = rfa[i] - rfb[i];
         }
 #pragma unroll
-        for (int i = 0; i < PTS_I_PER_T; ++i) {
+        for (int i = 0; i < PTS_I_PER_T * NBR / 4; ++i) {
             const int idx = tid + i * PTS_THREADS;
             const int at = (idx % PTS_KJ) * PTS_I_LD + idx / PTS_KJ;
             Ie[at] = rie[i];
@@ -316,11 +298,11 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
 
     fetch(cur, 0);
     {
-        double4_t acc_e[PTS_CB][4], acc_o[PTS_CB][4];
+        double4_t acc_e[PTS_CB][NBR], acc_o[PTS_CB][NBR];
 #pragma unroll
         for (int cb = 0; cb < PTS_CB; ++cb)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
+            for (int b = 0; b < NBR; ++b) {
                 acc_e[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
                 acc_o[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
             }
@@ -330,9 +312,9 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
             if (j0 + PTS_KJ < nt2) fetch(cur, j0 + PTS_KJ);
 #pragma unroll 1
             for (int jj = 0; jj < PTS_KJ; jj += 4) {
-                double be[4], bo[4];
+                double be[NBR], bo[NBR];
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
+                for (int b = 0; b < NBR; ++b) {
                     be[b] = Ie[(jj + lg) * PTS_I_LD + 16 * b + li];
                     bo[b] = Io[(jj + lg) * PTS_I_LD + 16 * b + li];
                 }
@@ -342,7 +324,7 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
                     const double a_e = Fe[row];
                     const double a_o = Fo[row];
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) {
+                    for (int b = 0; b < NBR; ++b) {
                         acc_e[cb][b] = mfma_f64(a_e, be[b], acc_e[cb][b]);
                         acc_o[cb][b] = mfma_f64(a_o, bo[b], acc_o[cb][b]);
                     }
@@ -351,11 +333,10 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
             __syncthreads();
         }
 
-        double *sb = s + s_base;
 #pragma unroll
         for (int cb = 0; cb < PTS_CB; ++cb)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
+            for (int b = 0; b < NBR; ++b) {
                 const int k = cur.k0 + 16 * b + li;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
@@ -367,6 +348,36 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
                 }
             }
     }
+}
+
+__global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_kernel(
+    const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
+{
+    __shared__ double Fe[PTS_TC * PTS_F_LD];
+    __shared__ double Fo[PTS_TC * PTS_F_LD];
+    __shared__ double Ie[PTS_KJ * PTS_I_LD];
+    __shared__ double Io[PTS_KJ * PTS_I_LD];
+
+    const int mp = uniform(work[blockIdx.x].mp);
+    if (mp < 0) return;  // padding item of the XCD-balanced schedule
+    const int pad = uniform(work[blockIdx.x].pad), pi = pad & 0xffff;
+    const prob_desc &P = probs[pi];
+    const int lmax = uniform(P.lmax), nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
+    double *sb = uniform(io.out[pi]) + uniform(uniform(P.s_off)[mp]);
+    pts_item cur;
+    cur.mp = mp;
+    cur.k0 = uniform(work[blockIdx.x].r0);
+    cur.c0 = uniform(work[blockIdx.x].c0);
+    cur.nl = lmax - uniform(uniform(P.mvals)[mp]) + 1;
+    cur.ne = (cur.nl + 1) >> 1;
+    cur.no = cur.nl >> 1;
+    cur.rp = make_rsrc(uniform(io.in[pi]) + (long)mp * nrhs * nt, (long)nrhs * nt * 8);
+    cur.rie = make_rsrc(uniform(P.ip_even) + uniform(uniform(P.te_off)[mp]), (long)cur.ne * nt2 * 8);
+    cur.rio = make_rsrc(uniform(P.ip_odd) + uniform(uniform(P.to_off)[mp]), (long)cur.no * nt2 * 8);
+    if (pad >> 16)
+        pts_tile<2>(cur, nt, nt2, nrhs, sb, Fe, Fo, Ie, Io);
+    else
+        pts_tile<4>(cur, nt, nt2, nrhs, sb, Fe, Fo, Ie, Io);
 }
 
 }  // namespace

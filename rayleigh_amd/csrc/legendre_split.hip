@@ -281,15 +281,39 @@ __device__ __forceinline__ void flush(f32x16 (&acc)[F::NG], double (&dacc)[16])
     for (int g = 0; g < F::NG; ++g) acc[g] = f32x16{};
 }
 
-// Slice s of x, both relative to the row/column scale (|x| < 2^-8s): the nearest
-// multiple of 2^(-7-8s), by the add-and-subtract of 1.5 * 2^(45-8s), is removed from x
-// and returned as its BF16 bit pattern (at most 8 significant bits: exact).
-__device__ __forceinline__ unsigned grid_slice(double &x, int s)
+// The data operand's NA digits, from one integer per value.  For |v| < 2^e,
+// X = round(v 2^(8 NA - 1 - e)) (|X| <= 2^(8 NA - 1)) is formed from two FP64 steps (floor of
+// v 2^(8 NA - 33 - e), then the rounded 32-bit remainder) and held with 128 added to each
+// of its NA - 1 low bytes: byte NA-1-s (s = 1 .. NA-1) is then d_s + 128 and the arithmetic
+// top bits are d_0, so that v 2^-e = sum_s d_s 2^(-7-8s) + O(2^(-8 NA)) with every
+// |d_s| <= 128 (exact in BF16).  Each digit then costs one byte conversion and one FMA
+// instead of the three FP64 adds and a conversion of a slice taken off the remainder.
+template <int NA>
+__device__ __forceinline__ unsigned long long offset_digits(double v, int e)
 {
-    const double c = __builtin_ldexp(1.5, 45 - 8 * s);
-    const double v = (x + c) - c;
-    x -= v;
-    return __builtin_bit_cast(unsigned, (float)v) >> 16;
+    static_assert(NA >= 2 && NA <= 7, "digits of one 64-bit integer");
+    const double t = __builtin_ldexp(v, 8 * NA - 33 - e);  // |t| < 2^(8 NA - 32)
+    const double h = __builtin_floor(t);
+    const double r = __builtin_fmin(__builtin_fma(t - h, 4294967296.0, 0.5), 4294967295.0);
+    const unsigned long long X = ((unsigned long long)(unsigned)(int)h << 32) | (unsigned)r;
+    return X + (0x0000808080808080ull >> (8 * (7 - NA)));
+}
+
+// digit s of an offset_digits<NA> value, times 2^(-7-8s), as FP32 (exact)
+template <int NA>
+__device__ __forceinline__ float digit(unsigned long long X, int s)
+{
+    const float k = __builtin_ldexpf(1.0f, -7 - 8 * s);
+    if (s == 0) return (float)(int)((long long)X >> (8 * (NA - 1))) * k;
+    return __builtin_fmaf((float)(unsigned)((X >> (8 * (NA - 1 - s))) & 0xffu), k, -128.0f * k);
+}
+
+// two FP32 values (exact in BF16) as one dword of two BF16
+__device__ __forceinline__ unsigned pack_bf16(float lo, float hi)
+{
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2_t{lo, hi}), bf16x2_t));
 }
 
 // LDS slot of MFMA fragment lane L: lanes 32..63 are swapped in groups of 8, so that
@@ -298,42 +322,23 @@ __device__ __forceinline__ unsigned grid_slice(double &x, int s)
 // cover distinct banks.
 __device__ __forceinline__ int frag_slot(int L) { return L ^ ((L >> 5) << 3); }
 
-// Writes the NA slices of two consecutive K values of one data column into the A
-// fragments of an LDS buffer laid out [block][slice][slot][8 BF16]: fragment lane
-// L = row % 32 + 32 * (k0 / 8), elements k0 % 8 and k0 % 8 + 1 (k0 even, within the step).
-template <int NA>
-__device__ __forceinline__ void put_pair(u32x4 *__restrict__ buf, int blk, int row, int k0, double v0, double v1,
-                                         int e)
-{
-    double x0 = __builtin_ldexp(v0, -e), x1 = __builtin_ldexp(v1, -e);
-    unsigned *b32 = reinterpret_cast<unsigned *>(buf);
-    const int at = ((blk * NA) * 64 + frag_slot((row & 31) + 32 * (k0 >> 3))) * 4 + ((k0 & 7) >> 1);
-#pragma unroll
-    for (int s = 0; s < NA; ++s) {
-        const unsigned h0 = grid_slice(x0, s), h1 = grid_slice(x1, s);
-        b32[at + s * 256] = h0 | (h1 << 16);
-    }
-}
-
-// The slicing of two consecutive K values into NA slices, one slice per step() call,
-// so that the calls can be spread between the MFMAs of the previous step.
+// The slicing of two consecutive K values of one data column into the A fragments of
+// an LDS buffer laid out [block][slice][slot][8 BF16] (fragment lane L = row % 32 +
+// 32 * (k0 / 8), elements k0 % 8 and k0 % 8 + 1, k0 even within the step), one slice per
+// step() call, so that the calls can be spread between the MFMAs of the previous step.
 template <int NA>
 struct pair_slicer {
-    double x0, x1;
+    unsigned long long X0, X1;
     unsigned *p;
     __device__ __forceinline__ void init(u32x4 *__restrict__ buf, int blk, int row, int k0, double v0, double v1,
                                          int e)
     {
-        x0 = __builtin_ldexp(v0, -e);
-        x1 = __builtin_ldexp(v1, -e);
+        X0 = offset_digits<NA>(v0, e);
+        X1 = offset_digits<NA>(v1, e);
         p = reinterpret_cast<unsigned *>(buf) + ((blk * NA) * 64 + frag_slot((row & 31) + 32 * (k0 >> 3))) * 4 +
             ((k0 & 7) >> 1);
     }
-    __device__ __forceinline__ void step(int s)
-    {
-        const unsigned h0 = grid_slice(x0, s), h1 = grid_slice(x1, s);
-        p[s * 256] = h0 | (h1 << 16);
-    }
+    __device__ __forceinline__ void step(int s) { p[s * 256] = pack_bf16(digit<NA>(X0, s), digit<NA>(X1, s)); }
 };
 
 template <int NA, int N>
@@ -406,31 +411,40 @@ __device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, i
 #pragma unroll
         for (int sc = 0; sc < F::NA; ++sc) sl.step(sc);
     }
-    raw_step r1 = fetch(1);
-    u32x4 b[F::NB];
-    get_b<F::NB>(rt, f0, lane, b);
+    // two register sets for the raw data and the table fragments, swapped by unrolling
+    // the step loop by two (no copies between steps)
+    raw_step rA = fetch(1), rB;
+    u32x4 bA[F::NB], bB[F::NB];
+    get_b<F::NB>(rt, f0, lane, bA);
     __syncthreads();
     f32x16 acc[F::NG];
 #pragma unroll
     for (int g = 0; g < F::NG; ++g) acc[g] = f32x16{};
+    // step ks: raw data of step ks + 1 in rc, table fragments of step ks in bc; loads
+    // step ks + 2's raw data into rn and step ks + 1's fragments into bn
+    auto step = [&](int ks, const raw_step &rc, raw_step &rn, const u32x4 (&bc)[F::NB], u32x4 (&bn)[F::NB]) {
+        rn = fetch(ks + 2);
+        get_b<F::NB>(rt, f0 + (ks + 1) * F::NB, lane, bn);
+        u32x4 a[F::NA];
+        get_a<F::NA>(abuf[ks & 1], ablk, lane, a);
+        auto sl = slice(rc, abuf[(ks + 1) & 1]);
+        mfma_pairs_sliced<F>(acc, a, bc, sl);
+        __syncthreads();
+    };
     for (int kc = 0; kc < nks; kc += kChunkSteps) {
         const int kend = min(nks, kc + kChunkSteps);
-        for (int ks = kc; ks < kend; ++ks) {
-            const raw_step r2 = fetch(ks + 2);
-            u32x4 bn[F::NB];
-            get_b<F::NB>(rt, f0 + (ks + 1) * F::NB, lane, bn);
-            u32x4 a[F::NA];
-            get_a<F::NA>(abuf[ks & 1], ablk, lane, a);
-            auto sl = slice(r1, abuf[(ks + 1) & 1]);
-            mfma_pairs_sliced<F>(acc, a, b, sl);
-#pragma unroll
-            for (int t = 0; t < F::NB; ++t) b[t] = bn[t];
-            r1 = r2;
-            __syncthreads();
+        int ks = kc;
+        for (; ks + 1 < kend; ks += 2) {
+            step(ks, rA, rB, bA, bB);
+            step(ks + 1, rB, rA, bB, bA);
+        }
+        if (ks < kend) {  // odd step count: only in the last chunk (kChunkSteps is even)
+            step(ks, rA, rB, bA, bB);
         }
         flush<F>(acc, dacc);
     }
 }
+static_assert(kChunkSteps % 2 == 0, "chunks hold whole step pairs");
 
 // --------------------------------------------------------------- StP
 // Work item (mp, c0): kTileC columns, every theta row, in tiles of kTileR rows with
@@ -447,7 +461,7 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
     __shared__ int cexp[kTileC];
     const int mp = uniform(work[blockIdx.x].mp);
     if (mp < 0) return;
-    const int pi = uniform(work[blockIdx.x].pad), c0 = uniform(work[blockIdx.x].c0);
+    const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
     const prob_desc &P = probs[pi];
     const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
     const long *frag_off = uniform(P.frag_stp_off);
@@ -548,7 +562,7 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void pts_split_kernel(
     __shared__ int cexp[2][kTileC];
     const int mp = uniform(work[blockIdx.x].mp);
     if (mp < 0) return;
-    const int pi = uniform(work[blockIdx.x].pad), c0 = uniform(work[blockIdx.x].c0);
+    const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
     const prob_desc &P = probs[pi];
     const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
     const long *frag_off = uniform(P.frag_pts_off);
@@ -638,9 +652,8 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void pts_split_kernel(
 // XCD-balanced work lists of (problem, m-block, column tile) items: whole m-blocks dealt
 // to the eight XCDs by longest-processing-time, interleaved so that item i runs on
 // XCD i % 8.  The problem index travels in the `pad` field.
-std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans, bool stp)
+std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans, bool stp, sched_opts o)
 {
-    constexpr int kXcds = 8;
     struct blk {
         int prob, mp;
         double item;  // K steps of one item (plus the exponent pass)
@@ -657,21 +670,16 @@ std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans,
         }
     }
     std::stable_sort(order.begin(), order.end(), [](const blk &a, const blk &b) { return a.item > b.item; });
-    std::vector<std::vector<work_item>> q(kXcds);
-    std::vector<double> load(kXcds, 0.0);
+    std::vector<std::vector<work_item>> blocks;
+    std::vector<double> cost;
     for (const blk &b : order) {
         const int nct = cdiv(plans[b.prob]->nrhs, kTileC);
-        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        for (int c = 0; c < nct; ++c) q[x].push_back(work_item{b.mp, 0, c * kTileC, b.prob});
-        load[x] += b.item * nct;
+        std::vector<work_item> items;
+        for (int c = 0; c < nct; ++c) items.push_back(work_item{b.mp, 0, c * kTileC, b.prob});
+        blocks.push_back(std::move(items));
+        cost.push_back(b.item * nct);
     }
-    size_t len = 0;
-    for (auto &v : q) len = std::max(len, v.size());
-    std::vector<work_item> w(len * kXcds, work_item{-1, 0, 0, 0});
-    for (size_t i = 0; i < len; ++i)
-        for (int x = 0; x < kXcds; ++x)
-            if (i < q[x].size()) w[i * kXcds + x] = q[x][i];
-    return w;
+    return deal_to_xcds(blocks, cost, o.xcd_groups);
 }
 
 namespace {

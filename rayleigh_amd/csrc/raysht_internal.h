@@ -27,7 +27,9 @@ struct dd_pair {
     double hi, lo;
 };
 
-// One unit of grouped-GEMM work: m-block mp, row tile start, column tile start.
+// One unit of grouped-GEMM work: m-block mp, row tile start, column tile start, and in
+// `pad` the problem index (low 16 bits) and the tile shape (high bits: 0 = the full
+// tile, 1 = the PtS half-height tile for the last rows of an m-block).
 struct work_item {
     int mp;
     int r0;
@@ -131,8 +133,19 @@ int launch_pts_split_batch(int precision, const prob_desc *d_desc, const batch_i
                            int n_work, hipStream_t st);
 // XCD-balanced work lists over the m-blocks of several problems (work_item.pad = index
 // into `plans`), for the FP64 and the split kernels
-std::vector<work_item> fp64_work(const std::vector<const raysht_plan *> &plans, bool stp);
-std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans, bool stp);
+// Schedule options of a work list: PtS half-height last tiles (FP64 kernels), and
+// whether whole m-blocks are kept on one XCD (else the items go out in LPT order and
+// the dispatcher spreads consecutive items over the XCDs).
+struct sched_opts {
+    bool half_tiles = true;
+    bool xcd_groups = true;
+};
+std::vector<work_item> fp64_work(const std::vector<const raysht_plan *> &plans, bool stp, sched_opts o = {});
+std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans, bool stp, sched_opts o = {});
+// deals per-block item lists to the XCDs (LPT over the block costs, blocks in the given
+// order) and interleaves them so that item i runs on XCD i % 8, or concatenates them
+std::vector<work_item> deal_to_xcds(const std::vector<std::vector<work_item>> &blocks,
+                                    const std::vector<double> &cost, bool xcd_groups);
 // allocates the device twiddle table of length-nphi FFTs (done by the first FFT call
 // otherwise; needed before a stream capture, which may not allocate)
 int fft_prepare(int nphi);

@@ -53,12 +53,38 @@ int n_odd(int lmax, int m) { return (lmax - m + 1) / 2; }
 // of one m-block, which share its P_lm table and its input columns, run on one XCD
 // and hit in its L2; within an XCD the largest contractions come first.  The list
 // is interleaved so that workgroup b runs item b/8 of XCD b%8, padded with empty
-// items (mp = -1) where XCD queues differ in length.  Inside an m-block the row tiles
-// of one column tile are consecutive, so they find the column tile's input in L2.
-// The m-blocks of several problems share one list; `pad` holds the problem index.
-std::vector<work_item> fp64_work(const std::vector<const raysht_plan *> &plans, bool stp)
+// items (mp = -1) where XCD queues differ in length.  Without XCD groups the blocks'
+// items are concatenated in the same longest-first order.
+std::vector<work_item> deal_to_xcds(const std::vector<std::vector<work_item>> &blocks,
+                                    const std::vector<double> &cost, bool xcd_groups)
 {
     constexpr int kXcds = 8;
+    if (!xcd_groups) {
+        std::vector<work_item> w;
+        for (const auto &b : blocks) w.insert(w.end(), b.begin(), b.end());
+        return w;
+    }
+    std::vector<std::vector<work_item>> q(kXcds);
+    std::vector<double> load(kXcds, 0.0);
+    for (size_t i = 0; i < blocks.size(); ++i) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        q[x].insert(q[x].end(), blocks[i].begin(), blocks[i].end());
+        load[x] += cost[i];
+    }
+    size_t len = 0;
+    for (auto &v : q) len = std::max(len, v.size());
+    std::vector<work_item> w(len * kXcds, work_item{-1, 0, 0, 0});
+    for (size_t i = 0; i < len; ++i)
+        for (int x = 0; x < kXcds; ++x)
+            if (i < q[x].size()) w[i * kXcds + x] = q[x][i];
+    return w;
+}
+
+// The FP64 kernels' work list.  Inside an m-block the row tiles of one column tile are
+// consecutive, so they find the column tile's input in L2.  The m-blocks of several
+// problems share one list; `pad` holds the problem index and the tile shape.
+std::vector<work_item> fp64_work(const std::vector<const raysht_plan *> &plans, bool stp, sched_opts o)
+{
     const int tc = stp ? stp_tile_cols() : pts_tile_cols();
     const int tr = stp ? stp_tile_rows() : pts_tile_rows();
     struct blk {
@@ -77,26 +103,24 @@ std::vector<work_item> fp64_work(const std::vector<const raysht_plan *> &plans, 
         if (a.prob != b.prob) return a.prob < b.prob;
         return plans[a.prob]->mvals[a.mp] < plans[b.prob]->mvals[b.mp];
     });
-    std::vector<std::vector<work_item>> q(kXcds);
-    std::vector<double> load(kXcds, 0.0);
+    std::vector<std::vector<work_item>> blocks;
+    std::vector<double> cost;
     for (const blk &b : order) {
         const raysht_plan *p = plans[b.prob];
         const int rows = stp ? p->nt2 : n_even(p->lmax, p->mvals[b.mp]);
         const int nrt = (rows + tr - 1) / tr, nct = (p->nrhs + tc - 1) / tc;
-        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        // PtS: a last row tile of at most half the height runs as the half-height shape
+        const bool half = !stp && o.half_tiles && rows - (nrt - 1) * tr <= tr / 2;
+        std::vector<work_item> items;
         // column tile outer, row tile inner: consecutive items share the column tile's
         // input (spectral columns for StP, hybrid columns for PtS) while it is in L2
         for (int c = 0; c < nct; ++c)
-            for (int r = 0; r < nrt; ++r) q[x].push_back(work_item{b.mp, r * tr, c * tc, b.prob});
-        load[x] += b.item * nrt * nct;
+            for (int r = 0; r < nrt; ++r)
+                items.push_back(work_item{b.mp, r * tr, c * tc, b.prob | ((half && r == nrt - 1) << 16)});
+        blocks.push_back(std::move(items));
+        cost.push_back(b.item * (nrt - (half ? 0.5 : 0.0)) * nct);
     }
-    size_t len = 0;
-    for (auto &v : q) len = std::max(len, v.size());
-    std::vector<work_item> w(len * kXcds, work_item{-1, 0, 0, 0});
-    for (size_t i = 0; i < len; ++i)
-        for (int x = 0; x < kXcds; ++x)
-            if (i < q[x].size()) w[i * kXcds + x] = q[x][i];
-    return w;
+    return deal_to_xcds(blocks, cost, o.xcd_groups);
 }
 
 void set_last_hip_error(hipError_t e, const char *what)

@@ -201,6 +201,27 @@ class LegendreBatch:
         """The grouped StP -> PtS round trip recorded into a hipGraph."""
         return BatchGraph(self, s_in, p2, s_out)
 
+    def autotune(self, s, p2, s_out, reps: int = 3, stream=None):
+        """Times the schedule candidates of both directions on these buffers (outputs are
+        overwritten) and keeps the fastest; returns {"stp": {candidate: ms}, "pts": {...}}."""
+        ss = [p.spectral_size for p in self.plans]
+        hs = [p.hybrid_size for p in self.plans]
+        ms = (ctypes.c_double * 8)()
+        _lib.check(_lib.lib().raysht_batch_autotune(
+            self._h, _ptr_array(s, "s", ss), _ptr_array(p2, "p2", hs), _ptr_array(s_out, "s_out", ss),
+            int(reps), _stream(stream), ms), "raysht_batch_autotune")
+        return {d: {c: ms[4 * i + c] for c in range(4) if ms[4 * i + c] >= 0.0}
+                for i, d in enumerate(("stp", "pts"))}
+
+    def schedule(self):
+        """The schedule candidate in use per direction."""
+        return {d: _lib.lib().raysht_batch_schedule(self._h, i) for i, d in enumerate(("stp", "pts"))}
+
+    def set_schedule(self, stp: int | None = None, pts: int | None = None):
+        for i, c in enumerate((stp, pts)):
+            if c is not None:
+                _lib.check(_lib.lib().raysht_batch_set_schedule(self._h, i, int(c)), "raysht_batch_set_schedule")
+
 
 class BatchGraph:
     """raysht_graph_create_batch / raysht_graph_launch."""

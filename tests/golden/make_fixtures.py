@@ -7,8 +7,8 @@ Chebyshev_Polynomials modules (with the bind(C) harness oracle/ref_harness.F90):
     python tests/golden/make_fixtures.py
 
 Every array in the .npz files is an input chosen here or an output of the
-reference's routines on that input; nothing is computed by this build's code
-except the seeded synthetic inputs (oracle.synthetic_spectral, the config-0
+reference's routines on that input; nothing is computed by the library's code
+except the seeded synthetic inputs (oracle.synthetic_spectral, the (l+1)^-2
 distribution) and the balanced m order (M_Balance_v0 on one rank).
 """
 import os
@@ -65,7 +65,7 @@ if __name__ == "__main__":
         sys.exit("compile the reference checker first: make -C oracle ref")
     # the reference's own unit-test grid (tests/unit_tests/SHT/main_input: N_theta=64 -> l_max=41)
     legendre_case("legendre_L41_nt64.npz", 64, 41, O.m_order_balanced(41), 2, 1, seed=41)
-    # config-0 grid (l_max=63, N_theta=96), balanced m order, reduced radial/field count
+    # the l_max=63, N_theta=96 grid, balanced m order, reduced radial/field count
     legendre_case("legendre_L63_nt96.npz", 96, 63, O.m_order_balanced(63), 2, 1, seed=63, sample_tables=4000)
     # an arbitrary partial m list (one rank of a row: m values not contiguous)
     legendre_case("legendre_L63_partial.npz", 96, 63, np.array([17, 0, 63, 5, 40], np.int32), 3, 2, seed=5)

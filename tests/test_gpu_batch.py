@@ -71,8 +71,47 @@ def test_batch_equals_single_plans(ra, precision):
     mv = m_values(lmax, kind)
     tabs = O.legendre_tables(nt, lmax, mv)
     p_ref = O.stp(nt, lmax, mv, nr, nf, tabs, inputs[0])
-    tol = {"fp64": 1e-13, "bf16x6": 1e-12, "bf16x3": 5e-5}[precision]
+    tol = {"fp64": 1e-13, "bf16x6": 1e-12, "bf16x3": 1e-4}[precision]
     assert rel_max(host(p_b[0]), p_ref) <= tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x6"])
+def test_batch_schedules_and_autotune(ra, precision):
+    """Every schedule candidate gives the same bits; the autotuner times the distinct
+    candidates, keeps one of them, and the results stay the same."""
+    plans, inputs = make(ra, precision)
+    batch = ra.LegendreBatch(plans)
+    s = [dev(x) for x in inputs]
+    want_p, want_s = [], []
+    for i, p in enumerate(plans):
+        p1 = empty(p.hybrid_size)
+        p.stp(s[i], p1)
+        s1 = empty(p.spectral_size)
+        p.pts(p1, s1)
+        want_p.append(host(p1))
+        want_s.append(host(s1))
+    p2 = [empty(p.hybrid_size) for p in plans]
+    s2 = [empty(p.spectral_size) for p in plans]
+    for c in range(4):
+        batch.set_schedule(stp=c, pts=c)
+        batch.stp(s, p2)
+        batch.pts(p2, s2)
+        for i in range(len(plans)):
+            assert np.array_equal(host(p2[i]), want_p[i]), (c, i)
+            assert np.array_equal(host(s2[i]), want_s[i]), (c, i)
+    times = batch.autotune(s, p2, s2, reps=2)
+    assert set(times["stp"]) == {0, 2}
+    assert set(times["pts"]) == ({0, 1, 2, 3} if precision == "fp64" else {0, 2})
+    chosen = batch.schedule()
+    assert chosen["stp"] == min(times["stp"], key=times["stp"].get)
+    assert chosen["pts"] == min(times["pts"], key=times["pts"].get)
+    batch.stp(s, p2)
+    batch.pts(p2, s2)
+    for i in range(len(plans)):
+        assert np.array_equal(host(p2[i]), want_p[i]) and np.array_equal(host(s2[i]), want_s[i]), i
+    from rayleigh_amd._lib import RayshtError
+    with pytest.raises(RayshtError):
+        batch.set_schedule(stp=4)
 
 
 def test_batch_graph_replays(ra):

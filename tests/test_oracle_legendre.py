@@ -130,7 +130,7 @@ def test_stp_pts_vs_reference(golden, name):
     nt, L, mv, nr, nf = int(g["nt"]), int(g["lmax"]), g["mvals"], int(g["nr"]), int(g["nf"])
     t = O.legendre_tables(nt, L, mv)
     s = O.synthetic_spectral(L, mv, nr, nf, int(g["seed"]))
-    assert np.array_equal(s, g["s_in"])  # the fixture's input is the seeded config-0 distribution
+    assert np.array_equal(s, g["s_in"])  # the fixture's input is the seeded (l+1)^-2 distribution
     p = O.stp(nt, L, mv, nr, nf, t, s)
     assert rel(p, g["p_out"]) < 1e-14
     s2 = O.pts(nt, L, mv, nr, nf, t, g["p_out"])
