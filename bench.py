@@ -29,9 +29,11 @@ sys.path.insert(0, ROOT)
 LMAX, NTHETA, NR, NF = 511, 768, 192, 8
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 matrix, dense
 PEAK_BF16_TFLOPS = 2500.0    # MI355X BF16 matrix, dense
+PEAK_INT8_TOPS = 5000.0      # MI355X INT8 matrix, dense
 METRIC = "SHT round-trip ms + TFLOP/s at l_max=511: FP64 vs split-BF16 at 1e-12 rel err"
-# BF16 slice pairs (one MFMA product each) per direction of the split forms
-SPLIT_PAIRS = {"bf16x6": {"stp": 27, "pts": 21}, "bf16x3": {"stp": 6, "pts": 6}}
+# slice pairs (one full-size MFMA product each) per direction of the split forms
+SPLIT_PAIRS = {"bf16x6": {"stp": 27, "pts": 21}, "bf16x3": {"stp": 6, "pts": 6},
+               "int8x6": {"stp": 27, "pts": 21}, "int8x3": {"stp": 6, "pts": 6}}
 
 
 def log(msg: str) -> None:
@@ -455,7 +457,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--forms", default="fp64,bf16x6,bf16x3",
+    ap.add_argument("--forms", default="fp64,bf16x6,bf16x3,int8x6,int8x3",
                     help="Legendre forms to run; the first is the headline, fp64 is always run as the error reference")
     ap.add_argument("--cpu-sample-m", type=int, default=48)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -516,9 +518,10 @@ def main():
     if head == "fp64":
         peak, kernel, achieved = PEAK_FP64_TFLOPS, f"{dom}_fp64_kernel", F / (dom_ms * 1e-3) / 1e12
     else:
-        # BF16 MFMA work actually issued: one F-sized product per kept slice pair
+        # BF16 / INT8 MFMA work actually issued: one F-sized product per kept slice pair
         pairs = SPLIT_PAIRS[head][dom]
-        peak, kernel, achieved = PEAK_BF16_TFLOPS, f"{dom}_split_kernel", pairs * F / (dom_ms * 1e-3) / 1e12
+        peak = PEAK_INT8_TOPS if head.startswith("int8") else PEAK_BF16_TFLOPS
+        kernel, achieved = f"{dom}_split_kernel", pairs * F / (dom_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
         "value": 2 * F / (r["ms_per_step"] * 1e-3) / 1e12,
@@ -530,12 +533,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "fp64" else "bf16",
+        "dtype": "f64" if head == "fp64" else ("i8" if head.startswith("int8") else "bf16"),
         "data": "synthetic",
         "config": {"workload": "Legendre round trip StP+PtS, l_max=511, N_theta=768, N_r=192, 8 fields, "
                                "all m (balanced order)", "l_max": LMAX, "n_theta": NTHETA, "n_r": NR,
                    "n_fields": NF, "form": head},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
+                     "unit": "TOP/s" if head.startswith("int8") else "TFLOP/s",
                      "frac": achieved / peak, "traffic": measured_traffic(kernel),
                      "kernel": kernel, "kernel_ms": dom_ms},
         "cpu_baseline": cpu,

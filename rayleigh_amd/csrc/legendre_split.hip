@@ -33,6 +33,7 @@
 // for StP, parity-l indices for PtS), so each 32-lane half stores 32 consecutive
 // outputs of the theta-fastest hybrid layout.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "raysht_internal.h"
@@ -49,6 +50,8 @@ namespace {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
+typedef __attribute__((ext_vector_type(16))) int i32x16;
+typedef __attribute__((ext_vector_type(4))) int i32x4;
 typedef __attribute__((ext_vector_type(4))) unsigned u32x4;  // one operand fragment, kept as 4 dwords
 
 // Workgroup layout: RAYSHT_SPLIT_WAVES waves, each one 32 x 32 output block, over
@@ -71,17 +74,28 @@ constexpr int kTileRP = 32 * kKB;                // PtS parity-l rows per tile a
 constexpr int kMinWaves = kWaves >= 8 ? 1 : 2;   // workgroups per CU the registers must allow
 constexpr int kABuf = 2 * kCB * 7 * 64;          // LDS A fragments of one step
 static_assert(kThreads / 8 == kTileC && kJB * kCB == kWaves && 2 * kKB * kCB == kWaves, "split layout");
-constexpr int kChunkK = 256;   // K terms per FP32 level sum: 4 * 256 * 2^14 <= 2^24
-constexpr int kChunkSteps = kChunkK / 16;
+// K steps per FP32 (BF16 forms) level sum between moves into FP64: 16 steps of 16 = 256
+// terms, 4 * 256 * 2^14 <= 2^24.  The INT8 forms' int32 level sums stay exact over a
+// whole contraction of up to kMaxI8K terms (7 pairs * 2^14 * 16384 < 2^31); plans
+// with longer contractions are refused for those forms.
+constexpr int kChunkSteps = 16;
+constexpr int kMaxI8K = 16384;
 
 // Slice counts and kept pairs of one form; pair q enumerates levels in increasing
 // order, so the FP32 sum it feeds (group) is nondecreasing in q.
-template <int NA_, int NB_, int LEV_, int NG_>
+template <int NA_, int NB_, int LEV_, int NG_, bool I8_ = false>
 struct form {
     static constexpr int NA = NA_;  // data slices
     static constexpr int NB = NB_;  // table slices
     static constexpr int LEV = LEV_;
-    static constexpr int NG = NG_;  // FP32 sums: one per level below NG - 1, the rest share the last
+    static constexpr int NG = NG_;  // sums: one per level below NG - 1, the rest share the last
+    // INT8 forms: the same 8-bit digits as signed bytes on v_mfma_i32_32x32x32_i8 (twice
+    // the BF16 rate, K = 32 per instruction), one exact int32 sum per level
+    static constexpr bool I8 = I8_;
+    static constexpr int KS = I8 ? 32 : 16;  // K per step (one MFMA)
+    static constexpr int VPT = KS / 8;       // data values per producer thread, block and step
+    static constexpr int EPL = KS / 2;       // K values per fragment lane (16 bytes)
+    using acc_t = typename std::conditional<I8, i32x16, f32x16>::type;
     static constexpr int pair(int q, bool want_s)
     {
         int n = 0;
@@ -110,17 +124,26 @@ struct form {
 using stp_x6 = form<7, 6, 6, 5>;
 using pts_x6 = form<6, 6, 5, 4>;
 using any_x3 = form<3, 3, 2, 1>;
+using stp_i6 = form<7, 6, 6, 7, true>;
+using pts_i6 = form<6, 6, 5, 6, true>;
+using any_i3 = form<3, 3, 2, 3, true>;
 static_assert(stp_x6::npairs() == 27 && pts_x6::npairs() == 21 && any_x3::npairs() == 6, "pair counts");
+static_assert(stp_i6::npairs() == 27 && pts_i6::npairs() == 21 && any_i3::npairs() == 6, "pair counts");
+
+bool is_i8(int precision) { return precision == RAYSHT_INT8X6 || precision == RAYSHT_INT8X3; }
+bool is_x6(int precision) { return precision == RAYSHT_BF16X6 || precision == RAYSHT_INT8X6; }
 
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-// 2^e with |v| < 2^e for every v of a row or column whose max |v| is amax
-__device__ __forceinline__ int scale_exponent(double amax)
+// 2^e with |v| < 2^e for every v of a row or column whose max |v| is amax; the INT8
+// forms take one more bit of headroom (|v| < 2^(e-1)), so that the leading digit
+// stays within [-64, 64] and every digit fits a signed byte
+__device__ __forceinline__ int scale_exponent(double amax, bool i8 = false)
 {
     if (!(amax > 0.0)) return 0;
     int e = 0;
     (void)__builtin_frexp(amax, &e);
-    return e;
+    return e + (i8 ? 1 : 0);
 }
 
 __device__ __forceinline__ double wave_max(double v)
@@ -154,8 +177,8 @@ __device__ __forceinline__ double2 load_d2(__amdgpu_buffer_rsrc_t rsrc, unsigned
 
 // --------------------------------------------------------------- table split (setup)
 // Writes the nb slices of 8 consecutive K values v[0..7] (relative to 2^e) as the
-// 16-byte elements of fragment lane L in nb consecutive fragments starting at f.
-__device__ __forceinline__ void put_table_group(uint4 *__restrict__ tdig, long f, int L, int nb, const double (&v)[8],
+// 16-byte elements (8 BF16) of fragment lane L in nb consecutive fragments starting at f.
+__device__ __forceinline__ void put_table_group(uint4 *__restrict__ tdig, long f, int L, int nb, const double *v,
                                                 int e)
 {
     double x[8];
@@ -168,10 +191,34 @@ __device__ __forceinline__ void put_table_group(uint4 *__restrict__ tdig, long f
     }
 }
 
+// The INT8 forms' table digits: nb signed-byte digits of 16 consecutive K values
+// (relative to 2^e, |v| < 2^(e-1)), from X = round(v 2^(8 nb - 1 - e)) in balanced base 256
+// (the same expansion as the data operand's, rounded at digit nb), as the 16 bytes of
+// fragment lane L in nb consecutive fragments starting at f.
+__device__ __forceinline__ void put_table_group_i8(uint4 *__restrict__ tdig, long f, int L, int nb, const double *v,
+                                                   int e)
+{
+    long long X[16];
+    for (int i = 0; i < 16; ++i) X[i] = (long long)__builtin_rint(__builtin_ldexp(v[i], 8 * nb - 1 - e));
+    for (int s = 0; s < nb; ++s) {
+        unsigned w[4] = {0u, 0u, 0u, 0u};
+        for (int i = 0; i < 16; ++i) {
+            // digit s: (X + B) >> 8 (nb - 1 - s), low byte, minus the offset 128 (s >= 1)
+            const unsigned long long off = 0x0000808080808080ull >> (8 * (7 - nb));
+            const long long Y = (long long)((unsigned long long)X[i] + off) >> (8 * (nb - 1 - s));
+            const unsigned byte = ((unsigned)Y & 0xffu) ^ (s ? 0x80u : 0u);
+            w[i >> 2] |= byte << (8 * (i & 3));
+        }
+        tdig[((f + s) << 6) + L] = uint4{w[0], w[1], w[2], w[3]};
+    }
+}
+
 // StP table of (mp, parity): rows j (theta), K = parity-l index k.  Fragment
 // ((jb * nks + ks) * NB + t) of the (mp, parity) region holds slice t of rows
-// 32 jb .. +31, k = 16 ks .. +15: lane L, element e -> j = 32 jb + L%32,
-// k = 16 ks + 8 (L/32) + e.  One wave per (j, parity, mp).
+// 32 jb .. +31, k = KS ks .. +KS-1: lane L, element e -> j = 32 jb + L%32,
+// k = KS ks + (KS/2) (L/32) + e (KS = 16 BF16 / 32 INT8 elements of 16 bytes per lane).
+// One wave per (j, parity, mp).
+template <bool I8>
 __global__ __launch_bounds__(64) void split_stp_table_kernel(int lmax, int nt2, int nb, const int *__restrict__ mvals,
                                                              const long *__restrict__ te_off,
                                                              const long *__restrict__ to_off,
@@ -182,25 +229,30 @@ __global__ __launch_bounds__(64) void split_stp_table_kernel(int lmax, int nt2, 
 {
     const int j = blockIdx.x, par = blockIdx.y, mp = blockIdx.z, lane = threadIdx.x;
     const int m = mvals[mp], nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
-    const int n = par ? no : ne, nks = cdiv(n, 16), njb = cdiv(nt2, 32);
+    constexpr int KS = I8 ? 32 : 16, EPL = KS / 2;
+    const int n = par ? no : ne, nks = cdiv(n, KS), njb = cdiv(nt2, 32);
     const double *t = par ? pto + to_off[mp] : pte + te_off[mp];  // [k][j]
     double amax = 0.0;
     for (int k = lane; k < n; k += 64) amax = fmax(amax, fabs(t[(long)k * nt2 + j]));
-    const int e = scale_exponent(wave_max(amax));
+    const int e = scale_exponent(wave_max(amax), I8);
     if (lane == 0) texp[((long)mp * 2 + par) * nt2 + j] = e;
-    const long region = frag_off[mp] + (par ? (long)njb * cdiv(ne, 16) * nb : 0);
+    const long region = frag_off[mp] + (par ? (long)njb * cdiv(ne, KS) * nb : 0);
     const int jb = j >> 5, jl = j & 31;
-    for (int g = lane; g < 2 * nks; g += 64) {  // k = 8 g .. 8 g + 7
-        double v[8];
-        for (int i = 0; i < 8; ++i) v[i] = 8 * g + i < n ? t[(long)(8 * g + i) * nt2 + j] : 0.0;
-        put_table_group(tdig, region + ((long)jb * nks + (g >> 1)) * nb, jl + 32 * (g & 1), nb, v, e);
+    for (int g = lane; g < 2 * nks; g += 64) {  // k = EPL g .. EPL g + EPL - 1
+        double v[EPL];
+        for (int i = 0; i < EPL; ++i) v[i] = EPL * g + i < n ? t[(long)(EPL * g + i) * nt2 + j] : 0.0;
+        if constexpr (I8)
+            put_table_group_i8(tdig, region + ((long)jb * nks + (g >> 1)) * nb, jl + 32 * (g & 1), nb, v, e);
+        else
+            put_table_group(tdig, region + ((long)jb * nks + (g >> 1)) * nb, jl + 32 * (g & 1), nb, v, e);
     }
 }
 
 // PtS table of (mp, parity): rows k (parity-l, the output), K = j (theta).  Fragment
-// ((kb * njs + js) * NB + t): lane L, element e -> k = 32 kb + L%32, j = 16 js + 8 (L/32) + e.
+// ((kb * njs + js) * NB + t): lane L, element e -> k = 32 kb + L%32, j = KS js + (KS/2) (L/32) + e.
 // One wave per (k, parity, mp); rows past the parity's length are skipped (their
 // fragments were cleared).
+template <bool I8>
 __global__ __launch_bounds__(64) void split_pts_table_kernel(int lmax, int nt2, int nb, const int *__restrict__ mvals,
                                                              const long *__restrict__ te_off,
                                                              const long *__restrict__ to_off,
@@ -214,37 +266,41 @@ __global__ __launch_bounds__(64) void split_pts_table_kernel(int lmax, int nt2, 
     const int m = mvals[mp], nl = lmax - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
     const int n = par ? no : ne;
     if (k >= n) return;
-    const int njs = cdiv(nt2, 16);
+    constexpr int KS = I8 ? 32 : 16, EPL = KS / 2;
+    const int njs = cdiv(nt2, KS);
     const double *t = (par ? ipo + to_off[mp] : ipe + te_off[mp]) + (long)k * nt2;  // [k][j]
     double amax = 0.0;
     for (int j = lane; j < nt2; j += 64) amax = fmax(amax, fabs(t[j]));
-    const int e = scale_exponent(wave_max(amax));
+    const int e = scale_exponent(wave_max(amax), I8);
     if (lane == 0) texp[exp_off[mp] + (par ? ne : 0) + k] = e;
     const long region = frag_off[mp] + (par ? (long)cdiv(ne, 32) * njs * nb : 0);
     const int kb = k >> 5, kl = k & 31;
-    for (int g = lane; g < 2 * njs; g += 64) {  // j = 8 g .. 8 g + 7
-        double v[8];
-        for (int i = 0; i < 8; ++i) v[i] = 8 * g + i < nt2 ? t[8 * g + i] : 0.0;
-        put_table_group(tdig, region + ((long)kb * njs + (g >> 1)) * nb, kl + 32 * (g & 1), nb, v, e);
+    for (int g = lane; g < 2 * njs; g += 64) {  // j = EPL g .. EPL g + EPL - 1
+        double v[EPL];
+        for (int i = 0; i < EPL; ++i) v[i] = EPL * g + i < nt2 ? t[EPL * g + i] : 0.0;
+        if constexpr (I8)
+            put_table_group_i8(tdig, region + ((long)kb * njs + (g >> 1)) * nb, kl + 32 * (g & 1), nb, v, e);
+        else
+            put_table_group(tdig, region + ((long)kb * njs + (g >> 1)) * nb, kl + 32 * (g & 1), nb, v, e);
     }
 }
 
 // --------------------------------------------------------------- contraction core
-// One K step of 16 for a wave's 32 x 32 block: acc[g] += sum over the form's pairs of
-// A_s (data) x B_t (table).
-template <class F, int Q = 0>
-__device__ __forceinline__ void mfma_pairs(f32x16 (&acc)[F::NG], const u32x4 (&a)[F::NA], const u32x4 (&b)[F::NB])
+// One MFMA of a slice pair: acc += A_s (data) x B_t (table) for one K step
+template <class F>
+__device__ __forceinline__ void mfma_pair(typename F::acc_t &acc, const u32x4 &a, const u32x4 &b)
 {
-    if constexpr (Q < F::npairs()) {
-        constexpr int s = F::pair(Q, true), t = F::pair(Q, false), g = F::group(Q);
-        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[s]), __builtin_bit_cast(bf16x8, b[t]),
-                                                         acc[g], 0, 0, 0);
-        mfma_pairs<F, Q + 1>(acc, a, b);
-    }
+    if constexpr (F::I8)
+        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i32x4, a), __builtin_bit_cast(i32x4, b), acc, 0,
+                                                    0, 0);
+    else
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                      0, 0, 0);
 }
 
-// mfma_pairs with the NA slicing steps of the next K step spread evenly between the
-// MFMAs (slice s after pair s * npairs / NA), in source order
+// The MFMAs of one K step for a wave's 32 x 32 block (acc[g] += sum over the form's pairs
+// of A_s x B_t), with the NA slicing steps of the next K step spread evenly between them
+// (slice s after pair s * npairs / NA), in source order
 template <class F>
 constexpr int slice_at(int q)
 {
@@ -254,31 +310,36 @@ constexpr int slice_at(int q)
 }
 
 template <class F, class SL, int Q = 0>
-__device__ __forceinline__ void mfma_pairs_sliced(f32x16 (&acc)[F::NG], const u32x4 (&a)[F::NA],
+__device__ __forceinline__ void mfma_pairs_sliced(typename F::acc_t (&acc)[F::NG], const u32x4 (&a)[F::NA],
                                                   const u32x4 (&b)[F::NB], SL &sl)
 {
     if constexpr (Q < F::npairs()) {
         constexpr int s = F::pair(Q, true), t = F::pair(Q, false), g = F::group(Q), sc = slice_at<F>(Q);
-        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[s]), __builtin_bit_cast(bf16x8, b[t]),
-                                                         acc[g], 0, 0, 0);
+        mfma_pair<F>(acc[g], a[s], b[t]);
         if constexpr (sc >= 0) sl.step(sc);
         mfma_pairs_sliced<F, SL, Q + 1>(acc, a, b, sl);
     }
 }
 
-// adds the FP32 level sums into the FP64 sums and clears them
+// adds the level sums into the FP64 sums and clears them (BF16 digits carry their
+// weights 2^(-7-8s); the INT8 forms' integer level L sums are scaled by 2^(-14-8L))
 template <class F>
-__device__ __forceinline__ void flush(f32x16 (&acc)[F::NG], double (&dacc)[16])
+__device__ __forceinline__ void flush(typename F::acc_t (&acc)[F::NG], double (&dacc)[16])
 {
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         double sum = 0.0;
 #pragma unroll
-        for (int g = F::NG - 1; g >= 0; --g) sum += (double)acc[g][v];
+        for (int g = F::NG - 1; g >= 0; --g) {
+            if constexpr (F::I8)
+                sum += __builtin_ldexp((double)acc[g][v], -14 - 8 * g);
+            else
+                sum += (double)acc[g][v];
+        }
         dacc[v] += sum;
     }
 #pragma unroll
-    for (int g = 0; g < F::NG; ++g) acc[g] = f32x16{};
+    for (int g = 0; g < F::NG; ++g) acc[g] = typename F::acc_t{};
 }
 
 // The data operand's NA digits, from one integer per value.  For |v| < 2^e,
@@ -322,28 +383,54 @@ __device__ __forceinline__ unsigned pack_bf16(float lo, float hi)
 // cover distinct banks.
 __device__ __forceinline__ int frag_slot(int L) { return L ^ ((L >> 5) << 3); }
 
-// The slicing of two consecutive K values of one data column into the A fragments of
-// an LDS buffer laid out [block][slice][slot][8 BF16] (fragment lane L = row % 32 +
-// 32 * (k0 / 8), elements k0 % 8 and k0 % 8 + 1, k0 even within the step), one slice per
-// step() call, so that the calls can be spread between the MFMAs of the previous step.
+// slice s of four offset_digits<NA> values as four signed bytes: byte NA-1-s of each
+// (the top byte is the two's-complement d_0; the others hold d_s + 128, flipped by the
+// XOR with 0x80), gathered with byte permutes
 template <int NA>
-struct pair_slicer {
-    unsigned long long X0, X1;
+__device__ __forceinline__ unsigned pack_i8(const unsigned long long (&X)[4], int s)
+{
+    const int p = NA - 1 - s, b = p & 3;
+    const unsigned w0 = p < 4 ? (unsigned)X[0] : (unsigned)(X[0] >> 32);
+    const unsigned w1 = p < 4 ? (unsigned)X[1] : (unsigned)(X[1] >> 32);
+    const unsigned w2 = p < 4 ? (unsigned)X[2] : (unsigned)(X[2] >> 32);
+    const unsigned w3 = p < 4 ? (unsigned)X[3] : (unsigned)(X[3] >> 32);
+    const unsigned sel = 0x0c0c0000u | ((4u + b) << 8) | (unsigned)b;  // (src1 byte b, src0 byte b, 0, 0)
+    const unsigned t01 = __builtin_amdgcn_perm(w1, w0, sel), t23 = __builtin_amdgcn_perm(w3, w2, sel);
+    const unsigned r = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+    return s ? r ^ 0x80808080u : r;
+}
+
+// The slicing of VPT consecutive K values of one data column into the A fragments of an
+// LDS buffer laid out [block][slice][slot][16 bytes] (fragment lane L = row % 32 +
+// 32 * (k0 / EPL), bytes of k0 % EPL, k0 a multiple of VPT within the step), one slice per
+// step() call, so that the calls can be spread between the MFMAs of the previous step.
+// BF16 forms: two values, one dword of two BF16 per slice; INT8 forms: four values, one
+// dword of four signed bytes.
+template <class F>
+struct digit_slicer {
+    unsigned long long X[F::VPT];
     unsigned *p;
-    __device__ __forceinline__ void init(u32x4 *__restrict__ buf, int blk, int row, int k0, double v0, double v1,
-                                         int e)
+    __device__ __forceinline__ void init(u32x4 *__restrict__ buf, int blk, int row, int k0,
+                                         const double (&v)[F::VPT], int e)
     {
-        X0 = offset_digits<NA>(v0, e);
-        X1 = offset_digits<NA>(v1, e);
-        p = reinterpret_cast<unsigned *>(buf) + ((blk * NA) * 64 + frag_slot((row & 31) + 32 * (k0 >> 3))) * 4 +
-            ((k0 & 7) >> 1);
+#pragma unroll
+        for (int i = 0; i < F::VPT; ++i) X[i] = offset_digits<F::NA>(v[i], e);
+        const int lane = (row & 31) + 32 * (k0 / F::EPL);
+        p = reinterpret_cast<unsigned *>(buf) + ((blk * F::NA) * 64 + frag_slot(lane)) * 4 +
+            (k0 % F::EPL) / (F::I8 ? 4 : 2);
     }
-    __device__ __forceinline__ void step(int s) { p[s * 256] = pack_bf16(digit<NA>(X0, s), digit<NA>(X1, s)); }
+    __device__ __forceinline__ void step(int s)
+    {
+        if constexpr (F::I8)
+            p[s * 256] = pack_i8<F::NA>(X, s);
+        else
+            p[s * 256] = pack_bf16(digit<F::NA>(X[0], s), digit<F::NA>(X[1], s));
+    }
 };
 
-template <int NA, int N>
+template <class F, int N>
 struct slicers {
-    pair_slicer<NA> sl[N];
+    digit_slicer<F> sl[N];
     __device__ __forceinline__ void step(int s)
     {
 #pragma unroll
@@ -374,34 +461,33 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_frag_rsrc(const u32x4 *ba
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<u32x4 *>(base), (short)0, (int)n, 0x00020000);
 }
 
-// Raw FP64 data of one K step for one thread (loaded two steps ahead of its slicing),
-// with the bit mask of the values that exist (applied when the data are sliced, so the
-// loads are not waited for until then).
+// Raw FP64 data of one K step for one thread (loaded two steps ahead of its slicing):
+// N 16-byte pairs, with the bit mask of the values that exist (bit 2 i + h for d[i].x
+// (h = 0) or d[i].y (h = 1); applied when the data are sliced, so that the loads are not
+// waited for until then).
+template <int N>
 struct raw_step {
-    double2 a, b;
+    double2 d[N];
     unsigned valid;
-    __device__ __forceinline__ double2 va() const
+    __device__ __forceinline__ double val(int i, int h) const
     {
-        return make_double2(valid & 1 ? a.x : 0.0, valid & 2 ? a.y : 0.0);
-    }
-    __device__ __forceinline__ double2 vb() const
-    {
-        return make_double2(valid & 4 ? b.x : 0.0, valid & 8 ? b.y : 0.0);
+        const double x = h ? d[i].y : d[i].x;
+        return (valid >> (2 * i + h)) & 1 ? x : 0.0;
     }
 };
 
-// The K loop of one contraction: nks steps of 16.  fetch(ks) issues the loads of step
+// The K loop of one contraction: nks steps of F::KS.  fetch(ks) issues the loads of step
 // ks's raw data (steps past the end read masked zeros) and slice(raw, buffer) returns
 // the slicers that write its A fragments into an LDS buffer, one slice per step() call
 // (spread between the MFMAs of the step before); step ks's fragments are in buffer
 // ks % 2 before its MFMAs.  Raw data run two steps ahead of the MFMAs and table fragments (fragment
 // f0 + ks * NB of resource rt) one step, so their latency hides behind a step of
 // MFMAs.  Wave w multiplies A block ablk of the buffer by its table fragments; the
-// result is added into dacc (FP32 level sums moved there every kChunkSteps steps).
+// result is added into dacc (level sums moved there every kChunkSteps steps).
 // The step body has no branches (the last step slices and loads harmlessly past the
 // end), so one wave keeps its matrix pipe and its vector ALU busy at once.  nks is uniform
 // over the workgroup; every step ends with one barrier.
-template <class F, class Fetch, class Slice>
+template <class F, class Raw, class Fetch, class Slice>
 __device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, int nks, __amdgpu_buffer_rsrc_t rt,
                                        int f0, u32x4 (*abuf)[kABuf], int ablk, int lane, double (&dacc)[16])
 {
@@ -413,16 +499,16 @@ __device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, i
     }
     // two register sets for the raw data and the table fragments, swapped by unrolling
     // the step loop by two (no copies between steps)
-    raw_step rA = fetch(1), rB;
+    Raw rA = fetch(1), rB;
     u32x4 bA[F::NB], bB[F::NB];
     get_b<F::NB>(rt, f0, lane, bA);
     __syncthreads();
-    f32x16 acc[F::NG];
+    typename F::acc_t acc[F::NG];
 #pragma unroll
-    for (int g = 0; g < F::NG; ++g) acc[g] = f32x16{};
+    for (int g = 0; g < F::NG; ++g) acc[g] = typename F::acc_t{};
     // step ks: raw data of step ks + 1 in rc, table fragments of step ks in bc; loads
     // step ks + 2's raw data into rn and step ks + 1's fragments into bn
-    auto step = [&](int ks, const raw_step &rc, raw_step &rn, const u32x4 (&bc)[F::NB], u32x4 (&bn)[F::NB]) {
+    auto step = [&](int ks, const Raw &rc, Raw &rn, const u32x4 (&bc)[F::NB], u32x4 (&bn)[F::NB]) {
         rn = fetch(ks + 2);
         get_b<F::NB>(rt, f0 + (ks + 1) * F::NB, lane, bn);
         u32x4 a[F::NA];
@@ -431,8 +517,7 @@ __device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, i
         mfma_pairs_sliced<F>(acc, a, bc, sl);
         __syncthreads();
     };
-    for (int kc = 0; kc < nks; kc += kChunkSteps) {
-        const int kend = min(nks, kc + kChunkSteps);
+    auto steps = [&](int kc, int kend) {
         int ks = kc;
         for (; ks + 1 < kend; ks += 2) {
             step(ks, rA, rB, bA, bB);
@@ -441,7 +526,16 @@ __device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, i
         if (ks < kend) {  // odd step count: only in the last chunk (kChunkSteps is even)
             step(ks, rA, rB, bA, bB);
         }
+    };
+    if constexpr (F::I8) {
+        // int32 level sums are exact over the whole contraction (kMaxI8K), one move into FP64
+        steps(0, nks);
         flush<F>(acc, dacc);
+    } else {
+        for (int kc = 0; kc < nks; kc += kChunkSteps) {
+            steps(kc, min(nks, kc + kChunkSteps));
+            flush<F>(acc, dacc);
+        }
     }
 }
 static_assert(kChunkSteps % 2 == 0, "chunks hold whole step pairs");
@@ -450,7 +544,7 @@ static_assert(kChunkSteps % 2 == 0, "chunks hold whole step pairs");
 // Work item (mp, c0): kTileC columns, every theta row, in tiles of kTileR rows with
 // both parities each.  Wave w multiplies column block w % kCB by theta block
 // j0/32 + w/kCB.  The spectral tile is read as 16-byte pairs (l, l+1); each parity pass
-// slices its half, two values per thread and step.  The column exponents come from one
+// slices its half, VPT values per thread and step.  The column exponents come from one
 // pass over the tile; the scaled even-parity sums wait in LDS for the odd pass.
 template <class F>
 __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
@@ -497,11 +591,12 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
         }
 #pragma unroll
         for (int off = 1; off < 8; off <<= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-        if (q == 0) cexp[ca] = scale_exponent(mx);
+        if (q == 0) cexp[ca] = scale_exponent(mx, F::I8);
     }
     __syncthreads();
     const int ea = cexp[ca];
-    const int cb = wave % kCB, njb = cdiv(nt2, 32), nks_e = cdiv(ne, 16);
+    const int cb = wave % kCB, njb = cdiv(nt2, 32), nks_e = cdiv(ne, F::KS);
+    using Raw = raw_step<F::VPT>;
     double *ob = uniform(io.out[pi]) + (long)mp * nrhs * nt;
 
     for (int r0 = 0; r0 < nt2; r0 += kTileR) {
@@ -513,23 +608,31 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
             double dacc[16];
 #pragma unroll
             for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
-            const int K = par ? no : ne, nks = cdiv(K, 16);
+            const int K = par ? no : ne, nks = cdiv(K, F::KS);
             const int f0 = (par ? njb * nks_e * F::NB : 0) + jb * nks * F::NB;
-            // step ks: parity-k = 16 ks + 2 q + {0, 1}, i.e. l = 32 ks + 4 q + {0, 2} + par
-            // (l >= nl is the next column or past the block: masked to zero)
+            // step ks: parity-k = KS ks + VPT q + i (i < VPT), i.e. l = 2 KS ks + 2 VPT q + 2 i + par,
+            // read as the pairs (l, l + 1) of both parities (l >= nl is the next column or
+            // past the block: masked to zero)
             auto fetch = [&](int ks) {
-                const int l0 = 32 * ks + 4 * q;
-                const unsigned valid = (l0 < nl) | (l0 + 1 < nl) << 1 | (l0 + 2 < nl) << 2 | (l0 + 3 < nl) << 3;
-                return raw_step{load_d2(rs, a_off + (unsigned)l0 * 8), load_d2(rs, a_off + (unsigned)(l0 + 2) * 8),
-                                valid};
+                const int l0 = 2 * F::KS * ks + 2 * F::VPT * q;
+                Raw r;
+                r.valid = 0;
+#pragma unroll
+                for (int i = 0; i < F::VPT; ++i) {
+                    r.d[i] = load_d2(rs, a_off + (unsigned)(l0 + 2 * i) * 8);
+                    r.valid |= (unsigned)(l0 + 2 * i < nl) << (2 * i) | (unsigned)(l0 + 2 * i + 1 < nl) << (2 * i + 1);
+                }
+                return r;
             };
-            auto slice = [&](const raw_step &r, u32x4 *buf) {
-                const double2 a = r.va(), b = r.vb();
-                slicers<F::NA, 1> sl;
-                sl.sl[0].init(buf, ca >> 5, ca, 2 * q, par ? a.y : a.x, par ? b.y : b.x, ea);
+            auto slice = [&](const Raw &r, u32x4 *buf) {
+                double v[F::VPT];
+#pragma unroll
+                for (int i = 0; i < F::VPT; ++i) v[i] = r.val(i, par);
+                slicers<F, 1> sl;
+                sl.sl[0].init(buf, ca >> 5, ca, F::VPT * q, v, ea);
                 return sl;
             };
-            k_loop<F>(fetch, slice, nks, rt, f0, abuf, cb, lane, dacc);
+            k_loop<F, Raw>(fetch, slice, nks, rt, f0, abuf, cb, lane, dacc);
             const bool store = active && j < nt2;
             const int ej = store ? texp[((long)mp * 2 + par) * nt2 + j] : 0;
 #pragma unroll
@@ -552,7 +655,7 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
 // Work item (mp, c0): kTileC columns, every parity-l row of both parities, in row
 // tiles of kTileRP.  Wave w multiplies column block w % kCB of parity w / (kCB kKB) by
 // row block k0/32 + (w / kCB) % kKB.  The data operand is the fold f(j) + f(nt-1-j) (even) or
-// f(j) - f(nt-1-j) (odd) of the hybrid columns; each thread slices both folds of two
+// f(j) - f(nt-1-j) (odd) of the hybrid columns; each thread slices both folds of VPT
 // nodes per step, and both folds' column exponents come from one pass over the tile.
 template <class F>
 __global__ __launch_bounds__(kThreads, kMinWaves) void pts_split_kernel(
@@ -573,30 +676,43 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void pts_split_kernel(
     const __amdgpu_buffer_rsrc_t rp = make_rsrc(pb, (long)nrhs * nt * 8);
     const int ca = tid >> 3, q = tid & 7;
     const unsigned a_off = (unsigned)min(c0 + ca, nrhs) * nt * 8;
-    const int njs = cdiv(nt2, 16);
+    const int njs = cdiv(nt2, F::KS);
+    constexpr int H = F::VPT / 2;
+    using Raw = raw_step<F::VPT>;
 
-    // raw data of step js: a = (f(j), f(j+1)) and b = (f(nt-1-j), f(nt-2-j)) for
-    // j = 16 js + 2 q, zero for j >= nt2 (the mirrored offset is negative only past the
+    // raw data of step js: nodes j = j0 + i, j0 = KS js + VPT q, i < VPT, as the pairs
+    // d[u] = (f(j0 + 2u), f(j0 + 2u + 1)) and their mirrors d[H + u] = (f(nt-1-j0-2u),
+    // f(nt-2-j0-2u)); zero for j >= nt2 (the mirrored offset is negative only past the
     // last valid j, where it wraps past the end or into a column whose values are masked)
     auto fetch = [&](int js) {
-        const int j0 = 16 * js + 2 * q;
-        const unsigned valid = (j0 < nt2 ? 5u : 0u) | (j0 + 1 < nt2 ? 10u : 0u);
-        const double2 f = load_d2(rp, a_off + j0 * 8u);
-        const double2 g = load_d2(rp, a_off + (unsigned)(nt - 2 - j0) * 8u);  // f(nt-2-j0), f(nt-1-j0)
-        return raw_step{f, make_double2(g.y, g.x), valid};
+        const int j0 = F::KS * js + F::VPT * q;
+        Raw r;
+        r.valid = 0;
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+            r.d[u] = load_d2(rp, a_off + (unsigned)(j0 + 2 * u) * 8u);
+            const double2 g = load_d2(rp, a_off + (unsigned)(nt - 2 - j0 - 2 * u) * 8u);  // f(nt-2-j), f(nt-1-j)
+            r.d[H + u] = make_double2(g.y, g.x);
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (j0 + 2 * u + h < nt2) r.valid |= 1u << (2 * u + h) | 1u << (2 * (H + u) + h);
+        }
+        return r;
     };
     {
         double me = 0.0, mo = 0.0;
         for (int j0 = 0; j0 < njs; j0 += 4) {
-            raw_step r[4];
+            Raw r[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) r[u] = fetch(j0 + u);  // steps past njs are masked
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const double2 a = r[u].va(), b = r[u].vb();
-                me = fmax(me, fmax(fabs(a.x + b.x), fabs(a.y + b.y)));
-                mo = fmax(mo, fmax(fabs(a.x - b.x), fabs(a.y - b.y)));
-            }
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int i = 0; i < F::VPT; ++i) {
+                    const double a = r[u].val(i >> 1, i & 1), b = r[u].val(H + (i >> 1), i & 1);
+                    me = fmax(me, fabs(a + b));
+                    mo = fmax(mo, fabs(a - b));
+                }
         }
 #pragma unroll
         for (int off = 1; off < 8; off <<= 1) {
@@ -604,8 +720,8 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void pts_split_kernel(
             mo = fmax(mo, __shfl_xor(mo, off, 64));
         }
         if (q == 0) {
-            cexp[0][ca] = scale_exponent(me);
-            cexp[1][ca] = scale_exponent(mo);
+            cexp[0][ca] = scale_exponent(me, F::I8);
+            cexp[1][ca] = scale_exponent(mo, F::I8);
         }
     }
     __syncthreads();
@@ -618,11 +734,17 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void pts_split_kernel(
     double *sb = uniform(io.out[pi]) + uniform(uniform(P.s_off)[mp]);
     const long e_base = uniform(uniform(P.exp_pts_off)[mp]);
     // A blocks of a buffer: [parity][column block]
-    auto slice = [&](const raw_step &r, u32x4 *buf) {
-        const double2 a = r.va(), b = r.vb();
-        slicers<F::NA, 2> sl;
-        sl.sl[0].init(buf, ca >> 5, ca, 2 * q, a.x + b.x, a.y + b.y, ee);
-        sl.sl[1].init(buf, kCB + (ca >> 5), ca, 2 * q, a.x - b.x, a.y - b.y, eo);
+    auto slice = [&](const Raw &r, u32x4 *buf) {
+        double ve[F::VPT], vo[F::VPT];
+#pragma unroll
+        for (int i = 0; i < F::VPT; ++i) {
+            const double a = r.val(i >> 1, i & 1), b = r.val(H + (i >> 1), i & 1);
+            ve[i] = a + b;
+            vo[i] = a - b;
+        }
+        slicers<F, 2> sl;
+        sl.sl[0].init(buf, ca >> 5, ca, F::VPT * q, ve, ee);
+        sl.sl[1].init(buf, kCB + (ca >> 5), ca, F::VPT * q, vo, eo);
         return sl;
     };
 
@@ -632,7 +754,7 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void pts_split_kernel(
         double dacc[16];
 #pragma unroll
         for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
-        k_loop<F>(fetch, slice, njs, rt, region + kb * njs * F::NB, abuf, kCB * par + cb, lane, dacc);
+        k_loop<F, Raw>(fetch, slice, njs, rt, region + kb * njs * F::NB, abuf, kCB * par + cb, lane, dacc);
 
         const int k = kb * 32 + (lane & 31);
         if (active && k < K_rows) {
@@ -661,10 +783,11 @@ std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans,
     std::vector<blk> order;
     for (int pi = 0; pi < (int)plans.size(); ++pi) {
         const raysht_plan *p = plans[pi];
-        const int nrt = cdiv(p->nt2, kTileR), njs = cdiv(p->nt2, 16);
+        const int ks = is_i8(p->precision) ? 32 : 16;
+        const int nrt = cdiv(p->nt2, kTileR), njs = cdiv(p->nt2, ks);
         for (int mp = 0; mp < p->nm; ++mp) {
             const int nl = p->lmax - p->mvals[mp] + 1, ne = (nl + 1) / 2, no = nl / 2;
-            const double steps = stp ? (double)nrt * (cdiv(ne, 16) + cdiv(no, 16)) + cdiv(nl, 32)
+            const double steps = stp ? (double)nrt * (cdiv(ne, ks) + cdiv(no, ks)) + cdiv(nl, 32)
                                      : (double)cdiv(ne, kTileRP) * njs + njs;
             order.push_back(blk{pi, mp, steps});
         }
@@ -705,19 +828,21 @@ int dupload(T **p, const std::vector<T> &v)
     return RAYSHT_OK;
 }
 
-bool x6(const raysht_plan *p) { return p->precision == RAYSHT_BF16X6; }
-
 }  // namespace
 
 int split_plan_init(raysht_plan *p)
 {
     raysht_split &sp = p->split;
-    const int nb_stp = x6(p) ? stp_x6::NB : any_x3::NB, nb_pts = x6(p) ? pts_x6::NB : any_x3::NB;
-    const int nm = p->nm, nt2 = p->nt2, njb = cdiv(nt2, 32), njs = cdiv(nt2, 16);
+    const bool x6 = is_x6(p->precision), i8 = is_i8(p->precision);
+    if (i8 && (p->nt2 > kMaxI8K || (p->lmax + 2) / 2 > kMaxI8K)) return RAYSHT_EUNSUPPORTED;
+    const int nb_stp = x6 ? stp_x6::NB : any_x3::NB, nb_pts = x6 ? pts_x6::NB : any_x3::NB;
+    static_assert(stp_x6::NB == stp_i6::NB && pts_x6::NB == pts_i6::NB && any_x3::NB == any_i3::NB, "table slices");
+    const int ks = i8 ? 32 : 16;
+    const int nm = p->nm, nt2 = p->nt2, njb = cdiv(nt2, 32), njs = cdiv(nt2, ks);
     std::vector<long> fs(nm + 1, 0), fp(nm + 1, 0), ep(nm + 1, 0);
     for (int mp = 0; mp < nm; ++mp) {
         const int nl = p->lmax - p->mvals[mp] + 1, ne = (nl + 1) / 2, no = nl / 2;
-        fs[mp + 1] = fs[mp] + (long)njb * (cdiv(ne, 16) + cdiv(no, 16)) * nb_stp;
+        fs[mp + 1] = fs[mp] + (long)njb * (cdiv(ne, ks) + cdiv(no, ks)) * nb_stp;
         fp[mp + 1] = fp[mp] + (long)(cdiv(ne, 32) + cdiv(no, 32)) * njs * nb_pts;
         ep[mp + 1] = ep[mp] + ne + no;
     }
@@ -734,13 +859,14 @@ int split_plan_init(raysht_plan *p)
     // rows past a parity's length in the last 32-row block of the PtS tables are read
     // as zeros: clear before the split kernel fills the rows that exist
     RAYSHT_HIP_TRY(hipMemset(sp.d_tdig_pts, 0, (size_t)fp[nm] * 64 * sizeof(uint4)));
-    hipLaunchKernelGGL(split_stp_table_kernel, dim3(nt2, 2, nm), dim3(64), 0, nullptr, p->lmax, nt2, nb_stp,
-                       p->d_mvals, p->d_te_off, p->d_to_off, p->d_pt_even, p->d_pt_odd, sp.d_frag_stp_off,
-                       sp.d_tdig_stp, sp.d_texp_stp);
+    hipLaunchKernelGGL(i8 ? split_stp_table_kernel<true> : split_stp_table_kernel<false>, dim3(nt2, 2, nm), dim3(64),
+                       0, nullptr, p->lmax, nt2, nb_stp, p->d_mvals, p->d_te_off, p->d_to_off, p->d_pt_even,
+                       p->d_pt_odd, sp.d_frag_stp_off, sp.d_tdig_stp, sp.d_texp_stp);
     RAYSHT_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(split_pts_table_kernel, dim3((p->lmax + 2) / 2, 2, nm), dim3(64), 0, nullptr, p->lmax, nt2,
-                       nb_pts, p->d_mvals, p->d_te_off, p->d_to_off, p->d_ip_even, p->d_ip_odd, sp.d_frag_pts_off,
-                       sp.d_exp_pts_off, sp.d_tdig_pts, sp.d_texp_pts);
+    hipLaunchKernelGGL(i8 ? split_pts_table_kernel<true> : split_pts_table_kernel<false>,
+                       dim3((p->lmax + 2) / 2, 2, nm), dim3(64), 0, nullptr, p->lmax, nt2, nb_pts, p->d_mvals,
+                       p->d_te_off, p->d_to_off, p->d_ip_even, p->d_ip_odd, sp.d_frag_pts_off, sp.d_exp_pts_off,
+                       sp.d_tdig_pts, sp.d_texp_pts);
     RAYSHT_HIP_TRY(hipGetLastError());
     RAYSHT_HIP_TRY(hipDeviceSynchronize());
     return RAYSHT_OK;
@@ -760,10 +886,14 @@ int launch_stp_split_batch(int precision, const prob_desc *d_desc, const batch_i
                            int n_work, hipStream_t st)
 {
     if (n_work == 0) return RAYSHT_OK;
-    if (precision == RAYSHT_BF16X6)
-        hipLaunchKernelGGL(stp_split_kernel<stp_x6>, dim3(n_work), dim3(kThreads), 0, st, d_desc, io, work);
-    else
-        hipLaunchKernelGGL(stp_split_kernel<any_x3>, dim3(n_work), dim3(kThreads), 0, st, d_desc, io, work);
+    const dim3 g(n_work), b(kThreads);
+    switch (precision) {
+    case RAYSHT_BF16X6: hipLaunchKernelGGL(stp_split_kernel<stp_x6>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_BF16X3: hipLaunchKernelGGL(stp_split_kernel<any_x3>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_INT8X6: hipLaunchKernelGGL(stp_split_kernel<stp_i6>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_INT8X3: hipLaunchKernelGGL(stp_split_kernel<any_i3>, g, b, 0, st, d_desc, io, work); break;
+    default: return RAYSHT_EINVAL;
+    }
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
@@ -772,10 +902,14 @@ int launch_pts_split_batch(int precision, const prob_desc *d_desc, const batch_i
                            int n_work, hipStream_t st)
 {
     if (n_work == 0) return RAYSHT_OK;
-    if (precision == RAYSHT_BF16X6)
-        hipLaunchKernelGGL(pts_split_kernel<pts_x6>, dim3(n_work), dim3(kThreads), 0, st, d_desc, io, work);
-    else
-        hipLaunchKernelGGL(pts_split_kernel<any_x3>, dim3(n_work), dim3(kThreads), 0, st, d_desc, io, work);
+    const dim3 g(n_work), b(kThreads);
+    switch (precision) {
+    case RAYSHT_BF16X6: hipLaunchKernelGGL(pts_split_kernel<pts_x6>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_BF16X3: hipLaunchKernelGGL(pts_split_kernel<any_x3>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_INT8X6: hipLaunchKernelGGL(pts_split_kernel<pts_i6>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_INT8X3: hipLaunchKernelGGL(pts_split_kernel<any_i3>, g, b, 0, st, d_desc, io, work); break;
+    default: return RAYSHT_EINVAL;
+    }
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }

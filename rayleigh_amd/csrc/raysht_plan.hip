@@ -208,7 +208,10 @@ int plan_create(raysht_plan **out, int lmax, int ntheta, const int *m_values, in
     if (lmax < 0 || lmax > 8191 || ntheta < 2 || (ntheta & 1) || !m_values || n_m < 1 || n_m > lmax + 1 ||
         ncols < 1 || nr < 0 || nfields < 0)
         return RAYSHT_EINVAL;
-    if (precision != RAYSHT_FP64 && precision != RAYSHT_BF16X3 && precision != RAYSHT_BF16X6) return RAYSHT_EINVAL;
+    if (precision < RAYSHT_FP64 || precision > RAYSHT_INT8X6) return RAYSHT_EINVAL;
+    // the INT8 forms' exact int32 sums cover contractions of up to 16384 terms
+    if ((precision == RAYSHT_INT8X3 || precision == RAYSHT_INT8X6) && (ntheta / 2 > 16384 || (lmax + 2) / 2 > 16384))
+        return RAYSHT_EUNSUPPORTED;
     // every m-block of the hybrid and spectral layouts is addressed with 32-bit byte offsets
     if (ncols * (long)ntheta * 8 >= (1L << 31) || ncols * (long)(lmax + 1) * 8 >= (1L << 31)) return RAYSHT_EINVAL;
     std::vector<int> seen(lmax + 1, -1);

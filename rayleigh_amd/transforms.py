@@ -24,7 +24,7 @@ import numpy as np
 
 from . import _lib
 
-PRECISIONS = {"fp64": 0, "bf16x3": 1, "bf16x6": 2}
+PRECISIONS = {"fp64": 0, "bf16x3": 1, "bf16x6": 2, "int8x3": 3, "int8x6": 4}
 
 
 def _stream(stream=None) -> ctypes.c_void_p:

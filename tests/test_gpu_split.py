@@ -1,6 +1,7 @@
-"""GPU checks of the split-precision Legendre forms (BF16X6, BF16X3) against the FP64
-oracle and the device FP64 path, at the error tiers the forms are built for:
-BF16X6 <= 1e-12 and BF16X3 <= 1e-4 of max |result| (RMS <= 5e-6).  The BF16X3 bound
+"""GPU checks of the split-precision Legendre forms (BF16X6, BF16X3 and the same digits
+on INT8 MFMA: INT8X6, INT8X3) against the FP64 oracle and the device FP64 path, at the
+error tiers the forms are built for: X6 <= 1e-12 and X3 <= 1e-4 (INT8X3 2e-4) of max
+|result| (RMS <= 5e-6, INT8X3 2e-5).  The BF16X3 bound
 comes from the pairs it drops (level s + t = 3, weight 2^-24 of the operand scales,
 summed over up to 384 K terms): measured 3e-5 - 5e-5 at l_max = 511 with spectra
 decaying as (l+1)^-2, RMS 2.6e-6."""
@@ -12,8 +13,11 @@ from tests.gpu_helpers import dev, empty, host, rel_max
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"bf16x6": 1e-12, "bf16x3": 1e-4}
-TOL_RMS_X3 = 5e-6
+# INT8 digits keep one bit of headroom (the leading signed byte stays within [-64, 64]),
+# which doubles the X3 truncation error
+TOL = {"bf16x6": 1e-12, "bf16x3": 1e-4, "int8x6": 1e-12, "int8x3": 2e-4}
+TOL_RMS = {"bf16x3": 5e-6, "int8x3": 2e-5}
+FORMS = ["bf16x6", "bf16x3", "int8x6", "int8x3"]
 
 
 @pytest.fixture(scope="module")
@@ -32,12 +36,12 @@ def rel_rms(a, b):
 def check(form, got, want):
     err = rel_max(got, want)
     assert err <= TOL[form], f"{form}: max rel {err:.3e}"
-    if form == "bf16x3":
-        assert rel_rms(got, want) <= TOL_RMS_X3
+    if form in TOL_RMS:
+        assert rel_rms(got, want) <= TOL_RMS[form], f"{form}: rms rel {rel_rms(got, want):.3e}"
     return err
 
 
-@pytest.mark.parametrize("form", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("nt,lmax,nr,nf,order", [
     (96, 63, 48, 4, "balanced"),    # l_max=63, N_r=48, 4 fields
     (64, 41, 5, 3, "balanced"),     # 30 columns: partial column tile
@@ -60,7 +64,7 @@ def test_split_vs_oracle(ra, form, nt, lmax, nr, nf, order):
     check(form, host(s), s_ref)
 
 
-@pytest.mark.parametrize("form", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("form", FORMS)
 def test_split_long_contractions(ra, form):
     """l_max = 1023 on 1536 nodes: StP sums 512 terms and PtS 768, so the FP32 level
     sums are moved into FP64 two and three times."""
@@ -79,7 +83,7 @@ def test_split_long_contractions(ra, form):
     check(form, host(s), s_ref)
 
 
-@pytest.mark.parametrize("form", ["bf16x6", "bf16x3"])
+@pytest.mark.parametrize("form", FORMS)
 def test_split_cols_plan(ra, form):
     """The rmcontainer pair (anonymous columns) in the split forms."""
     nt, lmax, ncols = 48, 30, 7
@@ -112,7 +116,7 @@ def test_split_full_size_vs_device_fp64(ra):
     ref.stp(s, p64)
     s64 = empty(ref.spectral_size)
     ref.pts(p64, s64)
-    for form in ("bf16x6", "bf16x3"):
+    for form in FORMS:
         plan = ra.LegendrePlan(nt, lmax, mv, nr, nf, precision=form)
         p = empty(plan.hybrid_size)
         plan.stp(s, p)
@@ -130,15 +134,23 @@ def test_split_full_size_vs_device_fp64(ra):
         torch.cuda.empty_cache()
 
 
-def test_split_round_trip_identity(ra):
-    """N_phi PtS(StP(s)) = s for band-limited s (N_phi = 2 N_theta): BF16X6 keeps the
+@pytest.mark.parametrize("form", ["bf16x6", "int8x6"])
+def test_split_round_trip_identity(ra, form):
+    """N_phi PtS(StP(s)) = s for band-limited s (N_phi = 2 N_theta): the X6 forms keep the
     round trip near the FP64 path's level (~1e-13), within a 1e-11 bound."""
     nt, lmax, nr, nf = 192, 127, 8, 2
     mv = O.m_order_balanced(lmax)
-    plan = ra.LegendrePlan(nt, lmax, mv, nr, nf, precision="bf16x6")
+    plan = ra.LegendrePlan(nt, lmax, mv, nr, nf, precision=form)
     s_np = O.synthetic_spectral(lmax, mv, nr, nf, seed=9)
     p = empty(plan.hybrid_size)
     plan.stp(dev(s_np), p)
     s2 = empty(plan.spectral_size)
     plan.pts(p, s2)
     assert rel_max(host(s2) * (2 * nt), s_np) < 1e-11
+
+
+def test_int8_refuses_long_contractions(ra):
+    """The INT8 forms keep exact int32 level sums up to 16384 terms per contraction."""
+    from rayleigh_amd._lib import RayshtError
+    with pytest.raises(RayshtError):
+        ra.LegendrePlan(32770, 3, np.arange(4, dtype=np.int32), 1, 1, precision="int8x6")
