@@ -52,6 +52,9 @@ __device__ __forceinline__ double4_t mfma_f64(double a, double b, double4_t c)
 #ifndef RAYSHT_PTS_KJ
 #define RAYSHT_PTS_KJ 16
 #endif
+#ifndef RAYSHT_PTS_TK
+#define RAYSHT_PTS_TK 64
+#endif
 // minimum resident waves per SIMD the register allocation must allow
 #ifndef RAYSHT_FP64_MIN_WAVES
 #define RAYSHT_FP64_MIN_WAVES 2
@@ -226,14 +229,15 @@ __global__ __launch_bounds__(STP_THREADS, RAYSHT_FP64_MIN_WAVES) void stp_fp64_k
 constexpr int PTS_THREADS = RAYSHT_PTS_THREADS;
 constexpr int PTS_CB = RAYSHT_PTS_CB;  // 16-column blocks per wave
 constexpr int PTS_TC = (PTS_THREADS / 64) * 16 * PTS_CB;
-constexpr int PTS_TK = 64;
+constexpr int PTS_TK = RAYSHT_PTS_TK;  // parity-l rows of a full tile, per parity
+constexpr int PTS_NBR = PTS_TK / 16;   // 16-row MFMA blocks of a full tile
 constexpr int PTS_KJ = RAYSHT_PTS_KJ;
 constexpr int PTS_F_LD = PTS_KJ + 1;   // Fe/Fo row stride (doubles)
 constexpr int PTS_I_LD = PTS_TK + 4;   // Ie/Io row stride (doubles)
 constexpr int PTS_F_PER_T = PTS_TC * PTS_KJ / PTS_THREADS;
 constexpr int PTS_I_PER_T = PTS_KJ * PTS_TK / PTS_THREADS;  // per parity
 static_assert(PTS_THREADS % 64 == 0 && PTS_THREADS % PTS_KJ == 0 && (PTS_KJ * PTS_TK) % PTS_THREADS == 0 &&
-                  PTS_KJ % 4 == 0,
+                  PTS_KJ % 4 == 0 && PTS_NBR % 4 == 0,
               "PtS tile geometry");
 
 struct pts_item {
@@ -241,9 +245,9 @@ struct pts_item {
     __amdgpu_buffer_rsrc_t rp, rie, rio;
 };
 
-// One PtS tile of NBR 16-row blocks of each parity (NBR = 4: the full 64-row tile; 2: a
-// 32-row tile for the last rows of an m-block, which would leave half of a full tile
-// idle).  LDS buffers come from the kernel so that both shapes share them.
+// One PtS tile of NBR 16-row blocks of each parity: PTS_NBR (the full tile), or a half or
+// quarter of it for the last rows of an m-block, which would leave most of a full tile
+// idle.  LDS buffers come from the kernel so that all shapes share them.
 template <int NBR>
 __device__ __forceinline__ void pts_tile(const pts_item &cur, int nt, int nt2, int nrhs, double *__restrict__ sb,
                                          double *__restrict__ Fe, double *__restrict__ Fo, double *__restrict__ Ie,
@@ -272,7 +276,7 @@ __device__ __forceinline__ void pts_tile(const pts_item &cur, int nt, int nt2, i
         }
         const int vi = ((it.k0 + fcc0) * nt2 + j) * 8;
 #pragma unroll
-        for (int i = 0; i < PTS_I_PER_T * NBR / 4; ++i) {
+        for (int i = 0; i < (PTS_I_PER_T * NBR + PTS_NBR - 1) / PTS_NBR; ++i) {
             const double e = load_f64(it.rie, vi, i * i_sstep);
             const double o = load_f64(it.rio, vi, i * i_sstep);
             rie[i] = jin ? e : 0.0;
@@ -288,7 +292,7 @@ __device__ __forceinline__ void pts_tile(const pts_item &cur, int nt, int nt2, i
             Fo[at] = rfa[i] - rfb[i];
         }
 #pragma unroll
-        for (int i = 0; i < PTS_I_PER_T * NBR / 4; ++i) {
+        for (int i = 0; i < (PTS_I_PER_T * NBR + PTS_NBR - 1) / PTS_NBR; ++i) {
             const int idx = tid + i * PTS_THREADS;
             const int at = (idx % PTS_KJ) * PTS_I_LD + idx / PTS_KJ;
             Ie[at] = rie[i];
@@ -374,10 +378,11 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
     cur.rp = make_rsrc(uniform(io.in[pi]) + (long)mp * nrhs * nt, (long)nrhs * nt * 8);
     cur.rie = make_rsrc(uniform(P.ip_even) + uniform(uniform(P.te_off)[mp]), (long)cur.ne * nt2 * 8);
     cur.rio = make_rsrc(uniform(P.ip_odd) + uniform(uniform(P.to_off)[mp]), (long)cur.no * nt2 * 8);
-    if (pad >> 16)
-        pts_tile<2>(cur, nt, nt2, nrhs, sb, Fe, Fo, Ie, Io);
-    else
-        pts_tile<4>(cur, nt, nt2, nrhs, sb, Fe, Fo, Ie, Io);
+    switch (pad >> 16) {
+    case 0: pts_tile<PTS_NBR>(cur, nt, nt2, nrhs, sb, Fe, Fo, Ie, Io); break;
+    case 1: pts_tile<PTS_NBR / 2>(cur, nt, nt2, nrhs, sb, Fe, Fo, Ie, Io); break;
+    default: pts_tile<PTS_NBR / 4>(cur, nt, nt2, nrhs, sb, Fe, Fo, Ie, Io); break;
+    }
 }
 
 }  // namespace

@@ -541,8 +541,9 @@ __device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, i
 static_assert(kChunkSteps % 2 == 0, "chunks hold whole step pairs");
 
 // --------------------------------------------------------------- StP
-// Work item (mp, c0): kTileC columns, every theta row, in tiles of kTileR rows with
-// both parities each.  Wave w multiplies column block w % kCB by theta block
+// Work item (mp, r0, c0): kTileC columns, kTileR theta rows from r0, both parities.
+// The row tiles of one column tile are consecutive items, so that they run together
+// and share the column tile's spectral data in L2.  Wave w multiplies column block w % kCB by theta block
 // j0/32 + w/kCB.  The spectral tile is read as 16-byte pairs (l, l+1); each parity pass
 // slices its half, VPT values per thread and step.  The column exponents come from one
 // pass over the tile; the scaled even-parity sums wait in LDS for the odd pass.
@@ -556,6 +557,7 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
     const int mp = uniform(work[blockIdx.x].mp);
     if (mp < 0) return;
     const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
+    const int r0 = uniform(work[blockIdx.x].r0);
     const prob_desc &P = probs[pi];
     const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
     const long *frag_off = uniform(P.frag_stp_off);
@@ -599,7 +601,7 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
     using Raw = raw_step<F::VPT>;
     double *ob = uniform(io.out[pi]) + (long)mp * nrhs * nt;
 
-    for (int r0 = 0; r0 < nt2; r0 += kTileR) {
+    {
         const int jb = r0 / 32 + wave / kCB;
         const bool active = jb < njb;
         const int j = jb * 32 + (lane & 31);
@@ -652,8 +654,9 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
 }
 
 // --------------------------------------------------------------- PtS
-// Work item (mp, c0): kTileC columns, every parity-l row of both parities, in row
-// tiles of kTileRP.  Wave w multiplies column block w % kCB of parity w / (kCB kKB) by
+// Work item (mp, k0, c0): kTileC columns, kTileRP parity-l rows from k0 of both
+// parities (the row tiles of one column tile are consecutive items and share its hybrid
+// data in L2).  Wave w multiplies column block w % kCB of parity w / (kCB kKB) by
 // row block k0/32 + (w / kCB) % kKB.  The data operand is the fold f(j) + f(nt-1-j) (even) or
 // f(j) - f(nt-1-j) (odd) of the hybrid columns; each thread slices both folds of VPT
 // nodes per step, and both folds' column exponents come from one pass over the tile.
@@ -666,6 +669,7 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void pts_split_kernel(
     const int mp = uniform(work[blockIdx.x].mp);
     if (mp < 0) return;
     const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
+    const int k0 = uniform(work[blockIdx.x].r0);
     const prob_desc &P = probs[pi];
     const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
     const long *frag_off = uniform(P.frag_pts_off);
@@ -748,7 +752,7 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void pts_split_kernel(
         return sl;
     };
 
-    for (int k0 = 0; k0 < ne; k0 += kTileRP) {
+    {
         const int kb = k0 / 32 + (wave / kCB) % kKB;
         const bool active = kb * 32 < K_rows;
         double dacc[16];
@@ -771,9 +775,10 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void pts_split_kernel(
 // --------------------------------------------------------------- host side
 }  // namespace
 
-// XCD-balanced work lists of (problem, m-block, column tile) items: whole m-blocks dealt
-// to the eight XCDs by longest-processing-time, interleaved so that item i runs on
-// XCD i % 8.  The problem index travels in the `pad` field.
+// XCD-balanced work lists of (problem, m-block, row tile, column tile) items: whole
+// m-blocks dealt to the eight XCDs by longest-processing-time, interleaved so that item
+// i runs on XCD i % 8; within a block the row tiles of one column tile are consecutive.
+// The problem index travels in the `pad` field.
 std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans, bool stp, sched_opts o)
 {
     struct blk {
@@ -787,8 +792,9 @@ std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans,
         const int nrt = cdiv(p->nt2, kTileR), njs = cdiv(p->nt2, ks);
         for (int mp = 0; mp < p->nm; ++mp) {
             const int nl = p->lmax - p->mvals[mp] + 1, ne = (nl + 1) / 2, no = nl / 2;
-            const double steps = stp ? (double)nrt * (cdiv(ne, ks) + cdiv(no, ks)) + cdiv(nl, 32)
-                                     : (double)cdiv(ne, kTileRP) * njs + njs;
+            // K steps (plus the exponent pass) of all row tiles of one column tile
+            const double steps = stp ? (double)nrt * (cdiv(ne, ks) + cdiv(no, ks) + cdiv(nl, 32))
+                                     : (double)cdiv(ne, kTileRP) * (njs + njs);
             order.push_back(blk{pi, mp, steps});
         }
     }
@@ -796,9 +802,12 @@ std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans,
     std::vector<std::vector<work_item>> blocks;
     std::vector<double> cost;
     for (const blk &b : order) {
-        const int nct = cdiv(plans[b.prob]->nrhs, kTileC);
+        const raysht_plan *p = plans[b.prob];
+        const int nct = cdiv(p->nrhs, kTileC), ne = (p->lmax - p->mvals[b.mp] + 2) / 2;
+        const int tile = stp ? kTileR : kTileRP, nrt = cdiv(stp ? p->nt2 : ne, tile);
         std::vector<work_item> items;
-        for (int c = 0; c < nct; ++c) items.push_back(work_item{b.mp, 0, c * kTileC, b.prob});
+        for (int c = 0; c < nct; ++c)
+            for (int r = 0; r < nrt; ++r) items.push_back(work_item{b.mp, r * tile, c * kTileC, b.prob});
         blocks.push_back(std::move(items));
         cost.push_back(b.item * nct);
     }

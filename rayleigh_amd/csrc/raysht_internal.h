@@ -29,7 +29,7 @@ struct dd_pair {
 
 // One unit of grouped-GEMM work: m-block mp, row tile start, column tile start, and in
 // `pad` the problem index (low 16 bits) and the tile shape (high bits: 0 = the full
-// tile, 1 = the PtS half-height tile for the last rows of an m-block).
+// tile, 1 / 2 = the PtS half- / quarter-height tile for the last rows of an m-block).
 struct work_item {
     int mp;
     int r0;

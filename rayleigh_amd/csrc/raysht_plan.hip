@@ -109,16 +109,18 @@ std::vector<work_item> fp64_work(const std::vector<const raysht_plan *> &plans, 
         const raysht_plan *p = plans[b.prob];
         const int rows = stp ? p->nt2 : n_even(p->lmax, p->mvals[b.mp]);
         const int nrt = (rows + tr - 1) / tr, nct = (p->nrhs + tc - 1) / tc;
-        // PtS: a last row tile of at most half the height runs as the half-height shape
-        const bool half = !stp && o.half_tiles && rows - (nrt - 1) * tr <= tr / 2;
+        // PtS: a last row tile of at most a quarter or a half of the height runs as the
+        // quarter- or half-height shape (shape code 2 or 1)
+        const int rem = rows - (nrt - 1) * tr;
+        const int shape = (stp || !o.half_tiles) ? 0 : rem <= tr / 4 ? 2 : rem <= tr / 2 ? 1 : 0;
         std::vector<work_item> items;
         // column tile outer, row tile inner: consecutive items share the column tile's
         // input (spectral columns for StP, hybrid columns for PtS) while it is in L2
         for (int c = 0; c < nct; ++c)
             for (int r = 0; r < nrt; ++r)
-                items.push_back(work_item{b.mp, r * tr, c * tc, b.prob | ((half && r == nrt - 1) << 16)});
+                items.push_back(work_item{b.mp, r * tr, c * tc, b.prob | ((r == nrt - 1 ? shape : 0) << 16)});
         blocks.push_back(std::move(items));
-        cost.push_back(b.item * (nrt - (half ? 0.5 : 0.0)) * nct);
+        cost.push_back(b.item * (nrt - (shape == 2 ? 0.75 : shape == 1 ? 0.5 : 0.0)) * nct);
     }
     return deal_to_xcds(blocks, cost, o.xcd_groups);
 }

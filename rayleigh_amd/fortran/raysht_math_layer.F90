@@ -28,10 +28,13 @@ Module RaySHT_Math_Layer
     Implicit None
     Private
 
-    Public :: Initialize_Legendre, Finalize_Legendre, Legendre_Transform
+    Public :: Initialize_Legendre, Finalize_Legendre, Legendre_Transform, Set_Legendre_Precision
     Public :: FFT_To_Physical, FFT_To_Spectral, FFT_To_Spectral_RSC
     Public :: Cheby_Init, Cheby_To_Spectral, Cheby_From_Spectral, Cheby_Finalize
     Public :: coloc, gl_weights
+    ! arithmetic of the Legendre contractions (enum raysht_precision of include/raysht.h)
+    Integer, Parameter, Public :: raysht_fp64 = 0, raysht_bf16x3 = 1, raysht_bf16x6 = 2, &
+                                  raysht_int8x3 = 3, raysht_int8x6 = 4
 
     ! the Gauss-Legendre grid, as the reference exposes it (Legendre_Polynomials.F90:28)
     Real*8, Allocatable, Save :: coloc(:), gl_weights(:)
@@ -39,7 +42,7 @@ Module RaySHT_Math_Layer
     Integer, Parameter :: max_plans = 8
     Type(C_PTR), Save :: plans(max_plans) = C_NULL_PTR
     Integer, Save :: plan_key(2, max_plans) = -1       ! (n_r, n_fields), or (-ncols, 0) for 2-D
-    Integer, Save :: leg_ntheta = 0, leg_lmax = -1
+    Integer, Save :: leg_ntheta = 0, leg_lmax = -1, leg_precision = 0
     Integer(C_INT), Allocatable, Save :: leg_mvals(:)
     Type(C_PTR), Save :: cheb = C_NULL_PTR
     Integer, Save :: cheb_n = 0
@@ -202,6 +205,24 @@ Contains
         If (allocated(gl_weights)) DeAllocate(gl_weights)
     End Subroutine Finalize_Legendre
 
+    ! Selects the arithmetic of the following Legendre transforms (raysht_fp64, the
+    ! default, or one of the split forms); plans built before are dropped.  No
+    ! counterpart in the reference, which always computes in FP64.
+    Subroutine Set_Legendre_Precision(precision)
+        Integer, Intent(In) :: precision
+        Integer :: i
+        If (precision < raysht_fp64 .or. precision > raysht_int8x6) Then
+            Write(6, *) 'RaySHT: unknown Legendre precision ', precision
+            Error Stop 1
+        Endif
+        leg_precision = precision
+        Do i = 1, max_plans
+            If (c_associated(plans(i))) Call c_plan_destroy(plans(i))
+            plans(i) = C_NULL_PTR
+            plan_key(:, i) = -1
+        Enddo
+    End Subroutine Set_Legendre_Precision
+
     ! The device plan for (nr, nf) (nf = 0: a 2-D plan over -nr columns), built on first use.
     Function plan_for(nr, nf) Result(plan)
         Integer, Intent(In) :: nr, nf
@@ -225,11 +246,11 @@ Contains
             Call c_plan_destroy(plans(i))
         Endif
         If (nf == 0) Then
-            Call check(c_plan_create_cols(plans(i), leg_lmax, leg_ntheta, leg_mvals, size(leg_mvals), -nr, 0), &
-                       'plan (2-D)')
+            Call check(c_plan_create_cols(plans(i), leg_lmax, leg_ntheta, leg_mvals, size(leg_mvals), -nr, &
+                                          leg_precision), 'plan (2-D)')
         Else
-            Call check(c_plan_create(plans(i), leg_lmax, leg_ntheta, leg_mvals, size(leg_mvals), nr, nf, 0), &
-                       'plan')
+            Call check(c_plan_create(plans(i), leg_lmax, leg_ntheta, leg_mvals, size(leg_mvals), nr, nf, &
+                                     leg_precision), 'plan')
         Endif
         plan_key(:, i) = (/ nr, nf /)
         plan = plans(i)

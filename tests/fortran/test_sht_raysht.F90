@@ -27,6 +27,12 @@ Program test_sht_raysht
     Call lt_fft_loop(nfail)
     Call two_d_pair(nfail)
     Call cheby_round_trip(nfail)
+    ! the split forms with 6 digits stay within the same tolerance over ten round trips
+    Call Set_Legendre_Precision(raysht_int8x6)
+    Call white_noise(nfail)
+    Call Set_Legendre_Precision(raysht_bf16x6)
+    Call white_noise(nfail)
+    Call Set_Legendre_Precision(raysht_fp64)
     Call Finalize_Legendre()
     If (nfail > 0) Then
         Write(6, '(A, I0, A)') 'RaySHT Fortran tests: ', nfail, ' FAILED'

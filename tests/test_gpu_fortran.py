@@ -21,4 +21,5 @@ def test_fortran_binding_test_sht():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = [ln for ln in r.stdout.splitlines() if "result =" in ln]
-    assert len(lines) == 6 and all("PASSING" in ln for ln in lines), r.stdout
+    # six FP64 checks, then the white-noise round trips with INT8X6 and BF16X6
+    assert len(lines) == 8 and all("PASSING" in ln for ln in lines), r.stdout
