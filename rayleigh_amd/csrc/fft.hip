@@ -531,34 +531,48 @@ __global__ __launch_bounds__(fft3_threads<N>()) void fft3_r2c_kernel(long howman
 
 // The one-rank row transposes are 2-D transposes per (r, f): the hybrid side is
 // (mode q = 2m+imi) x theta with theta contiguous, the physical side theta x q with q
-// contiguous.  A 32 x 32 tile goes through LDS so both sides are read and written in
-// 256-byte rows.
-constexpr int kTile = 32;
+// contiguous.  A 32 (q) x 64 (theta) tile goes through LDS; both sides move 16-byte
+// pairs (theta pairs on the hybrid side, (Re, Im) of one m on the physical side), so a
+// hybrid row is read or written as 512 contiguous bytes and a physical row as 256.
+// Rows are 16-byte aligned: N_theta and N_phi + 2 are even.
+constexpr int kPT = 64, kPQ = 32;
 
 // 2a -> 3a: p3 line (f, theta, r) gets Re/Im of m at 2m, 2m+1 and zeros elsewhere.
 __global__ __launch_bounds__(256) void permute_2a3a_kernel(int nt, int nr, int nf, int nphi, int lmax,
                                                            const int *__restrict__ mp_of_m,
                                                            const double *__restrict__ p2, double *__restrict__ p3)
 {
-    __shared__ double tile[kTile][kTile + 1];
-    const int t0 = blockIdx.x * kTile, q0 = blockIdx.y * kTile;
+    __shared__ double tile[kPQ][kPT + 2];  // [q][theta]
+    const int t0 = blockIdx.x * kPT, q0 = blockIdx.y * kPQ;
     const int rf = blockIdx.z, r = rf % nr, f = rf / nr;
     const long ld = nphi + 2, nrhs = 2L * nr * nf;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-    for (int i = ty; i < kTile; i += 8) {
-        const int q = q0 + i, t = t0 + tx;
-        double v = 0.0;
-        const int m = q >> 1;
-        if (t < nt && m <= lmax) {
-            const int mp = mp_of_m[m];
-            if (mp >= 0) v = p2[((long)mp * nrhs + r + (long)(q & 1) * nr + 2L * nr * f) * nt + t];
+    const int tid = threadIdx.x;
+    const bool any_m = (q0 >> 1) <= lmax;  // uniform: the tile holds modes of the plan
+    if (any_m) {
+        const int pr = tid & 31;  // theta pair of a hybrid row
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int qq = (tid >> 5) + 8 * i, q = q0 + qq, m = q >> 1, t = t0 + 2 * pr;
+            double2 v = make_double2(0.0, 0.0);
+            if (m <= lmax && t < nt) {
+                const int mp = mp_of_m[m];
+                if (mp >= 0)
+                    v = *reinterpret_cast<const double2 *>(
+                        p2 + ((long)mp * nrhs + r + (long)(q & 1) * nr + 2L * nr * f) * nt + t);
+            }
+            tile[qq][2 * pr] = v.x;
+            tile[qq][2 * pr + 1] = v.y;
         }
-        tile[i][tx] = v;
+        __syncthreads();
     }
-    __syncthreads();
-    for (int i = ty; i < kTile; i += 8) {
-        const int t = t0 + i, q = q0 + tx;
-        if (t < nt && q < ld) p3[(((long)f * nt + t) * nr + r) * ld + q] = tile[tx][i];
+    const int qp = tid & 15;  // (Re, Im) pair of a physical row
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int tt = (tid >> 4) + 16 * i, t = t0 + tt, q = q0 + 2 * qp;
+        if (t < nt && q < ld) {
+            const double2 v = any_m ? make_double2(tile[2 * qp][tt], tile[2 * qp + 1][tt]) : make_double2(0.0, 0.0);
+            *reinterpret_cast<double2 *>(p3 + (((long)f * nt + t) * nr + r) * ld + q) = v;
+        }
     }
 }
 
@@ -567,23 +581,30 @@ __global__ __launch_bounds__(256) void permute_3b2b_kernel(int nt, int nr, int n
                                                            const int *__restrict__ mp_of_m,
                                                            const double *__restrict__ p3, double *__restrict__ p2)
 {
-    __shared__ double tile[kTile][kTile + 1];
-    const int t0 = blockIdx.x * kTile, q0 = blockIdx.y * kTile;
+    __shared__ double tile[kPQ][kPT + 2];  // [q][theta]
+    const int t0 = blockIdx.x * kPT, q0 = blockIdx.y * kPQ;
     const int rf = blockIdx.z, r = rf % nr, f = rf / nr;
     const long ld = nphi + 2, nrhs = 2L * nr * nf;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    const int qmax = 2 * (lmax + 1);
-    for (int i = ty; i < kTile; i += 8) {
-        const int t = t0 + i, q = q0 + tx;
-        tile[i][tx] = (t < nt && q < qmax) ? p3[(((long)f * nt + t) * nr + r) * ld + q] : 0.0;
+    const int tid = threadIdx.x, qmax = 2 * (lmax + 1);
+    const int qp = tid & 15;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int tt = (tid >> 4) + 16 * i, t = t0 + tt, q = q0 + 2 * qp;
+        double2 v = make_double2(0.0, 0.0);
+        if (t < nt && q < qmax) v = *reinterpret_cast<const double2 *>(p3 + (((long)f * nt + t) * nr + r) * ld + q);
+        tile[2 * qp][tt] = v.x;
+        tile[2 * qp + 1][tt] = v.y;
     }
     __syncthreads();
-    for (int i = ty; i < kTile; i += 8) {
-        const int q = q0 + i, t = t0 + tx;
-        const int m = q >> 1;
-        if (t < nt && q < qmax) {
+    const int pr = tid & 31;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int qq = (tid >> 5) + 8 * i, q = q0 + qq, m = q >> 1, t = t0 + 2 * pr;
+        if (q < qmax && t < nt) {
             const int mp = mp_of_m[m];
-            if (mp >= 0) p2[((long)mp * nrhs + r + (long)(q & 1) * nr + 2L * nr * f) * nt + t] = tile[tx][i];
+            if (mp >= 0)
+                *reinterpret_cast<double2 *>(p2 + ((long)mp * nrhs + r + (long)(q & 1) * nr + 2L * nr * f) * nt + t) =
+                    make_double2(tile[qq][2 * pr], tile[qq][2 * pr + 1]);
         }
     }
 }
@@ -749,7 +770,7 @@ int raysht_permute_2a3a(raysht_plan *p, int nphi, const double *p2, double *p3, 
 {
     if (!p || !p2 || !p3 || p->nr < 1 || nphi < 2 * (p->lmax + 1) || (nphi & 1)) return RAYSHT_EINVAL;
     if ((long)p->nr * p->nf > 65535) return RAYSHT_EUNSUPPORTED;
-    dim3 grid((p->nt + kTile - 1) / kTile, (nphi + 2 + kTile - 1) / kTile, p->nr * p->nf);
+    dim3 grid((p->nt + kPT - 1) / kPT, (nphi + 2 + kPQ - 1) / kPQ, p->nr * p->nf);
     hipLaunchKernelGGL(permute_2a3a_kernel, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), p->nt, p->nr,
                        p->nf, nphi, p->lmax, p->d_mp_of_m, p2, p3);
     RAYSHT_HIP_TRY(hipGetLastError());
@@ -760,7 +781,7 @@ int raysht_permute_3b2b(raysht_plan *p, int nphi, const double *p3, double *p2, 
 {
     if (!p || !p2 || !p3 || p->nr < 1 || nphi < 2 * (p->lmax + 1) || (nphi & 1)) return RAYSHT_EINVAL;
     if ((long)p->nr * p->nf > 65535) return RAYSHT_EUNSUPPORTED;
-    dim3 grid((p->nt + kTile - 1) / kTile, (2 * (p->lmax + 1) + kTile - 1) / kTile, p->nr * p->nf);
+    dim3 grid((p->nt + kPT - 1) / kPT, (2 * (p->lmax + 1) + kPQ - 1) / kPQ, p->nr * p->nf);
     hipLaunchKernelGGL(permute_3b2b_kernel, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), p->nt, p->nr,
                        p->nf, nphi, p->lmax, p->d_mp_of_m, p3, p2);
     RAYSHT_HIP_TRY(hipGetLastError());
