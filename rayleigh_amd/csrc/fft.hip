@@ -531,19 +531,21 @@ __global__ __launch_bounds__(fft3_threads<N>()) void fft3_r2c_kernel(long howman
 
 // The one-rank row transposes are 2-D transposes per (r, f): the hybrid side is
 // (mode q = 2m+imi) x theta with theta contiguous, the physical side theta x q with q
-// contiguous.  A 32 (q) x 64 (theta) tile goes through LDS; both sides move 16-byte
+// contiguous.  A Q (q) x 64 (theta) tile goes through LDS; both sides move 16-byte
 // pairs (theta pairs on the hybrid side, (Re, Im) of one m on the physical side), so a
-// hybrid row is read or written as 512 contiguous bytes and a physical row as 256.
+// hybrid row is read or written as 512 contiguous bytes and a physical row as Q * 8.
 // Rows are 16-byte aligned: N_theta and N_phi + 2 are even.
 constexpr int kPT = 64, kPQ = 32;
 
 // 2a -> 3a: p3 line (f, theta, r) gets Re/Im of m at 2m, 2m+1 and zeros elsewhere.
+// Tiles of Q modes (q) x 64 theta: a physical row is written as Q * 8 contiguous bytes.
+template <int Q>
 __global__ __launch_bounds__(256) void permute_2a3a_kernel(int nt, int nr, int nf, int nphi, int lmax,
                                                            const int *__restrict__ mp_of_m,
                                                            const double *__restrict__ p2, double *__restrict__ p3)
 {
-    __shared__ double tile[kPQ][kPT + 2];  // [q][theta]
-    const int t0 = blockIdx.x * kPT, q0 = blockIdx.y * kPQ;
+    __shared__ double tile[Q][kPT + 2];  // [q][theta]
+    const int t0 = blockIdx.x * kPT, q0 = blockIdx.y * Q;
     const int rf = blockIdx.z, r = rf % nr, f = rf / nr;
     const long ld = nphi + 2, nrhs = 2L * nr * nf;
     const int tid = threadIdx.x;
@@ -551,7 +553,7 @@ __global__ __launch_bounds__(256) void permute_2a3a_kernel(int nt, int nr, int n
     if (any_m) {
         const int pr = tid & 31;  // theta pair of a hybrid row
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < Q / 8; ++i) {
             const int qq = (tid >> 5) + 8 * i, q = q0 + qq, m = q >> 1, t = t0 + 2 * pr;
             double2 v = make_double2(0.0, 0.0);
             if (m <= lmax && t < nt) {
@@ -565,16 +567,21 @@ __global__ __launch_bounds__(256) void permute_2a3a_kernel(int nt, int nr, int n
         }
         __syncthreads();
     }
-    const int qp = tid & 15;  // (Re, Im) pair of a physical row
+    constexpr int QP = Q / 2, TR = 256 / QP;  // (Re, Im) pairs of a physical row, rows per pass
+    const int qp = tid % QP;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int tt = (tid >> 4) + 16 * i, t = t0 + tt, q = q0 + 2 * qp;
+    for (int i = 0; i < kPT / TR; ++i) {
+        const int tt = tid / QP + TR * i, t = t0 + tt, q = q0 + 2 * qp;
         if (t < nt && q < ld) {
             const double2 v = any_m ? make_double2(tile[2 * qp][tt], tile[2 * qp + 1][tt]) : make_double2(0.0, 0.0);
             *reinterpret_cast<double2 *>(p3 + (((long)f * nt + t) * nr + r) * ld + q) = v;
         }
     }
 }
+
+// 64 modes per 2a -> 3a tile: 512-byte physical row segments; measured faster than 32 at
+// l_max = 255 and level with it at l_max = 1023.
+constexpr int kPQ2a3a = 64;
 
 // 3b -> 2b: the inverse map, reading back only the m of the plan (m <= lmax).
 __global__ __launch_bounds__(256) void permute_3b2b_kernel(int nt, int nr, int nf, int nphi, int lmax,
@@ -770,8 +777,8 @@ int raysht_permute_2a3a(raysht_plan *p, int nphi, const double *p2, double *p3, 
 {
     if (!p || !p2 || !p3 || p->nr < 1 || nphi < 2 * (p->lmax + 1) || (nphi & 1)) return RAYSHT_EINVAL;
     if ((long)p->nr * p->nf > 65535) return RAYSHT_EUNSUPPORTED;
-    dim3 grid((p->nt + kPT - 1) / kPT, (nphi + 2 + kPQ - 1) / kPQ, p->nr * p->nf);
-    hipLaunchKernelGGL(permute_2a3a_kernel, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), p->nt, p->nr,
+    dim3 grid((p->nt + kPT - 1) / kPT, (nphi + 2 + kPQ2a3a - 1) / kPQ2a3a, p->nr * p->nf);
+    hipLaunchKernelGGL(permute_2a3a_kernel<kPQ2a3a>, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), p->nt, p->nr,
                        p->nf, nphi, p->lmax, p->d_mp_of_m, p2, p3);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
