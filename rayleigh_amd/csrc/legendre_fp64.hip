@@ -46,30 +46,27 @@ __device__ __forceinline__ double4_t mfma_f64(double a, double b, double4_t c)
 #ifndef RAYSHT_PTS_THREADS
 #define RAYSHT_PTS_THREADS 512
 #endif
-#ifndef RAYSHT_PTS_CB
-#define RAYSHT_PTS_CB 1
-#endif
-#ifndef RAYSHT_PTS_KJ
-#define RAYSHT_PTS_KJ 16
-#endif
 #ifndef RAYSHT_PTS_TK
 #define RAYSHT_PTS_TK 64
 #endif
-// minimum resident waves per SIMD the register allocation must allow
+// minimum resident waves per SIMD the register allocation must allow: 4 = 128 VGPRs,
+// two 8-wave workgroups per CU
 #ifndef RAYSHT_FP64_MIN_WAVES
-#define RAYSHT_FP64_MIN_WAVES 2
+#define RAYSHT_FP64_MIN_WAVES 4
 #endif
 
 // Raw buffer access: loads whose byte offset falls outside [0, bytes) return 0
-// without touching memory, so tile edges need no branches.
+// without touching memory, so tile edges need no branches.  The whole offset goes in
+// the VGPR operand: the SGPR offset of a buffer load is not range-checked.  A negative
+// offset reads as a huge unsigned one, so it is out of range too.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const double *base, long bytes)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, (int)bytes, 0x00020000);
 }
 
-__device__ __forceinline__ double load_f64(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+__device__ __forceinline__ double load_f64(__amdgpu_buffer_rsrc_t r, int voff)
 {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
 }
 
 // ---------------------------------------------------------------- StP
@@ -134,34 +131,31 @@ __global__ __launch_bounds__(STP_THREADS, RAYSHT_FP64_MIN_WAVES) void stp_fp64_k
     const int a_voff = (akk0 * nt2 + cur.j0 + ajj) * 8;
     const int a_sstep = (STP_THREADS / STP_TJ) * nt2 * 8;
 
+    // The loads land in registers untouched; edge masks are applied when they are written
+    // to LDS, so nothing waits for them before the current chunk is multiplied.
     double rb[STP_B_PER_T], rae[STP_A_PER_T], rao[STP_A_PER_T];
     auto fetch = [&](const stp_item &it, int k0) {
-        const bool lin = 2 * k0 + bq < it.nl;  // rows past a column's last l belong to the next column
 #pragma unroll
-        for (int i = 0; i < STP_B_PER_T; ++i) {
-            const double v = load_f64(it.rs, b_voff + 2 * k0 * 8, i * b_sstep);
-            rb[i] = lin ? v : 0.0;
-        }
+        for (int i = 0; i < STP_B_PER_T; ++i) rb[i] = load_f64(it.rs, b_voff + 2 * k0 * 8 + i * b_sstep);
         const int a_s0 = k0 * nt2 * 8;
 #pragma unroll
         for (int i = 0; i < STP_A_PER_T; ++i) {
-            const double ve = load_f64(it.rpe, a_voff, a_s0 + i * a_sstep);
-            const double vo = load_f64(it.rpo, a_voff, a_s0 + i * a_sstep);
-            rae[i] = a_jin ? ve : 0.0;
-            rao[i] = a_jin ? vo : 0.0;
+            rae[i] = load_f64(it.rpe, a_voff + a_s0 + i * a_sstep);
+            rao[i] = load_f64(it.rpo, a_voff + a_s0 + i * a_sstep);
         }
     };
-    auto stash = [&]() {
+    auto stash = [&](int k0) {
+        const bool lin = 2 * k0 + bq < cur.nl;  // rows past a column's last l belong to the next column
 #pragma unroll
         for (int i = 0; i < STP_B_PER_T; ++i) {
             const int idx = tid + i * STP_THREADS;
-            Bs[(idx / (2 * STP_KC)) * STP_BS_LD + idx % (2 * STP_KC)] = rb[i];
+            Bs[(idx / (2 * STP_KC)) * STP_BS_LD + idx % (2 * STP_KC)] = lin ? rb[i] : 0.0;
         }
 #pragma unroll
         for (int i = 0; i < STP_A_PER_T; ++i) {
             const int idx = tid + i * STP_THREADS;
-            Ae[(idx / STP_TJ) * STP_A_LD + idx % STP_TJ] = rae[i];
-            Ao[(idx / STP_TJ) * STP_A_LD + idx % STP_TJ] = rao[i];
+            Ae[(idx / STP_TJ) * STP_A_LD + idx % STP_TJ] = a_jin ? rae[i] : 0.0;
+            Ao[(idx / STP_TJ) * STP_A_LD + idx % STP_TJ] = a_jin ? rao[i] : 0.0;
         }
     };
 
@@ -176,7 +170,7 @@ __global__ __launch_bounds__(STP_THREADS, RAYSHT_FP64_MIN_WAVES) void stp_fp64_k
                 acc_o[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
             }
         for (int k0 = 0; k0 < cur.ne; k0 += STP_KC) {
-            stash();
+            stash(k0);
             __syncthreads();
             if (k0 + STP_KC < cur.ne) fetch(cur, k0 + STP_KC);
 #pragma unroll 1
@@ -222,22 +216,23 @@ __global__ __launch_bounds__(STP_THREADS, RAYSHT_FP64_MIN_WAVES) void stp_fp64_k
 }
 
 // ---------------------------------------------------------------- PtS
-// Workgroup tile: (waves x 16 x PTS_CB) columns (c) x 64 parity-l values of each
-// parity (128 l).  Wave w owns PTS_CB blocks of 16 columns: PTS_CB x 4 l blocks x 2
-// parities.  K runs over chunks of PTS_KJ southern nodes; the fold f(j) +/- f(nt+1-j) is formed
-// on the way into LDS, and the next chunk is fetched while this one is multiplied.
+// Workgroup tile: (waves x 16) columns (c) x 64 parity-l values of each parity (128 l).
+// Wave w owns 16 columns: 4 l blocks x 2 parities of 16x16 accumulators.  K runs over
+// chunks of 16 southern nodes.  A wave's data columns are its own, so each lane loads its
+// part of the next chunk (4 nodes and their mirrors, two 16-byte loads each) while the
+// current chunk is multiplied, forms the fold f(j) +/- f(nt+1-j) and leaves it in the
+// wave's own LDS strip; only the table chunk is shared by the waves and goes through a
+// double-buffered LDS tile, so one barrier per chunk suffices.
 constexpr int PTS_THREADS = RAYSHT_PTS_THREADS;
-constexpr int PTS_CB = RAYSHT_PTS_CB;  // 16-column blocks per wave
-constexpr int PTS_TC = (PTS_THREADS / 64) * 16 * PTS_CB;
+constexpr int PTS_TC = (PTS_THREADS / 64) * 16;
 constexpr int PTS_TK = RAYSHT_PTS_TK;  // parity-l rows of a full tile, per parity
 constexpr int PTS_NBR = PTS_TK / 16;   // 16-row MFMA blocks of a full tile
-constexpr int PTS_KJ = RAYSHT_PTS_KJ;
-constexpr int PTS_F_LD = PTS_KJ + 1;   // Fe/Fo row stride (doubles)
-constexpr int PTS_I_LD = PTS_TK + 4;   // Ie/Io row stride (doubles)
-constexpr int PTS_F_PER_T = PTS_TC * PTS_KJ / PTS_THREADS;
-constexpr int PTS_I_PER_T = PTS_KJ * PTS_TK / PTS_THREADS;  // per parity
+constexpr int PTS_KJ = 16;             // nodes per K chunk: 4 MFMA steps of 4
+constexpr int PTS_I_LD = PTS_TK + 4;   // table chunk row stride (doubles): lane groups 0 / 1 on disjoint banks
+constexpr int PTS_IBUF = PTS_KJ * PTS_I_LD;  // one parity's table chunk in LDS
+constexpr int PTS_I_PER_T = PTS_KJ * PTS_TK / PTS_THREADS;  // table values per thread and parity
 static_assert(PTS_THREADS % 64 == 0 && PTS_THREADS % PTS_KJ == 0 && (PTS_KJ * PTS_TK) % PTS_THREADS == 0 &&
-                  PTS_KJ % 4 == 0 && PTS_NBR % 4 == 0,
+                  PTS_NBR % 4 == 0,
               "PtS tile geometry");
 
 struct pts_item {
@@ -245,122 +240,132 @@ struct pts_item {
     __amdgpu_buffer_rsrc_t rp, rie, rio;
 };
 
+__device__ __forceinline__ double2 load_f64x2(__amdgpu_buffer_rsrc_t r, int voff)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
+}
+
 // One PtS tile of NBR 16-row blocks of each parity: PTS_NBR (the full tile), or a half or
 // quarter of it for the last rows of an m-block, which would leave most of a full tile
 // idle.  LDS buffers come from the kernel so that all shapes share them.
+//
+// K order: MFMA step s of a chunk takes node j0 + 4 lg + s on lane group lg (any order
+// works as long as both operands follow it), so that a lane's four nodes are contiguous:
+// lane (li, lg) loads column c0 + 16 wave + li at nodes j0 + 4 lg .. + 3 and at their
+// mirrors nt - 1 - j as two 16-byte loads each.  Rows are 16-byte aligned (nt is even) and
+// a 16-byte piece never straddles a column end, so out-of-range pieces (columns past nrhs)
+// read 0 as a whole.
 template <int NBR>
 __device__ __forceinline__ void pts_tile(const pts_item &cur, int nt, int nt2, int nrhs, double *__restrict__ sb,
-                                         double *__restrict__ Fe, double *__restrict__ Fo, double *__restrict__ Ie,
-                                         double *__restrict__ Io)
+                                         double *__restrict__ Ib, double *__restrict__ Fb)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
-    // Per-thread load geometry.  F: column fcc0 + (PTS_THREADS / KJ) i, node j0 + fjj and
-    // its mirror; I: parity-l row fcc0 + (PTS_THREADS / KJ) i, node j0 + fjj.
+    const int c = cur.c0 + 16 * wave + li;
+    const int va = (c * nt + 4 * lg) * 8;           // + 8 j0
+    const int vb = (c * nt + nt - 4 - 4 * lg) * 8;  // - 8 j0: mirrors of nodes j0 + 4 lg + 3 .. + 0
+    // table loads: parity-l row fcc0 + (PTS_THREADS / KJ) i, node j0 + fjj
     const int fjj = tid % PTS_KJ, fcc0 = tid / PTS_KJ;
-    const int f_sstep = (PTS_THREADS / PTS_KJ) * nt * 8;
     const int i_sstep = (PTS_THREADS / PTS_KJ) * nt2 * 8;
+    constexpr int NI = (PTS_I_PER_T * NBR + PTS_NBR - 1) / PTS_NBR;
+    double *Fw = Fb + wave * 512;  // this wave's folded data: [parity][s][lane]
 
-    double rfa[PTS_F_PER_T], rfb[PTS_F_PER_T], rie[PTS_I_PER_T], rio[PTS_I_PER_T];
-    auto fetch = [&](const pts_item &it, int j0) {
-        const int j = j0 + fjj;
-        const bool jin = j < nt2;
-        const int va = ((it.c0 + fcc0) * nt + j) * 8;
-        const int vb = ((it.c0 + fcc0) * nt + (nt - 1 - j)) * 8;
+    // The loads land in registers untouched; edge masks are applied in settle(), so
+    // nothing waits for them before the current chunk is multiplied.
+    double2 fa0, fa1, fb0, fb1;  // raw data of the next chunk
+    double rie[NI], rio[NI];     // table values of the next chunk
+    auto fetch_f = [&](int j0) {
+        fa0 = load_f64x2(cur.rp, va + 8 * j0);
+        fa1 = load_f64x2(cur.rp, va + 8 * j0 + 16);
+        fb0 = load_f64x2(cur.rp, vb - 8 * j0);
+        fb1 = load_f64x2(cur.rp, vb - 8 * j0 + 16);
+    };
+    auto fetch_i = [&](int j0) {
+        const int vi = ((cur.k0 + fcc0) * nt2 + j0 + fjj) * 8;
 #pragma unroll
-        for (int i = 0; i < PTS_F_PER_T; ++i) {
-            const double a = load_f64(it.rp, va, i * f_sstep);
-            const double b = load_f64(it.rp, vb, i * f_sstep);
-            rfa[i] = jin ? a : 0.0;
-            rfb[i] = jin ? b : 0.0;
-        }
-        const int vi = ((it.k0 + fcc0) * nt2 + j) * 8;
-#pragma unroll
-        for (int i = 0; i < (PTS_I_PER_T * NBR + PTS_NBR - 1) / PTS_NBR; ++i) {
-            const double e = load_f64(it.rie, vi, i * i_sstep);
-            const double o = load_f64(it.rio, vi, i * i_sstep);
-            rie[i] = jin ? e : 0.0;
-            rio[i] = jin ? o : 0.0;
+        for (int i = 0; i < NI; ++i) {
+            rie[i] = load_f64(cur.rie, vi + i * i_sstep);
+            rio[i] = load_f64(cur.rio, vi + i * i_sstep);
         }
     };
-    auto stash = [&]() {
+    // fold the fetched data into the wave's strip and the table into LDS buffer `buf`
+    auto settle = [&](int j0, int buf) {
+        const double a[4] = {fa0.x, fa0.y, fa1.x, fa1.y};
+        const double b[4] = {fb1.y, fb1.x, fb0.y, fb0.x};  // mirror of node j0 + 4 lg + s
 #pragma unroll
-        for (int i = 0; i < PTS_F_PER_T; ++i) {
-            const int idx = tid + i * PTS_THREADS;
-            const int at = (idx / PTS_KJ) * PTS_F_LD + idx % PTS_KJ;
-            Fe[at] = rfa[i] + rfb[i];
-            Fo[at] = rfa[i] - rfb[i];
+        for (int s = 0; s < 4; ++s) {
+            const bool jin = j0 + 4 * lg + s < nt2;
+            Fw[s * 64 + lane] = jin ? a[s] + b[s] : 0.0;
+            Fw[256 + s * 64 + lane] = jin ? a[s] - b[s] : 0.0;
         }
+        double *Ie = Ib + buf * (2 * PTS_IBUF), *Io = Ie + PTS_IBUF;
+        const bool jin = j0 + fjj < nt2;
 #pragma unroll
-        for (int i = 0; i < (PTS_I_PER_T * NBR + PTS_NBR - 1) / PTS_NBR; ++i) {
+        for (int i = 0; i < NI; ++i) {
             const int idx = tid + i * PTS_THREADS;
             const int at = (idx % PTS_KJ) * PTS_I_LD + idx / PTS_KJ;
-            Ie[at] = rie[i];
-            Io[at] = rio[i];
+            Ie[at] = jin ? rie[i] : 0.0;
+            Io[at] = jin ? rio[i] : 0.0;
         }
     };
 
-    fetch(cur, 0);
-    {
-        double4_t acc_e[PTS_CB][NBR], acc_o[PTS_CB][NBR];
+    double4_t acc_e[NBR], acc_o[NBR];
 #pragma unroll
-        for (int cb = 0; cb < PTS_CB; ++cb)
+    for (int b = 0; b < NBR; ++b) {
+        acc_e[b] = double4_t{0.0, 0.0, 0.0, 0.0};
+        acc_o[b] = double4_t{0.0, 0.0, 0.0, 0.0};
+    }
+    fetch_f(0);
+    fetch_i(0);
+    settle(0, 0);
+    __syncthreads();
+    int buf = 0;
+    for (int j0 = 0; j0 < nt2; j0 += PTS_KJ) {
+        // buffer buf ^ 1 and the strip were last read before the previous barrier
+        const bool more = j0 + PTS_KJ < nt2;
+        if (more) fetch_f(j0 + PTS_KJ);
+        const double *Ie = Ib + buf * (2 * PTS_IBUF), *Io = Ie + PTS_IBUF;
+#pragma unroll 2
+        for (int s = 0; s < 4; ++s) {
+            double be[NBR], bo[NBR];
 #pragma unroll
             for (int b = 0; b < NBR; ++b) {
-                acc_e[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
-                acc_o[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+                be[b] = Ie[(4 * lg + s) * PTS_I_LD + 16 * b + li];
+                bo[b] = Io[(4 * lg + s) * PTS_I_LD + 16 * b + li];
             }
-        for (int j0 = 0; j0 < nt2; j0 += PTS_KJ) {
-            stash();
-            __syncthreads();
-            if (j0 + PTS_KJ < nt2) fetch(cur, j0 + PTS_KJ);
-#pragma unroll 1
-            for (int jj = 0; jj < PTS_KJ; jj += 4) {
-                double be[NBR], bo[NBR];
+            const double aes = Fw[s * 64 + lane], aos = Fw[256 + s * 64 + lane];
 #pragma unroll
-                for (int b = 0; b < NBR; ++b) {
-                    be[b] = Ie[(jj + lg) * PTS_I_LD + 16 * b + li];
-                    bo[b] = Io[(jj + lg) * PTS_I_LD + 16 * b + li];
-                }
-#pragma unroll
-                for (int cb = 0; cb < PTS_CB; ++cb) {
-                    const int row = (16 * PTS_CB * wave + 16 * cb + li) * PTS_F_LD + jj + lg;
-                    const double a_e = Fe[row];
-                    const double a_o = Fo[row];
-#pragma unroll
-                    for (int b = 0; b < NBR; ++b) {
-                        acc_e[cb][b] = mfma_f64(a_e, be[b], acc_e[cb][b]);
-                        acc_o[cb][b] = mfma_f64(a_o, bo[b], acc_o[cb][b]);
-                    }
-                }
+            for (int b = 0; b < NBR; ++b) {
+                acc_e[b] = mfma_f64(aes, be[b], acc_e[b]);
+                acc_o[b] = mfma_f64(aos, bo[b], acc_o[b]);
             }
-            __syncthreads();
+            // the table chunk comes from L2 (every column tile reads it): fetched last
+            if (s == 3 && more) fetch_i(j0 + PTS_KJ);
         }
+        buf ^= 1;
+        if (more) settle(j0 + PTS_KJ, buf);
+        __syncthreads();
+    }
 
 #pragma unroll
-        for (int cb = 0; cb < PTS_CB; ++cb)
+    for (int b = 0; b < NBR; ++b) {
+        const int k = cur.k0 + 16 * b + li;
 #pragma unroll
-            for (int b = 0; b < NBR; ++b) {
-                const int k = cur.k0 + 16 * b + li;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int c = cur.c0 + 16 * PTS_CB * wave + 16 * cb + lg + 4 * v;
-                    if (c < nrhs) {
-                        if (k < cur.ne) sb[(long)c * cur.nl + 2 * k] = acc_e[cb][b][v];
-                        if (k < cur.no) sb[(long)c * cur.nl + 2 * k + 1] = acc_o[cb][b][v];
-                    }
-                }
+        for (int v = 0; v < 4; ++v) {
+            const int cc = cur.c0 + 16 * wave + lg + 4 * v;
+            if (cc < nrhs) {
+                if (k < cur.ne) sb[(long)cc * cur.nl + 2 * k] = acc_e[b][v];
+                if (k < cur.no) sb[(long)cc * cur.nl + 2 * k + 1] = acc_o[b][v];
             }
+        }
     }
 }
 
 __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_kernel(
     const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
 {
-    __shared__ double Fe[PTS_TC * PTS_F_LD];
-    __shared__ double Fo[PTS_TC * PTS_F_LD];
-    __shared__ double Ie[PTS_KJ * PTS_I_LD];
-    __shared__ double Io[PTS_KJ * PTS_I_LD];
+    __shared__ double Ib[2 * 2 * PTS_IBUF];          // [buffer][parity] table chunks
+    __shared__ double Fb[PTS_THREADS / 64 * 512];    // per wave: folded data [parity][step][lane]
 
     const int mp = uniform(work[blockIdx.x].mp);
     if (mp < 0) return;  // padding item of the XCD-balanced schedule
@@ -379,9 +384,9 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
     cur.rie = make_rsrc(uniform(P.ip_even) + uniform(uniform(P.te_off)[mp]), (long)cur.ne * nt2 * 8);
     cur.rio = make_rsrc(uniform(P.ip_odd) + uniform(uniform(P.to_off)[mp]), (long)cur.no * nt2 * 8);
     switch (pad >> 16) {
-    case 0: pts_tile<PTS_NBR>(cur, nt, nt2, nrhs, sb, Fe, Fo, Ie, Io); break;
-    case 1: pts_tile<PTS_NBR / 2>(cur, nt, nt2, nrhs, sb, Fe, Fo, Ie, Io); break;
-    default: pts_tile<PTS_NBR / 4>(cur, nt, nt2, nrhs, sb, Fe, Fo, Ie, Io); break;
+    case 0: pts_tile<PTS_NBR>(cur, nt, nt2, nrhs, sb, Ib, Fb); break;
+    case 1: pts_tile<PTS_NBR / 2>(cur, nt, nt2, nrhs, sb, Ib, Fb); break;
+    default: pts_tile<PTS_NBR / 4>(cur, nt, nt2, nrhs, sb, Ib, Fb); break;
     }
 }
 
