@@ -339,8 +339,9 @@ __device__ __forceinline__ void pts_tile(const pts_item &cur, int nt, int nt2, i
                 acc_e[b] = mfma_f64(aes, be[b], acc_e[b]);
                 acc_o[b] = mfma_f64(aos, bo[b], acc_o[b]);
             }
-            // the table chunk comes from L2 (every column tile reads it): fetched last
-            if (s == 3 && more) fetch_i(j0 + PTS_KJ);
+            // the table chunk comes from L2 (every column tile reads it): fetched after the
+            // third step, late enough to keep its registers free, early enough to land
+            if (s == 2 && more) fetch_i(j0 + PTS_KJ);
         }
         buf ^= 1;
         if (more) settle(j0 + PTS_KJ, buf);
