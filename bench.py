@@ -210,7 +210,8 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
     """The transform stack (default l_max = 255, N_theta = 384, N_phi = 768, N_r = 128,
     8 fields; "large": l_max = 1023, N_theta = 1536, N_phi = 3072, N_r = 256, 5 fields):
     Chebyshev From_Spectral -> StP -> 2a->3a -> c2r, then r2c -> 3b->2b -> PtS -> To_Spectral,
-    each stage timed with HIP events; GB/s of the bandwidth-bound stages.  The radial
+    each stage timed with HIP events; GB/s of the bandwidth-bound stages; then the radial
+    derivative recursion of orders 1-3 on the same spectral radial buffer.  The radial
     buffer is transformed in place of Rayleigh's p1 layout (the 1a<->2a radial transposes
     between the Chebyshev and Legendre stages are not implemented)."""
     import torch
@@ -264,6 +265,24 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
             rec["gbps"] = nbytes / (ms * 1e-3) / 1e9
             rec["frac_hbm_peak"] = rec["gbps"] / 8000.0
         out["stages"][name] = rec
+    # the radial derivative recursion (d_by_dr_cp) of orders 1-3 on the spectral radial
+    # buffer, as Post_Solve applies it: 16 B per coefficient (read + write)
+    rad3 = torch.empty_like(rad)
+    nbytes = 16.0 * rad.numel()
+    out["radial_derivatives"] = {}
+    for d in (1, 2, 3):
+        for _ in range(max(warmup, 1)):
+            cheb.d_by_dr(rad, rad3, d)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            cheb.d_by_dr(rad, rad3, d)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out["radial_derivatives"][f"order{d}"] = {"ms": ms, "gbps": nbytes / (ms * 1e-3) / 1e9,
+                                                  "frac_hbm_peak": nbytes / (ms * 1e-3) / 8e12}
+    del rad3
     # the Legendre + FFT round trip replayed from one hipGraph (no Chebyshev stages)
     g_rt = plan.round_trip_graph(nphi, s, p2, p3, s2)
     for _ in range(warmup):

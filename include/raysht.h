@@ -188,6 +188,14 @@ int raysht_cheb_to_spectral_host(raysht_cheb *cheb, const double *f, double *c, 
 int raysht_cheb_from_spectral_host(raysht_cheb *cheb, const double *c, double *f, long ncols);
 /* Cheby_Grid%dealias_buffer = dealias_buffer4d (:353-374): zero n >= 2n/3 (0-based). */
 int raysht_cheb_dealias(raysht_cheb *cheb, double *c, long ncols, void *stream);
+/* Cheby_Grid%d_by_dr_cp = Cheby_Deriv_Buffer_4D (:560-640) with ind /= dind, and
+ * Cheby_Grid%d_by_dr_nobuffer = Cheby_Deriv_OutOf_Buffer (:642-730): d receives the
+ * coefficients of the dorder-th radial derivative of every column of c (ncols columns
+ * of n, device arrays).  Out of place only (overlapping c and d: RAYSHT_EINVAL; the
+ * reference's in-place branch only rescales); dorder 1..4 (larger: RAYSHT_EUNSUPPORTED). */
+int raysht_cheb_deriv(raysht_cheb *cheb, const double *c, double *d, long ncols, int dorder, void *stream);
+/* Host-array form of raysht_cheb_deriv (synchronous); what the Fortran d_by_dr_cp calls. */
+int raysht_cheb_deriv_host(raysht_cheb *cheb, const double *c, double *d, long ncols, int dorder);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,9 @@ def oracle_lib():
         lib.orc_cheby_to_spectral.argtypes = [_I, _L, _P, _P, _P, _P]
         lib.orc_cheby_from_spectral.argtypes = [_I, _L, _P, _P, _P, _P]
         lib.orc_cheby_dealias.argtypes = [_I, _L, _P]
+        lib.orc_cheby_deriv_scaling.restype = _D
+        lib.orc_cheby_deriv_scaling.argtypes = [_I, _D, _D]
+        lib.orc_cheby_deriv.argtypes = [_I, _D, _I, _L, _P, _P, _I]
         lib.orc_m_order_balanced.argtypes = [_I, _P]
         _oracle = lib
     return _oracle
@@ -93,6 +96,8 @@ def ref_lib():
         lib.ref_cheby_tables.argtypes = [_P, _P]
         lib.ref_cheby_to_spectral.argtypes = [_I, _P, _P]
         lib.ref_cheby_from_spectral.argtypes = [_I, _P, _P]
+        lib.ref_cheby_deriv.argtypes = [_I, _I, _P, _P]
+        lib.ref_cheby_deriv_nobuffer.argtypes = [_I, _I, _P, _P]
         _ref = lib
     return _ref
 
@@ -257,12 +262,31 @@ def cheby_to_spectral(n: int, f: np.ndarray) -> np.ndarray:
     return c
 
 
+# whether the compiled reference's recursion step b(i+2) + 2(i+1) a(i+1) is one fused
+# multiply-add (fixed by comparing against it: tests/test_oracle_cheby_deriv.py)
+CHEBY_DERIV_FUSED = False
+
+
 def cheby_from_spectral(n: int, c: np.ndarray) -> np.ndarray:
     ce, co = cheby_tables(n)
     c = np.ascontiguousarray(c, dtype=np.float64)
     f = np.zeros_like(c)
     oracle_lib().orc_cheby_from_spectral(n, c.size // n, _ptr(ce), _ptr(co), _ptr(c), _ptr(f))
     return f
+
+
+def cheby_deriv_scaling(n: int, rmin: float, rmax: float) -> float:
+    return oracle_lib().orc_cheby_deriv_scaling(n, rmin, rmax)
+
+
+def cheby_deriv(n: int, rmin: float, rmax: float, c: np.ndarray, dorder: int, fused: bool = CHEBY_DERIV_FUSED) -> np.ndarray:
+    """Chebyshev_Polynomials.F90:560-730: the dorder-th radial derivative's coefficients
+    of every column of n (d_by_dr_cp / d_by_dr_nobuffer), one domain on [rmin, rmax]."""
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    d = np.zeros_like(c)
+    ds = cheby_deriv_scaling(n, rmin, rmax)
+    oracle_lib().orc_cheby_deriv(n, ds, dorder, c.size // n, _ptr(c), _ptr(d), int(fused))
+    return d
 
 
 def cheby_dealias(n: int, c: np.ndarray) -> np.ndarray:
@@ -337,3 +361,10 @@ class Reference:
         f = np.zeros_like(c)
         self.lib.ref_cheby_from_spectral(c.size // self.ncheb, _ptr(c), _ptr(f))
         return f
+
+    def cheby_deriv(self, c: np.ndarray, dorder: int, nobuffer: bool = False) -> np.ndarray:
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        d = np.zeros_like(c)
+        fn = self.lib.ref_cheby_deriv_nobuffer if nobuffer else self.lib.ref_cheby_deriv
+        fn(c.size // self.ncheb, dorder, _ptr(c), _ptr(d))
+        return d

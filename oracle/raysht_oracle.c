@@ -558,6 +558,58 @@ void orc_cheby_dealias(int n, long ncols, double *c)
         for (int i = rda - 1; i < n; ++i) c[col * n + i] = 0.0;
 }
 
+/* Chebyshev_Polynomials.F90:176-191: the one-domain derivative scaling, 1 / scaling with
+ * scaling = (rmax - rmin) / (x_1 - x_n) (the zeros grid's end points). */
+double orc_cheby_deriv_scaling(int n, double rmin, double rmax)
+{
+    double *th = malloc(sizeof(double) * (size_t)n);
+    cheby_theta(n, th);
+    const double xmin = cos(th[n - 1]), xmax = cos(th[0]);
+    free(th);
+    const double scaling = (rmax - rmin) / (xmax - xmin);
+    return 1.0 / scaling;
+}
+
+/* Fortran's x**k for the small integer k of a derivative order: x, x*x, (x*x)*x,
+ * (x*x)*(x*x) */
+static double ipow_small(double x, int k)
+{
+    double r = 1.0;
+    if (k == 1) return x;
+    if (k == 2) return x * x;
+    if (k == 3) return (x * x) * x;
+    if (k == 4) return (x * x) * (x * x);
+    for (int i = 0; i < k; ++i) r *= x;
+    return r;
+}
+
+/* Chebyshev_Polynomials.F90:560-640 Cheby_Deriv_Buffer_4D (d_by_dr_cp, ind /= dind) and
+ * :642-730 Cheby_Deriv_OutOf_Buffer (d_by_dr_nobuffer), one domain of n coefficients per
+ * column: b(n-1) = 0, b(n-2) = 2(n-1) a(n-1), b(i) = b(i+2) + 2(i+1) a(i+1) for i = n-3 .. 0,
+ * applied dorder times (each pass to the previous pass's result), then every entry
+ * times deriv_scaling**dorder.  `fused` selects fma(2(i+1), a, b) for the step instead
+ * of a rounded product and a rounded sum; the compiled reference decides which. */
+void orc_cheby_deriv(int n, double ds, int dorder, long ncols, const double *c, double *d, int fused)
+{
+    double *cur = malloc(sizeof(double) * (size_t)n), *nxt = malloc(sizeof(double) * (size_t)n);
+    const double p = ipow_small(ds, dorder);
+    for (long col = 0; col < ncols; ++col) {
+        for (int i = 0; i < n; ++i) cur[i] = c[col * n + i];
+        for (int o = 0; o < dorder; ++o) {
+            nxt[n - 1] = 0.0;
+            if (n >= 2) nxt[n - 2] = 2.0 * (n - 1) * cur[n - 1];
+            for (int i = n - 3; i >= 0; --i)
+                nxt[i] = fused ? fma(2.0 * (i + 1), cur[i + 1], nxt[i + 2]) : nxt[i + 2] + 2.0 * (i + 1) * cur[i + 1];
+            double *t = cur;
+            cur = nxt;
+            nxt = t;
+        }
+        for (int i = 0; i < n; ++i) d[col * n + i] = cur[i] * p;
+    }
+    free(cur);
+    free(nxt);
+}
+
 /* Load_Balance.F90:126-226 M_Balance_v0 for one rank: pairs (i, n_m-1-i), then
  * the middle m when n_m is odd. */
 void orc_m_order_balanced(int lmax, int *mvals)

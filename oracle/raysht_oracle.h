@@ -71,6 +71,8 @@ void orc_cheby_to_spectral(int n, long ncols, const double *ceven, const double 
 void orc_cheby_from_spectral(int n, long ncols, const double *ceven, const double *codd,
                              const double *c, double *f);
 void orc_cheby_dealias(int n, long ncols, double *c);
+double orc_cheby_deriv_scaling(int n, double rmin, double rmax);
+void orc_cheby_deriv(int n, double ds, int dorder, long ncols, const double *c, double *d, int fused);
 
 /* Balanced single-rank m order of M_Balance_v0: 0, L, 1, L-1, ... */
 void orc_m_order_balanced(int lmax, int *mvals);

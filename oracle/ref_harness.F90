@@ -188,4 +188,32 @@ Contains
         Call h_grid%from_spectral(c_in, f_out)
     End Subroutine ref_cheby_from_spectral
 
+    ! Cheby_Deriv_Buffer_4D (d_by_dr_cp) out of place: buffer(:,:,1,1) = c_in,
+    ! buffer(:,:,1,2) receives the dorder-th derivative's coefficients.
+    Subroutine ref_cheby_deriv(ncols, dorder, c_in, c_out) bind(C, name='ref_cheby_deriv')
+        Integer(C_INT), Value :: ncols, dorder
+        Real(C_DOUBLE), Intent(In) :: c_in(h_ncheb, ncols)
+        Real(C_DOUBLE), Intent(Out) :: c_out(h_ncheb, ncols)
+        Real*8, Allocatable :: buffer(:,:,:,:)
+        Allocate(buffer(h_ncheb, ncols, 1, 2))
+        buffer(:,:,1,1) = c_in
+        buffer(:,:,1,2) = 0.0d0
+        Call h_grid%d_by_dr_cp(1, 2, buffer, dorder)
+        c_out = buffer(:,:,1,2)
+        DeAllocate(buffer)
+    End Subroutine ref_cheby_deriv
+
+    ! Cheby_Deriv_OutOf_Buffer (d_by_dr_nobuffer): the same recursion into a separate array.
+    Subroutine ref_cheby_deriv_nobuffer(ncols, dorder, c_in, c_out) bind(C, name='ref_cheby_deriv_nobuffer')
+        Integer(C_INT), Value :: ncols, dorder
+        Real(C_DOUBLE), Intent(In) :: c_in(h_ncheb, ncols)
+        Real(C_DOUBLE), Intent(Out) :: c_out(h_ncheb, ncols)
+        Real*8, Allocatable :: buffer(:,:,:,:), obuffer(:,:,:)
+        Allocate(buffer(h_ncheb, ncols, 1, 1), obuffer(h_ncheb, ncols, 1))
+        buffer(:,:,1,1) = c_in
+        Call h_grid%d_by_dr_nobuffer(1, buffer, obuffer, dorder)
+        c_out = obuffer(:,:,1)
+        DeAllocate(buffer, obuffer)
+    End Subroutine ref_cheby_deriv_nobuffer
+
 End Module raysht_ref_harness

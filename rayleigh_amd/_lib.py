@@ -32,6 +32,7 @@ SIGNATURES = {
     "raysht_fft_to_spectral_host": (_I, [_I, _L, _P, _I]),
     "raysht_cheb_to_spectral_host": (_I, [_P, _P, _P, _L]),
     "raysht_cheb_from_spectral_host": (_I, [_P, _P, _P, _L]),
+    "raysht_cheb_deriv_host": (_I, [_P, _P, _P, _L, _I]),
     "raysht_spectral_size": (_S, [_P]),
     "raysht_hybrid_size": (_S, [_P]),
     "raysht_legendre_stp": (_I, [_P, _P, _P, _P]),
@@ -63,6 +64,7 @@ SIGNATURES = {
     "raysht_cheb_to_spectral": (_I, [_P, _P, _P, _L, _P]),
     "raysht_cheb_from_spectral": (_I, [_P, _P, _P, _L, _P]),
     "raysht_cheb_dealias": (_I, [_P, _P, _L, _P]),
+    "raysht_cheb_deriv": (_I, [_P, _P, _P, _L, _I, _P]),
 }
 
 _lib = None

@@ -375,6 +375,68 @@ __global__ void cheb_dealias_kernel(int n, int first, long ncols, double *__rest
     }
 }
 
+// Cheby_Deriv_Buffer_4D / Cheby_Deriv_OutOf_Buffer (Chebyshev_Polynomials.F90:560-730):
+// per column the backward recursion b(i) = b(i+2) + 2(i+1) a(i+1) from b(n) = b(n+1) = 0,
+// applied dorder times (each time to the previous result), then every entry times
+// deriv_scaling**dorder.  The orders are chained in one descending pass: order o at i
+// needs order o-1 at i+1, which the pass produced one step earlier, so a thread carries
+// the two latest values of every order in registers and reads each input once.
+// A workgroup owns 256 columns and streams them from the top coefficient down in
+// chunks of 16: a chunk comes in as coalesced 128-byte column pieces into LDS (rows
+// padded to 17 doubles), one thread per column runs the recursion over it in place,
+// and the chunk leaves the same way.  Every step rounds the product and the sum
+// separately, as the reference's compiled recursion does.
+constexpr int kDerivCols = 256, kDerivChunk = 16, kDerivMaxOrder = 4;
+
+__global__ __launch_bounds__(kDerivCols) void cheb_deriv_kernel(int n, long ncols, int dorder, double scale,
+                                                                const double *__restrict__ c, double *__restrict__ d)
+{
+#pragma clang fp contract(off)
+    __shared__ double tile[kDerivCols][kDerivChunk + 1];
+    const int tid = threadIdx.x;
+    const long col0 = (long)blockIdx.x * kDerivCols;
+    const int cols = (int)min((long)kDerivCols, ncols - col0);
+    double x1[kDerivMaxOrder], x2[kDerivMaxOrder];  // order o + 1 at i + 1 and at i + 2
+#pragma unroll
+    for (int o = 0; o < kDerivMaxOrder; ++o) x1[o] = x2[o] = 0.0;
+    double a1 = 0.0;  // the input at i + 1
+    for (int i0 = ((n - 1) / kDerivChunk) * kDerivChunk; i0 >= 0; i0 -= kDerivChunk) {
+        const int w = min(kDerivChunk, n - i0);
+#pragma unroll 4
+        for (int q = 0; q < kDerivChunk; ++q) {
+            const int v = q * kDerivCols + tid, cl = v / kDerivChunk, e = v % kDerivChunk;
+            if (cl < cols && e < w) tile[cl][e] = c[(col0 + cl) * n + i0 + e];
+        }
+        __syncthreads();
+        if (tid < cols) {
+            for (int e = w - 1; e >= 0; --e) {
+                const double a = tile[tid][e];
+                const double k2 = 2.0 * (i0 + e + 1);
+                double prev = a1, out = 0.0;
+#pragma unroll
+                for (int o = 0; o < kDerivMaxOrder; ++o) {
+                    if (o < dorder) {
+                        const double nw = x2[o] + k2 * prev;  // not contracted (pragma above)
+                        prev = x1[o];
+                        x2[o] = x1[o];
+                        x1[o] = nw;
+                        out = nw;
+                    }
+                }
+                a1 = a;
+                tile[tid][e] = out * scale;
+            }
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int q = 0; q < kDerivChunk; ++q) {
+            const int v = q * kDerivCols + tid, cl = v / kDerivChunk, e = v % kDerivChunk;
+            if (cl < cols && e < w) d[(col0 + cl) * n + i0 + e] = tile[cl][e];
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 }  // namespace raysht
 
@@ -446,11 +508,14 @@ int raysht_cheb_create(raysht_cheb **out, int n, double rmin, double rmax, doubl
         x[i] = std::cos(arg);
         arg = arg + dtheta;
     }
-    if (grid) {
-        // Initialize_Cheby_Grid (:161-173): affine map of [x_n, x_1] onto [rmin, rmax]
+    {
+        // Initialize_Cheby_Grid (:161-191): affine map of [x_n, x_1] onto [rmin, rmax]; the
+        // derivative scaling is the reciprocal of the map's slope, rounded as there
         const double xmin = x[n - 1], xmax = x[0];
         const double scaling = (rmax - rmin) / (xmax - xmin);
-        for (int i = 0; i < n; ++i) grid[i] = (x[i] - xmin) * scaling + rmin;
+        if (grid)
+            for (int i = 0; i < n; ++i) grid[i] = (x[i] - xmin) * scaling + rmin;
+        cb->deriv_scaling = 1.0 / scaling;
     }
     // Gen_Tn (:251-297): cheby_even(i, k) = T_{2k-2}(x_i), cheby_odd(i, k) = T_{2k-1}(x_i), i <= n_x
     cb->ceven.assign((size_t)cb->n_x * cb->n_even, 0.0);
@@ -552,6 +617,26 @@ int raysht_cheb_from_spectral(raysht_cheb *cb, const double *c, double *f, long 
     hipLaunchKernelGGL(cheb_from_spectral_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds,
                        reinterpret_cast<hipStream_t>(stream), cb->n, cb->n_x, cb->n_even, cb->n_odd, cb->k4, cb->ip,
                        ncols, cb->d_t_e, cb->d_t_o, c, f);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+int raysht_cheb_deriv(raysht_cheb *cb, const double *c, double *d, long ncols, int dorder, void *stream)
+{
+    if (!cb || ncols < 0 || dorder < 1) return RAYSHT_EINVAL;
+    if (dorder > kDerivMaxOrder) return RAYSHT_EUNSUPPORTED;
+    if (ncols == 0) return RAYSHT_OK;
+    if (!c || !d) return RAYSHT_EINVAL;
+    // out of place only: the reference's in-place branch (ind == dind) is unimplemented
+    const long n_el = ncols * cb->n;
+    if (c < d + n_el && d < c + n_el) return RAYSHT_EINVAL;
+    const long blocks = (ncols + kDerivCols - 1) / kDerivCols;
+    if (blocks > 0x7fffffffL) return RAYSHT_EUNSUPPORTED;
+    // deriv_scaling**dorder as the compiled Fortran power forms it for small orders
+    const double s = cb->deriv_scaling;
+    const double scale = dorder == 1 ? s : dorder == 2 ? s * s : dorder == 3 ? (s * s) * s : (s * s) * (s * s);
+    hipLaunchKernelGGL(cheb_deriv_kernel, dim3((unsigned)blocks), dim3(kDerivCols), 0,
+                       reinterpret_cast<hipStream_t>(stream), cb->n, ncols, dorder, scale, c, d);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }

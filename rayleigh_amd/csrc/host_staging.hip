@@ -138,4 +138,18 @@ int raysht_cheb_from_spectral_host(raysht_cheb *cb, const double *c, double *f, 
     return RAYSHT_OK;
 }
 
+int raysht_cheb_deriv_host(raysht_cheb *cb, const double *c, double *d, long ncols, int dorder)
+{
+    if (!cb || ncols < 0 || (ncols > 0 && (!c || !d))) return RAYSHT_EINVAL;
+    if (ncols == 0) return RAYSHT_OK;
+    const size_t bytes = (size_t)ncols * cb->n * sizeof(double);
+    scoped_device_buffer din, dout;
+    if (hipMalloc(&din.ptr, bytes) != hipSuccess || hipMalloc(&dout.ptr, bytes) != hipSuccess) return RAYSHT_ENOMEM;
+    RAYSHT_HIP_TRY(hipMemcpy(din.ptr, c, bytes, hipMemcpyHostToDevice));
+    int rc = raysht_cheb_deriv(cb, din.ptr, dout.ptr, ncols, dorder, nullptr);
+    if (rc != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipMemcpy(d, dout.ptr, bytes, hipMemcpyDeviceToHost));
+    return RAYSHT_OK;
+}
+
 }  // extern "C"

@@ -106,6 +106,7 @@ struct raysht_cheb {
     // (ix4 x kp) and [k][i] for the synthesis (k4 x ip)
     int ix4 = 0, kp = 0, k4 = 0, ip = 0;
     double *d_tt_e = nullptr, *d_tt_o = nullptr, *d_t_e = nullptr, *d_t_o = nullptr;
+    double deriv_scaling = 0.0;  // d/dr of the domain map, as Initialize_Cheby_Grid rounds it
 };
 
 namespace raysht {

@@ -14,6 +14,7 @@
 !                                                     Fourier_Transform.F90:25-35
 !   Cheby_To_Spectral(f_in, c_out) / Cheby_From_Spectral(c_in, f_out) after Cheby_Init(n, rmin, rmax, grid)
 !                                                     Chebyshev_Polynomials.F90:65-558 (one domain)
+!   Cheby_d_by_dr_cp(ind, dind, buffer, dorder)      Chebyshev_Polynomials.F90:560-640
 !
 ! The data stay in the caller's host arrays: each call hands their addresses to
 ! the library's host-array entry points, which stage through device memory.
@@ -30,7 +31,7 @@ Module RaySHT_Math_Layer
 
     Public :: Initialize_Legendre, Finalize_Legendre, Legendre_Transform, Set_Legendre_Precision
     Public :: FFT_To_Physical, FFT_To_Spectral, FFT_To_Spectral_RSC
-    Public :: Cheby_Init, Cheby_To_Spectral, Cheby_From_Spectral, Cheby_Finalize
+    Public :: Cheby_Init, Cheby_To_Spectral, Cheby_From_Spectral, Cheby_Finalize, Cheby_d_by_dr_cp
     Public :: coloc, gl_weights
     ! arithmetic of the Legendre contractions (enum raysht_precision of include/raysht.h)
     Integer, Parameter, Public :: raysht_fp64 = 0, raysht_bf16x3 = 1, raysht_bf16x6 = 2, &
@@ -122,6 +123,13 @@ Module RaySHT_Math_Layer
             Import :: C_PTR, C_INT, C_LONG
             Type(C_PTR), Value :: cb, c, f
             Integer(C_LONG), Value :: ncols
+            Integer(C_INT) :: st
+        End Function
+        Function c_cheb_deriv(cb, c, d, ncols, dorder) Bind(C, name='raysht_cheb_deriv_host') Result(st)
+            Import :: C_PTR, C_INT, C_LONG
+            Type(C_PTR), Value :: cb, c, d
+            Integer(C_LONG), Value :: ncols
+            Integer(C_INT), Value :: dorder
             Integer(C_INT) :: st
         End Function
         Function c_strerror(status) Bind(C, name='raysht_strerror') Result(msg)
@@ -390,5 +398,25 @@ Contains
         Endif
         Call check(c_cheb_from_spec(cheb, c_loc(c_in), c_loc(f_out), ncols), 'Cheby_From_Spectral')
     End Subroutine Cheby_From_Spectral
+
+    ! Cheby_Deriv_Buffer_4D (d_by_dr_cp, Chebyshev_Polynomials.F90:560-640): buffer(:,:,:,dind)
+    ! receives the dorder-th radial derivative's coefficients of buffer(:,:,:,ind); ind /= dind
+    ! (the reference's in-place branch is unimplemented there).
+    Subroutine Cheby_d_by_dr_cp(ind, dind, buffer, dorder)
+        Integer, Intent(In) :: ind, dind, dorder
+        Real*8, Intent(InOut), Target, Contiguous :: buffer(:,:,:,:)
+        Integer(C_LONG) :: ncols
+        ncols = int(size(buffer, 2), C_LONG) * size(buffer, 3)
+        If (size(buffer, 1) /= cheb_n) Then
+            Write(6, *) 'RaySHT: Cheby_d_by_dr_cp radial size does not match Cheby_Init'
+            Error Stop 1
+        Endif
+        If (ind == dind) Then
+            Write(6, *) 'RaySHT: Cheby_d_by_dr_cp needs ind /= dind'
+            Error Stop 1
+        Endif
+        Call check(c_cheb_deriv(cheb, c_loc(buffer(1, 1, 1, ind)), c_loc(buffer(1, 1, 1, dind)), ncols, &
+                                int(dorder, C_INT)), 'Cheby_d_by_dr_cp')
+    End Subroutine Cheby_d_by_dr_cp
 
 End Module RaySHT_Math_Layer

@@ -337,3 +337,20 @@ class ChebyGrid:
             raise ValueError("buffer must hold whole columns of n")
         _lib.check(_lib.lib().raysht_cheb_dealias(self._h, _dptr(c, "c"), c.numel() // self.n, _stream(stream)),
                    "raysht_cheb_dealias")
+
+    def d_by_dr(self, c_in, c_out, dorder=1, stream=None):
+        """Cheby_Deriv_OutOf_Buffer (d_by_dr_nobuffer, Chebyshev_Polynomials.F90:642-730):
+        c_out = coefficients of the dorder-th radial derivative of every column of c_in."""
+        if c_in.numel() % self.n or c_in.numel() != c_out.numel():
+            raise ValueError("buffers must hold whole columns of n and match")
+        _lib.check(_lib.lib().raysht_cheb_deriv(self._h, _dptr(c_in, "c_in"), _dptr(c_out, "c_out"),
+                                                c_in.numel() // self.n, int(dorder), _stream(stream)),
+                   "raysht_cheb_deriv")
+
+    def d_by_dr_cp(self, ind, dind, buffer, dorder=1, stream=None):
+        """Cheby_Deriv_Buffer_4D (d_by_dr_cp, :560-640): buffer is the C-order view of the
+        reference's buffer(0:n-1, :, :, 1:nvar), i.e. slowest index = variable; variable
+        dind (1-based) receives the dorder-th derivative of variable ind."""
+        if not (1 <= ind <= buffer.shape[0] and 1 <= dind <= buffer.shape[0]) or ind == dind:
+            raise ValueError("ind and dind must name two different variables of the buffer")
+        self.d_by_dr(buffer[ind - 1], buffer[dind - 1], dorder, stream)

@@ -27,6 +27,7 @@ Program test_sht_raysht
     Call lt_fft_loop(nfail)
     Call two_d_pair(nfail)
     Call cheby_round_trip(nfail)
+    Call cheby_derivatives(nfail)
     ! the split forms with 6 digits stay within the same tolerance over ten round trips
     Call Set_Legendre_Precision(raysht_int8x6)
     Call white_noise(nfail)
@@ -286,5 +287,31 @@ Contains
         Call Cheby_Finalize()
         Call report('Chebyshev round trip', d, nfail)
     End Subroutine cheby_round_trip
+
+    ! Cheby_Deriv_Buffer_4D (d_by_dr_cp, Chebyshev_Polynomials.F90:560-640) as Test_Cheby's
+    ! derivative check uses it: variables 2..4 of the buffer receive the first three
+    ! derivatives of variable 1 = r^3 on [0.5, 2] (exact: 3 r^2, 6 r, 6)
+    Subroutine cheby_derivatives(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: n = 8
+        Real*8 :: grid(n), f(n, 2, 3, 4), c(n, 2, 3, 4), ans(n, 2, 3, 4), d
+        Integer :: i
+        Call Cheby_Init(n, 0.5d0, 2.0d0, grid)
+        f = 0.0d0
+        Do i = 1, n
+            f(i, :, :, 1) = grid(i)**3
+            ans(i, :, :, 2) = 3.0d0 * grid(i)**2
+            ans(i, :, :, 3) = 6.0d0 * grid(i)
+            ans(i, :, :, 4) = 6.0d0
+        Enddo
+        Call Cheby_To_Spectral(f, c)
+        Call Cheby_d_by_dr_cp(1, 2, c, 1)
+        Call Cheby_d_by_dr_cp(1, 3, c, 2)
+        Call Cheby_d_by_dr_cp(1, 4, c, 3)
+        Call Cheby_From_Spectral(c, f)
+        d = maxval(abs(f(:, :, :, 2:4) - ans(:, :, :, 2:4)))
+        Call Cheby_Finalize()
+        Call report('Chebyshev derivatives', d, nfail)
+    End Subroutine cheby_derivatives
 
 End Program test_sht_raysht

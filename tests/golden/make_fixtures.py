@@ -60,6 +60,22 @@ def cheby_case(name, n, ncols, seed):
     print("wrote", name)
 
 
+def cheby_deriv_case(name, n, ncols, seed, rmin, rmax):
+    """d_by_dr_cp (Cheby_Deriv_Buffer_4D) of orders 1-3 on seeded coefficients, and on the
+    coefficients of exp(-r) (the reference's Test_Cheby case: derivatives -f, f, -f)."""
+    R = O.Reference()
+    g, w = R.cheby_init(n, rmin, rmax)
+    rng = np.random.default_rng(seed)
+    c = rng.standard_normal(n * ncols)
+    ce = R.cheby_to_spectral(np.exp(-g))
+    rec = dict(n=n, ncols=ncols, seed=seed, rmin=rmin, rmax=rmax, grid=g, c_in=c, c_exp=ce)
+    for d in (1, 2, 3):
+        rec[f"d{d}"] = R.cheby_deriv(c, d)
+        rec[f"exp_d{d}"] = R.cheby_deriv(ce, d)
+    np.savez_compressed(os.path.join(OUT, name), **rec)
+    print("wrote", name)
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("compile the reference checker first: make -C oracle ref")
@@ -74,3 +90,6 @@ if __name__ == "__main__":
                   seed=511, sample_tables=6000)
     cheby_case("cheby_n48.npz", 48, 6, seed=48)
     cheby_case("cheby_n33.npz", 33, 4, seed=33)
+    cheby_deriv_case("cheby_deriv_n48.npz", 48, 6, seed=480, rmin=0.35, rmax=1.0)
+    cheby_deriv_case("cheby_deriv_n33.npz", 33, 5, seed=330, rmin=0.5, rmax=1.7)
+    cheby_deriv_case("cheby_deriv_n128.npz", 128, 3, seed=1280, rmin=0.35 / 0.65, rmax=1.0 / 0.65)
