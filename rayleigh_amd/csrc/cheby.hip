@@ -382,9 +382,9 @@ __global__ void cheb_dealias_kernel(int n, int first, long ncols, double *__rest
 // needs order o-1 at i+1, which the pass produced one step earlier, so a thread carries
 // the two latest values of every order in registers and reads each input once.
 // A workgroup owns 256 columns and streams them from the top coefficient down in
-// chunks of 16: a chunk comes in as coalesced 128-byte column pieces into LDS (rows
-// padded to 17 doubles), one thread per column runs the recursion over it in place,
-// and the chunk leaves the same way.  Every step rounds the product and the sum
+// chunks of 16: a chunk comes in as coalesced 128-byte column pieces (loaded into
+// registers one chunk ahead) and goes to LDS (rows padded to 17 doubles), one thread
+// per column runs the recursion over it in place, and the chunk leaves the same way.  Every step rounds the product and the sum
 // separately, as the reference's compiled recursion does.
 constexpr int kDerivCols = 256, kDerivChunk = 16, kDerivMaxOrder = 4;
 
@@ -400,14 +400,26 @@ __global__ __launch_bounds__(kDerivCols) void cheb_deriv_kernel(int n, long ncol
 #pragma unroll
     for (int o = 0; o < kDerivMaxOrder; ++o) x1[o] = x2[o] = 0.0;
     double a1 = 0.0;  // the input at i + 1
-    for (int i0 = ((n - 1) / kDerivChunk) * kDerivChunk; i0 >= 0; i0 -= kDerivChunk) {
+    // chunk values in flight: loaded one chunk ahead, while the current one is worked on
+    double pf[kDerivChunk];
+    auto fetch = [&](int i0) {
         const int w = min(kDerivChunk, n - i0);
-#pragma unroll 4
+#pragma unroll
         for (int q = 0; q < kDerivChunk; ++q) {
             const int v = q * kDerivCols + tid, cl = v / kDerivChunk, e = v % kDerivChunk;
-            if (cl < cols && e < w) tile[cl][e] = c[(col0 + cl) * n + i0 + e];
+            pf[q] = cl < cols && e < w ? c[(col0 + cl) * n + i0 + e] : 0.0;
+        }
+    };
+    fetch(((n - 1) / kDerivChunk) * kDerivChunk);
+    for (int i0 = ((n - 1) / kDerivChunk) * kDerivChunk; i0 >= 0; i0 -= kDerivChunk) {
+        const int w = min(kDerivChunk, n - i0);
+#pragma unroll
+        for (int q = 0; q < kDerivChunk; ++q) {
+            const int v = q * kDerivCols + tid, cl = v / kDerivChunk, e = v % kDerivChunk;
+            tile[cl][e] = pf[q];
         }
         __syncthreads();
+        if (i0 > 0) fetch(i0 - kDerivChunk);
         if (tid < cols) {
             for (int e = w - 1; e >= 0; --e) {
                 const double a = tile[tid][e];
