@@ -265,6 +265,25 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
             rec["gbps"] = nbytes / (ms * 1e-3) / 1e9
             rec["frac_hbm_peak"] = rec["gbps"] / 8000.0
         out["stages"][name] = rec
+    # the one-rank radial transposes around the implicit solve (Transpose_2b1b after PtS,
+    # Transpose_1a2a before StP): 16 B per spectral coefficient (read + write)
+    p1 = torch.empty_like(s)
+    nb = 16.0 * plan.spectral_size
+    out["radial_transposes"] = {}
+    for name, fn in (("transpose_2b1b", lambda: plan.transpose_2b1b(s2, p1)),
+                     ("transpose_1a2a", lambda: plan.transpose_1a2a(p1, s2))):
+        for _ in range(max(warmup, 1)):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out["radial_transposes"][name] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9,
+                                          "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
+    del p1
     # the radial derivative recursion (d_by_dr_cp) of orders 1-3 on the spectral radial
     # buffer, as Post_Solve applies it: 16 B per coefficient (read + write)
     rad3 = torch.empty_like(rad)

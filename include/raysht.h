@@ -119,6 +119,19 @@ int raysht_permute_2a3a(raysht_plan *plan, int nphi, const double *p2, double *p
 /* SphericalBuffer%reform p3b -> p2b = Transpose_3b2b (Spherical_Buffer.F90:461-703)
  * on one rank (the kill-signal cargo is not carried). */
 int raysht_permute_3b2b(raysht_plan *plan, int nphi, const double *p3, double *p2, void *stream);
+/* Transpose_2b1b (Spherical_Buffer.F90:705-843) on one radial rank: p1b(r, imi, lm, f)
+ * ([f][lm][imi][r], r fastest; spectral_size doubles) = s2b(mp)%data(l, r, imi, f) for
+ * the lm-th (l, m) pair of LM_Load_Balance (Load_Balance.F90:311-395): l slowest from the
+ * smallest m of the plan, the plan's m in m_values order within one l.  Device arrays,
+ * out of place; plans made by raysht_plan_create_cols: RAYSHT_EUNSUPPORTED. */
+int raysht_permute_2b1b(raysht_plan *plan, const double *s2b, double *p1b, void *stream);
+/* Transpose_1a2a (:845-932) on one radial rank: the inverse, s2a(mp)%data(l, r, imi, f) =
+ * p1a(r, imi, lm, f). */
+int raysht_permute_1a2a(raysht_plan *plan, const double *p1a, double *s2a, void *stream);
+/* The pair list the radial transposes follow: its length (lm_count), and l_lm_values /
+ * mp_lm_values (mp 0-based) into host arrays of lm_count ints. */
+int raysht_lm_count(const raysht_plan *plan);
+int raysht_lm_values(const raysht_plan *plan, int *l_out, int *mp_out);
 
 /* Fused forms of the same steps (results identical to the separate calls):
  * StP then 2a->3a then c2r, and r2c then 3b->2b then PtS.  `work` is a hybrid

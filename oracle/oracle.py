@@ -66,6 +66,10 @@ def oracle_lib():
         lib.orc_fft_r2c.argtypes = [_I, _L, _P, _I]
         lib.orc_permute_2a3a.argtypes = [_I, _I, _P, _I, _I, _I, _I, _P, _P]
         lib.orc_permute_3b2b.argtypes = [_I, _I, _P, _I, _I, _I, _I, _P, _P]
+        lib.orc_lm_values.restype = _I
+        lib.orc_lm_values.argtypes = [_I, _P, _I, _P, _P]
+        lib.orc_permute_2b1b.argtypes = [_I, _P, _I, _I, _I, _P, _P]
+        lib.orc_permute_1a2a.argtypes = [_I, _P, _I, _I, _I, _P, _P]
         lib.orc_cheby_grid.argtypes = [_I, _D, _D, _P]
         lib.orc_cheby_tables.argtypes = [_I, _P, _P]
         lib.orc_cheby_to_spectral.argtypes = [_I, _L, _P, _P, _P, _P]
@@ -238,6 +242,34 @@ def permute_3b2b(nt, lmax, mvals, nr, nf, nphi, p3b):
     out = np.zeros(nt * 2 * nr * nf * len(mv))
     oracle_lib().orc_permute_3b2b(nt, lmax, _ptr(mv), len(mv), nr, nf, nphi,
                                   _ptr(np.ascontiguousarray(p3b)), _ptr(out))
+    return out
+
+
+def lm_values(lmax, mvals):
+    """LM_Load_Balance's (l, mp) pair list on one radial rank (Load_Balance.F90:311-395)."""
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    n = oracle_lib().orc_lm_values(lmax, _ptr(mv), len(mv), None, None)
+    ls, mps = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+    oracle_lib().orc_lm_values(lmax, _ptr(mv), len(mv), _ptr(ls), _ptr(mps))
+    return ls[:n], mps[:n]
+
+
+def permute_2b1b(lmax, mvals, nr, nf, s2b):
+    """Transpose_2b1b on one rank (Spherical_Buffer.F90:705-843): p1b[f][lm][imi][r]."""
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    n = len(lm_values(lmax, mv)[0])
+    s2b = np.ascontiguousarray(s2b, dtype=np.float64)
+    out = np.zeros(nf * n * 2 * nr)
+    oracle_lib().orc_permute_2b1b(lmax, _ptr(mv), len(mv), nr, nf, _ptr(s2b), _ptr(out))
+    return out
+
+
+def permute_1a2a(lmax, mvals, nr, nf, p1a):
+    """Transpose_1a2a on one rank (Spherical_Buffer.F90:845-932): the packed spectral layout."""
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    p1a = np.ascontiguousarray(p1a, dtype=np.float64)
+    out = np.zeros(spectral_size(lmax, mv, nr, nf))
+    oracle_lib().orc_permute_1a2a(lmax, _ptr(mv), len(mv), nr, nf, _ptr(p1a), _ptr(out))
     return out
 
 

@@ -436,6 +436,72 @@ void orc_permute_3b2b(int nt, int lmax, const int *mvals, int nm, int nr, int nf
     }
 }
 
+/* Load_Balance.F90:311-395 LM_Load_Balance on one radial rank: the (l, m) pairs of the
+ * m values owned, l slowest (from the smallest m to lmax), m in m_values order within an
+ * l; l_out / mp_out (0-based mp) receive the list, the return value is its length. */
+int orc_lm_values(int lmax, const int *mvals, int nm, int *l_out, int *mp_out)
+{
+    int lmin = lmax;
+    for (int mp = 0; mp < nm; ++mp)
+        if (mvals[mp] < lmin) lmin = mvals[mp];
+    int k = 0;
+    for (int l = lmin; l <= lmax; ++l)
+        for (int mp = 0; mp < nm; ++mp)
+            if (mvals[mp] <= l) {
+                if (l_out) l_out[k] = l;
+                if (mp_out) mp_out[k] = mp;
+                ++k;
+            }
+    return k;
+}
+
+static long spectral_block_offset(int lmax, const int *mvals, int mp, int nr, int nf)
+{
+    long off = 0;
+    for (int i = 0; i < mp; ++i) off += (long)(lmax - mvals[i] + 1) * nr * 2 * nf;
+    return off;
+}
+
+/* Spherical_Buffer.F90:705-843 Transpose_2b1b on one rank (all radii, nfields = nf):
+ * p1b(r, imi, lm, f) = s2b(mp)%data(l, r, imi, f) for the lm-th pair (l, mp) of
+ * LM_Load_Balance's list.  p1b is [f][lm][imi][r] (r fastest), s2b the packed
+ * spectral layout. */
+void orc_permute_2b1b(int lmax, const int *mvals, int nm, int nr, int nf, const double *s2b, double *p1b)
+{
+    const int n = orc_lm_values(lmax, mvals, nm, NULL, NULL);
+    int *ls = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1)), *mps = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    orc_lm_values(lmax, mvals, nm, ls, mps);
+    for (int k = 0; k < n; ++k) {
+        const int mp = mps[k], m = mvals[mp], nl = lmax - m + 1;
+        const double *blk = s2b + spectral_block_offset(lmax, mvals, mp, nr, nf);
+        for (int f = 0; f < nf; ++f)
+            for (int imi = 0; imi < 2; ++imi)
+                for (int r = 0; r < nr; ++r)
+                    p1b[(((long)f * n + k) * 2 + imi) * nr + r] = blk[(((long)f * 2 + imi) * nr + r) * nl + (ls[k] - m)];
+    }
+    free(ls);
+    free(mps);
+}
+
+/* Spherical_Buffer.F90:845-932 Transpose_1a2a on one rank: the inverse,
+ * s2a(mp)%data(l, r, imi, f) = p1a(r, imi, lm, f). */
+void orc_permute_1a2a(int lmax, const int *mvals, int nm, int nr, int nf, const double *p1a, double *s2a)
+{
+    const int n = orc_lm_values(lmax, mvals, nm, NULL, NULL);
+    int *ls = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1)), *mps = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    orc_lm_values(lmax, mvals, nm, ls, mps);
+    for (int k = 0; k < n; ++k) {
+        const int mp = mps[k], m = mvals[mp], nl = lmax - m + 1;
+        double *blk = s2a + spectral_block_offset(lmax, mvals, mp, nr, nf);
+        for (int f = 0; f < nf; ++f)
+            for (int imi = 0; imi < 2; ++imi)
+                for (int r = 0; r < nr; ++r)
+                    blk[(((long)f * 2 + imi) * nr + r) * nl + (ls[k] - m)] = p1a[(((long)f * n + k) * 2 + imi) * nr + r];
+    }
+    free(ls);
+    free(mps);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Chebyshev_Polynomials.F90:210-248 Gen_Colocation_Points (zeros branch: the
  * angle accumulates by repeated addition) and :65-207 Initialize_Cheby_Grid's

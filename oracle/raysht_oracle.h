@@ -60,6 +60,9 @@ void orc_fft_r2c(int nphi, long howmany, double *x, int rescale);
 /* Row transposes on one rank. */
 void orc_permute_2a3a(int nt, int lmax, const int *mvals, int nm, int nr, int nf, int nphi,
                       const double *p2a, double *p3a);
+int orc_lm_values(int lmax, const int *mvals, int nm, int *l_out, int *mp_out);
+void orc_permute_2b1b(int lmax, const int *mvals, int nm, int nr, int nf, const double *s2b, double *p1b);
+void orc_permute_1a2a(int lmax, const int *mvals, int nm, int nr, int nf, const double *p1a, double *s2a);
 void orc_permute_3b2b(int nt, int lmax, const int *mvals, int nm, int nr, int nf, int nphi,
                       const double *p3b, double *p2b);
 

@@ -65,6 +65,10 @@ SIGNATURES = {
     "raysht_cheb_from_spectral": (_I, [_P, _P, _P, _L, _P]),
     "raysht_cheb_dealias": (_I, [_P, _P, _L, _P]),
     "raysht_cheb_deriv": (_I, [_P, _P, _P, _L, _I, _P]),
+    "raysht_permute_2b1b": (_I, [_P, _P, _P, _P]),
+    "raysht_permute_1a2a": (_I, [_P, _P, _P, _P]),
+    "raysht_lm_count": (_I, [_P]),
+    "raysht_lm_values": (_I, [_P, _P, _P]),
 }
 
 _lib = None

@@ -96,6 +96,12 @@ struct raysht_plan {
     raysht::raysht_split split;  // used by the BF16X3 / BF16X6 forms only
     raysht::prob_desc desc{};           // this plan as one problem of a grouped launch
     raysht::prob_desc *d_desc = nullptr;
+    // radial transposes: LM_Load_Balance's (l, mp) pairs, the pair index of every
+    // (m-block, l) at lm_base[mp] + l - m, and the (mp, first l - m) tiles
+    int lm_count = 0, n_rtiles = 0;
+    std::vector<int> lm_l, lm_mp;
+    int *d_lm_base = nullptr, *d_lm_slot = nullptr;
+    int2 *d_rtiles = nullptr;
 };
 
 struct raysht_cheb {
@@ -117,6 +123,10 @@ void gauss_legendre_dd(int n, double *x_hi, double *x_lo, double *w_hi, double *
                        double *renorm_hi, double *renorm_lo);
 void plm_amplitudes_dd(int lmax, double *amp_hi, double *amp_lo);
 void fft_twiddles(int n, double *cos_out, double *sin_out);
+
+// Radial transpose tables of a plan (radial_transpose.hip).
+int radial_plan_init(raysht_plan *p);
+void radial_plan_free(raysht_plan *p);
 
 // Device launches (each returns RAYSHT_OK or an error code).
 int launch_plm_tables(raysht_plan *p, const double *d_x, const double *d_renorm, const double *d_amp,

@@ -168,6 +168,7 @@ void raysht_plan_destroy(raysht_plan *p)
     device_free(p->d_stage_p);
     device_free(p->d_desc);
     split_plan_free(p);
+    radial_plan_free(p);
     delete p;
 }
 
@@ -301,6 +302,7 @@ int plan_create(raysht_plan **out, int lmax, int ntheta, const int *m_values, in
     device_free(d_rn);
     device_free(d_amp);
     if (rc == RAYSHT_OK && precision != RAYSHT_FP64) rc = split_plan_init(p);
+    if (rc == RAYSHT_OK) rc = radial_plan_init(p);
     if (rc == RAYSHT_OK) {
         p->desc = describe(p);
         rc = device_upload(&p->d_desc, &p->desc, 1);

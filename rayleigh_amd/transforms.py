@@ -113,6 +113,27 @@ class LegendrePlan:
                                                   _dptr(p2b, "p2b", self.hybrid_size), _stream(stream)),
                    "raysht_permute_3b2b")
 
+    def transpose_2b1b(self, s2b, p1b, stream=None):
+        """Transpose_2b1b on one rank: the spectral layout to p1b[f][lm][imi][r], the
+        (l, m) pairs in lm_values() order."""
+        _lib.check(_lib.lib().raysht_permute_2b1b(self._h, _dptr(s2b, "s2b", self.spectral_size),
+                                                  _dptr(p1b, "p1b", self.spectral_size), _stream(stream)),
+                   "raysht_permute_2b1b")
+
+    def transpose_1a2a(self, p1a, s2a, stream=None):
+        """Transpose_1a2a on one rank: p1a[f][lm][imi][r] back to the spectral layout."""
+        _lib.check(_lib.lib().raysht_permute_1a2a(self._h, _dptr(p1a, "p1a", self.spectral_size),
+                                                  _dptr(s2a, "s2a", self.spectral_size), _stream(stream)),
+                   "raysht_permute_1a2a")
+
+    def lm_values(self):
+        """LM_Load_Balance's (l, mp) pairs (l_lm_values, mp_lm_values with mp 0-based)."""
+        n = _lib.lib().raysht_lm_count(self._h)
+        ls, mps = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+        _lib.check(_lib.lib().raysht_lm_values(self._h, ls.ctypes.data_as(ctypes.c_void_p),
+                                               mps.ctypes.data_as(ctypes.c_void_p)), "raysht_lm_values")
+        return ls[:n], mps[:n]
+
     def spectral_to_physical(self, s, work, p3, n_phi: int, stream=None):
         n = (n_phi + 2) * self.n_r * self.n_theta * self.n_fields
         _lib.check(_lib.lib().raysht_spectral_to_physical(self._h, int(n_phi), _dptr(s, "s", self.spectral_size),
