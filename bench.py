@@ -284,6 +284,26 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
         out["radial_transposes"][name] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9,
                                           "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
     del p1
+    # the angular derivatives (d_by_dtheta, d_by_sdtheta, d_by_dphi) of one field of the
+    # spectral data into a one-field array: 16 B per coefficient of the field
+    per_field = plan.spectral_size // nf
+    f1 = torch.empty(per_field, dtype=torch.float64, device="cuda")
+    nb = 16.0 * per_field
+    out["angular_derivatives"] = {}
+    for kind in ("dtheta", "sdtheta", "dphi"):
+        fn = lambda: plan.angular_derivative(kind, s, 0, f1, 0)
+        for _ in range(max(warmup, 1)):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out["angular_derivatives"][kind] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9,
+                                            "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
+    del f1
     # the radial derivative recursion (d_by_dr_cp) of orders 1-3 on the spectral radial
     # buffer, as Post_Solve applies it: 16 B per coefficient (read + write)
     rad3 = torch.empty_like(rad)

@@ -133,6 +133,21 @@ int raysht_permute_1a2a(raysht_plan *plan, const double *p1a, double *s2a, void 
 int raysht_lm_count(const raysht_plan *plan);
 int raysht_lm_values(const raysht_plan *plan, int *l_out, int *mp_out);
 
+/* Spectral_Derivatives.F90 on the spectral layout of a plan made by raysht_plan_create:
+ * `in` and `out` are packed spectral arrays of nf_in / nf_out fields (field f of block mp =
+ * data(m:l_max, 1:N_r, 1:2, f)); field fin of `in` goes to field fout of `out` (0-based).
+ * kind 0: d_by_dtheta = sin(theta) d/dtheta (d_dtheta_buffer4D :224-249, d_dtheta_4dbuff2arr
+ * :329-350); kind 1: d_by_sdtheta = 1/sin(theta) d(sin^2(theta) .)/dtheta (:277-298,
+ * :381-401), with Initialize_Angular_Derivatives' coefficients (:49-83).  The same field of
+ * the same array (in place) is RAYSHT_EINVAL for these; plans over anonymous columns are
+ * RAYSHT_EUNSUPPORTED. */
+int raysht_d_by_dtheta(raysht_plan *plan, int kind, const double *in, int nf_in, int fin, double *out, int nf_out,
+                       int fout, void *stream);
+/* d_by_dphi (d_by_dphi_rlmbuff4d :573-603, d_by_dphi_4dbuff2arr :495-509): i m times field fin,
+ * (re, im) -> (-m im, m re); in place (same array, fin = fout) allowed. */
+int raysht_d_by_dphi(raysht_plan *plan, const double *in, int nf_in, int fin, double *out, int nf_out, int fout,
+                     void *stream);
+
 /* Fused forms of the same steps (results identical to the separate calls):
  * StP then 2a->3a then c2r, and r2c then 3b->2b then PtS.  `work` is a hybrid
  * buffer the call may overwrite. */

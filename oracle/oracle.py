@@ -66,6 +66,9 @@ def oracle_lib():
         lib.orc_fft_r2c.argtypes = [_I, _L, _P, _I]
         lib.orc_permute_2a3a.argtypes = [_I, _I, _P, _I, _I, _I, _I, _P, _P]
         lib.orc_permute_3b2b.argtypes = [_I, _I, _P, _I, _I, _I, _I, _P, _P]
+        lib.orc_angular_coefs.argtypes = [_I, _I, _I, _P]
+        lib.orc_angular_dtheta.argtypes = [_I, _I, _P, _I, _I, _P, _I, _I, _P, _I, _I]
+        lib.orc_angular_dphi.argtypes = [_I, _P, _I, _I, _P, _I, _I, _P, _I, _I]
         lib.orc_lm_values.restype = _I
         lib.orc_lm_values.argtypes = [_I, _P, _I, _P, _P]
         lib.orc_permute_2b1b.argtypes = [_I, _P, _I, _I, _I, _P, _P]
@@ -102,6 +105,8 @@ def ref_lib():
         lib.ref_cheby_from_spectral.argtypes = [_I, _P, _P]
         lib.ref_cheby_deriv.argtypes = [_I, _I, _P, _P]
         lib.ref_cheby_deriv_nobuffer.argtypes = [_I, _I, _P, _P]
+        lib.ref_angular_init.argtypes = [_I, _P, _I, _I]
+        lib.ref_angular_deriv.argtypes = [_I, _I, _I, _I, _P, _P]
         _ref = lib
     return _ref
 
@@ -243,6 +248,31 @@ def permute_3b2b(nt, lmax, mvals, nr, nf, nphi, p3b):
     oracle_lib().orc_permute_3b2b(nt, lmax, _ptr(mv), len(mv), nr, nf, nphi,
                                   _ptr(np.ascontiguousarray(p3b)), _ptr(out))
     return out
+
+
+ANGULAR = {"dtheta": 0, "sdtheta": 1, "dphi": 2}
+
+
+def angular_deriv(kind, lmax, mvals, nr, a, nf_in, fin, out=None, nf_out=1, fout=0):
+    """Spectral_Derivatives.F90: d_by_dtheta / d_by_sdtheta / d_by_dphi of field fin (0-based)
+    of the packed array a (nf_in fields) into field fout of out (nf_out fields; a fresh
+    one-field array when out is None).  Returns out."""
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if out is None:
+        out = np.zeros(spectral_size(lmax, mv, nr, nf_out))
+    if kind == "dphi":
+        oracle_lib().orc_angular_dphi(lmax, _ptr(mv), len(mv), nr, _ptr(a), nf_in, fin, _ptr(out), nf_out, fout)
+    else:
+        oracle_lib().orc_angular_dtheta(ANGULAR[kind], lmax, _ptr(mv), len(mv), nr, _ptr(a), nf_in, fin,
+                                        _ptr(out), nf_out, fout)
+    return out
+
+
+def angular_coefs(l, m, kind):
+    c = np.zeros(2)
+    oracle_lib().orc_angular_coefs(l, m, ANGULAR[kind], _ptr(c))
+    return c
 
 
 def lm_values(lmax, mvals):
@@ -393,6 +423,21 @@ class Reference:
         f = np.zeros_like(c)
         self.lib.ref_cheby_from_spectral(c.size // self.ncheb, _ptr(c), _ptr(f))
         return f
+
+    def angular_init(self, lmax: int, mvals, nr: int):
+        self.ang = (lmax, np.ascontiguousarray(mvals, dtype=np.int32), nr)
+        mv = self.ang[1]
+        self.lib.ref_angular_init(len(mv), _ptr(mv), lmax, nr)
+
+    def angular_deriv(self, kind: str, buf: np.ndarray, nf: int, fin: int, fout: int):
+        """The reference's derivative of field fin (1-based) of a packed nf-field buffer:
+        fout >= 1 the buffer form (returns the whole buffer after the call), fout = 0 the
+        4dbuff2arr form (returns the one-field array)."""
+        lmax, mv, nr = self.ang
+        b = np.ascontiguousarray(buf, dtype=np.float64).copy()
+        arr = np.zeros(spectral_size(lmax, mv, nr, 1))
+        self.lib.ref_angular_deriv(ANGULAR[kind], nf, fin, fout, _ptr(b), _ptr(arr))
+        return b if fout >= 1 else arr
 
     def cheby_deriv(self, c: np.ndarray, dorder: int, nobuffer: bool = False) -> np.ndarray:
         c = np.ascontiguousarray(c, dtype=np.float64)

@@ -436,6 +436,86 @@ void orc_permute_3b2b(int nt, int lmax, const int *mvals, int nm, int nr, int nf
     }
 }
 
+/* Spectral_Derivatives.F90:49-83 Initialize_Angular_Derivatives, in its operation order:
+ * which = 0 deriv_coefs (sin(theta) d/dtheta), 1 sderiv_coefs (1/sin(theta) d(sin^2 .)/dtheta);
+ * c1 multiplies A(l-1), c2 multiplies A(l+1). */
+static void angular_coefs(int l, int m, int which, double *c1, double *c2)
+{
+    double num = (double)((l + m) * (l - m)), denom = (double)((2 * l + 1) * (2 * l - 1));
+    double q = sqrt(num / denom);
+    *c1 = which == 0 ? (double)(l - 1) * q : (double)(l + 1) * q;
+    num = (double)((l + 1 + m) * (l + 1 - m));
+    denom = (double)((2 * l + 1) * (2 * l + 3));
+    q = sqrt(num / denom);
+    *c2 = which == 0 ? -((double)(l + 2) * q) : -((double)l * q);
+}
+
+void orc_angular_coefs(int l, int m, int which, double *c)
+{
+    angular_coefs(l, m, which, c, c + 1);
+}
+
+/* Offset (doubles) of block mp in a packed spectral array of nf fields. */
+static long field_block(int lmax, const int *mvals, int mp, int nr, int nf)
+{
+    long off = 0;
+    for (int i = 0; i < mp; ++i) off += (long)(lmax - mvals[i] + 1) * nr * 2 * nf;
+    return off;
+}
+
+/* Spectral_Derivatives.F90:224-249 d_dtheta_buffer4D / :277-298 d_sdtheta_buffer4d (field
+ * fin -> field fout of one packed array) and :329-350 / :381-401 the 4dbuff2arr forms (into
+ * another array): per m-block, per column of field fin,
+ *   B(l) = A(l-1) c1(l) + A(l+1) c2(l)  (m < l < lmax),  B(m) = A(m+1) c2(m),
+ *   B(lmax) = A(lmax-1) c1(lmax);  B(m) = 0 when m = lmax.
+ * Fields are 0-based here; in and out may be the same array (other fields). */
+void orc_angular_dtheta(int which, int lmax, const int *mvals, int nm, int nr, const double *in, int nf_in, int fin,
+                        double *out, int nf_out, int fout)
+{
+    const long ncol = 2L * nr;
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp], nl = lmax - m + 1;
+        const double *a = in + field_block(lmax, mvals, mp, nr, nf_in) + (long)fin * nl * ncol;
+        double *b = out + field_block(lmax, mvals, mp, nr, nf_out) + (long)fout * nl * ncol;
+        for (long k = 0; k < ncol; ++k) {
+            const double *ak = a + k * nl;
+            double *bk = b + k * nl;
+            if (m == lmax) {
+                bk[0] = 0.0;
+                continue;
+            }
+            for (int l = m + 1; l <= lmax - 1; ++l) {
+                double c1, c2;
+                angular_coefs(l, m, which, &c1, &c2);
+                bk[l - m] = ak[l - 1 - m] * c1 + ak[l + 1 - m] * c2;
+            }
+            double c1, c2;
+            angular_coefs(m, m, which, &c1, &c2);
+            bk[0] = ak[1] * c2;
+            angular_coefs(lmax, m, which, &c1, &c2);
+            bk[lmax - m] = ak[lmax - 1 - m] * c1;
+        }
+    }
+}
+
+/* Spectral_Derivatives.F90:573-603 d_by_dphi_rlmbuff4d (field fin -> fout, in place allowed)
+ * and :495-509 d_by_dphi_4dbuff2arr: B(:, r, re) = -(m A(:, r, im)), B(:, r, im) = m A(:, r, re). */
+void orc_angular_dphi(int lmax, const int *mvals, int nm, int nr, const double *in, int nf_in, int fin, double *out,
+                      int nf_out, int fout)
+{
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp], nl = lmax - m + 1;
+        const long fs = (long)nl * nr;  /* one (imi) half of a field */
+        const double *a = in + field_block(lmax, mvals, mp, nr, nf_in) + (long)fin * 2 * fs;
+        double *b = out + field_block(lmax, mvals, mp, nr, nf_out) + (long)fout * 2 * fs;
+        for (long i = 0; i < fs; ++i) {
+            const double re = a[i], im = a[fs + i];
+            b[i] = -((double)m * im);
+            b[fs + i] = (double)m * re;
+        }
+    }
+}
+
 /* Load_Balance.F90:311-395 LM_Load_Balance on one radial rank: the (l, m) pairs of the
  * m values owned, l slowest (from the smallest m to lmax), m in m_values order within an
  * l; l_out / mp_out (0-based mp) receive the list, the return value is its length. */

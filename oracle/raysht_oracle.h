@@ -60,6 +60,11 @@ void orc_fft_r2c(int nphi, long howmany, double *x, int rescale);
 /* Row transposes on one rank. */
 void orc_permute_2a3a(int nt, int lmax, const int *mvals, int nm, int nr, int nf, int nphi,
                       const double *p2a, double *p3a);
+void orc_angular_coefs(int l, int m, int which, double *c);
+void orc_angular_dtheta(int which, int lmax, const int *mvals, int nm, int nr, const double *in, int nf_in, int fin,
+                        double *out, int nf_out, int fout);
+void orc_angular_dphi(int lmax, const int *mvals, int nm, int nr, const double *in, int nf_in, int fin, double *out,
+                      int nf_out, int fout);
 int orc_lm_values(int lmax, const int *mvals, int nm, int *l_out, int *mp_out);
 void orc_permute_2b1b(int lmax, const int *mvals, int nm, int nr, int nf, const double *s2b, double *p1b);
 void orc_permute_1a2a(int lmax, const int *mvals, int nm, int nr, int nf, const double *p1a, double *s2a);

@@ -1,6 +1,7 @@
 !
 ! Checker-only harness: drives the reference's own Math_Layer modules
-! (Legendre_Polynomials, Legendre_Transforms, Chebyshev_Polynomials, Structures),
+! (Legendre_Polynomials, Legendre_Transforms, Chebyshev_Polynomials,
+! Spectral_Derivatives, Structures),
 ! compiled from their sources under the reference tree, through bind(C) entry
 ! points that Python (ctypes) calls when it validates the C restatement in
 ! oracle/raysht_oracle.c and writes the golden fixtures under tests/golden/.
@@ -19,10 +20,13 @@ Module raysht_ref_harness
     Use Legendre_Polynomials
     Use Legendre_Transforms
     Use Chebyshev_Polynomials, Only : Cheby_Grid
+    Use Spectral_Derivatives
     Implicit None
     Type(Cheby_Grid), Save, Target :: h_grid
     Logical, Save :: h_grid_ready = .false.
     Integer, Save :: h_ncheb = 0
+    Integer, Save :: h_ang_nm = 0, h_ang_lmax = 0, h_ang_nr = 0
+    Integer, Allocatable, Save :: h_ang_mvals(:)
 
 Contains
 
@@ -215,5 +219,104 @@ Contains
         c_out = obuffer(:,:,1)
         DeAllocate(buffer, obuffer)
     End Subroutine ref_cheby_deriv_nobuffer
+
+    ! Initialize_Angular_Derivatives (Spectral_Derivatives.F90:49-83) for nm m values,
+    ! l_max and nr radii (a fresh start each call).
+    Subroutine ref_angular_init(nm, mvals, lmax, nr) bind(C, name='ref_angular_init')
+        Integer(C_INT), Value :: nm, lmax, nr
+        Integer(C_INT), Intent(In) :: mvals(nm)
+        If (Allocated(mlocal)) DeAllocate(mlocal)
+        If (Allocated(deriv_coefs)) DeAllocate(deriv_coefs)
+        If (Allocated(sderiv_coefs)) DeAllocate(sderiv_coefs)
+        If (Allocated(h_ang_mvals)) DeAllocate(h_ang_mvals)
+        Allocate(h_ang_mvals(nm))
+        h_ang_mvals = mvals
+        h_ang_nm = nm
+        h_ang_lmax = lmax
+        h_ang_nr = nr
+        Call Initialize_Angular_Derivatives(h_ang_mvals, lmax, nr)
+    End Subroutine ref_angular_init
+
+    ! packed spectral blocks of nf fields <-> rmcontainer4d / rmcontainer3d
+    Subroutine unpack4d(buf, nf, a)
+        Real(C_DOUBLE), Intent(In) :: buf(*)
+        Integer, Intent(In) :: nf
+        Type(rmcontainer4d), Allocatable, Intent(Out) :: a(:)
+        Integer :: mp, m, n, off
+        Allocate(a(1:h_ang_nm))
+        off = 0
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            Allocate(a(mp)%data(m:h_ang_lmax, 1:h_ang_nr, 1:2, 1:nf))
+            n = (h_ang_lmax-m+1)*h_ang_nr*2*nf
+            a(mp)%data = Reshape(buf(off+1:off+n), (/ h_ang_lmax-m+1, h_ang_nr, 2, nf /))
+            off = off + n
+        Enddo
+    End Subroutine unpack4d
+
+    Subroutine pack4d(a, nf, buf)
+        Type(rmcontainer4d), Intent(In) :: a(:)
+        Integer, Intent(In) :: nf
+        Real(C_DOUBLE), Intent(InOut) :: buf(*)
+        Integer :: mp, m, n, off
+        off = 0
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            n = (h_ang_lmax-m+1)*h_ang_nr*2*nf
+            buf(off+1:off+n) = Reshape(a(mp)%data, (/ n /))
+            off = off + n
+        Enddo
+    End Subroutine pack4d
+
+    Subroutine alloc3d(a)
+        Type(rmcontainer3d), Allocatable, Intent(Out) :: a(:)
+        Integer :: mp, m
+        Allocate(a(1:h_ang_nm))
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            Allocate(a(mp)%data(m:h_ang_lmax, 1:h_ang_nr, 1:2))
+            a(mp)%data = 0.0d0
+        Enddo
+    End Subroutine alloc3d
+
+    Subroutine pack3d(a, buf)
+        Type(rmcontainer3d), Intent(In) :: a(:)
+        Real(C_DOUBLE), Intent(Out) :: buf(*)
+        Integer :: mp, m, n, off
+        off = 0
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            n = (h_ang_lmax-m+1)*h_ang_nr*2
+            buf(off+1:off+n) = Reshape(a(mp)%data, (/ n /))
+            off = off + n
+        Enddo
+    End Subroutine pack3d
+
+    ! which = 0: d_by_dtheta, 1: d_by_sdtheta, 2: d_by_dphi on a packed buffer of nf fields
+    ! (in place: buf is read and written).  fout >= 1: the buffer forms (d_dtheta_buffer4D,
+    ! d_sdtheta_buffer4d, d_by_dphi_rlmbuff4d: field fin -> field fout, 1-based);
+    ! fout = 0: the 4dbuff2arr forms into arr, a packed one-field array.
+    Subroutine ref_angular_deriv(which, nf, fin, fout, buf, arr) bind(C, name='ref_angular_deriv')
+        Integer(C_INT), Value :: which, nf, fin, fout
+        Real(C_DOUBLE), Intent(InOut) :: buf(*)
+        Real(C_DOUBLE), Intent(Out) :: arr(*)
+        Type(rmcontainer4d), Allocatable :: a(:)
+        Type(rmcontainer3d), Allocatable :: b(:)
+        Call unpack4d(buf, nf, a)
+        If (fout >= 1) Then
+            If (which == 0) Call d_by_dtheta(a, fin, fout)
+            If (which == 1) Call d_by_sdtheta(a, fin, fout)
+            If (which == 2) Call d_by_dphi(a, fin, fout)
+            Call pack4d(a, nf, buf)
+        Else
+            Call alloc3d(b)
+            If (which == 0) Call d_by_dtheta(a, fin, b)
+            If (which == 1) Call d_by_sdtheta(a, fin, b)
+            If (which == 2) Call d_by_dphi(a, fin, b)
+            Call pack3d(b, arr)
+            DeAllocate(b)
+        Endif
+        DeAllocate(a)
+    End Subroutine ref_angular_deriv
 
 End Module raysht_ref_harness

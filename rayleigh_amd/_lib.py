@@ -69,6 +69,8 @@ SIGNATURES = {
     "raysht_permute_1a2a": (_I, [_P, _P, _P, _P]),
     "raysht_lm_count": (_I, [_P]),
     "raysht_lm_values": (_I, [_P, _P, _P]),
+    "raysht_d_by_dtheta": (_I, [_P, _I, _P, _I, _I, _P, _I, _I, _P]),
+    "raysht_d_by_dphi": (_I, [_P, _P, _I, _I, _P, _I, _I, _P]),
 }
 
 _lib = None

@@ -102,6 +102,9 @@ struct raysht_plan {
     std::vector<int> lm_l, lm_mp;
     int *d_lm_base = nullptr, *d_lm_slot = nullptr;
     int2 *d_rtiles = nullptr;
+    // angular derivatives: (c1, c2) per (m-block, l) for d_by_dtheta, then d_by_sdtheta
+    long lm_total = 0;
+    double2 *d_acoef = nullptr;
 };
 
 struct raysht_cheb {
@@ -127,6 +130,9 @@ void fft_twiddles(int n, double *cos_out, double *sin_out);
 // Radial transpose tables of a plan (radial_transpose.hip).
 int radial_plan_init(raysht_plan *p);
 void radial_plan_free(raysht_plan *p);
+// Angular derivative coefficients of a plan (angular.hip).
+int angular_plan_init(raysht_plan *p);
+void angular_plan_free(raysht_plan *p);
 
 // Device launches (each returns RAYSHT_OK or an error code).
 int launch_plm_tables(raysht_plan *p, const double *d_x, const double *d_renorm, const double *d_amp,

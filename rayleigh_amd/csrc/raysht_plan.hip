@@ -169,6 +169,7 @@ void raysht_plan_destroy(raysht_plan *p)
     device_free(p->d_desc);
     split_plan_free(p);
     radial_plan_free(p);
+    angular_plan_free(p);
     delete p;
 }
 
@@ -303,6 +304,7 @@ int plan_create(raysht_plan **out, int lmax, int ntheta, const int *m_values, in
     device_free(d_amp);
     if (rc == RAYSHT_OK && precision != RAYSHT_FP64) rc = split_plan_init(p);
     if (rc == RAYSHT_OK) rc = radial_plan_init(p);
+    if (rc == RAYSHT_OK) rc = angular_plan_init(p);
     if (rc == RAYSHT_OK) {
         p->desc = describe(p);
         rc = device_upload(&p->d_desc, &p->desc, 1);

@@ -126,6 +126,26 @@ class LegendrePlan:
                                                   _dptr(s2a, "s2a", self.spectral_size), _stream(stream)),
                    "raysht_permute_1a2a")
 
+    def angular_derivative(self, kind: str, a, fin: int, out, fout: int, stream=None):
+        """Spectral_Derivatives.F90's d_by_dtheta (kind "dtheta"), d_by_sdtheta ("sdtheta") or
+        d_by_dphi ("dphi") of field fin of `a` into field fout of `out` (0-based fields; each
+        array a whole number of fields of this plan's spectral layout)."""
+        if self.n_fields < 1:
+            raise ValueError("a plan over anonymous columns has no fields")
+        per_field = self.spectral_size // self.n_fields
+        if a.numel() % per_field or out.numel() % per_field:
+            raise ValueError("arrays must hold whole fields of this plan's spectral layout")
+        nf_in, nf_out = a.numel() // per_field, out.numel() // per_field
+        if kind == "dphi":
+            _lib.check(_lib.lib().raysht_d_by_dphi(self._h, _dptr(a, "a"), nf_in, int(fin), _dptr(out, "out"),
+                                                   nf_out, int(fout), _stream(stream)), "raysht_d_by_dphi")
+        elif kind in ("dtheta", "sdtheta"):
+            _lib.check(_lib.lib().raysht_d_by_dtheta(self._h, 0 if kind == "dtheta" else 1, _dptr(a, "a"), nf_in,
+                                                     int(fin), _dptr(out, "out"), nf_out, int(fout),
+                                                     _stream(stream)), "raysht_d_by_dtheta")
+        else:
+            raise ValueError("kind must be dtheta, sdtheta or dphi")
+
     def lm_values(self):
         """LM_Load_Balance's (l, mp) pairs (l_lm_values, mp_lm_values with mp 0-based)."""
         n = _lib.lib().raysht_lm_count(self._h)

@@ -76,6 +76,23 @@ def cheby_deriv_case(name, n, ncols, seed, rmin, rmax):
     print("wrote", name)
 
 
+def angular_case(name, lmax, mvals, nr, seed):
+    """d_by_dtheta, d_by_sdtheta and d_by_dphi (Spectral_Derivatives.F90) of a seeded
+    two-field buffer: the buffer forms (field 1 -> field 2, 1-based) and the 4dbuff2arr
+    forms (field 2 -> a one-field array)."""
+    R = O.Reference()
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    R.angular_init(lmax, mv, nr)
+    rng = np.random.default_rng(seed)
+    buf = rng.standard_normal(O.spectral_size(lmax, mv, nr, 2))
+    rec = dict(lmax=lmax, mvals=mv, nr=nr, buf=buf)
+    for kind in ("dtheta", "sdtheta", "dphi"):
+        rec[f"{kind}_buffer"] = R.angular_deriv(kind, buf, 2, 1, 2)
+        rec[f"{kind}_arr"] = R.angular_deriv(kind, buf, 2, 2, 0)
+    np.savez_compressed(os.path.join(OUT, name), **rec)
+    print("wrote", name)
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("compile the reference checker first: make -C oracle ref")
@@ -93,3 +110,6 @@ if __name__ == "__main__":
     cheby_deriv_case("cheby_deriv_n48.npz", 48, 6, seed=480, rmin=0.35, rmax=1.0)
     cheby_deriv_case("cheby_deriv_n33.npz", 33, 5, seed=330, rmin=0.5, rmax=1.7)
     cheby_deriv_case("cheby_deriv_n128.npz", 128, 3, seed=1280, rmin=0.35 / 0.65, rmax=1.0 / 0.65)
+    angular_case("angular_L12_ascending.npz", 12, np.arange(13), 3, seed=12)
+    angular_case("angular_L41_balanced.npz", 41, O.m_order_balanced(41), 2, seed=41)
+    angular_case("angular_L9_partial.npz", 9, [9, 0, 4, 3, 1], 5, seed=9)
