@@ -304,6 +304,44 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
         out["angular_derivatives"][kind] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9,
                                             "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
     del f1
+    # the implicit solve's dense path for the coupled W-P system (n = 2 N_r) of every l:
+    # LU_Decompose_full on l_max + 1 matrices, then LU_Solve_full on 2 (l + 1) right-hand
+    # sides per l (re, im of every m <= l); 2 n^3 / 3 and 2 n^2 per column flop
+    n_lu, nb = 2 * nr, L + 1
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    mats0 = torch.randn(nb * n_lu * n_lu, dtype=torch.float64, device="cuda", generator=gen)
+    mats0.view(nb, n_lu, n_lu).diagonal(dim1=1, dim2=2).add_(float(n_lu))
+    mats = torch.empty_like(mats0)
+    piv = torch.empty(nb * n_lu, dtype=torch.int32, device="cuda")
+    info = torch.empty(nb, dtype=torch.int32, device="cuda")
+    ncols = [2 * (l + 1) for l in range(nb)]
+    offs = np.concatenate([[0], np.cumsum([c * n_lu for c in ncols])]).astype(np.int64)
+    rhs0 = torch.randn(int(offs[-1]), dtype=torch.float64, device="cuda", generator=gen)
+    rhs = torch.empty_like(rhs0)
+    d_offs = torch.from_numpy(offs[:-1]).cuda()
+    d_ncols = torch.tensor(ncols, dtype=torch.int32, device="cuda")
+    lu_rec = {}
+    for name, prep, fn, flop in (
+            ("lu_decompose_full", lambda: mats.copy_(mats0), lambda: ra.lu_decompose_full(mats, piv, info, n_lu),
+             nb * 2.0 * n_lu ** 3 / 3),
+            ("lu_solve_full", lambda: rhs.copy_(rhs0),
+             lambda: ra.lu_solve_full(mats, piv, rhs, d_offs, d_ncols, n_lu, max(ncols)),
+             2.0 * n_lu * n_lu * sum(ncols))):
+        ts = []
+        for it in range(max(warmup, 1) + steps):
+            prep()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            if it >= max(warmup, 1):
+                ts.append(e0.elapsed_time(e1))
+        ms = float(np.mean(ts))
+        lu_rec[name] = {"ms": ms, "gflops": flop / (ms * 1e-3) / 1e9,
+                        "frac_fp64_peak": flop / (ms * 1e-3) / (PEAK_FP64_TFLOPS * 1e12)}
+    out["implicit_solve"] = {"n": n_lu, "matrices": nb, **lu_rec}
+    del mats0, mats, rhs0, rhs
     # the radial derivative recursion (d_by_dr_cp) of orders 1-3 on the spectral radial
     # buffer, as Post_Solve applies it: 16 B per coefficient (read + write)
     rad3 = torch.empty_like(rad)

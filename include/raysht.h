@@ -148,6 +148,19 @@ int raysht_d_by_dtheta(raysht_plan *plan, int kind, const double *in, int nf_in,
 int raysht_d_by_dphi(raysht_plan *plan, const double *in, int nf_in, int fin, double *out, int nf_out, int fout,
                      void *stream);
 
+/* Implicit_Solve's dense path (Linear_Solve.F90:423-457) for many matrices at once.
+ * raysht_lu_decompose_full = LU_Decompose_full (:1034-1045, LAPACK dgetrf) on nbatch n x n
+ * column-major matrices stored one after another in `mats` (overwritten by L\U), pivots
+ * (nbatch x n ints, 1-based, as dgetrf returns them) and info (nbatch ints: 0, or the
+ * first zero pivot's 1-based index).  Device arrays; n <= 1024. */
+int raysht_lu_decompose_full(int n, int nbatch, double *mats, int *pivots, int *info, void *stream);
+/* LU_Solve_full (:1087-1107, dgetrs 'N') for every matrix b of the batch on its own
+ * right-hand sides: nrhs[b] columns of n (column-major, leading dimension n) at
+ * rhs + rhs_off[b] (rhs_off, nrhs: device arrays), overwritten by the solutions;
+ * max_nrhs >= every nrhs[b] sizes the launch. */
+int raysht_lu_solve_full(int n, int nbatch, const double *mats, const int *pivots, double *rhs, const long *rhs_off,
+                         const int *nrhs, int max_nrhs, void *stream);
+
 /* Fused forms of the same steps (results identical to the separate calls):
  * StP then 2a->3a then c2r, and r2c then 3b->2b then PtS.  `work` is a hybrid
  * buffer the call may overwrite. */

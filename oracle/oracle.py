@@ -69,6 +69,9 @@ def oracle_lib():
         lib.orc_angular_coefs.argtypes = [_I, _I, _I, _P]
         lib.orc_angular_dtheta.argtypes = [_I, _I, _P, _I, _I, _P, _I, _I, _P, _I, _I]
         lib.orc_angular_dphi.argtypes = [_I, _P, _I, _I, _P, _I, _I, _P, _I, _I]
+        lib.orc_lu_decompose.restype = _I
+        lib.orc_lu_decompose.argtypes = [_I, _P, _P]
+        lib.orc_lu_solve.argtypes = [_I, _P, _P, _I, _P]
         lib.orc_lm_values.restype = _I
         lib.orc_lm_values.argtypes = [_I, _P, _I, _P, _P]
         lib.orc_permute_2b1b.argtypes = [_I, _P, _I, _I, _I, _P, _P]
@@ -273,6 +276,25 @@ def angular_coefs(l, m, kind):
     c = np.zeros(2)
     oracle_lib().orc_angular_coefs(l, m, ANGULAR[kind], _ptr(c))
     return c
+
+
+def lu_decompose(a):
+    """LU_Decompose_full (LAPACK dgetrf) of one n x n matrix given as a column-major
+    (Fortran-order) array: returns (LU, 1-based pivots, info)."""
+    n = a.shape[0]
+    lu = np.asfortranarray(a, dtype=np.float64).copy(order="F")
+    piv = np.zeros(n, np.int32)
+    info = oracle_lib().orc_lu_decompose(n, _ptr(lu), _ptr(piv))
+    return lu, piv, info
+
+
+def lu_solve(lu, piv, b):
+    """LU_Solve_full (LAPACK dgetrs 'N'): b is n x nrhs, column-major."""
+    n = lu.shape[0]
+    x = np.asfortranarray(b, dtype=np.float64).copy(order="F")
+    nrhs = 1 if x.ndim == 1 else x.shape[1]
+    oracle_lib().orc_lu_solve(n, _ptr(np.asfortranarray(lu)), _ptr(np.ascontiguousarray(piv, np.int32)), nrhs, _ptr(x))
+    return x
 
 
 def lm_values(lmax, mvals):

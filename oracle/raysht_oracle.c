@@ -516,6 +516,81 @@ void orc_angular_dphi(int lmax, const int *mvals, int nm, int nr, const double *
     }
 }
 
+/* Linear_Solve.F90:1034-1045 LU_Decompose_full = LAPACK dgetrf on one n x n matrix (column
+ * major, leading dimension n), restated as the unblocked right-looking algorithm of
+ * dgetf2 (LAPACK 3.x, the reference links LAPACK): per column k the pivot is the first
+ * row of largest |a_ik|, rows k and p are swapped over all columns, the subdiagonal is
+ * scaled by the pivot's reciprocal (a division where |pivot| < the safe minimum), and
+ * the trailing block is updated a_ij - a_ik a_kj (rank-1, dger order).  pivots are
+ * 1-based; returns info (0, or the first k + 1 with a zero pivot). */
+int orc_lu_decompose(int n, double *a, int *piv)
+{
+    int info = 0;
+    const double sfmin = 2.2250738585072014e-308;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = fabs(a[k + (long)k * n]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(a[i + (long)k * n]) > best) {
+                best = fabs(a[i + (long)k * n]);
+                p = i;
+            }
+        piv[k] = p + 1;
+        const double pv = a[p + (long)k * n];
+        if (pv != 0.0) {
+            if (p != k)
+                for (int j = 0; j < n; ++j) {
+                    const double t = a[k + (long)j * n];
+                    a[k + (long)j * n] = a[p + (long)j * n];
+                    a[p + (long)j * n] = t;
+                }
+            if (fabs(pv) >= sfmin) {
+                const double r = 1.0 / pv;
+                for (int i = k + 1; i < n; ++i) a[i + (long)k * n] *= r;
+            } else {
+                for (int i = k + 1; i < n; ++i) a[i + (long)k * n] /= pv;
+            }
+        } else if (info == 0) {
+            info = k + 1;
+        }
+        for (int j = k + 1; j < n; ++j) {
+            const double y = a[k + (long)j * n];
+            if (y != 0.0) {
+                const double t = -y;
+                for (int i = k + 1; i < n; ++i) a[i + (long)j * n] += a[i + (long)k * n] * t;
+            }
+        }
+    }
+    return info;
+}
+
+/* Linear_Solve.F90:1087-1107 LU_Solve_full = LAPACK dgetrs('N'): the row interchanges of
+ * piv applied to the nrhs columns of b (leading dimension n) in order, then the unit
+ * lower and the upper triangular solves in the column order of the reference BLAS dtrsm
+ * (column j final, then subtracted from the rows after / before it). */
+void orc_lu_solve(int n, const double *a, const int *piv, int nrhs, double *b)
+{
+    for (int c = 0; c < nrhs; ++c) {
+        double *x = b + (long)c * n;
+        for (int k = 0; k < n; ++k) {
+            const int p = piv[k] - 1;
+            if (p != k) {
+                const double t = x[k];
+                x[k] = x[p];
+                x[p] = t;
+            }
+        }
+        for (int j = 0; j < n; ++j)
+            if (x[j] != 0.0)
+                for (int i = j + 1; i < n; ++i) x[i] -= x[j] * a[i + (long)j * n];
+        for (int j = n - 1; j >= 0; --j)
+            if (x[j] != 0.0) {
+                x[j] /= a[j + (long)j * n];
+                for (int i = 0; i < j; ++i) x[i] -= x[j] * a[i + (long)j * n];
+            }
+    }
+}
+
 /* Load_Balance.F90:311-395 LM_Load_Balance on one radial rank: the (l, m) pairs of the
  * m values owned, l slowest (from the smallest m to lmax), m in m_values order within an
  * l; l_out / mp_out (0-based mp) receive the list, the return value is its length. */

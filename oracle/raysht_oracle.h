@@ -65,6 +65,8 @@ void orc_angular_dtheta(int which, int lmax, const int *mvals, int nm, int nr, c
                         double *out, int nf_out, int fout);
 void orc_angular_dphi(int lmax, const int *mvals, int nm, int nr, const double *in, int nf_in, int fin, double *out,
                       int nf_out, int fout);
+int orc_lu_decompose(int n, double *a, int *piv);
+void orc_lu_solve(int n, const double *a, const int *piv, int nrhs, double *b);
 int orc_lm_values(int lmax, const int *mvals, int nm, int *l_out, int *mp_out);
 void orc_permute_2b1b(int lmax, const int *mvals, int nm, int nr, int nf, const double *s2b, double *p1b);
 void orc_permute_1a2a(int lmax, const int *mvals, int nm, int nr, int nf, const double *p1a, double *s2a);

@@ -4,7 +4,7 @@ The compute lives in the HIP library libraysht.so (C ABI in include/raysht.h);
 this package loads it and mirrors the reference's Math_Layer interface names.
 """
 from .transforms import (BatchGraph, ChebyGrid, LegendreBatch, LegendrePlan, RoundTripGraph,  # noqa: F401
-                         fft_to_physical, fft_to_spectral, fft_to_spectral_rsc)
+                         fft_to_physical, fft_to_spectral, fft_to_spectral_rsc, lu_decompose_full, lu_solve_full)
 
 __all__ = ["LegendrePlan", "LegendreBatch", "BatchGraph", "ChebyGrid", "RoundTripGraph", "fft_to_physical",
-           "fft_to_spectral", "fft_to_spectral_rsc"]
+           "fft_to_spectral", "fft_to_spectral_rsc", "lu_decompose_full", "lu_solve_full"]

@@ -71,6 +71,8 @@ SIGNATURES = {
     "raysht_lm_values": (_I, [_P, _P, _P]),
     "raysht_d_by_dtheta": (_I, [_P, _I, _P, _I, _I, _P, _I, _I, _P]),
     "raysht_d_by_dphi": (_I, [_P, _P, _I, _I, _P, _I, _I, _P]),
+    "raysht_lu_decompose_full": (_I, [_I, _I, _P, _P, _P, _P]),
+    "raysht_lu_solve_full": (_I, [_I, _I, _P, _P, _P, _P, _P, _I, _P]),
 }
 
 _lib = None

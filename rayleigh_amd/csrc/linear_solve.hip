@@ -1,0 +1,303 @@
+// Batched dense LU factorisation and solve for the implicit radial step.
+//
+// Implicit_Solve (Linear_Solve.F90:423-457) factors each mode's n x n matrix once per time
+// step size (LU_Decompose_full = LAPACK dgetrf, :1034-1045) and every step solves it for
+// the right-hand sides of all m of that l (LU_Solve_full = dgetrs 'N', :1087-1107).  Here
+// many matrices go in one launch, one workgroup per matrix (factorisation) or per
+// matrix and 16-column chunk of its right-hand sides (solve; n <= 256 with the columns in
+// registers, larger n with them in LDS).
+//
+// Factorisation: the unblocked right-looking algorithm of dgetf2 -- pivot = first row of
+// largest |a_ik| (a workgroup reduction), row swap, subdiagonal scaled by the pivot's
+// reciprocal, rank-1 update of the trailing block in global memory (L2) -- with
+// products and sums rounded separately.
+// Solve: the chunk of right-hand sides lives in LDS; the row interchanges, the unit lower
+// and the upper triangular solves run in the column order of the reference BLAS dtrsm
+// (column j final, then subtracted from the rows below / above it), one barrier per
+// column.
+#include <algorithm>
+#include <cmath>
+
+#include "raysht_internal.h"
+
+namespace raysht {
+namespace {
+
+constexpr int kLuThreads = 256;
+constexpr int kLuMaxN = 1024;
+
+__global__ __launch_bounds__(kLuThreads) void lu_decompose_kernel(int n, double *__restrict__ mats,
+                                                                  int *__restrict__ pivots, int *__restrict__ infos)
+{
+#pragma clang fp contract(off)
+    __shared__ double s_val[kLuThreads];
+    __shared__ int s_idx[kLuThreads];
+    double *a = mats + (long)blockIdx.x * n * n;
+    int *piv = pivots + (long)blockIdx.x * n;
+    const int tid = threadIdx.x;
+    const double sfmin = 2.2250738585072014e-308;
+    int info = 0;
+    for (int k = 0; k < n; ++k) {
+        // pivot: the first row of largest |a_ik|, i >= k
+        double best = -1.0;
+        int bi = n;
+        for (int i = k + tid; i < n; i += kLuThreads) {
+            const double v = fabs(a[i + (long)k * n]);
+            if (v > best) {
+                best = v;
+                bi = i;
+            }
+        }
+        s_val[tid] = best;
+        s_idx[tid] = bi;
+        __syncthreads();
+        for (int s = kLuThreads / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                const double v = s_val[tid + s];
+                const int j = s_idx[tid + s];
+                if (v > s_val[tid] || (v == s_val[tid] && j < s_idx[tid])) {
+                    s_val[tid] = v;
+                    s_idx[tid] = j;
+                }
+            }
+            __syncthreads();
+        }
+        const int p = s_idx[0] < n ? s_idx[0] : k;  // a column of NaNs: no larger entry, keep row k
+        const double pv = a[p + (long)k * n];
+        __syncthreads();  // every thread has read the pivot before the swap moves it
+        if (tid == 0) piv[k] = p + 1;
+        if (pv != 0.0) {
+            if (p != k)
+                for (int j = tid; j < n; j += kLuThreads) {
+                    const double t = a[k + (long)j * n];
+                    a[k + (long)j * n] = a[p + (long)j * n];
+                    a[p + (long)j * n] = t;
+                }
+            __syncthreads();
+            if (fabs(pv) >= sfmin) {
+                const double r = 1.0 / pv;
+                for (int i = k + 1 + tid; i < n; i += kLuThreads) a[i + (long)k * n] *= r;
+            } else {
+                for (int i = k + 1 + tid; i < n; i += kLuThreads) a[i + (long)k * n] /= pv;
+            }
+        } else if (info == 0) {
+            info = k + 1;
+        }
+        __syncthreads();
+        // trailing block: a_ij + a_ik (-a_kj), i, j > k, column-major sweep (i fastest)
+        const int m = n - k - 1;
+        const long mm = (long)m * m;
+        for (long e = tid; e < mm; e += kLuThreads) {
+            const int jj = (int)(e / m), ii = (int)(e - (long)jj * m);
+            const int i = k + 1 + ii, j = k + 1 + jj;
+            const double y = a[k + (long)j * n];
+            if (y != 0.0) a[i + (long)j * n] = a[i + (long)j * n] + a[i + (long)k * n] * -y;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) infos[blockIdx.x] = info;
+}
+
+// One workgroup per (matrix, chunk of CC right-hand-side columns); x in LDS as [c][n + 1].
+template <int CC>
+__global__ __launch_bounds__(kLuThreads) void lu_solve_kernel(int n, const double *__restrict__ mats,
+                                                              const int *__restrict__ pivots, double *__restrict__ rhs,
+                                                              const long *__restrict__ rhs_off,
+                                                              const int *__restrict__ nrhs)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double xs[];
+    const int b = blockIdx.x, col0 = blockIdx.y * CC, ncol = nrhs[b];
+    if (col0 >= ncol) return;  // uniform: the whole workgroup leaves
+    const int cols = min(CC, ncol - col0);
+    const double *a = mats + (long)b * n * n;
+    const int *piv = pivots + (long)b * n;
+    double *x = rhs + rhs_off[b] + (long)col0 * n;
+    const int tid = threadIdx.x, ld = n + 1;
+    for (int e = tid; e < n * CC; e += kLuThreads) {
+        const int c = e / n, i = e - c * n;
+        xs[c * ld + i] = c < cols ? x[(long)c * n + i] : 0.0;
+    }
+    __syncthreads();
+    if (tid < cols) {  // dlaswp: the interchanges in order
+        double *xc = xs + tid * ld;
+        for (int k = 0; k < n; ++k) {
+            const int p = piv[k] - 1;
+            if (p != k && p >= 0 && p < n) {
+                const double t = xc[k];
+                xc[k] = xc[p];
+                xc[p] = t;
+            }
+        }
+    }
+    __syncthreads();
+    constexpr int RG = kLuThreads / CC;  // row groups
+    const int c = tid % CC, rg = tid / CC;
+    double *xc = xs + c * ld;
+    // unit lower: for j, x_i -= x_j l_ij (i > j)
+    for (int j = 0; j < n - 1; ++j) {
+        const double xj = xc[j];
+        if (xj != 0.0)
+            for (int i = j + 1 + rg; i < n; i += RG) xc[i] = xc[i] - xj * a[i + (long)j * n];
+        __syncthreads();
+    }
+    // upper: for j descending, x_j /= u_jj, then x_i -= x_j u_ij (i < j); x_j is final then
+    for (int j = n - 1; j >= 0; --j) {
+        double xj = xc[j];
+        if (xj != 0.0) {
+            xj = xj / a[j + (long)j * n];
+            for (int i = rg; i < j; i += RG) xc[i] = xc[i] - xj * a[i + (long)j * n];
+        }
+        if (rg == 0 && c < cols) x[(long)c * n + j] = xj;
+        __syncthreads();
+    }
+}
+
+// Solve with the right-hand sides in registers: 16 lanes of a wave own one column, lane
+// r holds rows r, r + 16, ... (NT of them).  The interchanges are composed into one
+// permutation applied when the column is loaded; then each step j takes x_j from its
+// owner lane by a lane shuffle and updates the lane's rows in registers, while the next
+// column of L (or U) is loaded -- no barrier after the permutation.  Operation order and
+// rounding are those of the LDS kernel above.
+template <int NT>
+__global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const double *__restrict__ mats,
+                                                                    const int *__restrict__ pivots,
+                                                                    double *__restrict__ rhs,
+                                                                    const long *__restrict__ rhs_off,
+                                                                    const int *__restrict__ nrhs)
+{
+#pragma clang fp contract(off)
+    __shared__ int perm[16 * NT];
+    constexpr int kCols = kLuThreads / 16;
+    const int b = blockIdx.x, ncol = nrhs[b], col0 = blockIdx.y * kCols;
+    if (col0 >= ncol) return;  // uniform: the whole workgroup leaves
+    const double *a = mats + (long)b * n * n;
+    const int *piv = pivots + (long)b * n;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        for (int i = 0; i < n; ++i) perm[i] = i;
+        for (int k = 0; k < n; ++k) {
+            const int p = piv[k] - 1;
+            if (p != k && p >= 0 && p < n) {
+                const int t = perm[k];
+                perm[k] = perm[p];
+                perm[p] = t;
+            }
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 15, c = col0 + tid / 16;
+    const bool live = c < ncol;
+    double *xg = rhs + rhs_off[b] + (long)(live ? c : col0) * n;
+    double x[NT], cur[NT], nxt[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int i = 16 * t + lane;
+        x[t] = live && i < n ? xg[perm[i]] : 0.0;
+    }
+    auto column = [&](int j, double(&v)[NT]) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int i = 16 * t + lane;
+            v[t] = i < n ? a[i + (long)j * n] : 0.0;
+        }
+    };
+    // unit lower: x_i -= x_j l_ij for i > j
+    column(0, cur);
+#pragma unroll
+    for (int tb = 0; tb < NT; ++tb) {
+        for (int r = 0; r < 16; ++r) {
+            const int j = 16 * tb + r;
+            if (j >= n - 1) break;
+            column(j + 1, nxt);
+            const double xj = __shfl(x[tb], r, 16);
+            if (xj != 0.0) {
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    if (16 * t + lane > j) x[t] = x[t] - xj * cur[t];
+            }
+#pragma unroll
+            for (int t = 0; t < NT; ++t) cur[t] = nxt[t];
+        }
+    }
+    // upper: x_j /= u_jj, then x_i -= x_j u_ij for i < j; x_j is final
+    column(n - 1, cur);
+#pragma unroll
+    for (int tb = NT - 1; tb >= 0; --tb) {
+        for (int r = 15; r >= 0; --r) {
+            const int j = 16 * tb + r;
+            if (j >= n) continue;
+            if (j > 0) column(j - 1, nxt);
+            double xj = __shfl(x[tb], r, 16);
+            const double ujj = __shfl(cur[tb], r, 16);
+            if (xj != 0.0) {
+                xj = xj / ujj;
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    if (16 * t + lane < j) x[t] = x[t] - xj * cur[t];
+            }
+            if (lane == r) x[tb] = xj;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) cur[t] = nxt[t];
+        }
+    }
+    if (live) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int i = 16 * t + lane;
+            if (i < n) xg[i] = x[t];
+        }
+    }
+}
+
+}  // namespace
+}  // namespace raysht
+
+using namespace raysht;
+
+extern "C" {
+
+int raysht_lu_decompose_full(int n, int nbatch, double *mats, int *pivots, int *info, void *stream)
+{
+    if (n < 1 || nbatch < 0 || (nbatch > 0 && (!mats || !pivots || !info))) return RAYSHT_EINVAL;
+    if (n > kLuMaxN) return RAYSHT_EUNSUPPORTED;
+    if (nbatch == 0) return RAYSHT_OK;
+    hipLaunchKernelGGL(lu_decompose_kernel, dim3((unsigned)nbatch), dim3(kLuThreads), 0,
+                       reinterpret_cast<hipStream_t>(stream), n, mats, pivots, info);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+int raysht_lu_solve_full(int n, int nbatch, const double *mats, const int *pivots, double *rhs, const long *rhs_off,
+                         const int *nrhs, int max_nrhs, void *stream)
+{
+    if (n < 1 || nbatch < 0 || max_nrhs < 0) return RAYSHT_EINVAL;
+    if (nbatch > 0 && (!mats || !pivots || !rhs || !rhs_off || !nrhs)) return RAYSHT_EINVAL;
+    if (n > kLuMaxN) return RAYSHT_EUNSUPPORTED;
+    if (nbatch == 0 || max_nrhs == 0) return RAYSHT_OK;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    constexpr int kLaneCols = kLuThreads / 16;
+    const dim3 lgrid((unsigned)nbatch, (unsigned)((max_nrhs + kLaneCols - 1) / kLaneCols));
+    if (n <= 64) {
+        hipLaunchKernelGGL(lu_solve_lanes_kernel<4>, lgrid, dim3(kLuThreads), 0, st, n, mats, pivots, rhs, rhs_off, nrhs);
+    } else if (n <= 128) {
+        hipLaunchKernelGGL(lu_solve_lanes_kernel<8>, lgrid, dim3(kLuThreads), 0, st, n, mats, pivots, rhs, rhs_off, nrhs);
+    } else if (n <= 256) {
+        hipLaunchKernelGGL(lu_solve_lanes_kernel<16>, lgrid, dim3(kLuThreads), 0, st, n, mats, pivots, rhs, rhs_off,
+                           nrhs);
+    } else if (n <= 512) {
+        constexpr int CC = 16;
+        const size_t lds = (size_t)CC * (n + 1) * sizeof(double);
+        dim3 grid((unsigned)nbatch, (unsigned)((max_nrhs + CC - 1) / CC));
+        hipLaunchKernelGGL(lu_solve_kernel<CC>, grid, dim3(kLuThreads), lds, st, n, mats, pivots, rhs, rhs_off, nrhs);
+    } else {
+        constexpr int CC = 8;
+        const size_t lds = (size_t)CC * (n + 1) * sizeof(double);
+        dim3 grid((unsigned)nbatch, (unsigned)((max_nrhs + CC - 1) / CC));
+        hipLaunchKernelGGL(lu_solve_kernel<CC>, grid, dim3(kLuThreads), lds, st, n, mats, pivots, rhs, rhs_off, nrhs);
+    }
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+}  // extern "C"

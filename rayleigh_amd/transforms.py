@@ -335,6 +335,41 @@ def fft_to_spectral_rsc(x, n_phi: int, stream=None):
                                                  _stream(stream)), "raysht_fft_to_spectral")
 
 
+def _iptr(t, name: str, dtype, numel: int) -> ctypes.c_void_p:
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != dtype or not t.is_contiguous():
+        raise TypeError(f"{name} must be a contiguous device tensor of {dtype}")
+    if t.numel() < numel:
+        raise ValueError(f"{name} has {t.numel()} elements, expected at least {numel}")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def lu_decompose_full(mats, pivots, info, n: int, stream=None):
+    """LU_Decompose_full (LAPACK dgetrf, Linear_Solve.F90:1034-1045) of every n x n
+    column-major matrix in `mats` (float64, nbatch n^2), in place; pivots (int32, nbatch n,
+    1-based) and info (int32, nbatch) receive dgetrf's outputs."""
+    import torch
+    if mats.numel() % (n * n):
+        raise ValueError("mats must hold whole n x n matrices")
+    nb = mats.numel() // (n * n)
+    _lib.check(_lib.lib().raysht_lu_decompose_full(int(n), nb, _dptr(mats, "mats"),
+                                                   _iptr(pivots, "pivots", torch.int32, nb * n),
+                                                   _iptr(info, "info", torch.int32, nb), _stream(stream)),
+               "raysht_lu_decompose_full")
+
+
+def lu_solve_full(mats, pivots, rhs, rhs_off, nrhs, n: int, max_nrhs: int, stream=None):
+    """LU_Solve_full (dgetrs 'N', Linear_Solve.F90:1087-1107): matrix b of the batch solves
+    its nrhs[b] columns of n at rhs[rhs_off[b]:] in place (rhs_off int64, nrhs int32)."""
+    import torch
+    nb = mats.numel() // (n * n)
+    _lib.check(_lib.lib().raysht_lu_solve_full(int(n), nb, _dptr(mats, "mats"),
+                                               _iptr(pivots, "pivots", torch.int32, nb * n), _dptr(rhs, "rhs"),
+                                               _iptr(rhs_off, "rhs_off", torch.int64, nb),
+                                               _iptr(nrhs, "nrhs", torch.int32, nb), int(max_nrhs), _stream(stream)),
+               "raysht_lu_solve_full")
+
+
 class ChebyGrid:
     """Cheby_Grid with one domain of n zeros-grid points on [r_min, r_max]."""
 
