@@ -4,17 +4,15 @@
 // step size (LU_Decompose_full = LAPACK dgetrf, :1034-1045) and every step solves it for
 // the right-hand sides of all m of that l (LU_Solve_full = dgetrs 'N', :1087-1107).  Here
 // many matrices go in one launch, one workgroup per matrix (factorisation) or per
-// matrix and 16-column chunk of its right-hand sides (solve; n <= 256 with the columns in
-// registers, larger n with them in LDS).
+// matrix and chunk of its right-hand sides (solve).
 //
 // Factorisation: the unblocked right-looking algorithm of dgetf2 -- pivot = first row of
 // largest |a_ik| (a workgroup reduction), row swap, subdiagonal scaled by the pivot's
 // reciprocal, rank-1 update of the trailing block in global memory (L2) -- with
 // products and sums rounded separately.
-// Solve: the chunk of right-hand sides lives in LDS; the row interchanges, the unit lower
-// and the upper triangular solves run in the column order of the reference BLAS dtrsm
-// (column j final, then subtracted from the rows below / above it), one barrier per
-// column.
+// Solve: the row interchanges, the unit lower and the upper triangular solves in the
+// column order of the reference BLAS dtrsm (column j final, then subtracted from the rows
+// below / above it).
 #include <algorithm>
 #include <cmath>
 
@@ -98,68 +96,13 @@ __global__ __launch_bounds__(kLuThreads) void lu_decompose_kernel(int n, double 
     if (tid == 0) infos[blockIdx.x] = info;
 }
 
-// One workgroup per (matrix, chunk of CC right-hand-side columns); x in LDS as [c][n + 1].
-template <int CC>
-__global__ __launch_bounds__(kLuThreads) void lu_solve_kernel(int n, const double *__restrict__ mats,
-                                                              const int *__restrict__ pivots, double *__restrict__ rhs,
-                                                              const long *__restrict__ rhs_off,
-                                                              const int *__restrict__ nrhs)
-{
-#pragma clang fp contract(off)
-    extern __shared__ double xs[];
-    const int b = blockIdx.x, col0 = blockIdx.y * CC, ncol = nrhs[b];
-    if (col0 >= ncol) return;  // uniform: the whole workgroup leaves
-    const int cols = min(CC, ncol - col0);
-    const double *a = mats + (long)b * n * n;
-    const int *piv = pivots + (long)b * n;
-    double *x = rhs + rhs_off[b] + (long)col0 * n;
-    const int tid = threadIdx.x, ld = n + 1;
-    for (int e = tid; e < n * CC; e += kLuThreads) {
-        const int c = e / n, i = e - c * n;
-        xs[c * ld + i] = c < cols ? x[(long)c * n + i] : 0.0;
-    }
-    __syncthreads();
-    if (tid < cols) {  // dlaswp: the interchanges in order
-        double *xc = xs + tid * ld;
-        for (int k = 0; k < n; ++k) {
-            const int p = piv[k] - 1;
-            if (p != k && p >= 0 && p < n) {
-                const double t = xc[k];
-                xc[k] = xc[p];
-                xc[p] = t;
-            }
-        }
-    }
-    __syncthreads();
-    constexpr int RG = kLuThreads / CC;  // row groups
-    const int c = tid % CC, rg = tid / CC;
-    double *xc = xs + c * ld;
-    // unit lower: for j, x_i -= x_j l_ij (i > j)
-    for (int j = 0; j < n - 1; ++j) {
-        const double xj = xc[j];
-        if (xj != 0.0)
-            for (int i = j + 1 + rg; i < n; i += RG) xc[i] = xc[i] - xj * a[i + (long)j * n];
-        __syncthreads();
-    }
-    // upper: for j descending, x_j /= u_jj, then x_i -= x_j u_ij (i < j); x_j is final then
-    for (int j = n - 1; j >= 0; --j) {
-        double xj = xc[j];
-        if (xj != 0.0) {
-            xj = xj / a[j + (long)j * n];
-            for (int i = rg; i < j; i += RG) xc[i] = xc[i] - xj * a[i + (long)j * n];
-        }
-        if (rg == 0 && c < cols) x[(long)c * n + j] = xj;
-        __syncthreads();
-    }
-}
-
-// Solve with the right-hand sides in registers: 16 lanes of a wave own one column, lane
-// r holds rows r, r + 16, ... (NT of them).  The interchanges are composed into one
+// Solve with the right-hand sides in registers: G lanes of a wave own one column, lane
+// r holds rows r, r + G, ... (NT of them).  The interchanges are composed into one
 // permutation applied when the column is loaded; then each step j takes x_j from its
 // owner lane by a lane shuffle and updates the lane's rows in registers, while the next
 // column of L (or U) is loaded -- no barrier after the permutation.  Operation order and
-// rounding are those of the LDS kernel above.
-template <int NT>
+// rounding are those of the reference BLAS order.
+template <int G, int NT>
 __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const double *__restrict__ mats,
                                                                     const int *__restrict__ pivots,
                                                                     double *__restrict__ rhs,
@@ -167,8 +110,8 @@ __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const
                                                                     const int *__restrict__ nrhs)
 {
 #pragma clang fp contract(off)
-    __shared__ int perm[16 * NT];
-    constexpr int kCols = kLuThreads / 16;
+    __shared__ int perm[G * NT];
+    constexpr int kCols = kLuThreads / G;
     const int b = blockIdx.x, ncol = nrhs[b], col0 = blockIdx.y * kCols;
     if (col0 >= ncol) return;  // uniform: the whole workgroup leaves
     const double *a = mats + (long)b * n * n;
@@ -186,19 +129,19 @@ __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const
         }
     }
     __syncthreads();
-    const int lane = tid & 15, c = col0 + tid / 16;
+    const int lane = tid % G, c = col0 + tid / G;
     const bool live = c < ncol;
     double *xg = rhs + rhs_off[b] + (long)(live ? c : col0) * n;
     double x[NT], cur[NT], nxt[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const int i = 16 * t + lane;
+        const int i = G * t + lane;
         x[t] = live && i < n ? xg[perm[i]] : 0.0;
     }
     auto column = [&](int j, double(&v)[NT]) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const int i = 16 * t + lane;
+            const int i = G * t + lane;
             v[t] = i < n ? a[i + (long)j * n] : 0.0;
         }
     };
@@ -206,15 +149,15 @@ __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const
     column(0, cur);
 #pragma unroll
     for (int tb = 0; tb < NT; ++tb) {
-        for (int r = 0; r < 16; ++r) {
-            const int j = 16 * tb + r;
+        for (int r = 0; r < G; ++r) {
+            const int j = G * tb + r;
             if (j >= n - 1) break;
             column(j + 1, nxt);
-            const double xj = __shfl(x[tb], r, 16);
+            const double xj = __shfl(x[tb], r, G);
             if (xj != 0.0) {
 #pragma unroll
                 for (int t = 0; t < NT; ++t)
-                    if (16 * t + lane > j) x[t] = x[t] - xj * cur[t];
+                    if (G * t + lane > j) x[t] = x[t] - xj * cur[t];
             }
 #pragma unroll
             for (int t = 0; t < NT; ++t) cur[t] = nxt[t];
@@ -224,17 +167,17 @@ __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const
     column(n - 1, cur);
 #pragma unroll
     for (int tb = NT - 1; tb >= 0; --tb) {
-        for (int r = 15; r >= 0; --r) {
-            const int j = 16 * tb + r;
+        for (int r = G - 1; r >= 0; --r) {
+            const int j = G * tb + r;
             if (j >= n) continue;
             if (j > 0) column(j - 1, nxt);
-            double xj = __shfl(x[tb], r, 16);
-            const double ujj = __shfl(cur[tb], r, 16);
+            double xj = __shfl(x[tb], r, G);
+            const double ujj = __shfl(cur[tb], r, G);
             if (xj != 0.0) {
                 xj = xj / ujj;
 #pragma unroll
                 for (int t = 0; t < NT; ++t)
-                    if (16 * t + lane < j) x[t] = x[t] - xj * cur[t];
+                    if (G * t + lane < j) x[t] = x[t] - xj * cur[t];
             }
             if (lane == r) x[tb] = xj;
 #pragma unroll
@@ -244,10 +187,20 @@ __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const
     if (live) {
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const int i = 16 * t + lane;
+            const int i = G * t + lane;
             if (i < n) xg[i] = x[t];
         }
     }
+}
+
+template <int G, int NT>
+void launch_lanes(int n, int nbatch, const double *mats, const int *pivots, double *rhs, const long *rhs_off,
+                  const int *nrhs, int max_nrhs, hipStream_t st)
+{
+    constexpr int kCols = kLuThreads / G;
+    const dim3 grid((unsigned)nbatch, (unsigned)((max_nrhs + kCols - 1) / kCols));
+    hipLaunchKernelGGL((lu_solve_lanes_kernel<G, NT>), grid, dim3(kLuThreads), 0, st, n, mats, pivots, rhs, rhs_off,
+                       nrhs);
 }
 
 }  // namespace
@@ -276,26 +229,16 @@ int raysht_lu_solve_full(int n, int nbatch, const double *mats, const int *pivot
     if (n > kLuMaxN) return RAYSHT_EUNSUPPORTED;
     if (nbatch == 0 || max_nrhs == 0) return RAYSHT_OK;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    constexpr int kLaneCols = kLuThreads / 16;
-    const dim3 lgrid((unsigned)nbatch, (unsigned)((max_nrhs + kLaneCols - 1) / kLaneCols));
-    if (n <= 64) {
-        hipLaunchKernelGGL(lu_solve_lanes_kernel<4>, lgrid, dim3(kLuThreads), 0, st, n, mats, pivots, rhs, rhs_off, nrhs);
-    } else if (n <= 128) {
-        hipLaunchKernelGGL(lu_solve_lanes_kernel<8>, lgrid, dim3(kLuThreads), 0, st, n, mats, pivots, rhs, rhs_off, nrhs);
-    } else if (n <= 256) {
-        hipLaunchKernelGGL(lu_solve_lanes_kernel<16>, lgrid, dim3(kLuThreads), 0, st, n, mats, pivots, rhs, rhs_off,
-                           nrhs);
-    } else if (n <= 512) {
-        constexpr int CC = 16;
-        const size_t lds = (size_t)CC * (n + 1) * sizeof(double);
-        dim3 grid((unsigned)nbatch, (unsigned)((max_nrhs + CC - 1) / CC));
-        hipLaunchKernelGGL(lu_solve_kernel<CC>, grid, dim3(kLuThreads), lds, st, n, mats, pivots, rhs, rhs_off, nrhs);
-    } else {
-        constexpr int CC = 8;
-        const size_t lds = (size_t)CC * (n + 1) * sizeof(double);
-        dim3 grid((unsigned)nbatch, (unsigned)((max_nrhs + CC - 1) / CC));
-        hipLaunchKernelGGL(lu_solve_kernel<CC>, grid, dim3(kLuThreads), lds, st, n, mats, pivots, rhs, rhs_off, nrhs);
-    }
+    if (n <= 64)
+        launch_lanes<16, 4>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+    else if (n <= 128)
+        launch_lanes<16, 8>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+    else if (n <= 256)
+        launch_lanes<16, 16>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+    else if (n <= 512)
+        launch_lanes<32, 16>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+    else
+        launch_lanes<64, 16>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
