@@ -173,7 +173,7 @@ __global__ __launch_bounds__(STP_THREADS, RAYSHT_FP64_MIN_WAVES) void stp_fp64_k
             stash(k0);
             __syncthreads();
             if (k0 + STP_KC < cur.ne) fetch(cur, k0 + STP_KC);
-#pragma unroll 1
+#pragma unroll 2
             for (int kk = 0; kk < STP_KC; kk += 4) {
                 double be[4], bo[4];
 #pragma unroll
