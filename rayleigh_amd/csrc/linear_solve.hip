@@ -8,8 +8,9 @@
 //
 // Factorisation: the unblocked right-looking algorithm of dgetf2 -- pivot = first row of
 // largest |a_ik| (a workgroup reduction), row swap, subdiagonal scaled by the pivot's
-// reciprocal, rank-1 update of the trailing block in global memory (L2) -- with
-// products and sums rounded separately.
+// reciprocal, rank-1 update of the trailing block -- with products and sums rounded
+// separately.  The matrix stays in global memory (L2); per column the pivot row goes to
+// LDS and each thread sweeps whole rows of the trailing block (coalesced across threads).
 // Solve: the row interchanges, the unit lower and the upper triangular solves in the
 // column order of the reference BLAS dtrsm (column j final, then subtracted from the rows
 // below / above it).
@@ -30,6 +31,7 @@ __global__ __launch_bounds__(kLuThreads) void lu_decompose_kernel(int n, double 
 #pragma clang fp contract(off)
     __shared__ double s_val[kLuThreads];
     __shared__ int s_idx[kLuThreads];
+    __shared__ double rowk[kLuMaxN];  // the pivot row, columns k + 1 .. n - 1
     double *a = mats + (long)blockIdx.x * n * n;
     int *piv = pivots + (long)blockIdx.x * n;
     const int tid = threadIdx.x;
@@ -71,25 +73,36 @@ __global__ __launch_bounds__(kLuThreads) void lu_decompose_kernel(int n, double 
                     a[k + (long)j * n] = a[p + (long)j * n];
                     a[p + (long)j * n] = t;
                 }
-            __syncthreads();
-            if (fabs(pv) >= sfmin) {
-                const double r = 1.0 / pv;
-                for (int i = k + 1 + tid; i < n; i += kLuThreads) a[i + (long)k * n] *= r;
-            } else {
-                for (int i = k + 1 + tid; i < n; i += kLuThreads) a[i + (long)k * n] /= pv;
-            }
         } else if (info == 0) {
             info = k + 1;
         }
         __syncthreads();
-        // trailing block: a_ij + a_ik (-a_kj), i, j > k, column-major sweep (i fastest)
-        const int m = n - k - 1;
-        const long mm = (long)m * m;
-        for (long e = tid; e < mm; e += kLuThreads) {
-            const int jj = (int)(e / m), ii = (int)(e - (long)jj * m);
-            const int i = k + 1 + ii, j = k + 1 + jj;
-            const double y = a[k + (long)j * n];
-            if (y != 0.0) a[i + (long)j * n] = a[i + (long)j * n] + a[i + (long)k * n] * -y;
+        for (int j = k + 1 + tid; j < n; j += kLuThreads) rowk[j] = a[k + (long)j * n];
+        __syncthreads();
+        // each thread: its rows i > k -- scale a_ik by the pivot's reciprocal, then sweep the
+        // row, a_ij + a_ik (-a_kj), four columns at a time (loads issued together)
+        const double r = pv != 0.0 && fabs(pv) >= sfmin ? 1.0 / pv : 0.0;
+        for (int i = k + 1 + tid; i < n; i += kLuThreads) {
+            double lik = a[i + (long)k * n];
+            if (pv != 0.0) {
+                lik = fabs(pv) >= sfmin ? lik * r : lik / pv;
+                a[i + (long)k * n] = lik;
+            }
+            double *ai = a + i;
+            int j = k + 1;
+            for (; j + 3 < n; j += 4) {
+                const double v0 = ai[(long)j * n], v1 = ai[(long)(j + 1) * n];
+                const double v2 = ai[(long)(j + 2) * n], v3 = ai[(long)(j + 3) * n];
+                const double y0 = rowk[j], y1 = rowk[j + 1], y2 = rowk[j + 2], y3 = rowk[j + 3];
+                if (y0 != 0.0) ai[(long)j * n] = v0 + lik * -y0;
+                if (y1 != 0.0) ai[(long)(j + 1) * n] = v1 + lik * -y1;
+                if (y2 != 0.0) ai[(long)(j + 2) * n] = v2 + lik * -y2;
+                if (y3 != 0.0) ai[(long)(j + 3) * n] = v3 + lik * -y3;
+            }
+            for (; j < n; ++j) {
+                const double y = rowk[j];
+                if (y != 0.0) ai[(long)j * n] = ai[(long)j * n] + lik * -y;
+            }
         }
         __syncthreads();
     }
