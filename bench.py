@@ -199,6 +199,57 @@ def reference_check_and_baseline(res, mvals, sample_m: int):
     return cpu, err_stp, err_pts
 
 
+def cpu_reference_extras(L, mv, nr, nf, nlm, n_lu, ncols, out):
+    """The reference's CPU path beside the device figures of the radial derivative, the
+    angular derivative and the implicit solve, each on a bounded sample scaled to the full
+    size: the compiled reference routines (oracle/_ref, single-threaded; the harness's copies
+    into and out of the reference's containers are included) and LAPACK dgetrf / dgetrs
+    (the library the reference's solve calls, numpy's OpenBLAS here, 16 threads)."""
+    import scipy.linalg.lapack as lapack
+    from threadpoolctl import threadpool_limits
+    from oracle import oracle as O
+    res = {}
+    rng = np.random.default_rng(0)
+    if O.ref_available():
+        R = O.Reference()
+        cols = 2 * nlm * nf // 32
+        R.cheby_init(nr, 0.35, 1.0)
+        c = rng.standard_normal(nr * cols)
+        t0 = time.perf_counter()
+        R.cheby_deriv(c, 1)
+        t = time.perf_counter() - t0
+        res["d_by_dr_cp"] = {"ms_full_size": t * 32 * 1e3, "cores": 1, "kind": "reference",
+                             "sample": f"order 1 on {cols} of {2 * nlm * nf} columns of {nr}"}
+        R.angular_init(L, mv, nr)
+        buf = rng.standard_normal(O.spectral_size(L, mv, nr, 2))
+        t0 = time.perf_counter()
+        R.angular_deriv("dtheta", buf, 2, 1, 0)
+        t = time.perf_counter() - t0
+        res["d_by_dtheta"] = {"ms_full_size": t * 1e3, "cores": 1, "kind": "reference",
+                              "sample": "one field, the full size (a two-field buffer passed in)"}
+    threads = min(16, len(os.sched_getaffinity(0)))
+    k = 16
+    mats = [rng.standard_normal((n_lu, n_lu)) + n_lu * np.eye(n_lu) for _ in range(k)]
+    sel = np.linspace(0, len(ncols) - 1, k).round().astype(int)
+    with threadpool_limits(limits=threads):
+        t0 = time.perf_counter()
+        facs = [lapack.dgetrf(a) for a in mats]
+        t_f = time.perf_counter() - t0
+        rhs = [rng.standard_normal((n_lu, ncols[i])) for i in sel]
+        t0 = time.perf_counter()
+        for (lu, piv, _), b in zip(facs, rhs):
+            lapack.dgetrs(lu, piv, b)
+        t_s = time.perf_counter() - t0
+    frac = sum(ncols[i] for i in sel) / sum(ncols)
+    res["lu_decompose_full"] = {"ms_full_size": t_f * len(ncols) / k * 1e3, "cores": threads, "kind": "LAPACK",
+                                "sample": f"dgetrf on {k} of {len(ncols)} matrices of n = {n_lu}"}
+    res["lu_solve_full"] = {"ms_full_size": t_s / frac * 1e3, "cores": threads, "kind": "LAPACK",
+                            "sample": f"dgetrs on {k} of {len(ncols)} matrices and their right-hand sides"}
+    for key, v in res.items():
+        log(f"cpu reference {key}: {v['ms_full_size']:.1f} ms at full size ({v['sample']})")
+    return res
+
+
 STACKS = {
     # l_max, N_theta, N_phi, N_r, N_f
     "stack": (255, 384, 768, 128, 8),     # full FP64 stack at l_max = 255
@@ -342,6 +393,7 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
                         "frac_fp64_peak": flop / (ms * 1e-3) / (PEAK_FP64_TFLOPS * 1e12)}
     out["implicit_solve"] = {"n": n_lu, "matrices": nb, **lu_rec}
     del mats0, mats, rhs0, rhs
+    out["cpu_reference"] = cpu_reference_extras(L, mv, nr, nf, nlm, n_lu, ncols, out)
     # the radial derivative recursion (d_by_dr_cp) of orders 1-3 on the spectral radial
     # buffer, as Post_Solve applies it: 16 B per coefficient (read + write)
     rad3 = torch.empty_like(rad)
