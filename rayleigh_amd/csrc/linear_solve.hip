@@ -109,13 +109,13 @@ __global__ __launch_bounds__(kLuThreads) void lu_decompose_kernel(int n, double 
     if (tid == 0) infos[blockIdx.x] = info;
 }
 
-// Solve with the right-hand sides in registers: G lanes of a wave own one column, lane
-// r holds rows r, r + G, ... (NT of them).  The interchanges are composed into one
+// Solve with the right-hand sides in registers: G lanes of a wave own CPL columns, lane
+// r holds rows r, r + G, ... (NT of them) of each.  The interchanges are composed into one
 // permutation applied when the column is loaded; then each step j takes x_j from its
 // owner lane by a lane shuffle and updates the lane's rows in registers, while the next
 // column of L (or U) is loaded -- no barrier after the permutation.  Operation order and
 // rounding are those of the reference BLAS order.
-template <int G, int NT>
+template <int G, int NT, int CPL>
 __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const double *__restrict__ mats,
                                                                     const int *__restrict__ pivots,
                                                                     double *__restrict__ rhs,
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const
 {
 #pragma clang fp contract(off)
     __shared__ int perm[G * NT];
-    constexpr int kCols = kLuThreads / G;
+    constexpr int kCols = kLuThreads / G * CPL;
     const int b = blockIdx.x, ncol = nrhs[b], col0 = blockIdx.y * kCols;
     if (col0 >= ncol) return;  // uniform: the whole workgroup leaves
     const double *a = mats + (long)b * n * n;
@@ -142,14 +142,17 @@ __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const
         }
     }
     __syncthreads();
-    const int lane = tid % G, c = col0 + tid / G;
-    const bool live = c < ncol;
-    double *xg = rhs + rhs_off[b] + (long)(live ? c : col0) * n;
-    double x[NT], cur[NT], nxt[NT];
+    const int lane = tid % G, cg = col0 + (tid / G) * CPL;
+    double x[CPL][NT], cur[NT], nxt[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int i = G * t + lane;
-        x[t] = live && i < n ? xg[perm[i]] : 0.0;
+    for (int q = 0; q < CPL; ++q) {
+        const bool live = cg + q < ncol;
+        const double *xg = rhs + rhs_off[b] + (long)(live ? cg + q : col0) * n;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int i = G * t + lane;
+            x[q][t] = live && i < n ? xg[perm[i]] : 0.0;
+        }
     }
     auto column = [&](int j, double(&v)[NT]) {
 #pragma unroll
@@ -166,11 +169,14 @@ __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const
             const int j = G * tb + r;
             if (j >= n - 1) break;
             column(j + 1, nxt);
-            const double xj = __shfl(x[tb], r, G);
-            if (xj != 0.0) {
 #pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    if (G * t + lane > j) x[t] = x[t] - xj * cur[t];
+            for (int q = 0; q < CPL; ++q) {
+                const double xj = __shfl(x[q][tb], r, G);
+                if (xj != 0.0) {
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        if (G * t + lane > j) x[q][t] = x[q][t] - xj * cur[t];
+                }
             }
 #pragma unroll
             for (int t = 0; t < NT; ++t) cur[t] = nxt[t];
@@ -184,36 +190,44 @@ __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const
             const int j = G * tb + r;
             if (j >= n) continue;
             if (j > 0) column(j - 1, nxt);
-            double xj = __shfl(x[tb], r, G);
             const double ujj = __shfl(cur[tb], r, G);
-            if (xj != 0.0) {
-                xj = xj / ujj;
 #pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    if (G * t + lane < j) x[t] = x[t] - xj * cur[t];
+            for (int q = 0; q < CPL; ++q) {
+                double xj = __shfl(x[q][tb], r, G);
+                if (xj != 0.0) {
+                    xj = xj / ujj;
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        if (G * t + lane < j) x[q][t] = x[q][t] - xj * cur[t];
+                }
+                if (lane == r) x[q][tb] = xj;
             }
-            if (lane == r) x[tb] = xj;
 #pragma unroll
             for (int t = 0; t < NT; ++t) cur[t] = nxt[t];
         }
     }
-    if (live) {
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int i = G * t + lane;
-            if (i < n) xg[i] = x[t];
+    for (int q = 0; q < CPL; ++q)
+        if (cg + q < ncol) {
+            double *xg = rhs + rhs_off[b] + (long)(cg + q) * n;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int i = G * t + lane;
+                if (i < n) xg[i] = x[q][t];
+            }
         }
-    }
 }
 
-template <int G, int NT>
+// CPL right-hand sides per lane group share every loaded column of the factor (measured:
+// 2 for 16-lane groups, 3 for 32- and 64-lane groups, within the register budget)
+template <int G, int NT, int CPL>
 void launch_lanes(int n, int nbatch, const double *mats, const int *pivots, double *rhs, const long *rhs_off,
                   const int *nrhs, int max_nrhs, hipStream_t st)
 {
-    constexpr int kCols = kLuThreads / G;
+    constexpr int kCols = kLuThreads / G * CPL;
     const dim3 grid((unsigned)nbatch, (unsigned)((max_nrhs + kCols - 1) / kCols));
-    hipLaunchKernelGGL((lu_solve_lanes_kernel<G, NT>), grid, dim3(kLuThreads), 0, st, n, mats, pivots, rhs, rhs_off,
-                       nrhs);
+    hipLaunchKernelGGL((lu_solve_lanes_kernel<G, NT, CPL>), grid, dim3(kLuThreads), 0, st, n, mats, pivots, rhs,
+                       rhs_off, nrhs);
 }
 
 }  // namespace
@@ -243,15 +257,15 @@ int raysht_lu_solve_full(int n, int nbatch, const double *mats, const int *pivot
     if (nbatch == 0 || max_nrhs == 0) return RAYSHT_OK;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if (n <= 64)
-        launch_lanes<16, 4>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        launch_lanes<16, 4, 2>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else if (n <= 128)
-        launch_lanes<16, 8>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        launch_lanes<16, 8, 2>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else if (n <= 256)
-        launch_lanes<16, 16>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        launch_lanes<16, 16, 2>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else if (n <= 512)
-        launch_lanes<32, 16>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        launch_lanes<32, 16, 3>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else
-        launch_lanes<64, 16>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        launch_lanes<64, 16, 3>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
