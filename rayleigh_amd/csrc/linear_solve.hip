@@ -10,7 +10,8 @@
 // largest |a_ik| (a workgroup reduction), row swap, subdiagonal scaled by the pivot's
 // reciprocal, rank-1 update of the trailing block -- with products and sums rounded
 // separately.  The matrix stays in global memory (L2); per column the pivot row goes to
-// LDS and each thread sweeps whole rows of the trailing block (coalesced across threads).
+// LDS and each thread sweeps whole rows of the trailing block (coalesced across threads,
+// sixteen loads in flight per thread).
 // Solve: the row interchanges, the unit lower and the upper triangular solves in the
 // column order of the reference BLAS dtrsm (column j final, then subtracted from the rows
 // below / above it).
@@ -24,6 +25,7 @@ namespace {
 
 constexpr int kLuThreads = 256;
 constexpr int kLuMaxN = 1024;
+constexpr int kLuBatch = 16;  // row loads in flight per thread in the factorisation sweep (4: 1.3x slower)
 
 __global__ __launch_bounds__(kLuThreads) void lu_decompose_kernel(int n, double *__restrict__ mats,
                                                                   int *__restrict__ pivots, int *__restrict__ infos)
@@ -90,14 +92,15 @@ __global__ __launch_bounds__(kLuThreads) void lu_decompose_kernel(int n, double 
             }
             double *ai = a + i;
             int j = k + 1;
-            for (; j + 3 < n; j += 4) {
-                const double v0 = ai[(long)j * n], v1 = ai[(long)(j + 1) * n];
-                const double v2 = ai[(long)(j + 2) * n], v3 = ai[(long)(j + 3) * n];
-                const double y0 = rowk[j], y1 = rowk[j + 1], y2 = rowk[j + 2], y3 = rowk[j + 3];
-                if (y0 != 0.0) ai[(long)j * n] = v0 + lik * -y0;
-                if (y1 != 0.0) ai[(long)(j + 1) * n] = v1 + lik * -y1;
-                if (y2 != 0.0) ai[(long)(j + 2) * n] = v2 + lik * -y2;
-                if (y3 != 0.0) ai[(long)(j + 3) * n] = v3 + lik * -y3;
+            for (; j + kLuBatch - 1 < n; j += kLuBatch) {
+                double v[kLuBatch];
+#pragma unroll
+                for (int u = 0; u < kLuBatch; ++u) v[u] = ai[(long)(j + u) * n];
+#pragma unroll
+                for (int u = 0; u < kLuBatch; ++u) {
+                    const double y = rowk[j + u];
+                    if (y != 0.0) ai[(long)(j + u) * n] = v[u] + lik * -y;
+                }
             }
             for (; j < n; ++j) {
                 const double y = rowk[j];
