@@ -147,6 +147,12 @@ int raysht_d_by_dtheta(raysht_plan *plan, int kind, const double *in, int nf_in,
  * (re, im) -> (-m im, m re); in place (same array, fin = fout) allowed. */
 int raysht_d_by_dphi(raysht_plan *plan, const double *in, int nf_in, int fin, double *out, int nf_out, int fout,
                      void *stream);
+/* Host-array form of the three (synchronous), what the Fortran d_by_dtheta / d_by_sdtheta /
+ * d_by_dphi generics call: in_m[mp] / out_m[mp] point at each m-block's data(m:l_max, 1:N_r,
+ * 1:2, 1:nf) (kind 0 dtheta, 1 sdtheta, 2 dphi; 0-based fields); the same pointers for in
+ * and out (same nf) is the buffer form, other fields of a separate out are kept. */
+int raysht_angular_deriv_host(raysht_plan *plan, int kind, const double *const *in_m, int nf_in, int fin,
+                              double *const *out_m, int nf_out, int fout);
 
 /* Implicit_Solve's dense path (Linear_Solve.F90:423-457) for many matrices at once.
  * raysht_lu_decompose_full = LU_Decompose_full (:1034-1045, LAPACK dgetrf) on nbatch n x n

@@ -138,6 +138,49 @@ int raysht_cheb_from_spectral_host(raysht_cheb *cb, const double *c, double *f, 
     return RAYSHT_OK;
 }
 
+int raysht_angular_deriv_host(raysht_plan *p, int kind, const double *const *in_m, int nf_in, int fin,
+                              double *const *out_m, int nf_out, int fout)
+{
+    if (!p || !in_m || !out_m || kind < 0 || kind > 2 || nf_in < 1 || nf_out < 1) return RAYSHT_EINVAL;
+    if (p->nr < 1 || p->nf < 1) return RAYSHT_EUNSUPPORTED;
+    bool same = nf_in == nf_out;
+    for (int mp = 0; mp < p->nm; ++mp) {
+        if (!in_m[mp] || !out_m[mp]) return RAYSHT_EINVAL;
+        same = same && in_m[mp] == out_m[mp];
+    }
+    const long per_field = p->s_off[p->nm] / p->nf;  // doubles of one field over all m-blocks
+    const long blk_field = 2L * p->nr;               // doubles of one field per l
+    scoped_device_buffer din, dout;
+    if (hipMalloc(&din.ptr, (size_t)per_field * nf_in * sizeof(double)) != hipSuccess) return RAYSHT_ENOMEM;
+    if (!same && hipMalloc(&dout.ptr, (size_t)per_field * nf_out * sizeof(double)) != hipSuccess)
+        return RAYSHT_ENOMEM;
+    long off_in = 0, off_out = 0;
+    for (int mp = 0; mp < p->nm; ++mp) {
+        const long nl = p->lmax - p->mvals[mp] + 1;
+        RAYSHT_HIP_TRY(hipMemcpy(din.ptr + off_in, in_m[mp], (size_t)nl * blk_field * nf_in * sizeof(double),
+                                 hipMemcpyHostToDevice));
+        // the other fields of a separate output keep their values
+        if (!same && nf_out > 1)
+            RAYSHT_HIP_TRY(hipMemcpy(dout.ptr + off_out, out_m[mp], (size_t)nl * blk_field * nf_out * sizeof(double),
+                                     hipMemcpyHostToDevice));
+        off_in += nl * blk_field * nf_in;
+        off_out += nl * blk_field * nf_out;
+    }
+    double *out = same ? din.ptr : dout.ptr;
+    const int rc = kind == 2 ? raysht_d_by_dphi(p, din.ptr, nf_in, fin, out, nf_out, fout, nullptr)
+                             : raysht_d_by_dtheta(p, kind, din.ptr, nf_in, fin, out, nf_out, fout, nullptr);
+    if (rc != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipDeviceSynchronize());
+    off_out = 0;
+    for (int mp = 0; mp < p->nm; ++mp) {
+        const long nl = p->lmax - p->mvals[mp] + 1;
+        RAYSHT_HIP_TRY(hipMemcpy(out_m[mp], out + off_out, (size_t)nl * blk_field * nf_out * sizeof(double),
+                                 hipMemcpyDeviceToHost));
+        off_out += nl * blk_field * nf_out;
+    }
+    return RAYSHT_OK;
+}
+
 int raysht_cheb_deriv_host(raysht_cheb *cb, const double *c, double *d, long ncols, int dorder)
 {
     if (!cb || ncols < 0 || (ncols > 0 && (!c || !d))) return RAYSHT_EINVAL;

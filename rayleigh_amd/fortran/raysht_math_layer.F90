@@ -15,6 +15,7 @@
 !   Cheby_To_Spectral(f_in, c_out) / Cheby_From_Spectral(c_in, f_out) after Cheby_Init(n, rmin, rmax, grid)
 !                                                     Chebyshev_Polynomials.F90:65-558 (one domain)
 !   Cheby_d_by_dr_cp(ind, dind, buffer, dorder)      Chebyshev_Polynomials.F90:560-640
+!   d_by_dtheta / d_by_sdtheta / d_by_dphi           Spectral_Derivatives.F90 (rmcontainer4d forms)
 !
 ! The data stay in the caller's host arrays: each call hands their addresses to
 ! the library's host-array entry points, which stage through device memory.
@@ -25,13 +26,14 @@
 !
 Module RaySHT_Math_Layer
     Use ISO_C_BINDING
-    Use Structures, Only : rmcontainer, rmcontainer4d
+    Use Structures, Only : rmcontainer, rmcontainer3d, rmcontainer4d
     Implicit None
     Private
 
     Public :: Initialize_Legendre, Finalize_Legendre, Legendre_Transform, Set_Legendre_Precision
     Public :: FFT_To_Physical, FFT_To_Spectral, FFT_To_Spectral_RSC
     Public :: Cheby_Init, Cheby_To_Spectral, Cheby_From_Spectral, Cheby_Finalize, Cheby_d_by_dr_cp
+    Public :: d_by_dtheta, d_by_sdtheta, d_by_dphi
     Public :: coloc, gl_weights
     ! arithmetic of the Legendre contractions (enum raysht_precision of include/raysht.h)
     Integer, Parameter, Public :: raysht_fp64 = 0, raysht_bf16x3 = 1, raysht_bf16x6 = 2, &
@@ -132,6 +134,14 @@ Module RaySHT_Math_Layer
             Integer(C_INT), Value :: dorder
             Integer(C_INT) :: st
         End Function
+        Function c_ang_host(plan, kind, in_m, nf_in, fin, out_m, nf_out, fout) &
+            Bind(C, name='raysht_angular_deriv_host') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan
+            Integer(C_INT), Value :: kind, nf_in, fin, nf_out, fout
+            Type(C_PTR), Intent(In) :: in_m(*), out_m(*)
+            Integer(C_INT) :: st
+        End Function
         Function c_strerror(status) Bind(C, name='raysht_strerror') Result(msg)
             Import :: C_PTR, C_INT
             Integer(C_INT), Value :: status
@@ -146,6 +156,19 @@ Module RaySHT_Math_Layer
 
     Interface FFT_To_Spectral
         Module Procedure r2c_ip4d_raysht
+    End Interface
+
+    ! Spectral_Derivatives.F90's generics on the new (rmcontainer4d) layout: field fin -> field
+    ! fout of the same buffer (d_dtheta_buffer4D ...), or field fin -> a rmcontainer3d array
+    ! (d_dtheta_4dbuff2arr ...).  They use the m values and l_max of Initialize_Legendre.
+    Interface d_by_dtheta
+        Module Procedure dtheta_buffer4d_raysht, dtheta_4dbuff2arr_raysht
+    End Interface
+    Interface d_by_sdtheta
+        Module Procedure sdtheta_buffer4d_raysht, sdtheta_4dbuff2arr_raysht
+    End Interface
+    Interface d_by_dphi
+        Module Procedure dphi_buffer4d_raysht, dphi_4dbuff2arr_raysht
     End Interface
 
     Interface FFT_To_Spectral_RSC
@@ -418,5 +441,74 @@ Contains
         Call check(c_cheb_deriv(cheb, c_loc(buffer(1, 1, 1, ind)), c_loc(buffer(1, 1, 1, dind)), ncols, &
                                 int(dorder, C_INT)), 'Cheby_d_by_dr_cp')
     End Subroutine Cheby_d_by_dr_cp
+
+    ! the buffer forms: field fin -> field fout (1-based) of the same rmcontainer4d array
+    Subroutine angular_buffer4d(kind, A, fin, fout)
+        Integer, Intent(In) :: kind, fin, fout
+        Type(rmcontainer4d), Intent(InOut), Target :: A(:)
+        Type(C_PTR) :: ptrs(size(A))
+        Integer :: mp, nr, nf
+        nr = size(A(1)%data, 2)
+        nf = size(A(1)%data, 4)
+        Do mp = 1, size(A)
+            ptrs(mp) = c_loc(A(mp)%data)
+        Enddo
+        Call check(c_ang_host(plan_for(nr, nf), int(kind, C_INT), ptrs, int(nf, C_INT), int(fin - 1, C_INT), ptrs, &
+                              int(nf, C_INT), int(fout - 1, C_INT)), 'angular derivative')
+    End Subroutine angular_buffer4d
+
+    ! the 4dbuff2arr forms: field fin of A -> arr(mp)%data(m:lmax, 1:nr, 1:2)
+    Subroutine angular_4dbuff2arr(kind, A, fin, arr)
+        Integer, Intent(In) :: kind, fin
+        Type(rmcontainer4d), Intent(InOut), Target :: A(:)
+        Type(rmcontainer3d), Intent(InOut), Target :: arr(:)
+        Type(C_PTR) :: pin(size(A)), pout(size(A))
+        Integer :: mp, nr, nf
+        nr = size(A(1)%data, 2)
+        nf = size(A(1)%data, 4)
+        Do mp = 1, size(A)
+            pin(mp) = c_loc(A(mp)%data)
+            pout(mp) = c_loc(arr(mp)%data)
+        Enddo
+        Call check(c_ang_host(plan_for(nr, nf), int(kind, C_INT), pin, int(nf, C_INT), int(fin - 1, C_INT), pout, &
+                              1_C_INT, 0_C_INT), 'angular derivative')
+    End Subroutine angular_4dbuff2arr
+
+    ! d_dtheta_buffer4D (Spectral_Derivatives.F90:224-249) / d_dtheta_4dbuff2arr (:329-350)
+    Subroutine dtheta_buffer4d_raysht(A, fin, fout)
+        Type(rmcontainer4d), Intent(InOut), Target :: A(:)
+        Integer, Intent(In) :: fin, fout
+        Call angular_buffer4d(0, A, fin, fout)
+    End Subroutine dtheta_buffer4d_raysht
+    Subroutine dtheta_4dbuff2arr_raysht(A, fin, arr)
+        Type(rmcontainer4d), Intent(InOut), Target :: A(:)
+        Integer, Intent(In) :: fin
+        Type(rmcontainer3d), Intent(InOut), Target :: arr(:)
+        Call angular_4dbuff2arr(0, A, fin, arr)
+    End Subroutine dtheta_4dbuff2arr_raysht
+    ! d_sdtheta_buffer4d (:277-298) / d_sdtheta_4dbuff2arr (:381-401)
+    Subroutine sdtheta_buffer4d_raysht(A, fin, fout)
+        Type(rmcontainer4d), Intent(InOut), Target :: A(:)
+        Integer, Intent(In) :: fin, fout
+        Call angular_buffer4d(1, A, fin, fout)
+    End Subroutine sdtheta_buffer4d_raysht
+    Subroutine sdtheta_4dbuff2arr_raysht(A, fin, arr)
+        Type(rmcontainer4d), Intent(InOut), Target :: A(:)
+        Integer, Intent(In) :: fin
+        Type(rmcontainer3d), Intent(InOut), Target :: arr(:)
+        Call angular_4dbuff2arr(1, A, fin, arr)
+    End Subroutine sdtheta_4dbuff2arr_raysht
+    ! d_by_dphi_rlmbuff4d (:573-603, in place allowed) / d_by_dphi_4dbuff2arr (:495-509)
+    Subroutine dphi_buffer4d_raysht(A, fin, fout)
+        Type(rmcontainer4d), Intent(InOut), Target :: A(:)
+        Integer, Intent(In) :: fin, fout
+        Call angular_buffer4d(2, A, fin, fout)
+    End Subroutine dphi_buffer4d_raysht
+    Subroutine dphi_4dbuff2arr_raysht(A, fin, arr)
+        Type(rmcontainer4d), Intent(InOut), Target :: A(:)
+        Integer, Intent(In) :: fin
+        Type(rmcontainer3d), Intent(InOut), Target :: arr(:)
+        Call angular_4dbuff2arr(2, A, fin, arr)
+    End Subroutine dphi_4dbuff2arr_raysht
 
 End Module RaySHT_Math_Layer

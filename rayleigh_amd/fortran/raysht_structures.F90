@@ -10,6 +10,10 @@ Module Structures
         Real*8, Allocatable :: data(:,:)
     End Type rmcontainer
 
+    Type, Public :: rmcontainer3d
+        Real*8, Allocatable :: data(:,:,:)
+    End Type rmcontainer3d
+
     Type, Public :: rmcontainer4d
         Real*8, Allocatable :: data(:,:,:,:)
     End Type rmcontainer4d

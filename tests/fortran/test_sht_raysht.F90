@@ -28,6 +28,7 @@ Program test_sht_raysht
     Call two_d_pair(nfail)
     Call cheby_round_trip(nfail)
     Call cheby_derivatives(nfail)
+    Call angular_derivatives(nfail)
     ! the split forms with 6 digits stay within the same tolerance over ten round trips
     Call Set_Legendre_Precision(raysht_int8x6)
     Call white_noise(nfail)
@@ -287,6 +288,48 @@ Contains
         Call Cheby_Finalize()
         Call report('Chebyshev round trip', d, nfail)
     End Subroutine cheby_round_trip
+
+    ! Spectral_Derivatives.F90 through the generics: field 1 = Y_20 (m = 0, real) and Y_11
+    ! (m = 1, real); d_by_dtheta into field 2 and d_by_sdtheta into a separate array against
+    ! sin(t) dY_lm/dt = l c_{l+1,m} Y_{l+1,m} - (l+1) c_lm Y_{l-1,m} (c_lm = sqrt((l^2-m^2)/(4l^2-1)))
+    ! and 1/sin(t) d(sin^2(t) Y)/dt = (l+2) c_{l+1,m} Y_{l+1,m} - (l-1) c_lm Y_{l-1,m};
+    ! d_by_dphi in place: i m Y_11 -> imaginary part m
+    Subroutine angular_derivatives(nfail)
+        Integer, Intent(InOut) :: nfail
+        Type(rmcontainer4d), Allocatable :: s(:)
+        Type(rmcontainer3d), Allocatable :: arr(:)
+        Real*8 :: d, c20, c30, c21, want(0:lmax)
+        Integer :: mp, m0, m1
+        Call alloc_spec(s, 2)
+        Allocate(arr(lmax+1))
+        Do mp = 1, lmax+1
+            Allocate(arr(mp)%data(mvals(mp):lmax, 1:nr, 1:2))
+            arr(mp)%data = 0.0d0
+            If (mvals(mp) == 0) m0 = mp
+            If (mvals(mp) == 1) m1 = mp
+        Enddo
+        s(m0)%data(2, :, 1, 1) = 1.0d0
+        s(m1)%data(1, :, 1, 1) = 1.0d0
+        c20 = sqrt(4.0d0 / 15.0d0)   ! c_{2,0}
+        c30 = sqrt(9.0d0 / 35.0d0)   ! c_{3,0}
+        c21 = sqrt(3.0d0 / 15.0d0)   ! c_{2,1}
+        Call d_by_dtheta(s, 1, 2)
+        want = 0.0d0
+        want(1) = -3.0d0 * c20
+        want(3) = 2.0d0 * c30
+        d = maxval(abs(s(m0)%data(0:lmax, 1, 1, 2) - want(0:lmax)))
+        want = 0.0d0
+        want(2) = 1.0d0 * c21        ! l = 1, m = 1: l c_{2,1} Y_21 (no l - 1 = 0 term)
+        d = max(d, maxval(abs(s(m1)%data(1:lmax, 2, 1, 2) - want(1:lmax))))
+        Call d_by_sdtheta(s, 1, arr)
+        want = 0.0d0
+        want(1) = -1.0d0 * c20
+        want(3) = 4.0d0 * c30
+        d = max(d, maxval(abs(arr(m0)%data(0:lmax, 3, 1) - want(0:lmax))))
+        Call d_by_dphi(s, 1, 1)
+        d = max(d, abs(s(m1)%data(1, 4, 1, 1)) + abs(s(m1)%data(1, 4, 2, 1) - 1.0d0))
+        Call report('Angular derivatives', d, nfail)
+    End Subroutine angular_derivatives
 
     ! Cheby_Deriv_Buffer_4D (d_by_dr_cp, Chebyshev_Polynomials.F90:560-640) as Test_Cheby's
     ! derivative check uses it: variables 2..4 of the buffer receive the first three
