@@ -35,10 +35,19 @@ __global__ __launch_bounds__(kAThreads) void dtheta_kernel(int lmax, int nr, con
     const double *a = in + lb * ncol * nf_in + (long)fin * total;
     double *b = out + lb * ncol * nf_out + (long)fout * total;
     const double2 *c = coef + lb;
+    const float rnl = 1.0f / (float)nl;
     for (long e = blockIdx.x * (long)kAThreads + threadIdx.x; e < total; e += (long)gridDim.x * kAThreads) {
-        const long k = e / nl;
-        const int i = (int)(e - k * nl);  // l - m
-        const double *ak = a + k * nl;
+        // column k = e / nl from a float estimate and one exact correction (e < 2^31)
+        int k = (int)((float)e * rnl);
+        int i = (int)e - k * nl;  // l - m
+        if (i < 0) {
+            --k;
+            i += nl;
+        } else if (i >= nl) {
+            ++k;
+            i -= nl;
+        }
+        const double *ak = a + (long)k * nl;
         double v;
         if (nl == 1)
             v = 0.0;
@@ -48,7 +57,7 @@ __global__ __launch_bounds__(kAThreads) void dtheta_kernel(int lmax, int nr, con
             v = ak[i - 1] * c[i].x;
         else
             v = ak[i - 1] * c[i].x + ak[i + 1] * c[i].y;
-        b[k * nl + i] = v;
+        b[(long)k * nl + i] = v;
     }
 }
 
@@ -82,6 +91,7 @@ int launch_angular(raysht_plan *p, int kind, const double *in, int nf_in, int fi
     if (same && fin == fout && kind != 2) return RAYSHT_EINVAL;  // the l stencil cannot run in place
     if (!same && in < out + n_out && out < in + n_in) return RAYSHT_EINVAL;
     if (p->nm > 65535) return RAYSHT_EUNSUPPORTED;
+    if ((long)(p->lmax + 1) * 2 * p->nr >= (1L << 24)) return RAYSHT_EUNSUPPORTED;  // one block's field < 2^24 values
     const long ncol = 2L * p->nr;
     int max_nl = 0;
     for (int m : p->mvals) max_nl = std::max(max_nl, p->lmax - m + 1);
