@@ -227,6 +227,13 @@ def cpu_reference_extras(L, mv, nr, nf, nlm, n_lu, ncols, out):
         t = time.perf_counter() - t0
         res["d_by_dtheta"] = {"ms_full_size": t * 1e3, "cores": 1, "kind": "reference",
                               "sample": "one field, the full size (a two-field buffer passed in)"}
+        buf = rng.standard_normal(O.spectral_size(L, mv, nr, 3))
+        t0 = time.perf_counter()
+        R.rlm_spaceb(buf, 3, 1, 2, 3)
+        t = time.perf_counter() - t0
+        res["rlm_spaceb"] = {"ms_full_size": t * 1e3, "cores": 1, "kind": "reference",
+                             "sample": "W, P, Z at the full size (a three-field buffer; the reference's "
+                                       "d_by_sdtheta / d_by_dphi and rlm_spaceb's array statements)"}
     threads = min(16, len(os.sched_getaffinity(0)))
     k = 16
     mats = [rng.standard_normal((n_lu, n_lu)) + n_lu * np.eye(n_lu) for _ in range(k)]
@@ -266,9 +273,8 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
     buffer is transformed in place of Rayleigh's p1 layout (the 1a<->2a radial transposes
     between the Chebyshev and Legendre stages are not implemented)."""
     import torch
-    from oracle import oracle as O
     L, nt, nphi, nr, nf = STACKS[workload]
-    mv = O.m_order_balanced(L)
+    mv = ra.m_values_balanced(L)
     plan = ra.LegendrePlan(nt, L, mv, nr, nf)
     cheb = ra.ChebyGrid(nr, 0.35, 1.0)
     g = torch.Generator(device="cuda").manual_seed(3)
@@ -355,6 +361,23 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
         out["angular_derivatives"][kind] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9,
                                             "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
     del f1
+    # rlm_spaceb's combination after PtS, in place on W, P, Z (fields 1, 2, 3) of a copy of the
+    # 8-field spectral data: three fields read and written (48 B per coefficient of a field)
+    # and l = l_max of the other five fields zeroed
+    w8 = s.clone()
+    nb = 48.0 * per_field + 8.0 * (nf - 3) * 2 * nr * len(mv)
+    fn = lambda: plan.rlm_spaceb_hydro(w8, 1, 2, 3)
+    for _ in range(max(warmup, 1)):
+        fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    out["rlm_spaceb_hydro"] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9, "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
+    del w8
     # the implicit solve's dense path for the coupled W-P system (n = 2 N_r) of every l:
     # LU_Decompose_full on l_max + 1 matrices, then LU_Solve_full on 2 (l + 1) right-hand
     # sides per l (re, im of every m <= l); 2 n^3 / 3 and 2 n^2 per column flop
@@ -623,7 +646,6 @@ def main():
 
     import torch
     import rayleigh_amd as ra
-    from oracle import oracle as O
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a GPU")
     if args.workload in STACKS:
@@ -632,7 +654,7 @@ def main():
     if args.workload == "batch":
         batch_bench(ra, args.steps, args.warmup, [f.strip() for f in args.forms.split(",") if f.strip()])
         return
-    mvals = O.m_order_balanced(LMAX)
+    mvals = ra.m_values_balanced(LMAX)
     F = flops_per_direction()
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     run_order = ["fp64"] + [f for f in forms if f != "fp64"]

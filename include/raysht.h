@@ -154,6 +154,28 @@ int raysht_d_by_dphi(raysht_plan *plan, const double *in, int nf_in, int fin, do
 int raysht_angular_deriv_host(raysht_plan *plan, int kind, const double *const *in_m, int nf_in, int fin,
                               double *const *out_m, int nf_out, int fout);
 
+/* rlm_spaceb's hydro part (Physics/Sphere_Hybrid_Space.F90:127-214, without adjust_emf) in
+ * one pass, in place on the packed spectral buffer `buf` of nf fields after PtS, with wvar,
+ * pvar, zvar the 0-based W, P (dW/dr) and Z fields (distinct):
+ *   W <- W over_l_l_plus1(l);
+ *   Z <- (d_by_dphi P - d_by_sdtheta Z) over_l_l_plus1(l);
+ *   P <- (d_by_sdtheta P + d_by_dphi Z) over_l_l_plus1(l)      (P and Z as they came in);
+ * l = 0 of W, P, Z set to 0 for m = 0, and l = l_max of every field set to 0.
+ * over_l_l_plus1(l) = 1 / (l (l + 1)), 1 at l = 0 (ProblemSize.F90:506-513).  Results
+ * are the doubles the reference's sequence of d_by_sdtheta / d_by_dphi calls and array
+ * statements gives.  l_max + 1 <= 2048. */
+int raysht_rlm_spaceb_hydro(raysht_plan *plan, double *buf, int nf, int wvar, int pvar, int zvar, void *stream);
+/* The l(l+1) and radial scalings of rlm_spacea / rlm_spaceb (Velocity_Components,
+ * Velocity_Derivatives, Compute_BandCurlB: Sphere_Hybrid_Space.F90:244,324,353,367,373) on one
+ * field: out(l, r, imi) = (lfac(l) in(l, r, imi)) rfac(r), rounded in that order; lfac
+ * (l_max + 1 values) and rfac (N_r values) are device arrays, either may be NULL (factor 1).
+ * In place, or between fields of one buffer, allowed. */
+int raysht_lm_scale(raysht_plan *plan, const double *in, int nf_in, int fin, double *out, int nf_out, int fout,
+                    const double *lfac, const double *rfac, void *stream);
+/* "Zero out l_max mode" of rlm_spacea / rlm_spaceb (Sphere_Hybrid_Space.F90:70-72,102-104,
+ * 207-209): data(l_max, :, :, :) = 0 for every m-block of a buffer of nf fields. */
+int raysht_zero_lmax(raysht_plan *plan, double *buf, int nf, void *stream);
+
 /* Implicit_Solve's dense path (Linear_Solve.F90:423-457) for many matrices at once.
  * raysht_lu_decompose_full = LU_Decompose_full (:1034-1045, LAPACK dgetrf) on nbatch n x n
  * column-major matrices stored one after another in `mats` (overwritten by L\U), pivots

@@ -69,6 +69,9 @@ def oracle_lib():
         lib.orc_angular_coefs.argtypes = [_I, _I, _I, _P]
         lib.orc_angular_dtheta.argtypes = [_I, _I, _P, _I, _I, _P, _I, _I, _P, _I, _I]
         lib.orc_angular_dphi.argtypes = [_I, _P, _I, _I, _P, _I, _I, _P, _I, _I]
+        lib.orc_rlm_spaceb.argtypes = [_I, _P, _I, _I, _P, _I, _I, _I, _I]
+        lib.orc_lm_scale.argtypes = [_I, _P, _I, _I, _P, _I, _I, _P, _I, _I, _P, _P]
+        lib.orc_zero_lmax.argtypes = [_I, _P, _I, _I, _P, _I]
         lib.orc_lu_decompose.restype = _I
         lib.orc_lu_decompose.argtypes = [_I, _P, _P]
         lib.orc_lu_solve.argtypes = [_I, _P, _P, _I, _P]
@@ -110,6 +113,7 @@ def ref_lib():
         lib.ref_cheby_deriv_nobuffer.argtypes = [_I, _I, _P, _P]
         lib.ref_angular_init.argtypes = [_I, _P, _I, _I]
         lib.ref_angular_deriv.argtypes = [_I, _I, _I, _I, _P, _P]
+        lib.ref_rlm_spaceb.argtypes = [_I, _I, _I, _I, _P]
         _ref = lib
     return _ref
 
@@ -270,6 +274,36 @@ def angular_deriv(kind, lmax, mvals, nr, a, nf_in, fin, out=None, nf_out=1, fout
         oracle_lib().orc_angular_dtheta(ANGULAR[kind], lmax, _ptr(mv), len(mv), nr, _ptr(a), nf_in, fin,
                                         _ptr(out), nf_out, fout)
     return out
+
+
+def rlm_spaceb(lmax, mvals, nr, buf, nf, wv, pv, zv):
+    """Sphere_Hybrid_Space.F90:127-214 rlm_spaceb (hydro part) on a copy of the packed nf-field
+    buffer (0-based W, P, Z fields); returns the new buffer."""
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    b = np.ascontiguousarray(buf, dtype=np.float64).copy()
+    oracle_lib().orc_rlm_spaceb(lmax, _ptr(mv), len(mv), nr, _ptr(b), nf, wv, pv, zv)
+    return b
+
+
+def lm_scale(lmax, mvals, nr, a, nf_in, fin, lfac=None, rfac=None, out=None, nf_out=1, fout=0):
+    """(lfac(l) a(l, r, imi)) rfac(r) of field fin into field fout of out (a fresh one-field
+    array when out is None); returns out."""
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if out is None:
+        out = np.zeros(spectral_size(lmax, mv, nr, 1))
+    lf = None if lfac is None else np.ascontiguousarray(lfac, dtype=np.float64)
+    rf = None if rfac is None else np.ascontiguousarray(rfac, dtype=np.float64)
+    oracle_lib().orc_lm_scale(lmax, _ptr(mv), len(mv), nr, _ptr(a), nf_in, fin, _ptr(out), nf_out, fout,
+                              None if lf is None else _ptr(lf), None if rf is None else _ptr(rf))
+    return out
+
+
+def zero_lmax(lmax, mvals, nr, buf, nf):
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    b = np.ascontiguousarray(buf, dtype=np.float64).copy()
+    oracle_lib().orc_zero_lmax(lmax, _ptr(mv), len(mv), nr, _ptr(b), nf)
+    return b
 
 
 def angular_coefs(l, m, kind):
@@ -460,6 +494,13 @@ class Reference:
         arr = np.zeros(spectral_size(lmax, mv, nr, 1))
         self.lib.ref_angular_deriv(ANGULAR[kind], nf, fin, fout, _ptr(b), _ptr(arr))
         return b if fout >= 1 else arr
+
+    def rlm_spaceb(self, buf: np.ndarray, nf: int, wv: int, pv: int, zv: int) -> np.ndarray:
+        """rlm_spaceb's hydro steps through the reference's own d_by_sdtheta / d_by_dphi on a
+        copy of the buffer (fields 1-based); returns the new buffer."""
+        b = np.ascontiguousarray(buf, dtype=np.float64).copy()
+        self.lib.ref_rlm_spaceb(nf, wv, pv, zv, _ptr(b))
+        return b
 
     def cheby_deriv(self, c: np.ndarray, dorder: int, nobuffer: bool = False) -> np.ndarray:
         c = np.ascontiguousarray(c, dtype=np.float64)

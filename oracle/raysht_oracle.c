@@ -843,3 +843,91 @@ void orc_m_order_balanced(int lmax, int *mvals)
     }
     if (nmm % 2 == 1) mvals[k++] = nmm / 2;
 }
+
+/* over_l_l_plus1 (ProblemSize.F90:506-513): 1 / (l (l + 1)) from the double l, 1 at l = 0 */
+static double over_l_l_plus1(int l)
+{
+    const double ell = (double)l;
+    return l == 0 ? 1.0 : 1.0 / (ell * (ell + 1.0));
+}
+
+/* Sphere_Hybrid_Space.F90:127-214 rlm_spaceb, hydro part (adjust_emf left out), in place on a
+ * packed buffer of nf fields (0-based W, P, Z fields wv, pv, zv), step for step:
+ *   W = W ol;  t1 = d_by_sdtheta Z;  t2 = d_by_dphi P;  t1 = (t2 - t1) ol;
+ *   t2 = d_by_dphi Z;  Z = d_by_sdtheta P;  P = (Z + t2) ol;  Z = t1;
+ *   m = 0: l = 0 of P, W, Z = 0;  every block: l = l_max of every field = 0. */
+void orc_rlm_spaceb(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, int wv, int pv, int zv)
+{
+    const long fsz = field_block(lmax, mvals, nm, nr, 1);
+    double *t1 = (double *)malloc(sizeof(double) * (fsz + 1));
+    double *t2 = (double *)malloc(sizeof(double) * (fsz + 1));
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp], nl = lmax - m + 1;
+        double *w = buf + field_block(lmax, mvals, mp, nr, nf) + (long)wv * nl * 2 * nr;
+        for (long k = 0; k < 2L * nr; ++k)
+            for (int l = m; l <= lmax; ++l) w[k * nl + l - m] = w[k * nl + l - m] * over_l_l_plus1(l);
+    }
+    orc_angular_dtheta(1, lmax, mvals, nm, nr, buf, nf, zv, t1, 1, 0);
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, pv, t2, 1, 0);
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp], nl = lmax - m + 1;
+        const long o = field_block(lmax, mvals, mp, nr, 1);
+        for (long k = 0; k < 2L * nr; ++k)
+            for (int l = m; l <= lmax; ++l) {
+                const long e = o + k * nl + l - m;
+                t1[e] = (t2[e] - t1[e]) * over_l_l_plus1(l);
+            }
+    }
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, zv, t2, 1, 0);
+    orc_angular_dtheta(1, lmax, mvals, nm, nr, buf, nf, pv, buf, nf, zv);
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp], nl = lmax - m + 1;
+        const long fs = (long)nl * 2 * nr, o = field_block(lmax, mvals, mp, nr, 1);
+        double *blk = buf + field_block(lmax, mvals, mp, nr, nf);
+        double *P = blk + pv * fs, *Z = blk + zv * fs, *W = blk + wv * fs;
+        for (long k = 0; k < 2L * nr; ++k)
+            for (int l = m; l <= lmax; ++l) {
+                const long e = k * nl + l - m;
+                P[e] = (Z[e] + t2[o + e]) * over_l_l_plus1(l);
+            }
+        for (long e = 0; e < fs; ++e) Z[e] = t1[o + e];
+        if (m == 0)
+            for (long k = 0; k < 2L * nr; ++k) P[k * nl] = W[k * nl] = Z[k * nl] = 0.0;
+        for (long k = 0; k < 2L * nr * nf; ++k) blk[k * nl + nl - 1] = 0.0;
+    }
+    free(t1);
+    free(t2);
+}
+
+/* out(l, r, imi) = (lfac(l) in(l, r, imi)) rfac(r) on one field (either factor may be NULL),
+ * rounded in the order of l_l_plus1(m:l_max)*SBUFFA(IDX2,f)*Over_RhoRSQ(r)
+ * (Sphere_Hybrid_Space.F90:244); in place allowed. */
+void orc_lm_scale(int lmax, const int *mvals, int nm, int nr, const double *in, int nf_in, int fin, double *out,
+                  int nf_out, int fout, const double *lfac, const double *rfac)
+{
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp], nl = lmax - m + 1;
+        const long fs = (long)nl * 2 * nr;
+        const double *a = in + field_block(lmax, mvals, mp, nr, nf_in) + fin * fs;
+        double *b = out + field_block(lmax, mvals, mp, nr, nf_out) + fout * fs;
+        for (int imi = 0; imi < 2; ++imi)
+            for (int r = 0; r < nr; ++r)
+                for (int l = m; l <= lmax; ++l) {
+                    const long e = ((long)imi * nr + r) * nl + l - m;
+                    double v = a[e];
+                    if (lfac) v = lfac[l] * v;
+                    if (rfac) v = v * rfac[r];
+                    b[e] = v;
+                }
+    }
+}
+
+/* Sphere_Hybrid_Space.F90:70-72 (and :102-104, :207-209): data(l_max, :, :, :) = 0 per block */
+void orc_zero_lmax(int lmax, const int *mvals, int nm, int nr, double *buf, int nf)
+{
+    for (int mp = 0; mp < nm; ++mp) {
+        const int nl = lmax - mvals[mp] + 1;
+        double *blk = buf + field_block(lmax, mvals, mp, nr, nf);
+        for (long k = 0; k < 2L * nr * nf; ++k) blk[k * nl + nl - 1] = 0.0;
+    }
+}

@@ -65,6 +65,10 @@ void orc_angular_dtheta(int which, int lmax, const int *mvals, int nm, int nr, c
                         double *out, int nf_out, int fout);
 void orc_angular_dphi(int lmax, const int *mvals, int nm, int nr, const double *in, int nf_in, int fin, double *out,
                       int nf_out, int fout);
+void orc_rlm_spaceb(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, int wv, int pv, int zv);
+void orc_lm_scale(int lmax, const int *mvals, int nm, int nr, const double *in, int nf_in, int fin, double *out,
+                  int nf_out, int fout, const double *lfac, const double *rfac);
+void orc_zero_lmax(int lmax, const int *mvals, int nm, int nr, double *buf, int nf);
 int orc_lu_decompose(int n, double *a, int *piv);
 void orc_lu_solve(int n, const double *a, const int *piv, int nrhs, double *b);
 int orc_lm_values(int lmax, const int *mvals, int nm, int *l_out, int *mp_out);

@@ -319,4 +319,65 @@ Contains
         DeAllocate(a)
     End Subroutine ref_angular_deriv
 
+    ! rlm_spaceb's hydro steps (Physics/Sphere_Hybrid_Space.F90:127-214; that module needs
+    ! the whole physics state, so the harness issues the same calls of the reference's
+    ! d_by_sdtheta / d_by_dphi and the same array statements itself) on a packed buffer of nf
+    ! fields, in place; wv, pv, zv are 1-based.
+    Subroutine ref_rlm_spaceb(nf, wv, pv, zv, buf) bind(C, name='ref_rlm_spaceb')
+        Integer(C_INT), Value :: nf, wv, pv, zv
+        Real(C_DOUBLE), Intent(InOut) :: buf(*)
+        Type(rmcontainer4d), Allocatable :: a(:)
+        Type(rmcontainer3d), Allocatable :: t1(:), t2(:)
+        Real*8, Allocatable :: ol(:)
+        Real*8 :: ell
+        Integer :: mp, m, l, r, imi, lm
+        lm = h_ang_lmax
+        Allocate(ol(0:lm))
+        ol(0) = 1.0d0
+        Do l = 1, lm
+            ell = l*1.0d0
+            ol(l) = 1.0d0/(ell*(ell+1))
+        Enddo
+        Call unpack4d(buf, nf, a)
+        Call alloc3d(t1)
+        Call alloc3d(t2)
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            Do imi = 1, 2
+                Do r = 1, h_ang_nr
+                    a(mp)%data(m:lm, r, imi, wv) = a(mp)%data(m:lm, r, imi, wv)*ol(m:lm)
+                Enddo
+            Enddo
+        Enddo
+        Call d_by_sdtheta(a, zv, t1)
+        Call d_by_dphi(a, pv, t2)
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            Do imi = 1, 2
+                Do r = 1, h_ang_nr
+                    t1(mp)%data(m:lm, r, imi) = (t2(mp)%data(m:lm, r, imi) - t1(mp)%data(m:lm, r, imi))*ol(m:lm)
+                Enddo
+            Enddo
+        Enddo
+        Call d_by_dphi(a, zv, t2)
+        Call d_by_sdtheta(a, pv, zv)
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            Do imi = 1, 2
+                Do r = 1, h_ang_nr
+                    a(mp)%data(m:lm, r, imi, pv) = (a(mp)%data(m:lm, r, imi, zv) + t2(mp)%data(m:lm, r, imi))*ol(m:lm)
+                    a(mp)%data(m:lm, r, imi, zv) = t1(mp)%data(m:lm, r, imi)
+                Enddo
+            Enddo
+            If (m == 0) Then
+                a(mp)%data(0, :, :, pv) = 0.0d0
+                a(mp)%data(0, :, :, wv) = 0.0d0
+                a(mp)%data(0, :, :, zv) = 0.0d0
+            Endif
+            a(mp)%data(lm, :, :, :) = 0.0d0
+        Enddo
+        Call pack4d(a, nf, buf)
+        DeAllocate(a, t1, t2, ol)
+    End Subroutine ref_rlm_spaceb
+
 End Module raysht_ref_harness

@@ -20,6 +20,22 @@ namespace {
 constexpr int kAThreads = 256;
 constexpr int kAMaxBlocksX = 4096;
 
+// q = e / n for 0 <= e < 2^24 from a float estimate and one exact correction
+__device__ __forceinline__ int div_small(int e, int n, float rn, int &rem)
+{
+    int q = (int)((float)e * rn);
+    int i = e - q * n;
+    if (i < 0) {
+        --q;
+        i += n;
+    } else if (i >= n) {
+        ++q;
+        i -= n;
+    }
+    rem = i;
+    return q;
+}
+
 // out(l) of one field from field fin; grid (element blocks, m-block)
 // Block mp of a packed array of F fields starts at lm_base[mp] * 2 nr * F (lm_base = the
 // l count of the blocks before it); field f of the block at f * nl * 2 nr from there.
@@ -37,16 +53,8 @@ __global__ __launch_bounds__(kAThreads) void dtheta_kernel(int lmax, int nr, con
     const double2 *c = coef + lb;
     const float rnl = 1.0f / (float)nl;
     for (long e = blockIdx.x * (long)kAThreads + threadIdx.x; e < total; e += (long)gridDim.x * kAThreads) {
-        // column k = e / nl from a float estimate and one exact correction (e < 2^31)
-        int k = (int)((float)e * rnl);
-        int i = (int)e - k * nl;  // l - m
-        if (i < 0) {
-            --k;
-            i += nl;
-        } else if (i >= nl) {
-            ++k;
-            i -= nl;
-        }
+        int i;  // l - m
+        const int k = div_small((int)e, nl, rnl, i);
         const double *ak = a + (long)k * nl;
         double v;
         if (nl == 1)
@@ -79,23 +87,150 @@ __global__ __launch_bounds__(kAThreads) void dphi_kernel(int lmax, int nr, const
     }
 }
 
+// d_by_sdtheta of one l-column x (nl values, i = l - m) with coefficients c, rounded as
+// dtheta_kernel rounds it
+__device__ __forceinline__ double sdtheta_at(const double *x, const double2 *c, int i, int nl)
+{
+#pragma clang fp contract(off)
+    if (nl == 1) return 0.0;
+    if (i == 0) return x[1] * c[0].y;
+    if (i == nl - 1) return x[i - 1] * c[i].x;
+    return x[i - 1] * c[i].x + x[i + 1] * c[i].y;
+}
+
+// rlm_spaceb's hydro combination (Sphere_Hybrid_Space.F90:127-214), in place on fields
+// W, P, Z of a buffer of nf fields, one pass:
+//   W <- W / (l(l+1));  Z <- (d_by_dphi P - d_by_sdtheta Z) / (l(l+1));
+//   P <- (d_by_sdtheta P + d_by_dphi Z) / (l(l+1));
+// then l = 0 of W, P, Z zeroed for m = 0, and l = l_max of every field zeroed.  A workgroup
+// owns g whole l-columns (radii) of one m-block, so each thread has read the incoming P and
+// Z at l - 1, l, l + 1 before any thread writes: the outputs wait in registers for one barrier.
+constexpr int kRlmPerThread = 8;  // values per thread: l_max + 1 <= 2048
+__global__ __launch_bounds__(kAThreads) void rlm_spaceb_kernel(int lmax, int nr, const int *__restrict__ mvals,
+                                                               const int *__restrict__ lm_base,
+                                                               const double2 *__restrict__ scoef, double *buf, int nf,
+                                                               int wv, int pv, int zv)
+{
+#pragma clang fp contract(off)
+    const int mp = blockIdx.y, m = mvals[mp], nl = lmax - m + 1;
+    const int g = nl >= kAThreads ? 1 : kAThreads / nl;  // l-columns per workgroup
+    const int r0 = blockIdx.x * g;
+    if (r0 >= nr) return;  // the whole workgroup leaves together
+    const int nel = min(g, nr - r0) * nl;
+    const long lb = lm_base[mp], half = (long)nl * nr, fsz = 2 * half;
+    double *blk = buf + lb * 2L * nr * nf;
+    double *W = blk + wv * fsz, *P = blk + pv * fsz, *Z = blk + zv * fsz;
+    const double2 *c = scoef + lb;
+    const float rnl = 1.0f / (float)nl;
+    const double dm = (double)m;
+    double o[kRlmPerThread][6];
+#pragma unroll
+    for (int j = 0; j < kRlmPerThread; ++j) {
+        const int e = threadIdx.x + j * kAThreads;
+        if (e >= nel) break;
+        int i;
+        const long col = (long)(r0 + div_small(e, nl, rnl, i)) * nl;
+        const double *pr = P + col, *pi = P + half + col, *zr = Z + col, *zi = Z + half + col;
+        const int l = m + i;
+        const double ol = l == 0 ? 1.0 : 1.0 / ((double)l * ((double)l + 1.0));  // over_l_l_plus1
+        const double sdzr = sdtheta_at(zr, c, i, nl), sdzi = sdtheta_at(zi, c, i, nl);
+        const double sdpr = sdtheta_at(pr, c, i, nl), sdpi = sdtheta_at(pi, c, i, nl);
+        const bool zero = l == lmax || l == 0;
+        o[j][0] = zero ? 0.0 : W[col + i] * ol;
+        o[j][1] = zero ? 0.0 : W[half + col + i] * ol;
+        o[j][2] = zero ? 0.0 : (sdpr + -(dm * zi[i])) * ol;
+        o[j][3] = zero ? 0.0 : (sdpi + dm * zr[i]) * ol;
+        o[j][4] = zero ? 0.0 : (-(dm * pi[i]) - sdzr) * ol;
+        o[j][5] = zero ? 0.0 : (dm * pr[i] - sdzi) * ol;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRlmPerThread; ++j) {
+        const int e = threadIdx.x + j * kAThreads;
+        if (e >= nel) break;
+        int i;
+        const long col = (long)(r0 + div_small(e, nl, rnl, i)) * nl + i;
+        W[col] = o[j][0];
+        W[half + col] = o[j][1];
+        P[col] = o[j][2];
+        P[half + col] = o[j][3];
+        Z[col] = o[j][4];
+        Z[half + col] = o[j][5];
+        if (i == nl - 1)
+            for (int f = 0; f < nf; ++f)
+                if (f != wv && f != pv && f != zv) {
+                    blk[f * fsz + col] = 0.0;
+                    blk[f * fsz + half + col] = 0.0;
+                }
+    }
+}
+
+// out(l, r, imi) = (lfac(l) in(l, r, imi)) rfac(r) for one field (either factor may be
+// absent); in place allowed.  The products round in this order, as the reference's
+// l_l_plus1(m:l_max)*SBUFFA(IDX2, f)*Over_RhoRSQ(r) statements round them.
+__global__ __launch_bounds__(kAThreads) void lm_scale_kernel(int lmax, int nr, const int *__restrict__ mvals,
+                                                             const int *__restrict__ lm_base, const double *in,
+                                                             int nf_in, int fin, double *out, int nf_out, int fout,
+                                                             const double *__restrict__ lfac,
+                                                             const double *__restrict__ rfac)
+{
+#pragma clang fp contract(off)
+    const int mp = blockIdx.y, m = mvals[mp], nl = lmax - m + 1;
+    const long lb = lm_base[mp], total = 2L * nl * nr;
+    const double *a = in + lb * 2L * nr * nf_in + (long)fin * total;
+    double *b = out + lb * 2L * nr * nf_out + (long)fout * total;
+    const float rnl = 1.0f / (float)nl;
+    for (long e = blockIdx.x * (long)kAThreads + threadIdx.x; e < total; e += (long)gridDim.x * kAThreads) {
+        int i;
+        int k = div_small((int)e, nl, rnl, i);  // column r + nr imi
+        double v = a[e];
+        if (lfac) v = lfac[m + i] * v;
+        if (rfac) v = v * rfac[k >= nr ? k - nr : k];
+        b[e] = v;
+    }
+}
+
+// data(l_max, r, imi, f) = 0 for every m-block, radius, imi and field: one value per
+// l-column, at nl (k + 1) - 1 for column k = r + nr (imi + 2 f)
+__global__ __launch_bounds__(kAThreads) void zero_lmax_kernel(int lmax, int nr, const int *__restrict__ mvals,
+                                                              const int *__restrict__ lm_base, double *buf, int nf)
+{
+    const int mp = blockIdx.y, nl = lmax - mvals[mp] + 1;
+    const long ncol = 2L * nr * nf;
+    double *blk = buf + lm_base[mp] * ncol;
+    for (long k = blockIdx.x * (long)kAThreads + threadIdx.x; k < ncol; k += (long)gridDim.x * kAThreads)
+        blk[k * nl + nl - 1] = 0.0;
+}
+
+// the checks every spectral-field entry point shares: a plan with fields and m-blocks that
+// fit one launch, and one block's field below 2^24 values (div_small)
+int check_spectral(const raysht_plan *p)
+{
+    if (p->nr < 1 || p->nf < 1) return RAYSHT_EUNSUPPORTED;  // column plans have no fields
+    if (p->nm > 65535) return RAYSHT_EUNSUPPORTED;
+    if ((long)(p->lmax + 1) * 2 * p->nr >= (1L << 24)) return RAYSHT_EUNSUPPORTED;
+    return RAYSHT_OK;
+}
+
+int max_nl(const raysht_plan *p)
+{
+    int n = 0;
+    for (int m : p->mvals) n = std::max(n, p->lmax - m + 1);
+    return n;
+}
+
 int launch_angular(raysht_plan *p, int kind, const double *in, int nf_in, int fin, double *out, int nf_out, int fout,
                    hipStream_t st)
 {
     if (!p || !in || !out || kind < 0 || kind > 2) return RAYSHT_EINVAL;
-    if (p->nr < 1 || p->nf < 1) return RAYSHT_EUNSUPPORTED;  // column plans have no fields
+    if (int rc = check_spectral(p)) return rc;
     if (nf_in < 1 || nf_out < 1 || fin < 0 || fin >= nf_in || fout < 0 || fout >= nf_out) return RAYSHT_EINVAL;
     const long per_field = p->s_off[p->nm] / p->nf;  // sum over m-blocks of nl * 2 nr
     const long n_in = per_field * nf_in, n_out = per_field * nf_out;
     const bool same = in == out && nf_in == nf_out;
     if (same && fin == fout && kind != 2) return RAYSHT_EINVAL;  // the l stencil cannot run in place
     if (!same && in < out + n_out && out < in + n_in) return RAYSHT_EINVAL;
-    if (p->nm > 65535) return RAYSHT_EUNSUPPORTED;
-    if ((long)(p->lmax + 1) * 2 * p->nr >= (1L << 24)) return RAYSHT_EUNSUPPORTED;  // one block's field < 2^24 values
-    const long ncol = 2L * p->nr;
-    int max_nl = 0;
-    for (int m : p->mvals) max_nl = std::max(max_nl, p->lmax - m + 1);
-    const long work = kind == 2 ? (long)max_nl * p->nr : (long)max_nl * ncol;
+    const long work = (long)max_nl(p) * (kind == 2 ? p->nr : 2L * p->nr);
     const unsigned bx = (unsigned)std::min<long>((work + kAThreads - 1) / kAThreads, kAMaxBlocksX);
     dim3 grid(bx, (unsigned)p->nm);
     if (kind == 2)
@@ -104,6 +239,52 @@ int launch_angular(raysht_plan *p, int kind, const double *in, int nf_in, int fi
     else
         hipLaunchKernelGGL(dtheta_kernel, grid, dim3(kAThreads), 0, st, p->lmax, p->nr, p->d_mvals, p->d_lm_base,
                            p->d_acoef + (kind == 1 ? p->lm_total : 0), in, nf_in, fin, out, nf_out, fout);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+int launch_rlm_spaceb(raysht_plan *p, double *buf, int nf, int wv, int pv, int zv, hipStream_t st)
+{
+    if (!p || !buf) return RAYSHT_EINVAL;
+    if (int rc = check_spectral(p)) return rc;
+    if (nf < 3 || wv < 0 || pv < 0 || zv < 0 || wv >= nf || pv >= nf || zv >= nf || wv == pv || wv == zv || pv == zv)
+        return RAYSHT_EINVAL;
+    if (p->lmax + 1 > kRlmPerThread * kAThreads) return RAYSHT_EUNSUPPORTED;
+    unsigned bx = 1;  // enough column groups for the m-block that needs most
+    for (int m : p->mvals) {
+        const int nl = p->lmax - m + 1, g = nl >= kAThreads ? 1 : kAThreads / nl;
+        bx = std::max(bx, (unsigned)((p->nr + g - 1) / g));
+    }
+    hipLaunchKernelGGL(rlm_spaceb_kernel, dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr,
+                       p->d_mvals, p->d_lm_base, p->d_acoef + p->lm_total, buf, nf, wv, pv, zv);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+int launch_lm_scale(raysht_plan *p, const double *in, int nf_in, int fin, double *out, int nf_out, int fout,
+                    const double *lfac, const double *rfac, hipStream_t st)
+{
+    if (!p || !in || !out) return RAYSHT_EINVAL;
+    if (int rc = check_spectral(p)) return rc;
+    if (nf_in < 1 || nf_out < 1 || fin < 0 || fin >= nf_in || fout < 0 || fout >= nf_out) return RAYSHT_EINVAL;
+    const long per_field = p->s_off[p->nm] / p->nf;  // elementwise: in place, or another field of one buffer
+    const bool same = in == out && nf_in == nf_out;
+    if (!same && in < out + per_field * nf_out && out < in + per_field * nf_in) return RAYSHT_EINVAL;
+    const unsigned bx =
+        (unsigned)std::min<long>((2L * max_nl(p) * p->nr + kAThreads - 1) / kAThreads, kAMaxBlocksX);
+    hipLaunchKernelGGL(lm_scale_kernel, dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr,
+                       p->d_mvals, p->d_lm_base, in, nf_in, fin, out, nf_out, fout, lfac, rfac);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+int launch_zero_lmax(raysht_plan *p, double *buf, int nf, hipStream_t st)
+{
+    if (!p || !buf || nf < 1) return RAYSHT_EINVAL;
+    if (int rc = check_spectral(p)) return rc;
+    const unsigned bx = (unsigned)std::min<long>((2L * p->nr * nf + kAThreads - 1) / kAThreads, kAMaxBlocksX);
+    hipLaunchKernelGGL(zero_lmax_kernel, dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr,
+                       p->d_mvals, p->d_lm_base, buf, nf);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
@@ -162,6 +343,22 @@ int raysht_d_by_dphi(raysht_plan *p, const double *in, int nf_in, int fin, doubl
                      void *stream)
 {
     return launch_angular(p, 2, in, nf_in, fin, out, nf_out, fout, reinterpret_cast<hipStream_t>(stream));
+}
+
+int raysht_rlm_spaceb_hydro(raysht_plan *p, double *buf, int nf, int wvar, int pvar, int zvar, void *stream)
+{
+    return launch_rlm_spaceb(p, buf, nf, wvar, pvar, zvar, reinterpret_cast<hipStream_t>(stream));
+}
+
+int raysht_lm_scale(raysht_plan *p, const double *in, int nf_in, int fin, double *out, int nf_out, int fout,
+                    const double *lfac, const double *rfac, void *stream)
+{
+    return launch_lm_scale(p, in, nf_in, fin, out, nf_out, fout, lfac, rfac, reinterpret_cast<hipStream_t>(stream));
+}
+
+int raysht_zero_lmax(raysht_plan *p, double *buf, int nf, void *stream)
+{
+    return launch_zero_lmax(p, buf, nf, reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

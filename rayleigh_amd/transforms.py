@@ -146,6 +146,38 @@ class LegendrePlan:
         else:
             raise ValueError("kind must be dtheta, sdtheta or dphi")
 
+    def _whole_fields(self, *arrays):
+        if self.n_fields < 1:
+            raise ValueError("a plan over anonymous columns has no fields")
+        per_field = self.spectral_size // self.n_fields
+        if any(x.numel() % per_field for x in arrays):
+            raise ValueError("arrays must hold whole fields of this plan's spectral layout")
+        return [x.numel() // per_field for x in arrays]
+
+    def rlm_spaceb_hydro(self, buf, wvar: int, pvar: int, zvar: int, stream=None):
+        """Sphere_Hybrid_Space.F90:127-214 rlm_spaceb, hydro part, in place on `buf` (whole
+        fields of this plan's spectral layout; W, P, Z the 0-based fields wvar, pvar, zvar):
+        W /= l(l+1); Z = (d_by_dphi P - d_by_sdtheta Z) / l(l+1); P = (d_by_sdtheta P +
+        d_by_dphi Z) / l(l+1); l = 0 of W, P, Z zeroed for m = 0; l = l_max of every field zeroed."""
+        (nf,) = self._whole_fields(buf)
+        _lib.check(_lib.lib().raysht_rlm_spaceb_hydro(self._h, _dptr(buf, "buf"), nf, int(wvar), int(pvar), int(zvar),
+                                                      _stream(stream)), "raysht_rlm_spaceb_hydro")
+
+    def lm_scale(self, a, fin: int, out, fout: int, lfac=None, rfac=None, stream=None):
+        """out(l, r, imi) = (lfac(l) a(l, r, imi)) rfac(r) from field fin of `a` into field fout of
+        `out` (the l(l+1) / radial scalings of rlm_spacea and rlm_spaceb); lfac (l_max + 1
+        values) and rfac (n_r values) are float64 device tensors, or None for a factor of 1."""
+        nf_in, nf_out = self._whole_fields(a, out)
+        lf = None if lfac is None else _dptr(lfac, "lfac", self.l_max + 1)
+        rf = None if rfac is None else _dptr(rfac, "rfac", self.n_r)
+        _lib.check(_lib.lib().raysht_lm_scale(self._h, _dptr(a, "a"), nf_in, int(fin), _dptr(out, "out"), nf_out,
+                                              int(fout), lf, rf, _stream(stream)), "raysht_lm_scale")
+
+    def zero_lmax(self, buf, stream=None):
+        """data(l_max, :, :, :) = 0 in every m-block of `buf` (whole fields)."""
+        (nf,) = self._whole_fields(buf)
+        _lib.check(_lib.lib().raysht_zero_lmax(self._h, _dptr(buf, "buf"), nf, _stream(stream)), "raysht_zero_lmax")
+
     def lm_values(self):
         """LM_Load_Balance's (l, mp) pairs (l_lm_values, mp_lm_values with mp 0-based)."""
         n = _lib.lib().raysht_lm_count(self._h)
@@ -333,6 +365,28 @@ def fft_to_spectral_rsc(x, n_phi: int, stream=None):
         raise ValueError("x is not a whole number of lines of n_phi+2")
     _lib.check(_lib.lib().raysht_fft_to_spectral(int(n_phi), x.numel() // (n_phi + 2), _dptr(x, "x"), 1,
                                                  _stream(stream)), "raysht_fft_to_spectral")
+
+
+def m_values_balanced(l_max: int) -> np.ndarray:
+    """The m order of one rank under Load_Balance.F90:126-226 (M_Balance_v0): the pairs
+    (i, l_max - i) for i = 0, 1, ..., then the middle m when l_max + 1 is odd."""
+    n = l_max + 1
+    lo = np.arange(n // 2, dtype=np.int32)
+    mv = np.stack([lo, n - 1 - lo], axis=1).ravel()
+    return np.concatenate([mv, np.array([n // 2] if n % 2 else [], dtype=np.int32)]).astype(np.int32)
+
+
+def l_l_plus1(l_max: int) -> np.ndarray:
+    """l (l + 1) for l = 0..l_max, from the double l (ProblemSize.F90:506-513)."""
+    ell = np.arange(l_max + 1, dtype=np.float64)
+    return ell * (ell + 1.0)
+
+
+def over_l_l_plus1(l_max: int) -> np.ndarray:
+    """1 / (l (l + 1)), and 1 at l = 0 (ProblemSize.F90:506-513)."""
+    out = np.ones(l_max + 1)
+    out[1:] = 1.0 / l_l_plus1(l_max)[1:]
+    return out
 
 
 def _iptr(t, name: str, dtype, numel: int) -> ctypes.c_void_p:

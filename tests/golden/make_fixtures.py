@@ -93,6 +93,20 @@ def angular_case(name, lmax, mvals, nr, seed):
     print("wrote", name)
 
 
+def rlm_case(name, lmax, mvals, nr, nf, fields, seed):
+    """rlm_spaceb's hydro steps (Sphere_Hybrid_Space.F90:127-214) through the reference's
+    d_by_sdtheta / d_by_dphi on a seeded nf-field buffer; fields = 0-based (W, P, Z)."""
+    R = O.Reference()
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    R.angular_init(lmax, mv, nr)
+    buf = np.random.default_rng(seed).standard_normal(O.spectral_size(lmax, mv, nr, nf))
+    wv, pv, zv = fields
+    out = R.rlm_spaceb(buf, nf, wv + 1, pv + 1, zv + 1)
+    np.savez_compressed(os.path.join(OUT, name), lmax=lmax, mvals=mv, nr=nr, nf=nf, fields=np.array(fields, np.int32),
+                        buf=buf, out=out)
+    print("wrote", name)
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("compile the reference checker first: make -C oracle ref")
@@ -113,3 +127,5 @@ if __name__ == "__main__":
     angular_case("angular_L12_ascending.npz", 12, np.arange(13), 3, seed=12)
     angular_case("angular_L41_balanced.npz", 41, O.m_order_balanced(41), 2, seed=41)
     angular_case("angular_L9_partial.npz", 9, [9, 0, 4, 3, 1], 5, seed=9)
+    rlm_case("rlm_spaceb_L12_ascending.npz", 12, np.arange(13), 3, 5, (1, 2, 3), seed=120)
+    rlm_case("rlm_spaceb_L41_partial.npz", 41, [41, 0, 17, 3, 40, 1], 4, 4, (3, 0, 2), seed=410)
