@@ -163,7 +163,7 @@ int raysht_angular_deriv_host(raysht_plan *plan, int kind, const double *const *
  * l = 0 of W, P, Z set to 0 for m = 0, and l = l_max of every field set to 0.
  * over_l_l_plus1(l) = 1 / (l (l + 1)), 1 at l = 0 (ProblemSize.F90:506-513).  Results
  * are the doubles the reference's sequence of d_by_sdtheta / d_by_dphi calls and array
- * statements gives.  l_max + 1 <= 2048. */
+ * statements gives. */
 int raysht_rlm_spaceb_hydro(raysht_plan *plan, double *buf, int nf, int wvar, int pvar, int zvar, void *stream);
 /* The l(l+1) and radial scalings of rlm_spacea / rlm_spaceb (Velocity_Components,
  * Velocity_Derivatives, Compute_BandCurlB: Sphere_Hybrid_Space.F90:244,324,353,367,373) on one

@@ -87,80 +87,100 @@ __global__ __launch_bounds__(kAThreads) void dphi_kernel(int lmax, int nr, const
     }
 }
 
-// d_by_sdtheta of one l-column x (nl values, i = l - m) with coefficients c, rounded as
-// dtheta_kernel rounds it
-__device__ __forceinline__ double sdtheta_at(const double *x, const double2 *c, int i, int nl)
-{
-#pragma clang fp contract(off)
-    if (nl == 1) return 0.0;
-    if (i == 0) return x[1] * c[0].y;
-    if (i == nl - 1) return x[i - 1] * c[i].x;
-    return x[i - 1] * c[i].x + x[i + 1] * c[i].y;
-}
-
 // rlm_spaceb's hydro combination (Sphere_Hybrid_Space.F90:127-214), in place on fields
 // W, P, Z of a buffer of nf fields, one pass:
 //   W <- W / (l(l+1));  Z <- (d_by_dphi P - d_by_sdtheta Z) / (l(l+1));
 //   P <- (d_by_sdtheta P + d_by_dphi Z) / (l(l+1));
-// then l = 0 of W, P, Z zeroed for m = 0, and l = l_max of every field zeroed.  A workgroup
-// owns g whole l-columns (radii) of one m-block, so each thread has read the incoming P and
-// Z at l - 1, l, l + 1 before any thread writes: the outputs wait in registers for one barrier.
-constexpr int kRlmPerThread = 8;  // values per thread: l_max + 1 <= 2048
+// then l = 0 of W, P, Z zeroed for m = 0, and l = l_max of every field zeroed.
+// One wave owns one radius of one m-block (its re and im l-columns of W, P, Z) and walks it
+// 64 values of l at a time, so no other wave touches what it reads: the l + 1 neighbours
+// come from the next lane (lane 63 loads the first value of the next chunk, not yet
+// written), the l - 1 neighbours from the previous lane (lane 0 takes the last values of
+// the previous chunk as loaded, carried in registers, since the wave has overwritten them).
 __global__ __launch_bounds__(kAThreads) void rlm_spaceb_kernel(int lmax, int nr, const int *__restrict__ mvals,
                                                                const int *__restrict__ lm_base,
                                                                const double2 *__restrict__ scoef, double *buf, int nf,
                                                                int wv, int pv, int zv)
 {
 #pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * (kAThreads / 64) + (threadIdx.x >> 6);
+    if (r >= nr) return;  // whole waves
     const int mp = blockIdx.y, m = mvals[mp], nl = lmax - m + 1;
-    const int g = nl >= kAThreads ? 1 : kAThreads / nl;  // l-columns per workgroup
-    const int r0 = blockIdx.x * g;
-    if (r0 >= nr) return;  // the whole workgroup leaves together
-    const int nel = min(g, nr - r0) * nl;
     const long lb = lm_base[mp], half = (long)nl * nr, fsz = 2 * half;
     double *blk = buf + lb * 2L * nr * nf;
-    double *W = blk + wv * fsz, *P = blk + pv * fsz, *Z = blk + zv * fsz;
+    double *W = blk + wv * fsz + (long)r * nl, *P = blk + pv * fsz + (long)r * nl, *Z = blk + zv * fsz + (long)r * nl;
     const double2 *c = scoef + lb;
-    const float rnl = 1.0f / (float)nl;
     const double dm = (double)m;
-    double o[kRlmPerThread][6];
-#pragma unroll
-    for (int j = 0; j < kRlmPerThread; ++j) {
-        const int e = threadIdx.x + j * kAThreads;
-        if (e >= nel) break;
-        int i;
-        const long col = (long)(r0 + div_small(e, nl, rnl, i)) * nl;
-        const double *pr = P + col, *pi = P + half + col, *zr = Z + col, *zi = Z + half + col;
-        const int l = m + i;
+    double cpr = 0.0, cpi = 0.0, czr = 0.0, czi = 0.0;  // l - 1 values for lane 0
+    for (int base = 0; base < nl; base += 64) {
+        const int i = base + lane, l = m + i;
+        const bool in = i < nl;
+        double wr = 0.0, wi = 0.0, pr = 0.0, pi = 0.0, zr = 0.0, zi = 0.0;
+        double2 ci = make_double2(0.0, 0.0);
+        if (in) {
+            wr = W[i];
+            wi = W[half + i];
+            pr = P[i];
+            pi = P[half + i];
+            zr = Z[i];
+            zi = Z[half + i];
+            ci = c[i];
+        }
+        double prn = __shfl_down(pr, 1), pin = __shfl_down(pi, 1);  // l + 1
+        double zrn = __shfl_down(zr, 1), zin = __shfl_down(zi, 1);
+        if (lane == 63 && i + 1 < nl) {
+            prn = P[i + 1];
+            pin = P[half + i + 1];
+            zrn = Z[i + 1];
+            zin = Z[half + i + 1];
+        }
+        double prp = __shfl_up(pr, 1), pip = __shfl_up(pi, 1);  // l - 1
+        double zrp = __shfl_up(zr, 1), zip = __shfl_up(zi, 1);
+        if (lane == 0) {
+            prp = cpr;
+            pip = cpi;
+            zrp = czr;
+            zip = czi;
+        }
+        cpr = __shfl(pr, 63);
+        cpi = __shfl(pi, 63);
+        czr = __shfl(zr, 63);
+        czi = __shfl(zi, 63);
+        if (!in) continue;
+        // d_by_sdtheta, rounded as dtheta_kernel rounds it
+        double sdpr, sdpi, sdzr, sdzi;
+        if (nl == 1) {
+            sdpr = sdpi = sdzr = sdzi = 0.0;
+        } else if (i == 0) {
+            sdpr = prn * ci.y;
+            sdpi = pin * ci.y;
+            sdzr = zrn * ci.y;
+            sdzi = zin * ci.y;
+        } else if (i == nl - 1) {
+            sdpr = prp * ci.x;
+            sdpi = pip * ci.x;
+            sdzr = zrp * ci.x;
+            sdzi = zip * ci.x;
+        } else {
+            sdpr = prp * ci.x + prn * ci.y;
+            sdpi = pip * ci.x + pin * ci.y;
+            sdzr = zrp * ci.x + zrn * ci.y;
+            sdzi = zip * ci.x + zin * ci.y;
+        }
         const double ol = l == 0 ? 1.0 : 1.0 / ((double)l * ((double)l + 1.0));  // over_l_l_plus1
-        const double sdzr = sdtheta_at(zr, c, i, nl), sdzi = sdtheta_at(zi, c, i, nl);
-        const double sdpr = sdtheta_at(pr, c, i, nl), sdpi = sdtheta_at(pi, c, i, nl);
         const bool zero = l == lmax || l == 0;
-        o[j][0] = zero ? 0.0 : W[col + i] * ol;
-        o[j][1] = zero ? 0.0 : W[half + col + i] * ol;
-        o[j][2] = zero ? 0.0 : (sdpr + -(dm * zi[i])) * ol;
-        o[j][3] = zero ? 0.0 : (sdpi + dm * zr[i]) * ol;
-        o[j][4] = zero ? 0.0 : (-(dm * pi[i]) - sdzr) * ol;
-        o[j][5] = zero ? 0.0 : (dm * pr[i] - sdzi) * ol;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kRlmPerThread; ++j) {
-        const int e = threadIdx.x + j * kAThreads;
-        if (e >= nel) break;
-        int i;
-        const long col = (long)(r0 + div_small(e, nl, rnl, i)) * nl + i;
-        W[col] = o[j][0];
-        W[half + col] = o[j][1];
-        P[col] = o[j][2];
-        P[half + col] = o[j][3];
-        Z[col] = o[j][4];
-        Z[half + col] = o[j][5];
+        W[i] = zero ? 0.0 : wr * ol;
+        W[half + i] = zero ? 0.0 : wi * ol;
+        P[i] = zero ? 0.0 : (sdpr + -(dm * zi)) * ol;
+        P[half + i] = zero ? 0.0 : (sdpi + dm * zr) * ol;
+        Z[i] = zero ? 0.0 : (-(dm * pi) - sdzr) * ol;
+        Z[half + i] = zero ? 0.0 : (dm * pr - sdzi) * ol;
         if (i == nl - 1)
             for (int f = 0; f < nf; ++f)
                 if (f != wv && f != pv && f != zv) {
-                    blk[f * fsz + col] = 0.0;
-                    blk[f * fsz + half + col] = 0.0;
+                    blk[f * fsz + (long)r * nl + i] = 0.0;
+                    blk[f * fsz + half + (long)r * nl + i] = 0.0;
                 }
     }
 }
@@ -249,12 +269,7 @@ int launch_rlm_spaceb(raysht_plan *p, double *buf, int nf, int wv, int pv, int z
     if (int rc = check_spectral(p)) return rc;
     if (nf < 3 || wv < 0 || pv < 0 || zv < 0 || wv >= nf || pv >= nf || zv >= nf || wv == pv || wv == zv || pv == zv)
         return RAYSHT_EINVAL;
-    if (p->lmax + 1 > kRlmPerThread * kAThreads) return RAYSHT_EUNSUPPORTED;
-    unsigned bx = 1;  // enough column groups for the m-block that needs most
-    for (int m : p->mvals) {
-        const int nl = p->lmax - m + 1, g = nl >= kAThreads ? 1 : kAThreads / nl;
-        bx = std::max(bx, (unsigned)((p->nr + g - 1) / g));
-    }
+    const unsigned bx = (unsigned)((p->nr + kAThreads / 64 - 1) / (kAThreads / 64));  // a wave per radius
     hipLaunchKernelGGL(rlm_spaceb_kernel, dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr,
                        p->d_mvals, p->d_lm_base, p->d_acoef + p->lm_total, buf, nf, wv, pv, zv);
     RAYSHT_HIP_TRY(hipGetLastError());

@@ -1,7 +1,7 @@
 """GPU parity of rlm_spaceb's fused combination and the l(l+1) / radial scalings
 (raysht_rlm_spaceb_hydro, raysht_lm_scale, raysht_zero_lmax: Sphere_Hybrid_Space.F90:127-214,
 :244, :70-72): bit for bit against the compiled reference's outputs (golden fixtures) and
-against the oracle for l_max from 0 to 1500 (one to six values per thread of a column),
+against the oracle for l_max from 0 to 2049 (one to 33 chunks of 64 values per column),
 m = l_max blocks, W, P, Z in any field order; in-place and cross-field scalings; a large
 buffer by exact linearity; the rejected calls."""
 import numpy as np
@@ -40,6 +40,7 @@ def test_rlm_spaceb_vs_reference_fixture(ra, golden, name):
     (300, [300, 0, 44, 1, 299], 3, 3, (0, 2, 1)),
     (1023, [0, 1, 511, 1023, 1022], 2, 4, (3, 1, 2)),
     (1500, [0, 700, 1500], 1, 3, (1, 0, 2)),
+    (2049, [2049, 0, 1985, 1000], 2, 3, (2, 1, 0)),
 ])
 def test_rlm_spaceb_vs_oracle(ra, L, mv, nr, nf, fields):
     mv = O.m_order_balanced(L) if isinstance(mv, str) else np.array(mv, np.int32)
@@ -123,6 +124,3 @@ def test_rlm_rejects_bad_calls(ra):
         plan.lm_scale(a, 0, a, 1, empty(7), None)  # lfac needs l_max + 1 values
     with pytest.raises(RayshtError):
         plan.lm_scale(a, 0, a, 9, None, None)  # no field 9
-    big = ra.LegendrePlan(4100, 2049, np.array([2049], np.int32), 1, 3)
-    with pytest.raises(RayshtError):
-        big.rlm_spaceb_hydro(empty(big.spectral_size), 0, 1, 2)  # l_max + 1 > 2048
