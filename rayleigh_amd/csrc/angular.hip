@@ -5,9 +5,11 @@
 // (1/sin(theta) d(sin^2(theta) .)/dtheta) couple l to l - 1 and l + 1 with the per-(m, l)
 // coefficients of Initialize_Angular_Derivatives (:49-83); d_by_dphi multiplies by i m.
 // All three are memory-bound stencils along the contiguous l axis of every (r, imi)
-// column: one thread per output value, neighbours from the same 256-byte l-run (L1), the
-// coefficients from a small per-plan table in L2.  Products and sums are rounded
-// separately, as the reference's compiled loops round them, so the results are its doubles.
+// column: the l-stencils run one wave per l-column with the neighbours passed between
+// lanes, the coefficients come from a small per-plan table in L2.  Products and sums are
+// rounded separately, as the reference's compiled loops round them, so the results are its
+// doubles.  The same file holds the rlm_spaceb combination built from these derivatives and
+// the l(l+1) / radial scalings of the spectral fields.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -36,7 +38,9 @@ __device__ __forceinline__ int div_small(int e, int n, float rn, int &rem)
     return q;
 }
 
-// out(l) of one field from field fin; grid (element blocks, m-block)
+// out(l) of one field from field fin; grid (column groups, m-block).  One wave per (r, imi)
+// l-column, 64 values of l at a time: the l - 1 and l + 1 neighbours come from the
+// adjacent lanes' loads (lanes 0 and 63 load theirs), so each value is loaded once.
 // Block mp of a packed array of F fields starts at lm_base[mp] * 2 nr * F (lm_base = the
 // l count of the blocks before it); field f of the block at f * nl * 2 nr from there.
 __global__ __launch_bounds__(kAThreads) void dtheta_kernel(int lmax, int nr, const int *__restrict__ mvals,
@@ -46,26 +50,37 @@ __global__ __launch_bounds__(kAThreads) void dtheta_kernel(int lmax, int nr, con
                                                            int fout)
 {
 #pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * (kAThreads / 64) + (threadIdx.x >> 6);
+    if (k >= 2 * nr) return;  // whole waves
     const int mp = blockIdx.y, m = mvals[mp], nl = lmax - m + 1;
     const long lb = lm_base[mp], ncol = 2L * nr, total = (long)nl * ncol;
-    const double *a = in + lb * ncol * nf_in + (long)fin * total;
-    double *b = out + lb * ncol * nf_out + (long)fout * total;
+    const double *ak = in + lb * ncol * nf_in + (long)fin * total + (long)k * nl;
+    double *bk = out + lb * ncol * nf_out + (long)fout * total + (long)k * nl;
     const double2 *c = coef + lb;
-    const float rnl = 1.0f / (float)nl;
-    for (long e = blockIdx.x * (long)kAThreads + threadIdx.x; e < total; e += (long)gridDim.x * kAThreads) {
-        int i;  // l - m
-        const int k = div_small((int)e, nl, rnl, i);
-        const double *ak = a + (long)k * nl;
+    for (int base = 0; base < nl; base += 64) {
+        const int i = base + lane;
+        const bool inr = i < nl;
+        double x = 0.0;
+        double2 ci = make_double2(0.0, 0.0);
+        if (inr) {
+            x = ak[i];
+            ci = c[i];
+        }
+        double xn = __shfl_down(x, 1), xp = __shfl_up(x, 1);
+        if (lane == 63 && i + 1 < nl) xn = ak[i + 1];
+        if (lane == 0 && i > 0) xp = ak[i - 1];
+        if (!inr) continue;
         double v;
         if (nl == 1)
             v = 0.0;
         else if (i == 0)
-            v = ak[1] * c[0].y;
+            v = xn * ci.y;
         else if (i == nl - 1)
-            v = ak[i - 1] * c[i].x;
+            v = xp * ci.x;
         else
-            v = ak[i - 1] * c[i].x + ak[i + 1] * c[i].y;
-        b[(long)k * nl + i] = v;
+            v = xp * ci.x + xn * ci.y;
+        bk[i] = v;
     }
 }
 
@@ -250,8 +265,8 @@ int launch_angular(raysht_plan *p, int kind, const double *in, int nf_in, int fi
     const bool same = in == out && nf_in == nf_out;
     if (same && fin == fout && kind != 2) return RAYSHT_EINVAL;  // the l stencil cannot run in place
     if (!same && in < out + n_out && out < in + n_in) return RAYSHT_EINVAL;
-    const long work = (long)max_nl(p) * (kind == 2 ? p->nr : 2L * p->nr);
-    const unsigned bx = (unsigned)std::min<long>((work + kAThreads - 1) / kAThreads, kAMaxBlocksX);
+    const long work = kind == 2 ? (long)max_nl(p) * p->nr : 2L * p->nr * 64;  // dtheta: a wave per l-column
+    const unsigned bx = (unsigned)std::min<long>((work + kAThreads - 1) / kAThreads, kind == 2 ? kAMaxBlocksX : 1L << 30);
     dim3 grid(bx, (unsigned)p->nm);
     if (kind == 2)
         hipLaunchKernelGGL(dphi_kernel, grid, dim3(kAThreads), 0, st, p->lmax, p->nr, p->d_mvals, p->d_lm_base, in,
