@@ -248,6 +248,22 @@ def cpu_reference_extras(L, mv, nr, nf, nlm, n_lu, ncols, out):
             lapack.dgetrs(lu, piv, b)
         t_s = time.perf_counter() - t0
     frac = sum(ncols[i] for i in sel) / sum(ncols)
+    kl = min(3 * (nr // 4), n_lu - 1)
+    bands = [rng.standard_normal((3 * kl + 1, n_lu)) for _ in range(k)]
+    for ab in bands:
+        ab[2 * kl] += 2 * kl
+    with threadpool_limits(limits=threads):
+        t0 = time.perf_counter()
+        bfacs = [lapack.dgbtrf(ab, kl, kl) for ab in bands]
+        tb_f = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for (lu, piv, _), b in zip(bfacs, rhs):
+            lapack.dgbtrs(lu, kl, kl, b, piv)
+        tb_s = time.perf_counter() - t0
+    res["lu_decompose_band"] = {"ms_full_size": tb_f * len(ncols) / k * 1e3, "cores": threads, "kind": "LAPACK",
+                                "sample": f"dgbtrf on {k} of {len(ncols)} band matrices, n = {n_lu}, kl = ku = {kl}"}
+    res["lu_solve_band"] = {"ms_full_size": tb_s / frac * 1e3, "cores": threads, "kind": "LAPACK",
+                            "sample": f"dgbtrs on {k} of {len(ncols)} band matrices and their right-hand sides"}
     res["lu_decompose_full"] = {"ms_full_size": t_f * len(ncols) / k * 1e3, "cores": threads, "kind": "LAPACK",
                                 "sample": f"dgetrf on {k} of {len(ncols)} matrices of n = {n_lu}"}
     res["lu_solve_full"] = {"ms_full_size": t_s / frac * 1e3, "cores": threads, "kind": "LAPACK",
@@ -416,6 +432,39 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
                         "frac_fp64_peak": flop / (ms * 1e-3) / (PEAK_FP64_TFLOPS * 1e12)}
     out["implicit_solve"] = {"n": n_lu, "matrices": nb, **lu_rec}
     del mats0, mats, rhs0, rhs
+    # the band path (band_solve) for the same W-P system on a grid of 4 Chebyshev domains of
+    # N_r / 4 polynomials: kl = ku = 3 N_r / 4 (Band_Arrange's 3 max_npoly for the linked
+    # equation), band storage ldab = 3 kl + 1; ~2 n kl (kl + ku) flop per factorisation and
+    # 2 n (2 kl + ku) per right-hand side
+    kl = min(3 * (nr // 4), n_lu - 1)
+    ldab = 3 * kl + 1
+    band0 = torch.randn(nb * ldab * n_lu, dtype=torch.float64, device="cuda", generator=gen)
+    band0.view(nb, n_lu, ldab)[:, :, 2 * kl].add_(float(2 * kl))  # a strong diagonal
+    band = torch.empty_like(band0)
+    rhs0 = torch.randn(int(offs[-1]), dtype=torch.float64, device="cuda", generator=gen)
+    rhs = torch.empty_like(rhs0)
+    band_rec = {}
+    for name, prep, fn, flop in (
+            ("lu_decompose_band", lambda: band.copy_(band0), lambda: ra.lu_decompose_band(band, piv, info, n_lu, kl),
+             nb * 2.0 * n_lu * kl * 2 * kl),
+            ("lu_solve_band", lambda: rhs.copy_(rhs0),
+             lambda: ra.lu_solve_band(band, piv, rhs, d_offs, d_ncols, n_lu, kl, max(ncols)),
+             2.0 * n_lu * 3 * kl * sum(ncols))):
+        ts = []
+        for it in range(max(warmup, 1) + steps):
+            prep()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            if it >= max(warmup, 1):
+                ts.append(e0.elapsed_time(e1))
+        ms = float(np.mean(ts))
+        band_rec[name] = {"ms": ms, "gflops": flop / (ms * 1e-3) / 1e9,
+                          "frac_fp64_peak": flop / (ms * 1e-3) / (PEAK_FP64_TFLOPS * 1e12)}
+    out["implicit_solve_band"] = {"n": n_lu, "kl": kl, "ku": kl, "matrices": nb, **band_rec}
+    del band0, band, rhs0, rhs
     out["cpu_reference"] = cpu_reference_extras(L, mv, nr, nf, nlm, n_lu, ncols, out)
     # the radial derivative recursion (d_by_dr_cp) of orders 1-3 on the spectral radial
     # buffer, as Post_Solve applies it: 16 B per coefficient (read + write)

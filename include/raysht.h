@@ -188,6 +188,18 @@ int raysht_lu_decompose_full(int n, int nbatch, double *mats, int *pivots, int *
  * max_nrhs >= every nrhs[b] sizes the launch. */
 int raysht_lu_solve_full(int n, int nbatch, const double *mats, const int *pivots, double *rhs, const long *rhs_off,
                          const int *nrhs, int max_nrhs, void *stream);
+/* Implicit_Solve's band path (band_solve, Linear_Solve.F90:423-457):
+ * raysht_lu_decompose_band = LU_Decompose_band (:1041-1055, LAPACK dgbtrf; the reference passes
+ * kl = ku = (ldab - 1) / 3) on nbatch n x n band matrices in LAPACK band storage, ldab x n
+ * column-major each (ldab >= 2 kl + ku + 1; rows 1..kl are fill-in space), stored one after
+ * another; pivots (nbatch x n ints, 1-based) and info as for the dense path.  kl + ku <= 1024.
+ * raysht_lu_solve_band = LU_Solve_Band (:1056-1086, dgbtrs 'N'), right-hand sides as for
+ * raysht_lu_solve_full.  The interleave of linked variables (Band_Arrange_RHS) stays with the
+ * caller. */
+int raysht_lu_decompose_band(int n, int kl, int ku, int nbatch, double *mats, int ldab, int *pivots, int *info,
+                             void *stream);
+int raysht_lu_solve_band(int n, int kl, int ku, int nbatch, const double *mats, int ldab, const int *pivots,
+                         double *rhs, const long *rhs_off, const int *nrhs, int max_nrhs, void *stream);
 
 /* Fused forms of the same steps (results identical to the separate calls):
  * StP then 2a->3a then c2r, and r2c then 3b->2b then PtS.  `work` is a hybrid

@@ -75,6 +75,9 @@ def oracle_lib():
         lib.orc_lu_decompose.restype = _I
         lib.orc_lu_decompose.argtypes = [_I, _P, _P]
         lib.orc_lu_solve.argtypes = [_I, _P, _P, _I, _P]
+        lib.orc_band_lu_decompose.restype = _I
+        lib.orc_band_lu_decompose.argtypes = [_I, _I, _I, _P, _I, _P]
+        lib.orc_band_lu_solve.argtypes = [_I, _I, _I, _P, _I, _P, _I, _P]
         lib.orc_lm_values.restype = _I
         lib.orc_lm_values.argtypes = [_I, _P, _I, _P, _P]
         lib.orc_permute_2b1b.argtypes = [_I, _P, _I, _I, _I, _P, _P]
@@ -328,6 +331,28 @@ def lu_solve(lu, piv, b):
     x = np.asfortranarray(b, dtype=np.float64).copy(order="F")
     nrhs = 1 if x.ndim == 1 else x.shape[1]
     oracle_lib().orc_lu_solve(n, _ptr(np.asfortranarray(lu)), _ptr(np.ascontiguousarray(piv, np.int32)), nrhs, _ptr(x))
+    return x
+
+
+def band_lu_decompose(ab, kl, ku):
+    """LU_Decompose_band (LAPACK dgbtrf, unblocked dgbtf2 order) of a band matrix in LAPACK
+    band storage ab(ldab, n) (column-major, ldab >= 2 kl + ku + 1): returns (factor, 1-based
+    pivots, info)."""
+    f = np.asfortranarray(ab, dtype=np.float64).copy(order="F")
+    ldab, n = f.shape
+    piv = np.zeros(n, np.int32)
+    info = oracle_lib().orc_band_lu_decompose(n, kl, ku, _ptr(f), ldab, _ptr(piv))
+    return f, piv, info
+
+
+def band_lu_solve(fac, kl, ku, piv, b):
+    """LU_Solve_Band (LAPACK dgbtrs 'N'): b is n x nrhs, column-major."""
+    fac = np.asfortranarray(fac, dtype=np.float64)
+    ldab, n = fac.shape
+    x = np.asfortranarray(b, dtype=np.float64).copy(order="F")
+    nrhs = 1 if x.ndim == 1 else x.shape[1]
+    oracle_lib().orc_band_lu_solve(n, kl, ku, _ptr(fac), ldab, _ptr(np.ascontiguousarray(piv, np.int32)), nrhs,
+                                   _ptr(x))
     return x
 
 

@@ -931,3 +931,95 @@ void orc_zero_lmax(int lmax, const int *mvals, int nm, int nr, double *buf, int 
         for (long k = 0; k < 2L * nr * nf; ++k) blk[k * nl + nl - 1] = 0.0;
     }
 }
+
+/* LU_Decompose_band (Linear_Solve.F90:1041-1055): LAPACK dgbtrf on an n x n band matrix in
+ * LAPACK band storage ab(ldab, n), ldab >= 2 kl + ku + 1 (the reference passes kl = ku =
+ * (ldab - 1) / 3), restated as the unblocked dgbtf2 that dgbtrf runs for ku <= 64: fill-in rows
+ * zeroed, per column the first largest |a| of the kl + 1 candidates as pivot, the row
+ * interchange over columns j..ju, the multipliers scaled by the reciprocal of the pivot (dscal)
+ * and the rank-1 update in reference-BLAS dger order (a column skipped where its y is 0;
+ * a + x (-y), rounded separately).  Pivots 1-based; returns info (first zero pivot or 0). */
+int orc_band_lu_decompose(int n, int kl, int ku, double *ab, int ldab, int *piv)
+{
+    const int kv = ku + kl;
+#define AB(i, j) ab[(long)((j) - 1) * ldab + ((i) - 1)]
+    int info = 0;
+    for (int j = ku + 2; j <= (kv < n ? kv : n); ++j)
+        for (int i = kv - j + 2; i <= kl; ++i) AB(i, j) = 0.0;
+    int ju = 1;
+    for (int j = 1; j <= n; ++j) {
+        if (j + kv <= n)
+            for (int i = 1; i <= kl; ++i) AB(i, j + kv) = 0.0;
+        const int km = kl < n - j ? kl : n - j;
+        int jp = 1;
+        double best = fabs(AB(kv + 1, j));
+        for (int i = 2; i <= km + 1; ++i)
+            if (fabs(AB(kv + i, j)) > best) {
+                best = fabs(AB(kv + i, j));
+                jp = i;
+            }
+        piv[j - 1] = jp + j - 1;
+        if (AB(kv + jp, j) != 0.0) {
+            const int jn = j + ku + jp - 1 < n ? j + ku + jp - 1 : n;
+            if (jn > ju) ju = jn;
+            if (jp != 1)
+                for (int c = 0; c <= ju - j; ++c) {
+                    const double t = AB(kv + jp - c, j + c);
+                    AB(kv + jp - c, j + c) = AB(kv + 1 - c, j + c);
+                    AB(kv + 1 - c, j + c) = t;
+                }
+            if (km > 0) {
+                const double r = 1.0 / AB(kv + 1, j);
+                for (int i = 1; i <= km; ++i) AB(kv + 1 + i, j) = r * AB(kv + 1 + i, j);
+                for (int c = 1; c <= ju - j; ++c) {
+                    const double y = AB(kv + 1 - c, j + c);
+                    if (y == 0.0) continue;
+                    const double t = -y;
+                    for (int i = 1; i <= km; ++i) AB(kv + 1 + i - c, j + c) = AB(kv + 1 + i - c, j + c) + AB(kv + 1 + i, j) * t;
+                }
+            }
+        } else if (info == 0) {
+            info = j;
+        }
+    }
+#undef AB
+    return info;
+}
+
+/* LU_Solve_Band (Linear_Solve.F90:1056-1086): dgbtrs 'N' with nrhs columns of b (leading
+ * dimension n): L solve per column j of the factor (row interchange, then dger's b(j+i) +
+ * l(i) (-b(j)), a right-hand side skipped where b(j) = 0), then per right-hand side the
+ * reference-BLAS dtbsv upper non-unit solve (x(j) / u(j,j), then x(i) - x(j) u(i,j) for the
+ * kl + ku rows above, skipped where x(j) = 0). */
+void orc_band_lu_solve(int n, int kl, int ku, const double *ab, int ldab, const int *piv, int nrhs, double *b)
+{
+    const int kd = ku + kl + 1, k = kl + ku;
+#define AB(i, j) ab[(long)((j) - 1) * ldab + ((i) - 1)]
+#define B(i, c) b[(long)(c) * n + ((i) - 1)]
+    if (kl > 0)
+        for (int j = 1; j <= n - 1; ++j) {
+            const int lm = kl < n - j ? kl : n - j, l = piv[j - 1];
+            if (l != j)
+                for (int c = 0; c < nrhs; ++c) {
+                    const double t = B(l, c);
+                    B(l, c) = B(j, c);
+                    B(j, c) = t;
+                }
+            for (int c = 0; c < nrhs; ++c) {
+                const double y = B(j, c);
+                if (y == 0.0) continue;
+                const double t = -y;
+                for (int i = 1; i <= lm; ++i) B(j + i, c) = B(j + i, c) + AB(kd + i, j) * t;
+            }
+        }
+    for (int c = 0; c < nrhs; ++c)
+        for (int j = n; j >= 1; --j) {
+            if (B(j, c) == 0.0) continue;
+            B(j, c) = B(j, c) / AB(k + 1, j);
+            const double t = B(j, c);
+            const int lo = j - k > 1 ? j - k : 1;
+            for (int i = j - 1; i >= lo; --i) B(i, c) = B(i, c) - t * AB(k + 1 + i - j, j);
+        }
+#undef AB
+#undef B
+}

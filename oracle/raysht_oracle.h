@@ -71,6 +71,8 @@ void orc_lm_scale(int lmax, const int *mvals, int nm, int nr, const double *in, 
 void orc_zero_lmax(int lmax, const int *mvals, int nm, int nr, double *buf, int nf);
 int orc_lu_decompose(int n, double *a, int *piv);
 void orc_lu_solve(int n, const double *a, const int *piv, int nrhs, double *b);
+int orc_band_lu_decompose(int n, int kl, int ku, double *ab, int ldab, int *piv);
+void orc_band_lu_solve(int n, int kl, int ku, const double *ab, int ldab, const int *piv, int nrhs, double *b);
 int orc_lm_values(int lmax, const int *mvals, int nm, int *l_out, int *mp_out);
 void orc_permute_2b1b(int lmax, const int *mvals, int nm, int nr, int nf, const double *s2b, double *p1b);
 void orc_permute_1a2a(int lmax, const int *mvals, int nm, int nr, int nf, const double *p1a, double *s2a);

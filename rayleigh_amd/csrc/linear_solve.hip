@@ -238,6 +238,215 @@ void launch_lanes(int n, int nbatch, const double *mats, const int *pivots, doub
 
 using namespace raysht;
 
+// Banded factorisation: LU_Decompose_band (Linear_Solve.F90:1041-1055, dgbtrf with
+// kl = ku = (ldab - 1) / 3) in the unblocked order of dgbtf2, one workgroup per matrix in
+// LAPACK band storage ab(ldab, n) (1-based AB(i, j) below; full element (r, c) at row
+// kv + 1 + r - c of column c, kv = kl + ku).  Per column: the fill-in rows LAPACK clears, the
+// pivot (first largest |a| of the kl + 1 candidates, idamax's rules for NaN), the interchange
+// over columns j..ju, the multipliers scaled by the pivot's reciprocal, and the rank-1 update
+// of the km x (ju - j) block, a column skipped where the pivot row's entry is 0 (dger);
+// products and sums rounded separately, so the factors are the oracle's doubles.
+constexpr int kBandMaxKv = 1024;
+__global__ __launch_bounds__(kLuThreads) void band_lu_decompose_kernel(int n, int kl, int ku, int ldab,
+                                                                       double *__restrict__ mats,
+                                                                       int *__restrict__ pivots,
+                                                                       int *__restrict__ infos)
+{
+#pragma clang fp contract(off)
+    __shared__ double s_val[kLuThreads];
+    __shared__ int s_idx[kLuThreads];
+    __shared__ double s_y[kBandMaxKv + 1];  // the pivot row, columns j + 1 .. ju
+    const int tid = threadIdx.x;
+    double *ab = mats + (long)blockIdx.x * ldab * n;
+    int *piv = pivots + (long)blockIdx.x * n;
+    const int kv = ku + kl;
+#define AB(i, j) ab[(long)((j) - 1) * ldab + ((i) - 1)]
+    // dgbtf2's zeroed fill-in: rows max(1, kv - c + 2) .. kl of columns c = ku + 2 .. n
+    for (long e = tid; e < (long)kl * n; e += kLuThreads) {
+        const int i = 1 + (int)(e % kl), c = 1 + (int)(e / kl);
+        if (c >= ku + 2 && i >= kv - c + 2) AB(i, c) = 0.0;
+    }
+    __syncthreads();
+    int info = 0, ju = 1;
+    for (int j = 1; j <= n; ++j) {
+        const int km = min(kl, n - j);
+        double best = -1.0;
+        int bi = 1 << 30;
+        for (int i = 1 + tid; i <= km + 1; i += kLuThreads) {
+            const double v = fabs(AB(kv + i, j));
+            if (v > best) {
+                best = v;
+                bi = i;
+            }
+        }
+        s_val[tid] = best;
+        s_idx[tid] = bi;
+        __syncthreads();
+        for (int sh = kLuThreads / 2; sh > 0; sh >>= 1) {
+            if (tid < sh) {
+                const double v = s_val[tid + sh];
+                const int q = s_idx[tid + sh];
+                if (v > s_val[tid] || (v == s_val[tid] && q < s_idx[tid])) {
+                    s_val[tid] = v;
+                    s_idx[tid] = q;
+                }
+            }
+            __syncthreads();
+        }
+        // idamax: a NaN first entry is kept (nothing compares larger); later NaNs are passed over
+        const int jp = (s_idx[0] > km + 1 || isnan(AB(kv + 1, j))) ? 1 : s_idx[0];
+        const double pv = AB(kv + jp, j);
+        __syncthreads();  // every thread has the pivot before the interchange moves it
+        if (tid == 0) piv[j - 1] = jp + j - 1;
+        if (pv != 0.0) {
+            ju = max(ju, min(j + ku + jp - 1, n));
+            if (jp != 1)
+                for (int c = tid; c <= ju - j; c += kLuThreads) {
+                    const double t = AB(kv + jp - c, j + c);
+                    AB(kv + jp - c, j + c) = AB(kv + 1 - c, j + c);
+                    AB(kv + 1 - c, j + c) = t;
+                }
+            __syncthreads();
+            if (km > 0) {
+                const double r = 1.0 / AB(kv + 1, j);
+                for (int i = 1 + tid; i <= km; i += kLuThreads) AB(kv + 1 + i, j) = r * AB(kv + 1 + i, j);
+                for (int c = 1 + tid; c <= ju - j; c += kLuThreads) s_y[c] = AB(kv + 1 - c, j + c);
+                __syncthreads();
+                const int w = ju - j;
+                for (int e = tid; e < km * w; e += kLuThreads) {
+                    const int i = 1 + e % km, c = 1 + e / km;
+                    const double y = s_y[c];
+                    if (y != 0.0) AB(kv + 1 + i - c, j + c) = AB(kv + 1 + i - c, j + c) + AB(kv + 1 + i, j) * (-y);
+                }
+                __syncthreads();
+            }
+        } else if (info == 0) {
+            info = j;
+        }
+    }
+    if (tid == 0) infos[blockIdx.x] = info;
+#undef AB
+}
+
+// Banded solve: LU_Solve_Band (Linear_Solve.F90:1056-1086, dgbtrs 'N').  One wave solves CPL
+// right-hand-side columns of matrix blockIdx.y, held in registers across its lanes (row
+// 64 s + lane in slot s, NS slots).  Per step j the wave reads x_j from its owner lane and
+// updates only the slots the band reaches, with one coalesced load of the factor column shared
+// by its CPL columns: the L solve (interchange, then dger's x(r) + l (-x(j)), skipped where
+// x(j) = 0) and dtbsv's upper solve (x(j) / u(j, j), then x(r) - x(j) u(r, j) for the kl + ku
+// rows above, skipped where x(j) = 0).  Factor entry (r, j) of both is band row kv + r - j.
+template <int NS>
+__device__ __forceinline__ double band_row(const double (&x)[NS], int row)
+{
+    double v = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        if (s == (row >> 6)) v = x[s];
+    return __shfl(v, row & 63);
+}
+
+template <int NS>
+__device__ __forceinline__ void band_set(double (&x)[NS], int row, int lane, double v)
+{
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        if (s == (row >> 6) && lane == (row & 63)) x[s] = v;
+}
+
+template <int NS, int CPL>
+__global__ __launch_bounds__(64) void band_lu_solve_kernel(int n, int kl, int ku, int ldab,
+                                                           const double *__restrict__ mats,
+                                                           const int *__restrict__ pivots, double *__restrict__ rhs,
+                                                           const long *__restrict__ rhs_off,
+                                                           const int *__restrict__ nrhs)
+{
+#pragma clang fp contract(off)
+    const int b = blockIdx.y, lane = threadIdx.x, c0 = blockIdx.x * CPL;
+    const int nc = min(CPL, nrhs[b] - c0);
+    if (nc <= 0) return;
+    const double *ab = mats + (long)b * ldab * n;
+    const int *piv = pivots + (long)b * n;
+    double *x0 = rhs + rhs_off[b] + (long)c0 * n;
+    const int kv = kl + ku;
+    double x[CPL][NS];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int r = s * 64 + lane;
+            x[c][s] = (c < nc && r < n) ? x0[(long)c * n + r] : 0.0;
+        }
+    if (kl > 0)
+        for (int j = 0; j < n - 1; ++j) {
+            const int lm = min(kl, n - 1 - j), l = piv[j] - 1;
+            double y[CPL];
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                if (l != j) {
+                    const double xl = band_row(x[c], l), xj = band_row(x[c], j);
+                    band_set(x[c], j, lane, xl);
+                    band_set(x[c], l, lane, xj);
+                }
+                y[c] = band_row(x[c], j);
+            }
+            const int slo = (j + 1) >> 6, shi = (j + lm) >> 6;
+            const double *col = ab + (long)j * ldab + kv - j;  // col[r] = factor entry (r, j)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                if (s < slo || s > shi) continue;
+                const int r = s * 64 + lane;
+                if (r <= j || r > j + lm) continue;
+                const double a = col[r];
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                    if (y[c] != 0.0) x[c][s] = x[c][s] + a * (-y[c]);
+            }
+        }
+    for (int j = n - 1; j >= 0; --j) {
+        const double *col = ab + (long)j * ldab + kv - j;
+        const double d = col[j];
+        double t[CPL];
+        bool nz[CPL];  // dtbsv skips a column where x(j) is 0 before the division
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            t[c] = band_row(x[c], j);
+            nz[c] = t[c] != 0.0;
+            if (nz[c]) {
+                t[c] = t[c] / d;
+                band_set(x[c], j, lane, t[c]);
+            }
+        }
+        const int lo = max(0, j - kv), slo = lo >> 6, shi = (j - 1) >> 6;
+        if (j == 0) continue;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if (s < slo || s > shi) continue;
+            const int r = s * 64 + lane;
+            if (r < lo || r >= j) continue;
+            const double a = col[r];
+#pragma unroll
+            for (int c = 0; c < CPL; ++c)
+                if (nz[c]) x[c][s] = x[c][s] - t[c] * a;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int r = s * 64 + lane;
+            if (c < nc && r < n) x0[(long)c * n + r] = x[c][s];
+        }
+}
+
+template <int NS, int CPL>
+void launch_band_solve(int n, int kl, int ku, int nbatch, const double *mats, int ldab, const int *pivots,
+                       double *rhs, const long *rhs_off, const int *nrhs, int max_nrhs, hipStream_t st)
+{
+    dim3 grid((unsigned)((max_nrhs + CPL - 1) / CPL), (unsigned)nbatch);
+    hipLaunchKernelGGL((band_lu_solve_kernel<NS, CPL>), grid, dim3(64), 0, st, n, kl, ku, ldab, mats, pivots, rhs,
+                       rhs_off, nrhs);
+}
+
 extern "C" {
 
 int raysht_lu_decompose_full(int n, int nbatch, double *mats, int *pivots, int *info, void *stream)
@@ -269,6 +478,42 @@ int raysht_lu_solve_full(int n, int nbatch, const double *mats, const int *pivot
         launch_lanes<32, 16, 3>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else
         launch_lanes<64, 16, 3>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+
+int raysht_lu_decompose_band(int n, int kl, int ku, int nbatch, double *mats, int ldab, int *pivots, int *info,
+                             void *stream)
+{
+    if (n < 1 || kl < 0 || ku < 0 || nbatch < 0 || ldab < 2 * kl + ku + 1) return RAYSHT_EINVAL;
+    if (nbatch > 0 && (!mats || !pivots || !info)) return RAYSHT_EINVAL;
+    if (kl + ku > kBandMaxKv) return RAYSHT_EUNSUPPORTED;
+    if (nbatch == 0) return RAYSHT_OK;
+    hipLaunchKernelGGL(band_lu_decompose_kernel, dim3((unsigned)nbatch), dim3(kLuThreads), 0,
+                       reinterpret_cast<hipStream_t>(stream), n, kl, ku, ldab, mats, pivots, info);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+int raysht_lu_solve_band(int n, int kl, int ku, int nbatch, const double *mats, int ldab, const int *pivots,
+                         double *rhs, const long *rhs_off, const int *nrhs, int max_nrhs, void *stream)
+{
+    if (n < 1 || kl < 0 || ku < 0 || nbatch < 0 || max_nrhs < 0 || ldab < 2 * kl + ku + 1) return RAYSHT_EINVAL;
+    if (nbatch > 0 && (!mats || !pivots || !rhs || !rhs_off || !nrhs)) return RAYSHT_EINVAL;
+    if (nbatch > 65535 || n > kLuMaxN) return RAYSHT_EUNSUPPORTED;
+    if (nbatch == 0 || max_nrhs == 0) return RAYSHT_OK;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (n <= 64)
+        launch_band_solve<1, 8>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+    else if (n <= 128)
+        launch_band_solve<2, 8>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+    else if (n <= 256)
+        launch_band_solve<4, 4>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+    else if (n <= 512)
+        launch_band_solve<8, 4>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+    else
+        launch_band_solve<16, 2>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }

@@ -398,6 +398,33 @@ def _iptr(t, name: str, dtype, numel: int) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr())
 
 
+def band_storage(a, kl: int, ku: int | None = None, nlinks: int = 1) -> np.ndarray:
+    """Band_Load_Single (Linear_Solve.F90:1358-1409) on the host: an n x n matrix into LAPACK
+    band storage ab(2 kl + ku + 1, n) (rows 1..kl left for fill-in, a(r, i) at row
+    kl + ku + 1 + r - i of column i; ku = kl as the reference uses).  With nlinks > 1 the
+    rows and columns of the nlinks coupled variables (blocks of n / nlinks) are first
+    interleaved (block i row r -> row nlinks r + i), which makes a linked system banded."""
+    a = np.asarray(a, dtype=np.float64)
+    n = a.shape[0]
+    ku = kl if ku is None else ku
+    if nlinks > 1:
+        p = band_interleave(n // nlinks, nlinks)
+        a = a[np.ix_(p, p)]
+    ab = np.zeros((2 * kl + ku + 1, n), order="F")
+    for i in range(n):
+        lo, hi = max(0, i - ku), min(n, i + kl + 1)
+        ab[kl + ku + np.arange(lo, hi) - i, i] = a[lo:hi, i]
+    return ab
+
+
+def band_interleave(ndim1: int, nlinks: int) -> np.ndarray:
+    """The row order of Band_Arrange_RHS (Linear_Solve.F90:1139-1162): new row nlinks r + i takes
+    old row r + i ndim1 (0-based), so x_banded = x[band_interleave(...)] and
+    x[band_interleave(...)] = x_banded undoes it (Band_ReArrange_RHS)."""
+    r, i = np.meshgrid(np.arange(ndim1), np.arange(nlinks), indexing="ij")
+    return (r + i * ndim1).ravel()
+
+
 def lu_decompose_full(mats, pivots, info, n: int, stream=None):
     """LU_Decompose_full (LAPACK dgetrf, Linear_Solve.F90:1034-1045) of every n x n
     column-major matrix in `mats` (float64, nbatch n^2), in place; pivots (int32, nbatch n,
@@ -422,6 +449,39 @@ def lu_solve_full(mats, pivots, rhs, rhs_off, nrhs, n: int, max_nrhs: int, strea
                                                _iptr(rhs_off, "rhs_off", torch.int64, nb),
                                                _iptr(nrhs, "nrhs", torch.int32, nb), int(max_nrhs), _stream(stream)),
                "raysht_lu_solve_full")
+
+
+def lu_decompose_band(mats, pivots, info, n: int, kl: int, ku: int | None = None, ldab: int | None = None,
+                      stream=None):
+    """LU_Decompose_band (LAPACK dgbtrf, Linear_Solve.F90:1041-1055) of every band matrix in
+    `mats` (float64, nbatch ldab n, LAPACK band storage as band_storage builds it; ku = kl and
+    ldab = 2 kl + ku + 1 by default), in place; pivots (int32, nbatch n, 1-based), info
+    (int32, nbatch)."""
+    import torch
+    ku = kl if ku is None else ku
+    ldab = 2 * kl + ku + 1 if ldab is None else ldab
+    if mats.numel() % (ldab * n):
+        raise ValueError("mats must hold whole ldab x n band matrices")
+    nb = mats.numel() // (ldab * n)
+    _lib.check(_lib.lib().raysht_lu_decompose_band(int(n), int(kl), int(ku), nb, _dptr(mats, "mats"), int(ldab),
+                                                   _iptr(pivots, "pivots", torch.int32, nb * n),
+                                                   _iptr(info, "info", torch.int32, nb), _stream(stream)),
+               "raysht_lu_decompose_band")
+
+
+def lu_solve_band(mats, pivots, rhs, rhs_off, nrhs, n: int, kl: int, max_nrhs: int, ku: int | None = None,
+                  ldab: int | None = None, stream=None):
+    """LU_Solve_Band (dgbtrs 'N', Linear_Solve.F90:1056-1086): matrix b of the batch solves its
+    nrhs[b] columns of n at rhs[rhs_off[b]:] in place."""
+    import torch
+    ku = kl if ku is None else ku
+    ldab = 2 * kl + ku + 1 if ldab is None else ldab
+    nb = mats.numel() // (ldab * n)
+    _lib.check(_lib.lib().raysht_lu_solve_band(int(n), int(kl), int(ku), nb, _dptr(mats, "mats"), int(ldab),
+                                               _iptr(pivots, "pivots", torch.int32, nb * n), _dptr(rhs, "rhs"),
+                                               _iptr(rhs_off, "rhs_off", torch.int64, nb),
+                                               _iptr(nrhs, "nrhs", torch.int32, nb), int(max_nrhs), _stream(stream)),
+               "raysht_lu_solve_band")
 
 
 class ChebyGrid:
