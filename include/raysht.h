@@ -200,6 +200,13 @@ int raysht_lu_decompose_band(int n, int kl, int ku, int nbatch, double *mats, in
                              void *stream);
 int raysht_lu_solve_band(int n, int kl, int ku, int nbatch, const double *mats, int ldab, const int *pivots,
                          double *rhs, const long *rhs_off, const int *nrhs, int max_nrhs, void *stream);
+/* Host-array forms for one matrix (synchronous), what the Fortran LU_Decompose_full /
+ * LU_Solve_full / LU_Decompose_band / LU_Solve_Band call: kl < 0 selects the dense path (mat
+ * n x n, leading dimension lda >= n), kl >= 0 the band path (mat in band storage, lda = ldab);
+ * rhs holds nrhs columns with leading dimension ldb >= n. */
+int raysht_lu_decompose_host(int n, int kl, int ku, double *mat, int lda, int *pvt, int *info);
+int raysht_lu_solve_host(int n, int kl, int ku, const double *mat, int lda, const int *pvt, double *rhs, int ldb,
+                         int nrhs);
 
 /* Fused forms of the same steps (results identical to the separate calls):
  * StP then 2a->3a then c2r, and r2c then 3b->2b then PtS.  `work` is a hybrid

@@ -79,6 +79,8 @@ SIGNATURES = {
     "raysht_lu_solve_full": (_I, [_I, _I, _P, _P, _P, _P, _P, _I, _P]),
     "raysht_lu_decompose_band": (_I, [_I, _I, _I, _I, _P, _I, _P, _P, _P]),
     "raysht_lu_solve_band": (_I, [_I, _I, _I, _I, _P, _I, _P, _P, _P, _P, _I, _P]),
+    "raysht_lu_decompose_host": (_I, [_I, _I, _I, _P, _I, _P, _P]),
+    "raysht_lu_solve_host": (_I, [_I, _I, _I, _P, _I, _P, _P, _I, _I]),
 }
 
 _lib = None

@@ -37,6 +37,14 @@ struct scoped_device_buffer {
     }
 };
 
+struct scoped_device_bytes {
+    void *ptr = nullptr;
+    ~scoped_device_bytes()
+    {
+        if (ptr) (void)hipFree(ptr);
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -192,6 +200,67 @@ int raysht_cheb_deriv_host(raysht_cheb *cb, const double *c, double *d, long nco
     int rc = raysht_cheb_deriv(cb, din.ptr, dout.ptr, ncols, dorder, nullptr);
     if (rc != RAYSHT_OK) return rc;
     RAYSHT_HIP_TRY(hipMemcpy(d, dout.ptr, bytes, hipMemcpyDeviceToHost));
+    return RAYSHT_OK;
+}
+
+// One matrix of the implicit solve from host arrays (synchronous), what the Fortran
+// LU_Decompose_full / LU_Decompose_band call: kl < 0 is the dense path (mat n x n with leading
+// dimension lda >= n), kl >= 0 the band path (mat in LAPACK band storage, lda = ldab).
+int raysht_lu_decompose_host(int n, int kl, int ku, double *mat, int lda, int *pvt, int *info)
+{
+    const bool dense = kl < 0;
+    if (n < 1 || !mat || !pvt || !info || lda < (dense ? n : 1)) return RAYSHT_EINVAL;  // band: lda = ldab
+    const long rows = dense ? n : lda;  // device leading dimension
+    scoped_device_buffer dm;
+    scoped_device_bytes dp, di;
+    if (hipMalloc(&dm.ptr, (size_t)rows * n * sizeof(double)) != hipSuccess) return RAYSHT_ENOMEM;
+    if (hipMalloc(&dp.ptr, (size_t)n * sizeof(int)) != hipSuccess) return RAYSHT_ENOMEM;
+    if (hipMalloc(&di.ptr, sizeof(int)) != hipSuccess) return RAYSHT_ENOMEM;
+    RAYSHT_HIP_TRY(hipMemcpy2D(dm.ptr, rows * sizeof(double), mat, (size_t)lda * sizeof(double), rows * sizeof(double),
+                               n, hipMemcpyHostToDevice));
+    const int rc = dense ? raysht_lu_decompose_full(n, 1, dm.ptr, (int *)dp.ptr, (int *)di.ptr, nullptr)
+                         : raysht_lu_decompose_band(n, kl, ku, 1, dm.ptr, lda, (int *)dp.ptr, (int *)di.ptr, nullptr);
+    if (rc != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipDeviceSynchronize());
+    RAYSHT_HIP_TRY(hipMemcpy2D(mat, (size_t)lda * sizeof(double), dm.ptr, rows * sizeof(double), rows * sizeof(double),
+                               n, hipMemcpyDeviceToHost));
+    RAYSHT_HIP_TRY(hipMemcpy(pvt, dp.ptr, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    RAYSHT_HIP_TRY(hipMemcpy(info, di.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    return RAYSHT_OK;
+}
+
+// LU_Solve_full / LU_Solve_Band from host arrays (synchronous): nrhs columns of n at rhs with
+// leading dimension ldb >= n, overwritten by the solutions; mat and kl as for the factorisation.
+int raysht_lu_solve_host(int n, int kl, int ku, const double *mat, int lda, const int *pvt, double *rhs, int ldb,
+                         int nrhs)
+{
+    const bool dense = kl < 0;
+    if (n < 1 || nrhs < 0 || !mat || !pvt || !rhs || lda < (dense ? n : 1) || ldb < n) return RAYSHT_EINVAL;
+    if (nrhs == 0) return RAYSHT_OK;
+    const long rows = dense ? n : lda;
+    scoped_device_buffer dm, db;
+    scoped_device_bytes dp, doff, dn;
+    if (hipMalloc(&dm.ptr, (size_t)rows * n * sizeof(double)) != hipSuccess) return RAYSHT_ENOMEM;
+    if (hipMalloc(&db.ptr, (size_t)n * nrhs * sizeof(double)) != hipSuccess) return RAYSHT_ENOMEM;
+    if (hipMalloc(&dp.ptr, (size_t)n * sizeof(int)) != hipSuccess) return RAYSHT_ENOMEM;
+    if (hipMalloc(&doff.ptr, sizeof(long)) != hipSuccess) return RAYSHT_ENOMEM;
+    if (hipMalloc(&dn.ptr, sizeof(int)) != hipSuccess) return RAYSHT_ENOMEM;
+    const long zero = 0;
+    RAYSHT_HIP_TRY(hipMemcpy2D(dm.ptr, rows * sizeof(double), mat, (size_t)lda * sizeof(double), rows * sizeof(double),
+                               n, hipMemcpyHostToDevice));
+    RAYSHT_HIP_TRY(hipMemcpy2D(db.ptr, (size_t)n * sizeof(double), rhs, (size_t)ldb * sizeof(double),
+                               (size_t)n * sizeof(double), nrhs, hipMemcpyHostToDevice));
+    RAYSHT_HIP_TRY(hipMemcpy(dp.ptr, pvt, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    RAYSHT_HIP_TRY(hipMemcpy(doff.ptr, &zero, sizeof(long), hipMemcpyHostToDevice));
+    RAYSHT_HIP_TRY(hipMemcpy(dn.ptr, &nrhs, sizeof(int), hipMemcpyHostToDevice));
+    const int rc = dense ? raysht_lu_solve_full(n, 1, dm.ptr, (const int *)dp.ptr, db.ptr, (const long *)doff.ptr,
+                                                (const int *)dn.ptr, nrhs, nullptr)
+                         : raysht_lu_solve_band(n, kl, ku, 1, dm.ptr, lda, (const int *)dp.ptr, db.ptr,
+                                                (const long *)doff.ptr, (const int *)dn.ptr, nrhs, nullptr);
+    if (rc != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipDeviceSynchronize());
+    RAYSHT_HIP_TRY(hipMemcpy2D(rhs, (size_t)ldb * sizeof(double), db.ptr, (size_t)n * sizeof(double),
+                               (size_t)n * sizeof(double), nrhs, hipMemcpyDeviceToHost));
     return RAYSHT_OK;
 }
 

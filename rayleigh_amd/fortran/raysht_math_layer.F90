@@ -34,6 +34,7 @@ Module RaySHT_Math_Layer
     Public :: FFT_To_Physical, FFT_To_Spectral, FFT_To_Spectral_RSC
     Public :: Cheby_Init, Cheby_To_Spectral, Cheby_From_Spectral, Cheby_Finalize, Cheby_d_by_dr_cp
     Public :: d_by_dtheta, d_by_sdtheta, d_by_dphi
+    Public :: LU_Decompose_full, LU_Solve_full, LU_Decompose_band, LU_Solve_Band
     Public :: coloc, gl_weights
     ! arithmetic of the Legendre contractions (enum raysht_precision of include/raysht.h)
     Integer, Parameter, Public :: raysht_fp64 = 0, raysht_bf16x3 = 1, raysht_bf16x6 = 2, &
@@ -140,6 +141,18 @@ Module RaySHT_Math_Layer
             Type(C_PTR), Value :: plan
             Integer(C_INT), Value :: kind, nf_in, fin, nf_out, fout
             Type(C_PTR), Intent(In) :: in_m(*), out_m(*)
+            Integer(C_INT) :: st
+        End Function
+        Function c_lu_dec_host(n, kl, ku, mat, lda, pvt, info) Bind(C, name='raysht_lu_decompose_host') Result(st)
+            Import :: C_INT, C_PTR
+            Integer(C_INT), Value :: n, kl, ku, lda
+            Type(C_PTR), Value :: mat, pvt, info
+            Integer(C_INT) :: st
+        End Function
+        Function c_lu_sol_host(n, kl, ku, mat, lda, pvt, rhs, ldb, nrhs) Bind(C, name='raysht_lu_solve_host') Result(st)
+            Import :: C_INT, C_PTR
+            Integer(C_INT), Value :: n, kl, ku, lda, ldb, nrhs
+            Type(C_PTR), Value :: mat, pvt, rhs
             Integer(C_INT) :: st
         End Function
         Function c_strerror(status) Bind(C, name='raysht_strerror') Result(msg)
@@ -510,5 +523,58 @@ Contains
         Type(rmcontainer3d), Intent(InOut), Target :: arr(:)
         Call angular_4dbuff2arr(2, A, fin, arr)
     End Subroutine dphi_4dbuff2arr_raysht
+
+    ! Linear_Solve.F90:1034-1040 LU_Decompose_full (dgetrf) of one matrix, in place
+    Subroutine LU_Decompose_full(mat, pvt)
+        Real*8, Intent(InOut), Target :: mat(:,:)
+        Integer, Intent(Out), Target :: pvt(:)
+        Integer(C_INT), Target :: info
+        Call check(c_lu_dec_host(Int(Size(mat,2), C_INT), -1_C_INT, -1_C_INT, c_loc(mat), Int(Size(mat,1), C_INT), &
+                                 c_loc(pvt), c_loc(info)), 'LU_Decompose_full')
+    End Subroutine LU_Decompose_full
+
+    ! Linear_Solve.F90:1087-1107 LU_Solve_full (dgetrs 'N'): na rows (default Size(mat,1)),
+    ! nb right-hand sides (default Size(rhs)/Size(rhs,1)), leading dimension Size(rhs,1)
+    Subroutine LU_Solve_full(mat, rhs, pvt, na, nb)
+        Real*8, Intent(In), Target :: mat(:,:)
+        Real*8, Intent(InOut), Target :: rhs(:,:,:)
+        Integer, Intent(In), Target :: pvt(:)
+        Integer, Optional :: na, nb
+        Integer :: ma, mb
+        ma = Size(mat,1)
+        If (Present(na)) ma = na
+        mb = Size(rhs)/Size(rhs,1)
+        If (Present(nb)) mb = nb
+        Call check(c_lu_sol_host(Int(ma, C_INT), -1_C_INT, -1_C_INT, c_loc(mat), Int(Size(mat,1), C_INT), c_loc(pvt), &
+                                 c_loc(rhs), Int(Size(rhs,1), C_INT), Int(mb, C_INT)), 'LU_Solve_full')
+    End Subroutine LU_Solve_full
+
+    ! Linear_Solve.F90:1041-1055 LU_Decompose_band (dgbtrf, kl = ku = (ldab - 1) / 3)
+    Subroutine LU_Decompose_band(mat, pvt)
+        Real*8, Intent(InOut), Target :: mat(:,:)
+        Integer, Intent(Out), Target :: pvt(:)
+        Integer(C_INT), Target :: info
+        Integer(C_INT) :: kl
+        kl = Int((Size(mat,1) - 1) / 3, C_INT)
+        Call check(c_lu_dec_host(Int(Size(mat,2), C_INT), kl, kl, c_loc(mat), Int(Size(mat,1), C_INT), c_loc(pvt), &
+                                 c_loc(info)), 'LU_Decompose_band')
+    End Subroutine LU_Decompose_band
+
+    ! Linear_Solve.F90:1056-1086 LU_Solve_Band (dgbtrs 'N'): na rows (default Size(mat,2))
+    Subroutine LU_Solve_Band(mat, rhs, pvt, na, nb)
+        Real*8, Intent(In), Target :: mat(:,:)
+        Real*8, Intent(InOut), Target :: rhs(:,:,:)
+        Integer, Intent(In), Target :: pvt(:)
+        Integer, Optional :: na, nb
+        Integer :: ma, mb
+        Integer(C_INT) :: kl
+        ma = Size(mat,2)
+        If (Present(na)) ma = na
+        mb = Size(rhs)/Size(rhs,1)
+        If (Present(nb)) mb = nb
+        kl = Int((Size(mat,1) - 1) / 3, C_INT)
+        Call check(c_lu_sol_host(Int(ma, C_INT), kl, kl, c_loc(mat), Int(Size(mat,1), C_INT), c_loc(pvt), c_loc(rhs), &
+                                 Int(Size(rhs,1), C_INT), Int(mb, C_INT)), 'LU_Solve_Band')
+    End Subroutine LU_Solve_Band
 
 End Module RaySHT_Math_Layer

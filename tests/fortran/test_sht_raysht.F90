@@ -29,6 +29,7 @@ Program test_sht_raysht
     Call cheby_round_trip(nfail)
     Call cheby_derivatives(nfail)
     Call angular_derivatives(nfail)
+    Call implicit_solves(nfail)
     ! the split forms with 6 digits stay within the same tolerance over ten round trips
     Call Set_Legendre_Precision(raysht_int8x6)
     Call white_noise(nfail)
@@ -330,6 +331,55 @@ Contains
         d = max(d, abs(s(m1)%data(1, 4, 1, 1)) + abs(s(m1)%data(1, 4, 2, 1) - 1.0d0))
         Call report('Angular derivatives', d, nfail)
     End Subroutine angular_derivatives
+
+    ! Implicit_Solve's two paths as Linear_Solve.F90 calls them: LU_Decompose_full / LU_Solve_full
+    ! on a 12 x 12 matrix and LU_Decompose_band / LU_Solve_Band on a 20 x 20 matrix of band width
+    ! kl = ku = 2 in band storage (ldab = 7, element (r, c) at row 5 + r - c, as Band_Load_Single
+    ! lays it out), each with 2 x 2 right-hand sides A x for a known x
+    Subroutine implicit_solves(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: nd = 12, nb = 20, kl = 2
+        Real*8 :: a(nd, nd), x(nd, 2, 2), rhs(nd, 2, 2), band(3*kl+1, nb), full(nb, nb)
+        Real*8 :: xb(nb, 2, 2), rb(nb, 2, 2), d
+        Integer :: pvt(nd), pb(nb), r, c, i, j
+        Do c = 1, nd
+            Do r = 1, nd
+                a(r, c) = sin(1.3d0 * r + 0.7d0 * c * c)
+            Enddo
+            a(c, c) = a(c, c) + 0.1d0 * c
+        Enddo
+        Do j = 1, 2
+            Do i = 1, 2
+                Do r = 1, nd
+                    x(r, i, j) = cos(0.5d0 * r + i - 2.0d0 * j)
+                Enddo
+                rhs(:, i, j) = matmul(a, x(:, i, j))
+            Enddo
+        Enddo
+        Call LU_Decompose_full(a, pvt)
+        Call LU_Solve_full(a, rhs, pvt)
+        d = maxval(abs(rhs - x))
+        full = 0.0d0
+        band = 0.0d0
+        Do c = 1, nb
+            Do r = max(1, c - kl), min(nb, c + kl)
+                full(r, c) = cos(0.9d0 * r - 0.4d0 * c) + merge(0.0d0, 0.2d0, r /= c)
+                band(2*kl + 1 + r - c, c) = full(r, c)
+            Enddo
+        Enddo
+        Do j = 1, 2
+            Do i = 1, 2
+                Do r = 1, nb
+                    xb(r, i, j) = sin(0.3d0 * r * i + j)
+                Enddo
+                rb(:, i, j) = matmul(full, xb(:, i, j))
+            Enddo
+        Enddo
+        Call LU_Decompose_band(band, pb)
+        Call LU_Solve_Band(band, rb, pb)
+        d = max(d, maxval(abs(rb - xb)))
+        Call report('Implicit solves (dense, band)', d, nfail)
+    End Subroutine implicit_solves
 
     ! Cheby_Deriv_Buffer_4D (d_by_dr_cp, Chebyshev_Polynomials.F90:560-640) as Test_Cheby's
     ! derivative check uses it: variables 2..4 of the buffer receive the first three

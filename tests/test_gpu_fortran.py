@@ -1,6 +1,7 @@
 """The Fortran ISO_C_BINDING layer (rayleigh_amd/fortran) driven by a Test_SHT-style
 Fortran program: Legendre_Transform (4-D and 2-D), FFT_To_Physical/Spectral and the
-Chebyshev pair, radial and angular derivatives through the host-array entry points, on the GPU."""
+Chebyshev pair, radial and angular derivatives and the dense and band implicit solves through the
+host-array entry points, on the GPU."""
 import os
 import subprocess
 
@@ -23,4 +24,4 @@ def test_fortran_binding_test_sht():
     lines = [ln for ln in r.stdout.splitlines() if "result =" in ln]
     # eight FP64 checks (the last the Chebyshev and the angular derivatives), then the
     # white-noise round trips with INT8X6 and BF16X6
-    assert len(lines) == 10 and all("PASSING" in ln for ln in lines), r.stdout
+    assert len(lines) == 11 and all("PASSING" in ln for ln in lines), r.stdout
