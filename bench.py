@@ -394,6 +394,24 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
     ms = e0.elapsed_time(e1) / steps
     out["rlm_spaceb_hydro"] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9, "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
     del w8
+    # one field scaled by l(l+1) and a radial profile (rlm_spacea's l_l_plus1 * f * Over_RhoRSQ):
+    # 16 B per coefficient
+    lfac = torch.from_numpy(ra.l_l_plus1(L)).cuda()
+    rfac = torch.linspace(1.0, 8.0, nr, dtype=torch.float64, device="cuda")
+    f1 = torch.empty(per_field, dtype=torch.float64, device="cuda")
+    fn = lambda: plan.lm_scale(s, 0, f1, 0, lfac, rfac)
+    for _ in range(max(warmup, 1)):
+        fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nb = 16.0 * per_field
+    out["lm_scale"] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9, "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
+    del f1
     # the implicit solve's dense path for the coupled W-P system (n = 2 N_r) of every l:
     # LU_Decompose_full on l_max + 1 matrices, then LU_Solve_full on 2 (l + 1) right-hand
     # sides per l (re, im of every m <= l); 2 n^3 / 3 and 2 n^2 per column flop
