@@ -202,7 +202,8 @@ __global__ __launch_bounds__(kAThreads) void rlm_spaceb_kernel(int lmax, int nr,
 
 // out(l, r, imi) = (lfac(l) in(l, r, imi)) rfac(r) for one field (either factor may be
 // absent); in place allowed.  The products round in this order, as the reference's
-// l_l_plus1(m:l_max)*SBUFFA(IDX2, f)*Over_RhoRSQ(r) statements round them.
+// l_l_plus1(m:l_max)*SBUFFA(IDX2, f)*Over_RhoRSQ(r) statements round them.  One wave per
+// (r, imi) l-column: rfac(r) is one value per wave, lfac(m..l_max) a contiguous run.
 __global__ __launch_bounds__(kAThreads) void lm_scale_kernel(int lmax, int nr, const int *__restrict__ mvals,
                                                              const int *__restrict__ lm_base, const double *in,
                                                              int nf_in, int fin, double *out, int nf_out, int fout,
@@ -210,18 +211,20 @@ __global__ __launch_bounds__(kAThreads) void lm_scale_kernel(int lmax, int nr, c
                                                              const double *__restrict__ rfac)
 {
 #pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * (kAThreads / 64) + (threadIdx.x >> 6);  // column r + nr imi
+    if (k >= 2 * nr) return;                                            // whole waves
     const int mp = blockIdx.y, m = mvals[mp], nl = lmax - m + 1;
     const long lb = lm_base[mp], total = 2L * nl * nr;
-    const double *a = in + lb * 2L * nr * nf_in + (long)fin * total;
-    double *b = out + lb * 2L * nr * nf_out + (long)fout * total;
-    const float rnl = 1.0f / (float)nl;
-    for (long e = blockIdx.x * (long)kAThreads + threadIdx.x; e < total; e += (long)gridDim.x * kAThreads) {
-        int i;
-        int k = div_small((int)e, nl, rnl, i);  // column r + nr imi
-        double v = a[e];
-        if (lfac) v = lfac[m + i] * v;
-        if (rfac) v = v * rfac[k >= nr ? k - nr : k];
-        b[e] = v;
+    const double *a = in + lb * 2L * nr * nf_in + (long)fin * total + (long)k * nl;
+    double *b = out + lb * 2L * nr * nf_out + (long)fout * total + (long)k * nl;
+    const double rf = rfac ? rfac[k >= nr ? k - nr : k] : 1.0;
+    const double *lf = lfac ? lfac + m : nullptr;
+    for (int i = lane; i < nl; i += 64) {
+        double v = a[i];
+        if (lf) v = lf[i] * v;
+        if (rfac) v = v * rf;
+        b[i] = v;
     }
 }
 
@@ -300,8 +303,7 @@ int launch_lm_scale(raysht_plan *p, const double *in, int nf_in, int fin, double
     const long per_field = p->s_off[p->nm] / p->nf;  // elementwise: in place, or another field of one buffer
     const bool same = in == out && nf_in == nf_out;
     if (!same && in < out + per_field * nf_out && out < in + per_field * nf_in) return RAYSHT_EINVAL;
-    const unsigned bx =
-        (unsigned)std::min<long>((2L * max_nl(p) * p->nr + kAThreads - 1) / kAThreads, kAMaxBlocksX);
+    const unsigned bx = (unsigned)((2L * p->nr + kAThreads / 64 - 1) / (kAThreads / 64));  // a wave per l-column
     hipLaunchKernelGGL(lm_scale_kernel, dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr,
                        p->d_mvals, p->d_lm_base, in, nf_in, fin, out, nf_out, fout, lfac, rfac);
     RAYSHT_HIP_TRY(hipGetLastError());
