@@ -64,7 +64,9 @@ __global__ __launch_bounds__(kLuThreads) void lu_decompose_kernel(int n, double 
             }
             __syncthreads();
         }
-        const int p = s_idx[0] < n ? s_idx[0] : k;  // a column of NaNs: no larger entry, keep row k
+        // idamax: a NaN a_kk is kept (nothing compares larger than it), as is row k of a
+        // column of NaNs; NaNs below the diagonal are passed over
+        const int p = (s_idx[0] >= n || isnan(a[k + (long)k * n])) ? k : s_idx[0];
         const double pv = a[p + (long)k * n];
         __syncthreads();  // every thread has read the pivot before the swap moves it
         if (tid == 0) piv[k] = p + 1;

@@ -80,6 +80,29 @@ def test_lu_singular_info(ra):
     assert info.cpu().numpy().tolist() == [inf, 0] and inf == 8
 
 
+def test_lu_nan_diagonal_keeps_row(ra):
+    """idamax never ranks an entry above a NaN a_kk, so dgetrf keeps row k as the pivot
+    there (and passes over NaNs below the diagonal); the factors then match the oracle."""
+    import torch
+    n = 24
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((n, n)) + 4 * np.eye(n)
+    a[3, 3] = np.nan        # NaN pivot candidate with finite entries below it
+    b = rng.standard_normal((n, n)) + 4 * np.eye(n)
+    b[9, 2] = np.nan        # NaN below a finite diagonal
+    d = to_dev([a, b])
+    piv = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(2, dtype=torch.int32, device="cuda")
+    ra.lu_decompose_full(d, piv, info, n)
+    lu_g = d.cpu().numpy().reshape(2, n, n)
+    piv_g = piv.cpu().numpy().reshape(2, n)
+    for k, m in enumerate((a, b)):
+        lu, p, _ = O.lu_decompose(m)
+        assert np.array_equal(piv_g[k], p), k
+        assert np.array_equal(lu_g[k].T, lu, equal_nan=True), k
+    assert piv_g[0][3] == 4  # row k kept (1-based)
+
+
 def test_lu_rejects_bad_calls(ra):
     import torch
     from rayleigh_amd._lib import RayshtError

@@ -65,3 +65,16 @@ def test_planted_pivot_rule_fails():
     lu, piv, _ = O.lu_decompose(a)
     _, piv_l, _ = lapack.dgetrf(a)
     assert np.array_equal(piv - 1, piv_l) and piv[0] == 1  # ties |1| = |-1| = |1|: the first row
+
+
+def test_nan_diagonal_pivot_matches_lapack():
+    """A NaN a_kk with finite entries below it: idamax ranks nothing above the NaN, so
+    LAPACK keeps row k; the oracle agrees up to that column (after it the NaN spreads over
+    the trailing block and comparisons of NaNs decide nothing)."""
+    n, k = 24, 3
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((n, n)) + 4 * np.eye(n)
+    a[k, k] = np.nan
+    _, piv, _ = O.lu_decompose(a)
+    _, piv_l, _ = lapack.dgetrf(a)
+    assert np.array_equal(piv[: k + 1] - 1, piv_l[: k + 1]) and piv[k] == k + 1
