@@ -95,6 +95,10 @@ int raysht_legendre_pts_host(raysht_plan *plan, const double *p2, double *const 
  * orientation, concatenated in mp order: which = 0 p_lm_even (n_e x nt/2, l
  * fastest), 1 p_lm_odd, 2 ip_lm_even (nt/2 x n_e, theta fastest), 3 ip_lm_odd. */
 int raysht_plan_nodes(const raysht_plan *plan, double *x_hi, double *x_lo, double *w_hi, double *w_lo);
+/* The nodes and weights as the reference stores them, Real*16 coloc and gl_weights
+ * (Legendre_Polynomials.F90:28, :332-373): ntheta IEEE binary128 values each, 16 bytes
+ * per value in host byte order (Fortran Real*16 / C __float128 on x86-64). */
+int raysht_plan_nodes_quad(const raysht_plan *plan, void *x16, void *w16);
 size_t raysht_plan_table_size(const raysht_plan *plan, int which);
 int raysht_plan_table(const raysht_plan *plan, int which, double *host_out);
 
@@ -128,6 +132,13 @@ int raysht_permute_2b1b(raysht_plan *plan, const double *s2b, double *p1b, void 
 /* Transpose_1a2a (:845-932) on one radial rank: the inverse, s2a(mp)%data(l, r, imi, f) =
  * p1a(r, imi, lm, f). */
 int raysht_permute_1a2a(raysht_plan *plan, const double *p1a, double *s2a, void *stream);
+/* Host-array forms of the four permutes (synchronous), what the Fortran Reform_2a3a /
+ * Reform_3b2b / Reform_2b1b / Reform_1a2a call: p2 / p3 / p1 host arrays in the layouts
+ * above, s_m[mp] the host spectral block of each m (as for raysht_legendre_stp_host). */
+int raysht_permute_2a3a_host(raysht_plan *plan, int nphi, const double *p2, double *p3);
+int raysht_permute_3b2b_host(raysht_plan *plan, int nphi, const double *p3, double *p2);
+int raysht_permute_2b1b_host(raysht_plan *plan, const double *const *s_m, double *p1);
+int raysht_permute_1a2a_host(raysht_plan *plan, const double *p1, double *const *s_m);
 /* The pair list the radial transposes follow: its length (lm_count), and l_lm_values /
  * mp_lm_values (mp 0-based) into host arrays of lm_count ints. */
 int raysht_lm_count(const raysht_plan *plan);
@@ -242,8 +253,8 @@ void raysht_batch_destroy(raysht_batch *batch);
 int raysht_batch_size(const raysht_batch *batch);
 int raysht_batch_stp(raysht_batch *batch, const double *const *s, double *const *p2, void *stream);
 int raysht_batch_pts(raysht_batch *batch, const double *const *p2, double *const *s, void *stream);
-/* Schedule autotuner of a batch.  Candidates (index bits): bit 0 = no half-height last
- * row tiles in the FP64 PtS kernel, bit 1 = no XCD grouping of m-blocks (items go out
+/* Schedule autotuner of a batch.  Candidates (index bits): bit 0 = no reduced-height
+ * (3/4, 1/2, 1/4) last row tiles in the FP64 PtS kernel, bit 1 = no XCD grouping of m-blocks (items go out
  * longest first and the dispatcher spreads them over the XCDs); 0 is the default.
  * raysht_batch_autotune times every distinct candidate of each direction on the given
  * buffers (one warm-up and `reps` timed grouped launches on `stream`; outputs are
@@ -264,6 +275,13 @@ int raysht_graph_create_batch(raysht_graph **graph, raysht_batch *batch, const d
  * zeros-grid points on [rmin, rmax]; grid (host, n) receives the radii. */
 int raysht_cheb_create(raysht_cheb **cheb, int n, double rmin, double rmax, double *grid);
 void raysht_cheb_destroy(raysht_cheb *cheb);
+/* The rest of what Cheby_Grid%init computes for its caller and its components
+ * (:161-200, :210-248), n values each, any pointer may be NULL: x = cos(theta) the
+ * colocation points (descending), theta their angles, weights the radial integration
+ * weights (Init's integration_weights argument), deriv_scaling the d/dr factor of the
+ * domain map at every point (Cheby_Grid%deriv_scaling). */
+int raysht_cheb_grid_data(const raysht_cheb *cheb, double *x, double *theta, double *weights,
+                          double *deriv_scaling);
 /* Gen_Tn tables (:251-297) as stored: which = 0 cheby_even (n_x x n_even), 1 cheby_odd. */
 int raysht_cheb_table(const raysht_cheb *cheb, int which, double *host_out);
 

@@ -84,6 +84,8 @@ def oracle_lib():
         lib.orc_permute_1a2a.argtypes = [_I, _P, _I, _I, _I, _P, _P]
         lib.orc_cheby_grid.argtypes = [_I, _D, _D, _P]
         lib.orc_cheby_tables.argtypes = [_I, _P, _P]
+        lib.orc_cheby_weights.argtypes = [_I, _D, _D, _P]
+        lib.orc_gauss_legendre_binary128.argtypes = [_I, _P, _P]
         lib.orc_cheby_to_spectral.argtypes = [_I, _L, _P, _P, _P, _P]
         lib.orc_cheby_from_spectral.argtypes = [_I, _L, _P, _P, _P, _P]
         lib.orc_cheby_dealias.argtypes = [_I, _L, _P]
@@ -105,6 +107,7 @@ def ref_lib():
         lib = ctypes.CDLL(REF_SO)
         lib.ref_legendre_init.argtypes = [_I, _I, _I, _P]
         lib.ref_legendre_nodes.argtypes = [_P, _P, _P, _P]
+        lib.ref_legendre_nodes_q.argtypes = [_P, _P]
         lib.ref_legendre_tables.argtypes = [_P, _P, _P, _P]
         lib.ref_stp.argtypes = [_I, _I, _P, _P]
         lib.ref_pts.argtypes = [_I, _I, _P, _P]
@@ -177,6 +180,13 @@ def gauss_legendre(n: int):
     xh, xl, wh, wl = (np.zeros(n) for _ in range(4))
     oracle_lib().orc_gauss_legendre(n, _ptr(xh), _ptr(xl), _ptr(wh), _ptr(wl))
     return xh, xl, wh, wl
+
+
+def gauss_legendre_binary128(n: int):
+    """The nodes and weights as the reference stores them (Real*16): (n, 16) byte arrays."""
+    x, w = np.zeros((n, 16), np.uint8), np.zeros((n, 16), np.uint8)
+    oracle_lib().orc_gauss_legendre_binary128(n, _ptr(x), _ptr(w))
+    return x, w
 
 
 def legendre_tables(nt: int, lmax: int, mvals):
@@ -390,6 +400,13 @@ def cheby_grid(n: int, rmin: float, rmax: float) -> np.ndarray:
     return g
 
 
+def cheby_weights(n: int, rmin: float, rmax: float) -> np.ndarray:
+    """Initialize_Cheby_Grid's radial integration weights (one domain)."""
+    w = np.zeros(n)
+    oracle_lib().orc_cheby_weights(n, rmin, rmax, _ptr(w))
+    return w
+
+
 def cheby_tables(n: int):
     nx = n - n // 2
     ce, co = np.zeros(nx * nx), np.zeros(max(nx * (n // 2), 1))
@@ -459,6 +476,12 @@ class Reference:
         xh, xl, wh, wl = (np.zeros(self.nt) for _ in range(4))
         self.lib.ref_legendre_nodes(_ptr(xh), _ptr(xl), _ptr(wh), _ptr(wl))
         return xh, xl, wh, wl
+
+    def nodes_binary128(self):
+        """Real*16 coloc and gl_weights as (n_theta, 16) byte arrays."""
+        x, w = np.zeros((self.nt, 16), np.uint8), np.zeros((self.nt, 16), np.uint8)
+        self.lib.ref_legendre_nodes_q(_ptr(x), _ptr(w))
+        return x, w
 
     def tables(self):
         lib = oracle_lib()

@@ -684,6 +684,36 @@ void orc_cheby_grid(int n, double rmin, double rmax, double *grid)
     free(x);
 }
 
+/* Chebyshev_Polynomials.F90:176-200 (one domain): the radial integration weights
+ * int_scale r_i^2 sqrt(1 - x_i^2), int_scale = (3 pi scaling) / ((rmax^3 - rmin^3) n), the
+ * first and last halved; scaling = (rmax - rmin) / (x_1 - x_n) as for the grid.  Products
+ * and powers are rounded left to right (x**3 as (x x) x). */
+void orc_cheby_weights(int n, double rmin, double rmax, double *w)
+{
+    const double pi = 3.1415926535897932384626433832795028841972;
+    double *th = malloc(sizeof(double) * (size_t)n), *x = malloc(sizeof(double) * (size_t)n);
+    cheby_theta(n, th);
+    for (int i = 0; i < n; ++i) x[i] = cos(th[i]);
+    const double xmin = x[n - 1], xmax = x[0];
+    const double scaling = (rmax - rmin) / (xmax - xmin);
+    const double int_scale = (3 * pi * scaling) / ((rmax * rmax * rmax - rmin * rmin * rmin) * n);
+    for (int i = 0; i < n; ++i) {
+        const double r = (x[i] - xmin) * scaling + rmin;
+        w[i] = int_scale * (r * r) * sqrt(1.0 - x[i] * x[i]);
+    }
+    w[0] *= 0.5;
+    w[n - 1] *= 0.5;
+    free(th);
+    free(x);
+}
+
+/* Legendre_Polynomials.F90:332-373 as stored (Real*16 coloc, gl_weights): the quad
+ * values' 16 bytes each. */
+void orc_gauss_legendre_binary128(int n, void *x16, void *w16)
+{
+    gauss_legendre_q(n, (quad *)x16, (quad *)w16);
+}
+
 /* Chebyshev_Polynomials.F90:251-297 Gen_Tn: T_k(x_r) = cos(k theta_r);
  * ceven[i][r] (r fastest, 1..n_x) = T_{2i}, codd = T_{2i+1}; the x = 0 row is
  * halved when n is odd. */

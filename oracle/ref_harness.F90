@@ -75,6 +75,16 @@ Contains
         Enddo
     End Subroutine ref_legendre_nodes
 
+    ! The same nodes and weights as stored (Real*16), 16 bytes per value.
+    Subroutine ref_legendre_nodes_q(px, pw) bind(C, name='ref_legendre_nodes_q')
+        Type(C_PTR), Value :: px, pw
+        Real*16, Pointer :: x16(:), w16(:)
+        Call c_f_pointer(px, x16, [n_theta])
+        Call c_f_pointer(pw, w16, [n_theta])
+        x16(:) = coloc(1:n_theta)
+        w16(:) = gl_weights(1:n_theta)
+    End Subroutine ref_legendre_nodes_q
+
     ! The four parity tables of every mp, concatenated in mp order, each in its
     ! stored orientation: p_lm_even(mp)%data(1:n_e,1:nt2) (l fastest) and
     ! ip_lm_even(mp)%data(1:nt2,1:n_e) (theta fastest); odd twins likewise.

@@ -38,6 +38,7 @@ SIGNATURES = {
     "raysht_legendre_stp": (_I, [_P, _P, _P, _P]),
     "raysht_legendre_pts": (_I, [_P, _P, _P, _P]),
     "raysht_plan_nodes": (_I, [_P, _P, _P, _P, _P]),
+    "raysht_plan_nodes_quad": (_I, [_P, _P, _P]),
     "raysht_plan_table_size": (_S, [_P, _I]),
     "raysht_plan_table": (_I, [_P, _I, _P]),
     "raysht_fft_to_physical": (_I, [_I, _L, _P, _P]),
@@ -61,12 +62,17 @@ SIGNATURES = {
     "raysht_cheb_create": (_I, [ctypes.POINTER(_P), _I, _D, _D, _P]),
     "raysht_cheb_destroy": (None, [_P]),
     "raysht_cheb_table": (_I, [_P, _I, _P]),
+    "raysht_cheb_grid_data": (_I, [_P, _P, _P, _P, _P]),
     "raysht_cheb_to_spectral": (_I, [_P, _P, _P, _L, _P]),
     "raysht_cheb_from_spectral": (_I, [_P, _P, _P, _L, _P]),
     "raysht_cheb_dealias": (_I, [_P, _P, _L, _P]),
     "raysht_cheb_deriv": (_I, [_P, _P, _P, _L, _I, _P]),
     "raysht_permute_2b1b": (_I, [_P, _P, _P, _P]),
     "raysht_permute_1a2a": (_I, [_P, _P, _P, _P]),
+    "raysht_permute_2a3a_host": (_I, [_P, _I, _P, _P]),
+    "raysht_permute_3b2b_host": (_I, [_P, _I, _P, _P]),
+    "raysht_permute_2b1b_host": (_I, [_P, _P, _P]),
+    "raysht_permute_1a2a_host": (_I, [_P, _P, _P]),
     "raysht_lm_count": (_I, [_P]),
     "raysht_lm_values": (_I, [_P, _P, _P]),
     "raysht_d_by_dtheta": (_I, [_P, _I, _P, _I, _I, _P, _I, _I, _P]),

@@ -64,20 +64,25 @@ def build(verbose: bool = False) -> str:
 FORTRAN_DIR = os.path.join(HERE, "fortran")
 FORTRAN_BUILD = os.path.join(FORTRAN_DIR, "_build")
 FORTRAN_TEST = os.path.join(FORTRAN_BUILD, "test_sht_raysht")
+FORTRAN_RATES = os.path.join(FORTRAN_BUILD, "rates_raysht")
+ROCM_LIB = "/opt/rocm/lib"
 FC = os.environ.get("RAYSHT_FC", "/opt/rocm/bin/amdflang")
 
 
 def build_fortran() -> str | None:
-    """The ISO_C_BINDING module (RaySHT_Math_Layer) and the Fortran test program that
-    drives it; skipped (None) where no Fortran compiler is installed."""
+    """The ISO_C_BINDING module (RaySHT_Math_Layer), the Fortran test program that drives
+    it and the program that times its two call forms; skipped (None) where no Fortran
+    compiler is installed."""
     if not os.path.exists(FC):
         return None
     os.makedirs(FORTRAN_BUILD, exist_ok=True)
     srcs = [os.path.join(FORTRAN_DIR, "raysht_structures.F90"), os.path.join(FORTRAN_DIR, "raysht_math_layer.F90")]
     _run_in(FORTRAN_BUILD, [FC, "-O2", "-fPIC", "-c"] + srcs)
-    test_src = os.path.join(os.path.dirname(HERE), "tests", "fortran", "test_sht_raysht.F90")
-    _run_in(FORTRAN_BUILD, [FC, "-O2", "-o", FORTRAN_TEST, test_src, "raysht_structures.o", "raysht_math_layer.o",
-                            "-L" + HERE, "-lraysht", "-Wl,-rpath,$ORIGIN/../.."])
+    for prog, out in (("test_sht_raysht.F90", FORTRAN_TEST), ("rates_raysht.F90", FORTRAN_RATES)):
+        src = os.path.join(os.path.dirname(HERE), "tests", "fortran", prog)
+        _run_in(FORTRAN_BUILD, [FC, "-O2", "-o", out, src, "raysht_structures.o", "raysht_math_layer.o",
+                                "-L" + HERE, "-lraysht", "-L" + ROCM_LIB, "-lamdhip64",
+                                "-Wl,-rpath,$ORIGIN/../..", "-Wl,-rpath," + ROCM_LIB])
     return FORTRAN_TEST
 
 

@@ -521,13 +521,24 @@ int raysht_cheb_create(raysht_cheb **out, int n, double rmin, double rmax, doubl
         arg = arg + dtheta;
     }
     {
-        // Initialize_Cheby_Grid (:161-191): affine map of [x_n, x_1] onto [rmin, rmax]; the
-        // derivative scaling is the reciprocal of the map's slope, rounded as there
+        // Initialize_Cheby_Grid (:161-200): affine map of [x_n, x_1] onto [rmin, rmax]; the
+        // derivative scaling is the reciprocal of the map's slope; the radial integration
+        // weights int_scale r_i^2 sqrt(1 - x_i^2), halved at both ends, with
+        // int_scale = 3 pi slope / ((rmax^3 - rmin^3) n); each rounded in the order written there
         const double xmin = x[n - 1], xmax = x[0];
         const double scaling = (rmax - rmin) / (xmax - xmin);
+        std::vector<double> r(n);
+        for (int i = 0; i < n; ++i) r[i] = (x[i] - xmin) * scaling + rmin;
         if (grid)
-            for (int i = 0; i < n; ++i) grid[i] = (x[i] - xmin) * scaling + rmin;
+            for (int i = 0; i < n; ++i) grid[i] = r[i];
+        const double int_scale = (3.0 * pi * scaling) / ((rmax * rmax * rmax - rmin * rmin * rmin) * (double)n);
+        cb->weights.resize(n);
+        for (int i = 0; i < n; ++i) cb->weights[i] = int_scale * (r[i] * r[i]) * std::sqrt(1.0 - x[i] * x[i]);
+        cb->weights[0] *= 0.5;
+        cb->weights[n - 1] *= 0.5;
         cb->deriv_scaling = 1.0 / scaling;
+        cb->x = x;
+        cb->theta = theta;
     }
     // Gen_Tn (:251-297): cheby_even(i, k) = T_{2k-2}(x_i), cheby_odd(i, k) = T_{2k-1}(x_i), i <= n_x
     cb->ceven.assign((size_t)cb->n_x * cb->n_even, 0.0);
@@ -583,6 +594,18 @@ void raysht_cheb_destroy(raysht_cheb *cb)
     for (double *d : {cb->d_tt_e, cb->d_tt_o, cb->d_t_e, cb->d_t_o})
         if (d) (void)hipFree(d);
     delete cb;
+}
+
+int raysht_cheb_grid_data(const raysht_cheb *cb, double *x, double *theta, double *weights, double *deriv_scaling)
+{
+    if (!cb) return RAYSHT_EINVAL;
+    for (int i = 0; i < cb->n; ++i) {
+        if (x) x[i] = cb->x[i];
+        if (theta) theta[i] = cb->theta[i];
+        if (weights) weights[i] = cb->weights[i];
+        if (deriv_scaling) deriv_scaling[i] = cb->deriv_scaling;
+    }
+    return RAYSHT_OK;
 }
 
 int raysht_cheb_table(const raysht_cheb *cb, int which, double *host_out)

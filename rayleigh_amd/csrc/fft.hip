@@ -41,7 +41,54 @@ __device__ __forceinline__ cplx twiddle(const double2 *__restrict__ tw, int nphi
     return {v.x, sign * v.y};
 }
 
-// One radix-R Stockham butterfly (R = 2, 3 or 4, fixed at compile time) of one line.
+// cos / sin of 2 pi k / 5 and 2 pi k / 7 (k = 1, 2 / 1, 2, 3), the constants of the radix-5
+// and radix-7 butterflies
+constexpr double kC51 = 0.3090169943749474241022934171828190588602, kS51 = 0.9510565162951535721164393333793821434783;
+constexpr double kC52 = -0.8090169943749474241022934171828190588602, kS52 = 0.5877852522924731291687059546390727685978;
+constexpr double kC71 = 0.6234898018587335305250048840042398106324, kS71 = 0.7818314824680298087084445266740577502325;
+constexpr double kC72 = -0.2225209339563144042889025644967947594664, kS72 = 0.9749279121818236070181316829939312172327;
+constexpr double kC73 = -0.9009688679024191262361023195074450511660, kS73 = 0.4338837391175581204757683328483587546099;
+
+// y_k = sum_r v_r e^{sign 2 pi i r k / 5}: with t = v1 + v4, v2 + v3 and u = v1 - v4, v2 - v3,
+// y_1,4 = v0 + c1 t1 + c2 t2 +- i sign (s1 u1 + s2 u2), y_2,3 = v0 + c2 t1 + c1 t2 +- i sign (s2 u1 - s1 u2)
+__host__ __device__ __forceinline__ void dft5(const cplx (&v)[5], cplx (&y)[5], int sign)
+{
+    const cplx t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
+    y[0] = cadd(v[0], cadd(t1, t2));
+    const cplx a1 = {v[0].re + kC51 * t1.re + kC52 * t2.re, v[0].im + kC51 * t1.im + kC52 * t2.im};
+    const cplx a2 = {v[0].re + kC52 * t1.re + kC51 * t2.re, v[0].im + kC52 * t1.im + kC51 * t2.im};
+    const cplx b1 = {kS51 * u1.re + kS52 * u2.re, kS51 * u1.im + kS52 * u2.im};
+    const cplx b2 = {kS52 * u1.re - kS51 * u2.re, kS52 * u1.im - kS51 * u2.im};
+    const cplx ib1 = {-sign * b1.im, sign * b1.re}, ib2 = {-sign * b2.im, sign * b2.re};
+    y[1] = cadd(a1, ib1);
+    y[4] = csub(a1, ib1);
+    y[2] = cadd(a2, ib2);
+    y[3] = csub(a2, ib2);
+}
+
+// the same for 7: t_k = v_k + v_{7-k}, u_k = v_k - v_{7-k} (k = 1, 2, 3),
+// y_j, y_{7-j} = v0 + sum_k cos(2 pi j k / 7) t_k +- i sign sum_k sin(2 pi j k / 7) u_k
+__host__ __device__ __forceinline__ void dft7(const cplx (&v)[7], cplx (&y)[7], int sign)
+{
+    const cplx t1 = cadd(v[1], v[6]), t2 = cadd(v[2], v[5]), t3 = cadd(v[3], v[4]);
+    const cplx u1 = csub(v[1], v[6]), u2 = csub(v[2], v[5]), u3 = csub(v[3], v[4]);
+    y[0] = cadd(v[0], cadd(t1, cadd(t2, t3)));
+    // cos / sin of 2 pi j k / 7 for j = 1, 2, 3 and k = 1, 2, 3 (jk mod 7 folded into 1..3 with signs)
+    const double C[3][3] = {{kC71, kC72, kC73}, {kC72, kC73, kC71}, {kC73, kC71, kC72}};
+    const double S[3][3] = {{kS71, kS72, kS73}, {kS72, -kS73, -kS71}, {kS73, -kS71, kS72}};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const cplx a = {v[0].re + C[j][0] * t1.re + C[j][1] * t2.re + C[j][2] * t3.re,
+                        v[0].im + C[j][0] * t1.im + C[j][1] * t2.im + C[j][2] * t3.im};
+        const cplx b = {S[j][0] * u1.re + S[j][1] * u2.re + S[j][2] * u3.re,
+                        S[j][0] * u1.im + S[j][1] * u2.im + S[j][2] * u3.im};
+        const cplx ib = {-sign * b.im, sign * b.re};
+        y[j + 1] = cadd(a, ib);
+        y[6 - j] = csub(a, ib);
+    }
+}
+
+// One radix-R Stockham butterfly (R = 2, 3, 4, 5, 7 or 8, fixed at compile time) of one line.
 template <int R>
 __device__ __forceinline__ void stockham_butterfly(const cplx *sl, cplx *dl, int j, int nb, int ns, int nphi,
                                                    const double2 *__restrict__ tw, int sign)
@@ -57,7 +104,11 @@ __device__ __forceinline__ void stockham_butterfly(const cplx *sl, cplx *dl, int
         v[r] = a;
     }
     cplx y[R];
-    if constexpr (R == 2) {
+    if constexpr (R == 5) {
+        dft5(v, y, sign);
+    } else if constexpr (R == 7) {
+        dft7(v, y, sign);
+    } else if constexpr (R == 2) {
         y[0] = cadd(v[0], v[1]);
         y[1] = csub(v[0], v[1]);
     } else if constexpr (R == 8) {
@@ -131,7 +182,7 @@ __device__ __forceinline__ void stockham_stage(const cplx *src, cplx *dst, int n
 }
 
 // `lines` independent length-n2 complex DFTs of buf[line][n2] (LDS); the radix of stage
-// s is 2 bits of `code` (0: 2, 1: 4, 2: 3, 3: 8), product n2; sign = +1 (inverse,
+// s is 3 bits of `code` (0: 2, 1: 4, 2: 3, 3: 8, 4: 5, 5: 7), product n2; sign = +1 (inverse,
 // unnormalised) or -1 (forward).  Stockham auto-sort, ping-ponging with tmp.  Returns
 // the buffer that holds the result.
 __device__ cplx *stockham(cplx *buf, cplx *tmp, int n2, int lines, int nphi, int code, int nrad,
@@ -140,9 +191,15 @@ __device__ cplx *stockham(cplx *buf, cplx *tmp, int n2, int lines, int nphi, int
     cplx *src = buf, *dst = tmp;
     int ns = 1;
     for (int s = 0; s < nrad; ++s) {
-        const int c = (code >> (2 * s)) & 3;
+        const int c = (code >> (3 * s)) & 7;
         int R;
-        if (c == 3) {
+        if (c == 4) {
+            stockham_stage<5>(src, dst, n2, lines, ns, nphi, tw, sign);
+            R = 5;
+        } else if (c == 5) {
+            stockham_stage<7>(src, dst, n2, lines, ns, nphi, tw, sign);
+            R = 7;
+        } else if (c == 3) {
             stockham_stage<8>(src, dst, n2, lines, ns, nphi, tw, sign);
             R = 8;
         } else if (c == 1) {
@@ -678,27 +735,31 @@ bool fft3_dispatch(int n2, bool c2r, long howmany, double *x, const double2 *tw,
     }
 }
 
-// radix schedule for n2 = 2^a 3^b: radix-8 stages, then one radix-4 or radix-2 for
-// the remaining power of two, then the radix-3 stages; packed 2 bits per stage
-// (0: 2, 1: 4, 2: 3, 3: 8).
+// radix schedule for n2 = 2^a 3^b 5^c 7^d: radix-8 stages, then one radix-4 or radix-2
+// for the remaining power of two, then the radix-3, -5 and -7 stages; packed 3 bits per
+// stage (0: 2, 1: 4, 2: 3, 3: 8, 4: 5, 5: 7), at most 10 stages.
 bool fft_schedule(int n2, int *code, int *nrad)
 {
-    int a = 0, b = 0, n = n2;
-    while (n % 2 == 0) {
-        n /= 2;
-        ++a;
-    }
-    while (n % 3 == 0) {
-        n /= 3;
-        ++b;
-    }
+    int e[4] = {0, 0, 0, 0}, n = n2;
+    const int primes[4] = {2, 3, 5, 7};
+    for (int i = 0; i < 4; ++i)
+        while (n % primes[i] == 0) {
+            n /= primes[i];
+            ++e[i];
+        }
     if (n != 1) return false;
     int c = 0, s = 0;
-    for (int i = 0; i < a / 3; ++i) c |= 3 << (2 * s++);
-    if (a % 3 == 2) c |= 1 << (2 * s++);
-    if (a % 3 == 1) c |= 0 << (2 * s++);
-    for (int i = 0; i < b; ++i) c |= 2 << (2 * s++);
-    if (s > 15) return false;
+    auto push = [&](int radix_code) {
+        if (s < 10) c |= radix_code << (3 * s);
+        ++s;
+    };
+    for (int i = 0; i < e[0] / 3; ++i) push(3);
+    if (e[0] % 3 == 2) push(1);
+    if (e[0] % 3 == 1) push(0);
+    for (int i = 0; i < e[1]; ++i) push(2);
+    for (int i = 0; i < e[2]; ++i) push(4);
+    for (int i = 0; i < e[3]; ++i) push(5);
+    if (s > 10) return false;
     *code = c;
     *nrad = s;
     return true;

@@ -8,6 +8,7 @@
 // GPU in double-double (legendre_tables.hip).
 #include <quadmath.h>
 
+#include <cstring>
 #include <vector>
 
 namespace raysht {
@@ -45,10 +46,8 @@ void legendre_pn(quad x, int n, quad &pn, quad &dpn)
 // by Newton's method from cos(pi (i - 1/4) / (n + 1/2)) until the step is at
 // most 3e-14; the weight is formed from the derivative at the iterate before
 // the last step, which is what the reference stores.
-void gauss_legendre_dd(int n, double *x_hi, double *x_lo, double *w_hi, double *w_lo,
-                       double *renorm_hi, double *renorm_lo)
+static void gauss_legendre_q(int n, quad *x, quad *w)
 {
-    std::vector<quad> x(n), w(n);
     const int half = (n + 1) / 2;
     for (int i = 1; i <= half; ++i) {
         quad root = cosq(kPi * (static_cast<quad>(i) - 0.25Q) / (static_cast<quad>(n) + 0.5Q));
@@ -65,11 +64,28 @@ void gauss_legendre_dd(int n, double *x_hi, double *x_lo, double *w_hi, double *
         w[i - 1] = 2.0Q / ((1.0Q - root * root) * dpn * dpn);
         w[n - i] = w[i - 1];
     }
+}
+
+void gauss_legendre_dd(int n, double *x_hi, double *x_lo, double *w_hi, double *w_lo,
+                       double *renorm_hi, double *renorm_lo)
+{
+    std::vector<quad> x(n), w(n);
+    gauss_legendre_q(n, x.data(), w.data());
     for (int i = 0; i < n; ++i) {
         to_pair(x[i], &x_hi[i], &x_lo[i]);
         to_pair(w[i], &w_hi[i], &w_lo[i]);
         to_pair(2.0Q * kPi * w[i], &renorm_hi[i], &renorm_lo[i]);
     }
+}
+
+// The same nodes and weights as the reference stores them (Real*16 coloc, gl_weights,
+// Legendre_Polynomials.F90:28): IEEE binary128, 16 bytes each, host byte order.
+void gauss_legendre_binary128(int n, void *x16, void *w16)
+{
+    std::vector<quad> x(n), w(n);
+    gauss_legendre_q(n, x.data(), w.data());
+    std::memcpy(x16, x.data(), sizeof(quad) * n);
+    std::memcpy(w16, w.data(), sizeof(quad) * n);
 }
 
 // amp(m) = sqrt((m + 1/2) / (2 pi)) * prod_{i<=m} sqrt((i - 1/2) / i)

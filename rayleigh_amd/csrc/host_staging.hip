@@ -118,6 +118,93 @@ int raysht_fft_to_spectral_host(int nphi, long howmany, double *x, int rescale)
     return RAYSHT_OK;
 }
 
+// SphericalBuffer%reform between the hybrid and physical layouts from host arrays (the
+// Fortran Reform_2a3a / Reform_3b2b): the hybrid array goes through the plan's staging
+// buffer, the physical one through a buffer of this call.
+int raysht_permute_2a3a_host(raysht_plan *p, int nphi, const double *p2, double *p3)
+{
+    if (!p || !p2 || !p3 || nphi < 2) return RAYSHT_EINVAL;
+    int rc = ensure_stage(p);
+    if (rc != RAYSHT_OK) return rc;
+    const size_t nh = (size_t)p->nt * p->nrhs * p->nm * sizeof(double);
+    const size_t ng = (size_t)(nphi + 2) * p->nt * p->nr * p->nf * sizeof(double);
+    scoped_device_buffer d;
+    if (ng > 0 && hipMalloc(&d.ptr, ng) != hipSuccess) {
+        d.ptr = nullptr;
+        return RAYSHT_ENOMEM;
+    }
+    RAYSHT_HIP_TRY(hipMemcpy(p->d_stage_p, p2, nh, hipMemcpyHostToDevice));
+    if ((rc = raysht_permute_2a3a(p, nphi, p->d_stage_p, d.ptr, nullptr)) != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipMemcpy(p3, d.ptr, ng, hipMemcpyDeviceToHost));
+    return RAYSHT_OK;
+}
+
+int raysht_permute_3b2b_host(raysht_plan *p, int nphi, const double *p3, double *p2)
+{
+    if (!p || !p2 || !p3 || nphi < 2) return RAYSHT_EINVAL;
+    int rc = ensure_stage(p);
+    if (rc != RAYSHT_OK) return rc;
+    const size_t nh = (size_t)p->nt * p->nrhs * p->nm * sizeof(double);
+    const size_t ng = (size_t)(nphi + 2) * p->nt * p->nr * p->nf * sizeof(double);
+    scoped_device_buffer d;
+    if (ng > 0 && hipMalloc(&d.ptr, ng) != hipSuccess) {
+        d.ptr = nullptr;
+        return RAYSHT_ENOMEM;
+    }
+    RAYSHT_HIP_TRY(hipMemcpy(d.ptr, p3, ng, hipMemcpyHostToDevice));
+    if ((rc = raysht_permute_3b2b(p, nphi, d.ptr, p->d_stage_p, nullptr)) != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipMemcpy(p2, p->d_stage_p, nh, hipMemcpyDeviceToHost));
+    return RAYSHT_OK;
+}
+
+// SphericalBuffer%reform between the spectral (rmcontainer4d) and radial layouts from host
+// arrays (the Fortran Reform_2b1b / Reform_1a2a): s_m[mp] as for the Legendre host forms,
+// p1 the host radial array p1(1:nr, 1:2, 1:lm_count, 1:nf).
+int raysht_permute_2b1b_host(raysht_plan *p, const double *const *s_m, double *p1)
+{
+    if (!p || !s_m || !p1) return RAYSHT_EINVAL;
+    for (int mp = 0; mp < p->nm; ++mp)
+        if (!s_m[mp]) return RAYSHT_EINVAL;
+    int rc = ensure_stage(p);
+    if (rc != RAYSHT_OK) return rc;
+    const size_t ns = (size_t)p->s_off[p->nm] * sizeof(double);
+    scoped_device_buffer d;
+    if (hipMalloc(&d.ptr, ns) != hipSuccess) {
+        d.ptr = nullptr;
+        return RAYSHT_ENOMEM;
+    }
+    for (int mp = 0; mp < p->nm; ++mp) {
+        const size_t n = (size_t)(p->s_off[mp + 1] - p->s_off[mp]) * sizeof(double);
+        RAYSHT_HIP_TRY(hipMemcpy(p->d_stage_s + p->s_off[mp], s_m[mp], n, hipMemcpyHostToDevice));
+    }
+    if ((rc = raysht_permute_2b1b(p, p->d_stage_s, d.ptr, nullptr)) != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipMemcpy(p1, d.ptr, ns, hipMemcpyDeviceToHost));
+    return RAYSHT_OK;
+}
+
+int raysht_permute_1a2a_host(raysht_plan *p, const double *p1, double *const *s_m)
+{
+    if (!p || !s_m || !p1) return RAYSHT_EINVAL;
+    for (int mp = 0; mp < p->nm; ++mp)
+        if (!s_m[mp]) return RAYSHT_EINVAL;
+    int rc = ensure_stage(p);
+    if (rc != RAYSHT_OK) return rc;
+    const size_t ns = (size_t)p->s_off[p->nm] * sizeof(double);
+    scoped_device_buffer d;
+    if (hipMalloc(&d.ptr, ns) != hipSuccess) {
+        d.ptr = nullptr;
+        return RAYSHT_ENOMEM;
+    }
+    RAYSHT_HIP_TRY(hipMemcpy(d.ptr, p1, ns, hipMemcpyHostToDevice));
+    if ((rc = raysht_permute_1a2a(p, d.ptr, p->d_stage_s, nullptr)) != RAYSHT_OK) return rc;
+    RAYSHT_HIP_TRY(hipDeviceSynchronize());
+    for (int mp = 0; mp < p->nm; ++mp) {
+        const size_t n = (size_t)(p->s_off[mp + 1] - p->s_off[mp]) * sizeof(double);
+        RAYSHT_HIP_TRY(hipMemcpy(s_m[mp], p->d_stage_s + p->s_off[mp], n, hipMemcpyDeviceToHost));
+    }
+    return RAYSHT_OK;
+}
+
 int raysht_cheb_to_spectral_host(raysht_cheb *cb, const double *f, double *c, long ncols)
 {
     if (!cb || ncols < 0 || (ncols > 0 && (!f || !c))) return RAYSHT_EINVAL;

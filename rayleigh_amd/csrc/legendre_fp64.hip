@@ -169,29 +169,37 @@ __global__ __launch_bounds__(STP_THREADS, RAYSHT_FP64_MIN_WAVES) void stp_fp64_k
                 acc_e[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
                 acc_o[cb][b] = double4_t{0.0, 0.0, 0.0, 0.0};
             }
+        // one MFMA step: parity-l rows k0 + kk .. + 3 of the chunk in LDS
+        auto kstep = [&](int kk) {
+            double be[4], bo[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                be[b] = Ae[(kk + lg) * STP_A_LD + 16 * b + li];
+                bo[b] = Ao[(kk + lg) * STP_A_LD + 16 * b + li];
+            }
+#pragma unroll
+            for (int cb = 0; cb < STP_CB; ++cb) {
+                const int row = (16 * STP_CB * wave + 16 * cb + li) * STP_BS_LD + 2 * (kk + lg);
+                const double a_e = Bs[row];
+                const double a_o = Bs[row + 1];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    acc_e[cb][b] = mfma_f64(a_e, be[b], acc_e[cb][b]);
+                    acc_o[cb][b] = mfma_f64(a_o, bo[b], acc_o[cb][b]);
+                }
+            }
+        };
         for (int k0 = 0; k0 < cur.ne; k0 += STP_KC) {
             stash(k0);
             __syncthreads();
             if (k0 + STP_KC < cur.ne) fetch(cur, k0 + STP_KC);
-#pragma unroll 2
-            for (int kk = 0; kk < STP_KC; kk += 4) {
-                double be[4], bo[4];
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    be[b] = Ae[(kk + lg) * STP_A_LD + 16 * b + li];
-                    bo[b] = Ao[(kk + lg) * STP_A_LD + 16 * b + li];
-                }
-#pragma unroll
-                for (int cb = 0; cb < STP_CB; ++cb) {
-                    const int row = (16 * STP_CB * wave + 16 * cb + li) * STP_BS_LD + 2 * (kk + lg);
-                    const double a_e = Bs[row];
-                    const double a_o = Bs[row + 1];
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        acc_e[cb][b] = mfma_f64(a_e, be[b], acc_e[cb][b]);
-                        acc_o[cb][b] = mfma_f64(a_o, bo[b], acc_o[cb][b]);
-                    }
-                }
+            // the last, partial chunk stops after the pair of MFMA steps that holds its last
+            // parity-l row (the rows past ne are zeros in LDS, so the steps skipped would
+            // add exact zeros)
+            const int kend = min(STP_KC, (cur.ne - k0 + 7) & ~7);
+            for (int kk = 0; kk < kend; kk += 8) {
+                kstep(kk);
+                kstep(kk + 4);
             }
             __syncthreads();
         }
@@ -245,9 +253,9 @@ __device__ __forceinline__ double2 load_f64x2(__amdgpu_buffer_rsrc_t r, int voff
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
 }
 
-// One PtS tile of NBR 16-row blocks of each parity: PTS_NBR (the full tile), or a half or
-// quarter of it for the last rows of an m-block, which would leave most of a full tile
-// idle.  LDS buffers come from the kernel so that all shapes share them.
+// One PtS tile of NBR 16-row blocks of each parity: PTS_NBR (the full tile), or three
+// quarters, a half or a quarter of it for the last rows of an m-block, which would leave
+// part of a full tile idle.  LDS buffers come from the kernel so that all shapes share them.
 //
 // K order: MFMA step s of a chunk takes node j0 + 4 lg + s on lane group lg (any order
 // works as long as both operands follow it), so that a lane's four nodes are contiguous:
@@ -387,6 +395,7 @@ __global__ __launch_bounds__(PTS_THREADS, RAYSHT_FP64_MIN_WAVES) void pts_fp64_k
     switch (pad >> 16) {
     case 0: pts_tile<PTS_NBR>(cur, nt, nt2, nrhs, sb, Ib, Fb); break;
     case 1: pts_tile<PTS_NBR / 2>(cur, nt, nt2, nrhs, sb, Ib, Fb); break;
+    case 3: pts_tile<3 * PTS_NBR / 4>(cur, nt, nt2, nrhs, sb, Ib, Fb); break;
     default: pts_tile<PTS_NBR / 4>(cur, nt, nt2, nrhs, sb, Ib, Fb); break;
     }
 }

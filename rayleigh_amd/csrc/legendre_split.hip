@@ -540,7 +540,199 @@ __device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, i
 }
 static_assert(kChunkSteps % 2 == 0, "chunks hold whole step pairs");
 
+// The MFMAs of one K step for a wave's 32 x 32 block, every kept pair in level order
+template <class F, int Q = 0>
+__device__ __forceinline__ void mfma_pairs(typename F::acc_t (&acc)[F::NG], const u32x4 (&a)[F::NA],
+                                           const u32x4 (&b)[F::NB])
+{
+    if constexpr (Q < F::npairs()) {
+        constexpr int s = F::pair(Q, true), t = F::pair(Q, false), g = F::group(Q);
+        mfma_pair<F>(acc[g], a[s], b[t]);
+        mfma_pairs<F, Q + 1>(acc, a, b);
+    }
+}
+
+// The K loop of a contraction whose two operands are both digit fragments in memory: A
+// fragments fa0 + ks NA .. (resource ra), B fragments fb0 + ks NB .. (resource rt), both
+// one step ahead in registers (two sets, swapped by unrolling the step loop by two); no
+// LDS and no barrier, so each wave runs on its own.  The last prefetch reads one step
+// past the wave's fragments (inside the resource or as zeros) and is not used.
+template <class F>
+__device__ __forceinline__ void direct_k_loop(__amdgpu_buffer_rsrc_t ra, int fa0, __amdgpu_buffer_rsrc_t rt, int fb0,
+                                              int nks, int lane, double (&dacc)[16])
+{
+    if (nks <= 0) return;
+    u32x4 aA[F::NA], aB[F::NA], bA[F::NB], bB[F::NB];
+    get_b<F::NA>(ra, fa0, lane, aA);
+    get_b<F::NB>(rt, fb0, lane, bA);
+    typename F::acc_t acc[F::NG];
+#pragma unroll
+    for (int g = 0; g < F::NG; ++g) acc[g] = typename F::acc_t{};
+    auto step = [&](int ks, const u32x4 (&ac)[F::NA], u32x4 (&an)[F::NA], const u32x4 (&bc)[F::NB],
+                    u32x4 (&bn)[F::NB]) {
+        get_b<F::NA>(ra, fa0 + (ks + 1) * F::NA, lane, an);
+        get_b<F::NB>(rt, fb0 + (ks + 1) * F::NB, lane, bn);
+        mfma_pairs<F>(acc, ac, bc);
+    };
+    auto steps = [&](int kc, int kend) {
+        int ks = kc;
+        for (; ks + 1 < kend; ks += 2) {
+            step(ks, aA, aB, bA, bB);
+            step(ks + 1, aB, aA, bB, bA);
+        }
+        if (ks < kend) step(ks, aA, aB, bA, bB);  // odd step count: only in the last chunk
+    };
+    if constexpr (F::I8) {
+        steps(0, nks);
+        flush<F>(acc, dacc);
+    } else {
+        for (int kc = 0; kc < nks; kc += kChunkSteps) {
+            steps(kc, min(nks, kc + kChunkSteps));
+            flush<F>(acc, dacc);
+        }
+    }
+}
+
+// --------------------------------------------------------------- StP data slicing
+// One workgroup of 256 threads per (problem, m-block, 32-column block): the exponent of
+// each column (from all its l), then every digit fragment of the block in both parities.
+// Fragment (cb nks + ks) NA + s of a parity's region (even region first, ncb nks_e NA
+// fragments): lane L, element e -> column 32 cb + L%32, parity-k = KS ks + EPL (L/32) + e,
+// i.e. l = 2 k + par; values past the m-block or the last column are zeros.  The digits are
+// the ones the contraction's A operand needs (offset_digits / pack_*), computed once per
+// column instead of once per row tile.
+template <class F>
+__global__ __launch_bounds__(256) void stp_slice_kernel(const prob_desc *__restrict__ probs, const batch_io io,
+                                                        const work_item *__restrict__ work)
+{
+    __shared__ int cexp[32];
+    const int mp = uniform(work[blockIdx.x].mp);
+    if (mp < 0) return;
+    const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
+    const prob_desc &P = probs[pi];
+    const int nrhs = uniform(P.nrhs);
+    const int m = uniform(uniform(P.mvals)[mp]), nl = uniform(P.lmax) - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
+    const double *sb = uniform(io.in[pi]) + uniform(uniform(P.s_off)[mp]);
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(sb, (long)nl * nrhs * 8);
+    const int tid = threadIdx.x;
+    {
+        const int cc = tid >> 3, q = tid & 7, c = c0 + cc;
+        const unsigned off = (unsigned)min(c, nrhs) * nl * 8;  // column nrhs and beyond read as 0
+        double mx = 0.0;
+        for (int l = q; l < nl; l += 8)
+            mx = fmax(mx, fabs(__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(off + 8u * l), 0,
+                                                                                             0))));
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+        if (q == 0) {
+            const int e = scale_exponent(mx, F::I8);
+            cexp[cc] = e;
+            if (c < nrhs) uniform(P.scexp)[(long)mp * nrhs + c] = e;
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 63, grp = tid >> 6;
+    const int cc = lane & 31, h = lane >> 5, c = c0 + cc;
+    const int e = cexp[cc];
+    const bool cin = c < nrhs;
+    const unsigned off = (unsigned)min(c, nrhs) * nl * 8;
+    const int nks_e = cdiv(ne, F::KS), nks_o = cdiv(no, F::KS), cb = c0 >> 5, ncb = cdiv(nrhs, 32);
+    uint4 *dig = uniform(P.sdig) + (uniform(uniform(P.sdig_off)[mp]) << 6);
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+        const int nks = par ? nks_o : nks_e;
+        const long region = (par ? (long)ncb * nks_e * F::NA : 0) + (long)cb * nks * F::NA;
+        for (int ks = grp; ks < nks; ks += 4) {
+            unsigned long long X[F::EPL];
+#pragma unroll
+            for (int i = 0; i < F::EPL; ++i) {
+                const int l = 2 * (F::KS * ks + F::EPL * h + i) + par;
+                const double v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+                                                                rs, (int)(off + 8u * (unsigned)l), 0, 0));
+                X[i] = offset_digits<F::NA>(cin && l < nl ? v : 0.0, e);
+            }
+#pragma unroll
+            for (int sd = 0; sd < F::NA; ++sd) {
+                unsigned w[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if constexpr (F::I8) {
+                        const unsigned long long x4[4] = {X[4 * q], X[4 * q + 1], X[4 * q + 2], X[4 * q + 3]};
+                        w[q] = pack_i8<F::NA>(x4, sd);
+                    } else {
+                        w[q] = pack_bf16(digit<F::NA>(X[2 * q], sd), digit<F::NA>(X[2 * q + 1], sd));
+                    }
+                }
+                dig[((region + (long)ks * F::NA + sd) << 6) + lane] = uint4{w[0], w[1], w[2], w[3]};
+            }
+        }
+    }
+}
+
 // --------------------------------------------------------------- StP
+// Work item (mp, r0, c0): kTileC columns, kTileR theta rows from r0, both parities, of a
+// plan whose data digits the slicing pass has written.  Wave w multiplies column block
+// c0/32 by theta block r0/32 + w: both operands come straight from the digit fragments
+// in memory (the data fragments are shared by the item's waves and by the other row
+// tiles of the column tile through the caches), so the waves need no LDS and no barrier
+// in the K loop; the scaled even-parity sums wait in LDS (one slot per thread) for the
+// odd pass, which writes E + O and E - O.
+template <class F>
+__global__ __launch_bounds__(kThreads, kMinWaves) void stp_split2_kernel(
+    const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
+{
+    static_assert(kCB == 1, "one 32-column block per StP item");
+    __shared__ double ebuf[16][kThreads];
+    const int mp = uniform(work[blockIdx.x].mp);
+    if (mp < 0) return;
+    const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
+    const int r0 = uniform(work[blockIdx.x].r0);
+    const prob_desc &P = probs[pi];
+    const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
+    const long *frag_off = uniform(P.frag_stp_off);
+    const int *texp = uniform(P.texp_stp);
+    const int m = uniform(uniform(P.mvals)[mp]), nl = uniform(P.lmax) - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const long f_lo = uniform(frag_off[mp]), f_hi = uniform(frag_off[mp + 1]);
+    const __amdgpu_buffer_rsrc_t rt =
+        make_frag_rsrc(reinterpret_cast<const u32x4 *>(uniform(P.tdig_stp)) + (f_lo << 6), f_hi - f_lo);
+    const long d_lo = uniform(uniform(P.sdig_off)[mp]), d_hi = uniform(uniform(P.sdig_off)[mp + 1]);
+    const __amdgpu_buffer_rsrc_t ra =
+        make_frag_rsrc(reinterpret_cast<const u32x4 *>(uniform(P.sdig)) + (d_lo << 6), d_hi - d_lo);
+    const int *cexp = uniform(P.scexp) + (long)mp * nrhs;
+    const int njb = cdiv(nt2, 32), nks_e = cdiv(ne, F::KS), nks_o = cdiv(no, F::KS);
+    const int cb = c0 >> 5, ncb = cdiv(nrhs, 32);
+    double *ob = uniform(io.out[pi]) + (long)mp * nrhs * nt;
+    const int jb = r0 / 32 + wave;
+    if (jb >= njb) return;  // a row block past the last theta (no barrier follows)
+    const int j = jb * 32 + (lane & 31);
+    const bool jin = j < nt2;
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+        double dacc[16];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
+        const int nks = par ? nks_o : nks_e;
+        const int fb0 = (par ? njb * nks_e * F::NB : 0) + jb * nks * F::NB;
+        const int fa0 = (par ? ncb * nks_e * F::NA : 0) + cb * nks * F::NA;
+        direct_k_loop<F>(ra, fa0, rt, fb0, nks, lane, dacc);
+        const int ej = jin ? texp[((long)mp * 2 + par) * nt2 + j] : 0;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int cc = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = c0 + cc;
+            const double r = __builtin_ldexp(dacc[v], (c < nrhs ? cexp[cc] : 0) + ej);
+            if (par == 0) {
+                ebuf[v][tid] = r;
+            } else if (jin && c < nrhs) {
+                const double e = ebuf[v][tid];
+                ob[(long)c * nt + j] = e + r;
+                ob[(long)c * nt + (nt - 1 - j)] = e - r;
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------- StP (sliced in the kernel)
 // Work item (mp, r0, c0): kTileC columns, kTileR theta rows from r0, both parities.
 // The row tiles of one column tile are consecutive items, so that they run together
 // and share the column tile's spectral data in L2.  Wave w multiplies column block w % kCB by theta block
@@ -815,6 +1007,30 @@ std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans,
     return deal_to_xcds(blocks, cost, o.xcd_groups);
 }
 
+// The slicing pass's items: (m-block, 0, first column) for every 32-column block of every
+// problem (problem index in `pad`), whole m-blocks dealt to the XCDs longest first.
+std::vector<work_item> split_slice_work(const std::vector<const raysht_plan *> &plans)
+{
+    struct blk {
+        int prob, mp, nl;
+    };
+    std::vector<blk> order;
+    for (int pi = 0; pi < (int)plans.size(); ++pi)
+        for (int mp = 0; mp < plans[pi]->nm; ++mp)
+            order.push_back(blk{pi, mp, plans[pi]->lmax - plans[pi]->mvals[mp] + 1});
+    std::stable_sort(order.begin(), order.end(), [](const blk &a, const blk &b) { return a.nl > b.nl; });
+    std::vector<std::vector<work_item>> blocks;
+    std::vector<double> cost;
+    for (const blk &b : order) {
+        const int ncb = cdiv(plans[b.prob]->nrhs, 32);
+        std::vector<work_item> items;
+        for (int c = 0; c < ncb; ++c) items.push_back(work_item{b.mp, 0, 32 * c, b.prob});
+        blocks.push_back(std::move(items));
+        cost.push_back((double)b.nl * ncb);
+    }
+    return deal_to_xcds(blocks, cost, true);
+}
+
 namespace {
 
 template <typename T>
@@ -849,22 +1065,28 @@ int split_plan_init(raysht_plan *p)
     static_assert(stp_x6::NB == stp_i6::NB && pts_x6::NB == pts_i6::NB && any_x3::NB == any_i3::NB, "table slices");
     const int ks = i8 ? 32 : 16;
     const int nm = p->nm, nt2 = p->nt2, njb = cdiv(nt2, 32), njs = cdiv(nt2, ks);
-    std::vector<long> fs(nm + 1, 0), fp(nm + 1, 0), ep(nm + 1, 0);
+    const int na_stp = x6 ? stp_x6::NA : any_x3::NA, ncb = cdiv(p->nrhs, 32);
+    static_assert(stp_x6::NA == stp_i6::NA && any_x3::NA == any_i3::NA, "data slices");
+    std::vector<long> fs(nm + 1, 0), fp(nm + 1, 0), ep(nm + 1, 0), sd(nm + 1, 0);
     for (int mp = 0; mp < nm; ++mp) {
         const int nl = p->lmax - p->mvals[mp] + 1, ne = (nl + 1) / 2, no = nl / 2;
         fs[mp + 1] = fs[mp] + (long)njb * (cdiv(ne, ks) + cdiv(no, ks)) * nb_stp;
         fp[mp + 1] = fp[mp] + (long)(cdiv(ne, 32) + cdiv(no, 32)) * njs * nb_pts;
         ep[mp + 1] = ep[mp] + ne + no;
+        sd[mp + 1] = sd[mp] + (long)ncb * (cdiv(ne, ks) + cdiv(no, ks)) * na_stp;
     }
-    const std::vector<work_item> ws = split_work({p}, true), wp = split_work({p}, false);
+    const std::vector<work_item> ws = split_work({p}, true), wp = split_work({p}, false), wsl = split_slice_work({p});
     sp.n_work_stp = (int)ws.size();
     sp.n_work_pts = (int)wp.size();
+    sp.n_work_slice = (int)wsl.size();
     int rc;
     if ((rc = dupload(&sp.d_frag_stp_off, fs)) || (rc = dupload(&sp.d_frag_pts_off, fp)) ||
         (rc = dupload(&sp.d_exp_pts_off, ep)) || (rc = dupload(&sp.d_work_stp, ws)) ||
         (rc = dupload(&sp.d_work_pts, wp)) || (rc = dalloc(&sp.d_tdig_stp, (size_t)fs[nm] * 64)) ||
         (rc = dalloc(&sp.d_tdig_pts, (size_t)fp[nm] * 64)) || (rc = dalloc(&sp.d_texp_stp, (size_t)nm * 2 * nt2)) ||
-        (rc = dalloc(&sp.d_texp_pts, (size_t)ep[nm])))
+        (rc = dalloc(&sp.d_texp_pts, (size_t)ep[nm])) || (rc = dupload(&sp.d_sdig_off, sd)) ||
+        (rc = dalloc(&sp.d_sdig, (size_t)sd[nm] * 64)) || (rc = dalloc(&sp.d_scexp, (size_t)nm * p->nrhs)) ||
+        (rc = dupload(&sp.d_work_slice, wsl)))
         return rc;
     // rows past a parity's length in the last 32-row block of the PtS tables are read
     // as zeros: clear before the split kernel fills the rows that exist
@@ -887,21 +1109,32 @@ void split_plan_free(raysht_plan *p)
     raysht_split &sp = p->split;
     for (void *ptr : {(void *)sp.d_frag_stp_off, (void *)sp.d_frag_pts_off, (void *)sp.d_exp_pts_off,
                       (void *)sp.d_work_stp, (void *)sp.d_work_pts, (void *)sp.d_tdig_stp, (void *)sp.d_tdig_pts,
-                      (void *)sp.d_texp_stp, (void *)sp.d_texp_pts})
+                      (void *)sp.d_texp_stp, (void *)sp.d_texp_pts, (void *)sp.d_sdig, (void *)sp.d_sdig_off,
+                      (void *)sp.d_scexp, (void *)sp.d_work_slice})
         if (ptr) (void)hipFree(ptr);
     sp = raysht_split();
 }
 
-int launch_stp_split_batch(int precision, const prob_desc *d_desc, const batch_io &io, const work_item *work,
-                           int n_work, hipStream_t st)
+namespace {
+// the slicing pass, then the contraction, on one stream
+template <class F>
+void launch_stp2(const prob_desc *d_desc, const batch_io &io, const work_item *slice_work, int n_slice,
+                 const work_item *work, int n_work, hipStream_t st)
+{
+    if (n_slice > 0) hipLaunchKernelGGL(stp_slice_kernel<F>, dim3(n_slice), dim3(256), 0, st, d_desc, io, slice_work);
+    hipLaunchKernelGGL(stp_split2_kernel<F>, dim3(n_work), dim3(kThreads), 0, st, d_desc, io, work);
+}
+}  // namespace
+
+int launch_stp_split_batch(int precision, const prob_desc *d_desc, const batch_io &io, const work_item *slice_work,
+                           int n_slice, const work_item *work, int n_work, hipStream_t st)
 {
     if (n_work == 0) return RAYSHT_OK;
-    const dim3 g(n_work), b(kThreads);
     switch (precision) {
-    case RAYSHT_BF16X6: hipLaunchKernelGGL(stp_split_kernel<stp_x6>, g, b, 0, st, d_desc, io, work); break;
-    case RAYSHT_BF16X3: hipLaunchKernelGGL(stp_split_kernel<any_x3>, g, b, 0, st, d_desc, io, work); break;
-    case RAYSHT_INT8X6: hipLaunchKernelGGL(stp_split_kernel<stp_i6>, g, b, 0, st, d_desc, io, work); break;
-    case RAYSHT_INT8X3: hipLaunchKernelGGL(stp_split_kernel<any_i3>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_BF16X6: launch_stp2<stp_x6>(d_desc, io, slice_work, n_slice, work, n_work, st); break;
+    case RAYSHT_BF16X3: launch_stp2<any_x3>(d_desc, io, slice_work, n_slice, work, n_work, st); break;
+    case RAYSHT_INT8X6: launch_stp2<stp_i6>(d_desc, io, slice_work, n_slice, work, n_work, st); break;
+    case RAYSHT_INT8X3: launch_stp2<any_i3>(d_desc, io, slice_work, n_slice, work, n_work, st); break;
     default: return RAYSHT_EINVAL;
     }
     RAYSHT_HIP_TRY(hipGetLastError());
@@ -929,7 +1162,8 @@ int launch_stp_split(raysht_plan *p, const double *s, double *p2, hipStream_t st
     batch_io io{};
     io.in[0] = s;
     io.out[0] = p2;
-    return launch_stp_split_batch(p->precision, p->d_desc, io, p->split.d_work_stp, p->split.n_work_stp, st);
+    return launch_stp_split_batch(p->precision, p->d_desc, io, p->split.d_work_slice, p->split.n_work_slice,
+                                  p->split.d_work_stp, p->split.n_work_stp, st);
 }
 
 int launch_pts_split(raysht_plan *p, const double *p2, double *s, hipStream_t st)

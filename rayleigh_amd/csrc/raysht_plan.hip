@@ -109,10 +109,14 @@ std::vector<work_item> fp64_work(const std::vector<const raysht_plan *> &plans, 
         const raysht_plan *p = plans[b.prob];
         const int rows = stp ? p->nt2 : n_even(p->lmax, p->mvals[b.mp]);
         const int nrt = (rows + tr - 1) / tr, nct = (p->nrhs + tc - 1) / tc;
-        // PtS: a last row tile of at most a quarter or a half of the height runs as the
-        // quarter- or half-height shape (shape code 2 or 1)
+        // PtS: a last row tile of at most a quarter, a half or three quarters of the height
+        // runs as that shape (shape code 2, 1 or 3)
         const int rem = rows - (nrt - 1) * tr;
-        const int shape = (stp || !o.half_tiles) ? 0 : rem <= tr / 4 ? 2 : rem <= tr / 2 ? 1 : 0;
+        const int shape = (stp || !o.half_tiles) ? 0
+                          : rem <= tr / 4        ? 2
+                          : rem <= tr / 2        ? 1
+                          : rem <= 3 * tr / 4    ? 3
+                                                 : 0;
         std::vector<work_item> items;
         // column tile outer, row tile inner: consecutive items share the column tile's
         // input (spectral columns for StP, hybrid columns for PtS) while it is in L2
@@ -120,7 +124,7 @@ std::vector<work_item> fp64_work(const std::vector<const raysht_plan *> &plans, 
             for (int r = 0; r < nrt; ++r)
                 items.push_back(work_item{b.mp, r * tr, c * tc, b.prob | ((r == nrt - 1 ? shape : 0) << 16)});
         blocks.push_back(std::move(items));
-        cost.push_back(b.item * (nrt - (shape == 2 ? 0.75 : shape == 1 ? 0.5 : 0.0)) * nct);
+        cost.push_back(b.item * (nrt - (shape == 2 ? 0.75 : shape == 1 ? 0.5 : shape == 3 ? 0.25 : 0.0)) * nct);
     }
     return deal_to_xcds(blocks, cost, o.xcd_groups);
 }
@@ -195,6 +199,9 @@ prob_desc describe(const raysht_plan *p)
     d.tdig_pts = p->split.d_tdig_pts;
     d.texp_stp = p->split.d_texp_stp;
     d.texp_pts = p->split.d_texp_pts;
+    d.sdig = p->split.d_sdig;
+    d.sdig_off = p->split.d_sdig_off;
+    d.scexp = p->split.d_scexp;
     d.lmax = p->lmax;
     d.nt = p->nt;
     d.nt2 = p->nt2;
@@ -364,6 +371,13 @@ int raysht_plan_nodes(const raysht_plan *p, double *x_hi, double *x_lo, double *
     std::memcpy(x_lo, p->x_lo.data(), sizeof(double) * p->nt);
     std::memcpy(w_hi, p->w_hi.data(), sizeof(double) * p->nt);
     std::memcpy(w_lo, p->w_lo.data(), sizeof(double) * p->nt);
+    return RAYSHT_OK;
+}
+
+int raysht_plan_nodes_quad(const raysht_plan *p, void *x16, void *w16)
+{
+    if (!p || !x16 || !w16) return RAYSHT_EINVAL;
+    gauss_legendre_binary128(p->nt, x16, w16);
     return RAYSHT_OK;
 }
 

@@ -12,13 +12,20 @@
 !   Legendre_Transform(Real*8(:,:,:), rmcontainer(:))     PtS_2d_dgpv2 (Legendre_Transforms.F90:506)
 !   FFT_To_Physical(x, plan, rsc) / FFT_To_Spectral(x, plan, rsc) / FFT_To_Spectral_RSC(x, plan)
 !                                                     Fourier_Transform.F90:25-35
+!   Type(Cheby_Grid): %Init / %to_spectral / %from_spectral / %dealias_buffer / %d_by_dr_cp /
+!       %d_by_dr_nobuffer, with the reference's components  Chebyshev_Polynomials.F90:32-59 (one domain)
 !   Cheby_To_Spectral(f_in, c_out) / Cheby_From_Spectral(c_in, f_out) after Cheby_Init(n, rmin, rmax, grid)
-!                                                     Chebyshev_Polynomials.F90:65-558 (one domain)
+!                                                     the same on one module-level grid
 !   Cheby_d_by_dr_cp(ind, dind, buffer, dorder)      Chebyshev_Polynomials.F90:560-640
+!   Reform_2a3a / Reform_3b2b / Reform_2b1b / Reform_1a2a   SphericalBuffer%reform on one rank
+!                                                     (Spherical_Buffer.F90:196-932)
 !   d_by_dtheta / d_by_sdtheta / d_by_dphi           Spectral_Derivatives.F90 (rmcontainer4d forms)
 !
 ! The data stay in the caller's host arrays: each call hands their addresses to
-! the library's host-array entry points, which stage through device memory.
+! the library's host-array entry points, which stage through device memory.  Callers
+! that keep their buffers in device memory use the *_Device procedures (device
+! pointers, an optional HIP stream; no transfer) and Device_Allocate / Device_Upload /
+! Device_Download / Device_Free / Device_Synchronize for the buffers themselves.
 ! As in the reference, sizes come from the shapes of the arguments; a device plan
 ! is built (once) for each (n_r, n_fields) or column count the transforms meet.
 ! Errors stop the program with the library's message (the reference has no
@@ -33,6 +40,11 @@ Module RaySHT_Math_Layer
     Public :: Initialize_Legendre, Finalize_Legendre, Legendre_Transform, Set_Legendre_Precision
     Public :: FFT_To_Physical, FFT_To_Spectral, FFT_To_Spectral_RSC
     Public :: Cheby_Init, Cheby_To_Spectral, Cheby_From_Spectral, Cheby_Finalize, Cheby_d_by_dr_cp
+    Public :: Cheby_Grid
+    Public :: Reform_2a3a, Reform_3b2b, Reform_2b1b, Reform_1a2a, LM_Count
+    Public :: StP_Device, PtS_Device, FFT_To_Physical_Device, FFT_To_Spectral_Device
+    Public :: Reform_2a3a_Device, Reform_3b2b_Device, Reform_2b1b_Device, Reform_1a2a_Device
+    Public :: Device_Allocate, Device_Free, Device_Upload, Device_Download, Device_Synchronize
     Public :: d_by_dtheta, d_by_sdtheta, d_by_dphi
     Public :: LU_Decompose_full, LU_Solve_full, LU_Decompose_band, LU_Solve_Band
     Public :: coloc, gl_weights
@@ -40,8 +52,32 @@ Module RaySHT_Math_Layer
     Integer, Parameter, Public :: raysht_fp64 = 0, raysht_bf16x3 = 1, raysht_bf16x6 = 2, &
                                   raysht_int8x3 = 3, raysht_int8x6 = 4
 
-    ! the Gauss-Legendre grid, as the reference exposes it (Legendre_Polynomials.F90:28)
-    Real*8, Allocatable, Save :: coloc(:), gl_weights(:)
+    ! the Gauss-Legendre grid, as the reference exposes it (Real*16, Legendre_Polynomials.F90:28)
+    Real*16, Allocatable, Save, Target :: coloc(:), gl_weights(:)
+
+    ! Chebyshev_Polynomials.F90:32-59 for one domain: the same components, the tables kept
+    ! on the device behind `handle`
+    Type :: Cheby_Grid
+        Integer :: domain_count = 0
+        Integer :: max_npoly = 0
+        Integer :: ntotal = 0
+        Integer, Allocatable :: npoly(:), n_x(:), rda(:)
+        Real*8, Allocatable :: domain_bounds(:)
+        Real*8, Allocatable :: n_even(:), n_odd(:)
+        Real*8, Allocatable :: x(:,:), theta(:,:)
+        Real*8, Allocatable :: deriv_scaling(:)
+        Type(C_PTR) :: handle = C_NULL_PTR
+    Contains
+        Procedure :: Init => Initialize_Cheby_Grid
+        Procedure :: dealias_buffer => dealias_buffer4d
+        Procedure :: to_spectral => To_Spectral4D
+        Procedure :: from_spectral => From_Spectral4D
+        Procedure :: d_by_dr_cp => Cheby_Deriv_Buffer_4D
+        Procedure :: d_by_dr_nobuffer => Cheby_Deriv_OutOf_Buffer
+        Procedure :: to_spectral_device => To_Spectral_Device
+        Procedure :: from_spectral_device => From_Spectral_Device
+        Procedure :: finalize => Finalize_Cheby_Grid
+    End Type Cheby_Grid
 
     Integer, Parameter :: max_plans = 8
     Type(C_PTR), Save :: plans(max_plans) = C_NULL_PTR
@@ -155,6 +191,126 @@ Module RaySHT_Math_Layer
             Type(C_PTR), Value :: mat, pvt, rhs
             Integer(C_INT) :: st
         End Function
+        Function c_plan_nodes_quad(plan, x16, w16) Bind(C, name='raysht_plan_nodes_quad') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, x16, w16
+            Integer(C_INT) :: st
+        End Function
+        Function c_cheb_grid_data(cb, x, theta, w, ds) Bind(C, name='raysht_cheb_grid_data') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: cb, x, theta, w, ds
+            Integer(C_INT) :: st
+        End Function
+        Function c_cheb_to_spec_dev(cb, f, c, ncols, stream) Bind(C, name='raysht_cheb_to_spectral') Result(st)
+            Import :: C_PTR, C_INT, C_LONG
+            Type(C_PTR), Value :: cb, f, c, stream
+            Integer(C_LONG), Value :: ncols
+            Integer(C_INT) :: st
+        End Function
+        Function c_cheb_from_spec_dev(cb, c, f, ncols, stream) Bind(C, name='raysht_cheb_from_spectral') Result(st)
+            Import :: C_PTR, C_INT, C_LONG
+            Type(C_PTR), Value :: cb, c, f, stream
+            Integer(C_LONG), Value :: ncols
+            Integer(C_INT) :: st
+        End Function
+        Function c_p2a3a_host(plan, nphi, p2, p3) Bind(C, name='raysht_permute_2a3a_host') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, p2, p3
+            Integer(C_INT), Value :: nphi
+            Integer(C_INT) :: st
+        End Function
+        Function c_p3b2b_host(plan, nphi, p3, p2) Bind(C, name='raysht_permute_3b2b_host') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, p3, p2
+            Integer(C_INT), Value :: nphi
+            Integer(C_INT) :: st
+        End Function
+        Function c_p2b1b_host(plan, s_m, p1) Bind(C, name='raysht_permute_2b1b_host') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, p1
+            Type(C_PTR), Intent(In) :: s_m(*)
+            Integer(C_INT) :: st
+        End Function
+        Function c_p1a2a_host(plan, p1, s_m) Bind(C, name='raysht_permute_1a2a_host') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, p1
+            Type(C_PTR), Intent(In) :: s_m(*)
+            Integer(C_INT) :: st
+        End Function
+        Function c_lm_count(plan) Bind(C, name='raysht_lm_count') Result(n)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan
+            Integer(C_INT) :: n
+        End Function
+        Function c_stp_dev(plan, s, p2, stream) Bind(C, name='raysht_legendre_stp') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, s, p2, stream
+            Integer(C_INT) :: st
+        End Function
+        Function c_pts_dev(plan, p2, s, stream) Bind(C, name='raysht_legendre_pts') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, p2, s, stream
+            Integer(C_INT) :: st
+        End Function
+        Function c_fft_phys_dev(nphi, howmany, x, stream) Bind(C, name='raysht_fft_to_physical') Result(st)
+            Import :: C_PTR, C_INT, C_LONG
+            Integer(C_INT), Value :: nphi
+            Integer(C_LONG), Value :: howmany
+            Type(C_PTR), Value :: x, stream
+            Integer(C_INT) :: st
+        End Function
+        Function c_fft_spec_dev(nphi, howmany, x, rescale, stream) Bind(C, name='raysht_fft_to_spectral') Result(st)
+            Import :: C_PTR, C_INT, C_LONG
+            Integer(C_INT), Value :: nphi, rescale
+            Integer(C_LONG), Value :: howmany
+            Type(C_PTR), Value :: x, stream
+            Integer(C_INT) :: st
+        End Function
+        Function c_p2a3a_dev(plan, nphi, p2, p3, stream) Bind(C, name='raysht_permute_2a3a') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, p2, p3, stream
+            Integer(C_INT), Value :: nphi
+            Integer(C_INT) :: st
+        End Function
+        Function c_p3b2b_dev(plan, nphi, p3, p2, stream) Bind(C, name='raysht_permute_3b2b') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, p3, p2, stream
+            Integer(C_INT), Value :: nphi
+            Integer(C_INT) :: st
+        End Function
+        Function c_p2b1b_dev(plan, s, p1, stream) Bind(C, name='raysht_permute_2b1b') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, s, p1, stream
+            Integer(C_INT) :: st
+        End Function
+        Function c_p1a2a_dev(plan, p1, s, stream) Bind(C, name='raysht_permute_1a2a') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, p1, s, stream
+            Integer(C_INT) :: st
+        End Function
+        ! the HIP runtime's own allocation and copy calls, for device-resident callers
+        Function hip_malloc(ptr, nbytes) Bind(C, name='hipMalloc') Result(st)
+            Import :: C_PTR, C_INT, C_SIZE_T
+            Type(C_PTR), Intent(Out) :: ptr
+            Integer(C_SIZE_T), Value :: nbytes
+            Integer(C_INT) :: st
+        End Function
+        Function hip_free(ptr) Bind(C, name='hipFree') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: ptr
+            Integer(C_INT) :: st
+        End Function
+        Function hip_memcpy(dst, src, nbytes, kind) Bind(C, name='hipMemcpy') Result(st)
+            Import :: C_PTR, C_INT, C_SIZE_T
+            Type(C_PTR), Value :: dst, src
+            Integer(C_SIZE_T), Value :: nbytes
+            Integer(C_INT), Value :: kind
+            Integer(C_INT) :: st
+        End Function
+        Function hip_device_synchronize() Bind(C, name='hipDeviceSynchronize') Result(st)
+            Import :: C_INT
+            Integer(C_INT) :: st
+        End Function
         Function c_strerror(status) Bind(C, name='raysht_strerror') Result(msg)
             Import :: C_PTR, C_INT
             Integer(C_INT), Value :: status
@@ -216,7 +372,6 @@ Contains
         Integer, Intent(In) :: nt, lmax, mval(:)
         Logical, Intent(In) :: parity_in
         Type(C_PTR) :: probe
-        Real(C_DOUBLE), Allocatable :: xh(:), xl(:), wh(:), wl(:)
         If (.not. parity_in) Then
             Write(6, *) 'RaySHT: only the parity-split transforms are provided'
             Error Stop 1
@@ -229,10 +384,8 @@ Contains
         ! the grid, from a one-column plan (the nodes do not depend on the columns)
         Call check(c_plan_create_cols(probe, leg_lmax, leg_ntheta, leg_mvals, size(leg_mvals), 1, 0), &
                    'Initialize_Legendre')
-        Allocate(xh(nt), xl(nt), wh(nt), wl(nt), coloc(nt), gl_weights(nt))
-        Call check(c_plan_nodes(probe, xh, xl, wh, wl), 'Initialize_Legendre')
-        coloc(:) = xh(:)
-        gl_weights(:) = wh(:)
+        Allocate(coloc(nt), gl_weights(nt))
+        Call check(c_plan_nodes_quad(probe, c_loc(coloc), c_loc(gl_weights)), 'Initialize_Legendre')
         plans(1) = probe
         plan_key(:, 1) = (/ -1, 0 /)
     End Subroutine Initialize_Legendre
@@ -454,6 +607,368 @@ Contains
         Call check(c_cheb_deriv(cheb, c_loc(buffer(1, 1, 1, ind)), c_loc(buffer(1, 1, 1, dind)), ncols, &
                                 int(dorder, C_INT)), 'Cheby_d_by_dr_cp')
     End Subroutine Cheby_d_by_dr_cp
+
+    ! ------------------------------------------------------------------ Cheby_Grid
+    ! Initialize_Cheby_Grid (Chebyshev_Polynomials.F90:65-207) for one domain of npoly(1)
+    ! zeros-grid points between bounds(1) and bounds(2) (ascending, as the caller passes
+    ! them): grid(1:n) receives the radii (descending), integration_weights(1:n) the radial
+    ! integration weights; the components are filled as the reference fills them.  dmax and
+    ! nthread are accepted (the device kernels need neither); more than one domain, or a
+    ! custom dealias_by, stop with a message.
+    Subroutine Initialize_Cheby_Grid(self, grid, integration_weights, ndomains, npoly, bounds, dmax, nthread, &
+                                     dealias_by, verbose)
+        Class(Cheby_Grid) :: self
+        Real*8, Intent(InOut) :: grid(1:), integration_weights(1:)
+        Real*8, Intent(In) :: bounds(1:)
+        Integer, Intent(In) :: ndomains, npoly(1:)
+        Integer, Intent(In), Optional :: nthread, dmax
+        Integer, Intent(In), Optional :: dealias_by(1:)
+        Logical, Intent(In), Optional :: verbose
+        Integer :: n
+        Real*8, Allocatable, Target :: x(:), th(:), ds(:), g(:), w(:)
+        If (ndomains /= 1) Then
+            Write(6, *) 'RaySHT: Cheby_Grid%Init supports one Chebyshev domain, got ', ndomains
+            Error Stop 1
+        Endif
+        n = npoly(1)
+        If (present(dealias_by)) Then
+            If (size(dealias_by) >= 1) Then
+                If (dealias_by(1) >= 1 .and. dealias_by(1) < n) Then
+                    Write(6, *) 'RaySHT: Cheby_Grid%Init: custom dealias_by is not supported'
+                    Error Stop 1
+                Endif
+            Endif
+        Endif
+        If (present(verbose)) Then
+            If (verbose) Write(6, *) 'Ndomains: ', ndomains, ' npoly: ', n, ' bounds: ', bounds(1:2)
+        Endif
+        Call self%finalize()
+        Allocate(x(n), th(n), ds(n), g(n), w(n))
+        Call check(c_cheb_create(self%handle, int(n, C_INT), bounds(1), bounds(2), g), 'Cheby_Grid%Init')
+        Call check(c_cheb_grid_data(self%handle, c_loc(x), c_loc(th), c_loc(w), c_loc(ds)), 'Cheby_Grid%Init')
+        grid(:) = 0.0d0
+        integration_weights(:) = 0.0d0
+        grid(1:n) = g(:)
+        integration_weights(1:n) = w(:)
+        self%domain_count = 1
+        self%max_npoly = n
+        self%ntotal = n
+        Allocate(self%npoly(1), self%n_x(1), self%rda(1), self%domain_bounds(2), self%n_even(1), self%n_odd(1))
+        self%npoly(1) = n
+        self%rda(1) = (2*n)/3 + 1
+        self%domain_bounds(1) = bounds(2)
+        self%domain_bounds(2) = bounds(1)
+        self%n_odd(1) = n/2
+        self%n_even(1) = n/2 + mod(n, 2)
+        self%n_x(1) = n/2 + mod(n, 2)
+        Allocate(self%x(n, 1), self%theta(n, 1), self%deriv_scaling(n))
+        self%x(:, 1) = x(:)
+        self%theta(:, 1) = th(:)
+        self%deriv_scaling(:) = ds(:)
+    End Subroutine Initialize_Cheby_Grid
+
+    Subroutine Finalize_Cheby_Grid(self)
+        Class(Cheby_Grid) :: self
+        If (c_associated(self%handle)) Call c_cheb_destroy(self%handle)
+        self%handle = C_NULL_PTR
+        If (allocated(self%npoly)) DeAllocate(self%npoly, self%n_x, self%rda, self%domain_bounds, self%n_even, &
+                                               self%n_odd, self%x, self%theta, self%deriv_scaling)
+        self%domain_count = 0
+    End Subroutine Finalize_Cheby_Grid
+
+    Subroutine require_grid(self, n1, what)
+        Class(Cheby_Grid), Intent(In) :: self
+        Integer, Intent(In) :: n1
+        Character(len=*), Intent(In) :: what
+        If (.not. c_associated(self%handle)) Then
+            Write(6, *) 'RaySHT: ', what, ' before Cheby_Grid%Init'
+            Error Stop 1
+        Endif
+        If (n1 /= self%npoly(1)) Then
+            Write(6, *) 'RaySHT: ', what, ': radial size ', n1, ' does not match the grid (', self%npoly(1), ')'
+            Error Stop 1
+        Endif
+    End Subroutine require_grid
+
+    ! To_Spectral4D (Chebyshev_Polynomials.F90:375-448)
+    Subroutine To_Spectral4D(self, f_in, c_out)
+        Class(Cheby_Grid) :: self
+        Real*8, Intent(In), Target, Contiguous :: f_in(:,:,:,:)
+        Real*8, Intent(InOut), Target, Contiguous :: c_out(:,:,:,:)
+        Call require_grid(self, size(f_in, 1), 'to_spectral')
+        Call check(c_cheb_to_spec(self%handle, c_loc(f_in), c_loc(c_out), int(size(f_in) / size(f_in, 1), C_LONG)), &
+                   'Cheby_Grid%to_spectral')
+    End Subroutine To_Spectral4D
+
+    ! From_Spectral4D (Chebyshev_Polynomials.F90:450-558)
+    Subroutine From_Spectral4D(self, c_in, f_out)
+        Class(Cheby_Grid) :: self
+        Real*8, Intent(In), Target, Contiguous :: c_in(:,:,:,:)
+        Real*8, Intent(InOut), Target, Contiguous :: f_out(:,:,:,:)
+        Call require_grid(self, size(c_in, 1), 'from_spectral')
+        Call check(c_cheb_from_spec(self%handle, c_loc(c_in), c_loc(f_out), int(size(c_in) / size(c_in, 1), C_LONG)), &
+                   'Cheby_Grid%from_spectral')
+    End Subroutine From_Spectral4D
+
+    ! dealias_buffer4d (Chebyshev_Polynomials.F90:353-374): coefficients rda..npoly of every
+    ! column set to zero, in the caller's (host) array
+    Subroutine dealias_buffer4d(self, buffer)
+        Class(Cheby_Grid) :: self
+        Real*8, Intent(InOut) :: buffer(1:,1:,1:,1:)
+        Call require_grid(self, size(buffer, 1), 'dealias_buffer')
+        buffer(self%rda(1):self%npoly(1), :, :, :) = 0.0d0
+    End Subroutine dealias_buffer4d
+
+    ! Cheby_Deriv_Buffer_4D (d_by_dr_cp, :560-640): buffer(:,:,:,dind) receives the dorder-th
+    ! radial derivative's coefficients of buffer(:,:,:,ind); ind /= dind (the reference's
+    ! in-place branch only rescales)
+    Subroutine Cheby_Deriv_Buffer_4D(self, ind, dind, buffer, dorder)
+        Class(Cheby_Grid) :: self
+        Real*8, Intent(InOut), Target, Contiguous :: buffer(0:,1:,1:,1:)
+        Integer, Intent(In) :: ind, dind, dorder
+        Call require_grid(self, size(buffer, 1), 'd_by_dr_cp')
+        If (ind == dind) Then
+            Write(6, *) 'RaySHT: Cheby_Grid%d_by_dr_cp needs ind /= dind'
+            Error Stop 1
+        Endif
+        Call check(c_cheb_deriv(self%handle, c_loc(buffer(0, 1, 1, ind)), c_loc(buffer(0, 1, 1, dind)), &
+                                int(size(buffer, 2), C_LONG) * size(buffer, 3), int(dorder, C_INT)), &
+                   'Cheby_Grid%d_by_dr_cp')
+    End Subroutine Cheby_Deriv_Buffer_4D
+
+    ! Cheby_Deriv_OutOf_Buffer (d_by_dr_nobuffer, :646-730): obuffer receives the dorder-th
+    ! radial derivative's coefficients of buffer(:,:,:,ind)
+    Subroutine Cheby_Deriv_OutOf_Buffer(self, ind, buffer, obuffer, dorder)
+        Class(Cheby_Grid) :: self
+        Real*8, Intent(InOut), Target, Contiguous :: buffer(0:,1:,1:,1:)
+        Real*8, Intent(InOut), Target, Contiguous :: obuffer(0:,1:,1:)
+        Integer, Intent(In) :: ind, dorder
+        Call require_grid(self, size(buffer, 1), 'd_by_dr_nobuffer')
+        Call check(c_cheb_deriv(self%handle, c_loc(buffer(0, 1, 1, ind)), c_loc(obuffer), &
+                                int(size(buffer, 2), C_LONG) * size(buffer, 3), int(dorder, C_INT)), &
+                   'Cheby_Grid%d_by_dr_nobuffer')
+    End Subroutine Cheby_Deriv_OutOf_Buffer
+
+    ! the device-resident forms: ncols columns of npoly values at device pointers f_dev / c_dev
+    Subroutine To_Spectral_Device(self, ncols, f_dev, c_dev, stream)
+        Class(Cheby_Grid) :: self
+        Integer*8, Intent(In) :: ncols
+        Type(C_PTR), Intent(In) :: f_dev, c_dev
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_cheb_to_spec_dev(self%handle, f_dev, c_dev, int(ncols, C_LONG), opt_stream(stream)), &
+                   'Cheby_Grid%to_spectral_device')
+    End Subroutine To_Spectral_Device
+
+    Subroutine From_Spectral_Device(self, ncols, c_dev, f_dev, stream)
+        Class(Cheby_Grid) :: self
+        Integer*8, Intent(In) :: ncols
+        Type(C_PTR), Intent(In) :: c_dev, f_dev
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_cheb_from_spec_dev(self%handle, c_dev, f_dev, int(ncols, C_LONG), opt_stream(stream)), &
+                   'Cheby_Grid%from_spectral_device')
+    End Subroutine From_Spectral_Device
+
+    Function opt_stream(stream) Result(st)
+        Type(C_PTR), Intent(In), Optional :: stream
+        Type(C_PTR) :: st
+        st = C_NULL_PTR
+        If (present(stream)) st = stream
+    End Function opt_stream
+
+    ! ------------------------------------------------------------------ reform (one rank)
+    ! Transpose_2a3a (Spherical_Buffer.F90:196-459): p2a(1:ntheta, 1:2 nr nf, 1:n_m) ->
+    ! p3a(1:nphi+2, 1:nr, 1:ntheta, 1:nf), zero-filled beyond m = l_max
+    Subroutine Reform_2a3a(p2a, p3a)
+        Real*8, Intent(In), Target, Contiguous :: p2a(:,:,:)
+        Real*8, Intent(InOut), Target, Contiguous :: p3a(:,:,:,:)
+        Integer :: nr, nf
+        nr = size(p3a, 2)
+        nf = size(p3a, 4)
+        If (size(p2a, 2) /= 2*nr*nf .or. size(p3a, 3) /= leg_ntheta .or. size(p2a, 1) /= leg_ntheta) Then
+            Write(6, *) 'RaySHT: Reform_2a3a argument shapes do not match the Legendre setup'
+            Error Stop 1
+        Endif
+        Call check(c_p2a3a_host(plan_for(nr, nf), int(size(p3a, 1) - 2, C_INT), c_loc(p2a), c_loc(p3a)), &
+                   'Reform_2a3a')
+    End Subroutine Reform_2a3a
+
+    ! Transpose_3b2b (Spherical_Buffer.F90:461-703): p3b -> p2b, m <= l_max read back
+    Subroutine Reform_3b2b(p3b, p2b)
+        Real*8, Intent(In), Target, Contiguous :: p3b(:,:,:,:)
+        Real*8, Intent(InOut), Target, Contiguous :: p2b(:,:,:)
+        Integer :: nr, nf
+        nr = size(p3b, 2)
+        nf = size(p3b, 4)
+        If (size(p2b, 2) /= 2*nr*nf .or. size(p3b, 3) /= leg_ntheta .or. size(p2b, 1) /= leg_ntheta) Then
+            Write(6, *) 'RaySHT: Reform_3b2b argument shapes do not match the Legendre setup'
+            Error Stop 1
+        Endif
+        Call check(c_p3b2b_host(plan_for(nr, nf), int(size(p3b, 1) - 2, C_INT), c_loc(p3b), c_loc(p2b)), &
+                   'Reform_3b2b')
+    End Subroutine Reform_3b2b
+
+    ! Transpose_2b1b (Spherical_Buffer.F90:705-843): s2b(mp)%data(m:lmax, 1:nr, 1:2, 1:nf) ->
+    ! p1b(1:nr, 1:2, 1:LM_Count(nr, nf), 1:nf), (l, m) pairs in LM_Load_Balance's order
+    Subroutine Reform_2b1b(s2b, p1b)
+        Type(rmcontainer4d), Intent(In), Target :: s2b(:)
+        Real*8, Intent(InOut), Target, Contiguous :: p1b(:,:,:,:)
+        Type(C_PTR) :: ptrs(size(s2b))
+        Integer :: mp, nr, nf
+        nr = size(s2b(1)%data, 2)
+        nf = size(s2b(1)%data, 4)
+        If (size(p1b, 1) /= nr .or. size(p1b, 4) /= nf .or. size(p1b, 3) /= LM_Count(nr, nf)) Then
+            Write(6, *) 'RaySHT: Reform_2b1b argument shapes do not match'
+            Error Stop 1
+        Endif
+        Do mp = 1, size(s2b)
+            ptrs(mp) = c_loc(s2b(mp)%data)
+        Enddo
+        Call check(c_p2b1b_host(plan_for(nr, nf), ptrs, c_loc(p1b)), 'Reform_2b1b')
+    End Subroutine Reform_2b1b
+
+    ! Transpose_1a2a (Spherical_Buffer.F90:845-932): the inverse
+    Subroutine Reform_1a2a(p1a, s2a)
+        Real*8, Intent(In), Target, Contiguous :: p1a(:,:,:,:)
+        Type(rmcontainer4d), Intent(InOut), Target :: s2a(:)
+        Type(C_PTR) :: ptrs(size(s2a))
+        Integer :: mp, nr, nf
+        nr = size(s2a(1)%data, 2)
+        nf = size(s2a(1)%data, 4)
+        If (size(p1a, 1) /= nr .or. size(p1a, 4) /= nf .or. size(p1a, 3) /= LM_Count(nr, nf)) Then
+            Write(6, *) 'RaySHT: Reform_1a2a argument shapes do not match'
+            Error Stop 1
+        Endif
+        Do mp = 1, size(s2a)
+            ptrs(mp) = c_loc(s2a(mp)%data)
+        Enddo
+        Call check(c_p1a2a_host(plan_for(nr, nf), c_loc(p1a), ptrs), 'Reform_1a2a')
+    End Subroutine Reform_1a2a
+
+    ! my_num_lm of the radial layout: the number of (l, m) pairs of the Legendre setup
+    Function LM_Count(nr, nf) Result(n)
+        Integer, Intent(In) :: nr, nf
+        Integer :: n
+        n = int(c_lm_count(plan_for(nr, nf)))
+    End Function LM_Count
+
+    ! ------------------------------------------------------------------ device-resident path
+    ! The same transforms on device buffers (Type(C_PTR) from Device_Allocate, the layouts of
+    ! include/raysht.h: spectral blocks packed in m_values order), asynchronous on `stream`
+    ! (default: the null stream); nothing is copied between host and device.
+    Subroutine StP_Device(nr, nf, s_dev, p2_dev, stream)
+        Integer, Intent(In) :: nr, nf
+        Type(C_PTR), Intent(In) :: s_dev, p2_dev
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_stp_dev(plan_for(nr, nf), s_dev, p2_dev, opt_stream(stream)), 'StP_Device')
+    End Subroutine StP_Device
+
+    Subroutine PtS_Device(nr, nf, p2_dev, s_dev, stream)
+        Integer, Intent(In) :: nr, nf
+        Type(C_PTR), Intent(In) :: p2_dev, s_dev
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_pts_dev(plan_for(nr, nf), p2_dev, s_dev, opt_stream(stream)), 'PtS_Device')
+    End Subroutine PtS_Device
+
+    Subroutine FFT_To_Physical_Device(nphi, howmany, x_dev, stream)
+        Integer, Intent(In) :: nphi
+        Integer*8, Intent(In) :: howmany
+        Type(C_PTR), Intent(In) :: x_dev
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_fft_phys_dev(int(nphi, C_INT), int(howmany, C_LONG), x_dev, opt_stream(stream)), &
+                   'FFT_To_Physical_Device')
+    End Subroutine FFT_To_Physical_Device
+
+    Subroutine FFT_To_Spectral_Device(nphi, howmany, x_dev, rsc, stream)
+        Integer, Intent(In) :: nphi
+        Integer*8, Intent(In) :: howmany
+        Type(C_PTR), Intent(In) :: x_dev
+        Logical, Intent(In), Optional :: rsc
+        Type(C_PTR), Intent(In), Optional :: stream
+        Integer(C_INT) :: r
+        r = 0
+        If (present(rsc)) Then
+            If (rsc) r = 1
+        Endif
+        Call check(c_fft_spec_dev(int(nphi, C_INT), int(howmany, C_LONG), x_dev, r, opt_stream(stream)), &
+                   'FFT_To_Spectral_Device')
+    End Subroutine FFT_To_Spectral_Device
+
+    Subroutine Reform_2a3a_Device(nr, nf, nphi, p2_dev, p3_dev, stream)
+        Integer, Intent(In) :: nr, nf, nphi
+        Type(C_PTR), Intent(In) :: p2_dev, p3_dev
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_p2a3a_dev(plan_for(nr, nf), int(nphi, C_INT), p2_dev, p3_dev, opt_stream(stream)), &
+                   'Reform_2a3a_Device')
+    End Subroutine Reform_2a3a_Device
+
+    Subroutine Reform_3b2b_Device(nr, nf, nphi, p3_dev, p2_dev, stream)
+        Integer, Intent(In) :: nr, nf, nphi
+        Type(C_PTR), Intent(In) :: p3_dev, p2_dev
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_p3b2b_dev(plan_for(nr, nf), int(nphi, C_INT), p3_dev, p2_dev, opt_stream(stream)), &
+                   'Reform_3b2b_Device')
+    End Subroutine Reform_3b2b_Device
+
+    Subroutine Reform_2b1b_Device(nr, nf, s_dev, p1_dev, stream)
+        Integer, Intent(In) :: nr, nf
+        Type(C_PTR), Intent(In) :: s_dev, p1_dev
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_p2b1b_dev(plan_for(nr, nf), s_dev, p1_dev, opt_stream(stream)), 'Reform_2b1b_Device')
+    End Subroutine Reform_2b1b_Device
+
+    Subroutine Reform_1a2a_Device(nr, nf, p1_dev, s_dev, stream)
+        Integer, Intent(In) :: nr, nf
+        Type(C_PTR), Intent(In) :: p1_dev, s_dev
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_p1a2a_dev(plan_for(nr, nf), p1_dev, s_dev, opt_stream(stream)), 'Reform_1a2a_Device')
+    End Subroutine Reform_1a2a_Device
+
+    ! device buffers for the *_Device procedures (the HIP runtime's hipMalloc / hipMemcpy /
+    ! hipFree); errors stop the program
+    Function Device_Allocate(nbytes) Result(ptr)
+        Integer*8, Intent(In) :: nbytes
+        Type(C_PTR) :: ptr
+        If (hip_malloc(ptr, int(nbytes, C_SIZE_T)) /= 0) Then
+            Write(6, *) 'RaySHT: Device_Allocate of ', nbytes, ' bytes failed'
+            Error Stop 1
+        Endif
+    End Function Device_Allocate
+
+    Subroutine Device_Free(ptr)
+        Type(C_PTR), Intent(InOut) :: ptr
+        If (c_associated(ptr)) Then
+            If (hip_free(ptr) /= 0) Then
+                Write(6, *) 'RaySHT: Device_Free failed'
+                Error Stop 1
+            Endif
+        Endif
+        ptr = C_NULL_PTR
+    End Subroutine Device_Free
+
+    Subroutine Device_Upload(dev, host, nbytes)
+        Type(C_PTR), Intent(In) :: dev, host
+        Integer*8, Intent(In) :: nbytes
+        If (hip_memcpy(dev, host, int(nbytes, C_SIZE_T), 1_C_INT) /= 0) Then
+            Write(6, *) 'RaySHT: Device_Upload failed'
+            Error Stop 1
+        Endif
+    End Subroutine Device_Upload
+
+    Subroutine Device_Download(host, dev, nbytes)
+        Type(C_PTR), Intent(In) :: host, dev
+        Integer*8, Intent(In) :: nbytes
+        If (hip_memcpy(host, dev, int(nbytes, C_SIZE_T), 2_C_INT) /= 0) Then
+            Write(6, *) 'RaySHT: Device_Download failed'
+            Error Stop 1
+        Endif
+    End Subroutine Device_Download
+
+    Subroutine Device_Synchronize()
+        If (hip_device_synchronize() /= 0) Then
+            Write(6, *) 'RaySHT: Device_Synchronize failed'
+            Error Stop 1
+        Endif
+    End Subroutine Device_Synchronize
 
     ! the buffer forms: field fin -> field fout (1-based) of the same rmcontainer4d array
     Subroutine angular_buffer4d(kind, A, fin, fout)

@@ -2,11 +2,14 @@
 ! Test program for the RaySHT Fortran binding, written the way Rayleigh's own
 ! Test_SHT (src/Test_Suite/Test_SHT.F90) exercises its transforms, on one rank:
 ! single (l,m) = (2,1) mode against the analytic field, 10 white-noise round trips,
-! the Legendre + transpose + FFT loop, the 2-D Legendre pair and a Chebyshev round
-! trip.  Prints one PASSING / FAIL line per check (tolerance 1e-10, as Test_SHT.F90:34)
+! the Legendre + transpose + FFT loop (index loops, then Reform_2a3a / Reform_3b2b, then
+! the device-resident procedures), the radial reforms, the 2-D Legendre pair, the Real*16
+! grid, and the Chebyshev transforms both through Cheby_Init and through a Cheby_Grid
+! object called as Rayleigh calls gridcp (Test_Cheby.F90).  Prints one PASSING / FAIL line per check (tolerance 1e-10, as Test_SHT.F90:34)
 ! and exits non-zero on any failure.
 !
 Program test_sht_raysht
+    Use ISO_C_BINDING
     Use Structures
     Use RaySHT_Math_Layer
     Implicit None
@@ -25,11 +28,15 @@ Program test_sht_raysht
     Call single_mode(nfail)
     Call white_noise(nfail)
     Call lt_fft_loop(nfail)
+    Call reform_loop(nfail)
+    Call radial_reforms(nfail)
+    Call real16_grid(nfail)
     Call two_d_pair(nfail)
     Call cheby_round_trip(nfail)
     Call cheby_derivatives(nfail)
     Call angular_derivatives(nfail)
     Call implicit_solves(nfail)
+    Call cheby_grid_object(nfail)
     ! the split forms with 6 digits stay within the same tolerance over ten round trips
     Call Set_Legendre_Precision(raysht_int8x6)
     Call white_noise(nfail)
@@ -244,6 +251,167 @@ Contains
         Call report('LT/FFT/transpose', d, nfail)
     End Subroutine lt_fft_loop
 
+    ! Test_SHT.F90:449-575 with SphericalBuffer%reform's one-rank transposes as Reform_2a3a /
+    ! Reform_3b2b (checked against the index loops above), then the same chain on device
+    ! buffers through the *_Device procedures (checked bit for bit against the host arrays)
+    Subroutine reform_loop(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: nf = 5
+        Type(rmcontainer4d), Allocatable :: s(:)
+        Real*8, Allocatable, Target :: p(:,:,:), x(:,:,:,:), xi(:,:,:,:), norm(:,:), packed(:), back(:)
+        Type(C_PTR) :: ds, dp, dx
+        Real*8 :: d, dd, ans
+        Integer :: mp, m, l, f, r, t, off, nl
+        Integer*8 :: nbs, nbp, nbx
+        Call alloc_spec(s, nf)
+        Allocate(p(nt, 2*nr*nf, lmax+1), x(nphi+2, nr, nt, nf), xi(nphi+2, nr, nt, nf), norm(lmax+1, 2))
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Do l = m, lmax-1
+                Do r = 1, nr
+                    Do f = 1, nf
+                        s(mp)%data(l, r, 1, f) = (l+1)*1.0d0*(2*f+3)*r
+                        s(mp)%data(l, r, 2, f) = (m+1)*1.0d0*(4*f+5)*r*r
+                    Enddo
+                Enddo
+            Enddo
+            norm(mp, 1) = max(maxval(abs(s(mp)%data(:, :, 1, :))), 1.0d-300)
+            norm(mp, 2) = max(maxval(abs(s(mp)%data(:, :, 2, :))), 1.0d-300)
+            s(mp)%data(:, :, 1, :) = s(mp)%data(:, :, 1, :)/norm(mp, 1)
+            s(mp)%data(:, :, 2, :) = s(mp)%data(:, :, 2, :)/norm(mp, 2)
+            s(mp)%data(lmax, :, :, :) = 0.0d0
+            If (m == 0) s(mp)%data(:, :, 2, :) = 0.0d0
+        Enddo
+        ! the device copy of the input: blocks packed in m_values order
+        nbs = 0
+        Do mp = 1, lmax+1
+            nbs = nbs + size(s(mp)%data)
+        Enddo
+        Allocate(packed(nbs), back(nbs))
+        off = 0
+        Do mp = 1, lmax+1
+            nl = size(s(mp)%data)
+            packed(off+1:off+nl) = reshape(s(mp)%data, (/ nl /))
+            off = off + nl
+        Enddo
+        nbs = nbs * 8
+        nbp = int(size(p), 8) * 8
+        nbx = int(size(x), 8) * 8
+        ds = Device_Allocate(nbs)
+        dp = Device_Allocate(nbp)
+        dx = Device_Allocate(nbx)
+        Call Device_Upload(ds, c_loc(packed), nbs)
+        ! host arrays: Legendre_Transform, Reform_2a3a, the FFTs, Reform_3b2b, Legendre_Transform
+        Call Legendre_Transform(s, p)
+        Call Reform_2a3a(p, x)
+        xi(:, :, :, :) = 0.0d0
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Do f = 1, nf
+                Do t = 1, nt
+                    Do r = 1, nr
+                        xi(2*m+1, r, t, f) = p(t, r + (f-1)*2*nr, mp)
+                        xi(2*m+2, r, t, f) = p(t, r + nr + (f-1)*2*nr, mp)
+                    Enddo
+                Enddo
+            Enddo
+        Enddo
+        dd = maxval(abs(x - xi))          ! a pure permutation: exactly the index loop
+        Call FFT_To_Physical(x, rsc=.true.)
+        Call FFT_To_Spectral(x, rsc=.true.)
+        Call Reform_3b2b(x, p)
+        Call Legendre_Transform(p, s)
+        d = 0.0d0
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Do l = m, lmax-1
+                Do r = 1, nr
+                    Do f = 1, nf
+                        ans = (l+1)*1.0d0*(2*f+3)*r/norm(mp, 1)
+                        d = max(d, abs(ans - s(mp)%data(l, r, 1, f)))
+                        ans = 0.0d0
+                        If (m > 0) ans = (m+1)*1.0d0*(4*f+5)*r*r/norm(mp, 2)
+                        d = max(d, abs(ans - s(mp)%data(l, r, 2, f)))
+                    Enddo
+                Enddo
+            Enddo
+        Enddo
+        Call report('LT/FFT/reform (Reform_2a3a, Reform_3b2b)', max(d, dd), nfail)
+        ! device buffers: the same six steps, nothing copied between them
+        Call StP_Device(nr, nf, ds, dp)
+        Call Reform_2a3a_Device(nr, nf, nphi, dp, dx)
+        Call FFT_To_Physical_Device(nphi, int(nr, 8)*nt*nf, dx)
+        Call FFT_To_Spectral_Device(nphi, int(nr, 8)*nt*nf, dx)
+        Call Reform_3b2b_Device(nr, nf, nphi, dx, dp)
+        Call PtS_Device(nr, nf, dp, ds)
+        Call Device_Synchronize()
+        Call Device_Download(c_loc(back), ds, nbs)
+        Call Device_Free(ds)
+        Call Device_Free(dp)
+        Call Device_Free(dx)
+        d = 0.0d0
+        off = 0
+        Do mp = 1, lmax+1
+            nl = size(s(mp)%data)
+            d = max(d, maxval(abs(back(off+1:off+nl) - reshape(s(mp)%data, (/ nl /)))))
+            off = off + nl
+        Enddo
+        Call report('LT/FFT/reform on device buffers (equal to host arrays)', d, nfail)
+        If (d /= 0.0d0) Then
+            Write(6, '(A)') 'device path differs from the host-array path'
+            nfail = nfail + 1
+        Endif
+    End Subroutine reform_loop
+
+    ! Transpose_2b1b then Transpose_1a2a (Spherical_Buffer.F90:705-932) as Reform_2b1b /
+    ! Reform_1a2a: a permutation of the values (same sum of squares) whose inverse returns
+    ! the spectral array exactly
+    Subroutine radial_reforms(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: nf = 3
+        Type(rmcontainer4d), Allocatable :: s(:), s2(:)
+        Real*8, Allocatable :: p1(:,:,:,:)
+        Real*8 :: d, ss, sp
+        Integer :: mp, m, l
+        Call alloc_spec(s, nf)
+        Call alloc_spec(s2, nf)
+        ss = 0.0d0
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Do l = m, lmax
+                s(mp)%data(l, :, 1, :) = sin(0.3d0*l + 0.17d0*m)
+                s(mp)%data(l, :, 2, :) = cos(0.7d0*l - 0.29d0*m)
+            Enddo
+            ss = ss + sum(s(mp)%data**2)
+        Enddo
+        Allocate(p1(nr, 2, LM_Count(nr, nf), nf))
+        Call Reform_2b1b(s, p1)
+        sp = sum(p1**2)
+        Call Reform_1a2a(p1, s2)
+        d = abs(ss - sp) / ss
+        Do mp = 1, lmax+1
+            d = max(d, maxval(abs(s2(mp)%data - s(mp)%data)))
+        Enddo
+        Call report('Reform_2b1b / Reform_1a2a', d, nfail)
+    End Subroutine radial_reforms
+
+    ! coloc and gl_weights as the reference keeps them (Real*16): ascending from -1,
+    ! symmetric, weights summing to 2
+    Subroutine real16_grid(nfail)
+        Integer, Intent(InOut) :: nfail
+        Real*8 :: d
+        Integer :: i
+        d = real(abs(sum(gl_weights) - 2.0_16), 8)
+        Do i = 1, nt
+            d = max(d, real(abs(coloc(i) + coloc(nt+1-i)), 8))
+            If (i > 1) Then
+                If (coloc(i) <= coloc(i-1)) d = 1.0d0
+            Endif
+        Enddo
+        If (kind(coloc) /= 16) d = 1.0d0
+        Call report('Real*16 Gauss-Legendre grid', d, nfail)
+    End Subroutine real16_grid
+
     ! StP_2d_dgp / PtS_2d_dgpv2 with 7 anonymous columns: the round trip is the identity / N_phi
     Subroutine two_d_pair(nfail)
         Integer, Intent(InOut) :: nfail
@@ -380,6 +548,60 @@ Contains
         d = max(d, maxval(abs(rb - xb)))
         Call report('Implicit solves (dense, band)', d, nfail)
     End Subroutine implicit_solves
+
+    ! A Cheby_Grid object used as Rayleigh uses gridcp (ProblemSize.F90:539, Test_Cheby.F90:
+    ! 230-340): Init with the grid and integration weights, to_spectral / from_spectral of a
+    ! 4-D array, d_by_dr_cp into other variables of the buffer, d_by_dr_nobuffer, and
+    ! dealias_buffer; the grid equals Cheby_Init's, the weights integrate r^2 dr over the shell
+    ! to 1 (to the quadrature's accuracy)
+    Subroutine cheby_grid_object(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: n = 48, nd = 8
+        Type(Cheby_Grid) :: gridcp, small
+        Real*8 :: radius(n), rw(n), g2(n), f(n, 3, 3, 2), c(n, 3, 3, 2), fc(n, 3, 3, 2), c2(n, 3, 3, 2)
+        Real*8 :: r8(nd), w8(nd), fd(nd, 2, 2, 4), cd(nd, 2, 2, 4), ans(nd, 2, 2, 4), ob(nd, 2, 2), d
+        Integer :: npoly(1), i, j
+        npoly(1) = n
+        Call gridcp%Init(radius, rw, 1, npoly, (/ 0.35d0, 1.0d0 /), nthread = 1, dealias_by = (/ -1 /))
+        Call Cheby_Init(n, 0.35d0, 1.0d0, g2)
+        Call Cheby_Finalize()
+        d = maxval(abs(radius - g2))
+        If (abs(sum(rw) - 1.0d0) > 1.0d-3 .or. gridcp%npoly(1) /= n .or. gridcp%rda(1) /= (2*n)/3 + 1) d = 1.0d0
+        Do j = 1, 3
+            Do i = 1, 3
+                f(:, i, j, 1) = exp(-4.0d0*radius(:)**2) + exp(-j*radius(:)**2)
+            Enddo
+        Enddo
+        f(:, :, :, 2) = 2.0d0*f(:, :, :, 1)
+        Call gridcp%to_spectral(f, c)
+        Call gridcp%from_spectral(c, fc)
+        d = max(d, maxval(abs(fc - f)))
+        c2 = c
+        Call gridcp%dealias_buffer(c2)
+        d = max(d, maxval(abs(c2(gridcp%rda(1):n, :, :, :))) + maxval(abs(c2(1:gridcp%rda(1)-1, :, :, :) - &
+                                                                           c(1:gridcp%rda(1)-1, :, :, :))))
+        Call gridcp%finalize()
+        ! derivatives of r^3 on [0.5, 2] (exact: 3 r^2, 6 r, 6)
+        npoly(1) = nd
+        Call small%Init(r8, w8, 1, npoly, (/ 0.5d0, 2.0d0 /))
+        fd = 0.0d0
+        Do i = 1, nd
+            fd(i, :, :, 1) = r8(i)**3
+            ans(i, :, :, 2) = 3.0d0 * r8(i)**2
+            ans(i, :, :, 3) = 6.0d0 * r8(i)
+            ans(i, :, :, 4) = 6.0d0
+        Enddo
+        Call small%to_spectral(fd, cd)
+        Call small%d_by_dr_cp(1, 2, cd, 1)
+        Call small%d_by_dr_cp(1, 3, cd, 2)
+        Call small%d_by_dr_cp(1, 4, cd, 3)
+        Call small%d_by_dr_nobuffer(1, cd, ob, 1)
+        d = max(d, maxval(abs(ob - cd(:, :, :, 2))))
+        Call small%from_spectral(cd, fd)
+        d = max(d, maxval(abs(fd(:, :, :, 2:4) - ans(:, :, :, 2:4))))
+        Call small%finalize()
+        Call report('Cheby_Grid object (Init, to/from_spectral, d_by_dr, dealias)', d, nfail)
+    End Subroutine cheby_grid_object
 
     ! Cheby_Deriv_Buffer_4D (d_by_dr_cp, Chebyshev_Polynomials.F90:560-640) as Test_Cheby's
     ! derivative check uses it: variables 2..4 of the buffer receive the first three

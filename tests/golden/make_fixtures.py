@@ -107,6 +107,30 @@ def rlm_case(name, lmax, mvals, nr, nf, fields, seed):
     print("wrote", name)
 
 
+def grid_case(name, thetas, chebs):
+    """Initialize_Legendre's Real*16 nodes and weights as stored (16 bytes per value) for
+    several N_theta, and Initialize_Cheby_Grid's radial grid and integration weights for
+    several (n, rmin, rmax)."""
+    R = O.Reference()
+    rec = {}
+    for nt in thetas:
+        R.legendre_init(nt, (2 * nt) // 3 - 1, np.arange(1, dtype=np.int32))
+        x16, w16 = R.nodes_binary128()
+        rec[f"coloc_q_{nt}"] = x16
+        rec[f"gl_weights_q_{nt}"] = w16
+    for i, (n, rmin, rmax) in enumerate(chebs):
+        g, w = R.cheby_init(n, rmin, rmax)
+        rec[f"cheb_{i}"] = np.array([n, rmin, rmax])
+        rec[f"cheb_grid_{i}"] = g
+        rec[f"cheb_weights_{i}"] = w
+    np.savez_compressed(os.path.join(OUT, name), **rec)
+    print("wrote", name)
+
+
+GRID_THETAS = (64, 96, 768)
+GRID_CHEBS = ((48, 0.35, 1.0), (33, 0.5, 1.7), (128, 0.35 / 0.65, 1.0 / 0.65), (256, 0.35, 1.0))
+
+
 if __name__ == "__main__":
     if not O.ref_available():
         sys.exit("compile the reference checker first: make -C oracle ref")
@@ -129,3 +153,4 @@ if __name__ == "__main__":
     angular_case("angular_L9_partial.npz", 9, [9, 0, 4, 3, 1], 5, seed=9)
     rlm_case("rlm_spaceb_L12_ascending.npz", 12, np.arange(13), 3, 5, (1, 2, 3), seed=120)
     rlm_case("rlm_spaceb_L41_partial.npz", 41, [41, 0, 17, 3, 40, 1], 4, 4, (3, 0, 2), seed=410)
+    grid_case("grid_nodes_weights.npz", GRID_THETAS, GRID_CHEBS)

@@ -17,7 +17,8 @@ def ra():
     return rayleigh_amd
 
 
-@pytest.mark.parametrize("nphi", [6, 12, 18, 36, 48, 72, 96, 192, 384, 768, 1536, 3072, 6144])
+@pytest.mark.parametrize("nphi", [6, 12, 18, 36, 48, 72, 96, 192, 384, 768, 1536, 3072, 6144,
+                                  10, 14, 70, 210, 336, 480, 960, 1920, 1050, 686])
 def test_fft_c2r_r2c_vs_oracle(ra, nphi):
     rng = np.random.default_rng(nphi)
     nl = 37
@@ -40,11 +41,11 @@ def test_fft_c2r_r2c_vs_oracle(ra, nphi):
 
 
 def test_fft_unsupported_length_is_an_error(ra):
-    """Lengths with a prime factor other than 2 and 3 are refused, not computed wrongly."""
+    """Lengths with a prime factor above 7 are refused, not computed wrongly."""
     from rayleigh_amd._lib import RayshtError
-    t = dev(np.zeros((2, 12)))
+    t = dev(np.zeros((2, 24)))
     with pytest.raises(RayshtError):
-        ra.fft_to_physical(t, 10)
+        ra.fft_to_physical(t, 22)
 
 
 def test_fft_round_trip_large_batch(ra):
@@ -188,3 +189,59 @@ def test_round_trip_graph_replays(ra, precision):
         # eager work between the replays, on the same buffers
         plan.stp(s_in, w)
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("n,rmin,rmax", [(48, 0.35, 1.0), (33, 0.5, 1.7), (256, 0.35, 1.0)])
+def test_cheb_grid_data_vs_oracle(ra, n, rmin, rmax):
+    """raysht_cheb_grid_data: Cheby_Grid%init's integration weights and components as the
+    oracle (bit-exact with the compiled reference) forms them."""
+    import ctypes
+    from rayleigh_amd import _lib
+    grid = ra.ChebyGrid(n, rmin, rmax)
+    x, th, w, ds = (np.zeros(n) for _ in range(4))
+    P = ctypes.c_void_p
+    assert _lib.lib().raysht_cheb_grid_data(grid._h, x.ctypes.data_as(P), th.ctypes.data_as(P),
+                                            w.ctypes.data_as(P), ds.ctypes.data_as(P)) == 0
+    assert np.array_equal(w, O.cheby_weights(n, rmin, rmax))
+    assert np.array_equal(grid.grid, O.cheby_grid(n, rmin, rmax))
+    assert np.abs(np.cos(th) - x).max() <= 2.3e-16 and np.all(np.diff(x) < 0)
+    assert np.all(ds == O.cheby_deriv_scaling(n, rmin, rmax))
+
+
+def test_permutes_host_forms_equal_device_forms(ra):
+    """The host-array permutes (what the Fortran Reform_* call) give the device forms' bits."""
+    import ctypes
+    from rayleigh_amd import _lib
+    P = ctypes.c_void_p
+    nt, lmax, nr, nf = 24, 15, 3, 2
+    nphi = 2 * nt
+    mv = O.m_order_balanced(lmax)
+    plan = ra.LegendrePlan(nt, lmax, mv, nr, nf)
+    lib = _lib.lib()
+    rng = np.random.default_rng(9)
+    p2 = rng.standard_normal(plan.hybrid_size)
+    n3 = (nphi + 2) * nr * nt * nf
+    p3 = np.zeros(n3)
+    assert lib.raysht_permute_2a3a_host(plan.handle, nphi, p2.ctypes.data_as(P), p3.ctypes.data_as(P)) == 0
+    t3 = empty(n3)
+    plan.reform_2a3a(dev(p2), t3, nphi)
+    assert np.array_equal(p3, host(t3))
+    p3b = rng.standard_normal(n3)
+    p2b = np.zeros(plan.hybrid_size)
+    assert lib.raysht_permute_3b2b_host(plan.handle, nphi, p3b.ctypes.data_as(P), p2b.ctypes.data_as(P)) == 0
+    t2 = empty(plan.hybrid_size)
+    plan.reform_3b2b(dev(p3b), t2, nphi)
+    assert np.array_equal(p2b, host(t2))
+    # radial: blocks through a pointer array
+    s = rng.standard_normal(plan.spectral_size)
+    off = O.spectral_offsets(lmax, mv, nr, nf)
+    ptrs = (P * len(mv))(*[s.ctypes.data + 8 * int(off[i]) for i in range(len(mv))])
+    p1 = np.zeros(plan.spectral_size)
+    assert lib.raysht_permute_2b1b_host(plan.handle, ptrs, p1.ctypes.data_as(P)) == 0
+    t1 = empty(plan.spectral_size)
+    plan.transpose_2b1b(dev(s), t1)
+    assert np.array_equal(p1, host(t1))
+    s2 = np.zeros(plan.spectral_size)
+    ptrs2 = (P * len(mv))(*[s2.ctypes.data + 8 * int(off[i]) for i in range(len(mv))])
+    assert lib.raysht_permute_1a2a_host(plan.handle, p1.ctypes.data_as(P), ptrs2) == 0
+    assert np.array_equal(s2, s)

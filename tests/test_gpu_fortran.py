@@ -22,6 +22,8 @@ def test_fortran_binding_test_sht():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = [ln for ln in r.stdout.splitlines() if "result =" in ln]
-    # eight FP64 checks (the last the Chebyshev and the angular derivatives), then the
-    # white-noise round trips with INT8X6 and BF16X6
-    assert len(lines) == 11 and all("PASSING" in ln for ln in lines), r.stdout
+    # fourteen FP64 checks (Legendre, the LT/FFT loop with index loops, with Reform_* and on
+    # device buffers, the radial reforms, the Real*16 grid, the 2-D pair, Chebyshev, angular
+    # derivatives, implicit solves, the Cheby_Grid object), then the white-noise round trips
+    # with INT8X6 and BF16X6
+    assert len(lines) == 16 and all("PASSING" in ln for ln in lines), r.stdout

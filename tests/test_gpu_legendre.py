@@ -213,3 +213,19 @@ def test_host_array_entry_points(ra):
     optrs = (ctypes.c_void_p * len(mv))(*[b.ctypes.data for b in outs])
     _lib.check(_lib.lib().raysht_legendre_pts_host(plan.handle, pr.ctypes.data_as(ctypes.c_void_p), optrs), "pts_host")
     assert rel_max(np.concatenate(outs), O.pts(nt, L, mv, nr, nf, tabs, pr)) < TOL_FP64
+
+
+@pytest.mark.parametrize("nt", [64, 96, 768])
+def test_nodes_binary128_match_oracle(ra, nt):
+    """raysht_plan_nodes_quad: the Real*16 coloc / gl_weights the reference stores, byte
+    for byte as the oracle forms them (the oracle's are the compiled reference's)."""
+    import ctypes
+    from rayleigh_amd import _lib
+    lmax = (2 * nt) // 3 - 1
+    plan = ra.LegendrePlan(nt, lmax, np.arange(lmax + 1, dtype=np.int32), 1, 1)
+    x = np.zeros((nt, 16), np.uint8)
+    w = np.zeros((nt, 16), np.uint8)
+    assert _lib.lib().raysht_plan_nodes_quad(plan.handle, x.ctypes.data_as(ctypes.c_void_p),
+                                             w.ctypes.data_as(ctypes.c_void_p)) == 0
+    xo, wo = O.gauss_legendre_binary128(nt)
+    assert np.array_equal(x, xo) and np.array_equal(w, wo)
