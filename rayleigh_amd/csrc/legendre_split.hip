@@ -720,7 +720,7 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split2_kernel(
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int cc = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = c0 + cc;
-            const double r = __builtin_ldexp(dacc[v], (c < nrhs ? cexp[cc] : 0) + ej);
+            const double r = __builtin_ldexp(dacc[v], (c < nrhs ? cexp[c] : 0) + ej);
             if (par == 0) {
                 ebuf[v][tid] = r;
             } else if (jin && c < nrhs) {
