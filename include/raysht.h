@@ -38,7 +38,8 @@ enum raysht_status {
     RAYSHT_EINVAL = 1,      /* bad size, layout or argument */
     RAYSHT_EHIP = 2,        /* the HIP runtime reported an error */
     RAYSHT_ENOMEM = 3,      /* device or host allocation failed */
-    RAYSHT_EUNSUPPORTED = 4 /* valid request the library does not implement */
+    RAYSHT_EUNSUPPORTED = 4, /* valid request the library does not implement */
+    RAYSHT_ENOTREADY = 5     /* a table the call may not make itself has not been made yet */
 };
 
 /* Arithmetic used by the Legendre contractions. */
@@ -226,6 +227,20 @@ int raysht_spectral_to_physical(raysht_plan *plan, int nphi, const double *s, do
                                 double *p3, void *stream);
 int raysht_physical_to_spectral(raysht_plan *plan, int nphi, double *p3, double *work,
                                 double *s, void *stream);
+/* raysht_spectral_to_physical without a `work` argument and without any device
+ * allocation: no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under hipGraph stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The hybrid intermediate lives in a library-owned static device array of
+ * RAYSHT_SCRATCH_BYTES per GPU (held until the process ends); plans whose hybrid array
+ * (raysht_hybrid_size doubles) is larger are refused with RAYSHT_EINVAL before anything
+ * is launched.  The FFT twiddle table of nphi must already exist on the device (any
+ * earlier FFT, round-trip or graph call of that nphi makes it); otherwise
+ * RAYSHT_ENOTREADY.  Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (other streams, other host threads) race; p3 and the plan's
+ * outputs are as raysht_spectral_to_physical's, bit for bit. */
+#define RAYSHT_SCRATCH_BYTES (32u << 20)
+int raysht_spectral_to_physical_scratch(raysht_plan *plan, int nphi, const double *s, double *p3,
+                                        void *stream);
 
 /* The spectral -> physical -> spectral round trip (StP, 2a->3a, c2r, r2c, 3b->2b,
  * PtS: Sphere_Hybrid_Space.F90:111,143 / Sphere_Physical_Space.F90:115,215 around an

@@ -47,6 +47,7 @@ SIGNATURES = {
     "raysht_permute_3b2b": (_I, [_P, _I, _P, _P, _P]),
     "raysht_spectral_to_physical": (_I, [_P, _I, _P, _P, _P, _P]),
     "raysht_physical_to_spectral": (_I, [_P, _I, _P, _P, _P, _P]),
+    "raysht_spectral_to_physical_scratch": (_I, [_P, _I, _P, _P, _P]),
     "raysht_graph_create_round_trip": (_I, [ctypes.POINTER(_P), _P, _I, _P, _P, _P, _P]),
     "raysht_graph_create_batch": (_I, [ctypes.POINTER(_P), _P, _P, _P, _P]),
     "raysht_graph_launch": (_I, [_P, _P]),

@@ -18,6 +18,12 @@
 
 #include "raysht_internal.h"
 
+// The hybrid intermediate of raysht_spectral_to_physical_scratch: a file-scope device array,
+// one copy per GPU, held until the process ends.  Host code never passes this name to a
+// kernel or a copy (on the host it is only a stand-in for the symbol): it uses the device
+// address that hipGetSymbolAddress gives for the current device (scratch_address below).
+static __device__ __align__(256) double scratch[RAYSHT_SCRATCH_BYTES / sizeof(double)];
+
 namespace raysht {
 namespace {
 
@@ -678,7 +684,10 @@ struct twiddle_cache {
     std::map<std::pair<int, int>, double2 *> tables;
 } g_tw;
 
-int get_twiddles(int nphi, const double2 **out)
+// The device table of e^{+2 pi i u / nphi} of the current device, made on first use.  With
+// create = false the table must exist already (RAYSHT_ENOTREADY otherwise): nothing is
+// allocated or copied.
+int get_twiddles(int nphi, const double2 **out, bool create = true)
 {
     int dev = 0;
     RAYSHT_HIP_TRY(hipGetDevice(&dev));
@@ -689,6 +698,7 @@ int get_twiddles(int nphi, const double2 **out)
         *out = it->second;
         return RAYSHT_OK;
     }
+    if (!create) return RAYSHT_ENOTREADY;
     std::vector<double> c(nphi), s(nphi);
     fft_twiddles(nphi, c.data(), s.data());
     std::vector<double2> h(nphi);
@@ -782,6 +792,22 @@ int check_fft_args(int nphi, long howmany, const double *x)
     return RAYSHT_OK;
 }
 
+// the c2r of `howmany` lines with the twiddle table tw of nphi (arguments checked)
+int launch_c2r(int nphi, long howmany, double *x, const double2 *tw, hipStream_t st)
+{
+    int rc = RAYSHT_OK;
+    if (fft3_dispatch(nphi / 2, true, howmany, x, tw, 0, st, &rc)) return rc;
+    int code = 0, nrad = 0;
+    fft_schedule(nphi / 2, &code, &nrad);
+    const int lines = fft_lines_per_block(nphi / 2);
+    const long blocks = std::min<long>((howmany + lines - 1) / lines, 1L << 20);
+    const size_t lds = 2 * sizeof(double2) * (size_t)lines * (nphi / 2);
+    hipLaunchKernelGGL(fft_c2r_kernel, dim3((unsigned)blocks), dim3(kFftThreads), lds, st, nphi, howmany, lines, x,
+                       tw, code, nrad);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
 }  // namespace
 
 int fft_prepare(int nphi)
@@ -802,16 +828,7 @@ int raysht_fft_to_physical(int nphi, long howmany, double *x, void *stream)
     if (rc != RAYSHT_OK || howmany == 0) return rc;
     const double2 *tw = nullptr;
     if ((rc = get_twiddles(nphi, &tw)) != RAYSHT_OK) return rc;
-    if (fft3_dispatch(nphi / 2, true, howmany, x, tw, 0, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
-    int code = 0, nrad = 0;
-    fft_schedule(nphi / 2, &code, &nrad);
-    const int lines = fft_lines_per_block(nphi / 2);
-    const long blocks = std::min<long>((howmany + lines - 1) / lines, 1L << 20);
-    const size_t lds = 2 * sizeof(double2) * (size_t)lines * (nphi / 2);
-    hipLaunchKernelGGL(fft_c2r_kernel, dim3((unsigned)blocks), dim3(kFftThreads), lds,
-                       reinterpret_cast<hipStream_t>(stream), nphi, howmany, lines, x, tw, code, nrad);
-    RAYSHT_HIP_TRY(hipGetLastError());
-    return RAYSHT_OK;
+    return launch_c2r(nphi, howmany, x, tw, reinterpret_cast<hipStream_t>(stream));
 }
 
 int raysht_fft_to_spectral(int nphi, long howmany, double *x, int rescale, void *stream)
@@ -832,6 +849,47 @@ int raysht_fft_to_spectral(int nphi, long howmany, double *x, int rescale, void 
                        rescale ? 1 : 0);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
+}
+
+namespace {
+// the device address of `scratch` on the current device, looked up once per device
+struct scratch_table {
+    std::mutex mu;
+    std::map<int, double *> addr;
+} g_scratch;
+
+int scratch_address(double **out)
+{
+    int dev = 0;
+    RAYSHT_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch.mu);
+    auto it = g_scratch.addr.find(dev);
+    if (it == g_scratch.addr.end()) {
+        void *ptr = nullptr;
+        RAYSHT_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return RAYSHT_EHIP;
+        it = g_scratch.addr.emplace(dev, static_cast<double *>(ptr)).first;
+    }
+    *out = it->second;
+    return RAYSHT_OK;
+}
+}  // namespace
+
+int raysht_spectral_to_physical_scratch(raysht_plan *p, int nphi, const double *s, double *p3, void *stream)
+{
+    if (!p || !s || !p3) return RAYSHT_EINVAL;
+    int rc = check_fft_args(nphi, 1, p3);
+    if (rc != RAYSHT_OK) return rc;
+    if (raysht_hybrid_size(p) > RAYSHT_SCRATCH_BYTES / sizeof(double)) return RAYSHT_EINVAL;
+    if (p->nr < 1 || nphi < 2 * (p->lmax + 1) || (long)p->nr * p->nf > 65535) return RAYSHT_EINVAL;
+    const double2 *tw = nullptr;
+    if ((rc = get_twiddles(nphi, &tw, false)) != RAYSHT_OK) return rc;
+    double *work = nullptr;
+    if ((rc = scratch_address(&work)) != RAYSHT_OK) return rc;
+    const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if ((rc = raysht_legendre_stp(p, s, work, stream)) != RAYSHT_OK) return rc;
+    if ((rc = raysht_permute_2a3a(p, nphi, work, p3, stream)) != RAYSHT_OK) return rc;
+    return launch_c2r(nphi, (long)p->nr * p->nt * p->nf, p3, tw, st);
 }
 
 int raysht_permute_2a3a(raysht_plan *p, int nphi, const double *p2, double *p3, void *stream)

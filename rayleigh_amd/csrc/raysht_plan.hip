@@ -148,6 +148,7 @@ const char *raysht_strerror(int status)
     case RAYSHT_EHIP: return g_last_error.empty() ? "HIP runtime error" : g_last_error.c_str();
     case RAYSHT_ENOMEM: return "out of memory";
     case RAYSHT_EUNSUPPORTED: return "not supported";
+    case RAYSHT_ENOTREADY: return "a table this call may not make has not been made yet";
     default: return "unknown status";
     }
 }
