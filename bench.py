@@ -32,8 +32,8 @@ PEAK_BF16_TFLOPS = 2500.0    # MI355X BF16 matrix, dense
 PEAK_INT8_TOPS = 5000.0      # MI355X INT8 matrix, dense
 METRIC = "SHT round-trip ms + TFLOP/s at l_max=511: FP64 vs split-BF16 at 1e-12 rel err"
 # slice pairs (one full-size MFMA product each) per direction of the split forms
-SPLIT_PAIRS = {"bf16x6": {"stp": 27, "pts": 21}, "bf16x3": {"stp": 6, "pts": 6},
-               "int8x6": {"stp": 27, "pts": 21}, "int8x3": {"stp": 6, "pts": 6}}
+SPLIT_PAIRS = {"bf16x6": {"stp": 27, "pts": 21}, "bf16x3": {"stp": 10, "pts": 10},
+               "int8x6": {"stp": 27, "pts": 21}, "int8x3": {"stp": 10, "pts": 10}}
 
 
 def log(msg: str) -> None:
@@ -159,6 +159,7 @@ def reference_check_and_baseline(res, mvals, sample_m: int):
     s_dev = np.concatenate([res["s2"][off[mp]: off[mp + 1]].cpu().numpy() for mp in mps])
     flops = sum(2.0 * 2 * (NTHETA // 2) * 2 * NR * NF * (LMAX - int(m) + 1) for m in sample)
     threads = min(16, len(os.sched_getaffinity(0)))
+    one = None
     if O.ref_available():
         kind = "reference"
         from threadpoolctl import threadpool_limits
@@ -172,6 +173,23 @@ def reference_check_and_baseline(res, mvals, sample_m: int):
                 p_ref = R.stp(NR, NF, s_in)
                 s_ref = R.pts(NR, NF, p_dev)
                 times.append(time.perf_counter() - t0)
+        # the same path on one core: every fourth block of the sample, median of two runs
+        sub = sample[::4]
+        R.legendre_init(NTHETA, LMAX, sub)
+        s_sub = np.concatenate([s_all[off[pos[int(m)]]: off[pos[int(m)] + 1]].cpu().numpy() for m in sub])
+        p_sub = np.concatenate([res["p2"][pos[int(m)] * hyb: (pos[int(m)] + 1) * hyb].cpu().numpy() for m in sub])
+        f_sub = sum(2.0 * 2 * (NTHETA // 2) * 2 * NR * NF * (LMAX - int(m) + 1) for m in sub)
+        t1 = []
+        with threadpool_limits(limits=1):
+            for _ in range(2):
+                t0 = time.perf_counter()
+                R.stp(NR, NF, s_sub)
+                R.pts(NR, NF, p_sub)
+                t1.append(time.perf_counter() - t0)
+        one = {"value": f_sub / float(np.median(t1)) / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": kind,
+               "sample": f"StP+PtS on {len(sub)} of 512 m-blocks (m = {int(sub[0])}..{int(sub[-1])}); "
+                         f"{f_sub / 1e9:.1f} GFLOP in {float(np.median(t1)):.3f} s (median of 2)"}
+        log(f"cpu baseline, 1 core: {one['value']:.4f} TFLOP/s")
     else:
         kind = "port"
         threads = 1
@@ -196,15 +214,18 @@ def reference_check_and_baseline(res, mvals, sample_m: int):
                      f"N_theta={NTHETA}, N_r={NR}, 8 fields; {flops / 1e9:.1f} GFLOP in {t:.3f} s; "
                      f"full round trip at this rate = {2 * flops_per_direction() / (flops / t) * 1e3:.0f} ms"}
     log(f"cpu baseline: {cpu['value']:.4f} TFLOP/s; device vs {kind}: StP {err_stp:.2e}, PtS {err_pts:.2e}")
+    cpu["one_core"] = one
     return cpu, err_stp, err_pts
 
 
 def cpu_reference_extras(L, mv, nr, nf, nlm, n_lu, ncols, out):
-    """The reference's CPU path beside the device figures of the radial derivative, the
-    angular derivative and the implicit solve, each on a bounded sample scaled to the full
-    size: the compiled reference routines (oracle/_ref, single-threaded; the harness's copies
-    into and out of the reference's containers are included) and LAPACK dgetrf / dgetrs
-    (the library the reference's solve calls, numpy's OpenBLAS here, 16 threads)."""
+    """The reference's CPU path beside the device figures of the stack's stages, each on a
+    bounded sample scaled to the full size: the compiled reference routines (oracle/_ref,
+    one core; the harness's copies into and out of the reference's containers are
+    included) for StP / PtS, the Chebyshev pair, the radial and angular derivatives and
+    rlm_spaceb; numpy's pocketfft for the phi FFTs (one core: FFTW, the reference's
+    library, is not available); LAPACK dgetrf / dgetrs / dgbtrf / dgbtrs for the implicit
+    solve (the library the reference's solve calls, numpy's OpenBLAS here, 16 threads)."""
     import scipy.linalg.lapack as lapack
     from threadpoolctl import threadpool_limits
     from oracle import oracle as O
@@ -212,6 +233,38 @@ def cpu_reference_extras(L, mv, nr, nf, nlm, n_lu, ncols, out):
     rng = np.random.default_rng(0)
     if O.ref_available():
         R = O.Reference()
+        # the Legendre pair on a sample of m-blocks and the Chebyshev pair on a sample of
+        # columns, one core (the reference's own DGEMM calls), scaled to the full size
+        from threadpoolctl import threadpool_limits
+        nt = 3 * (L + 1) // 2
+        sample = np.unique(np.linspace(0, L, 8).round().astype(np.int32))
+        R.legendre_init(nt, L, sample)
+        s_in = rng.standard_normal(O.spectral_size(L, sample, nr, nf))
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            p_out = R.stp(nr, nf, s_in)
+            t_stp = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            R.pts(nr, nf, p_out)
+            t_pts = time.perf_counter() - t0
+        frac_m = sum(L - int(m) + 1 for m in sample) / ((L + 1) * (L + 2) / 2)
+        for key, t in (("stp", t_stp), ("pts", t_pts)):
+            res[key] = {"ms_full_size": t / frac_m * 1e3, "cores": 1, "kind": "reference",
+                        "sample": f"{len(sample)} m-blocks (m = {int(sample[0])}..{int(sample[-1])}), "
+                                  f"{frac_m * 100:.1f} % of the coefficients"}
+        R.cheby_init(nr, 0.35, 1.0)
+        ccols = max(1, 2 * nlm * nf // 64)
+        f = rng.standard_normal(nr * ccols)
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            c = R.cheby_to_spectral(f)
+            t_to = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            R.cheby_from_spectral(c)
+            t_from = time.perf_counter() - t0
+        for key, t in (("cheb_to_spectral", t_to), ("cheb_from_spectral", t_from)):
+            res[key] = {"ms_full_size": t * (2 * nlm * nf / ccols) * 1e3, "cores": 1, "kind": "reference",
+                        "sample": f"{ccols} of {2 * nlm * nf} columns of {nr}"}
         cols = 2 * nlm * nf // 32
         R.cheby_init(nr, 0.35, 1.0)
         c = rng.standard_normal(nr * cols)
@@ -234,6 +287,20 @@ def cpu_reference_extras(L, mv, nr, nf, nlm, n_lu, ncols, out):
         res["rlm_spaceb"] = {"ms_full_size": t * 1e3, "cores": 1, "kind": "reference",
                              "sample": "W, P, Z at the full size (a three-field buffer; the reference's "
                                        "d_by_sdtheta / d_by_dphi and rlm_spaceb's array statements)"}
+    # the phi FFTs: numpy's pocketfft (FFTW, the reference's library, is not available) on
+    # a sample of lines, one core, scaled to every (r, theta, field) line
+    nphi = 3 * (L + 1)
+    nlines = nr * (3 * (L + 1) // 2) * nf
+    lines = rng.standard_normal((4096, nphi))
+    t0 = time.perf_counter()
+    spec = np.fft.rfft(lines, axis=1)
+    t_r2c = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    np.fft.irfft(spec, n=nphi, axis=1)
+    t_c2r = time.perf_counter() - t0
+    for key, t in (("fft_r2c", t_r2c), ("fft_c2r", t_c2r)):
+        res[key] = {"ms_full_size": t * nlines / 4096 * 1e3, "cores": 1, "kind": "pocketfft (numpy.fft)",
+                    "sample": f"4096 of {nlines} lines of {nphi}"}
     threads = min(16, len(os.sched_getaffinity(0)))
     k = 16
     mats = [rng.standard_normal((n_lu, n_lu)) + n_lu * np.eye(n_lu) for _ in range(k)]
@@ -280,37 +347,57 @@ STACKS = {
 }
 
 
+def radial_coeffs_device(lmax, l_of_lm, m_of_lm, nr, nf, seed: int):
+    """Chebyshev coefficients in the radial layout p1[f][lm][imi][n] with the configuration's
+    distribution: (N(0,1) + i N(0,1)) (l+1)^-2 (n+1)^-3, m = 0 real, on the device."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    nlm = len(l_of_lm)
+    c = torch.randn((nf, nlm, 2, nr), dtype=torch.float64, device="cuda", generator=g)
+    c *= ((torch.as_tensor(np.asarray(l_of_lm), dtype=torch.float64, device="cuda") + 1.0) ** -2).view(1, nlm, 1, 1)
+    c *= ((torch.arange(nr, dtype=torch.float64, device="cuda") + 1.0) ** -3).view(1, 1, 1, nr)
+    c[:, torch.as_tensor(np.asarray(m_of_lm) == 0, device="cuda"), 1, :] = 0.0
+    return c.reshape(-1)
+
+
 def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
     """The transform stack (default l_max = 255, N_theta = 384, N_phi = 768, N_r = 128,
-    8 fields; "large": l_max = 1023, N_theta = 1536, N_phi = 3072, N_r = 256, 5 fields):
-    Chebyshev From_Spectral -> StP -> 2a->3a -> c2r, then r2c -> 3b->2b -> PtS -> To_Spectral,
-    each stage timed with HIP events; GB/s of the bandwidth-bound stages; then the radial
-    derivative recursion of orders 1-3 on the same spectral radial buffer.  The radial
-    buffer is transformed in place of Rayleigh's p1 layout (the 1a<->2a radial transposes
-    between the Chebyshev and Legendre stages are not implemented)."""
+    8 fields; "large": l_max = 1023, N_theta = 1536, N_phi = 3072, N_r = 256, 5 fields),
+    chained on the same device buffers as one time step runs it: Chebyshev From_Spectral ->
+    Transpose_1a2a -> StP -> 2a->3a -> c2r, then r2c -> 3b->2b -> PtS -> Transpose_2b1b ->
+    Chebyshev To_Spectral, each stage timed with HIP events (GB/s of the bandwidth-bound
+    stages, TFLOP/s of the contractions), on Chebyshev coefficients with the configuration's
+    distribution; the chain must return its input (N_phi = 2 N_theta, so the unnormalised
+    FFT pair and the weighted PtS cancel).  Then the radial derivative recursion, the
+    angular derivatives and the implicit solve on the same data."""
     import torch
     L, nt, nphi, nr, nf = STACKS[workload]
     mv = ra.m_values_balanced(L)
     plan = ra.LegendrePlan(nt, L, mv, nr, nf)
     cheb = ra.ChebyGrid(nr, 0.35, 1.0)
-    g = torch.Generator(device="cuda").manual_seed(3)
-    nlm = (L + 1) * (L + 2) // 2
-    rad = torch.randn(nr * 2 * nlm * nf, dtype=torch.float64, device="cuda", generator=g)
-    rad2 = torch.empty_like(rad)
-    s = torch.randn(plan.spectral_size, dtype=torch.float64, device="cuda", generator=g)
+    l_lm, mp_lm = plan.lm_values()
+    nlm = len(l_lm)
+    c0 = radial_coeffs_device(L, l_lm, mv[mp_lm], nr, nf, seed=3)
+    rad = torch.empty_like(c0)
+    rad2 = torch.empty_like(c0)
+    c2 = torch.empty_like(c0)
+    s = torch.empty(plan.spectral_size, dtype=torch.float64, device="cuda")
     p2 = torch.empty(plan.hybrid_size, dtype=torch.float64, device="cuda")
     n3 = (nphi + 2) * nr * nt * nf
     p3 = torch.empty(n3, dtype=torch.float64, device="cuda")
     s2 = torch.empty_like(s)
+    nb_rad = 16.0 * c0.numel()
     stages = [
-        ("cheb_from_spectral", lambda: cheb.from_spectral(rad, rad2), 16.0 * nr * 2 * nlm * nf),
+        ("cheb_from_spectral", lambda: cheb.from_spectral(c0, rad), nb_rad),
+        ("transpose_1a2a", lambda: plan.transpose_1a2a(rad, s), 16.0 * plan.spectral_size),
         ("stp", lambda: plan.stp(s, p2), None),
         ("permute_2a3a", lambda: plan.reform_2a3a(p2, p3, nphi), 8.0 * (plan.hybrid_size + n3)),
         ("fft_c2r", lambda: ra.fft_to_physical(p3, nphi), 16.0 * n3),
         ("fft_r2c", lambda: ra.fft_to_spectral(p3, nphi), 16.0 * n3),
         ("permute_3b2b", lambda: plan.reform_3b2b(p3, p2, nphi), 8.0 * 2 * plan.hybrid_size),
         ("pts", lambda: plan.pts(p2, s2), None),
-        ("cheb_to_spectral", lambda: cheb.to_spectral(rad2, rad), 16.0 * nr * 2 * nlm * nf),
+        ("transpose_2b1b", lambda: plan.transpose_2b1b(s2, rad2), 16.0 * plan.spectral_size),
+        ("cheb_to_spectral", lambda: cheb.to_spectral(rad2, c2), nb_rad),
     ]
     for _ in range(warmup):
         for _, fn, _ in stages:
@@ -325,9 +412,13 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
             ev[k][i + 1].record()
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / steps
+    rt = float(((c2 - c0).abs().max() / c0.abs().max()).item())
+    log(f"stack {workload}: chained round trip {wall * 1e3:.3f} ms, returns its input to {rt:.2e}")
+    if not rt < 1e-11:
+        raise RuntimeError(f"stack {workload}: the chained round trip does not return its input ({rt:.3e})")
     F = flops_per_direction(L, nt, nr, nf)
     out = {"workload": workload, "l_max": L, "n_theta": nt, "n_phi": nphi, "n_r": nr, "n_fields": nf,
-           "ms_per_round_trip": wall * 1e3, "stages": {}}
+           "ms_per_round_trip": wall * 1e3, "round_trip_rel_err": rt, "stages": {}}
     for i, (name, _, nbytes) in enumerate(stages):
         ms = float(np.mean([e[i].elapsed_time(e[i + 1]) for e in ev]))
         rec = {"ms": ms}
@@ -337,7 +428,14 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
         else:
             rec["gbps"] = nbytes / (ms * 1e-3) / 1e9
             rec["frac_hbm_peak"] = rec["gbps"] / 8000.0
+        if name.startswith("cheb_"):
+            # the matrix form is FP64-MFMA work: N_r^2 flop per radial column (two parity
+            # halves of (N_r / 2)^2 multiply-adds)
+            flop = float(nr) * nr * (c0.numel() // nr)
+            rec["tflops"] = flop / (ms * 1e-3) / 1e12
+            rec["frac_fp64_peak"] = rec["tflops"] / PEAK_FP64_TFLOPS
         out["stages"][name] = rec
+    rad = c0  # the later figures run on the chain's own data: s (spectral), c0 (Chebyshev coefficients)
     # the one-rank radial transposes around the implicit solve (Transpose_2b1b after PtS,
     # Transpose_1a2a before StP): 16 B per spectral coefficient (read + write)
     p1 = torch.empty_like(s)

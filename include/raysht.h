@@ -45,7 +45,7 @@ enum raysht_status {
 /* Arithmetic used by the Legendre contractions. */
 enum raysht_precision {
     RAYSHT_FP64 = 0,    /* FP64 MFMA */
-    RAYSHT_BF16X3 = 1,  /* 3-slice BF16 split, FP32 accumulation (about 1e-6) */
+    RAYSHT_BF16X3 = 1,  /* 4-slice BF16 split, pairs of levels s + t <= 3, FP32 sums (about 1e-6) */
     RAYSHT_BF16X6 = 2,  /* 6-slice BF16 split, exact FP32 sums, FP64 recombination (about 1e-12) */
     RAYSHT_INT8X3 = 3,  /* the BF16X3 digits as signed bytes on INT8 MFMA, exact int32 sums */
     RAYSHT_INT8X6 = 4   /* the BF16X6 digits as signed bytes on INT8 MFMA, exact int32 sums */

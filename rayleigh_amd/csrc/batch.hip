@@ -20,8 +20,6 @@ struct raysht_batch {
     std::vector<raysht_plan *> plans;
     raysht::prob_desc *d_desc = nullptr;
     raysht::work_item *d_work[2][kSchedules] = {};  // [direction: 0 StP, 1 PtS][candidate]
-    raysht::work_item *d_slice = nullptr;           // split forms: the StP slicing pass's items
-    int n_slice = 0;
     int n_work[2][kSchedules] = {};
     int choice[2] = {0, 0};
 };
@@ -33,7 +31,6 @@ namespace {
 void batch_free(raysht_batch *b)
 {
     if (b->d_desc) (void)hipFree(b->d_desc);
-    if (b->d_slice) (void)hipFree(b->d_slice);
     for (auto &dir : b->d_work)
         for (work_item *w : dir)
             if (w) (void)hipFree(w);
@@ -55,7 +52,7 @@ int launch_dir(raysht_batch *b, int dir, int c, const batch_io &io, hipStream_t 
     const int n = b->n_work[dir][c];
     if (b->precision == RAYSHT_FP64)
         return dir == 0 ? launch_stp_fp64_batch(b->d_desc, io, w, n, st) : launch_pts_fp64_batch(b->d_desc, io, w, n, st);
-    return dir == 0 ? launch_stp_split_batch(b->precision, b->d_desc, io, b->d_slice, b->n_slice, w, n, st)
+    return dir == 0 ? launch_stp_split_batch(b->precision, b->d_desc, io, w, n, st)
                     : launch_pts_split_batch(b->precision, b->d_desc, io, w, n, st);
 }
 
@@ -108,11 +105,6 @@ int raysht_batch_create(raysht_batch **out, raysht_plan *const *plans, int n_pla
     for (int i = 0; i < n_plans; ++i) desc[i] = plans[i]->desc;
     const bool fp64 = b->precision == RAYSHT_FP64;
     int rc = upload(&b->d_desc, desc);
-    if (rc == RAYSHT_OK && !fp64) {
-        const std::vector<work_item> sl = split_slice_work(cp);
-        b->n_slice = (int)sl.size();
-        rc = upload(&b->d_slice, sl);
-    }
     for (int dir = 0; dir < 2 && rc == RAYSHT_OK; ++dir)
         for (int c = 0; c < kSchedules && rc == RAYSHT_OK; ++c) {
             const std::vector<work_item> w =

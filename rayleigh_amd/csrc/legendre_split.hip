@@ -19,7 +19,9 @@
 //     (l+1)^-2 spectrum, 7 x 6 ~5e-14.
 //   BF16X6, PtS: 6 x 6, s + t <= 5 (21 MFMAs), four FP32 sums (levels 0, 1, 2, >= 3);
 //     the folded hybrid data does not decay (~2e-14).
-//   BF16X3, both: 3 x 3, s + t <= 2 (6 MFMAs), one FP32 sum: ~1e-6.
+//   BF16X3, both: 4 x 4, s + t <= 3 (10 MFMAs), two FP32 sums (level 0, levels >= 1): ~5e-7
+//     of max|result| (3 x 3 with s + t <= 2 left 2e-5 - 4e-5 in StP on an (l+1)^-2 spectrum
+//     and 1.6e-6 in PtS on unscaled random data, INT8 digits).
 //
 // Tables are split once at plan creation into "fragment-major" planes, so that a wave's
 // B operand for one K step of 16 is a single coalesced 1 KiB load per slice.  The data
@@ -123,12 +125,16 @@ struct form {
 };
 using stp_x6 = form<7, 6, 6, 5>;
 using pts_x6 = form<6, 6, 5, 4>;
-using any_x3 = form<3, 3, 2, 1>;
+using stp_x3 = form<4, 4, 3, 2>;
+using pts_x3 = form<4, 4, 3, 2>;
 using stp_i6 = form<7, 6, 6, 7, true>;
 using pts_i6 = form<6, 6, 5, 6, true>;
-using any_i3 = form<3, 3, 2, 3, true>;
-static_assert(stp_x6::npairs() == 27 && pts_x6::npairs() == 21 && any_x3::npairs() == 6, "pair counts");
-static_assert(stp_i6::npairs() == 27 && pts_i6::npairs() == 21 && any_i3::npairs() == 6, "pair counts");
+using stp_i3 = form<4, 4, 3, 4, true>;
+using pts_i3 = form<4, 4, 3, 4, true>;
+static_assert(stp_x6::npairs() == 27 && pts_x6::npairs() == 21 && stp_x3::npairs() == 10 && pts_x3::npairs() == 10,
+              "pair counts");
+static_assert(stp_i6::npairs() == 27 && pts_i6::npairs() == 21 && stp_i3::npairs() == 10 && pts_i3::npairs() == 10,
+              "pair counts");
 
 bool is_i8(int precision) { return precision == RAYSHT_INT8X6 || precision == RAYSHT_INT8X3; }
 bool is_x6(int precision) { return precision == RAYSHT_BF16X6 || precision == RAYSHT_INT8X6; }
@@ -285,6 +291,25 @@ __global__ __launch_bounds__(64) void split_pts_table_kernel(int lmax, int nt2, 
     }
 }
 
+// floor(g / d) for 0 <= g < 2^16 and 1 <= d < 2^16 as one high multiply by r = ceil(2^32 / d):
+// r d = 2^32 + eps with 0 <= eps < d, so g r / 2^32 = g / d + g eps / (d 2^32) and the added
+// term stays below 1 / d (g eps < g d < 2^32), which keeps the floor exact (d = 1 apart)
+struct small_div {
+    unsigned r;
+    int d;
+    __device__ explicit small_div(int d_) : r(d_ > 1 ? (unsigned)((0x100000000ull + (unsigned)d_ - 1) / (unsigned)d_) : 0u), d(d_) {}
+    __device__ __forceinline__ int quo(int g) const { return d > 1 ? (int)__umulhi((unsigned)g, r) : g; }
+};
+
+// x, hidden from the optimiser: what is computed from it inside a loop stays inside the
+// loop (an epilogue that runs once per contraction inside the K loop keeps its addresses
+// to itself instead of holding them in registers through every step)
+__device__ __forceinline__ int opaque(int x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // --------------------------------------------------------------- contraction core
 // One MFMA of a slice pair: acc += A_s (data) x B_t (table) for one K step
 template <class F>
@@ -321,30 +346,44 @@ __device__ __forceinline__ void mfma_pairs_sliced(typename F::acc_t (&acc)[F::NG
     }
 }
 
-// adds the level sums into the FP64 sums and clears them (BF16 digits carry their
-// weights 2^(-7-8s); the INT8 forms' integer level L sums are scaled by 2^(-14-8L))
+// adds the level sums into the FP64 sums and clears them.  BF16 digits carry their weights
+// 2^(-7-8s), so the FP32 level sums are values; the INT8 forms' exact int32 level-L sums S_L
+// are worth S_L 2^(-14-8L).  INT8: levels L >= 4 (weight <= 2^-32 of level 0) are first
+// combined in FP32, which rounds at 2^-24 of their size, i.e. at 2^-56 of the operand scales
+// (the BF16 forms' shared FP32 sum of levels >= 4 rounds at the same place); levels 0-3 go to
+// FP64 exactly and every scaling is by an exact power of two.
 template <class F>
 __device__ __forceinline__ void flush(typename F::acc_t (&acc)[F::NG], double (&dacc)[16])
 {
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
-        double sum = 0.0;
+        if constexpr (F::I8) {
+            constexpr int NX = F::NG < 4 ? F::NG : 4;  // levels summed in FP64
+            double lo = (double)acc[NX - 1][v];       // in units of 2^(-14-8 (NX-1))
 #pragma unroll
-        for (int g = F::NG - 1; g >= 0; --g) {
-            if constexpr (F::I8)
-                sum += __builtin_ldexp((double)acc[g][v], -14 - 8 * g);
-            else
-                sum += (double)acc[g][v];
+            for (int g = NX - 2; g >= 0; --g) lo = __builtin_fma(lo, 0x1p-8, (double)acc[g][v]);
+            if constexpr (F::NG > 4) {
+                float hi = (float)acc[F::NG - 1][v];  // in units of 2^(-14-8 (NG-1))
+#pragma unroll
+                for (int g = F::NG - 2; g >= 4; --g) hi = __builtin_fmaf(hi, 0x1p-8f, (float)acc[g][v]);
+                lo = __builtin_fma((double)hi, 0x1p-32, lo);
+            }
+            dacc[v] += lo * 0x1p-14;
+        } else {
+            double sum = 0.0;
+#pragma unroll
+            for (int g = F::NG - 1; g >= 0; --g) sum += (double)acc[g][v];
+            dacc[v] += sum;
         }
-        dacc[v] += sum;
     }
 #pragma unroll
     for (int g = 0; g < F::NG; ++g) acc[g] = typename F::acc_t{};
 }
 
 // The data operand's NA digits, from one integer per value.  For |v| < 2^e,
-// X = round(v 2^(8 NA - 1 - e)) (|X| <= 2^(8 NA - 1)) is formed from two FP64 steps (floor of
-// v 2^(8 NA - 33 - e), then the rounded 32-bit remainder) and held with 128 added to each
+// X = round(v 2^(8 NA - 1 - e)) (|X| <= 2^(8 NA - 1)) is formed by the 1.5 2^52 rounding
+// constant for NA <= 6 and, for the seven-digit data operand, from two FP64 steps (floor of
+// v 2^(8 NA - 33 - e), then the rounded 32-bit remainder); it is held with 128 added to each
 // of its NA - 1 low bytes: byte NA-1-s (s = 1 .. NA-1) is then d_s + 128 and the arithmetic
 // top bits are d_0, so that v 2^-e = sum_s d_s 2^(-7-8s) + O(2^(-8 NA)) with every
 // |d_s| <= 128 (exact in BF16).  Each digit then costs one byte conversion and one FMA
@@ -353,11 +392,21 @@ template <int NA>
 __device__ __forceinline__ unsigned long long offset_digits(double v, int e)
 {
     static_assert(NA >= 2 && NA <= 7, "digits of one 64-bit integer");
-    const double t = __builtin_ldexp(v, 8 * NA - 33 - e);  // |t| < 2^(8 NA - 32)
-    const double h = __builtin_floor(t);
-    const double r = __builtin_fmin(__builtin_fma(t - h, 4294967296.0, 0.5), 4294967295.0);
-    const unsigned long long X = ((unsigned long long)(unsigned)(int)h << 32) | (unsigned)r;
-    return X + (0x0000808080808080ull >> (8 * (7 - NA)));
+    constexpr unsigned long long kOffset = 0x0000808080808080ull >> (8 * (7 - NA));
+    if constexpr (NA <= 6) {
+        // |X| <= 2^47 < 2^51: adding 1.5 2^52 rounds t to an integer in the low mantissa
+        // bits (ties to even), and the bit patterns of t + 1.5 2^52 and 1.5 2^52 differ by
+        // exactly that integer (one binade): two FP64 operations and one 64-bit add
+        const double t = __builtin_ldexp(v, 8 * NA - 1 - e);
+        const long long bits = __builtin_bit_cast(long long, t + 0x1.8p52);
+        return (unsigned long long)(bits - __builtin_bit_cast(long long, 0x1.8p52)) + kOffset;
+    } else {
+        const double t = __builtin_ldexp(v, 8 * NA - 33 - e);  // |t| < 2^(8 NA - 32)
+        const double h = __builtin_floor(t);
+        const double r = __builtin_fmin(__builtin_fma(t - h, 4294967296.0, 0.5), 4294967295.0);
+        const unsigned long long X = ((unsigned long long)(unsigned)(int)h << 32) | (unsigned)r;
+        return X + kOffset;
+    }
 }
 
 // digit s of an offset_digits<NA> value, times 2^(-7-8s), as FP32 (exact)
@@ -476,269 +525,93 @@ struct raw_step {
     }
 };
 
-// The K loop of one contraction: nks steps of F::KS.  fetch(ks) issues the loads of step
-// ks's raw data (steps past the end read masked zeros) and slice(raw, buffer) returns
-// the slicers that write its A fragments into an LDS buffer, one slice per step() call
-// (spread between the MFMAs of the step before); step ks's fragments are in buffer
-// ks % 2 before its MFMAs.  Raw data run two steps ahead of the MFMAs and table fragments (fragment
-// f0 + ks * NB of resource rt) one step, so their latency hides behind a step of
-// MFMAs.  Wave w multiplies A block ablk of the buffer by its table fragments; the
-// result is added into dacc (level sums moved there every kChunkSteps steps).
-// The step body has no branches (the last step slices and loads harmlessly past the
-// end), so one wave keeps its matrix pipe and its vector ALU busy at once.  nks is uniform
-// over the workgroup; every step ends with one barrier.
-template <class F, class Raw, class Fetch, class Slice>
-__device__ __forceinline__ void k_loop(const Fetch &fetch, const Slice &slice, int nks, __amdgpu_buffer_rsrc_t rt,
-                                       int f0, u32x4 (*abuf)[kABuf], int ablk, int lane, double (&dacc)[16])
+// The K loop of a stream of contractions that one workgroup runs back to back: global
+// steps g = 0 .. G - 1 of F::KS terms each.  fetch(g) issues the loads of step g's raw data
+// (steps past the end read masked zeros), slice(raw, g, buffer) returns the slicers that
+// write its A fragments into an LDS buffer, one slice per step() call (spread between the
+// MFMAs of the step before), and bfrag(g) is the first table fragment of step g (resource
+// rt).  close(g) says what ends with step g: 0 nothing, 1 a run of FP32 level sums (BF16
+// forms: they move into FP64), 2 a contraction (the sums move into FP64 and done(g, dacc)
+// receives its FP64 sums).  Raw data run two steps ahead of the MFMAs and table fragments
+// one step, across the ends of contractions, so only the first step of the stream waits for
+// memory.  The caller issues that step's loads (r0, r1: raw data of steps 0 and 1, b0: table
+// fragments of step 0) before its exponent pass, so that their latency overlaps the pass,
+// and has the exponents in LDS (after a barrier) before the call.  Wave w multiplies A
+// block ablk of the buffer by its table fragments.  The step body has no branches but the
+// uniform close test (the last steps slice and load harmlessly past the end), so one wave
+// keeps its matrix pipe and its vector ALU busy at once; every step ends with one barrier.
+template <class F, class Raw, class Fetch, class Slice, class BFrag, class Close, class Done>
+__device__ __forceinline__ void k_stream(int G, const Fetch &fetch, const Slice &slice, const BFrag &bfrag,
+                                         const Close &close, const Done &done, __amdgpu_buffer_rsrc_t rt,
+                                         const Raw &r0, const Raw &r1, const u32x4 (&b0)[F::NB],
+                                         u32x4 (*abuf)[kABuf], int ablk, int lane)
 {
-    if (nks <= 0) return;
+    if (G <= 0) return;
     {
-        auto sl = slice(fetch(0), abuf[0]);
+        auto sl = slice(r0, 0, abuf[0]);
 #pragma unroll
         for (int sc = 0; sc < F::NA; ++sc) sl.step(sc);
     }
     // two register sets for the raw data and the table fragments, swapped by unrolling
     // the step loop by two (no copies between steps)
-    Raw rA = fetch(1), rB;
+    Raw rA = r1, rB;
     u32x4 bA[F::NB], bB[F::NB];
-    get_b<F::NB>(rt, f0, lane, bA);
+#pragma unroll
+    for (int t = 0; t < F::NB; ++t) bA[t] = b0[t];
     __syncthreads();
     typename F::acc_t acc[F::NG];
 #pragma unroll
     for (int g = 0; g < F::NG; ++g) acc[g] = typename F::acc_t{};
-    // step ks: raw data of step ks + 1 in rc, table fragments of step ks in bc; loads
-    // step ks + 2's raw data into rn and step ks + 1's fragments into bn
-    auto step = [&](int ks, const Raw &rc, Raw &rn, const u32x4 (&bc)[F::NB], u32x4 (&bn)[F::NB]) {
-        rn = fetch(ks + 2);
-        get_b<F::NB>(rt, f0 + (ks + 1) * F::NB, lane, bn);
+    // FP64 sums of the BF16 forms' runs of level sums (the INT8 forms' int32 sums are exact
+    // over a whole contraction, so they go straight to done())
+    double dacc[F::I8 ? 1 : 16];
+#pragma unroll
+    for (int v = 0; v < (F::I8 ? 1 : 16); ++v) dacc[v] = 0.0;
+    // step g: raw data of step g + 1 in rc, table fragments of step g in bc; loads step
+    // g + 2's raw data into rn and step g + 1's fragments into bn
+    auto step = [&](int g, const Raw &rc, Raw &rn, const u32x4 (&bc)[F::NB], u32x4 (&bn)[F::NB]) {
+        rn = fetch(g + 2);
+        get_b<F::NB>(rt, bfrag(g + 1), lane, bn);
         u32x4 a[F::NA];
-        get_a<F::NA>(abuf[ks & 1], ablk, lane, a);
-        auto sl = slice(rc, abuf[(ks + 1) & 1]);
+        get_a<F::NA>(abuf[g & 1], ablk, lane, a);
+        auto sl = slice(rc, g + 1, abuf[(g + 1) & 1]);
         mfma_pairs_sliced<F>(acc, a, bc, sl);
         __syncthreads();
-    };
-    auto steps = [&](int kc, int kend) {
-        int ks = kc;
-        for (; ks + 1 < kend; ks += 2) {
-            step(ks, rA, rB, bA, bB);
-            step(ks + 1, rB, rA, bB, bA);
-        }
-        if (ks < kend) {  // odd step count: only in the last chunk (kChunkSteps is even)
-            step(ks, rA, rB, bA, bB);
-        }
-    };
-    if constexpr (F::I8) {
-        // int32 level sums are exact over the whole contraction (kMaxI8K), one move into FP64
-        steps(0, nks);
-        flush<F>(acc, dacc);
-    } else {
-        for (int kc = 0; kc < nks; kc += kChunkSteps) {
-            steps(kc, min(nks, kc + kChunkSteps));
-            flush<F>(acc, dacc);
-        }
-    }
-}
-static_assert(kChunkSteps % 2 == 0, "chunks hold whole step pairs");
-
-// The MFMAs of one K step for a wave's 32 x 32 block, every kept pair in level order
-template <class F, int Q = 0>
-__device__ __forceinline__ void mfma_pairs(typename F::acc_t (&acc)[F::NG], const u32x4 (&a)[F::NA],
-                                           const u32x4 (&b)[F::NB])
-{
-    if constexpr (Q < F::npairs()) {
-        constexpr int s = F::pair(Q, true), t = F::pair(Q, false), g = F::group(Q);
-        mfma_pair<F>(acc[g], a[s], b[t]);
-        mfma_pairs<F, Q + 1>(acc, a, b);
-    }
-}
-
-// The K loop of a contraction whose two operands are both digit fragments in memory: A
-// fragments fa0 + ks NA .. (resource ra), B fragments fb0 + ks NB .. (resource rt), both
-// one step ahead in registers (two sets, swapped by unrolling the step loop by two); no
-// LDS and no barrier, so each wave runs on its own.  The last prefetch reads one step
-// past the wave's fragments (inside the resource or as zeros) and is not used.
-template <class F>
-__device__ __forceinline__ void direct_k_loop(__amdgpu_buffer_rsrc_t ra, int fa0, __amdgpu_buffer_rsrc_t rt, int fb0,
-                                              int nks, int lane, double (&dacc)[16])
-{
-    if (nks <= 0) return;
-    u32x4 aA[F::NA], aB[F::NA], bA[F::NB], bB[F::NB];
-    get_b<F::NA>(ra, fa0, lane, aA);
-    get_b<F::NB>(rt, fb0, lane, bA);
-    typename F::acc_t acc[F::NG];
+        const int c = close(g);
+        if constexpr (F::I8) {
+            if (c == 2) {
+                double sums[16];
 #pragma unroll
-    for (int g = 0; g < F::NG; ++g) acc[g] = typename F::acc_t{};
-    auto step = [&](int ks, const u32x4 (&ac)[F::NA], u32x4 (&an)[F::NA], const u32x4 (&bc)[F::NB],
-                    u32x4 (&bn)[F::NB]) {
-        get_b<F::NA>(ra, fa0 + (ks + 1) * F::NA, lane, an);
-        get_b<F::NB>(rt, fb0 + (ks + 1) * F::NB, lane, bn);
-        mfma_pairs<F>(acc, ac, bc);
-    };
-    auto steps = [&](int kc, int kend) {
-        int ks = kc;
-        for (; ks + 1 < kend; ks += 2) {
-            step(ks, aA, aB, bA, bB);
-            step(ks + 1, aB, aA, bB, bA);
-        }
-        if (ks < kend) step(ks, aA, aB, bA, bB);  // odd step count: only in the last chunk
-    };
-    if constexpr (F::I8) {
-        steps(0, nks);
-        flush<F>(acc, dacc);
-    } else {
-        for (int kc = 0; kc < nks; kc += kChunkSteps) {
-            steps(kc, min(nks, kc + kChunkSteps));
-            flush<F>(acc, dacc);
-        }
-    }
-}
-
-// --------------------------------------------------------------- StP data slicing
-// One workgroup of 256 threads per (problem, m-block, 32-column block): the exponent of
-// each column (from all its l), then every digit fragment of the block in both parities.
-// Fragment (cb nks + ks) NA + s of a parity's region (even region first, ncb nks_e NA
-// fragments): lane L, element e -> column 32 cb + L%32, parity-k = KS ks + EPL (L/32) + e,
-// i.e. l = 2 k + par; values past the m-block or the last column are zeros.  The digits are
-// the ones the contraction's A operand needs (offset_digits / pack_*), computed once per
-// column instead of once per row tile.
-template <class F>
-__global__ __launch_bounds__(256) void stp_slice_kernel(const prob_desc *__restrict__ probs, const batch_io io,
-                                                        const work_item *__restrict__ work)
-{
-    __shared__ int cexp[32];
-    const int mp = uniform(work[blockIdx.x].mp);
-    if (mp < 0) return;
-    const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
-    const prob_desc &P = probs[pi];
-    const int nrhs = uniform(P.nrhs);
-    const int m = uniform(uniform(P.mvals)[mp]), nl = uniform(P.lmax) - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
-    const double *sb = uniform(io.in[pi]) + uniform(uniform(P.s_off)[mp]);
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(sb, (long)nl * nrhs * 8);
-    const int tid = threadIdx.x;
-    {
-        const int cc = tid >> 3, q = tid & 7, c = c0 + cc;
-        const unsigned off = (unsigned)min(c, nrhs) * nl * 8;  // column nrhs and beyond read as 0
-        double mx = 0.0;
-        for (int l = q; l < nl; l += 8)
-            mx = fmax(mx, fabs(__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)(off + 8u * l), 0,
-                                                                                             0))));
-#pragma unroll
-        for (int o = 1; o < 8; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-        if (q == 0) {
-            const int e = scale_exponent(mx, F::I8);
-            cexp[cc] = e;
-            if (c < nrhs) uniform(P.scexp)[(long)mp * nrhs + c] = e;
-        }
-    }
-    __syncthreads();
-    const int lane = tid & 63, grp = tid >> 6;
-    const int cc = lane & 31, h = lane >> 5, c = c0 + cc;
-    const int e = cexp[cc];
-    const bool cin = c < nrhs;
-    const unsigned off = (unsigned)min(c, nrhs) * nl * 8;
-    const int nks_e = cdiv(ne, F::KS), nks_o = cdiv(no, F::KS), cb = c0 >> 5, ncb = cdiv(nrhs, 32);
-    uint4 *dig = uniform(P.sdig) + (uniform(uniform(P.sdig_off)[mp]) << 6);
-#pragma unroll
-    for (int par = 0; par < 2; ++par) {
-        const int nks = par ? nks_o : nks_e;
-        const long region = (par ? (long)ncb * nks_e * F::NA : 0) + (long)cb * nks * F::NA;
-        for (int ks = grp; ks < nks; ks += 4) {
-            unsigned long long X[F::EPL];
-#pragma unroll
-            for (int i = 0; i < F::EPL; ++i) {
-                const int l = 2 * (F::KS * ks + F::EPL * h + i) + par;
-                const double v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
-                                                                rs, (int)(off + 8u * (unsigned)l), 0, 0));
-                X[i] = offset_digits<F::NA>(cin && l < nl ? v : 0.0, e);
+                for (int v = 0; v < 16; ++v) sums[v] = 0.0;
+                flush<F>(acc, sums);
+                done(g, sums);
             }
+        } else if (c != 0) {
+            flush<F>(acc, dacc);
+            if (c == 2) {
+                done(g, dacc);
 #pragma unroll
-            for (int sd = 0; sd < F::NA; ++sd) {
-                unsigned w[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if constexpr (F::I8) {
-                        const unsigned long long x4[4] = {X[4 * q], X[4 * q + 1], X[4 * q + 2], X[4 * q + 3]};
-                        w[q] = pack_i8<F::NA>(x4, sd);
-                    } else {
-                        w[q] = pack_bf16(digit<F::NA>(X[2 * q], sd), digit<F::NA>(X[2 * q + 1], sd));
-                    }
-                }
-                dig[((region + (long)ks * F::NA + sd) << 6) + lane] = uint4{w[0], w[1], w[2], w[3]};
+                for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
             }
         }
+    };
+    int g = 0;
+    for (; g + 1 < G; g += 2) {
+        step(g, rA, rB, bA, bB);
+        step(g + 1, rB, rA, bB, bA);
     }
+    if (g < G) step(g, rA, rB, bA, bB);
 }
 
 // --------------------------------------------------------------- StP
-// Work item (mp, r0, c0): kTileC columns, kTileR theta rows from r0, both parities, of a
-// plan whose data digits the slicing pass has written.  Wave w multiplies column block
-// c0/32 by theta block r0/32 + w: both operands come straight from the digit fragments
-// in memory (the data fragments are shared by the item's waves and by the other row
-// tiles of the column tile through the caches), so the waves need no LDS and no barrier
-// in the K loop; the scaled even-parity sums wait in LDS (one slot per thread) for the
-// odd pass, which writes E + O and E - O.
-template <class F>
-__global__ __launch_bounds__(kThreads, kMinWaves) void stp_split2_kernel(
-    const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
-{
-    static_assert(kCB == 1, "one 32-column block per StP item");
-    __shared__ double ebuf[16][kThreads];
-    const int mp = uniform(work[blockIdx.x].mp);
-    if (mp < 0) return;
-    const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
-    const int r0 = uniform(work[blockIdx.x].r0);
-    const prob_desc &P = probs[pi];
-    const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
-    const long *frag_off = uniform(P.frag_stp_off);
-    const int *texp = uniform(P.texp_stp);
-    const int m = uniform(uniform(P.mvals)[mp]), nl = uniform(P.lmax) - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const long f_lo = uniform(frag_off[mp]), f_hi = uniform(frag_off[mp + 1]);
-    const __amdgpu_buffer_rsrc_t rt =
-        make_frag_rsrc(reinterpret_cast<const u32x4 *>(uniform(P.tdig_stp)) + (f_lo << 6), f_hi - f_lo);
-    const long d_lo = uniform(uniform(P.sdig_off)[mp]), d_hi = uniform(uniform(P.sdig_off)[mp + 1]);
-    const __amdgpu_buffer_rsrc_t ra =
-        make_frag_rsrc(reinterpret_cast<const u32x4 *>(uniform(P.sdig)) + (d_lo << 6), d_hi - d_lo);
-    const int *cexp = uniform(P.scexp) + (long)mp * nrhs;
-    const int njb = cdiv(nt2, 32), nks_e = cdiv(ne, F::KS), nks_o = cdiv(no, F::KS);
-    const int cb = c0 >> 5, ncb = cdiv(nrhs, 32);
-    double *ob = uniform(io.out[pi]) + (long)mp * nrhs * nt;
-    const int jb = r0 / 32 + wave;
-    if (jb >= njb) return;  // a row block past the last theta (no barrier follows)
-    const int j = jb * 32 + (lane & 31);
-    const bool jin = j < nt2;
-#pragma unroll
-    for (int par = 0; par < 2; ++par) {
-        double dacc[16];
-#pragma unroll
-        for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
-        const int nks = par ? nks_o : nks_e;
-        const int fb0 = (par ? njb * nks_e * F::NB : 0) + jb * nks * F::NB;
-        const int fa0 = (par ? ncb * nks_e * F::NA : 0) + cb * nks * F::NA;
-        direct_k_loop<F>(ra, fa0, rt, fb0, nks, lane, dacc);
-        const int ej = jin ? texp[((long)mp * 2 + par) * nt2 + j] : 0;
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int cc = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = c0 + cc;
-            const double r = __builtin_ldexp(dacc[v], (c < nrhs ? cexp[c] : 0) + ej);
-            if (par == 0) {
-                ebuf[v][tid] = r;
-            } else if (jin && c < nrhs) {
-                const double e = ebuf[v][tid];
-                ob[(long)c * nt + j] = e + r;
-                ob[(long)c * nt + (nt - 1 - j)] = e - r;
-            }
-        }
-    }
-}
-
-// --------------------------------------------------------------- StP (sliced in the kernel)
-// Work item (mp, r0, c0): kTileC columns, kTileR theta rows from r0, both parities.
-// The row tiles of one column tile are consecutive items, so that they run together
-// and share the column tile's spectral data in L2.  Wave w multiplies column block w % kCB by theta block
-// j0/32 + w/kCB.  The spectral tile is read as 16-byte pairs (l, l+1); each parity pass
-// slices its half, VPT values per thread and step.  The column exponents come from one
-// pass over the tile; the scaled even-parity sums wait in LDS for the odd pass.
+// Work item (mp, c0): kTileC columns of one m-block, every theta row, both parities: the
+// row tiles (kTileR theta rows each) run one after the other as one stream of contractions
+// (row tile, parity), so the column exponents are found once per column tile and the K
+// loop never drains between contractions.  Wave w multiplies column block w % kCB by theta
+// block kJB rt + w / kCB of row tile rt.  The spectral tile is read as 16-byte pairs (l,
+// l+1); each parity's steps slice their half, VPT values per thread and step (the pairs
+// of a step come from L2 the second time).  The scaled even-parity sums of a row tile wait
+// in LDS for its odd contraction, which writes E + O and E - O.
 template <class F>
 __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
     const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
@@ -749,7 +622,6 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
     const int mp = uniform(work[blockIdx.x].mp);
     if (mp < 0) return;
     const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
-    const int r0 = uniform(work[blockIdx.x].r0);
     const prob_desc &P = probs[pi];
     const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
     const long *frag_off = uniform(P.frag_stp_off);
@@ -761,21 +633,62 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(sb, (long)nl * nrhs * 8);
     const __amdgpu_buffer_rsrc_t rt =
         make_frag_rsrc(reinterpret_cast<const u32x4 *>(uniform(P.tdig_stp)) + (f_lo << 6), f_hi - f_lo);
+    const int cb = wave % kCB, njb = cdiv(nt2, 32);
+    // steps of the even and odd contractions (an odd contraction of an m-block without odd
+    // l, nl = 1, multiplies zeros: its table region is empty and its data masked)
+    const int nks_e = cdiv(ne, F::KS), nks_o = max(1, cdiv(no, F::KS)), nseg = nks_e + nks_o;
+    const int G = cdiv(njb, kJB) * nseg;
+    const small_div by_seg(nseg);
+    struct pos {
+        int rt, par, ks;
+    };
+    auto at = [&](int g) {
+        pos p;
+        p.rt = by_seg.quo(g);
+        const int r = g - p.rt * nseg;
+        p.par = r >= nks_e;
+        p.ks = p.par ? r - nks_e : r;
+        return p;
+    };
 
     // producer role of the thread: column ca, l = 32 i + 4 q .. +3 of step i
     const int ca = tid >> 3, q = tid & 7;
     const unsigned a_off = (unsigned)min(c0 + ca, nrhs) * nl * 8;  // column nrhs and beyond read as 0
+    using Raw = raw_step<F::VPT>;
+    // step g: parity-k = KS ks + VPT q + i (i < VPT), i.e. l = 2 KS ks + 2 VPT q + 2 i + par,
+    // read as the pairs (l, l + 1) of both parities (l >= nl is the next column or past
+    // the block: masked to zero)
+    auto fetch = [&](int g) {
+        const int l0 = 2 * F::KS * at(g).ks + 2 * F::VPT * q;
+        Raw r;
+        r.valid = 0;
+#pragma unroll
+        for (int i = 0; i < F::VPT; ++i) {
+            r.d[i] = load_d2(rs, a_off + (unsigned)(l0 + 2 * i) * 8);
+            r.valid |= (unsigned)(l0 + 2 * i < nl) << (2 * i) | (unsigned)(l0 + 2 * i + 1 < nl) << (2 * i + 1);
+        }
+        return r;
+    };
+    auto bfrag = [&](int g) {
+        const pos p = at(g);
+        const int jb = p.rt * kJB + wave / kCB;
+        return (p.par ? njb * nks_e * F::NB : 0) + (jb * (p.par ? nks_o : nks_e) + p.ks) * F::NB;
+    };
+    const Raw r0 = fetch(0), r1 = fetch(1);
+    u32x4 b0[F::NB];
+    get_b<F::NB>(rt, bfrag(0), lane, b0);
     {
+        // the column exponents: sixteen 16-byte loads in flight per thread
         double mx = 0.0;
         const int n32 = cdiv(nl, 32);
-        for (int i0 = 0; i0 < n32; i0 += 4) {
-            double2 d[4][2];
+        for (int i0 = 0; i0 < n32; i0 += 8) {
+            double2 d[8][2];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 8; ++u)
 #pragma unroll
                 for (int e = 0; e < 2; ++e) d[u][e] = load_d2(rs, a_off + (unsigned)(32 * (i0 + u) + 4 * q + 2 * e) * 8);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 8; ++u)
 #pragma unroll
                 for (int e = 0; e < 2; ++e) {
                     const int l = 32 * (i0 + u) + 4 * q + 2 * e;
@@ -789,72 +702,54 @@ __global__ __launch_bounds__(kThreads, kMinWaves) void stp_split_kernel(
     }
     __syncthreads();
     const int ea = cexp[ca];
-    const int cb = wave % kCB, njb = cdiv(nt2, 32), nks_e = cdiv(ne, F::KS);
-    using Raw = raw_step<F::VPT>;
     double *ob = uniform(io.out[pi]) + (long)mp * nrhs * nt;
-
-    {
-        const int jb = r0 / 32 + wave / kCB;
-        const bool active = jb < njb;
-        const int j = jb * 32 + (lane & 31);
+    auto slice = [&](const Raw &r, int g, u32x4 *buf) {
+        const int par = at(g).par;
+        double v[F::VPT];
 #pragma unroll
-        for (int par = 0; par < 2; ++par) {
-            double dacc[16];
+        for (int i = 0; i < F::VPT; ++i) v[i] = r.val(i, par);
+        slicers<F, 1> sl;
+        sl.sl[0].init(buf, ca >> 5, ca, F::VPT * q, v, ea);
+        return sl;
+    };
+    auto close = [&](int g) {
+        const pos p = at(g);
+        if (p.ks == (p.par ? nks_o : nks_e) - 1) return 2;
+        return (!F::I8 && (p.ks + 1) % kChunkSteps == 0) ? 1 : 0;
+    };
+    auto done = [&](int g, const double (&dacc)[16]) {
+        const pos p = at(g);
+        const int ln = opaque(lane);
+        const int jb = p.rt * kJB + wave / kCB, j = jb * 32 + (ln & 31);
+        const bool store = jb < njb && j < nt2;
+        const int ej = store ? texp[((long)mp * 2 + p.par) * nt2 + j] : 0;
 #pragma unroll
-            for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
-            const int K = par ? no : ne, nks = cdiv(K, F::KS);
-            const int f0 = (par ? njb * nks_e * F::NB : 0) + jb * nks * F::NB;
-            // step ks: parity-k = KS ks + VPT q + i (i < VPT), i.e. l = 2 KS ks + 2 VPT q + 2 i + par,
-            // read as the pairs (l, l + 1) of both parities (l >= nl is the next column or
-            // past the block: masked to zero)
-            auto fetch = [&](int ks) {
-                const int l0 = 2 * F::KS * ks + 2 * F::VPT * q;
-                Raw r;
-                r.valid = 0;
-#pragma unroll
-                for (int i = 0; i < F::VPT; ++i) {
-                    r.d[i] = load_d2(rs, a_off + (unsigned)(l0 + 2 * i) * 8);
-                    r.valid |= (unsigned)(l0 + 2 * i < nl) << (2 * i) | (unsigned)(l0 + 2 * i + 1 < nl) << (2 * i + 1);
-                }
-                return r;
-            };
-            auto slice = [&](const Raw &r, u32x4 *buf) {
-                double v[F::VPT];
-#pragma unroll
-                for (int i = 0; i < F::VPT; ++i) v[i] = r.val(i, par);
-                slicers<F, 1> sl;
-                sl.sl[0].init(buf, ca >> 5, ca, F::VPT * q, v, ea);
-                return sl;
-            };
-            k_loop<F, Raw>(fetch, slice, nks, rt, f0, abuf, cb, lane, dacc);
-            const bool store = active && j < nt2;
-            const int ej = store ? texp[((long)mp * 2 + par) * nt2 + j] : 0;
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = c0 + cc;
-                const double r = __builtin_ldexp(dacc[v], cexp[cc] + ej);
-                if (par == 0) {
-                    ebuf[v][tid] = r;
-                } else if (store && c < nrhs) {
-                    const double e = ebuf[v][tid];
-                    ob[(long)c * nt + j] = e + r;
-                    ob[(long)c * nt + (nt - 1 - j)] = e - r;
-                }
+        for (int v = 0; v < 16; ++v) {
+            const int cc = cb * 32 + 8 * (v >> 2) + 4 * (ln >> 5) + (v & 3), c = c0 + cc;
+            const double r = __builtin_ldexp(dacc[v], cexp[cc] + ej);
+            if (p.par == 0) {
+                ebuf[v][tid] = r;
+            } else if (store && c < nrhs) {
+                const double e = ebuf[v][tid];
+                ob[(long)c * nt + j] = e + r;
+                ob[(long)c * nt + (nt - 1 - j)] = e - r;
             }
         }
-    }
+    };
+    k_stream<F, Raw>(G, fetch, slice, bfrag, close, done, rt, r0, r1, b0, abuf, cb, lane);
 }
 
 // --------------------------------------------------------------- PtS
-// Work item (mp, k0, c0): kTileC columns, kTileRP parity-l rows from k0 of both
-// parities (the row tiles of one column tile are consecutive items and share its hybrid
-// data in L2).  Wave w multiplies column block w % kCB of parity w / (kCB kKB) by
-// row block k0/32 + (w / kCB) % kKB.  The data operand is the fold f(j) + f(nt-1-j) (even) or
-// f(j) - f(nt-1-j) (odd) of the hybrid columns; each thread slices both folds of VPT
-// nodes per step, and both folds' column exponents come from one pass over the tile.
-// (INT8 forms: registers for three waves per SIMD, measured 7 % faster than two)
+// Work item (mp, c0): kTileC columns of one m-block, every parity-l row: the row tiles
+// (kTileRP parity-l rows of each parity) run one after the other as one stream, so the
+// fold's column exponents are found once per column tile.  Wave w multiplies column block
+// w % kCB of parity w / (kCB kKB) by row block kKB rt + (w / kCB) % kKB of row tile rt.  The
+// data operand is the fold f(j) + f(nt-1-j) (even) or f(j) - f(nt-1-j) (odd) of the hybrid
+// columns; each thread slices both folds of VPT nodes per step, and both folds' column
+// exponents come from one pass over the tile.
+// (INT8X3: registers for three waves per SIMD; INT8X6 spills at three)
 template <class F>
-__global__ __launch_bounds__(kThreads, F::I8 ? 3 : kMinWaves) void pts_split_kernel(
+__global__ __launch_bounds__(kThreads, F::I8 && F::NA < 6 ? 3 : kMinWaves) void pts_split_kernel(
     const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
 {
     __shared__ u32x4 abuf[2][kABuf];
@@ -862,7 +757,6 @@ __global__ __launch_bounds__(kThreads, F::I8 ? 3 : kMinWaves) void pts_split_ker
     const int mp = uniform(work[blockIdx.x].mp);
     if (mp < 0) return;
     const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
-    const int k0 = uniform(work[blockIdx.x].r0);
     const prob_desc &P = probs[pi];
     const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
     const long *frag_off = uniform(P.frag_pts_off);
@@ -873,15 +767,22 @@ __global__ __launch_bounds__(kThreads, F::I8 ? 3 : kMinWaves) void pts_split_ker
     const __amdgpu_buffer_rsrc_t rp = make_rsrc(pb, (long)nrhs * nt * 8);
     const int ca = tid >> 3, q = tid & 7;
     const unsigned a_off = (unsigned)min(c0 + ca, nrhs) * nt * 8;
-    const int njs = cdiv(nt2, F::KS);
+    const int njs = cdiv(nt2, F::KS), G = cdiv(ne, kTileRP) * njs;
+    const small_div by_njs(njs);
     constexpr int H = F::VPT / 2;
     using Raw = raw_step<F::VPT>;
+    const int par = wave / (kCB * kKB), cb = wave % kCB, K_rows = par ? no : ne;
+    const int region = par ? cdiv(ne, 32) * njs * F::NB : 0;
+    const long f_lo = uniform(frag_off[mp]), f_hi = uniform(frag_off[mp + 1]);
+    const __amdgpu_buffer_rsrc_t rt =
+        make_frag_rsrc(reinterpret_cast<const u32x4 *>(uniform(P.tdig_pts)) + (f_lo << 6), f_hi - f_lo);
 
-    // raw data of step js: nodes j = j0 + i, j0 = KS js + VPT q, i < VPT, as the pairs
-    // d[u] = (f(j0 + 2u), f(j0 + 2u + 1)) and their mirrors d[H + u] = (f(nt-1-j0-2u),
-    // f(nt-2-j0-2u)); zero for j >= nt2 (the mirrored offset is negative only past the
-    // last valid j, where it wraps past the end or into a column whose values are masked)
-    auto fetch = [&](int js) {
+    // raw data of step g (node step js = g % njs): nodes j = j0 + i, j0 = KS js + VPT q, i <
+    // VPT, as the pairs d[u] = (f(j0 + 2u), f(j0 + 2u + 1)) and their mirrors d[H + u] =
+    // (f(nt-1-j0-2u), f(nt-2-j0-2u)); zero for j >= nt2 (the mirrored offset is negative
+    // only past the last valid j, where it wraps past the end or into a column whose values
+    // are masked)
+    auto fetch_js = [&](int js) {
         const int j0 = F::KS * js + F::VPT * q;
         Raw r;
         r.valid = 0;
@@ -896,14 +797,24 @@ __global__ __launch_bounds__(kThreads, F::I8 ? 3 : kMinWaves) void pts_split_ker
         }
         return r;
     };
+    auto fetch = [&](int g) { return fetch_js(g - by_njs.quo(g) * njs); };
+    auto bfrag = [&](int g) {
+        const int rt_ = by_njs.quo(g), js = g - rt_ * njs;
+        return region + ((kKB * rt_ + (wave / kCB) % kKB) * njs + js) * F::NB;
+    };
+    const Raw r0 = fetch(0), r1 = fetch(1);
+    u32x4 b0[F::NB];
+    get_b<F::NB>(rt, bfrag(0), lane, b0);
     {
+        // both folds' column exponents: sixteen 16-byte loads in flight per thread
+        constexpr int EB = 8 / H;
         double me = 0.0, mo = 0.0;
-        for (int j0 = 0; j0 < njs; j0 += 4) {
-            Raw r[4];
+        for (int j0 = 0; j0 < njs; j0 += EB) {
+            Raw r[EB];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) r[u] = fetch(j0 + u);  // steps past njs are masked
+            for (int u = 0; u < EB; ++u) r[u] = fetch_js(j0 + u);  // steps past njs are masked
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < EB; ++u)
 #pragma unroll
                 for (int i = 0; i < F::VPT; ++i) {
                     const double a = r[u].val(i >> 1, i & 1), b = r[u].val(H + (i >> 1), i & 1);
@@ -923,15 +834,10 @@ __global__ __launch_bounds__(kThreads, F::I8 ? 3 : kMinWaves) void pts_split_ker
     }
     __syncthreads();
     const int ee = cexp[0][ca], eo = cexp[1][ca];
-    const int par = wave / (kCB * kKB), cb = wave % kCB, K_rows = par ? no : ne;
-    const int region = par ? cdiv(ne, 32) * njs * F::NB : 0;
-    const long f_lo = uniform(frag_off[mp]), f_hi = uniform(frag_off[mp + 1]);
-    const __amdgpu_buffer_rsrc_t rt =
-        make_frag_rsrc(reinterpret_cast<const u32x4 *>(uniform(P.tdig_pts)) + (f_lo << 6), f_hi - f_lo);
     double *sb = uniform(io.out[pi]) + uniform(uniform(P.s_off)[mp]);
     const long e_base = uniform(uniform(P.exp_pts_off)[mp]);
     // A blocks of a buffer: [parity][column block]
-    auto slice = [&](const Raw &r, u32x4 *buf) {
+    auto slice = [&](const Raw &r, int, u32x4 *buf) {
         double ve[F::VPT], vo[F::VPT];
 #pragma unroll
         for (int i = 0; i < F::VPT; ++i) {
@@ -944,34 +850,392 @@ __global__ __launch_bounds__(kThreads, F::I8 ? 3 : kMinWaves) void pts_split_ker
         sl.sl[1].init(buf, kCB + (ca >> 5), ca, F::VPT * q, vo, eo);
         return sl;
     };
-
-    {
-        const int kb = k0 / 32 + (wave / kCB) % kKB;
-        const bool active = kb * 32 < K_rows;
-        double dacc[16];
-#pragma unroll
-        for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
-        k_loop<F, Raw>(fetch, slice, njs, rt, region + kb * njs * F::NB, abuf, kCB * par + cb, lane, dacc);
-
-        const int k = kb * 32 + (lane & 31);
-        if (active && k < K_rows) {
+    auto close = [&](int g) {
+        const int js = g - by_njs.quo(g) * njs;
+        if (js == njs - 1) return 2;
+        return (!F::I8 && (js + 1) % kChunkSteps == 0) ? 1 : 0;
+    };
+    auto done = [&](int g, const double (&dacc)[16]) {
+        const int ln = opaque(lane);
+        const int kb = kKB * by_njs.quo(g) + (wave / kCB) % kKB, k = kb * 32 + (ln & 31);
+        if (k < K_rows) {
             const int ek = texp[e_base + (par ? ne : 0) + k];
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int cc = cb * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), c = c0 + cc;
+                const int cc = cb * 32 + 8 * (v >> 2) + 4 * (ln >> 5) + (v & 3), c = c0 + cc;
                 if (c < nrhs) sb[(long)c * nl + 2 * k + par] = __builtin_ldexp(dacc[v], cexp[par][cc] + ek);
             }
         }
+    };
+    k_stream<F, Raw>(G, fetch, slice, bfrag, close, done, rt, r0, r1, b0, abuf, kCB * par + cb, lane);
+}
+
+// --------------------------------------------------------------- wave-specialised forms
+// A 512-thread workgroup of two roles.  Waves 0-3 (consumers) only multiply: each streams
+// table fragments from memory one step ahead, reads the data's A fragments from LDS and
+// issues its MFMAs, and at the end of each contraction moves its level sums into FP64 and
+// stores them.  Waves 4-7 (producers) only slice: they load the raw data two steps ahead,
+// find the column exponents and write the A fragments of step g + 1 into a three-slot LDS
+// ring while the consumers multiply step g.  One wave of each role shares a SIMD, so the
+// digit arithmetic runs on the vector ALU while the other wave keeps the matrix pipe busy
+// (in the one-role kernels both waves of a SIMD interleave the two, and the vector work,
+// about ten instructions per MFMA, left the matrix pipe idle most of the time).  Both roles
+// pass the same barriers: two before the stream, one per step.
+constexpr int kWsThreads = 512;
+constexpr int kWsRing = 3;
+constexpr int kWsABuf = 2 * 7 * 64;  // A fragments of one step: up to two blocks of seven digits
+
+// The MFMAs of one K step for a wave's 32 x 32 block, every kept pair in level order
+template <class F, int Q = 0>
+__device__ __forceinline__ void mfma_pairs(typename F::acc_t (&acc)[F::NG], const u32x4 (&a)[F::NA],
+                                           const u32x4 (&b)[F::NB])
+{
+    if constexpr (Q < F::npairs()) {
+        constexpr int s = F::pair(Q, true), t = F::pair(Q, false), g = F::group(Q);
+        mfma_pair<F>(acc[g], a[s], b[t]);
+        mfma_pairs<F, Q + 1>(acc, a, b);
+    }
+}
+
+// Consumer side of the stream: G steps; step g multiplies A block ablk of ring slot g % 3
+// by the table fragments bfrag(g) (resource rt, loaded one step ahead), then passes the
+// step's barrier; close(g) / done(g, sums) as in k_stream.
+template <class F, class BFrag, class Close, class Done>
+__device__ __forceinline__ void ws_consume(int G, const BFrag &bfrag, const Close &close, const Done &done,
+                                           __amdgpu_buffer_rsrc_t rt, const u32x4 (&b0)[F::NB],
+                                           u32x4 (*ring)[kWsABuf], int ablk, int lane)
+{
+    u32x4 bA[F::NB], bB[F::NB];
+#pragma unroll
+    for (int t = 0; t < F::NB; ++t) bA[t] = b0[t];
+    __syncthreads();  // the exponents
+    __syncthreads();  // step 0's A fragments
+    typename F::acc_t acc[F::NG];
+#pragma unroll
+    for (int g = 0; g < F::NG; ++g) acc[g] = typename F::acc_t{};
+    double dacc[F::I8 ? 1 : 16];
+#pragma unroll
+    for (int v = 0; v < (F::I8 ? 1 : 16); ++v) dacc[v] = 0.0;
+    int slot = 0;
+    auto step = [&](int g, const u32x4 (&bc)[F::NB], u32x4 (&bn)[F::NB]) {
+        get_b<F::NB>(rt, bfrag(g + 1), lane, bn);
+        u32x4 a[F::NA];
+        get_a<F::NA>(ring[slot], ablk, lane, a);
+        mfma_pairs<F>(acc, a, bc);
+        slot = slot == kWsRing - 1 ? 0 : slot + 1;
+        __syncthreads();
+        const int c = close(g);
+        if constexpr (F::I8) {
+            if (c == 2) {
+                double sums[16];
+#pragma unroll
+                for (int v = 0; v < 16; ++v) sums[v] = 0.0;
+                flush<F>(acc, sums);
+                done(g, sums);
+            }
+        } else if (c != 0) {
+            flush<F>(acc, dacc);
+            if (c == 2) {
+                done(g, dacc);
+#pragma unroll
+                for (int v = 0; v < 16; ++v) dacc[v] = 0.0;
+            }
+        }
+    };
+    int g = 0;
+    for (; g + 1 < G; g += 2) {
+        step(g, bA, bB);
+        step(g + 1, bB, bA);
+    }
+    if (g < G) step(g, bA, bB);
+}
+
+// Producer side: slices step 0 into slot 0 before the stream, then in step g loads step
+// g + 2's raw data and slices step g + 1 into slot (g + 1) % 3 (the consumers read slot g % 3;
+// slot (g + 1) % 3 was last read in step g - 2, before the barrier that ended step g - 1).
+template <class F, class Raw, class Fetch, class Slice>
+__device__ __forceinline__ void ws_produce(int G, const Fetch &fetch, const Slice &slice, const Raw &r0,
+                                           const Raw &r1, u32x4 (*ring)[kWsABuf])
+{
+    {
+        auto sl = slice(r0, 0, ring[0]);
+#pragma unroll
+        for (int sc = 0; sc < F::NA; ++sc) sl.step(sc);
+    }
+    __syncthreads();  // step 0's A fragments
+    Raw rA = r1, rB;
+    int slot = 1;
+    auto step = [&](int g, const Raw &rc, Raw &rn) {
+        rn = fetch(g + 2);
+        auto sl = slice(rc, g + 1, ring[slot]);
+#pragma unroll
+        for (int sc = 0; sc < F::NA; ++sc) sl.step(sc);
+        slot = slot == kWsRing - 1 ? 0 : slot + 1;
+        __syncthreads();
+    };
+    int g = 0;
+    for (; g + 1 < G; g += 2) {
+        step(g, rA, rB);
+        step(g + 1, rB, rA);
+    }
+    if (g < G) step(g, rA, rB);
+}
+
+// StP, wave-specialised: work item (mp, c0) as stp_split_kernel; consumer wave w owns theta
+// block kJB rt + w of row tile rt, the producers are 256 threads over the 32 columns.
+template <class F>
+__global__ __launch_bounds__(kWsThreads, 1) void stp_split_ws_kernel(
+    const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
+{
+    static_assert(kCB == 1 && kJB == 4, "four 32-row blocks of 32 columns per row tile");
+    __shared__ u32x4 ring[kWsRing][kWsABuf];
+    __shared__ double ebuf[16][256];
+    __shared__ int cexp[kTileC];
+    const int mp = uniform(work[blockIdx.x].mp);
+    if (mp < 0) return;
+    const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
+    const prob_desc &P = probs[pi];
+    const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
+    const int m = uniform(uniform(P.mvals)[mp]), nl = uniform(P.lmax) - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const bool producer = wave >= 4;
+    const int njb = cdiv(nt2, 32), nks_e = cdiv(ne, F::KS), nks_o = max(1, cdiv(no, F::KS)), nseg = nks_e + nks_o;
+    const int G = cdiv(njb, kJB) * nseg;
+    const small_div by_seg(nseg);
+    struct pos {
+        int rt, par, ks;
+    };
+    auto at = [&](int g) {
+        pos p;
+        p.rt = by_seg.quo(g);
+        const int r = g - p.rt * nseg;
+        p.par = r >= nks_e;
+        p.ks = p.par ? r - nks_e : r;
+        return p;
+    };
+    if (producer) {
+        const double *sb = uniform(io.in[pi]) + uniform(uniform(P.s_off)[mp]);
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(sb, (long)nl * nrhs * 8);
+        const int ptid = tid - 256, ca = ptid >> 3, q = ptid & 7;
+        const unsigned a_off = (unsigned)min(c0 + ca, nrhs) * nl * 8;  // column nrhs and beyond read as 0
+        using Raw = raw_step<F::VPT>;
+        auto fetch = [&](int g) {
+            const int l0 = 2 * F::KS * at(g).ks + 2 * F::VPT * q;
+            Raw r;
+            r.valid = 0;
+#pragma unroll
+            for (int i = 0; i < F::VPT; ++i) {
+                r.d[i] = load_d2(rs, a_off + (unsigned)(l0 + 2 * i) * 8);
+                r.valid |= (unsigned)(l0 + 2 * i < nl) << (2 * i) | (unsigned)(l0 + 2 * i + 1 < nl) << (2 * i + 1);
+            }
+            return r;
+        };
+        const Raw r0 = fetch(0), r1 = fetch(1);
+        double mx = 0.0;
+        const int n32 = cdiv(nl, 32);
+        for (int i0 = 0; i0 < n32; i0 += 8) {
+            double2 d[8][2];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) d[u][e] = load_d2(rs, a_off + (unsigned)(32 * (i0 + u) + 4 * q + 2 * e) * 8);
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int l = 32 * (i0 + u) + 4 * q + 2 * e;
+                    if (l < nl) mx = fmax(mx, fabs(d[u][e].x));
+                    if (l + 1 < nl) mx = fmax(mx, fabs(d[u][e].y));
+                }
+        }
+#pragma unroll
+        for (int off = 1; off < 8; off <<= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+        if (q == 0) cexp[ca] = scale_exponent(mx, F::I8);
+        __syncthreads();  // the exponents
+        const int ea = cexp[ca];
+        auto slice = [&](const Raw &r, int g, u32x4 *buf) {
+            const int par = at(g).par;
+            double v[F::VPT];
+#pragma unroll
+            for (int i = 0; i < F::VPT; ++i) v[i] = r.val(i, par);
+            slicers<F, 1> sl;
+            sl.sl[0].init(buf, 0, ca, F::VPT * q, v, ea);
+            return sl;
+        };
+        ws_produce<F, Raw>(G, fetch, slice, r0, r1, ring);
+    } else {
+        const long *frag_off = uniform(P.frag_stp_off);
+        const int *texp = uniform(P.texp_stp);
+        const long f_lo = uniform(frag_off[mp]), f_hi = uniform(frag_off[mp + 1]);
+        const __amdgpu_buffer_rsrc_t rt =
+            make_frag_rsrc(reinterpret_cast<const u32x4 *>(uniform(P.tdig_stp)) + (f_lo << 6), f_hi - f_lo);
+        double *ob = uniform(io.out[pi]) + (long)mp * nrhs * nt;
+        auto bfrag = [&](int g) {
+            const pos p = at(g);
+            const int jb = p.rt * kJB + wave;
+            return (p.par ? njb * nks_e * F::NB : 0) + (jb * (p.par ? nks_o : nks_e) + p.ks) * F::NB;
+        };
+        u32x4 b0[F::NB];
+        get_b<F::NB>(rt, bfrag(0), lane, b0);
+        auto close = [&](int g) {
+            const pos p = at(g);
+            if (p.ks == (p.par ? nks_o : nks_e) - 1) return 2;
+            return (!F::I8 && (p.ks + 1) % kChunkSteps == 0) ? 1 : 0;
+        };
+        auto done = [&](int g, const double (&dacc)[16]) {
+            const pos p = at(g);
+            const int ln = opaque(lane);
+            const int jb = p.rt * kJB + wave, j = jb * 32 + (ln & 31);
+            const bool store = jb < njb && j < nt2;
+            const int ej = store ? texp[((long)mp * 2 + p.par) * nt2 + j] : 0;
+            const int ct = opaque(tid);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int cc = 8 * (v >> 2) + 4 * (ln >> 5) + (v & 3), c = c0 + cc;
+                const double r = __builtin_ldexp(dacc[v], cexp[cc] + ej);
+                if (p.par == 0) {
+                    ebuf[v][ct] = r;
+                } else if (store && c < nrhs) {
+                    const double e = ebuf[v][ct];
+                    ob[(long)c * nt + j] = e + r;
+                    ob[(long)c * nt + (nt - 1 - j)] = e - r;
+                }
+            }
+        };
+        ws_consume<F>(G, bfrag, close, done, rt, b0, ring, 0, lane);
+    }
+}
+
+// PtS, wave-specialised: work item (mp, c0) as pts_split_kernel; consumer wave w owns
+// parity w / 2 and row block kKB rt + w % 2 of row tile rt; the producers slice both folds.
+template <class F>
+__global__ __launch_bounds__(kWsThreads, 1) void pts_split_ws_kernel(
+    const prob_desc *__restrict__ probs, const batch_io io, const work_item *__restrict__ work)
+{
+    static_assert(kCB == 1 && kKB == 2, "two 32-row blocks per parity and row tile");
+    __shared__ u32x4 ring[kWsRing][kWsABuf];
+    __shared__ int cexp[2][kTileC];
+    const int mp = uniform(work[blockIdx.x].mp);
+    if (mp < 0) return;
+    const int pi = uniform(work[blockIdx.x].pad) & 0xffff, c0 = uniform(work[blockIdx.x].c0);
+    const prob_desc &P = probs[pi];
+    const int nt = uniform(P.nt), nt2 = uniform(P.nt2), nrhs = uniform(P.nrhs);
+    const int m = uniform(uniform(P.mvals)[mp]), nl = uniform(P.lmax) - m + 1, ne = (nl + 1) >> 1, no = nl >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const bool producer = wave >= 4;
+    const int njs = cdiv(nt2, F::KS), G = cdiv(ne, kTileRP) * njs;
+    const small_div by_njs(njs);
+    if (producer) {
+        const double *pb = uniform(io.in[pi]) + (long)mp * nrhs * nt;
+        const __amdgpu_buffer_rsrc_t rp = make_rsrc(pb, (long)nrhs * nt * 8);
+        const int ptid = tid - 256, ca = ptid >> 3, q = ptid & 7;
+        const unsigned a_off = (unsigned)min(c0 + ca, nrhs) * nt * 8;
+        constexpr int H = F::VPT / 2;
+        using Raw = raw_step<F::VPT>;
+        // raw data of node step js: nodes j = j0 + i, j0 = KS js + VPT q, i < VPT, as the pairs
+        // d[u] = (f(j0 + 2u), f(j0 + 2u + 1)) and their mirrors d[H + u]; zero for j >= nt2
+        auto fetch_js = [&](int js) {
+            const int j0 = F::KS * js + F::VPT * q;
+            Raw r;
+            r.valid = 0;
+#pragma unroll
+            for (int u = 0; u < H; ++u) {
+                r.d[u] = load_d2(rp, a_off + (unsigned)(j0 + 2 * u) * 8u);
+                const double2 g = load_d2(rp, a_off + (unsigned)(nt - 2 - j0 - 2 * u) * 8u);
+                r.d[H + u] = make_double2(g.y, g.x);
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    if (j0 + 2 * u + h < nt2) r.valid |= 1u << (2 * u + h) | 1u << (2 * (H + u) + h);
+            }
+            return r;
+        };
+        auto fetch = [&](int g) { return fetch_js(g - by_njs.quo(g) * njs); };
+        const Raw r0 = fetch(0), r1 = fetch(1);
+        constexpr int EB = 8 / H;
+        double me = 0.0, mo = 0.0;
+        for (int j0 = 0; j0 < njs; j0 += EB) {
+            Raw r[EB];
+#pragma unroll
+            for (int u = 0; u < EB; ++u) r[u] = fetch_js(j0 + u);  // steps past njs are masked
+#pragma unroll
+            for (int u = 0; u < EB; ++u)
+#pragma unroll
+                for (int i = 0; i < F::VPT; ++i) {
+                    const double a = r[u].val(i >> 1, i & 1), b = r[u].val(H + (i >> 1), i & 1);
+                    me = fmax(me, fabs(a + b));
+                    mo = fmax(mo, fabs(a - b));
+                }
+        }
+#pragma unroll
+        for (int off = 1; off < 8; off <<= 1) {
+            me = fmax(me, __shfl_xor(me, off, 64));
+            mo = fmax(mo, __shfl_xor(mo, off, 64));
+        }
+        if (q == 0) {
+            cexp[0][ca] = scale_exponent(me, F::I8);
+            cexp[1][ca] = scale_exponent(mo, F::I8);
+        }
+        __syncthreads();  // the exponents
+        const int ee = cexp[0][ca], eo = cexp[1][ca];
+        // A blocks of a slot: [parity]
+        auto slice = [&](const Raw &r, int, u32x4 *buf) {
+            double ve[F::VPT], vo[F::VPT];
+#pragma unroll
+            for (int i = 0; i < F::VPT; ++i) {
+                const double a = r.val(i >> 1, i & 1), b = r.val(H + (i >> 1), i & 1);
+                ve[i] = a + b;
+                vo[i] = a - b;
+            }
+            slicers<F, 2> sl;
+            sl.sl[0].init(buf, 0, ca, F::VPT * q, ve, ee);
+            sl.sl[1].init(buf, 1, ca, F::VPT * q, vo, eo);
+            return sl;
+        };
+        ws_produce<F, Raw>(G, fetch, slice, r0, r1, ring);
+    } else {
+        const int par = wave >> 1, K_rows = par ? no : ne;
+        const long *frag_off = uniform(P.frag_pts_off);
+        const int *texp = uniform(P.texp_pts);
+        const int region = par ? cdiv(ne, 32) * njs * F::NB : 0;
+        const long f_lo = uniform(frag_off[mp]), f_hi = uniform(frag_off[mp + 1]);
+        const __amdgpu_buffer_rsrc_t rt =
+            make_frag_rsrc(reinterpret_cast<const u32x4 *>(uniform(P.tdig_pts)) + (f_lo << 6), f_hi - f_lo);
+        double *sb = uniform(io.out[pi]) + uniform(uniform(P.s_off)[mp]);
+        const long e_base = uniform(uniform(P.exp_pts_off)[mp]);
+        auto bfrag = [&](int g) {
+            const int rt_ = by_njs.quo(g), js = g - rt_ * njs;
+            return region + ((kKB * rt_ + (wave & 1)) * njs + js) * F::NB;
+        };
+        u32x4 b0[F::NB];
+        get_b<F::NB>(rt, bfrag(0), lane, b0);
+        auto close = [&](int g) {
+            const int js = g - by_njs.quo(g) * njs;
+            if (js == njs - 1) return 2;
+            return (!F::I8 && (js + 1) % kChunkSteps == 0) ? 1 : 0;
+        };
+        auto done = [&](int g, const double (&dacc)[16]) {
+            const int ln = opaque(lane);
+            const int kb = kKB * by_njs.quo(g) + (wave & 1), k = kb * 32 + (ln & 31);
+            if (k < K_rows) {
+                const int ek = texp[e_base + (par ? ne : 0) + k];
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int cc = 8 * (v >> 2) + 4 * (ln >> 5) + (v & 3), c = c0 + cc;
+                    if (c < nrhs) sb[(long)c * nl + 2 * k + par] = __builtin_ldexp(dacc[v], cexp[par][cc] + ek);
+                }
+            }
+        };
+        ws_consume<F>(G, bfrag, close, done, rt, b0, ring, par, lane);
     }
 }
 
 // --------------------------------------------------------------- host side
 }  // namespace
 
-// XCD-balanced work lists of (problem, m-block, row tile, column tile) items: whole
-// m-blocks dealt to the eight XCDs by longest-processing-time, interleaved so that item
-// i runs on XCD i % 8; within a block the row tiles of one column tile are consecutive.
-// The problem index travels in the `pad` field.
+// XCD-balanced work lists of (problem, m-block, column tile) items (each runs every row
+// tile of its column tile): whole m-blocks dealt to the eight XCDs by
+// longest-processing-time, interleaved so that item i runs on XCD i % 8.  The problem
+// index travels in the `pad` field.
 std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans, bool stp, sched_opts o)
 {
     struct blk {
@@ -985,9 +1249,8 @@ std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans,
         const int nrt = cdiv(p->nt2, kTileR), njs = cdiv(p->nt2, ks);
         for (int mp = 0; mp < p->nm; ++mp) {
             const int nl = p->lmax - p->mvals[mp] + 1, ne = (nl + 1) / 2, no = nl / 2;
-            // K steps (plus the exponent pass) of all row tiles of one column tile
-            const double steps = stp ? (double)nrt * (cdiv(ne, ks) + cdiv(no, ks) + cdiv(nl, 32))
-                                     : (double)cdiv(ne, kTileRP) * (njs + njs);
+            const double steps = stp ? (double)nrt * (cdiv(ne, ks) + std::max(1, cdiv(no, ks))) + cdiv(nl, 256)
+                                     : (double)cdiv(ne, kTileRP) * njs + cdiv(njs, 8);
             order.push_back(blk{pi, mp, steps});
         }
     }
@@ -996,39 +1259,13 @@ std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans,
     std::vector<double> cost;
     for (const blk &b : order) {
         const raysht_plan *p = plans[b.prob];
-        const int nct = cdiv(p->nrhs, kTileC), ne = (p->lmax - p->mvals[b.mp] + 2) / 2;
-        const int tile = stp ? kTileR : kTileRP, nrt = cdiv(stp ? p->nt2 : ne, tile);
+        const int nct = cdiv(p->nrhs, kTileC);
         std::vector<work_item> items;
-        for (int c = 0; c < nct; ++c)
-            for (int r = 0; r < nrt; ++r) items.push_back(work_item{b.mp, r * tile, c * kTileC, b.prob});
+        for (int c = 0; c < nct; ++c) items.push_back(work_item{b.mp, 0, c * kTileC, b.prob});
         blocks.push_back(std::move(items));
         cost.push_back(b.item * nct);
     }
     return deal_to_xcds(blocks, cost, o.xcd_groups);
-}
-
-// The slicing pass's items: (m-block, 0, first column) for every 32-column block of every
-// problem (problem index in `pad`), whole m-blocks dealt to the XCDs longest first.
-std::vector<work_item> split_slice_work(const std::vector<const raysht_plan *> &plans)
-{
-    struct blk {
-        int prob, mp, nl;
-    };
-    std::vector<blk> order;
-    for (int pi = 0; pi < (int)plans.size(); ++pi)
-        for (int mp = 0; mp < plans[pi]->nm; ++mp)
-            order.push_back(blk{pi, mp, plans[pi]->lmax - plans[pi]->mvals[mp] + 1});
-    std::stable_sort(order.begin(), order.end(), [](const blk &a, const blk &b) { return a.nl > b.nl; });
-    std::vector<std::vector<work_item>> blocks;
-    std::vector<double> cost;
-    for (const blk &b : order) {
-        const int ncb = cdiv(plans[b.prob]->nrhs, 32);
-        std::vector<work_item> items;
-        for (int c = 0; c < ncb; ++c) items.push_back(work_item{b.mp, 0, 32 * c, b.prob});
-        blocks.push_back(std::move(items));
-        cost.push_back((double)b.nl * ncb);
-    }
-    return deal_to_xcds(blocks, cost, true);
 }
 
 namespace {
@@ -1061,32 +1298,28 @@ int split_plan_init(raysht_plan *p)
     raysht_split &sp = p->split;
     const bool x6 = is_x6(p->precision), i8 = is_i8(p->precision);
     if (i8 && (p->nt2 > kMaxI8K || (p->lmax + 2) / 2 > kMaxI8K)) return RAYSHT_EUNSUPPORTED;
-    const int nb_stp = x6 ? stp_x6::NB : any_x3::NB, nb_pts = x6 ? pts_x6::NB : any_x3::NB;
-    static_assert(stp_x6::NB == stp_i6::NB && pts_x6::NB == pts_i6::NB && any_x3::NB == any_i3::NB, "table slices");
+    const int nb_stp = x6 ? stp_x6::NB : stp_x3::NB, nb_pts = x6 ? pts_x6::NB : pts_x3::NB;
+    static_assert(stp_x6::NB == stp_i6::NB && pts_x6::NB == pts_i6::NB && stp_x3::NB == stp_i3::NB &&
+                      pts_x3::NB == pts_i3::NB,
+                  "table slices");
     const int ks = i8 ? 32 : 16;
     const int nm = p->nm, nt2 = p->nt2, njb = cdiv(nt2, 32), njs = cdiv(nt2, ks);
-    const int na_stp = x6 ? stp_x6::NA : any_x3::NA, ncb = cdiv(p->nrhs, 32);
-    static_assert(stp_x6::NA == stp_i6::NA && any_x3::NA == any_i3::NA, "data slices");
-    std::vector<long> fs(nm + 1, 0), fp(nm + 1, 0), ep(nm + 1, 0), sd(nm + 1, 0);
+    std::vector<long> fs(nm + 1, 0), fp(nm + 1, 0), ep(nm + 1, 0);
     for (int mp = 0; mp < nm; ++mp) {
         const int nl = p->lmax - p->mvals[mp] + 1, ne = (nl + 1) / 2, no = nl / 2;
         fs[mp + 1] = fs[mp] + (long)njb * (cdiv(ne, ks) + cdiv(no, ks)) * nb_stp;
         fp[mp + 1] = fp[mp] + (long)(cdiv(ne, 32) + cdiv(no, 32)) * njs * nb_pts;
         ep[mp + 1] = ep[mp] + ne + no;
-        sd[mp + 1] = sd[mp] + (long)ncb * (cdiv(ne, ks) + cdiv(no, ks)) * na_stp;
     }
-    const std::vector<work_item> ws = split_work({p}, true), wp = split_work({p}, false), wsl = split_slice_work({p});
+    const std::vector<work_item> ws = split_work({p}, true), wp = split_work({p}, false);
     sp.n_work_stp = (int)ws.size();
     sp.n_work_pts = (int)wp.size();
-    sp.n_work_slice = (int)wsl.size();
     int rc;
     if ((rc = dupload(&sp.d_frag_stp_off, fs)) || (rc = dupload(&sp.d_frag_pts_off, fp)) ||
         (rc = dupload(&sp.d_exp_pts_off, ep)) || (rc = dupload(&sp.d_work_stp, ws)) ||
         (rc = dupload(&sp.d_work_pts, wp)) || (rc = dalloc(&sp.d_tdig_stp, (size_t)fs[nm] * 64)) ||
         (rc = dalloc(&sp.d_tdig_pts, (size_t)fp[nm] * 64)) || (rc = dalloc(&sp.d_texp_stp, (size_t)nm * 2 * nt2)) ||
-        (rc = dalloc(&sp.d_texp_pts, (size_t)ep[nm])) || (rc = dupload(&sp.d_sdig_off, sd)) ||
-        (rc = dalloc(&sp.d_sdig, (size_t)sd[nm] * 64)) || (rc = dalloc(&sp.d_scexp, (size_t)nm * p->nrhs)) ||
-        (rc = dupload(&sp.d_work_slice, wsl)))
+        (rc = dalloc(&sp.d_texp_pts, (size_t)ep[nm])))
         return rc;
     // rows past a parity's length in the last 32-row block of the PtS tables are read
     // as zeros: clear before the split kernel fills the rows that exist
@@ -1109,32 +1342,21 @@ void split_plan_free(raysht_plan *p)
     raysht_split &sp = p->split;
     for (void *ptr : {(void *)sp.d_frag_stp_off, (void *)sp.d_frag_pts_off, (void *)sp.d_exp_pts_off,
                       (void *)sp.d_work_stp, (void *)sp.d_work_pts, (void *)sp.d_tdig_stp, (void *)sp.d_tdig_pts,
-                      (void *)sp.d_texp_stp, (void *)sp.d_texp_pts, (void *)sp.d_sdig, (void *)sp.d_sdig_off,
-                      (void *)sp.d_scexp, (void *)sp.d_work_slice})
+                      (void *)sp.d_texp_stp, (void *)sp.d_texp_pts})
         if (ptr) (void)hipFree(ptr);
     sp = raysht_split();
 }
 
-namespace {
-// the slicing pass, then the contraction, on one stream
-template <class F>
-void launch_stp2(const prob_desc *d_desc, const batch_io &io, const work_item *slice_work, int n_slice,
-                 const work_item *work, int n_work, hipStream_t st)
-{
-    if (n_slice > 0) hipLaunchKernelGGL(stp_slice_kernel<F>, dim3(n_slice), dim3(256), 0, st, d_desc, io, slice_work);
-    hipLaunchKernelGGL(stp_split2_kernel<F>, dim3(n_work), dim3(kThreads), 0, st, d_desc, io, work);
-}
-}  // namespace
-
-int launch_stp_split_batch(int precision, const prob_desc *d_desc, const batch_io &io, const work_item *slice_work,
-                           int n_slice, const work_item *work, int n_work, hipStream_t st)
+int launch_stp_split_batch(int precision, const prob_desc *d_desc, const batch_io &io, const work_item *work,
+                           int n_work, hipStream_t st)
 {
     if (n_work == 0) return RAYSHT_OK;
+    const dim3 g(n_work), b(kThreads);
     switch (precision) {
-    case RAYSHT_BF16X6: launch_stp2<stp_x6>(d_desc, io, slice_work, n_slice, work, n_work, st); break;
-    case RAYSHT_BF16X3: launch_stp2<any_x3>(d_desc, io, slice_work, n_slice, work, n_work, st); break;
-    case RAYSHT_INT8X6: launch_stp2<stp_i6>(d_desc, io, slice_work, n_slice, work, n_work, st); break;
-    case RAYSHT_INT8X3: launch_stp2<any_i3>(d_desc, io, slice_work, n_slice, work, n_work, st); break;
+    case RAYSHT_BF16X6: hipLaunchKernelGGL(stp_split_kernel<stp_x6>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_BF16X3: hipLaunchKernelGGL(stp_split_kernel<stp_x3>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_INT8X6: hipLaunchKernelGGL(stp_split_kernel<stp_i6>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_INT8X3: hipLaunchKernelGGL(stp_split_kernel<stp_i3>, g, b, 0, st, d_desc, io, work); break;
     default: return RAYSHT_EINVAL;
     }
     RAYSHT_HIP_TRY(hipGetLastError());
@@ -1148,9 +1370,9 @@ int launch_pts_split_batch(int precision, const prob_desc *d_desc, const batch_i
     const dim3 g(n_work), b(kThreads);
     switch (precision) {
     case RAYSHT_BF16X6: hipLaunchKernelGGL(pts_split_kernel<pts_x6>, g, b, 0, st, d_desc, io, work); break;
-    case RAYSHT_BF16X3: hipLaunchKernelGGL(pts_split_kernel<any_x3>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_BF16X3: hipLaunchKernelGGL(pts_split_kernel<pts_x3>, g, b, 0, st, d_desc, io, work); break;
     case RAYSHT_INT8X6: hipLaunchKernelGGL(pts_split_kernel<pts_i6>, g, b, 0, st, d_desc, io, work); break;
-    case RAYSHT_INT8X3: hipLaunchKernelGGL(pts_split_kernel<any_i3>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_INT8X3: hipLaunchKernelGGL(pts_split_kernel<pts_i3>, g, b, 0, st, d_desc, io, work); break;
     default: return RAYSHT_EINVAL;
     }
     RAYSHT_HIP_TRY(hipGetLastError());
@@ -1162,8 +1384,7 @@ int launch_stp_split(raysht_plan *p, const double *s, double *p2, hipStream_t st
     batch_io io{};
     io.in[0] = s;
     io.out[0] = p2;
-    return launch_stp_split_batch(p->precision, p->d_desc, io, p->split.d_work_slice, p->split.n_work_slice,
-                                  p->split.d_work_stp, p->split.n_work_stp, st);
+    return launch_stp_split_batch(p->precision, p->d_desc, io, p->split.d_work_stp, p->split.n_work_stp, st);
 }
 
 int launch_pts_split(raysht_plan *p, const double *p2, double *s, hipStream_t st)

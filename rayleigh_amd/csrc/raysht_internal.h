@@ -28,9 +28,10 @@ struct dd_pair {
 };
 
 // One unit of grouped-GEMM work: m-block mp, row tile start, column tile start, and in
-// `pad` the problem index (low 16 bits) and the tile shape (high bits: 0 = the full
-// tile, 1 / 2 / 3 = the PtS half- / quarter- / three-quarter-height tile for the last rows of
-// an m-block).
+// `pad` the problem index (low 16 bits) and, for the FP64 PtS kernel, the tile shape (high
+// bits: 0 = the full tile, 1 / 2 / 3 = the half- / quarter- / three-quarter-height tile for
+// the last rows of an m-block).  The split kernels' items run every row tile of their
+// column tile.
 struct work_item {
     int mp;
     int r0;
@@ -50,15 +51,6 @@ struct raysht_split {
     uint4 *d_tdig_stp = nullptr, *d_tdig_pts = nullptr;
     int *d_texp_stp = nullptr;  // [mp][parity][j]
     int *d_texp_pts = nullptr;
-    // StP's data operand, cut into digits once per call by a slicing pass before the
-    // contraction: fragment-major digit planes of every (m-block, parity, 32-column
-    // block, K step, digit), the first fragment of block mp, and the exponent of every
-    // (m-block, column)
-    uint4 *d_sdig = nullptr;
-    long *d_sdig_off = nullptr;
-    int *d_scexp = nullptr;
-    work_item *d_work_slice = nullptr;  // (m-block, 0, first column) items of the slicing pass
-    int n_work_slice = 0;
 };
 
 // Everything a contraction kernel reads about one problem (one plan), in device
@@ -72,9 +64,6 @@ struct prob_desc {
     const long *frag_stp_off, *frag_pts_off, *exp_pts_off;
     const uint4 *tdig_stp, *tdig_pts;
     const int *texp_stp, *texp_pts;
-    uint4 *sdig;  // StP data digits (written by the slicing pass of each call)
-    const long *sdig_off;
-    int *scexp;
     int lmax, nt, nt2, nrhs;
 };
 
@@ -159,8 +148,8 @@ int launch_stp_fp64_batch(const prob_desc *d_desc, const batch_io &io, const wor
                           hipStream_t st);
 int launch_pts_fp64_batch(const prob_desc *d_desc, const batch_io &io, const work_item *work, int n_work,
                           hipStream_t st);
-int launch_stp_split_batch(int precision, const prob_desc *d_desc, const batch_io &io, const work_item *slice_work,
-                           int n_slice, const work_item *work, int n_work, hipStream_t st);
+int launch_stp_split_batch(int precision, const prob_desc *d_desc, const batch_io &io, const work_item *work,
+                           int n_work, hipStream_t st);
 int launch_pts_split_batch(int precision, const prob_desc *d_desc, const batch_io &io, const work_item *work,
                            int n_work, hipStream_t st);
 // XCD-balanced work lists over the m-blocks of several problems (work_item.pad = index
@@ -174,8 +163,6 @@ struct sched_opts {
 };
 std::vector<work_item> fp64_work(const std::vector<const raysht_plan *> &plans, bool stp, sched_opts o = {});
 std::vector<work_item> split_work(const std::vector<const raysht_plan *> &plans, bool stp, sched_opts o = {});
-// the slicing pass's items: one per (problem, m-block, 32-column block), longest first
-std::vector<work_item> split_slice_work(const std::vector<const raysht_plan *> &plans);
 // deals per-block item lists to the XCDs (LPT over the block costs, blocks in the given
 // order) and interleaves them so that item i runs on XCD i % 8, or concatenates them
 std::vector<work_item> deal_to_xcds(const std::vector<std::vector<work_item>> &blocks,

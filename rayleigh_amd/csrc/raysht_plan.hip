@@ -200,9 +200,6 @@ prob_desc describe(const raysht_plan *p)
     d.tdig_pts = p->split.d_tdig_pts;
     d.texp_stp = p->split.d_texp_stp;
     d.texp_pts = p->split.d_texp_pts;
-    d.sdig = p->split.d_sdig;
-    d.sdig_off = p->split.d_sdig_off;
-    d.scexp = p->split.d_scexp;
     d.lmax = p->lmax;
     d.nt = p->nt;
     d.nt2 = p->nt2;
