@@ -364,7 +364,7 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
     """The transform stack (default l_max = 255, N_theta = 384, N_phi = 768, N_r = 128,
     8 fields; "large": l_max = 1023, N_theta = 1536, N_phi = 3072, N_r = 256, 5 fields),
     chained on the same device buffers as one time step runs it: Chebyshev From_Spectral ->
-    Transpose_1a2a -> StP -> 2a->3a -> c2r, then r2c -> 3b->2b -> PtS -> Transpose_2b1b ->
+    Transpose_1a2a -> StP -> 2a->3a -> c2r, then (r2c + 3b->2b in one kernel) -> PtS -> Transpose_2b1b ->
     Chebyshev To_Spectral, each stage timed with HIP events (GB/s of the bandwidth-bound
     stages, TFLOP/s of the contractions), on Chebyshev coefficients with the configuration's
     distribution; the chain must return its input (N_phi = 2 N_theta, so the unnormalised
@@ -391,10 +391,10 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
         ("cheb_from_spectral", lambda: cheb.from_spectral(c0, rad), nb_rad),
         ("transpose_1a2a", lambda: plan.transpose_1a2a(rad, s), 16.0 * plan.spectral_size),
         ("stp", lambda: plan.stp(s, p2), None),
-        ("permute_2a3a", lambda: plan.reform_2a3a(p2, p3, nphi), 8.0 * (plan.hybrid_size + n3)),
-        ("fft_c2r", lambda: ra.fft_to_physical(p3, nphi), 16.0 * n3),
-        ("fft_r2c", lambda: ra.fft_to_spectral(p3, nphi), 16.0 * n3),
-        ("permute_3b2b", lambda: plan.reform_3b2b(p3, p2, nphi), 8.0 * 2 * plan.hybrid_size),
+        # the 2a->3a reform then c2r (two kernels): bytes of one pass, hybrid read, padded grid written
+        ("reform_c2r", lambda: plan.reform_to_physical(p2, p3, nphi), 8.0 * (plan.hybrid_size + n3)),
+        # r2c and the 3b->2b reform in one pass: padded grid read, hybrid written
+        ("r2c_reform", lambda: plan.reform_to_spectral(p3, p2, nphi), 8.0 * (plan.hybrid_size + n3)),
         ("pts", lambda: plan.pts(p2, s2), None),
         ("transpose_2b1b", lambda: plan.transpose_2b1b(s2, rad2), 16.0 * plan.spectral_size),
         ("cheb_to_spectral", lambda: cheb.to_spectral(rad2, c2), nb_rad),
@@ -612,8 +612,24 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
     e1.record()
     torch.cuda.synchronize()
     out["graph_round_trip_ms"] = e0.elapsed_time(e1) / steps
-    out["eager_round_trip_ms"] = sum(out["stages"][k]["ms"] for k in
-                                     ("stp", "permute_2a3a", "fft_c2r", "fft_r2c", "permute_3b2b", "pts"))
+    out["eager_round_trip_ms"] = sum(out["stages"][k]["ms"] for k in ("stp", "reform_c2r", "r2c_reform", "pts"))
+    # the separate kernels the fused stages replace, on the same buffers (FFTs in place)
+    sep = {}
+    for name, fn, nbytes in (("permute_2a3a", lambda: plan.reform_2a3a(p2, p3, nphi), 8.0 * (plan.hybrid_size + n3)),
+                             ("fft_c2r", lambda: ra.fft_to_physical(p3, nphi), 16.0 * n3),
+                             ("fft_r2c", lambda: ra.fft_to_spectral(p3, nphi), 16.0 * n3),
+                             ("permute_3b2b", lambda: plan.reform_3b2b(p3, p2, nphi), 8.0 * 2 * plan.hybrid_size)):
+        for _ in range(max(warmup, 1)):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        sep[name] = {"ms": ms, "gbps": nbytes / (ms * 1e-3) / 1e9, "frac_hbm_peak": nbytes / (ms * 1e-3) / 8e12}
+    out["separate_transpose_fft"] = sep
     print(json.dumps(out), flush=True)
 
 

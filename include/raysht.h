@@ -220,9 +220,19 @@ int raysht_lu_decompose_host(int n, int kl, int ku, double *mat, int lda, int *p
 int raysht_lu_solve_host(int n, int kl, int ku, const double *mat, int lda, const int *pvt, double *rhs, int ldb,
                          int nrhs);
 
+/* The hybrid <-> physical reform and the phi FFT in one call (SphericalBuffer%reform,
+ * Spherical_Buffer.F90:196-703, with FFT_To_Physical / FFT_To_Spectral,
+ * Fourier_Transform.F90:56-183): raysht_reform_to_physical = raysht_permute_2a3a then
+ * raysht_fft_to_physical on the plan's lines, raysht_reform_to_spectral = raysht_fft_to_spectral
+ * (unscaled) then raysht_permute_3b2b; results identical to the separate calls. */
+int raysht_reform_to_physical(raysht_plan *plan, int nphi, const double *p2, double *p3, void *stream);
+int raysht_reform_to_spectral(raysht_plan *plan, int nphi, double *p3, double *p2, void *stream);
 /* Fused forms of the same steps (results identical to the separate calls):
  * StP then 2a->3a then c2r, and r2c then 3b->2b then PtS.  `work` is a hybrid
- * buffer the call may overwrite. */
+ * buffer the call may overwrite.  For nphi / 2 = 3 * 2^k (24 .. 3072, Rayleigh's
+ * nphi = 3 (l_max + 1)) the 3b->2b transpose runs inside the r2c kernel (the padded grid
+ * is read once); p3 is then left unchanged by raysht_reform_to_spectral /
+ * raysht_physical_to_spectral, which otherwise transform it in place. */
 int raysht_spectral_to_physical(raysht_plan *plan, int nphi, const double *s, double *work,
                                 double *p3, void *stream);
 int raysht_physical_to_spectral(raysht_plan *plan, int nphi, double *p3, double *work,

@@ -45,6 +45,8 @@ SIGNATURES = {
     "raysht_fft_to_spectral": (_I, [_I, _L, _P, _I, _P]),
     "raysht_permute_2a3a": (_I, [_P, _I, _P, _P, _P]),
     "raysht_permute_3b2b": (_I, [_P, _I, _P, _P, _P]),
+    "raysht_reform_to_physical": (_I, [_P, _I, _P, _P, _P]),
+    "raysht_reform_to_spectral": (_I, [_P, _I, _P, _P, _P]),
     "raysht_spectral_to_physical": (_I, [_P, _I, _P, _P, _P, _P]),
     "raysht_physical_to_spectral": (_I, [_P, _I, _P, _P, _P, _P]),
     "raysht_spectral_to_physical_scratch": (_I, [_P, _I, _P, _P, _P]),

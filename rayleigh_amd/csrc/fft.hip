@@ -592,6 +592,111 @@ __global__ __launch_bounds__(fft3_threads<N>()) void fft3_r2c_kernel(long howman
     }
 }
 
+// ---------------------------------------------------------------------------
+// The 3b -> 2b row transpose fused into the register-resident r2c (the
+// reform_to_spectral / physical_to_spectral entry points).  A workgroup transforms TB
+// lines of one (r, f) at TB consecutive theta; each thread turns its Z pairs into X_k and
+// X_{N-k} in place in its line's LDS buffer, and the workgroup then writes the m <= l_max
+// coefficients of its TB lines to the hybrid rows p2[mp][c][theta], TB consecutive theta of
+// every mode q = 2m + imi at a time.  The padded grid is read once and the separate pass
+// that read it back for 3b -> 2b is gone (1.26 ms against 0.94 + 0.67 ms at l_max = 255,
+// N_r = 128, 8 fields); the X arithmetic is fft3_r2c_kernel's own, so the results are
+// identical.  (The mirror fusion for c2r, gathering the hybrid rows through an LDS tile,
+// measured slower than the 2a -> 3a kernel plus c2r: one wave per 31 KB workgroup left
+// the gather latency-bound.)  Work unit w = (f, r, theta block), theta block fastest;
+// workgroup b takes unit (b % 8) per + b / 8, so consecutive theta blocks of one (r, f) run
+// on one XCD and share the cache lines of the hybrid rows in its L2.
+template <int N>
+__host__ __device__ constexpr int hyb_lines()
+{
+    return N <= 384 ? fft3_threads<N>() / (N / kP) : N == 768 ? 4 : N == 1536 ? 2 : 1;
+}
+
+struct hyb_args {
+    int nt, nr, nf, lmax, ntb, units;
+    const int *mp_of_m;
+};
+
+__device__ __forceinline__ bool hyb_unit(const hyb_args &a, int &r, int &f, int &t0, int TB)
+{
+    const int per = gridDim.x >> 3, b = blockIdx.x;
+    const int w = (b & 7) * per + (b >> 3);
+    if (w >= a.units) return false;
+    const int tb = w % a.ntb, rf = w / a.ntb;
+    r = rf % a.nr;
+    f = rf / a.nr;
+    t0 = tb * TB;
+    return true;
+}
+
+template <int N>
+__global__ __launch_bounds__(hyb_lines<N>() * (N / kP), 2) void fft3_r2c_hyb_kernel(hyb_args a,
+                                                                              const double *__restrict__ p3,
+                                                                              double *__restrict__ p2,
+                                                                              const double2 *__restrict__ tw)
+{
+    constexpr int T = N / kP, TB = hyb_lines<N>(), NTH = TB * T, A = fft3_n8(N), F = fft3_last(N);
+    constexpr int R0 = A > 0 ? 8 : F;
+    constexpr int STRIDE = fft3_pad(N - 1) + 1;
+    extern __shared__ double hyb_lds[];
+    int r, f, t0;
+    if (!hyb_unit(a, r, f, t0, TB)) return;
+    const int tid = threadIdx.x, t = tid % T, li = tid / T, th = t0 + li;
+    const bool line_in = th < a.nt;
+    const double *xl = p3 + (((long)f * a.nt + (line_in ? th : 0)) * a.nr + r) * (2L * N + 2);
+    cplx *buf = reinterpret_cast<cplx *>(hyb_lds) + li * STRIDE;
+    cplx v[kP];
+#pragma unroll
+    for (int u = 0; u < kP / R0; ++u)
+#pragma unroll
+        for (int rr = 0; rr < R0; ++rr) {
+            const double2 z = reinterpret_cast<const double2 *>(xl)[t + T * u + rr * (N / R0)];
+            v[u * R0 + rr] = {z.x, z.y};
+        }
+    fft3_pass<N, R0, 1>(v, t, tw, -1);
+    fft3_middle<N, 1, A>(buf, v, t, tw, -1);
+    constexpr int NSL = N / F;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kP / F; ++u)
+#pragma unroll
+        for (int rr = 0; rr < F; ++rr) buf[fft3_pad(t + T * u + rr * NSL)] = v[u * F + rr];
+    __syncthreads();
+    // X_k and X_{N-k} from Z_k and Z_{N-k}, written back in place of the two Z: each thread
+    // owns the pairs (k, N - k) for k = t + T u <= N / 2, so no other thread reads those
+    // slots (k = 0 and k = N / 2 pair with themselves; the Nyquist term X_N, m = N > l_max, is
+    // not read back).  The arithmetic of each X is fft3_r2c_kernel's own.
+    auto x_of = [&](int k, const cplx &zk, const cplx &zn) {  // zn = Z_{(N-k) % N}
+        const cplx zc = cconj(zn);
+        const cplx E = {0.5 * (zk.re + zc.re), 0.5 * (zk.im + zc.im)};
+        const cplx D = csub(zk, zc);
+        const cplx O = {0.5 * D.im, -0.5 * D.re};
+        const double2 w = tw[k];
+        return cadd(E, cmul(O, cplx{w.x, -w.y}));
+    };
+#pragma unroll
+    for (int u = 0; u <= kP / 2; ++u) {
+        const int k = t + T * u;
+        if (k > N / 2) continue;  // u = kP / 2 only for k = N / 2 (thread 0)
+        const int kn = (N - k) % N;
+        const cplx zk = buf[fft3_pad(k)], zn = buf[fft3_pad(kn)];
+        const cplx xk = x_of(k, zk, zn);
+        if (kn != k) buf[fft3_pad(kn)] = x_of(kn, zn, zk);
+        buf[fft3_pad(k)] = xk;
+    }
+    __syncthreads();
+    // the m <= l_max coefficients of the TB lines to the hybrid rows, TB theta at a time
+    const int nq = 2 * (a.lmax + 1);
+    const long nrhs = 2L * a.nr * a.nf;
+    const double *xs = hyb_lds;  // line i's buffer at i STRIDE complex
+    for (int idx = tid; idx < nq * TB; idx += NTH) {
+        const int q = idx / TB, i = idx - q * TB, th2 = t0 + i, mp = a.mp_of_m[q >> 1];
+        if (mp >= 0 && th2 < a.nt)
+            p2[((long)mp * nrhs + r + (long)(q & 1) * a.nr + 2L * a.nr * f) * a.nt + th2] =
+                xs[2 * ((long)i * STRIDE + fft3_pad(q >> 1)) + (q & 1)];
+    }
+}
+
 // The one-rank row transposes are 2-D transposes per (r, f): the hybrid side is
 // (mode q = 2m+imi) x theta with theta contiguous, the physical side theta x q with q
 // contiguous.  A Q (q) x 64 (theta) tile goes through LDS; both sides move 16-byte
@@ -741,6 +846,46 @@ bool fft3_dispatch(int n2, bool c2r, long howmany, double *x, const double2 *tw,
     case 768: *rc = launch_fft3<768>(c2r, howmany, x, tw, rescale, st); return true;
     case 1536: *rc = launch_fft3<1536>(c2r, howmany, x, tw, rescale, st); return true;
     case 3072: *rc = launch_fft3<3072>(c2r, howmany, x, tw, rescale, st); return true;
+    default: return false;
+    }
+}
+
+// the fused r2c + 3b -> 2b kernel of one register-resident length (arguments checked)
+template <int N>
+int launch_hyb(const raysht_plan *p, const double *src, double *dst, const double2 *tw, hipStream_t st)
+{
+    constexpr int TB = hyb_lines<N>(), NTH = TB * (N / kP), STRIDE = fft3_pad(N - 1) + 1;
+    hyb_args a;
+    a.nt = p->nt;
+    a.nr = p->nr;
+    a.nf = p->nf;
+    a.lmax = p->lmax;
+    a.ntb = (p->nt + TB - 1) / TB;
+    a.mp_of_m = p->d_mp_of_m;
+    const long units = (long)a.ntb * p->nr * p->nf;
+    if (units > 0x7fffffffL / 16) return RAYSHT_EUNSUPPORTED;
+    a.units = (int)units;
+    const size_t lds = (size_t)TB * STRIDE * sizeof(cplx);
+    const dim3 grid((unsigned)(8 * ((units + 7) / 8))), block(NTH);
+    hipLaunchKernelGGL(fft3_r2c_hyb_kernel<N>, grid, block, lds, st, a, src, dst, tw);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+// the fused kernel for the register-resident lengths; false for other lengths (handled
+// by r2c and the separate transpose)
+bool hyb_dispatch(int n2, const raysht_plan *p, const double *src, double *dst, const double2 *tw, hipStream_t st,
+                  int *rc)
+{
+    switch (n2) {
+    case 24: *rc = launch_hyb<24>(p, src, dst, tw, st); return true;
+    case 48: *rc = launch_hyb<48>(p, src, dst, tw, st); return true;
+    case 96: *rc = launch_hyb<96>(p, src, dst, tw, st); return true;
+    case 192: *rc = launch_hyb<192>(p, src, dst, tw, st); return true;
+    case 384: *rc = launch_hyb<384>(p, src, dst, tw, st); return true;
+    case 768: *rc = launch_hyb<768>(p, src, dst, tw, st); return true;
+    case 1536: *rc = launch_hyb<1536>(p, src, dst, tw, st); return true;
+    case 3072: *rc = launch_hyb<3072>(p, src, dst, tw, st); return true;
     default: return false;
     }
 }
@@ -914,23 +1059,44 @@ int raysht_permute_3b2b(raysht_plan *p, int nphi, const double *p3, double *p2, 
     return RAYSHT_OK;
 }
 
+int raysht_reform_to_physical(raysht_plan *p, int nphi, const double *p2, double *p3, void *stream)
+{
+    if (!p || !p2 || !p3) return RAYSHT_EINVAL;
+    int rc = check_fft_args(nphi, 1, p3);
+    if (rc != RAYSHT_OK) return rc;
+    if (p->nr < 1 || nphi < 2 * (p->lmax + 1) || (long)p->nr * p->nf > 65535) return RAYSHT_EINVAL;
+    if ((rc = raysht_permute_2a3a(p, nphi, p2, p3, stream)) != RAYSHT_OK) return rc;
+    return raysht_fft_to_physical(nphi, (long)p->nr * p->nt * p->nf, p3, stream);
+}
+
+int raysht_reform_to_spectral(raysht_plan *p, int nphi, double *p3, double *p2, void *stream)
+{
+    if (!p || !p2 || !p3) return RAYSHT_EINVAL;
+    int rc = check_fft_args(nphi, 1, p3);
+    if (rc != RAYSHT_OK) return rc;
+    if (p->nr < 1 || nphi < 2 * (p->lmax + 1) || (long)p->nr * p->nf > 65535) return RAYSHT_EINVAL;
+    const double2 *tw = nullptr;
+    if ((rc = get_twiddles(nphi, &tw)) != RAYSHT_OK) return rc;
+    if (hyb_dispatch(nphi / 2, p, p3, p2, tw, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
+    if ((rc = raysht_fft_to_spectral(nphi, (long)p->nr * p->nt * p->nf, p3, 0, stream)) != RAYSHT_OK) return rc;
+    return raysht_permute_3b2b(p, nphi, p3, p2, stream);
+}
+
 int raysht_spectral_to_physical(raysht_plan *p, int nphi, const double *s, double *work, double *p3, void *stream)
 {
     if (!p || !s || !work || !p3) return RAYSHT_EINVAL;
     int rc = check_fft_args(nphi, 1, p3);
     if (rc != RAYSHT_OK) return rc;
+    if (p->nr < 1 || nphi < 2 * (p->lmax + 1) || (long)p->nr * p->nf > 65535) return RAYSHT_EINVAL;
     if ((rc = raysht_legendre_stp(p, s, work, stream)) != RAYSHT_OK) return rc;
-    if ((rc = raysht_permute_2a3a(p, nphi, work, p3, stream)) != RAYSHT_OK) return rc;
-    return raysht_fft_to_physical(nphi, (long)p->nr * p->nt * p->nf, p3, stream);
+    return raysht_reform_to_physical(p, nphi, work, p3, stream);
 }
 
 int raysht_physical_to_spectral(raysht_plan *p, int nphi, double *p3, double *work, double *s, void *stream)
 {
     if (!p || !s || !work || !p3) return RAYSHT_EINVAL;
-    int rc = check_fft_args(nphi, 1, p3);
+    int rc = raysht_reform_to_spectral(p, nphi, p3, work, stream);
     if (rc != RAYSHT_OK) return rc;
-    if ((rc = raysht_fft_to_spectral(nphi, (long)p->nr * p->nt * p->nf, p3, 0, stream)) != RAYSHT_OK) return rc;
-    if ((rc = raysht_permute_3b2b(p, nphi, p3, work, stream)) != RAYSHT_OK) return rc;
     return raysht_legendre_pts(p, work, s, stream);
 }
 

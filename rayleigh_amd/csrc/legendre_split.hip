@@ -880,7 +880,11 @@ __global__ __launch_bounds__(kThreads, F::I8 && F::NA < 6 ? 3 : kMinWaves) void 
 // digit arithmetic runs on the vector ALU while the other wave keeps the matrix pipe busy
 // (in the one-role kernels both waves of a SIMD interleave the two, and the vector work,
 // about ten instructions per MFMA, left the matrix pipe idle most of the time).  Both roles
-// pass the same barriers: two before the stream, one per step.
+// pass the same barriers: two before the stream, one per step.  Used for the X3 forms,
+// whose registers leave room for two such workgroups per CU (two multiplying waves per
+// SIMD); the X6 forms' level sums allow one, and one multiplying wave per SIMD measured
+// slower than the one-role kernels' two (INT8X6 14.7 ms against 13.8 ms per round trip at
+// l_max = 511; INT8X3 10.7 ms against 11.0 ms).
 constexpr int kWsThreads = 512;
 constexpr int kWsRing = 3;
 constexpr int kWsABuf = 2 * 7 * 64;  // A fragments of one step: up to two blocks of seven digits
@@ -1351,12 +1355,14 @@ int launch_stp_split_batch(int precision, const prob_desc *d_desc, const batch_i
                            int n_work, hipStream_t st)
 {
     if (n_work == 0) return RAYSHT_OK;
-    const dim3 g(n_work), b(kThreads);
+    // X6 forms: the one-role kernel (two multiplying waves per SIMD); X3 forms: the
+    // wave-specialised kernel (its registers leave room for two workgroups per CU)
+    const dim3 g(n_work), b(kThreads), bw(kWsThreads);
     switch (precision) {
     case RAYSHT_BF16X6: hipLaunchKernelGGL(stp_split_kernel<stp_x6>, g, b, 0, st, d_desc, io, work); break;
-    case RAYSHT_BF16X3: hipLaunchKernelGGL(stp_split_kernel<stp_x3>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_BF16X3: hipLaunchKernelGGL(stp_split_ws_kernel<stp_x3>, g, bw, 0, st, d_desc, io, work); break;
     case RAYSHT_INT8X6: hipLaunchKernelGGL(stp_split_kernel<stp_i6>, g, b, 0, st, d_desc, io, work); break;
-    case RAYSHT_INT8X3: hipLaunchKernelGGL(stp_split_kernel<stp_i3>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_INT8X3: hipLaunchKernelGGL(stp_split_ws_kernel<stp_i3>, g, bw, 0, st, d_desc, io, work); break;
     default: return RAYSHT_EINVAL;
     }
     RAYSHT_HIP_TRY(hipGetLastError());
@@ -1367,12 +1373,12 @@ int launch_pts_split_batch(int precision, const prob_desc *d_desc, const batch_i
                            int n_work, hipStream_t st)
 {
     if (n_work == 0) return RAYSHT_OK;
-    const dim3 g(n_work), b(kThreads);
+    const dim3 g(n_work), b(kThreads), bw(kWsThreads);
     switch (precision) {
     case RAYSHT_BF16X6: hipLaunchKernelGGL(pts_split_kernel<pts_x6>, g, b, 0, st, d_desc, io, work); break;
-    case RAYSHT_BF16X3: hipLaunchKernelGGL(pts_split_kernel<pts_x3>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_BF16X3: hipLaunchKernelGGL(pts_split_ws_kernel<pts_x3>, g, bw, 0, st, d_desc, io, work); break;
     case RAYSHT_INT8X6: hipLaunchKernelGGL(pts_split_kernel<pts_i6>, g, b, 0, st, d_desc, io, work); break;
-    case RAYSHT_INT8X3: hipLaunchKernelGGL(pts_split_kernel<pts_i3>, g, b, 0, st, d_desc, io, work); break;
+    case RAYSHT_INT8X3: hipLaunchKernelGGL(pts_split_ws_kernel<pts_i3>, g, bw, 0, st, d_desc, io, work); break;
     default: return RAYSHT_EINVAL;
     }
     RAYSHT_HIP_TRY(hipGetLastError());

@@ -107,6 +107,21 @@ class LegendrePlan:
         _lib.check(_lib.lib().raysht_permute_2a3a(self._h, int(n_phi), _dptr(p2a, "p2a", self.hybrid_size),
                                                   _dptr(p3a, "p3a", n), _stream(stream)), "raysht_permute_2a3a")
 
+    def reform_to_physical(self, p2a, p3a, n_phi: int, stream=None):
+        """reform_2a3a then fft_to_physical in one call."""
+        n = (n_phi + 2) * self.n_r * self.n_theta * self.n_fields
+        _lib.check(_lib.lib().raysht_reform_to_physical(self._h, int(n_phi), _dptr(p2a, "p2a", self.hybrid_size),
+                                                        _dptr(p3a, "p3a", n), _stream(stream)),
+                   "raysht_reform_to_physical")
+
+    def reform_to_spectral(self, p3b, p2b, n_phi: int, stream=None):
+        """fft_to_spectral then reform_3b2b in one call (one pass, p3b left unchanged, for
+        Rayleigh's lengths N_phi = 3 * 2^k)."""
+        n = (n_phi + 2) * self.n_r * self.n_theta * self.n_fields
+        _lib.check(_lib.lib().raysht_reform_to_spectral(self._h, int(n_phi), _dptr(p3b, "p3b", n),
+                                                        _dptr(p2b, "p2b", self.hybrid_size), _stream(stream)),
+                   "raysht_reform_to_spectral")
+
     def reform_3b2b(self, p3b, p2b, n_phi: int, stream=None):
         n = (n_phi + 2) * self.n_r * self.n_theta * self.n_fields
         _lib.check(_lib.lib().raysht_permute_3b2b(self._h, int(n_phi), _dptr(p3b, "p3b", n),

@@ -245,3 +245,47 @@ def test_permutes_host_forms_equal_device_forms(ra):
     ptrs2 = (P * len(mv))(*[s2.ctypes.data + 8 * int(off[i]) for i in range(len(mv))])
     assert lib.raysht_permute_1a2a_host(plan.handle, p1.ctypes.data_as(P), ptrs2) == 0
     assert np.array_equal(s2, s)
+
+
+@pytest.mark.parametrize("nt,lmax,nr,nf,nphi,msub", [
+    (384, 255, 3, 2, 768, False),   # N = 384: four lines per workgroup
+    (98, 63, 2, 3, 192, False),     # N = 96: 98 theta, a partial last theta block
+    (64, 41, 3, 1, 128, True),      # a plan without every m (the others read and write nothing)
+    (24, 1023, 1, 1, 3072, False),  # N = 1536: two lines per workgroup
+    (40, 19, 2, 2, 60, False),      # N = 30: not register-resident, the separate kernels
+])
+def test_fused_transposes_equal_separate(ra, nt, lmax, nr, nf, nphi, msub):
+    """spectral_to_physical / physical_to_spectral (the 3b -> 2b transpose inside the r2c
+    kernel where the length allows) equal StP -> 2a3a -> c2r and r2c -> 3b2b -> PtS bit
+    for bit."""
+    mv = np.arange(0, lmax + 1, 2 if msub else 1, dtype=np.int32)
+    mv = mv[np.random.default_rng(nt).permutation(len(mv))]
+    plan = ra.LegendrePlan(nt, lmax, mv, nr, nf)
+    s = dev(O.synthetic_spectral(lmax, mv, nr, nf, seed=nt + lmax))
+    n3 = (nphi + 2) * nr * nt * nf
+    w = empty(plan.hybrid_size)
+    p3 = empty(n3)
+    plan.spectral_to_physical(s, w, p3, nphi)
+    grid = host(p3)  # physical_to_spectral may transform p3 in place (lengths not fused)
+    p2a = empty(plan.hybrid_size)
+    plan.stp(s, p2a)
+    p3s = empty(n3)
+    plan.reform_2a3a(p2a, p3s, nphi)
+    ra.fft_to_physical(p3s, nphi)
+    assert np.array_equal(host(p3), host(p3s))
+    # physical -> spectral on the same grid: the fused form against the separate steps
+    s2 = empty(plan.spectral_size)
+    plan.physical_to_spectral(p3, w, s2, nphi)
+    ra.fft_to_spectral(p3s, nphi)
+    p2b = empty(plan.hybrid_size)
+    plan.reform_3b2b(p3s, p2b, nphi)
+    s2s = empty(plan.spectral_size)
+    plan.pts(p2b, s2s)
+    assert np.array_equal(host(s2), host(s2s))
+    # the reform + FFT calls on their own
+    p3r = empty(n3)
+    plan.reform_to_physical(p2a, p3r, nphi)
+    assert np.array_equal(host(p3r), grid)
+    p2r = empty(plan.hybrid_size)
+    plan.reform_to_spectral(p3r, p2r, nphi)
+    assert np.array_equal(host(p2r), host(p2b))
