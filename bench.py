@@ -40,6 +40,20 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def time_ms(fn, steps: int, warmup: int) -> float:
+    """Mean HIP-event time of fn over steps calls after max(warmup, 1) untimed ones."""
+    import torch
+    for _ in range(max(warmup, 1)):
+        fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
 def flops_per_direction(lmax=LMAX, nt=NTHETA, nr=NR, nf=NF) -> float:
     return float(nt) * nr * nf * (lmax + 1) * (lmax + 2)
 
@@ -375,6 +389,7 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
     mv = ra.m_values_balanced(L)
     plan = ra.LegendrePlan(nt, L, mv, nr, nf)
     cheb = ra.ChebyGrid(nr, 0.35, 1.0)
+    cheb_form = "dct" if nr in (128, 256) else "matrix"  # the library's default for n_r
     l_lm, mp_lm = plan.lm_values()
     nlm = len(l_lm)
     c0 = radial_coeffs_device(L, l_lm, mv[mp_lm], nr, nf, seed=3)
@@ -429,12 +444,29 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
             rec["gbps"] = nbytes / (ms * 1e-3) / 1e9
             rec["frac_hbm_peak"] = rec["gbps"] / 8000.0
         if name.startswith("cheb_"):
-            # the matrix form is FP64-MFMA work: N_r^2 flop per radial column (two parity
-            # halves of (N_r / 2)^2 multiply-adds)
-            flop = float(nr) * nr * (c0.numel() // nr)
-            rec["tflops"] = flop / (ms * 1e-3) / 1e12
-            rec["frac_fp64_peak"] = rec["tflops"] / PEAK_FP64_TFLOPS
+            rec["form"] = cheb_form
+            if cheb_form == "matrix":
+                # the table form is FP64-MFMA work: N_r^2 flop per radial column (two parity
+                # halves of (N_r / 2)^2 multiply-adds)
+                flop = float(nr) * nr * (c0.numel() // nr)
+                rec["tflops"] = flop / (ms * 1e-3) / 1e12
+                rec["frac_fp64_peak"] = rec["tflops"] / PEAK_FP64_TFLOPS
         out["stages"][name] = rec
+    # both Chebyshev forms on the chain's buffers, and what a plain streaming kernel (torch's
+    # elementwise multiply, one read and one write of the same bytes) reaches on them
+    forms = {}
+    for form in (("dct", "matrix") if cheb_form == "dct" else ("matrix",)):
+        cheb.set_form(form)
+        rec = {}
+        for name, fn in (("to_spectral", lambda: cheb.to_spectral(rad2, c2)),
+                         ("from_spectral", lambda: cheb.from_spectral(c0, rad))):
+            ms = time_ms(fn, steps, warmup)
+            rec[name] = {"ms": ms, "gbps": nb_rad / (ms * 1e-3) / 1e9, "frac_hbm_peak": nb_rad / (ms * 1e-3) / 8e12}
+        forms[form] = rec
+    cheb.set_form(cheb_form)
+    ms = time_ms(lambda: torch.mul(c0, 1.0000001, out=rad), steps, warmup)
+    forms["stream_ceiling"] = {"ms": ms, "gbps": nb_rad / (ms * 1e-3) / 1e9, "frac_hbm_peak": nb_rad / (ms * 1e-3) / 8e12}
+    out["cheb_forms"] = forms
     rad = c0  # the later figures run on the chain's own data: s (spectral), c0 (Chebyshev coefficients)
     # the one-rank radial transposes around the implicit solve (Transpose_2b1b after PtS,
     # Transpose_1a2a before StP): 16 B per spectral coefficient (read + write)

@@ -314,6 +314,19 @@ int raysht_cheb_table(const raysht_cheb *cheb, int which, double *host_out);
 int raysht_cheb_to_spectral(raysht_cheb *cheb, const double *f, double *c, long ncols, void *stream);
 /* Cheby_Grid%from_spectral = From_Spectral4D (:450-558). */
 int raysht_cheb_from_spectral(raysht_cheb *cheb, const double *c, double *f, long ncols, void *stream);
+/* How the two transforms above are computed.  RAYSHT_CHEB_DCT (available where n is a
+ * power of two, 16 <= n <= 256; the default for n = 128 and 256, where it is the faster):
+ * a DCT-II / DCT-III through one complex FFT of length n
+ * per column, which also carries the first-order correction for the reference's angles
+ * (its tables are cos(k theta_i) at theta_i accumulated by repeated addition, up to
+ * 8e-15 away from (i + 1/2) pi / n at n = 256); within 1e-14 of the table form and
+ * closer than it to the exact sums on those angles.  RAYSHT_CHEB_MATRIX: the
+ * parity-split products with the Gen_Tn tables, the reference's own form.  Columns must
+ * start 16-byte aligned for the DCT form; other pointers take the table form.  Setting
+ * RAYSHT_CHEB_DCT for an n without it returns RAYSHT_EUNSUPPORTED.  No counterpart in
+ * the reference (a choice of algorithm for Cheby_Grid%to_spectral / %from_spectral). */
+enum raysht_cheb_form { RAYSHT_CHEB_DCT = 0, RAYSHT_CHEB_MATRIX = 1 };
+int raysht_cheb_set_form(raysht_cheb *cheb, int form);
 /* Host-array forms of the two Chebyshev transforms (synchronous). */
 int raysht_cheb_to_spectral_host(raysht_cheb *cheb, const double *f, double *c, long ncols);
 int raysht_cheb_from_spectral_host(raysht_cheb *cheb, const double *c, double *f, long ncols);

@@ -435,6 +435,46 @@ def cheby_from_spectral(n: int, c: np.ndarray) -> np.ndarray:
     return f
 
 
+def cheby_angles_longdouble(n: int) -> np.ndarray:
+    """The angles the reference's Gen_Tn tables use for every radius, in long double:
+    theta_i accumulated in doubles by repeated addition (Chebyshev_Polynomials.F90:233-243)
+    for the first n_x radii, pi - theta_(n-1-i) for the rest (the parity split of
+    To/From_Spectral4D, :375-558, applies the first half's cosines to the mirror points)."""
+    d = np.pi / n
+    arg = d * 0.5
+    th = np.zeros(n)
+    for i in range(n):
+        th[i] = arg
+        arg = arg + d
+    nx = n - n // 2
+    pi_l = np.longdouble("3.14159265358979323846264338327950288")
+    out = th.astype(np.longdouble)
+    out[nx:] = pi_l - out[n - 1 - np.arange(nx, n)]
+    return out
+
+
+def cheby_to_spectral_exact(n: int, f: np.ndarray, theta=None) -> np.ndarray:
+    """To_Spectral4D's sums c_k = (2/n) sum_i f_i cos(k theta_i) on the reference's angles
+    (or on theta, long double, if given), with the cosines and sums in long double (the
+    reference's tables round k theta_i and cos to doubles, which moves its result by up to
+    about 1e-14 of max |c|)."""
+    th = cheby_angles_longdouble(n) if theta is None else np.asarray(theta, dtype=np.longdouble)
+    k = np.arange(n, dtype=np.longdouble)
+    T = np.cos(np.outer(k, th))  # [k][i]
+    f = np.asarray(f, dtype=np.float64).reshape(-1, n).astype(np.longdouble)
+    return ((f @ T.T) * (np.longdouble(2) / n)).astype(np.float64).ravel()
+
+
+def cheby_from_spectral_exact(n: int, c: np.ndarray, theta=None) -> np.ndarray:
+    """From_Spectral4D's sums f_i = sum_k c_k cos(k theta_i) - c_0 / 2 on the reference's
+    angles (or on theta), in long double."""
+    th = cheby_angles_longdouble(n) if theta is None else np.asarray(theta, dtype=np.longdouble)
+    k = np.arange(n, dtype=np.longdouble)
+    T = np.cos(np.outer(k, th))
+    c = np.asarray(c, dtype=np.float64).reshape(-1, n).astype(np.longdouble)
+    return (c @ T - c[:, :1] / 2).astype(np.float64).ravel()
+
+
 def cheby_deriv_scaling(n: int, rmin: float, rmax: float) -> float:
     return oracle_lib().orc_cheby_deriv_scaling(n, rmin, rmax)
 

@@ -69,6 +69,7 @@ SIGNATURES = {
     "raysht_cheb_to_spectral": (_I, [_P, _P, _P, _L, _P]),
     "raysht_cheb_from_spectral": (_I, [_P, _P, _P, _L, _P]),
     "raysht_cheb_dealias": (_I, [_P, _P, _L, _P]),
+    "raysht_cheb_set_form": (_I, [_P, _I]),
     "raysht_cheb_deriv": (_I, [_P, _P, _P, _L, _I, _P]),
     "raysht_permute_2b1b": (_I, [_P, _P, _P, _P]),
     "raysht_permute_1a2a": (_I, [_P, _P, _P, _P]),

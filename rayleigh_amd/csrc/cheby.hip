@@ -10,10 +10,12 @@
 // by repeated addition, libm cos -- so they are the reference's doubles.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <new>
 #include <vector>
 
 #include "raysht_internal.h"
+#include "small_dft.h"
 
 namespace raysht {
 namespace {
@@ -449,6 +451,307 @@ __global__ __launch_bounds__(kDerivCols) void cheb_deriv_kernel(int n, long ncol
     }
 }
 
+// ---------------------------------------------------------------------------
+// DCT form (power-of-two n = P G <= 256).  With theta_i = (i + 1/2) pi / n the reference's
+// tables are cos(k theta'_i), theta'_i = theta_i + delta_i (the accumulated angles, mirrored
+// to pi - theta'_{n-1-i} in the second half, which is what the parity split amounts to).
+// To first order (k delta <= 2e-12, so the next term is below 1e-23):
+//   To_Spectral:   c_k = (2/n) [sum_i f_i cos(k theta_i) - k sum_i delta_i f_i sin(k theta_i)]
+//   From_Spectral: f_i = sum_k' c_k cos(k theta_i) - delta_i sum_k k c_k sin(k theta_i)
+// (k' halving k = 0).  sin(k theta_i) = (-1)^i cos((n - k) theta_i), so each sine sum is a
+// cosine sum of a second real sequence, and the two real transforms go through one complex
+// FFT of length n with Makhoul's reordering (x_2m and x_2m+1 to v_m and v_(n-1-m)):
+//   analysis:  u_i = f_i (1 + i e_i), e_i = (-1)^i delta_i S; V = FFT(v);
+//              c_j = (1/n) [Re(W_j (V_j + conj V_(n-j))) + (j/S) Re(conj W_j (V_(n-j) - conj V_j))]
+//   synthesis: V_j = conj W_j ((1 + j/n) c_j - i (j/n) c_(n-j)), v = unnormalised inverse FFT,
+//              f_i = (Re v - g_i Im v) / 2, g_i = n (-1)^i delta_i
+// with W_j = e^{-i pi j / (2n)} and S a power of two near 1 / max |delta|: both real
+// channels then carry values of one magnitude, so rounding in one (about 1e-16 of |V|)
+// does not reach the other multiplied by j.
+//
+// Length n = P G: a column belongs to G lanes of a wave (64 / G columns per wave), lane t
+// holding the indices t + G s, s < P, before and after the FFT.  A P-point DFT in
+// registers, the twiddle w_n^{t q}, a transpose through the wave's own LDS strip, then
+// G-point DFTs (P / G per lane).  Loads and stores are 16-byte (x_2m, x_2m+1) pairs or
+// G consecutive doubles per column; the partner values (v_(n-1-m), V_(n-j)) come from lane
+// G - 1 - t or (G - t) mod G by lane shuffles.  Waves share no data, so the only
+// synchronisation is the wave's own LDS ordering.
+constexpr int kDctWaves = 4;
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// pass-1 results x[q] of lane t -> pass-2 operands y[r][t2] = X_t2[t + G r]
+template <int P, int G>
+__device__ __forceinline__ void dct_transpose(double *buf, int t, const dcplx (&x)[P], dcplx (&y)[P / G][G])
+{
+    constexpr int LD = G + 1;  // odd stride: the column reads hit distinct banks
+    wave_lds_sync();           // the previous tile's reads of buf are done
+#pragma unroll
+    for (int q = 0; q < P; ++q) buf[q * LD + t] = x[q].x;
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < P / G; ++r)
+#pragma unroll
+        for (int t2 = 0; t2 < G; ++t2) y[r][t2].x = buf[(t + G * r) * LD + t2];
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < P; ++q) buf[q * LD + t] = x[q].y;
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < P / G; ++r)
+#pragma unroll
+        for (int t2 = 0; t2 < G; ++t2) y[r][t2].y = buf[(t + G * r) * LD + t2];
+}
+
+// Raw buffer accesses: a descriptor in SGPRs (the wave's column tile, or the tables) and a
+// 32-bit lane offset; reads past the descriptor's range return 0 and writes there are
+// dropped, so a partial last tile needs no branches.
+typedef unsigned int dct_u2 __attribute__((ext_vector_type(2)));
+typedef unsigned int dct_u4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t dct_rsrc(const double *base, long bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double dct_ld(__amdgpu_buffer_rsrc_t r, int off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ double2 dct_ld2(__amdgpu_buffer_rsrc_t r, int off)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ void dct_st(double v, __amdgpu_buffer_rsrc_t r, int off)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(dct_u2, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void dct_st2(double2 v, __amdgpu_buffer_rsrc_t r, int off)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(dct_u4, v), r, off, 0, 0);
+}
+
+// a lane's base byte offset with its bits hidden from the optimiser: base + constant then
+// stays an add, which the buffer instructions take as base VGPR + immediate (known-disjoint
+// bits turn it into an or, and every access gets its own offset register)
+__device__ __forceinline__ int lane_base(int v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// FFT of length P G on this lane's x (indices t + G s in, t + G u out); SG = -1 forward.
+// btab (LDS): e^{-2 pi i 2^k t / n} stored [k][t]; the twiddle w^{t q} is applied as the
+// product of the powers for the bits of q (at most 4 roundings, so within a few ulps),
+// which reads 4 values where a table of w^{t q} would take P - 1.  (The scheduling
+// barriers keep each phase's loads next to their use.)
+template <int P, int G, int SG>
+__device__ __forceinline__ void dct_fft(dcplx (&x)[P], double *buf, int t, const double2 *btab)
+{
+    constexpr int NB = P >= 16 ? 4 : (P >= 8 ? 3 : 2);
+    double2 b[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) b[k] = btab[k * G + t];
+    small_dft<P, SG>(x);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 1; q < P; ++q)
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+            if ((q >> k) & 1) x[q] = cmul(x[q], b[k].x, SG < 0 ? b[k].y : -b[k].y);
+    __builtin_amdgcn_sched_barrier(0);
+    dcplx y[P / G][G];
+    dct_transpose<P, G>(buf, t, x, y);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int r = 0; r < P / G; ++r) {
+        small_dft<G, SG>(y[r]);
+#pragma unroll
+        for (int k = 0; k < G; ++k) x[r + (P / G) * k] = y[r][k];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// a * e^{2 pi i e / 64} (e a compile-time constant once unrolled)
+__device__ __forceinline__ dcplx rot64(dcplx a, int e)
+{
+    if ((e & 63) == 0) return a;
+    return cmul(a, cos64(e), sin64(e));
+}
+
+// Lane exchanges inside a G-lane group (G = 4, 8, 16: a quad, half row or row of 16) as
+// DPP moves on the vector ALU; as ds_bpermute they queued behind the LDS transposes.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// the value of lane G - 1 - t (row_mirror, row_half_mirror, quad_perm 3,2,1,0)
+template <int G>
+__device__ __forceinline__ double lane_reverse(double v)
+{
+    return dpp_d<G == 16 ? 0x140 : (G == 8 ? 0x141 : 0x1B)>(v);
+}
+// the value of lane (G - t) mod G for t > 0 (the mirror, then row_shr:1; quad_perm 0,3,2,1);
+// what lane t = 0 receives is not meaningful
+template <int G>
+__device__ __forceinline__ double lane_partner(double v)
+{
+    if constexpr (G == 4)
+        return dpp_d<0x6C>(v);
+    else
+        return dpp_d<0x111>(lane_reverse<G>(v));
+}
+
+// tab: e^{-2 pi i 2^k t / n} [4][G] and e^{-i pi t / (2n)} [G] as (cos, sin) pairs (10 G
+// doubles), then floats: the analysis correction e and the synthesis correction g, each
+// [k][t][4], entry 4 k + r of lane t being the value at radius 2 (t + G s) + b, 2 s + b =
+// 4 k + r (the corrections move the result by 1e-12 of itself at most, so single
+// precision in them is far below the double rounding).  W_j = e^{-i pi j/(2n)} for
+// j = t + G u is W_t times the constant e^{-i pi u/(2P)}.  Every workgroup copies the
+// table into LDS once: read per column tile from global memory it took as many vector
+// memory operations as the columns themselves, and those are what the kernels wait on.
+constexpr int dct_tab_doubles(int n, int G) { return 10 * G + n; }
+
+__device__ __forceinline__ void dct_copy_table(double *ltab, const double *__restrict__ tab, int nd)
+{
+    for (int i = threadIdx.x; i < nd / 2; i += blockDim.x)
+        reinterpret_cast<double2 *>(ltab)[i] = reinterpret_cast<const double2 *>(tab)[i];
+    __syncthreads();
+}
+template <int P, int G>
+__global__ __launch_bounds__(64 * kDctWaves) void cheb_to_spectral_dct_kernel(long ncols, const double *__restrict__ f,
+                                                                           double *__restrict__ c,
+                                                                           const double *__restrict__ tab,
+                                                                           double inv_s)
+{
+    constexpr int n = P * G, CPW = 64 / G, LD = G + 1;
+    extern __shared__ double lds[];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int t = lane % G, cs = lane / G;
+    constexpr int TD = dct_tab_doubles(n, G);
+    double *buf = lds + TD + (wave * CPW + cs) * P * LD;
+    dct_copy_table(lds, tab, TD);
+    const double2 *btab = reinterpret_cast<const double2 *>(lds), *wtab = btab + 4 * G;
+    const float4 *etab = reinterpret_cast<const float4 *>(lds + 10 * G);
+    const long step = (long)gridDim.x * kDctWaves * CPW;
+    for (long col0 = ((long)blockIdx.x * kDctWaves + wave) * CPW; col0 < ncols; col0 += step) {
+        const long bytes = (ncols - col0 < CPW ? ncols - col0 : CPW) * n * 8;
+        const __amdgpu_buffer_rsrc_t rf = dct_rsrc(f + col0 * n, bytes), rc = dct_rsrc(c + col0 * n, bytes);
+        // lane offsets made inside the loop, so base + constant is folded into each access
+        // (hoisted, every sum would hold a register)
+        const int v16 = lane_base(cs * n * 8 + 16 * t), v8 = lane_base(cs * n * 8 + 8 * t);
+        const int tj = lane_base(t);
+        float ef[P];  // e at radii 2 (t + G s) and 2 (t + G s) + 1
+#pragma unroll
+        for (int k = 0; k < P / 4; ++k) {
+            const float4 e4 = etab[k * G + t];
+            ef[4 * k] = e4.x;
+            ef[4 * k + 1] = e4.y;
+            ef[4 * k + 2] = e4.z;
+            ef[4 * k + 3] = e4.w;
+        }
+        dcplx x[P];
+#pragma unroll
+        for (int s = 0; s < P / 2; ++s) {
+            const double2 v = dct_ld2(rf, v16 + 16 * G * s);  // (f_2m, f_2m+1), m = t + G s
+            const double2 e{(double)ef[2 * s], (double)ef[2 * s + 1]};
+            x[s] = dcplx{v.x, v.x * e.x};
+            // u_(2m+1) = v_(n-1-m) belongs to lane G-1-t as its register P-1-s
+            x[P - 1 - s] = dcplx{lane_reverse<G>(v.y), lane_reverse<G>(v.y * e.y)};
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        dct_fft<P, G, -1>(x, buf, t, btab);
+        const double2 wt = wtab[t];  // W_t
+        // outputs u and P-1-u read the same two registers (own and partner's), so taking
+        // them together frees registers as the loop goes
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const int u = (k & 1) ? P - 1 - k / 2 : k / 2;
+            if (k % 4 == 0) __builtin_amdgcn_sched_barrier(0);
+            const dcplx pv = x[u];
+            dcplx qv{lane_partner<G>(x[P - 1 - u].x), lane_partner<G>(x[P - 1 - u].y)};
+            if (t == 0) qv = x[(P - u) % P];
+            const dcplx wj = rot64(dcplx{wt.x, wt.y}, -u * (16 / P));  // W_j, j = t + G u
+            const double2 w{wj.x, wj.y};
+            const double sr = w.x * (pv.x + qv.x) - w.y * (pv.y - qv.y);
+            const double co = w.x * (qv.x - pv.x) + w.y * (qv.y + pv.y);
+            dct_st((sr + ((double)(tj + G * u) * inv_s) * co) * (1.0 / n), rc, v8 + 8 * G * u);
+        }
+    }
+}
+
+template <int P, int G>
+__global__ __launch_bounds__(64 * kDctWaves) void cheb_from_spectral_dct_kernel(long ncols,
+                                                                             const double *__restrict__ c,
+                                                                             double *__restrict__ f,
+                                                                             const double *__restrict__ tab)
+{
+    constexpr int n = P * G, CPW = 64 / G, LD = G + 1;
+    extern __shared__ double lds[];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int t = lane % G, cs = lane / G;
+    constexpr int TD = dct_tab_doubles(n, G);
+    double *buf = lds + TD + (wave * CPW + cs) * P * LD;
+    dct_copy_table(lds, tab, TD);
+    const double2 *btab = reinterpret_cast<const double2 *>(lds), *wtab = btab + 4 * G;
+    const float4 *gtab = reinterpret_cast<const float4 *>(lds + 10 * G + n / 2);
+    const long step = (long)gridDim.x * kDctWaves * CPW;
+    for (long col0 = ((long)blockIdx.x * kDctWaves + wave) * CPW; col0 < ncols; col0 += step) {
+        const long bytes = (ncols - col0 < CPW ? ncols - col0 : CPW) * n * 8;
+        const __amdgpu_buffer_rsrc_t rc = dct_rsrc(c + col0 * n, bytes), rf = dct_rsrc(f + col0 * n, bytes);
+        const int v8 = lane_base(cs * n * 8 + 8 * t),
+                  v16 = lane_base(cs * n * 8 + 16 * t), tj = lane_base(t);
+        const double2 wt = wtab[t];  // W_t
+        double cv[P];  // c_j, j = t + G s
+#pragma unroll
+        for (int s = 0; s < P; ++s) cv[s] = dct_ld(rc, v8 + 8 * G * s);
+        __builtin_amdgcn_sched_barrier(0);
+        dcplx x[P];
+#pragma unroll
+        for (int s = 0; s < P; ++s) {
+            if (s % 4 == 0) __builtin_amdgcn_sched_barrier(0);
+            const int j = tj + G * s;
+            const double cj = cv[s];
+            // c_(n-j): lane (G - t) mod G holds it as its register P-1-s, lane 0 as its own
+            // register (P - s) mod P; c_n = 0
+            double cn = lane_partner<G>(cv[P - 1 - s]);
+            if (t == 0) cn = s > 0 ? cv[P - s] : 0.0;
+            const dcplx wj = rot64(dcplx{wt.x, wt.y}, -s * (16 / P));  // W_j; conj W_j = (w.x, -w.y)
+            const double2 w{wj.x, wj.y};
+            const double a = (1.0 + (double)j * (1.0 / n)) * cj, b = -((double)j * (1.0 / n)) * cn;
+            x[s] = dcplx{w.x * a + w.y * b, w.x * b - w.y * a};
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        dct_fft<P, G, 1>(x, buf, t, btab);
+        float gf[P];  // g at radii 2 (t + G u) and 2 (t + G u) + 1
+#pragma unroll
+        for (int k = 0; k < P / 4; ++k) {
+            const float4 g4 = gtab[k * G + t];
+            gf[4 * k] = g4.x;
+            gf[4 * k + 1] = g4.y;
+            gf[4 * k + 2] = g4.z;
+            gf[4 * k + 3] = g4.w;
+        }
+#pragma unroll
+        for (int u = 0; u < P / 2; ++u) {
+            if (u % 4 == 0) __builtin_amdgcn_sched_barrier(0);
+            const dcplx a = x[u];  // v_m, m = t + G u
+            // v_(n-1-m) is lane G-1-t's register P-1-u
+            const dcplx b{lane_reverse<G>(x[P - 1 - u].x), lane_reverse<G>(x[P - 1 - u].y)};
+            const double2 g{(double)gf[2 * u], (double)gf[2 * u + 1]};
+            dct_st2(double2{0.5 * (a.x - g.x * a.y), 0.5 * (b.x - g.y * b.y)}, rf, v16 + 16 * G * u);
+        }
+    }
+}
+
 }  // namespace
 }  // namespace raysht
 
@@ -473,6 +776,57 @@ int launch_cheb_reg(raysht_cheb *cb, bool to, const double *in, double *out, lon
                            cb->n_x, cb->n_even, cb->k4, cb->ip, ncols, cb->d_t_e, cb->d_t_o, in, out);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
+}
+
+// persistent workgroups: as many as are resident at once (a grid of several rounds leaves
+// the last round's CUs partly idle)
+template <typename K>
+long resident_blocks(K kernel, int device, size_t lds)
+{
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * kDctWaves, lds) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || per_cu < 1)
+        return 1024;
+    return (long)per_cu * cus;
+}
+
+template <int P, int G>
+int launch_cheb_dct(raysht_cheb *cb, bool to, const double *in, double *out, long ncols, hipStream_t st)
+{
+    constexpr int n = P * G, CPW = 64 / G;
+    const long waves = (ncols + CPW - 1) / CPW;
+    const size_t lds = sizeof(double) * (dct_tab_doubles(n, G) + kDctWaves * CPW * P * (G + 1));
+    const long cap = to ? resident_blocks(cheb_to_spectral_dct_kernel<P, G>, cb->device, lds)
+                        : resident_blocks(cheb_from_spectral_dct_kernel<P, G>, cb->device, lds);
+    const unsigned blocks = (unsigned)std::min<long>((waves + kDctWaves - 1) / kDctWaves, cap);
+    static_assert(n * 8 * (64 / G) < (1 << 20), "tile offsets fit 32 bits");
+    if (to)
+        hipLaunchKernelGGL((cheb_to_spectral_dct_kernel<P, G>), dim3(blocks), dim3(64 * kDctWaves), lds, st, ncols, in,
+                           out, cb->d_dct, cb->dct_inv_s);
+    else
+        hipLaunchKernelGGL((cheb_from_spectral_dct_kernel<P, G>), dim3(blocks), dim3(64 * kDctWaves), lds, st, ncols,
+                           in, out, cb->d_dct);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+bool dct_length(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256; }
+// lanes per column G of the DCT kernel for n (P = n / G registers per lane)
+int dct_lanes(int n) { return n == 256 ? 16 : (n >= 64 ? 8 : 4); }
+
+// the DCT kernels where the plan has them and the columns are 16-byte aligned
+bool cheb_dct_dispatch(raysht_cheb *cb, bool to, const double *in, double *out, long ncols, hipStream_t st, int *rc)
+{
+    if (cb->form != RAYSHT_CHEB_DCT || !cb->d_dct) return false;
+    if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) return false;
+    switch (cb->n) {
+    case 16: *rc = launch_cheb_dct<4, 4>(cb, to, in, out, ncols, st); return true;
+    case 32: *rc = launch_cheb_dct<8, 4>(cb, to, in, out, ncols, st); return true;
+    case 64: *rc = launch_cheb_dct<8, 8>(cb, to, in, out, ncols, st); return true;
+    case 128: *rc = launch_cheb_dct<16, 8>(cb, to, in, out, ncols, st); return true;
+    case 256: *rc = launch_cheb_dct<16, 16>(cb, to, in, out, ncols, st); return true;
+    default: return false;
+    }
 }
 
 // returns false when n is too long for the register tables (> 256)
@@ -568,11 +922,52 @@ int raysht_cheb_create(raysht_cheb **out, int n, double rmin, double rmax, doubl
             tt_o[(size_t)i * cb->kp + k] = cb->codd[(size_t)k * cb->n_x + i];
             t_o[(size_t)k * cb->ip + i] = cb->codd[(size_t)k * cb->n_x + i];
         }
+    // DCT form: twiddles, e^{-i pi j/(2n)}, and the two correction sequences (delta mirrored
+    // as the parity split uses the first half's angles for the second half)
+    std::vector<double> dct;
+    if (dct_length(n)) {
+        std::vector<double> twc(n), tws(n), qc(n), qs(n), delta(n);
+        cheb_dct_tables(n, theta.data(), twc.data(), tws.data(), qc.data(), qs.data(), delta.data());
+        std::vector<double> dp(n);
+        double dmax = 0.0;
+        for (int i = 0; i < n; ++i) {
+            dp[i] = i < n / 2 ? delta[i] : -delta[n - 1 - i];
+            dmax = std::max(dmax, std::fabs(dp[i]));
+        }
+        const double scale = dmax > 0.0 ? std::ldexp(1.0, -std::ilogb(dmax)) : 1.0;  // S dmax in [1, 2)
+        cb->dct_inv_s = 1.0 / scale;
+        const int G = dct_lanes(n), P = n / G;
+        dct.assign((size_t)dct_tab_doubles(n, G), 0.0);
+        for (int t = 0; t < G; ++t) {
+            for (int k = 0; k < 4; ++k) {  // e^{-2 pi i 2^k t / n}, [k][t]
+                const int e = (t << k) % n;
+                dct[2 * (k * G + t)] = twc[e];
+                dct[2 * (k * G + t) + 1] = tws[e];
+            }
+            dct[8 * G + 2 * t] = qc[t];  // e^{-i pi t / (2n)}
+            dct[8 * G + 2 * t + 1] = qs[t];
+        }
+        float *ef = reinterpret_cast<float *>(dct.data() + 10 * G), *gf = ef + n;
+        for (int t = 0; t < G; ++t)
+            for (int s = 0; s < P / 2; ++s)
+                for (int b = 0; b < 2; ++b) {
+                    const int i = 2 * (t + G * s) + b, entry = 2 * s + b;  // lane t's entry
+                    const int at = ((entry / 4) * G + t) * 4 + entry % 4;   // [k][t][4]
+                    const double sg = (i & 1) ? -1.0 : 1.0;
+                    ef[at] = (float)(sg * dp[i] * scale);
+                    gf[at] = (float)(sg * dp[i] * (double)n);
+                }
+    }
+    // the DCT form by default where it is the faster (n = 128: 0.22 / 0.22 ms against 0.27 /
+    // 0.35 ms for 526336 columns; n = 256: 4.45 / 4.40 ms against 7.32 / 7.63 ms for 5248000);
+    // below that the table form's products are cheap enough to keep
+    cb->form = (!dct.empty() && n >= 128) ? RAYSHT_CHEB_DCT : RAYSHT_CHEB_MATRIX;
     struct up_t {
         double **dst;
         const std::vector<double> *src;
-    } ups[4] = {{&cb->d_tt_e, &tt_e}, {&cb->d_tt_o, &tt_o}, {&cb->d_t_e, &t_e}, {&cb->d_t_o, &t_o}};
+    } ups[5] = {{&cb->d_tt_e, &tt_e}, {&cb->d_tt_o, &tt_o}, {&cb->d_t_e, &t_e}, {&cb->d_t_o, &t_o}, {&cb->d_dct, &dct}};
     for (auto &u : ups) {
+        if (u.src->empty()) continue;
         const size_t bytes = u.src->size() * sizeof(double);
         if (hipMalloc(u.dst, bytes) != hipSuccess) {
             *u.dst = nullptr;
@@ -591,7 +986,7 @@ int raysht_cheb_create(raysht_cheb **out, int n, double rmin, double rmax, doubl
 void raysht_cheb_destroy(raysht_cheb *cb)
 {
     if (!cb) return;
-    for (double *d : {cb->d_tt_e, cb->d_tt_o, cb->d_t_e, cb->d_t_o})
+    for (double *d : {cb->d_tt_e, cb->d_tt_o, cb->d_t_e, cb->d_t_o, cb->d_dct})
         if (d) (void)hipFree(d);
     delete cb;
 }
@@ -627,6 +1022,7 @@ int raysht_cheb_to_spectral(raysht_cheb *cb, const double *f, double *c, long nc
     if (!cb || ncols < 0 || (ncols > 0 && (!f || !c))) return RAYSHT_EINVAL;
     if (ncols == 0) return RAYSHT_OK;
     int rc = RAYSHT_OK;
+    if (cheb_dct_dispatch(cb, true, f, c, ncols, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
     if (cheb_reg_dispatch(cb, true, f, c, ncols, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
     const int waves = cheb_waves(cb->n), tc = 16 * waves;
     const long blocks = (ncols + tc - 1) / tc;
@@ -644,6 +1040,7 @@ int raysht_cheb_from_spectral(raysht_cheb *cb, const double *c, double *f, long 
     if (!cb || ncols < 0 || (ncols > 0 && (!f || !c))) return RAYSHT_EINVAL;
     if (ncols == 0) return RAYSHT_OK;
     int rc = RAYSHT_OK;
+    if (cheb_dct_dispatch(cb, false, c, f, ncols, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
     if (cheb_reg_dispatch(cb, false, c, f, ncols, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
     const int waves = cheb_waves(cb->n), tc = 16 * waves;
     const long blocks = (ncols + tc - 1) / tc;
@@ -653,6 +1050,14 @@ int raysht_cheb_from_spectral(raysht_cheb *cb, const double *c, double *f, long 
                        reinterpret_cast<hipStream_t>(stream), cb->n, cb->n_x, cb->n_even, cb->n_odd, cb->k4, cb->ip,
                        ncols, cb->d_t_e, cb->d_t_o, c, f);
     RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+int raysht_cheb_set_form(raysht_cheb *cb, int form)
+{
+    if (!cb || (form != RAYSHT_CHEB_DCT && form != RAYSHT_CHEB_MATRIX)) return RAYSHT_EINVAL;
+    if (form == RAYSHT_CHEB_DCT && !cb->d_dct) return RAYSHT_EUNSUPPORTED;
+    cb->form = form;
     return RAYSHT_OK;
 }
 

@@ -119,4 +119,22 @@ void fft_twiddles(int n, double *cos_out, double *sin_out)
     }
 }
 
+// The DCT form of the Chebyshev transforms: e^{-2 pi i k / n} (quarter turns exact),
+// e^{-i pi j / (2 n)}, and the distance of each accumulated zeros-grid angle theta[i] (the
+// doubles Gen_Colocation_Points produces by repeated addition) from (i + 1/2) pi / n.
+void cheb_dct_tables(int n, const double *theta, double *tw_cos, double *tw_sin, double *q_cos, double *q_sin,
+                     double *delta)
+{
+    fft_twiddles(n, tw_cos, tw_sin);
+    for (int k = 0; k < n; ++k) tw_sin[k] = -tw_sin[k];
+    for (int j = 0; j < n; ++j) {
+        const quad b = kPi * static_cast<quad>(j) / static_cast<quad>(2 * n);
+        q_cos[j] = static_cast<double>(cosq(b));
+        q_sin[j] = -static_cast<double>(sinq(b));
+    }
+    for (int i = 0; i < n; ++i)
+        delta[i] = static_cast<double>(static_cast<quad>(theta[i]) -
+                                       (static_cast<quad>(i) + 0.5Q) * kPi / static_cast<quad>(n));
+}
+
 }  // namespace raysht

@@ -119,6 +119,11 @@ struct raysht_cheb {
     int ix4 = 0, kp = 0, k4 = 0, ip = 0;
     double *d_tt_e = nullptr, *d_tt_o = nullptr, *d_t_e = nullptr, *d_t_o = nullptr;
     double deriv_scaling = 0.0;  // d/dr of the domain map, as Initialize_Cheby_Grid rounds it
+    // DCT form (power-of-two n <= 256): one device block of e^{-2 pi i k/n} [n] (double2),
+    // e^{-i pi j/(2n)} [n] (double2), the analysis correction [n] and the synthesis one [n]
+    double *d_dct = nullptr;
+    double dct_inv_s = 1.0;  // 1 / the power of two the analysis correction is scaled by
+    int form = 0;            // RAYSHT_CHEB_DCT where d_dct exists, else RAYSHT_CHEB_MATRIX
 };
 
 namespace raysht {
@@ -130,6 +135,8 @@ void gauss_legendre_dd(int n, double *x_hi, double *x_lo, double *w_hi, double *
 void gauss_legendre_binary128(int n, void *x16, void *w16);
 void plm_amplitudes_dd(int lmax, double *amp_hi, double *amp_lo);
 void fft_twiddles(int n, double *cos_out, double *sin_out);
+void cheb_dct_tables(int n, const double *theta, double *tw_cos, double *tw_sin, double *q_cos, double *q_sin,
+                     double *delta);
 
 // Radial transpose tables of a plan (radial_transpose.hip).
 int radial_plan_init(raysht_plan *p);

@@ -516,6 +516,15 @@ class ChebyGrid:
             _lib.lib().raysht_cheb_destroy(h)
             self._h = None
 
+    FORMS = {"dct": 0, "matrix": 1}
+
+    def set_form(self, form: str):
+        """"dct" (power-of-two n, 16 <= n <= 256; the default for n = 128 and 256): DCT-II / DCT-III through
+        one complex FFT per column, with the first-order correction for the accumulated
+        angles of the reference's tables; "matrix": the parity-split products with the
+        Gen_Tn tables (Chebyshev_Polynomials.F90:375-558)."""
+        _lib.check(_lib.lib().raysht_cheb_set_form(self._h, self.FORMS[form]), "raysht_cheb_set_form")
+
     def tables(self):
         nx = self.n - self.n // 2
         ce, co = np.zeros(nx * nx), np.zeros(max(nx * (self.n // 2), 1))

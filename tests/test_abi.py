@@ -50,4 +50,5 @@ def test_fft_and_cheb_reject_bad_arguments():
     h = ctypes.c_void_p()
     assert lib.raysht_cheb_create(ctypes.byref(h), 1, 0.0, 1.0, None) == 1
     assert lib.raysht_cheb_create(ctypes.byref(h), 48, 1.0, 0.5, None) == 1
+    assert lib.raysht_cheb_set_form(None, 0) == 1
     assert lib.raysht_legendre_stp(None, dummy, dummy, None) == 1

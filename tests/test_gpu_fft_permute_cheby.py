@@ -127,7 +127,9 @@ def test_full_stack_loop(ra):
 
 @pytest.mark.parametrize("n", [48, 33, 128, 256])
 def test_cheby_vs_oracle(ra, n):
+    """The table form (Chebyshev_Polynomials.F90:375-558 as written) at <= 1e-14."""
     grid = ra.ChebyGrid(n, 0.35, 1.0)
+    grid.set_form("matrix")
     assert np.array_equal(grid.grid, O.cheby_grid(n, 0.35, 1.0))
     ce, co = grid.tables()
     oe, oo = O.cheby_tables(n)
@@ -145,6 +147,62 @@ def test_cheby_vs_oracle(ra, n):
     d = dev(want)
     grid.dealias_buffer(d)
     assert np.array_equal(host(d), O.cheby_dealias(n, want))
+
+
+DCT_TOL = 3e-14  # against the table form's oracle (the table form's own rounding is ~1e-14)
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256])
+def test_cheby_dct_form(ra, n):
+    """The DCT form (the default for n = 128 and 256) against the table form's oracle, and against
+    the exact sums on the reference's own angles (long double), which it meets to 1e-15
+    where the table form, whose cosines are rounded doubles of rounded k theta_i, is up to
+    about 1e-14 away; a partial last tile (1003 columns) writes nothing past the buffer."""
+    import torch
+    grid = ra.ChebyGrid(n, 0.35, 1.0)
+    grid.set_form("dct")
+    rng = np.random.default_rng(7 * n)
+    ncols = 1003
+    f = rng.standard_normal(n * ncols)
+    sub = slice(0, n * 48)
+    buf = torch.full((n * ncols + 2 * n,), float("nan"), dtype=torch.float64, device="cuda")
+    c = buf[: n * ncols]
+    grid.to_spectral(dev(f), c)
+    got = host(c)
+    assert np.isnan(host(buf[n * ncols:])).all()
+    assert rel_max(got, O.cheby_to_spectral(n, f)) < DCT_TOL
+    assert rel_max(got[sub], O.cheby_to_spectral_exact(n, f[sub])) < 1e-15
+    buf.fill_(float("nan"))
+    grid.from_spectral(dev(got), c)
+    back = host(c)
+    assert np.isnan(host(buf[n * ncols:])).all()
+    assert rel_max(back, O.cheby_from_spectral(n, got)) < DCT_TOL
+    assert rel_max(back[sub], O.cheby_from_spectral_exact(n, got[sub])) < 1e-15
+    # the round trip: the reference's own pair returns f to 3.6e-13 at n = 256 (its tables'
+    # angles are not exactly the zeros grid's, so the pair is not exactly orthogonal)
+    assert rel_max(back, f) < 1e-12
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_cheby_dct_unaligned_and_smooth(ra, n):
+    """Columns that start 8-byte aligned take the table form; a smooth radial profile
+    (exp(-r)) keeps its rapidly decaying coefficients to the same tolerance."""
+    import torch
+    grid = ra.ChebyGrid(n, 0.35, 1.0)
+    grid.set_form("dct")
+    r = O.cheby_grid(n, 0.35, 1.0)
+    f = np.concatenate([np.exp(-r * (1 + 0.1 * k)) for k in range(40)])
+    want = O.cheby_to_spectral(n, f)
+    buf = torch.zeros(f.size + 1, dtype=torch.float64, device="cuda")
+    buf[1:].copy_(torch.from_numpy(f))
+    out = torch.zeros(f.size + 1, dtype=torch.float64, device="cuda")
+    grid.to_spectral(buf[1:], out[1:])
+    assert rel_max(host(out[1:]), want) < 1e-14
+    c = empty(f.size)
+    grid.to_spectral(dev(f), c)
+    assert rel_max(host(c), want) < DCT_TOL
+    with pytest.raises(Exception):
+        ra.ChebyGrid(48, 0.35, 1.0).set_form("dct")
 
 
 def test_cheby_vs_reference_fixture(ra, golden):

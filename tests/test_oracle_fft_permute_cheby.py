@@ -115,3 +115,36 @@ def test_cheby_dealias():
     d = O.cheby_dealias(n, c).reshape(2, n)
     rda = (2 * n) // 3 + 1
     assert np.all(d[:, : rda - 1] == 1) and np.all(d[:, rda - 1:] == 0)
+
+
+@pytest.mark.parametrize("n", [33, 48, 128, 256])
+def test_cheby_exact_sums_vs_table_form(n):
+    """The long-double sums on the reference's angles (the check for the DCT form) agree with
+    the table form, itself pinned to the compiled reference above, to the table form's own
+    rounding (rounded k theta_i and cos in every entry: up to about 1e-14 of max |c|); the
+    angles are the accumulated zeros grid, within 1e-13 of (i + 1/2) pi / n."""
+    rng = np.random.default_rng(n)
+    f = rng.standard_normal(8 * n)
+    c = O.cheby_to_spectral_exact(n, f)
+    assert np.abs(c - O.cheby_to_spectral(n, f)).max() < 2e-14 * np.abs(c).max()
+    back = O.cheby_from_spectral_exact(n, c)
+    assert np.abs(back - O.cheby_from_spectral(n, c)).max() < 2e-14 * np.abs(back).max()
+    th = O.cheby_angles_longdouble(n)
+    exact = (np.arange(n) + 0.5) * np.pi / n
+    assert np.abs(th.astype(np.float64) - exact).max() < 1e-13
+    assert th[0] == np.float64(np.pi / n * 0.5)  # the first angle is dtheta / 2 itself
+
+
+@pytest.mark.parametrize("n", [16, 64, 256])
+def test_cheby_exact_sums_on_exact_grid_are_dct(n):
+    """On the exact zeros-grid angles the long-double sums are scipy's DCT-II / DCT-III (a
+    wrong scale, a dropped half weight on c_0 or a mirrored angle fails this)."""
+    rng = np.random.default_rng(n + 1)
+    f = rng.standard_normal((3, n))
+    th = (np.arange(n, dtype=np.longdouble) + np.longdouble(0.5)) * np.longdouble(
+        "3.14159265358979323846264338327950288") / n
+    c = O.cheby_to_spectral_exact(n, f.ravel(), theta=th).reshape(3, n)
+    ref = scipy.fft.dct(f, type=2, axis=1) / n
+    assert np.abs(c - ref).max() < 1e-15 * np.abs(ref).max()
+    f2 = O.cheby_from_spectral_exact(n, ref.ravel(), theta=th).reshape(3, n)
+    assert np.abs(f2 - f).max() < 1e-14 * np.abs(f).max()
