@@ -114,6 +114,195 @@ __global__ __launch_bounds__(kLuThreads) void lu_decompose_kernel(int n, double 
     if (tid == 0) infos[blockIdx.x] = info;
 }
 
+// Blocked form of the same factorisation (n > kLuNB): panels of kLuNB columns.  Within a
+// panel, each column step is the one above (pivot, interchange of the whole row, scaling),
+// with the rank-1 update restricted to the panel's columns; the rest of the matrix then
+// takes the panel's kLuNB updates at once: U12 (the panel's rows, later columns) by forward
+// substitution with L11, and the trailing block by register tiles that apply the updates
+// k = k0 .. k0 + kLuNB - 1 one after the other, a_ij + l_ik (-u_kj), product and sum rounded
+// separately and skipped where u_kj = 0 (dger).  Every element thus sees the same
+// operations, in the same order, as in the column-by-column sweep (an interchange moves a
+// row's multipliers with it, so applying the panel's updates after the panel's
+// interchanges is the same arithmetic), and the factors are bit-identical; but the
+// trailing block is read and written once per panel instead of once per column, which
+// was 0.72 GB per matrix of n = 512.
+constexpr int kLuNB = 32;  // panel width
+constexpr int kLuT = 64;   // trailing-update chunk: kLuT x kLuT, 4 x 4 per thread
+
+__device__ __forceinline__ void lu_pivot_reduce(double &best, int &bi, double *s_val, int *s_idx)
+{
+    // first row of the largest |a|: (value, index) pairs ordered by value, then by lower index
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double v = __shfl_down(best, off, 64);
+        const int j = __shfl_down(bi, off, 64);
+        if (v > best || (v == best && j < bi)) {
+            best = v;
+            bi = j;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_val[wave] = best;
+        s_idx[wave] = bi;
+    }
+    __syncthreads();
+    best = s_val[0];
+    bi = s_idx[0];
+#pragma unroll
+    for (int w = 1; w < kLuThreads / 64; ++w) {
+        const double v = s_val[w];
+        const int j = s_idx[w];
+        if (v > best || (v == best && j < bi)) {
+            best = v;
+            bi = j;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kLuThreads) void lu_decompose_blocked_kernel(int n, double *__restrict__ mats,
+                                                                          int *__restrict__ pivots,
+                                                                          int *__restrict__ infos)
+{
+#pragma clang fp contract(off)
+    __shared__ double s_val[kLuThreads / 64];
+    __shared__ int s_idx[kLuThreads / 64];
+    __shared__ double rowk[kLuNB];            // the pivot row within the panel
+    __shared__ double s_l[kLuNB][kLuT];       // L21 chunk [k][row]
+    __shared__ double s_u[kLuNB][kLuT];       // U12 chunk [k][column]
+    __shared__ double s_l11[kLuNB][kLuNB + 1];  // L11 [i][k]
+    double *a = mats + (long)blockIdx.x * n * n;
+    int *piv = pivots + (long)blockIdx.x * n;
+    const int tid = threadIdx.x;
+    const double sfmin = 2.2250738585072014e-308;
+    int info = 0;
+    for (int k0 = 0; k0 < n; k0 += kLuNB) {
+        const int kend = min(n, k0 + kLuNB);
+        // ---- the panel, column by column
+        for (int k = k0; k < kend; ++k) {
+            double best = -1.0;
+            int bi = n;
+            for (int i = k + tid; i < n; i += kLuThreads) {
+                const double v = fabs(a[i + (long)k * n]);
+                if (v > best) {
+                    best = v;
+                    bi = i;
+                }
+            }
+            lu_pivot_reduce(best, bi, s_val, s_idx);
+            // idamax: a NaN a_kk is kept, as is row k of a column of NaNs
+            const int p = (bi >= n || isnan(a[k + (long)k * n])) ? k : bi;
+            const double pv = a[p + (long)k * n];
+            __syncthreads();  // every thread has read the pivot before the swap moves it
+            if (tid == 0) piv[k] = p + 1;
+            if (pv != 0.0) {
+                if (p != k)
+                    for (int j = tid; j < n; j += kLuThreads) {
+                        const double t = a[k + (long)j * n];
+                        a[k + (long)j * n] = a[p + (long)j * n];
+                        a[p + (long)j * n] = t;
+                    }
+            } else if (info == 0) {
+                info = k + 1;
+            }
+            __syncthreads();
+            if (tid < kend - k - 1) rowk[tid] = a[k + (long)(k + 1 + tid) * n];
+            __syncthreads();
+            const double r = pv != 0.0 && fabs(pv) >= sfmin ? 1.0 / pv : 0.0;
+            for (int i = k + 1 + tid; i < n; i += kLuThreads) {
+                double lik = a[i + (long)k * n];
+                if (pv != 0.0) {
+                    lik = fabs(pv) >= sfmin ? lik * r : lik / pv;
+                    a[i + (long)k * n] = lik;
+                }
+                double *ai = a + i;
+                for (int j = k + 1; j < kend; ++j) {
+                    const double y = rowk[j - k - 1];
+                    if (y != 0.0) ai[(long)j * n] = ai[(long)j * n] + lik * -y;
+                }
+            }
+            __syncthreads();
+        }
+        if (kend >= n) break;
+        // ---- U12: rows k0 .. k0 + kLuNB - 1 of the later columns, forward substitution
+        for (int e = tid; e < kLuNB * kLuNB; e += kLuThreads) {
+            const int i = e % kLuNB, kk = e / kLuNB;
+            s_l11[i][kk] = a[k0 + i + (long)(k0 + kk) * n];
+        }
+        __syncthreads();
+        // a half wave per four columns: lane i % 32 holds row k0 + i of each, and step kk takes
+        // u_kk from lane kk of its half (the row-kk value is final once steps 0 .. kk - 1 are
+        // done); the four columns' shuffle chains are independent, so their latencies overlap
+        for (int j0 = kend + 4 * (tid / 32); j0 < n; j0 += 4 * (kLuThreads / 32)) {
+            const int i = tid % 32, half = tid & 32;
+            double u[4];
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) u[cc] = j0 + cc < n ? a[k0 + i + (long)(j0 + cc) * n] : 0.0;
+            for (int kk = 0; kk < kLuNB - 1; ++kk) {
+                const double lik = s_l11[i][kk];
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    const double y = __shfl(u[cc], half | kk, 64);
+                    const double upd = u[cc] + lik * -y;
+                    u[cc] = (i > kk && y != 0.0) ? upd : u[cc];
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc)
+                if (j0 + cc < n) a[k0 + i + (long)(j0 + cc) * n] = u[cc];
+        }
+        __syncthreads();
+        // ---- the trailing block, kLuT x kLuT chunks
+        const int tr = tid / 16, tc = tid % 16;
+        for (int r0 = kend; r0 < n; r0 += kLuT) {
+            for (int e = tid; e < kLuNB * kLuT; e += kLuThreads) {
+                const int row = e % kLuT, kk = e / kLuT;
+                s_l[kk][row] = r0 + row < n ? a[r0 + row + (long)(k0 + kk) * n] : 0.0;
+            }
+            for (int c0 = kend; c0 < n; c0 += kLuT) {
+                for (int e = tid; e < kLuNB * kLuT; e += kLuThreads) {
+                    const int col = e / kLuNB, kk = e % kLuNB;
+                    s_u[kk][col] = c0 + col < n ? a[k0 + kk + (long)(c0 + col) * n] : 0.0;
+                }
+                __syncthreads();
+                const int i0 = r0 + 4 * tr, j0 = c0 + 4 * tc;
+                double t[4][4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr)
+                        t[rr][c] = (i0 + rr < n && j0 + c < n) ? a[i0 + rr + (long)(j0 + c) * n] : 0.0;
+#pragma unroll 4
+                for (int kk = 0; kk < kLuNB; ++kk) {
+                    double l[4], y[4];
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr) l[rr] = s_l[kk][4 * tr + rr];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) y[c] = s_u[kk][4 * tc + c];
+                    // dger skips a zero u_kj: a select, not a branch per column
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const bool nz = y[c] != 0.0;
+#pragma unroll
+                        for (int rr = 0; rr < 4; ++rr) {
+                            const double upd = t[rr][c] + l[rr] * -y[c];
+                            t[rr][c] = nz ? upd : t[rr][c];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int rr = 0; rr < 4; ++rr)
+                        if (i0 + rr < n && j0 + c < n) a[i0 + rr + (long)(j0 + c) * n] = t[rr][c];
+                __syncthreads();  // the chunk's U12 and (at the row chunk's end) L21 are free again
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) infos[blockIdx.x] = info;
+}
+
 // Solve with the right-hand sides in registers: G lanes of a wave own CPL columns, lane
 // r holds rows r, r + G, ... (NT of them) of each.  The interchanges are composed into one
 // permutation applied when the column is loaded; then each step j takes x_j from its
@@ -456,8 +645,12 @@ int raysht_lu_decompose_full(int n, int nbatch, double *mats, int *pivots, int *
     if (n < 1 || nbatch < 0 || (nbatch > 0 && (!mats || !pivots || !info))) return RAYSHT_EINVAL;
     if (n > kLuMaxN) return RAYSHT_EUNSUPPORTED;
     if (nbatch == 0) return RAYSHT_OK;
-    hipLaunchKernelGGL(lu_decompose_kernel, dim3((unsigned)nbatch), dim3(kLuThreads), 0,
-                       reinterpret_cast<hipStream_t>(stream), n, mats, pivots, info);
+    if (n > kLuNB)
+        hipLaunchKernelGGL(lu_decompose_blocked_kernel, dim3((unsigned)nbatch), dim3(kLuThreads), 0,
+                           reinterpret_cast<hipStream_t>(stream), n, mats, pivots, info);
+    else
+        hipLaunchKernelGGL(lu_decompose_kernel, dim3((unsigned)nbatch), dim3(kLuThreads), 0,
+                           reinterpret_cast<hipStream_t>(stream), n, mats, pivots, info);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
