@@ -27,6 +27,34 @@ constexpr int kLuThreads = 256;
 constexpr int kLuMaxN = 1024;
 constexpr int kLuBatch = 16;  // row loads in flight per thread in the factorisation sweep (4: 1.3x slower)
 
+// Workgroup order of the solves.  Workgroups go to the 8 XCDs round-robin by linear id, so
+// XCD k runs ids k, k + 8, k + 16, ...; its r-th workgroup takes column block r % ncb of
+// matrix 8 (r / ncb) + k.  Each XCD then works through its own matrices (every eighth)
+// one at a time, all column blocks of a matrix together, and the factor they share stays
+// in that XCD's L2 instead of every XCD streaming every matrix from the last-level cache.
+// The grid is ncb * (nbatch rounded up to 8) workgroups; ids past nbatch leave at once.
+// Ungrouped (grouped = 0), matrix b runs as workgroups b, b + nbatch, ...: for small
+// factors (dense n <= 256, 0.5 MB) measured faster, as the cache lines the waves read at
+// one step spread over many matrices.
+struct solve_block {
+    int b, cb;
+};
+__device__ __forceinline__ solve_block solve_block_of(int ncb, int nbatch, int grouped)
+{
+    if (!grouped) return solve_block{(int)(blockIdx.x % (unsigned)nbatch), (int)(blockIdx.x / (unsigned)nbatch)};
+    const int r = (int)(blockIdx.x >> 3);
+    return solve_block{8 * (r / ncb) + (int)(blockIdx.x & 7), r % ncb};
+}
+inline long solve_grid(int ncb, int nbatch, int grouped)
+{
+    return (long)ncb * (grouped ? (nbatch + 7L) / 8 * 8 : (long)nbatch);
+}
+// a launch stays below 2^32 threads in all
+inline bool solve_grid_fits(int ncb, int nbatch, int grouped, int threads)
+{
+    return solve_grid(ncb, nbatch, grouped) * threads < (1L << 32);
+}
+
 __global__ __launch_bounds__(kLuThreads) void lu_decompose_kernel(int n, double *__restrict__ mats,
                                                                   int *__restrict__ pivots, int *__restrict__ infos)
 {
@@ -310,7 +338,8 @@ __global__ __launch_bounds__(kLuThreads) void lu_decompose_blocked_kernel(int n,
 // column of L (or U) is loaded -- no barrier after the permutation.  Operation order and
 // rounding are those of the reference BLAS order.
 template <int G, int NT, int CPL>
-__global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const double *__restrict__ mats,
+__global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, int nbatch, int ncb, int grouped,
+                                                                    const double *__restrict__ mats,
                                                                     const int *__restrict__ pivots,
                                                                     double *__restrict__ rhs,
                                                                     const long *__restrict__ rhs_off,
@@ -319,7 +348,9 @@ __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const
 #pragma clang fp contract(off)
     __shared__ int perm[G * NT];
     constexpr int kCols = kLuThreads / G * CPL;
-    const int b = blockIdx.x, ncol = nrhs[b], col0 = blockIdx.y * kCols;
+    const solve_block sb = solve_block_of(ncb, nbatch, grouped);
+    if (sb.b >= nbatch) return;
+    const int b = sb.b, ncol = nrhs[b], col0 = sb.cb * kCols;
     if (col0 >= ncol) return;  // uniform: the whole workgroup leaves
     const double *a = mats + (long)b * n * n;
     const int *piv = pivots + (long)b * n;
@@ -415,13 +446,15 @@ __global__ __launch_bounds__(kLuThreads) void lu_solve_lanes_kernel(int n, const
 // CPL right-hand sides per lane group share every loaded column of the factor (measured:
 // 2 for 16-lane groups, 3 for 32- and 64-lane groups, within the register budget)
 template <int G, int NT, int CPL>
-void launch_lanes(int n, int nbatch, const double *mats, const int *pivots, double *rhs, const long *rhs_off,
-                  const int *nrhs, int max_nrhs, hipStream_t st)
+int launch_lanes(int n, int nbatch, const double *mats, const int *pivots, double *rhs, const long *rhs_off,
+                 const int *nrhs, int max_nrhs, hipStream_t st)
 {
     constexpr int kCols = kLuThreads / G * CPL;
-    const dim3 grid((unsigned)nbatch, (unsigned)((max_nrhs + kCols - 1) / kCols));
-    hipLaunchKernelGGL((lu_solve_lanes_kernel<G, NT, CPL>), grid, dim3(kLuThreads), 0, st, n, mats, pivots, rhs,
-                       rhs_off, nrhs);
+    const int ncb = (max_nrhs + kCols - 1) / kCols, grouped = n > 256;
+    if (!solve_grid_fits(ncb, nbatch, grouped, kLuThreads)) return RAYSHT_EUNSUPPORTED;
+    hipLaunchKernelGGL((lu_solve_lanes_kernel<G, NT, CPL>), dim3((unsigned)solve_grid(ncb, nbatch, grouped)),
+                       dim3(kLuThreads), 0, st, n, nbatch, ncb, grouped, mats, pivots, rhs, rhs_off, nrhs);
+    return RAYSHT_OK;
 }
 
 }  // namespace
@@ -526,33 +559,59 @@ __global__ __launch_bounds__(kLuThreads) void band_lu_decompose_kernel(int n, in
 // by its CPL columns: the L solve (interchange, then dger's x(r) + l (-x(j)), skipped where
 // x(j) = 0) and dtbsv's upper solve (x(j) / u(j, j), then x(r) - x(j) u(r, j) for the kl + ku
 // rows above, skipped where x(j) = 0).  Factor entry (r, j) of both is band row kv + r - j.
+// The slots rotate: while step j runs over the 64 rows of segment S = j / 64, slot k of a
+// lane holds absolute slot (S + k) mod NS, so row j is always in slot 0 and every slot index
+// in the loop bodies is a compile-time register index (a select over NS slots per access
+// was most of the kernel's instructions).  Segments advance by one rotation of the NS
+// slots, and NS rotations restore the natural order.
 template <int NS>
-__device__ __forceinline__ double band_row(const double (&x)[NS], int row)
+__device__ __forceinline__ void slots_rotate_down(double (&x)[NS])  // slot k takes slot k + 1
 {
-    double v = 0.0;
+    const double x0 = x[0];
 #pragma unroll
-    for (int s = 0; s < NS; ++s)
-        if (s == (row >> 6)) v = x[s];
+    for (int k = 0; k + 1 < NS; ++k) x[k] = x[k + 1];
+    x[NS - 1] = x0;
+}
+
+template <int NS>
+__device__ __forceinline__ void slots_rotate_up(double (&x)[NS])  // slot k takes slot k - 1
+{
+    const double xl = x[NS - 1];
+#pragma unroll
+    for (int k = NS - 1; k > 0; --k) x[k] = x[k - 1];
+    x[0] = xl;
+}
+
+// value of row `row`, in relative slot (row >> 6) - S (0 .. NS - 1 - S)
+template <int NS>
+__device__ __forceinline__ double band_row(const double (&x)[NS], int rel, int row)
+{
+    double v = x[0];
+#pragma unroll
+    for (int k = 1; k < NS; ++k)
+        if (k == rel) v = x[k];
     return __shfl(v, row & 63);
 }
 
 template <int NS>
-__device__ __forceinline__ void band_set(double (&x)[NS], int row, int lane, double v)
+__device__ __forceinline__ void band_set(double (&x)[NS], int rel, int row, int lane, double v)
 {
 #pragma unroll
-    for (int s = 0; s < NS; ++s)
-        if (s == (row >> 6) && lane == (row & 63)) x[s] = v;
+    for (int k = 0; k < NS; ++k)
+        if (k == rel && lane == (row & 63)) x[k] = v;
 }
 
 template <int NS, int CPL>
-__global__ __launch_bounds__(64) void band_lu_solve_kernel(int n, int kl, int ku, int ldab,
+__global__ __launch_bounds__(64) void band_lu_solve_kernel(int n, int kl, int ku, int ldab, int nbatch, int ncb,
                                                            const double *__restrict__ mats,
                                                            const int *__restrict__ pivots, double *__restrict__ rhs,
                                                            const long *__restrict__ rhs_off,
                                                            const int *__restrict__ nrhs)
 {
 #pragma clang fp contract(off)
-    const int b = blockIdx.y, lane = threadIdx.x, c0 = blockIdx.x * CPL;
+    const solve_block sb = solve_block_of(ncb, nbatch, 1);
+    if (sb.b >= nbatch) return;
+    const int b = sb.b, lane = threadIdx.x, c0 = sb.cb * CPL;
     const int nc = min(CPL, nrhs[b] - c0);
     if (nc <= 0) return;
     const double *ab = mats + (long)b * ldab * n;
@@ -567,57 +626,71 @@ __global__ __launch_bounds__(64) void band_lu_solve_kernel(int n, int kl, int ku
             const int r = s * 64 + lane;
             x[c][s] = (c < nc && r < n) ? x0[(long)c * n + r] : 0.0;
         }
-    if (kl > 0)
-        for (int j = 0; j < n - 1; ++j) {
-            const int lm = min(kl, n - 1 - j), l = piv[j] - 1;
+    // L solve with the interchanges, segment by segment (row j in slot 0)
+#pragma unroll 1
+    for (int S = 0; S < NS; ++S) {
+        const int jend = kl > 0 ? min(64 * (S + 1), n - 1) : 0;
+        for (int j = 64 * S; j < jend; ++j) {
+            const int jj = j & 63, lm = min(kl, n - 1 - j), l = piv[j] - 1, lrel = (l >> 6) - S;
             double y[CPL];
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
                 if (l != j) {
-                    const double xl = band_row(x[c], l), xj = band_row(x[c], j);
-                    band_set(x[c], j, lane, xl);
-                    band_set(x[c], l, lane, xj);
+                    const double xl = band_row(x[c], lrel, l), xj = __shfl(x[c][0], jj);
+                    if (lane == jj) x[c][0] = xl;
+                    band_set(x[c], lrel, l, lane, xj);
                 }
-                y[c] = band_row(x[c], j);
+                y[c] = __shfl(x[c][0], jj);
             }
-            const int slo = (j + 1) >> 6, shi = (j + lm) >> 6;
+            const int klo = ((j + 1) >> 6) - S, khi = ((j + lm) >> 6) - S;
             const double *col = ab + (long)j * ldab + kv - j;  // col[r] = factor entry (r, j)
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                if (s < slo || s > shi) continue;
-                const int r = s * 64 + lane;
+            for (int k = 0; k < NS; ++k) {
+                if (k < klo || k > khi) continue;
+                const int r = (S + k) * 64 + lane;
                 if (r <= j || r > j + lm) continue;
                 const double a = col[r];
 #pragma unroll
                 for (int c = 0; c < CPL; ++c)
-                    if (y[c] != 0.0) x[c][s] = x[c][s] + a * (-y[c]);
+                    if (y[c] != 0.0) x[c][k] = x[c][k] + a * (-y[c]);
             }
         }
-    for (int j = n - 1; j >= 0; --j) {
-        const double *col = ab + (long)j * ldab + kv - j;
-        const double d = col[j];
-        double t[CPL];
-        bool nz[CPL];  // dtbsv skips a column where x(j) is 0 before the division
 #pragma unroll
-        for (int c = 0; c < CPL; ++c) {
-            t[c] = band_row(x[c], j);
-            nz[c] = t[c] != 0.0;
-            if (nz[c]) {
-                t[c] = t[c] / d;
-                band_set(x[c], j, lane, t[c]);
+        for (int c = 0; c < CPL; ++c) slots_rotate_down(x[c]);
+    }
+    // U solve, segments from the last down (row j in slot 0, rows above in slots NS - 1, ...)
+#pragma unroll 1
+    for (int S = NS - 1; S >= 0; --S) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) slots_rotate_up(x[c]);
+        for (int j = min(64 * S + 63, n - 1); j >= 64 * S; --j) {
+            const int jj = j & 63;
+            const double *col = ab + (long)j * ldab + kv - j;
+            const double d = col[j];
+            double t[CPL];
+            bool nz[CPL];  // dtbsv skips a column where x(j) is 0 before the division
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                t[c] = __shfl(x[c][0], jj);
+                nz[c] = t[c] != 0.0;
+                if (nz[c]) {
+                    t[c] = t[c] / d;
+                    if (lane == jj) x[c][0] = t[c];
+                }
             }
-        }
-        const int lo = max(0, j - kv), slo = lo >> 6, shi = (j - 1) >> 6;
-        if (j == 0) continue;
+            if (j == 0) continue;
+            const int lo = max(0, j - kv), slo = lo >> 6, shi = (j - 1) >> 6;
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            if (s < slo || s > shi) continue;
-            const int r = s * 64 + lane;
-            if (r < lo || r >= j) continue;
-            const double a = col[r];
+            for (int k = 0; k < NS; ++k) {
+                const int sa = k == 0 ? S : S - (NS - k);  // absolute slot of relative slot k
+                if (sa < slo || sa > shi) continue;
+                const int r = sa * 64 + lane;
+                if (r < lo || r >= j) continue;
+                const double a = col[r];
 #pragma unroll
-            for (int c = 0; c < CPL; ++c)
-                if (nz[c]) x[c][s] = x[c][s] - t[c] * a;
+                for (int c = 0; c < CPL; ++c)
+                    if (nz[c]) x[c][k] = x[c][k] - t[c] * a;
+            }
         }
     }
 #pragma unroll
@@ -630,12 +703,14 @@ __global__ __launch_bounds__(64) void band_lu_solve_kernel(int n, int kl, int ku
 }
 
 template <int NS, int CPL>
-void launch_band_solve(int n, int kl, int ku, int nbatch, const double *mats, int ldab, const int *pivots,
-                       double *rhs, const long *rhs_off, const int *nrhs, int max_nrhs, hipStream_t st)
+int launch_band_solve(int n, int kl, int ku, int nbatch, const double *mats, int ldab, const int *pivots,
+                      double *rhs, const long *rhs_off, const int *nrhs, int max_nrhs, hipStream_t st)
 {
-    dim3 grid((unsigned)((max_nrhs + CPL - 1) / CPL), (unsigned)nbatch);
-    hipLaunchKernelGGL((band_lu_solve_kernel<NS, CPL>), grid, dim3(64), 0, st, n, kl, ku, ldab, mats, pivots, rhs,
-                       rhs_off, nrhs);
+    const int ncb = (max_nrhs + CPL - 1) / CPL;
+    if (!solve_grid_fits(ncb, nbatch, 1, 64)) return RAYSHT_EUNSUPPORTED;
+    hipLaunchKernelGGL((band_lu_solve_kernel<NS, CPL>), dim3((unsigned)solve_grid(ncb, nbatch, 1)), dim3(64), 0, st, n,
+                       kl, ku, ldab, nbatch, ncb, mats, pivots, rhs, rhs_off, nrhs);
+    return RAYSHT_OK;
 }
 
 extern "C" {
@@ -663,16 +738,18 @@ int raysht_lu_solve_full(int n, int nbatch, const double *mats, const int *pivot
     if (n > kLuMaxN) return RAYSHT_EUNSUPPORTED;
     if (nbatch == 0 || max_nrhs == 0) return RAYSHT_OK;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    int rc;
     if (n <= 64)
-        launch_lanes<16, 4, 2>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        rc = launch_lanes<16, 4, 2>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else if (n <= 128)
-        launch_lanes<16, 8, 2>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        rc = launch_lanes<16, 8, 2>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else if (n <= 256)
-        launch_lanes<16, 16, 2>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        rc = launch_lanes<16, 16, 2>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else if (n <= 512)
-        launch_lanes<32, 16, 3>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        rc = launch_lanes<32, 16, 3>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else
-        launch_lanes<64, 16, 3>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        rc = launch_lanes<64, 16, 3>(n, nbatch, mats, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+    if (rc != RAYSHT_OK) return rc;
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
@@ -696,19 +773,21 @@ int raysht_lu_solve_band(int n, int kl, int ku, int nbatch, const double *mats, 
 {
     if (n < 1 || kl < 0 || ku < 0 || nbatch < 0 || max_nrhs < 0 || ldab < 2 * kl + ku + 1) return RAYSHT_EINVAL;
     if (nbatch > 0 && (!mats || !pivots || !rhs || !rhs_off || !nrhs)) return RAYSHT_EINVAL;
-    if (nbatch > 65535 || n > kLuMaxN) return RAYSHT_EUNSUPPORTED;
+    if (n > kLuMaxN) return RAYSHT_EUNSUPPORTED;
     if (nbatch == 0 || max_nrhs == 0) return RAYSHT_OK;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    int rc;
     if (n <= 64)
-        launch_band_solve<1, 8>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        rc = launch_band_solve<1, 8>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else if (n <= 128)
-        launch_band_solve<2, 8>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        rc = launch_band_solve<2, 8>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else if (n <= 256)
-        launch_band_solve<4, 4>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        rc = launch_band_solve<4, 4>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else if (n <= 512)
-        launch_band_solve<8, 4>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        rc = launch_band_solve<8, 4>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
     else
-        launch_band_solve<16, 2>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+        rc = launch_band_solve<16, 2>(n, kl, ku, nbatch, mats, ldab, pivots, rhs, rhs_off, nrhs, max_nrhs, st);
+    if (rc != RAYSHT_OK) return rc;
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
