@@ -301,6 +301,21 @@ def cpu_reference_extras(L, mv, nr, nf, nlm, n_lu, ncols, out):
         res["rlm_spaceb"] = {"ms_full_size": t * 1e3, "cores": 1, "kind": "reference",
                              "sample": "W, P, Z at the full size (a three-field buffer; the reference's "
                                        "d_by_sdtheta / d_by_dphi and rlm_spaceb's array statements)"}
+        # rlm_spacea's hydro steps and Compute_BandCurlB on a 16-field buffer, a radial sample
+        nrs = min(nr, 8)
+        R.angular_init(L, mv, nrs)
+        buf = rng.standard_normal(O.spectral_size(L, mv, nrs, 16))
+        prof = {k: rng.uniform(0.5, 2.0, nrs) for k in O.HYDRO_PROFILES}
+        for key, call in (("rlm_spacea_hydro", lambda: R.rlm_spacea_hydro(buf, 16, list(range(10)), prof)),
+                          ("compute_bandcurlb", lambda: R.compute_bandcurlb(buf, 16, list(range(10, 16)),
+                                                                            prof["over_rhorsq"]))):
+            t0 = time.perf_counter()
+            call()
+            t = time.perf_counter() - t0
+            res[key] = {"ms_full_size": t * (nr / nrs) * 1e3, "cores": 1, "kind": "reference",
+                        "sample": f"{nrs} of {nr} radii of a 16-field buffer, scaled (the reference's "
+                                  "d_by_dtheta / d_by_dphi and the routines' array statements)"}
+        R.angular_init(L, mv, nr)
     # the phi FFTs: numpy's pocketfft (FFTW, the reference's library, is not available) on
     # a sample of lines, one core, scaled to every (r, theta, field) line
     nphi = 3 * (L + 1)
@@ -524,6 +539,28 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
     ms = e0.elapsed_time(e1) / steps
     out["rlm_spaceb_hydro"] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9, "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
     del w8
+    # rlm_spacea's passes before StP on a 16-field buffer of the same spectral layout: the
+    # hydro pass (Velocity_Components + Velocity_Derivatives + dtdt: 6 fields read, 9 written,
+    # 120 B per coefficient of a field) on fields 0-9, Compute_BandCurlB (5 read, 6 written,
+    # 88 B) on fields 10-15; repeated in place on the same buffer (the time of these
+    # passes does not depend on the values)
+    b16 = torch.randn(16 * per_field, dtype=torch.float64, device="cuda")
+    prof = {k: torch.rand(nr, dtype=torch.float64, device="cuda") + 0.5 for k in plan.SPACEA_PROFILES}
+    out["rlm_spacea"] = {}
+    for name, fn, nb in (("hydro", lambda: plan.rlm_spacea_hydro(b16, list(range(10)), prof), 120.0 * per_field),
+                         ("bandcurlb", lambda: plan.compute_bandcurlb(b16, list(range(10, 16)), prof["over_rhorsq"]),
+                          88.0 * per_field)):
+        for _ in range(max(warmup, 1)):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out["rlm_spacea"][name] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9, "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
+    del b16
     # one field scaled by l(l+1) and a radial profile (rlm_spacea's l_l_plus1 * f * Over_RhoRSQ):
     # 16 B per coefficient
     lfac = torch.from_numpy(ra.l_l_plus1(L)).cuda()

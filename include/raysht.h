@@ -177,6 +177,26 @@ int raysht_angular_deriv_host(raysht_plan *plan, int kind, const double *const *
  * are the doubles the reference's sequence of d_by_sdtheta / d_by_dphi calls and array
  * statements gives. */
 int raysht_rlm_spaceb_hydro(raysht_plan *plan, double *buf, int nf, int wvar, int pvar, int zvar, void *stream);
+/* rlm_spacea's Velocity_Components and Velocity_Derivatives, then its d_by_dtheta(tvar, dtdt)
+ * (Physics/Sphere_Hybrid_Space.F90:80,235-355), in one pass, in place on the packed spectral
+ * buffer `buf` of nf fields before StP.  fields (host array of 10 distinct 0-based fields) =
+ * W, dWdr, d2Wdr2, Z, dZdr, T, vtheta, vphi, dvtdr, dvpdr; vr, dvrdr, dtdt and dvrdt are written
+ * over W, dWdr, d2Wdr2 and dZdr (Fields.F90:323-326).  Radial profiles (device arrays of N_r
+ * values, Hybrid_Init :45-62 and ProblemSize.F90:624): over_rhor, over_rhorsq, drho_term,
+ * two_over_r (Two_Over_R), dlnrho (ref%dlnrho), dsdr_over_cp (ref%dsdr_over_cp; NULL when not
+ * pseudo-incompressible).  The l_max zeroing around it in rlm_spacea (:69-72, :95-98) is
+ * raysht_zero_lmax.  Results are the doubles of the reference's sequence of d_by_dtheta /
+ * d_by_dphi calls and array statements. */
+int raysht_rlm_spacea_hydro(raysht_plan *plan, double *buf, int nf, const int *fields, const double *over_rhor,
+                            const double *over_rhorsq, const double *drho_term, const double *two_over_r,
+                            const double *dlnrho, const double *dsdr_over_cp, void *stream);
+/* Compute_BandCurlB (Physics/Sphere_Hybrid_Space.F90:377-460) in one pass, in place on `buf`
+ * (nf fields).  fields (host array of 6 distinct 0-based fields) = C, A, dCdr, dAdr, d2Cdr2,
+ * curlbr; Br, Btheta, Bphi, curlbtheta and curlbphi are written over C, A, dCdr, d2Cdr2 and
+ * dAdr (Fields.F90:329-334).  one_over_rsq = OneOverRSquared (device, N_r values).  Results
+ * are the doubles of the reference's sequence. */
+int raysht_compute_bandcurlb(raysht_plan *plan, double *buf, int nf, const int *fields, const double *one_over_rsq,
+                             void *stream);
 /* The l(l+1) and radial scalings of rlm_spacea / rlm_spaceb (Velocity_Components,
  * Velocity_Derivatives, Compute_BandCurlB: Sphere_Hybrid_Space.F90:244,324,353,367,373) on one
  * field: out(l, r, imi) = (lfac(l) in(l, r, imi)) rfac(r), rounded in that order; lfac

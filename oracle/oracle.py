@@ -72,6 +72,8 @@ def oracle_lib():
         lib.orc_rlm_spaceb.argtypes = [_I, _P, _I, _I, _P, _I, _I, _I, _I]
         lib.orc_lm_scale.argtypes = [_I, _P, _I, _I, _P, _I, _I, _P, _I, _I, _P, _P]
         lib.orc_zero_lmax.argtypes = [_I, _P, _I, _I, _P, _I]
+        lib.orc_rlm_spacea_hydro.argtypes = [_I, _P, _I, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P]
+        lib.orc_compute_bandcurlb.argtypes = [_I, _P, _I, _I, _P, _I, _P, _P]
         lib.orc_lu_decompose.restype = _I
         lib.orc_lu_decompose.argtypes = [_I, _P, _P]
         lib.orc_lu_solve.argtypes = [_I, _P, _P, _I, _P]
@@ -120,6 +122,8 @@ def ref_lib():
         lib.ref_angular_init.argtypes = [_I, _P, _I, _I]
         lib.ref_angular_deriv.argtypes = [_I, _I, _I, _I, _P, _P]
         lib.ref_rlm_spaceb.argtypes = [_I, _I, _I, _I, _P]
+        lib.ref_rlm_spacea_hydro.argtypes = [_I, _P, _I, _P, _P, _P, _P, _P, _P, _P]
+        lib.ref_compute_bandcurlb.argtypes = [_I, _P, _P, _P]
         _ref = lib
     return _ref
 
@@ -295,6 +299,37 @@ def rlm_spaceb(lmax, mvals, nr, buf, nf, wv, pv, zv):
     mv = np.ascontiguousarray(mvals, dtype=np.int32)
     b = np.ascontiguousarray(buf, dtype=np.float64).copy()
     oracle_lib().orc_rlm_spaceb(lmax, _ptr(mv), len(mv), nr, _ptr(b), nf, wv, pv, zv)
+    return b
+
+
+# radial profiles of the rlm_spacea steps, in the order the entry points take them
+HYDRO_PROFILES = ("over_rhor", "over_rhorsq", "drho_term", "two_over_r", "dlnrho", "dsdr_over_cp")
+
+
+def rlm_spacea_hydro(lmax, mvals, nr, buf, nf, fields, prof):
+    """Sphere_Hybrid_Space.F90:235-355 Velocity_Components + Velocity_Derivatives and :80
+    d_by_dtheta(tvar, dtdt) on a copy of the packed nf-field buffer; fields = 0-based (W, dWdr,
+    d2Wdr2, Z, dZdr, T, vtheta, vphi, dvtdr, dvpdr); prof = dict of the HYDRO_PROFILES arrays
+    (N_r values each; dsdr_over_cp None unless pseudo-incompressible).  Returns the new buffer."""
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    b = np.ascontiguousarray(buf, dtype=np.float64).copy()
+    v = np.ascontiguousarray(fields, dtype=np.int32)
+    assert v.size == 10
+    arrs = [None if prof.get(k) is None else np.ascontiguousarray(prof[k], dtype=np.float64) for k in HYDRO_PROFILES]
+    oracle_lib().orc_rlm_spacea_hydro(lmax, _ptr(mv), len(mv), nr, _ptr(b), nf, _ptr(v),
+                                      *[None if x is None else _ptr(x) for x in arrs])
+    return b
+
+
+def compute_bandcurlb(lmax, mvals, nr, buf, nf, fields, one_over_rsq):
+    """Sphere_Hybrid_Space.F90:377-460 Compute_BandCurlB on a copy of the packed nf-field buffer;
+    fields = 0-based (C, A, dCdr, dAdr, d2Cdr2, curlbr).  Returns the new buffer."""
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    b = np.ascontiguousarray(buf, dtype=np.float64).copy()
+    v = np.ascontiguousarray(fields, dtype=np.int32)
+    assert v.size == 6
+    o2 = np.ascontiguousarray(one_over_rsq, dtype=np.float64)
+    oracle_lib().orc_compute_bandcurlb(lmax, _ptr(mv), len(mv), nr, _ptr(b), nf, _ptr(v), _ptr(o2))
     return b
 
 
@@ -588,6 +623,26 @@ class Reference:
         copy of the buffer (fields 1-based); returns the new buffer."""
         b = np.ascontiguousarray(buf, dtype=np.float64).copy()
         self.lib.ref_rlm_spaceb(nf, wv, pv, zv, _ptr(b))
+        return b
+
+    def rlm_spacea_hydro(self, buf: np.ndarray, nf: int, fields, prof) -> np.ndarray:
+        """Velocity_Components, Velocity_Derivatives and d_by_dtheta(tvar, dtdt) through the
+        reference's own d_by_dtheta / d_by_dphi on a copy of the buffer (fields 0-based, as
+        rlm_spacea_hydro above); returns the new buffer."""
+        b = np.ascontiguousarray(buf, dtype=np.float64).copy()
+        v = np.ascontiguousarray(np.asarray(fields) + 1, dtype=np.int32)
+        pseudo = prof.get("dsdr_over_cp") is not None
+        arrs = [np.ascontiguousarray(prof[k] if prof.get(k) is not None else np.zeros(1), dtype=np.float64)
+                for k in HYDRO_PROFILES]
+        self.lib.ref_rlm_spacea_hydro(nf, _ptr(v), int(pseudo), *[_ptr(x) for x in arrs], _ptr(b))
+        return b
+
+    def compute_bandcurlb(self, buf: np.ndarray, nf: int, fields, one_over_rsq) -> np.ndarray:
+        """Compute_BandCurlB through the reference's own derivatives (fields 0-based)."""
+        b = np.ascontiguousarray(buf, dtype=np.float64).copy()
+        v = np.ascontiguousarray(np.asarray(fields) + 1, dtype=np.int32)
+        o2 = np.ascontiguousarray(one_over_rsq, dtype=np.float64)
+        self.lib.ref_compute_bandcurlb(nf, _ptr(v), _ptr(o2), _ptr(b))
         return b
 
     def cheby_deriv(self, c: np.ndarray, dorder: int, nobuffer: bool = False) -> np.ndarray:

@@ -962,6 +962,177 @@ void orc_zero_lmax(int lmax, const int *mvals, int nm, int nr, double *buf, int 
     }
 }
 
+/* l_l_plus1 (ProblemSize.F90:506-513): l (l + 1) from the double l */
+static double l_l_plus1(int l)
+{
+    const double ell = (double)l;
+    return ell * (ell + 1.0);
+}
+
+/* The elementwise statements of the rlm_spacea routines on one field-sized slab (every
+ * m-block, radius, imi, l).  e is the index of (mp, imi, r, l) in a one-field buffer. */
+typedef struct {
+    int lmax, nm, nr;
+    const int *mvals;
+} slab;
+
+static double *fld(const slab *g, double *buf, int nf, int f, int mp)
+{
+    const int nl = g->lmax - g->mvals[mp] + 1;
+    return buf + field_block(g->lmax, g->mvals, mp, g->nr, nf) + (long)f * nl * 2 * g->nr;
+}
+
+/* field f = (l_l_plus1(l) field g) rfac(r)   (SBUFFA(IDX2,f) = l_l_plus1(m:l_max)*SBUFFA(IDX2,g)*rfac(r)) */
+static void st_lscale(const slab *g, double *buf, int nf, int fo, int fi, const double *rfac)
+{
+    for (int mp = 0; mp < g->nm; ++mp) {
+        const int m = g->mvals[mp], nl = g->lmax - m + 1;
+        double *o = fld(g, buf, nf, fo, mp), *a = fld(g, buf, nf, fi, mp);
+        for (int imi = 0; imi < 2; ++imi)
+            for (int r = 0; r < g->nr; ++r)
+                for (int l = m; l <= g->lmax; ++l) {
+                    const long e = ((long)imi * g->nr + r) * nl + l - m;
+                    o[e] = l_l_plus1(l) * a[e] * rfac[r];
+                }
+    }
+}
+
+/* field f = field f - field v rfac(r)   (SBUFFA(IDX2,f) = SBUFFA(IDX2,f) - SBUFFA(IDX2,v)*rfac(r)) */
+static void st_sub_scaled(const slab *g, double *buf, int nf, int f, int v, const double *rfac)
+{
+    for (int mp = 0; mp < g->nm; ++mp) {
+        const int m = g->mvals[mp], nl = g->lmax - m + 1;
+        double *o = fld(g, buf, nf, f, mp), *a = fld(g, buf, nf, v, mp);
+        for (int imi = 0; imi < 2; ++imi)
+            for (int r = 0; r < g->nr; ++r)
+                for (int i = 0; i < nl; ++i) {
+                    const long e = ((long)imi * g->nr + r) * nl + i;
+                    o[e] = o[e] - (rfac ? a[e] * rfac[r] : a[e]);
+                }
+    }
+}
+
+/* one-field slab t times rfac(r) into field f   (SBUFFA(IDX2,f) = ftemp1(mp)%data(IDX2)*rfac(r)) */
+static void st_store_scaled(const slab *g, double *buf, int nf, int f, const double *t, const double *rfac)
+{
+    for (int mp = 0; mp < g->nm; ++mp) {
+        const int m = g->mvals[mp], nl = g->lmax - m + 1;
+        double *o = fld(g, buf, nf, f, mp);
+        const double *a = t + field_block(g->lmax, g->mvals, mp, g->nr, 1);
+        for (int imi = 0; imi < 2; ++imi)
+            for (int r = 0; r < g->nr; ++r)
+                for (int i = 0; i < nl; ++i) {
+                    const long e = ((long)imi * g->nr + r) * nl + i;
+                    o[e] = rfac ? a[e] * rfac[r] : a[e];
+                }
+    }
+}
+
+/* t1 = t1 + sg t2 on one-field slabs (sg = +1 or -1) */
+static void st_combine(long n, double *t1, const double *t2, int sg)
+{
+    for (long e = 0; e < n; ++e) t1[e] = sg > 0 ? t1[e] + t2[e] : t1[e] - t2[e];
+}
+
+/* Sphere_Hybrid_Space.F90:235-355 Velocity_Components then Velocity_Derivatives, then
+ * rlm_spacea's d_by_dtheta(tvar, dtdt) (:80), in place on a packed buffer of nf fields, step for
+ * step with two one-field temporaries.  v = 0-based (W, dWdr, d2Wdr2, Z, dZdr, T, vtheta, vphi,
+ * dvtdr, dvpdr); vr, dvrdr, dtdt and dvrdt are the W, dWdr, d2Wdr2 and dZdr fields
+ * (Fields.F90:323-326).  dsdr_over_cp NULL: not pseudo-incompressible. */
+void orc_rlm_spacea_hydro(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, const int *v,
+                          const double *over_rhor, const double *over_rhorsq, const double *drho_term,
+                          const double *two_over_r, const double *dlnrho, const double *dsdr_over_cp)
+{
+    const slab g = {lmax, nm, nr, mvals};
+    const int W = v[0], dW = v[1], d2W = v[2], Z = v[3], dZ = v[4], T = v[5];
+    const int vt = v[6], vp = v[7], dvt = v[8], dvp = v[9];
+    const long fsz = field_block(lmax, mvals, nm, nr, 1);
+    double *t1 = (double *)malloc(sizeof(double) * (fsz + 1));
+    double *t2 = (double *)malloc(sizeof(double) * (fsz + 1));
+    /* Velocity_Components */
+    st_lscale(&g, buf, nf, W, W, over_rhorsq);
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, buf, nf, dW, t1, 1, 0);
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, Z, t2, 1, 0);
+    st_combine(fsz, t1, t2, +1);
+    st_store_scaled(&g, buf, nf, vt, t1, over_rhor);
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, dW, t1, 1, 0);
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, buf, nf, Z, t2, 1, 0);
+    st_combine(fsz, t1, t2, -1);
+    st_store_scaled(&g, buf, nf, vp, t1, over_rhor);
+    /* Velocity_Derivatives */
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, buf, nf, d2W, t1, 1, 0);
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, dZ, t2, 1, 0);
+    st_combine(fsz, t1, t2, +1);
+    st_store_scaled(&g, buf, nf, dvt, t1, over_rhor);
+    st_sub_scaled(&g, buf, nf, dvt, vt, drho_term);
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, d2W, t1, 1, 0);
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, buf, nf, dZ, t2, 1, 0);
+    st_combine(fsz, t1, t2, -1);
+    st_store_scaled(&g, buf, nf, dvp, t1, over_rhor);
+    st_sub_scaled(&g, buf, nf, dvp, vp, drho_term);
+    st_lscale(&g, buf, nf, dW, dW, over_rhorsq);
+    st_sub_scaled(&g, buf, nf, dW, W, two_over_r);
+    st_sub_scaled(&g, buf, nf, dW, W, dlnrho);
+    if (dsdr_over_cp) st_sub_scaled(&g, buf, nf, dW, W, dsdr_over_cp);
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, buf, nf, W, buf, nf, dZ);
+    st_lscale(&g, buf, nf, Z, Z, over_rhorsq);
+    /* rlm_spacea: d_by_dtheta(wsp%s2a, tvar, dtdt) */
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, buf, nf, T, buf, nf, d2W);
+    free(t1);
+    free(t2);
+}
+
+/* Sphere_Hybrid_Space.F90:377-460 Compute_BandCurlB, in place on a packed buffer of nf fields,
+ * step for step.  v = 0-based (C, A, dCdr, dAdr, d2Cdr2, curlbr); Br, Btheta, Bphi,
+ * curlbtheta and curlbphi are the C, A, dCdr, d2Cdr2 and dAdr fields (Fields.F90:329-334);
+ * one_over_rsq = OneOverRSquared. */
+void orc_compute_bandcurlb(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, const int *v,
+                           const double *one_over_rsq)
+{
+    const slab g = {lmax, nm, nr, mvals};
+    const int C = v[0], A = v[1], dC = v[2], dA = v[3], d2C = v[4], cbr = v[5];
+    const long fsz = field_block(lmax, mvals, nm, nr, 1);
+    double *t1 = (double *)malloc(sizeof(double) * (fsz + 1));
+    double *t2 = (double *)malloc(sizeof(double) * (fsz + 1));
+    st_lscale(&g, buf, nf, C, C, one_over_rsq);      /* Br */
+    st_lscale(&g, buf, nf, cbr, A, one_over_rsq);    /* [curl B]_r */
+    st_sub_scaled(&g, buf, nf, d2C, C, NULL);        /* d2cdr2 - Br */
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, buf, nf, dA, t1, 1, 0);
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, dA, t2, 1, 0);
+    st_store_scaled(&g, buf, nf, dA, t2, NULL);
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, buf, nf, d2C, t2, 1, 0);
+    {   /* curlbphi = curlbphi + ftemp2 */
+        for (int mp = 0; mp < nm; ++mp) {
+            const int nl = lmax - mvals[mp] + 1;
+            double *o = fld(&g, buf, nf, dA, mp);
+            const double *a = t2 + field_block(lmax, mvals, mp, nr, 1);
+            for (long e = 0; e < 2L * nr * nl; ++e) o[e] = o[e] + a[e];
+        }
+    }
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, d2C, t2, 1, 0);
+    st_combine(fsz, t1, t2, -1);
+    st_store_scaled(&g, buf, nf, d2C, t1, NULL);     /* curlbtheta */
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, buf, nf, A, t1, 1, 0);
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, A, t2, 1, 0);
+    st_store_scaled(&g, buf, nf, A, t2, NULL);
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, buf, nf, dC, t2, 1, 0);
+    {   /* Btheta = A + ftemp2 */
+        for (int mp = 0; mp < nm; ++mp) {
+            const int nl = lmax - mvals[mp] + 1;
+            double *o = fld(&g, buf, nf, A, mp);
+            const double *a = t2 + field_block(lmax, mvals, mp, nr, 1);
+            for (long e = 0; e < 2L * nr * nl; ++e) o[e] = o[e] + a[e];
+        }
+    }
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, dC, t2, 1, 0);
+    {   /* Bphi = ftemp2 - ftemp1 */
+        for (long e = 0; e < fsz; ++e) t2[e] = t2[e] - t1[e];
+        st_store_scaled(&g, buf, nf, dC, t2, NULL);
+    }
+    free(t1);
+    free(t2);
+}
+
 /* LU_Decompose_band (Linear_Solve.F90:1041-1055): LAPACK dgbtrf on an n x n band matrix in
  * LAPACK band storage ab(ldab, n), ldab >= 2 kl + ku + 1 (the reference passes kl = ku =
  * (ldab - 1) / 3), restated as the unblocked dgbtf2 that dgbtrf runs for ku <= 64: fill-in rows

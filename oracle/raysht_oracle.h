@@ -69,6 +69,11 @@ void orc_rlm_spaceb(int lmax, const int *mvals, int nm, int nr, double *buf, int
 void orc_lm_scale(int lmax, const int *mvals, int nm, int nr, const double *in, int nf_in, int fin, double *out,
                   int nf_out, int fout, const double *lfac, const double *rfac);
 void orc_zero_lmax(int lmax, const int *mvals, int nm, int nr, double *buf, int nf);
+void orc_rlm_spacea_hydro(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, const int *v,
+                          const double *over_rhor, const double *over_rhorsq, const double *drho_term,
+                          const double *two_over_r, const double *dlnrho, const double *dsdr_over_cp);
+void orc_compute_bandcurlb(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, const int *v,
+                           const double *one_over_rsq);
 int orc_lu_decompose(int n, double *a, int *piv);
 void orc_lu_solve(int n, const double *a, const int *piv, int nrhs, double *b);
 int orc_band_lu_decompose(int n, int kl, int ku, double *ab, int ldab, int *piv);

@@ -390,4 +390,188 @@ Contains
         DeAllocate(a, t1, t2, ol)
     End Subroutine ref_rlm_spaceb
 
+    ! rlm_spacea's Velocity_Components and Velocity_Derivatives, then d_by_dtheta(tvar, dtdt)
+    ! (Physics/Sphere_Hybrid_Space.F90:80,235-355; the module needs the whole physics state,
+    ! so the harness issues the same calls of the reference's d_by_dtheta / d_by_dphi and the
+    ! same array statements itself) on a packed buffer of nf fields, in place.  v = 1-based
+    ! (W, dWdr, d2Wdr2, Z, dZdr, T, vtheta, vphi, dvtdr, dvpdr); vr, dvrdr, dtdt and dvrdt are
+    ! the W, dWdr, d2Wdr2 and dZdr fields (Fields.F90:323-326).  pseudo /= 0 adds the
+    ! dsdr_over_cp term.
+    Subroutine ref_rlm_spacea_hydro(nf, v, pseudo, over_rhor, over_rhorsq, drho_term, two_over_r, dlnrho, &
+            dsdr_over_cp, buf) bind(C, name='ref_rlm_spacea_hydro')
+        Integer(C_INT), Value :: nf, pseudo
+        Integer(C_INT), Intent(In) :: v(10)
+        Real(C_DOUBLE), Intent(In) :: over_rhor(*), over_rhorsq(*), drho_term(*), two_over_r(*), dlnrho(*)
+        Real(C_DOUBLE), Intent(In) :: dsdr_over_cp(*)
+        Real(C_DOUBLE), Intent(InOut) :: buf(*)
+        Type(rmcontainer4d), Allocatable :: a(:)
+        Type(rmcontainer3d), Allocatable :: ftemp1(:), ftemp2(:)
+        Real*8, Allocatable :: l_l_plus1(:)
+        Real*8 :: ell
+        Integer :: mp, m, l, r, imi, lm
+        Integer :: vr, dwdr, d2wdr2, zvar, dzdr, tvar, vtheta, vphi, dvtdr, dvpdr, dvrdr, dvrdt, dtdt
+        vr = v(1); dwdr = v(2); d2wdr2 = v(3); zvar = v(4); dzdr = v(5); tvar = v(6)
+        vtheta = v(7); vphi = v(8); dvtdr = v(9); dvpdr = v(10)
+        dvrdr = dwdr; dtdt = d2wdr2; dvrdt = dzdr
+        lm = h_ang_lmax
+        Allocate(l_l_plus1(0:lm))
+        Do l = 0, lm
+            ell = l*1.0d0
+            l_l_plus1(l) = ell*(ell+1)
+        Enddo
+        Call unpack4d(buf, nf, a)
+        Call alloc3d(ftemp1)
+        Call alloc3d(ftemp2)
+        ! Velocity_Components
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            Do imi = 1, 2
+                Do r = 1, h_ang_nr
+                    a(mp)%data(m:lm,r,imi,vr) = l_l_plus1(m:lm)*a(mp)%data(m:lm,r,imi,vr)*over_rhorsq(r)
+                Enddo
+            Enddo
+        Enddo
+        Call d_by_dtheta(a, dwdr, ftemp1)
+        Call d_by_dphi(a, zvar, ftemp2)
+        Do mp = 1, h_ang_nm
+            ftemp1(mp)%data = ftemp1(mp)%data+ftemp2(mp)%data
+            Do r = 1, h_ang_nr
+                a(mp)%data(:,r,:,vtheta) = ftemp1(mp)%data(:,r,:)*over_rhor(r)
+            Enddo
+        Enddo
+        Call d_by_dphi(a, dwdr, ftemp1)
+        Call d_by_dtheta(a, zvar, ftemp2)
+        Do mp = 1, h_ang_nm
+            ftemp1(mp)%data = ftemp1(mp)%data-ftemp2(mp)%data
+            Do r = 1, h_ang_nr
+                a(mp)%data(:,r,:,vphi) = ftemp1(mp)%data(:,r,:)*over_rhor(r)
+            Enddo
+        Enddo
+        ! Velocity_Derivatives
+        Call d_by_dtheta(a, d2wdr2, ftemp1)
+        Call d_by_dphi(a, dzdr, ftemp2)
+        Do mp = 1, h_ang_nm
+            ftemp1(mp)%data = ftemp1(mp)%data+ftemp2(mp)%data
+            Do r = 1, h_ang_nr
+                a(mp)%data(:,r,:,dvtdr) = ftemp1(mp)%data(:,r,:)*over_rhor(r)
+            Enddo
+            Do r = 1, h_ang_nr
+                a(mp)%data(:,r,:,dvtdr) = a(mp)%data(:,r,:,dvtdr)-a(mp)%data(:,r,:,vtheta)*drho_term(r)
+            Enddo
+        Enddo
+        Call d_by_dphi(a, d2wdr2, ftemp1)
+        Call d_by_dtheta(a, dzdr, ftemp2)
+        Do mp = 1, h_ang_nm
+            ftemp1(mp)%data = ftemp1(mp)%data-ftemp2(mp)%data
+            Do r = 1, h_ang_nr
+                a(mp)%data(:,r,:,dvpdr) = ftemp1(mp)%data(:,r,:)*over_rhor(r)
+            Enddo
+            Do r = 1, h_ang_nr
+                a(mp)%data(:,r,:,dvpdr) = a(mp)%data(:,r,:,dvpdr)-a(mp)%data(:,r,:,vphi)*drho_term(r)
+            Enddo
+        Enddo
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            Do imi = 1, 2
+                Do r = 1, h_ang_nr
+                    a(mp)%data(m:lm,r,imi,dvrdr) = l_l_plus1(m:lm)*a(mp)%data(m:lm,r,imi,dvrdr)*over_rhorsq(r)
+                Enddo
+            Enddo
+            Do r = 1, h_ang_nr
+                a(mp)%data(:,r,:,dvrdr) = a(mp)%data(:,r,:,dvrdr)-a(mp)%data(:,r,:,vr)*two_over_r(r)
+            Enddo
+            Do r = 1, h_ang_nr
+                a(mp)%data(:,r,:,dvrdr) = a(mp)%data(:,r,:,dvrdr)-a(mp)%data(:,r,:,vr)*dlnrho(r)
+            Enddo
+            If (pseudo /= 0) Then
+                Do r = 1, h_ang_nr
+                    a(mp)%data(:,r,:,dvrdr) = a(mp)%data(:,r,:,dvrdr)-a(mp)%data(:,r,:,vr)*dsdr_over_cp(r)
+                Enddo
+            Endif
+        Enddo
+        Call d_by_dtheta(a, vr, dvrdt)
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            Do imi = 1, 2
+                Do r = 1, h_ang_nr
+                    a(mp)%data(m:lm,r,imi,zvar) = l_l_plus1(m:lm)*a(mp)%data(m:lm,r,imi,zvar)*over_rhorsq(r)
+                Enddo
+            Enddo
+        Enddo
+        ! rlm_spacea
+        Call d_by_dtheta(a, tvar, dtdt)
+        Call pack4d(a, nf, buf)
+        DeAllocate(a, ftemp1, ftemp2, l_l_plus1)
+    End Subroutine ref_rlm_spacea_hydro
+
+    ! Compute_BandCurlB (Physics/Sphere_Hybrid_Space.F90:377-460) through the reference's
+    ! d_by_dtheta / d_by_dphi and the same array statements, in place on a packed buffer of nf
+    ! fields.  v = 1-based (C, A, dCdr, dAdr, d2Cdr2, curlbr); Br, Btheta, Bphi, curlbtheta and
+    ! curlbphi are the C, A, dCdr, d2Cdr2 and dAdr fields (Fields.F90:329-334).
+    Subroutine ref_compute_bandcurlb(nf, v, one_over_rsq, buf) bind(C, name='ref_compute_bandcurlb')
+        Integer(C_INT), Value :: nf
+        Integer(C_INT), Intent(In) :: v(6)
+        Real(C_DOUBLE), Intent(In) :: one_over_rsq(*)
+        Real(C_DOUBLE), Intent(InOut) :: buf(*)
+        Type(rmcontainer4d), Allocatable :: a(:)
+        Type(rmcontainer3d), Allocatable :: ftemp1(:), ftemp2(:)
+        Real*8, Allocatable :: l_l_plus1(:)
+        Real*8 :: ell
+        Integer :: mp, m, l, r, imi, lm
+        Integer :: cvar, avar, dcdr, dadr, d2cdr2, curlbr, br, btheta, bphi, curlbtheta, curlbphi
+        cvar = v(1); avar = v(2); dcdr = v(3); dadr = v(4); d2cdr2 = v(5); curlbr = v(6)
+        br = cvar; btheta = avar; bphi = dcdr; curlbtheta = d2cdr2; curlbphi = dadr
+        lm = h_ang_lmax
+        Allocate(l_l_plus1(0:lm))
+        Do l = 0, lm
+            ell = l*1.0d0
+            l_l_plus1(l) = ell*(ell+1)
+        Enddo
+        Call unpack4d(buf, nf, a)
+        Call alloc3d(ftemp1)
+        Call alloc3d(ftemp2)
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            Do imi = 1, 2
+                Do r = 1, h_ang_nr
+                    a(mp)%data(m:lm,r,imi,br) = l_l_plus1(m:lm)*a(mp)%data(m:lm,r,imi,br)*one_over_rsq(r)
+                Enddo
+            Enddo
+            Do imi = 1, 2
+                Do r = 1, h_ang_nr
+                    a(mp)%data(m:lm,r,imi,curlbr) = l_l_plus1(m:lm)*a(mp)%data(m:lm,r,imi,avar)*one_over_rsq(r)
+                Enddo
+            Enddo
+            a(mp)%data(:,:,:,d2cdr2) = a(mp)%data(:,:,:,d2cdr2)-a(mp)%data(:,:,:,br)
+        Enddo
+        Call d_by_dtheta(a, dadr, ftemp1)
+        Call d_by_dphi(a, dadr, ftemp2)
+        Do mp = 1, h_ang_nm
+            a(mp)%data(:,:,:,dadr) = ftemp2(mp)%data
+        Enddo
+        Call d_by_dtheta(a, d2cdr2, ftemp2)
+        Do mp = 1, h_ang_nm
+            a(mp)%data(:,:,:,curlbphi) = a(mp)%data(:,:,:,curlbphi)+ftemp2(mp)%data
+        Enddo
+        Call d_by_dphi(a, d2cdr2, ftemp2)
+        Do mp = 1, h_ang_nm
+            a(mp)%data(:,:,:,curlbtheta) = ftemp1(mp)%data-ftemp2(mp)%data
+        Enddo
+        Call d_by_dtheta(a, avar, ftemp1)
+        Call d_by_dphi(a, avar, ftemp2)
+        Do mp = 1, h_ang_nm
+            a(mp)%data(:,:,:,avar) = ftemp2(mp)%data
+        Enddo
+        Call d_by_dtheta(a, dcdr, ftemp2)
+        Do mp = 1, h_ang_nm
+            a(mp)%data(:,:,:,avar) = a(mp)%data(:,:,:,avar)+ftemp2(mp)%data
+        Enddo
+        Call d_by_dphi(a, dcdr, ftemp2)
+        Do mp = 1, h_ang_nm
+            a(mp)%data(:,:,:,dcdr) = ftemp2(mp)%data-ftemp1(mp)%data
+        Enddo
+        Call pack4d(a, nf, buf)
+        DeAllocate(a, ftemp1, ftemp2, l_l_plus1)
+    End Subroutine ref_compute_bandcurlb
+
 End Module raysht_ref_harness

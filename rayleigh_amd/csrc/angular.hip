@@ -200,6 +200,258 @@ __global__ __launch_bounds__(kAThreads) void rlm_spaceb_kernel(int lmax, int nr,
     }
 }
 
+// l - 1 and l + 1 neighbours of a per-lane value (lane = position in the 64-value l chunk):
+// from the adjacent lanes; lane 63 takes `next` (the first value of the next chunk, not yet
+// overwritten), lane 0 `carry` (the last value of the previous chunk, kept from before the
+// wave overwrote it); `carry` then becomes this chunk's last value.
+struct lnb {
+    double p, n;
+};
+__device__ __forceinline__ lnb l_neighbours(double x, double next, double &carry, int lane)
+{
+    // both shuffles by every lane: a lane permute reads nothing from a lane that is not
+    // executing it, so the shuffle must not sit under the lane test
+    lnb v;
+    v.n = __shfl_down(x, 1);
+    v.p = __shfl_up(x, 1);
+    if (lane == 63) v.n = next;
+    if (lane == 0) v.p = carry;
+    carry = __shfl(x, 63);
+    return v;
+}
+
+// d_by_dtheta at position i of an l-column of nl values, as dtheta_kernel rounds it
+__device__ __forceinline__ double dth_at(lnb v, double2 c, int i, int nl)
+{
+#pragma clang fp contract(off)  // the pragma of the calling kernel does not reach an inlined callee
+    if (nl == 1) return 0.0;
+    if (i == 0) return v.n * c.y;
+    if (i == nl - 1) return v.p * c.x;
+    return v.p * c.x + v.n * c.y;
+}
+
+__device__ __forceinline__ double l_l_plus1(int l)
+{
+#pragma clang fp contract(off)
+    const double ell = (double)l;
+    return ell * (ell + 1.0);
+}
+
+// field roles of the rlm_spacea passes (0-based fields of one packed buffer)
+struct spacea_hydro_fields {
+    int w, dw, d2w, z, dz, t, vt, vp, dvt, dvp;
+};
+struct bandcurlb_fields {
+    int c, a, dc, da, d2c, cbr;
+};
+
+// rlm_spacea's Velocity_Components and Velocity_Derivatives, then d_by_dtheta(tvar, dtdt)
+// (Sphere_Hybrid_Space.F90:80,235-355), in place, one pass.  With L = l(l+1), D = d_by_dtheta,
+// P = d_by_dphi ((re, im) -> (-(m im), m re)) and the radial factors of radius r:
+//   vr = (L W) over_rhorsq;  vtheta = (D dWdr + P Z) over_rhor;  vphi = (P dWdr - D Z) over_rhor;
+//   dvtdr = (D d2Wdr2 + P dZdr) over_rhor - vtheta drho_term;
+//   dvpdr = (P d2Wdr2 - D dZdr) over_rhor - vphi drho_term;
+//   dvrdr = (L dWdr) over_rhorsq - vr two_over_r - vr dlnrho [- vr dsdr_over_cp];
+//   dvrdt = D vr;  Z = (L Z) over_rhorsq;  dtdt = D T;
+// vr, dvrdr, dtdt and dvrdt replace W, dWdr, d2Wdr2 and dZdr.  Every product and sum is
+// rounded where the reference's statements round it, so the results are its doubles.  One
+// wave owns one radius of one m-block (both imi of every field it reads or writes), as in
+// rlm_spaceb_kernel.
+__global__ __launch_bounds__(kAThreads) void rlm_spacea_hydro_kernel(
+    int lmax, int nr, const int *__restrict__ mvals, const int *__restrict__ lm_base, const double2 *__restrict__ dcoef,
+    double *buf, int nf, spacea_hydro_fields f, const double *__restrict__ over_rhor,
+    const double *__restrict__ over_rhorsq, const double *__restrict__ drho_term,
+    const double *__restrict__ two_over_r, const double *__restrict__ dlnrho,
+    const double *__restrict__ dsdr_over_cp)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * (kAThreads / 64) + (threadIdx.x >> 6);
+    if (r >= nr) return;  // whole waves
+    const int mp = blockIdx.y, m = mvals[mp], nl = lmax - m + 1;
+    const long lb = lm_base[mp], half = (long)nl * nr, fsz = 2 * half;
+    double *blk = buf + lb * 2L * nr * nf + (long)r * nl;
+    double *W = blk + f.w * fsz, *dW = blk + f.dw * fsz, *d2W = blk + f.d2w * fsz, *Z = blk + f.z * fsz;
+    double *dZ = blk + f.dz * fsz, *T = blk + f.t * fsz;
+    double *Vt = blk + f.vt * fsz, *Vp = blk + f.vp * fsz, *dVt = blk + f.dvt * fsz, *dVp = blk + f.dvp * fsz;
+    const double2 *c = dcoef + lb;
+    const double dm = (double)m, orr = over_rhor[r], orsq = over_rhorsq[r], drho = drho_term[r];
+    const double tor = two_over_r[r], dl = dlnrho[r], ds = dsdr_over_cp ? dsdr_over_cp[r] : 0.0;
+    // carries for lane 0: dWdr, Z, d2Wdr2, dZdr, T and vr (re, im)
+    double k_dwr = 0.0, k_dwi = 0.0, k_zr = 0.0, k_zi = 0.0, k_d2r = 0.0, k_d2i = 0.0;
+    double k_dzr = 0.0, k_dzi = 0.0, k_tr = 0.0, k_ti = 0.0, k_vrr = 0.0, k_vri = 0.0;
+    for (int base = 0; base < nl; base += 64) {
+        const int i = base + lane, l = m + i;
+        const bool in = i < nl;
+        double wr = 0.0, wi = 0.0, dwr = 0.0, dwi = 0.0, d2r = 0.0, d2i = 0.0;
+        double zr = 0.0, zi = 0.0, dzr = 0.0, dzi = 0.0, tr = 0.0, ti = 0.0;
+        double2 ci = make_double2(0.0, 0.0);
+        if (in) {
+            wr = W[i];
+            wi = W[half + i];
+            dwr = dW[i];
+            dwi = dW[half + i];
+            d2r = d2W[i];
+            d2i = d2W[half + i];
+            zr = Z[i];
+            zi = Z[half + i];
+            dzr = dZ[i];
+            dzi = dZ[half + i];
+            tr = T[i];
+            ti = T[half + i];
+            ci = c[i];
+        }
+        // the first values of the next chunk (lane 63 only)
+        double n_wr = 0.0, n_wi = 0.0, n_dwr = 0.0, n_dwi = 0.0, n_d2r = 0.0, n_d2i = 0.0;
+        double n_zr = 0.0, n_zi = 0.0, n_dzr = 0.0, n_dzi = 0.0, n_tr = 0.0, n_ti = 0.0;
+        if (lane == 63 && i + 1 < nl) {
+            n_wr = W[i + 1];
+            n_wi = W[half + i + 1];
+            n_dwr = dW[i + 1];
+            n_dwi = dW[half + i + 1];
+            n_d2r = d2W[i + 1];
+            n_d2i = d2W[half + i + 1];
+            n_zr = Z[i + 1];
+            n_zi = Z[half + i + 1];
+            n_dzr = dZ[i + 1];
+            n_dzi = dZ[half + i + 1];
+            n_tr = T[i + 1];
+            n_ti = T[half + i + 1];
+        }
+        const double L = l_l_plus1(l);
+        const double vrr = L * wr * orsq, vri = L * wi * orsq;
+        const double Ln = l_l_plus1(l + 1);
+        const lnb b_dwr = l_neighbours(dwr, n_dwr, k_dwr, lane), b_dwi = l_neighbours(dwi, n_dwi, k_dwi, lane);
+        const lnb b_zr = l_neighbours(zr, n_zr, k_zr, lane), b_zi = l_neighbours(zi, n_zi, k_zi, lane);
+        const lnb b_d2r = l_neighbours(d2r, n_d2r, k_d2r, lane), b_d2i = l_neighbours(d2i, n_d2i, k_d2i, lane);
+        const lnb b_dzr = l_neighbours(dzr, n_dzr, k_dzr, lane), b_dzi = l_neighbours(dzi, n_dzi, k_dzi, lane);
+        const lnb b_tr = l_neighbours(tr, n_tr, k_tr, lane), b_ti = l_neighbours(ti, n_ti, k_ti, lane);
+        const lnb b_vrr = l_neighbours(vrr, Ln * n_wr * orsq, k_vrr, lane);
+        const lnb b_vri = l_neighbours(vri, Ln * n_wi * orsq, k_vri, lane);
+        if (!in) continue;
+        const double vtr = (dth_at(b_dwr, ci, i, nl) + -(dm * zi)) * orr;
+        const double vti = (dth_at(b_dwi, ci, i, nl) + dm * zr) * orr;
+        const double vpr = (-(dm * dwi) - dth_at(b_zr, ci, i, nl)) * orr;
+        const double vpi = (dm * dwr - dth_at(b_zi, ci, i, nl)) * orr;
+        double dvtr = (dth_at(b_d2r, ci, i, nl) + -(dm * dzi)) * orr;
+        double dvti = (dth_at(b_d2i, ci, i, nl) + dm * dzr) * orr;
+        dvtr = dvtr - vtr * drho;
+        dvti = dvti - vti * drho;
+        double dvpr = (-(dm * d2i) - dth_at(b_dzr, ci, i, nl)) * orr;
+        double dvpi = (dm * d2r - dth_at(b_dzi, ci, i, nl)) * orr;
+        dvpr = dvpr - vpr * drho;
+        dvpi = dvpi - vpi * drho;
+        double dvrr = L * dwr * orsq, dvri = L * dwi * orsq;
+        dvrr = dvrr - vrr * tor;
+        dvri = dvri - vri * tor;
+        dvrr = dvrr - vrr * dl;
+        dvri = dvri - vri * dl;
+        if (dsdr_over_cp) {
+            dvrr = dvrr - vrr * ds;
+            dvri = dvri - vri * ds;
+        }
+        W[i] = vrr;
+        W[half + i] = vri;
+        Vt[i] = vtr;
+        Vt[half + i] = vti;
+        Vp[i] = vpr;
+        Vp[half + i] = vpi;
+        dVt[i] = dvtr;
+        dVt[half + i] = dvti;
+        dVp[i] = dvpr;
+        dVp[half + i] = dvpi;
+        dW[i] = dvrr;
+        dW[half + i] = dvri;
+        dZ[i] = dth_at(b_vrr, ci, i, nl);
+        dZ[half + i] = dth_at(b_vri, ci, i, nl);
+        Z[i] = L * zr * orsq;
+        Z[half + i] = L * zi * orsq;
+        d2W[i] = dth_at(b_tr, ci, i, nl);
+        d2W[half + i] = dth_at(b_ti, ci, i, nl);
+    }
+}
+
+// Compute_BandCurlB (Sphere_Hybrid_Space.F90:377-460), in place, one pass.  With L, D, P as
+// above and o = OneOverRSquared(r), e = d2Cdr2 - Br:
+//   Br = (L C) o;  [curl B]_r = (L A) o;  curlbphi = P dAdr + D e;  curlbtheta = D dAdr - P e;
+//   Btheta = P A + D dCdr;  Bphi = P dCdr - D A;
+// Br, Btheta, Bphi, curlbtheta and curlbphi replace C, A, dCdr, d2Cdr2 and dAdr.  Rounded
+// where the reference rounds; one wave per radius of one m-block.
+__global__ __launch_bounds__(kAThreads) void bandcurlb_kernel(int lmax, int nr, const int *__restrict__ mvals,
+                                                              const int *__restrict__ lm_base,
+                                                              const double2 *__restrict__ dcoef, double *buf, int nf,
+                                                              bandcurlb_fields f,
+                                                              const double *__restrict__ one_over_rsq)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * (kAThreads / 64) + (threadIdx.x >> 6);
+    if (r >= nr) return;  // whole waves
+    const int mp = blockIdx.y, m = mvals[mp], nl = lmax - m + 1;
+    const long lb = lm_base[mp], half = (long)nl * nr, fsz = 2 * half;
+    double *blk = buf + lb * 2L * nr * nf + (long)r * nl;
+    double *C = blk + f.c * fsz, *A = blk + f.a * fsz, *dC = blk + f.dc * fsz, *dA = blk + f.da * fsz;
+    double *d2C = blk + f.d2c * fsz, *Cbr = blk + f.cbr * fsz;
+    const double2 *c = dcoef + lb;
+    const double dm = (double)m, o = one_over_rsq[r];
+    double k_dar = 0.0, k_dai = 0.0, k_er = 0.0, k_ei = 0.0, k_dcr = 0.0, k_dci = 0.0, k_ar = 0.0, k_ai = 0.0;
+    for (int base = 0; base < nl; base += 64) {
+        const int i = base + lane, l = m + i;
+        const bool in = i < nl;
+        double cr = 0.0, cim = 0.0, ar = 0.0, ai = 0.0, dcr = 0.0, dci = 0.0, dar = 0.0, dai = 0.0;
+        double d2r = 0.0, d2i = 0.0;
+        double2 ci = make_double2(0.0, 0.0);
+        if (in) {
+            cr = C[i];
+            cim = C[half + i];
+            ar = A[i];
+            ai = A[half + i];
+            dcr = dC[i];
+            dci = dC[half + i];
+            dar = dA[i];
+            dai = dA[half + i];
+            d2r = d2C[i];
+            d2i = d2C[half + i];
+            ci = c[i];
+        }
+        double n_cr = 0.0, n_ci = 0.0, n_ar = 0.0, n_ai = 0.0, n_dcr = 0.0, n_dci = 0.0, n_dar = 0.0, n_dai = 0.0;
+        double n_d2r = 0.0, n_d2i = 0.0;
+        if (lane == 63 && i + 1 < nl) {
+            n_cr = C[i + 1];
+            n_ci = C[half + i + 1];
+            n_ar = A[i + 1];
+            n_ai = A[half + i + 1];
+            n_dcr = dC[i + 1];
+            n_dci = dC[half + i + 1];
+            n_dar = dA[i + 1];
+            n_dai = dA[half + i + 1];
+            n_d2r = d2C[i + 1];
+            n_d2i = d2C[half + i + 1];
+        }
+        const double L = l_l_plus1(l), Ln = l_l_plus1(l + 1);
+        const double brr = L * cr * o, bri = L * cim * o;
+        const double er = d2r - brr, ei = d2i - bri;
+        const lnb b_dar = l_neighbours(dar, n_dar, k_dar, lane), b_dai = l_neighbours(dai, n_dai, k_dai, lane);
+        const lnb b_er = l_neighbours(er, n_d2r - Ln * n_cr * o, k_er, lane);
+        const lnb b_ei = l_neighbours(ei, n_d2i - Ln * n_ci * o, k_ei, lane);
+        const lnb b_dcr = l_neighbours(dcr, n_dcr, k_dcr, lane), b_dci = l_neighbours(dci, n_dci, k_dci, lane);
+        const lnb b_ar = l_neighbours(ar, n_ar, k_ar, lane), b_ai = l_neighbours(ai, n_ai, k_ai, lane);
+        if (!in) continue;
+        C[i] = brr;
+        C[half + i] = bri;
+        Cbr[i] = L * ar * o;
+        Cbr[half + i] = L * ai * o;
+        dA[i] = -(dm * dai) + dth_at(b_er, ci, i, nl);
+        dA[half + i] = dm * dar + dth_at(b_ei, ci, i, nl);
+        d2C[i] = dth_at(b_dar, ci, i, nl) - -(dm * ei);
+        d2C[half + i] = dth_at(b_dai, ci, i, nl) - dm * er;
+        A[i] = -(dm * ai) + dth_at(b_dcr, ci, i, nl);
+        A[half + i] = dm * ar + dth_at(b_dci, ci, i, nl);
+        dC[i] = -(dm * dci) - dth_at(b_ar, ci, i, nl);
+        dC[half + i] = dm * dcr - dth_at(b_ai, ci, i, nl);
+    }
+}
+
 // out(l, r, imi) = (lfac(l) in(l, r, imi)) rfac(r) for one field (either factor may be
 // absent); in place allowed.  The products round in this order, as the reference's
 // l_l_plus1(m:l_max)*SBUFFA(IDX2, f)*Over_RhoRSQ(r) statements round them.  One wave per
@@ -294,6 +546,49 @@ int launch_rlm_spaceb(raysht_plan *p, double *buf, int nf, int wv, int pv, int z
     return RAYSHT_OK;
 }
 
+// every role a distinct field of the buffer
+bool distinct_fields(const int *v, int n, int nf)
+{
+    for (int a = 0; a < n; ++a) {
+        if (v[a] < 0 || v[a] >= nf) return false;
+        for (int b = 0; b < a; ++b)
+            if (v[a] == v[b]) return false;
+    }
+    return true;
+}
+
+int launch_spacea_hydro(raysht_plan *p, double *buf, int nf, const int *fields, const double *over_rhor,
+                        const double *over_rhorsq, const double *drho_term, const double *two_over_r,
+                        const double *dlnrho, const double *dsdr_over_cp, hipStream_t st)
+{
+    if (!p || !buf || !fields || !over_rhor || !over_rhorsq || !drho_term || !two_over_r || !dlnrho)
+        return RAYSHT_EINVAL;
+    if (int rc = check_spectral(p)) return rc;
+    if (nf < 10 || !distinct_fields(fields, 10, nf)) return RAYSHT_EINVAL;
+    const spacea_hydro_fields f{fields[0], fields[1], fields[2], fields[3], fields[4],
+                                fields[5], fields[6], fields[7], fields[8], fields[9]};
+    const unsigned bx = (unsigned)((p->nr + kAThreads / 64 - 1) / (kAThreads / 64));  // a wave per radius
+    hipLaunchKernelGGL(rlm_spacea_hydro_kernel, dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr,
+                       p->d_mvals, p->d_lm_base, p->d_acoef, buf, nf, f, over_rhor, over_rhorsq, drho_term,
+                       two_over_r, dlnrho, dsdr_over_cp);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+int launch_bandcurlb(raysht_plan *p, double *buf, int nf, const int *fields, const double *one_over_rsq,
+                     hipStream_t st)
+{
+    if (!p || !buf || !fields || !one_over_rsq) return RAYSHT_EINVAL;
+    if (int rc = check_spectral(p)) return rc;
+    if (nf < 6 || !distinct_fields(fields, 6, nf)) return RAYSHT_EINVAL;
+    const bandcurlb_fields f{fields[0], fields[1], fields[2], fields[3], fields[4], fields[5]};
+    const unsigned bx = (unsigned)((p->nr + kAThreads / 64 - 1) / (kAThreads / 64));
+    hipLaunchKernelGGL(bandcurlb_kernel, dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr,
+                       p->d_mvals, p->d_lm_base, p->d_acoef, buf, nf, f, one_over_rsq);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
 int launch_lm_scale(raysht_plan *p, const double *in, int nf_in, int fin, double *out, int nf_out, int fout,
                     const double *lfac, const double *rfac, hipStream_t st)
 {
@@ -380,6 +675,20 @@ int raysht_d_by_dphi(raysht_plan *p, const double *in, int nf_in, int fin, doubl
 int raysht_rlm_spaceb_hydro(raysht_plan *p, double *buf, int nf, int wvar, int pvar, int zvar, void *stream)
 {
     return launch_rlm_spaceb(p, buf, nf, wvar, pvar, zvar, reinterpret_cast<hipStream_t>(stream));
+}
+
+int raysht_rlm_spacea_hydro(raysht_plan *p, double *buf, int nf, const int *fields, const double *over_rhor,
+                            const double *over_rhorsq, const double *drho_term, const double *two_over_r,
+                            const double *dlnrho, const double *dsdr_over_cp, void *stream)
+{
+    return launch_spacea_hydro(p, buf, nf, fields, over_rhor, over_rhorsq, drho_term, two_over_r, dlnrho,
+                               dsdr_over_cp, reinterpret_cast<hipStream_t>(stream));
+}
+
+int raysht_compute_bandcurlb(raysht_plan *p, double *buf, int nf, const int *fields, const double *one_over_rsq,
+                             void *stream)
+{
+    return launch_bandcurlb(p, buf, nf, fields, one_over_rsq, reinterpret_cast<hipStream_t>(stream));
 }
 
 int raysht_lm_scale(raysht_plan *p, const double *in, int nf_in, int fin, double *out, int nf_out, int fout,

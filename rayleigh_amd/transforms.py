@@ -178,6 +178,42 @@ class LegendrePlan:
         _lib.check(_lib.lib().raysht_rlm_spaceb_hydro(self._h, _dptr(buf, "buf"), nf, int(wvar), int(pvar), int(zvar),
                                                       _stream(stream)), "raysht_rlm_spaceb_hydro")
 
+    # the radial profiles rlm_spacea_hydro takes, in order
+    SPACEA_PROFILES = ("over_rhor", "over_rhorsq", "drho_term", "two_over_r", "dlnrho", "dsdr_over_cp")
+
+    def rlm_spacea_hydro(self, buf, fields, profiles, stream=None):
+        """Sphere_Hybrid_Space.F90:235-355 Velocity_Components + Velocity_Derivatives and :80
+        d_by_dtheta(tvar, dtdt), in place on `buf` (whole fields).  fields = 0-based (W, dWdr,
+        d2Wdr2, Z, dZdr, T, vtheta, vphi, dvtdr, dvpdr); vr, dvrdr, dtdt, dvrdt replace W, dWdr,
+        d2Wdr2, dZdr.  profiles: dict of SPACEA_PROFILES float64 device tensors of n_r values
+        (dsdr_over_cp None or absent when not pseudo-incompressible)."""
+        (nf,) = self._whole_fields(buf)
+        v = np.ascontiguousarray(fields, dtype=np.int32)
+        if v.size != 10:
+            raise ValueError("fields = W, dWdr, d2Wdr2, Z, dZdr, T, vtheta, vphi, dvtdr, dvpdr")
+        prof = []
+        for k in self.SPACEA_PROFILES:
+            x = profiles.get(k)
+            if x is None and k != "dsdr_over_cp":
+                raise ValueError(f"profile {k} is required")
+            prof.append(None if x is None else _dptr(x, k, self.n_r))
+        _lib.check(_lib.lib().raysht_rlm_spacea_hydro(self._h, _dptr(buf, "buf"), nf,
+                                                      v.ctypes.data_as(ctypes.c_void_p), *prof, _stream(stream)),
+                   "raysht_rlm_spacea_hydro")
+
+    def compute_bandcurlb(self, buf, fields, one_over_rsq, stream=None):
+        """Sphere_Hybrid_Space.F90:377-460 Compute_BandCurlB, in place on `buf` (whole fields).
+        fields = 0-based (C, A, dCdr, dAdr, d2Cdr2, curlbr); Br, Btheta, Bphi, curlbtheta,
+        curlbphi replace C, A, dCdr, d2Cdr2, dAdr.  one_over_rsq: float64 device, n_r values."""
+        (nf,) = self._whole_fields(buf)
+        v = np.ascontiguousarray(fields, dtype=np.int32)
+        if v.size != 6:
+            raise ValueError("fields = C, A, dCdr, dAdr, d2Cdr2, curlbr")
+        _lib.check(_lib.lib().raysht_compute_bandcurlb(self._h, _dptr(buf, "buf"), nf,
+                                                       v.ctypes.data_as(ctypes.c_void_p),
+                                                       _dptr(one_over_rsq, "one_over_rsq", self.n_r),
+                                                       _stream(stream)), "raysht_compute_bandcurlb")
+
     def lm_scale(self, a, fin: int, out, fout: int, lfac=None, rfac=None, stream=None):
         """out(l, r, imi) = (lfac(l) a(l, r, imi)) rfac(r) from field fin of `a` into field fout of
         `out` (the l(l+1) / radial scalings of rlm_spacea and rlm_spaceb); lfac (l_max + 1

@@ -1,0 +1,117 @@
+"""GPU parity of the fused rlm_spacea passes (raysht_rlm_spacea_hydro: Velocity_Components +
+Velocity_Derivatives + d_by_dtheta(tvar, dtdt); raysht_compute_bandcurlb: Compute_BandCurlB;
+Physics/Sphere_Hybrid_Space.F90:80,235-460): bit for bit against the compiled reference's
+outputs (golden fixtures) and against the oracle for l_max from 0 to 2049 (one to 33 chunks of
+64 values per column), m = l_max blocks, roles in any field order, with and without the
+pseudo-incompressible term; a large buffer by exact linearity; the rejected calls."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_helpers import dev, host
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ra():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import rayleigh_amd
+    return rayleigh_amd
+
+
+def _dev_profiles(prof):
+    return {k: (None if v is None else dev(v)) for k, v in prof.items()}
+
+
+@pytest.mark.parametrize("name", ["rlm_spacea_L12_ascending.npz", "rlm_spacea_L41_partial.npz"])
+def test_spacea_vs_reference_fixture(ra, golden, name):
+    g = golden(name)
+    L, mv, nr, nf = int(g["lmax"]), g["mvals"], int(g["nr"]), int(g["nf"])
+    plan = ra.LegendrePlan(2 * (L + 1), L, mv, nr, nf)
+    prof = {k: (g[k] if k in g else None) for k in O.HYDRO_PROFILES}
+    buf = dev(g["buf"])
+    plan.rlm_spacea_hydro(buf, g["hydro"], _dev_profiles(prof))
+    assert np.array_equal(host(buf), g["hydro_out"])
+    buf = dev(g["buf"])
+    plan.compute_bandcurlb(buf, g["magnetic"], dev(g["one_over_rsq"]))
+    assert np.array_equal(host(buf), g["magnetic_out"])
+
+
+@pytest.mark.parametrize("L,mv,nr,pseudo", [
+    (0, [0], 2, True),
+    (1, [1, 0], 3, False),
+    (63, "balanced", 9, True),
+    (64, [64, 0, 63, 1], 3, False),
+    (255, "balanced", 3, True),
+    (300, [300, 0, 44, 1, 299], 2, False),
+    (1023, [0, 1, 511, 1023, 1022], 2, True),
+    (2049, [2049, 0, 1985, 1000], 1, False),
+])
+def test_spacea_vs_oracle(ra, L, mv, nr, pseudo):
+    mv = O.m_order_balanced(L) if isinstance(mv, str) else np.array(mv, np.int32)
+    nf = 17
+    rng = np.random.default_rng(L * 3 + nr)
+    perm = rng.permutation(nf)
+    hydro, magnetic = perm[:10], perm[10:16]
+    plan = ra.LegendrePlan(2 * (L + 1), L, mv, nr, nf)
+    a = rng.standard_normal(plan.spectral_size)
+    prof = {k: rng.uniform(0.5, 2.0, nr) for k in O.HYDRO_PROFILES}
+    if not pseudo:
+        prof["dsdr_over_cp"] = None
+    o2 = rng.uniform(0.5, 2.0, nr)
+    buf = dev(a)
+    plan.rlm_spacea_hydro(buf, hydro, _dev_profiles(prof))
+    assert np.array_equal(host(buf), O.rlm_spacea_hydro(L, mv, nr, a, nf, hydro, prof)), (L, "hydro")
+    buf = dev(a)
+    plan.compute_bandcurlb(buf, magnetic, dev(o2))
+    assert np.array_equal(host(buf), O.compute_bandcurlb(L, mv, nr, a, nf, magnetic, o2)), (L, "bandcurlb")
+
+
+def test_spacea_large_linearity(ra):
+    """l_max = 511, N_r = 64, 16 fields: pass(2a) = 2 pass(a) bit for bit for both passes, and
+    the fields outside the roles are untouched."""
+    import torch
+    L, nr, nf = 511, 64, 16
+    mv = O.m_order_balanced(L)
+    plan = ra.LegendrePlan(768, L, mv, nr, nf)
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    a = torch.randn(plan.spectral_size, dtype=torch.float64, device="cuda", generator=gen)
+    hydro, magnetic = [3, 0, 7, 12, 1, 9, 4, 15, 2, 10], [5, 13, 8, 14, 6, 11]
+    prof = {k: torch.rand(nr, dtype=torch.float64, device="cuda", generator=gen) + 0.5 for k in O.HYDRO_PROFILES}
+    o2 = torch.rand(nr, dtype=torch.float64, device="cuda", generator=gen) + 0.5
+    per_field = plan.spectral_size // nf
+    field = torch.cat([torch.arange(nf, device="cuda").repeat_interleave((L - m + 1) * 2 * nr) for m in mv])
+    assert field.numel() == nf * per_field
+    b1, b2 = a.clone(), 2.0 * a
+    plan.rlm_spacea_hydro(b1, hydro, prof)
+    plan.rlm_spacea_hydro(b2, hydro, prof)
+    assert torch.equal(b2, 2.0 * b1)
+    keep = ~torch.isin(field, torch.tensor(hydro, device="cuda"))
+    assert torch.equal(b1[keep], a[keep])
+    c1, c2 = a.clone(), 2.0 * a
+    plan.compute_bandcurlb(c1, magnetic, o2)
+    plan.compute_bandcurlb(c2, magnetic, o2)
+    assert torch.equal(c2, 2.0 * c1)
+    keep = ~torch.isin(field, torch.tensor(magnetic, device="cuda"))
+    assert torch.equal(c1[keep], a[keep])
+
+
+def test_spacea_rejected_calls(ra):
+    import torch
+    L, nr, nf = 15, 2, 10
+    plan = ra.LegendrePlan(32, L, np.arange(L + 1, dtype=np.int32), nr, nf)
+    buf = torch.zeros(plan.spectral_size, dtype=torch.float64, device="cuda")
+    prof = {k: torch.ones(nr, dtype=torch.float64, device="cuda") for k in O.HYDRO_PROFILES}
+    with pytest.raises(RuntimeError):
+        plan.rlm_spacea_hydro(buf, [0, 1, 2, 3, 4, 5, 6, 7, 8, 8], prof)  # a field in two roles
+    with pytest.raises(RuntimeError):
+        plan.rlm_spacea_hydro(buf, [0, 1, 2, 3, 4, 5, 6, 7, 8, 10], prof)  # no field 10
+    with pytest.raises(ValueError):
+        plan.rlm_spacea_hydro(buf, [0, 1, 2, 3, 4, 5, 6, 7, 8, 9], dict(prof, dlnrho=None))
+    with pytest.raises(RuntimeError):
+        plan.compute_bandcurlb(buf, [0, 1, 2, 3, 4, 0], prof["dlnrho"])
+    with pytest.raises(RuntimeError):
+        plan.compute_bandcurlb(buf, [0, 1, 2, 3, 4, -1], prof["dlnrho"])
