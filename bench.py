@@ -308,7 +308,8 @@ def cpu_reference_extras(L, mv, nr, nf, nlm, n_lu, ncols, out):
         prof = {k: rng.uniform(0.5, 2.0, nrs) for k in O.HYDRO_PROFILES}
         for key, call in (("rlm_spacea_hydro", lambda: R.rlm_spacea_hydro(buf, 16, list(range(10)), prof)),
                           ("compute_bandcurlb", lambda: R.compute_bandcurlb(buf, 16, list(range(10, 16)),
-                                                                            prof["over_rhorsq"]))):
+                                                                            prof["over_rhorsq"])),
+                          ("adjust_emf", lambda: R.adjust_emf(buf, 16, 10, 11))):
             t0 = time.perf_counter()
             call()
             t = time.perf_counter() - t0
@@ -542,14 +543,16 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
     # rlm_spacea's passes before StP on a 16-field buffer of the same spectral layout: the
     # hydro pass (Velocity_Components + Velocity_Derivatives + dtdt: 6 fields read, 9 written,
     # 120 B per coefficient of a field) on fields 0-9, Compute_BandCurlB (5 read, 6 written,
-    # 88 B) on fields 10-15; repeated in place on the same buffer (the time of these
-    # passes does not depend on the values)
+    # 88 B) on fields 10-15, and rlm_spaceb's Adjust_Emf (2 fields read and written, 32 B) on
+    # fields 10-11; repeated in place on the same buffer (the time of these passes does not
+    # depend on the values)
     b16 = torch.randn(16 * per_field, dtype=torch.float64, device="cuda")
     prof = {k: torch.rand(nr, dtype=torch.float64, device="cuda") + 0.5 for k in plan.SPACEA_PROFILES}
     out["rlm_spacea"] = {}
     for name, fn, nb in (("hydro", lambda: plan.rlm_spacea_hydro(b16, list(range(10)), prof), 120.0 * per_field),
                          ("bandcurlb", lambda: plan.compute_bandcurlb(b16, list(range(10, 16)), prof["over_rhorsq"]),
-                          88.0 * per_field)):
+                          88.0 * per_field),
+                         ("adjust_emf", lambda: plan.adjust_emf(b16, 10, 11), 32.0 * per_field)):
         for _ in range(max(warmup, 1)):
             fn()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

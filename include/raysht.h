@@ -197,6 +197,13 @@ int raysht_rlm_spacea_hydro(raysht_plan *plan, double *buf, int nf, const int *f
  * are the doubles of the reference's sequence. */
 int raysht_compute_bandcurlb(raysht_plan *plan, double *buf, int nf, const int *fields, const double *one_over_rsq,
                              void *stream);
+/* rlm_spaceb's Adjust_Emf (Physics/Sphere_Hybrid_Space.F90:557-602) in one pass, in place on `buf`
+ * (nf fields) after PtS: emftheta <- (d_by_sdtheta emfphi - d_by_dphi emftheta) over_l_l_plus1(l)
+ * (Cvar is the emftheta field, Fields.F90:363-364), emfphi <- (d_by_dphi emfphi + d_by_sdtheta
+ * emftheta) over_l_l_plus1(l), with emftheta as it came in; emftheta, emfphi 0-based and distinct.
+ * It reads and writes only those two fields, so calling it before raysht_rlm_spaceb_hydro (whose
+ * last step zeroes l_max) gives rlm_spaceb's magnetic order.  Results are the reference's doubles. */
+int raysht_adjust_emf(raysht_plan *plan, double *buf, int nf, int emftheta, int emfphi, void *stream);
 /* The l(l+1) and radial scalings of rlm_spacea / rlm_spaceb (Velocity_Components,
  * Velocity_Derivatives, Compute_BandCurlB: Sphere_Hybrid_Space.F90:244,324,353,367,373) on one
  * field: out(l, r, imi) = (lfac(l) in(l, r, imi)) rfac(r), rounded in that order; lfac

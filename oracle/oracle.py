@@ -74,6 +74,7 @@ def oracle_lib():
         lib.orc_zero_lmax.argtypes = [_I, _P, _I, _I, _P, _I]
         lib.orc_rlm_spacea_hydro.argtypes = [_I, _P, _I, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P]
         lib.orc_compute_bandcurlb.argtypes = [_I, _P, _I, _I, _P, _I, _P, _P]
+        lib.orc_adjust_emf.argtypes = [_I, _P, _I, _I, _P, _I, _I, _I]
         lib.orc_lu_decompose.restype = _I
         lib.orc_lu_decompose.argtypes = [_I, _P, _P]
         lib.orc_lu_solve.argtypes = [_I, _P, _P, _I, _P]
@@ -124,6 +125,7 @@ def ref_lib():
         lib.ref_rlm_spaceb.argtypes = [_I, _I, _I, _I, _P]
         lib.ref_rlm_spacea_hydro.argtypes = [_I, _P, _I, _P, _P, _P, _P, _P, _P, _P]
         lib.ref_compute_bandcurlb.argtypes = [_I, _P, _P, _P]
+        lib.ref_adjust_emf.argtypes = [_I, _I, _I, _P]
         _ref = lib
     return _ref
 
@@ -330,6 +332,15 @@ def compute_bandcurlb(lmax, mvals, nr, buf, nf, fields, one_over_rsq):
     assert v.size == 6
     o2 = np.ascontiguousarray(one_over_rsq, dtype=np.float64)
     oracle_lib().orc_compute_bandcurlb(lmax, _ptr(mv), len(mv), nr, _ptr(b), nf, _ptr(v), _ptr(o2))
+    return b
+
+
+def adjust_emf(lmax, mvals, nr, buf, nf, et, ep):
+    """Sphere_Hybrid_Space.F90:557-602 Adjust_Emf on a copy of the packed nf-field buffer (0-based
+    emftheta, emfphi fields); returns the new buffer."""
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    b = np.ascontiguousarray(buf, dtype=np.float64).copy()
+    oracle_lib().orc_adjust_emf(lmax, _ptr(mv), len(mv), nr, _ptr(b), nf, et, ep)
     return b
 
 
@@ -643,6 +654,12 @@ class Reference:
         v = np.ascontiguousarray(np.asarray(fields) + 1, dtype=np.int32)
         o2 = np.ascontiguousarray(one_over_rsq, dtype=np.float64)
         self.lib.ref_compute_bandcurlb(nf, _ptr(v), _ptr(o2), _ptr(b))
+        return b
+
+    def adjust_emf(self, buf: np.ndarray, nf: int, et: int, ep: int) -> np.ndarray:
+        """Adjust_Emf through the reference's own derivatives (fields 0-based)."""
+        b = np.ascontiguousarray(buf, dtype=np.float64).copy()
+        self.lib.ref_adjust_emf(nf, et + 1, ep + 1, _ptr(b))
         return b
 
     def cheby_deriv(self, c: np.ndarray, dorder: int, nobuffer: bool = False) -> np.ndarray:

@@ -1133,6 +1133,54 @@ void orc_compute_bandcurlb(int lmax, const int *mvals, int nm, int nr, double *b
     free(t2);
 }
 
+/* Sphere_Hybrid_Space.F90:557-602 Adjust_Emf, in place on a packed buffer of nf fields after
+ * PtS, step for step: t1 = d_by_sdtheta emfphi; t2 = d_by_dphi emftheta; t3 = emftheta;
+ * emftheta = (t1 - t2) ol (Cvar is the emftheta field, Fields.F90:363-364);
+ * t2 = d_by_dphi emfphi; emfphi = t3; t1 = d_by_sdtheta emfphi; emfphi = (t2 + t1) ol;
+ * ol = over_l_l_plus1.  et, ep = 0-based emftheta, emfphi. */
+void orc_adjust_emf(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, int et, int ep)
+{
+    const slab g = {lmax, nm, nr, mvals};
+    const long fsz = field_block(lmax, mvals, nm, nr, 1);
+    double *t1 = (double *)malloc(sizeof(double) * (fsz + 1));
+    double *t2 = (double *)malloc(sizeof(double) * (fsz + 1));
+    double *t3 = (double *)malloc(sizeof(double) * (fsz + 1));
+    orc_angular_dtheta(1, lmax, mvals, nm, nr, buf, nf, ep, t1, 1, 0);
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, et, t2, 1, 0);
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp], nl = lmax - m + 1;
+        const long o = field_block(lmax, mvals, mp, nr, 1);
+        double *T = fld(&g, buf, nf, et, mp);
+        for (long e = 0; e < 2L * nr * nl; ++e) t3[o + e] = T[e];
+        for (long k = 0; k < 2L * nr; ++k)
+            for (int l = m; l <= lmax; ++l) {
+                const long e = k * nl + l - m;
+                T[e] = (t1[o + e] - t2[o + e]) * over_l_l_plus1(l);
+            }
+    }
+    orc_angular_dphi(lmax, mvals, nm, nr, buf, nf, ep, t2, 1, 0);
+    for (int mp = 0; mp < nm; ++mp) {
+        const int nl = lmax - mvals[mp] + 1;
+        const long o = field_block(lmax, mvals, mp, nr, 1);
+        double *P = fld(&g, buf, nf, ep, mp);
+        for (long e = 0; e < 2L * nr * nl; ++e) P[e] = t3[o + e];
+    }
+    orc_angular_dtheta(1, lmax, mvals, nm, nr, buf, nf, ep, t1, 1, 0);
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp], nl = lmax - m + 1;
+        const long o = field_block(lmax, mvals, mp, nr, 1);
+        double *P = fld(&g, buf, nf, ep, mp);
+        for (long k = 0; k < 2L * nr; ++k)
+            for (int l = m; l <= lmax; ++l) {
+                const long e = k * nl + l - m;
+                P[e] = (t2[o + e] + t1[o + e]) * over_l_l_plus1(l);
+            }
+    }
+    free(t1);
+    free(t2);
+    free(t3);
+}
+
 /* LU_Decompose_band (Linear_Solve.F90:1041-1055): LAPACK dgbtrf on an n x n band matrix in
  * LAPACK band storage ab(ldab, n), ldab >= 2 kl + ku + 1 (the reference passes kl = ku =
  * (ldab - 1) / 3), restated as the unblocked dgbtf2 that dgbtrf runs for ku <= 64: fill-in rows

@@ -72,6 +72,7 @@ void orc_zero_lmax(int lmax, const int *mvals, int nm, int nr, double *buf, int 
 void orc_rlm_spacea_hydro(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, const int *v,
                           const double *over_rhor, const double *over_rhorsq, const double *drho_term,
                           const double *two_over_r, const double *dlnrho, const double *dsdr_over_cp);
+void orc_adjust_emf(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, int et, int ep);
 void orc_compute_bandcurlb(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, const int *v,
                            const double *one_over_rsq);
 int orc_lu_decompose(int n, double *a, int *piv);

@@ -574,4 +574,57 @@ Contains
         DeAllocate(a, ftemp1, ftemp2, l_l_plus1)
     End Subroutine ref_compute_bandcurlb
 
+    ! Adjust_Emf (Physics/Sphere_Hybrid_Space.F90:557-602) through the reference's d_by_sdtheta /
+    ! d_by_dphi and the same array statements, in place on a packed buffer of nf fields; et, ep =
+    ! 1-based emftheta (= Cvar, Fields.F90:363-364) and emfphi.
+    Subroutine ref_adjust_emf(nf, et, ep, buf) bind(C, name='ref_adjust_emf')
+        Integer(C_INT), Value :: nf, et, ep
+        Real(C_DOUBLE), Intent(InOut) :: buf(*)
+        Type(rmcontainer4d), Allocatable :: a(:)
+        Type(rmcontainer3d), Allocatable :: ftemp1(:), ftemp2(:), ftemp3(:)
+        Real*8, Allocatable :: ol(:)
+        Real*8 :: ell
+        Integer :: mp, m, l, r, imi, lm, emftheta, emfphi, cvar
+        emftheta = et; emfphi = ep; cvar = et
+        lm = h_ang_lmax
+        Allocate(ol(0:lm))
+        ol(0) = 1.0d0
+        Do l = 1, lm
+            ell = l*1.0d0
+            ol(l) = 1.0d0/(ell*(ell+1))
+        Enddo
+        Call unpack4d(buf, nf, a)
+        Call alloc3d(ftemp1)
+        Call alloc3d(ftemp2)
+        Call d_by_sdtheta(a, emfphi, ftemp1)
+        Call d_by_dphi(a, emftheta, ftemp2)
+        Call alloc3d(ftemp3)
+        Do mp = 1, h_ang_nm
+            ftemp3(mp)%data = a(mp)%data(:,:,:,emftheta)
+        Enddo
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            Do imi = 1, 2
+                Do r = 1, h_ang_nr
+                    a(mp)%data(m:lm,r,imi,cvar) = (ftemp1(mp)%data(m:lm,r,imi)-ftemp2(mp)%data(m:lm,r,imi))*ol(m:lm)
+                Enddo
+            Enddo
+        Enddo
+        Call d_by_dphi(a, emfphi, ftemp2)
+        Do mp = 1, h_ang_nm
+            a(mp)%data(:,:,:,emfphi) = ftemp3(mp)%data
+        Enddo
+        Call d_by_sdtheta(a, emfphi, ftemp1)
+        Do mp = 1, h_ang_nm
+            m = h_ang_mvals(mp)
+            Do imi = 1, 2
+                Do r = 1, h_ang_nr
+                    a(mp)%data(m:lm,r,imi,emfphi) = (ftemp2(mp)%data(m:lm,r,imi)+ftemp1(mp)%data(m:lm,r,imi))*ol(m:lm)
+                Enddo
+            Enddo
+        Enddo
+        Call pack4d(a, nf, buf)
+        DeAllocate(a, ftemp1, ftemp2, ftemp3, ol)
+    End Subroutine ref_adjust_emf
+
 End Module raysht_ref_harness

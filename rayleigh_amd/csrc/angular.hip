@@ -8,8 +8,10 @@
 // column: the l-stencils run one wave per l-column with the neighbours passed between
 // lanes, the coefficients come from a small per-plan table in L2.  Products and sums are
 // rounded separately, as the reference's compiled loops round them, so the results are its
-// doubles.  The same file holds the rlm_spaceb combination built from these derivatives and
-// the l(l+1) / radial scalings of the spectral fields.
+// doubles.  The same file holds the passes built from these derivatives, each one kernel
+// whose waves own whole radii: rlm_spaceb's combination and its Adjust_Emf after PtS,
+// rlm_spacea's velocity steps and Compute_BandCurlB before StP, and the l(l+1) / radial
+// scalings of the spectral fields.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -452,6 +454,57 @@ __global__ __launch_bounds__(kAThreads) void bandcurlb_kernel(int lmax, int nr, 
     }
 }
 
+// rlm_spaceb's Adjust_Emf (Sphere_Hybrid_Space.F90:557-602), in place, one pass.  With S =
+// d_by_sdtheta, P = d_by_dphi and ol = over_l_l_plus1(l):
+//   emftheta <- (S emfphi - P emftheta) ol   (Cvar is the emftheta field);
+//   emfphi   <- (P emfphi + S emftheta) ol   (emftheta as it came in).
+// Rounded where the reference rounds; one wave per radius of one m-block.
+__global__ __launch_bounds__(kAThreads) void adjust_emf_kernel(int lmax, int nr, const int *__restrict__ mvals,
+                                                               const int *__restrict__ lm_base,
+                                                               const double2 *__restrict__ scoef, double *buf, int nf,
+                                                               int et, int ep)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * (kAThreads / 64) + (threadIdx.x >> 6);
+    if (r >= nr) return;  // whole waves
+    const int mp = blockIdx.y, m = mvals[mp], nl = lmax - m + 1;
+    const long lb = lm_base[mp], half = (long)nl * nr, fsz = 2 * half;
+    double *blk = buf + lb * 2L * nr * nf + (long)r * nl;
+    double *T = blk + et * fsz, *P = blk + ep * fsz;
+    const double2 *c = scoef + lb;
+    const double dm = (double)m;
+    double k_tr = 0.0, k_ti = 0.0, k_pr = 0.0, k_pi = 0.0;
+    for (int base = 0; base < nl; base += 64) {
+        const int i = base + lane, l = m + i;
+        const bool in = i < nl;
+        double tr = 0.0, ti = 0.0, pr = 0.0, pi = 0.0;
+        double2 ci = make_double2(0.0, 0.0);
+        if (in) {
+            tr = T[i];
+            ti = T[half + i];
+            pr = P[i];
+            pi = P[half + i];
+            ci = c[i];
+        }
+        double n_tr = 0.0, n_ti = 0.0, n_pr = 0.0, n_pi = 0.0;
+        if (lane == 63 && i + 1 < nl) {
+            n_tr = T[i + 1];
+            n_ti = T[half + i + 1];
+            n_pr = P[i + 1];
+            n_pi = P[half + i + 1];
+        }
+        const lnb b_tr = l_neighbours(tr, n_tr, k_tr, lane), b_ti = l_neighbours(ti, n_ti, k_ti, lane);
+        const lnb b_pr = l_neighbours(pr, n_pr, k_pr, lane), b_pi = l_neighbours(pi, n_pi, k_pi, lane);
+        if (!in) continue;
+        const double ol = l == 0 ? 1.0 : 1.0 / l_l_plus1(l);
+        T[i] = (dth_at(b_pr, ci, i, nl) - -(dm * ti)) * ol;
+        T[half + i] = (dth_at(b_pi, ci, i, nl) - dm * tr) * ol;
+        P[i] = (-(dm * pi) + dth_at(b_tr, ci, i, nl)) * ol;
+        P[half + i] = (dm * pr + dth_at(b_ti, ci, i, nl)) * ol;
+    }
+}
+
 // out(l, r, imi) = (lfac(l) in(l, r, imi)) rfac(r) for one field (either factor may be
 // absent); in place allowed.  The products round in this order, as the reference's
 // l_l_plus1(m:l_max)*SBUFFA(IDX2, f)*Over_RhoRSQ(r) statements round them.  One wave per
@@ -589,6 +642,19 @@ int launch_bandcurlb(raysht_plan *p, double *buf, int nf, const int *fields, con
     return RAYSHT_OK;
 }
 
+int launch_adjust_emf(raysht_plan *p, double *buf, int nf, int et, int ep, hipStream_t st)
+{
+    if (!p || !buf) return RAYSHT_EINVAL;
+    if (int rc = check_spectral(p)) return rc;
+    const int v[2] = {et, ep};
+    if (nf < 2 || !distinct_fields(v, 2, nf)) return RAYSHT_EINVAL;
+    const unsigned bx = (unsigned)((p->nr + kAThreads / 64 - 1) / (kAThreads / 64));
+    hipLaunchKernelGGL(adjust_emf_kernel, dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr,
+                       p->d_mvals, p->d_lm_base, p->d_acoef + p->lm_total, buf, nf, et, ep);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
 int launch_lm_scale(raysht_plan *p, const double *in, int nf_in, int fin, double *out, int nf_out, int fout,
                     const double *lfac, const double *rfac, hipStream_t st)
 {
@@ -683,6 +749,11 @@ int raysht_rlm_spacea_hydro(raysht_plan *p, double *buf, int nf, const int *fiel
 {
     return launch_spacea_hydro(p, buf, nf, fields, over_rhor, over_rhorsq, drho_term, two_over_r, dlnrho,
                                dsdr_over_cp, reinterpret_cast<hipStream_t>(stream));
+}
+
+int raysht_adjust_emf(raysht_plan *p, double *buf, int nf, int emftheta, int emfphi, void *stream)
+{
+    return launch_adjust_emf(p, buf, nf, emftheta, emfphi, reinterpret_cast<hipStream_t>(stream));
 }
 
 int raysht_compute_bandcurlb(raysht_plan *p, double *buf, int nf, const int *fields, const double *one_over_rsq,

@@ -214,6 +214,14 @@ class LegendrePlan:
                                                        _dptr(one_over_rsq, "one_over_rsq", self.n_r),
                                                        _stream(stream)), "raysht_compute_bandcurlb")
 
+    def adjust_emf(self, buf, emftheta: int, emfphi: int, stream=None):
+        """Sphere_Hybrid_Space.F90:557-602 Adjust_Emf, in place on `buf` (whole fields; 0-based
+        emftheta, emfphi): emftheta = (d_by_sdtheta emfphi - d_by_dphi emftheta) / l(l+1),
+        emfphi = (d_by_dphi emfphi + d_by_sdtheta emftheta) / l(l+1)."""
+        (nf,) = self._whole_fields(buf)
+        _lib.check(_lib.lib().raysht_adjust_emf(self._h, _dptr(buf, "buf"), nf, int(emftheta), int(emfphi),
+                                                _stream(stream)), "raysht_adjust_emf")
+
     def lm_scale(self, a, fin: int, out, fout: int, lfac=None, rfac=None, stream=None):
         """out(l, r, imi) = (lfac(l) a(l, r, imi)) rfac(r) from field fin of `a` into field fout of
         `out` (the l(l+1) / radial scalings of rlm_spacea and rlm_spaceb); lfac (l_max + 1

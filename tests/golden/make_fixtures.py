@@ -107,12 +107,12 @@ def rlm_case(name, lmax, mvals, nr, nf, fields, seed):
     print("wrote", name)
 
 
-def rlm_spacea_case(name, lmax, mvals, nr, nf, hydro, magnetic, pseudo, seed):
-    """rlm_spacea's Velocity_Components + Velocity_Derivatives + d_by_dtheta(tvar, dtdt) and
-    Compute_BandCurlB (Sphere_Hybrid_Space.F90:80,235-460) through the reference's own
-    derivatives on a seeded nf-field buffer with seeded radial profiles; hydro = 0-based (W,
-    dWdr, d2Wdr2, Z, dZdr, T, vtheta, vphi, dvtdr, dvpdr), magnetic = 0-based (C, A, dCdr,
-    dAdr, d2Cdr2, curlbr)."""
+def rlm_spacea_case(name, lmax, mvals, nr, nf, hydro, magnetic, emf, pseudo, seed):
+    """rlm_spacea's Velocity_Components + Velocity_Derivatives + d_by_dtheta(tvar, dtdt),
+    Compute_BandCurlB and rlm_spaceb's Adjust_Emf (Sphere_Hybrid_Space.F90:80,235-460,557-602)
+    through the reference's own derivatives on a seeded nf-field buffer with seeded radial
+    profiles; hydro = 0-based (W, dWdr, d2Wdr2, Z, dZdr, T, vtheta, vphi, dvtdr, dvpdr),
+    magnetic = 0-based (C, A, dCdr, dAdr, d2Cdr2, curlbr), emf = 0-based (emftheta, emfphi)."""
     R = O.Reference()
     mv = np.ascontiguousarray(mvals, dtype=np.int32)
     R.angular_init(lmax, mv, nr)
@@ -125,7 +125,8 @@ def rlm_spacea_case(name, lmax, mvals, nr, nf, hydro, magnetic, pseudo, seed):
     rec = dict(lmax=lmax, mvals=mv, nr=nr, nf=nf, buf=buf, hydro=np.array(hydro, np.int32),
                magnetic=np.array(magnetic, np.int32), one_over_rsq=one_over_rsq, pseudo=int(pseudo),
                hydro_out=R.rlm_spacea_hydro(buf, nf, hydro, prof),
-               magnetic_out=R.compute_bandcurlb(buf, nf, magnetic, one_over_rsq))
+               magnetic_out=R.compute_bandcurlb(buf, nf, magnetic, one_over_rsq),
+               emf=np.array(emf, np.int32), emf_out=R.adjust_emf(buf, nf, *emf))
     for k in O.HYDRO_PROFILES:
         if prof[k] is not None:
             rec[k] = prof[k]
@@ -181,10 +182,10 @@ if __name__ == "__main__":
     rlm_case("rlm_spaceb_L41_partial.npz", 41, [41, 0, 17, 3, 40, 1], 4, 4, (3, 0, 2), seed=410)
     grid_case("grid_nodes_weights.npz", GRID_THETAS, GRID_CHEBS)
     SPACEA_CASES = (
-        ("rlm_spacea_L12_ascending.npz", 12, np.arange(13), 3, 16, tuple(range(10)), (10, 11, 12, 13, 14, 15), True,
-         121),
+        ("rlm_spacea_L12_ascending.npz", 12, np.arange(13), 3, 16, tuple(range(10)), (10, 11, 12, 13, 14, 15),
+         (12, 11), True, 121),
         ("rlm_spacea_L41_partial.npz", 41, [41, 0, 17, 3, 40, 1], 4, 17, (11, 3, 0, 7, 2, 9, 1, 5, 12, 6),
-         (16, 4, 13, 8, 10, 14), False, 411),
+         (16, 4, 13, 8, 10, 14), (4, 15), False, 411),
     )
     for args in SPACEA_CASES:
         rlm_spacea_case(*args)

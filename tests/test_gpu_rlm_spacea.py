@@ -1,6 +1,7 @@
 """GPU parity of the fused rlm_spacea passes (raysht_rlm_spacea_hydro: Velocity_Components +
 Velocity_Derivatives + d_by_dtheta(tvar, dtdt); raysht_compute_bandcurlb: Compute_BandCurlB;
-Physics/Sphere_Hybrid_Space.F90:80,235-460): bit for bit against the compiled reference's
+Physics/Sphere_Hybrid_Space.F90:80,235-460) and rlm_spaceb's Adjust_Emf (raysht_adjust_emf,
+:557-602): bit for bit against the compiled reference's
 outputs (golden fixtures) and against the oracle for l_max from 0 to 2049 (one to 33 chunks of
 64 values per column), m = l_max blocks, roles in any field order, with and without the
 pseudo-incompressible term; a large buffer by exact linearity; the rejected calls."""
@@ -38,6 +39,9 @@ def test_spacea_vs_reference_fixture(ra, golden, name):
     buf = dev(g["buf"])
     plan.compute_bandcurlb(buf, g["magnetic"], dev(g["one_over_rsq"]))
     assert np.array_equal(host(buf), g["magnetic_out"])
+    buf = dev(g["buf"])
+    plan.adjust_emf(buf, *(int(x) for x in g["emf"]))
+    assert np.array_equal(host(buf), g["emf_out"])
 
 
 @pytest.mark.parametrize("L,mv,nr,pseudo", [
@@ -52,10 +56,10 @@ def test_spacea_vs_reference_fixture(ra, golden, name):
 ])
 def test_spacea_vs_oracle(ra, L, mv, nr, pseudo):
     mv = O.m_order_balanced(L) if isinstance(mv, str) else np.array(mv, np.int32)
-    nf = 17
+    nf = 18
     rng = np.random.default_rng(L * 3 + nr)
     perm = rng.permutation(nf)
-    hydro, magnetic = perm[:10], perm[10:16]
+    hydro, magnetic, emf = perm[:10], perm[10:16], (int(perm[16]), int(perm[17]))
     plan = ra.LegendrePlan(2 * (L + 1), L, mv, nr, nf)
     a = rng.standard_normal(plan.spectral_size)
     prof = {k: rng.uniform(0.5, 2.0, nr) for k in O.HYDRO_PROFILES}
@@ -68,6 +72,9 @@ def test_spacea_vs_oracle(ra, L, mv, nr, pseudo):
     buf = dev(a)
     plan.compute_bandcurlb(buf, magnetic, dev(o2))
     assert np.array_equal(host(buf), O.compute_bandcurlb(L, mv, nr, a, nf, magnetic, o2)), (L, "bandcurlb")
+    buf = dev(a)
+    plan.adjust_emf(buf, *emf)
+    assert np.array_equal(host(buf), O.adjust_emf(L, mv, nr, a, nf, *emf)), (L, "adjust_emf")
 
 
 def test_spacea_large_linearity(ra):
@@ -115,3 +122,7 @@ def test_spacea_rejected_calls(ra):
         plan.compute_bandcurlb(buf, [0, 1, 2, 3, 4, 0], prof["dlnrho"])
     with pytest.raises(RuntimeError):
         plan.compute_bandcurlb(buf, [0, 1, 2, 3, 4, -1], prof["dlnrho"])
+    with pytest.raises(RuntimeError):
+        plan.adjust_emf(buf, 3, 3)
+    with pytest.raises(RuntimeError):
+        plan.adjust_emf(buf, 3, 10)

@@ -1,5 +1,6 @@
 """The oracle's rlm_spacea steps: Velocity_Components + Velocity_Derivatives + d_by_dtheta(tvar,
-dtdt) and Compute_BandCurlB (Physics/Sphere_Hybrid_Space.F90:80,235-460), pinned against:
+dtdt) and Compute_BandCurlB (Physics/Sphere_Hybrid_Space.F90:80,235-460), and rlm_spaceb's
+Adjust_Emf (:557-602), pinned against:
   * the compiled reference's outputs (golden fixtures), bit for bit: the harness runs the
     reference's own d_by_dtheta / d_by_dphi in the routines' order with the same array
     statements (the physics module itself needs the whole model state to build);
@@ -35,6 +36,15 @@ def test_bandcurlb_vs_reference_fixture(golden, name):
     L, mv, nr, nf = int(g["lmax"]), g["mvals"], int(g["nr"]), int(g["nf"])
     out = O.compute_bandcurlb(L, mv, nr, g["buf"], nf, g["magnetic"], g["one_over_rsq"])
     assert np.array_equal(out, g["magnetic_out"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_adjust_emf_vs_reference_fixture(golden, name):
+    g = golden(name)
+    L, mv, nr, nf = int(g["lmax"]), g["mvals"], int(g["nr"]), int(g["nf"])
+    et, ep = (int(x) for x in g["emf"])
+    assert np.array_equal(O.adjust_emf(L, mv, nr, g["buf"], nf, et, ep), g["emf_out"])
+    assert not np.array_equal(O.adjust_emf(L, mv, nr, g["buf"], nf, ep, et), g["emf_out"])  # roles exchanged
 
 
 @pytest.mark.parametrize("swap", [(1, 2), (3, 4), (6, 7)])
@@ -101,6 +111,8 @@ def test_spacea_vs_compiled_reference(L, mv, nr, pseudo):
     assert np.array_equal(O.rlm_spacea_hydro(L, mv, nr, buf, nf, hydro, prof), R.rlm_spacea_hydro(buf, nf, hydro, prof))
     assert np.array_equal(O.compute_bandcurlb(L, mv, nr, buf, nf, magnetic, o2),
                           R.compute_bandcurlb(buf, nf, magnetic, o2))
+    et, ep = int(perm[16]), int(perm[17])
+    assert np.array_equal(O.adjust_emf(L, mv, nr, buf, nf, et, ep), R.adjust_emf(buf, nf, et, ep))
 
 
 def test_spacea_linear_and_other_fields_untouched():
@@ -113,8 +125,10 @@ def test_spacea_linear_and_other_fields_untouched():
     prof = _random_profiles(rng, nr, True)
     o2 = rng.uniform(0.5, 2.0, nr)
     field = _field_of(L, mv, nr, nf)
+    emf = (int(perm[16]), int(perm[17]))
     for fn, roles in ((lambda x: O.rlm_spacea_hydro(L, mv, nr, x, nf, hydro, prof), hydro),
-                      (lambda x: O.compute_bandcurlb(L, mv, nr, x, nf, magnetic, o2), magnetic)):
+                      (lambda x: O.compute_bandcurlb(L, mv, nr, x, nf, magnetic, o2), magnetic),
+                      (lambda x: O.adjust_emf(L, mv, nr, x, nf, *emf), emf)):
         ra, rb = fn(a), fn(b)
         assert np.array_equal(fn(2.0 * a), 2.0 * ra)
         scale = np.abs(ra).max() + np.abs(rb).max()  # l(l+1) factors make values up to ~1e3
