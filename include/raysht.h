@@ -326,12 +326,25 @@ int raysht_graph_create_batch(raysht_graph **graph, raysht_batch *batch, const d
 /* Cheby_Grid%init (Chebyshev_Polynomials.F90:65-207) for one domain of n
  * zeros-grid points on [rmin, rmax]; grid (host, n) receives the radii. */
 int raysht_cheb_create(raysht_cheb **cheb, int n, double rmin, double rmax, double *grid);
+/* Cheby_Grid%init with ndomains domains (:65-207; ProblemSize.F90's ncheby / domain_bounds):
+ * npoly[ndomains] points and bounds[ndomains + 1] in ascending order of radius, as the
+ * reference takes them; dealias_by (NULL, or ndomains values in the same order) as Init's
+ * optional argument (rda = npoly - dealias_by + 1 where 1 <= dealias_by < npoly).  Like the
+ * reference, the grid holds the domains outermost first: a column is the total of npoly
+ * values, domain h (from the outside) on rows hoff .. hoff + npoly - 1, and grid (host,
+ * total) receives the radii in that order.  Every transform, derivative and dealias call
+ * on such a grid works domain by domain as the reference's loops over domains do; with
+ * several domains each has at most 256 points and the table form runs (RAYSHT_CHEB_DCT is
+ * RAYSHT_EUNSUPPORTED, as is raysht_cheb_table).  ndomains = 1 is raysht_cheb_create. */
+int raysht_cheb_create_domains(raysht_cheb **cheb, int ndomains, const int *npoly, const double *bounds,
+                               const int *dealias_by, double *grid);
 void raysht_cheb_destroy(raysht_cheb *cheb);
 /* The rest of what Cheby_Grid%init computes for its caller and its components
- * (:161-200, :210-248), n values each, any pointer may be NULL: x = cos(theta) the
- * colocation points (descending), theta their angles, weights the radial integration
- * weights (Init's integration_weights argument), deriv_scaling the d/dr factor of the
- * domain map at every point (Cheby_Grid%deriv_scaling). */
+ * (:161-200, :210-248), n values each (n = the total of every domain's points), any pointer
+ * may be NULL: x = cos(theta) the colocation points (descending within each domain), theta
+ * their angles, weights the radial integration weights (Init's integration_weights
+ * argument), deriv_scaling the d/dr factor of the domain map at every point
+ * (Cheby_Grid%deriv_scaling). */
 int raysht_cheb_grid_data(const raysht_cheb *cheb, double *x, double *theta, double *weights,
                           double *deriv_scaling);
 /* Gen_Tn tables (:251-297) as stored: which = 0 cheby_even (n_x x n_even), 1 cheby_odd. */

@@ -88,6 +88,7 @@ def oracle_lib():
         lib.orc_cheby_grid.argtypes = [_I, _D, _D, _P]
         lib.orc_cheby_tables.argtypes = [_I, _P, _P]
         lib.orc_cheby_weights.argtypes = [_I, _D, _D, _P]
+        lib.orc_cheby_weights_domain.argtypes = [_I, _D, _D, _D, _D, _P]
         lib.orc_gauss_legendre_binary128.argtypes = [_I, _P, _P]
         lib.orc_cheby_to_spectral.argtypes = [_I, _L, _P, _P, _P, _P]
         lib.orc_cheby_from_spectral.argtypes = [_I, _L, _P, _P, _P, _P]
@@ -116,6 +117,9 @@ def ref_lib():
         lib.ref_pts.argtypes = [_I, _I, _P, _P]
         lib.ref_cheby_init.argtypes = [_I, _D, _D, _P, _P]
         lib.ref_cheby_tables.argtypes = [_P, _P]
+        lib.ref_cheby_init_domains.argtypes = [_I, _P, _P, _I, _P, _I, _P, _P]
+        lib.ref_cheby_grid_arrays.argtypes = [_P, _P]
+        lib.ref_cheby_dealias.argtypes = [_I, _P]
         lib.ref_cheby_to_spectral.argtypes = [_I, _P, _P]
         lib.ref_cheby_from_spectral.argtypes = [_I, _P, _P]
         lib.ref_cheby_deriv.argtypes = [_I, _I, _P, _P]
@@ -535,6 +539,70 @@ def cheby_deriv(n: int, rmin: float, rmax: float, c: np.ndarray, dorder: int, fu
     return d
 
 
+def cheby_domains(npoly, bounds, dealias_by=None) -> dict:
+    """Initialize_Cheby_Grid (Chebyshev_Polynomials.F90:101-200) with several domains: npoly and
+    bounds in ascending order of radius as the reference takes them, the domains kept outermost
+    first (internal domain h is input domain nd - 1 - h); per domain its row offset, its
+    1-based first dealiased row rda ((2n)/3 + 1, or n - dealias_by + 1 for 1 <= dealias_by < n),
+    the one-domain map onto its bounds (grid, derivative scaling) and the weights with the
+    whole grid's gmax^3 - gmin^3."""
+    nd = len(npoly)
+    gmin, gmax = float(bounds[0]), float(bounds[nd])
+    g = dict(n=[], hoff=[], rda=[], lo=[], hi=[])
+    grid, weights, dscale = [], [], []
+    hoff = 0
+    for h in range(nd):
+        src = nd - 1 - h
+        n, lo, hi = int(npoly[src]), float(bounds[src]), float(bounds[src + 1])
+        rda = (2 * n) // 3 + 1
+        if dealias_by is not None and 1 <= int(dealias_by[src]) < n:
+            rda = n - int(dealias_by[src]) + 1
+        w = np.zeros(n)
+        oracle_lib().orc_cheby_weights_domain(n, lo, hi, gmin, gmax, _ptr(w))
+        grid.append(cheby_grid(n, lo, hi))
+        weights.append(w)
+        dscale.append(np.full(n, cheby_deriv_scaling(n, lo, hi)))
+        for k, v in (("n", n), ("hoff", hoff), ("rda", rda), ("lo", lo), ("hi", hi)):
+            g[k].append(v)
+        hoff += n
+    g.update(ntotal=hoff, grid=np.concatenate(grid), weights=np.concatenate(weights), dscale=np.concatenate(dscale))
+    return g
+
+
+def _per_domain(g, a, fn):
+    """fn(h, block) on every domain's rows of every column of length g['ntotal']"""
+    a = np.asarray(a, dtype=np.float64).reshape(-1, g["ntotal"])
+    out = np.zeros_like(a)
+    for h, (n, o) in enumerate(zip(g["n"], g["hoff"])):
+        out[:, o:o + n] = np.asarray(fn(h, np.ascontiguousarray(a[:, o:o + n]))).reshape(-1, n)
+    return out.ravel()
+
+
+def cheby_domains_to_spectral(g, f):
+    """To_Spectral4D's loop over domains (:390-445): each domain's one-domain transform on its rows."""
+    return _per_domain(g, f, lambda h, b: cheby_to_spectral(g["n"][h], b.ravel()))
+
+
+def cheby_domains_from_spectral(g, c):
+    """From_Spectral4D's loop over domains (:466-556), the c_0 / 2 of each domain included."""
+    return _per_domain(g, c, lambda h, b: cheby_from_spectral(g["n"][h], b.ravel()))
+
+
+def cheby_domains_deriv(g, c, dorder: int):
+    """Cheby_Deriv_Buffer_4D over domains (:560-644): the recursion per domain, then each row
+    times its domain's deriv_scaling**dorder."""
+    return _per_domain(g, c, lambda h, b: cheby_deriv(g["n"][h], g["lo"][h], g["hi"][h], b.ravel(), dorder))
+
+
+def cheby_domains_dealias(g, c):
+    """dealias_buffer4d (:353-374): rows rda .. npoly (1-based) of every domain zeroed."""
+    def zero(h, b):
+        b = b.copy()
+        b[:, g["rda"][h] - 1:] = 0.0
+        return b
+    return _per_domain(g, c, zero)
+
+
 def cheby_dealias(n: int, c: np.ndarray) -> np.ndarray:
     y = np.array(c, dtype=np.float64, copy=True)
     oracle_lib().orc_cheby_dealias(n, y.size // n, _ptr(y))
@@ -607,6 +675,30 @@ class Reference:
         c = np.zeros_like(f)
         self.lib.ref_cheby_to_spectral(f.size // self.ncheb, _ptr(f), _ptr(c))
         return c
+
+    def cheby_init_domains(self, npoly, bounds, dealias_by=None):
+        """the reference's Cheby_Grid%init with several domains; returns (grid, weights)"""
+        nd = len(npoly)
+        npl = np.ascontiguousarray(npoly, dtype=np.int32)
+        bd = np.ascontiguousarray(bounds, dtype=np.float64)
+        db = np.ascontiguousarray(dealias_by if dealias_by is not None else np.zeros(nd), dtype=np.int32)
+        self.ncheb = int(npl.sum())
+        grid, w = np.zeros(self.ncheb), np.zeros(self.ncheb)
+        self.lib.ref_cheby_init_domains(nd, _ptr(npl), _ptr(bd), int(dealias_by is not None), _ptr(db), self.ncheb,
+                                        _ptr(grid), _ptr(w))
+        self.ndom = nd
+        return grid, w
+
+    def cheby_grid_arrays(self):
+        """the grid's deriv_scaling (every radius) and rda (every domain, internal order)"""
+        ds, rda = np.zeros(self.ncheb), np.zeros(self.ndom, dtype=np.int32)
+        self.lib.ref_cheby_grid_arrays(_ptr(ds), _ptr(rda))
+        return ds, rda
+
+    def cheby_dealias(self, c: np.ndarray) -> np.ndarray:
+        b = np.ascontiguousarray(c, dtype=np.float64).copy()
+        self.lib.ref_cheby_dealias(b.size // self.ncheb, _ptr(b))
+        return b
 
     def cheby_from_spectral(self, c: np.ndarray) -> np.ndarray:
         c = np.ascontiguousarray(c, dtype=np.float64)

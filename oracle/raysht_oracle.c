@@ -688,23 +688,31 @@ void orc_cheby_grid(int n, double rmin, double rmax, double *grid)
  * int_scale r_i^2 sqrt(1 - x_i^2), int_scale = (3 pi scaling) / ((rmax^3 - rmin^3) n), the
  * first and last halved; scaling = (rmax - rmin) / (x_1 - x_n) as for the grid.  Products
  * and powers are rounded left to right (x**3 as (x x) x). */
-void orc_cheby_weights(int n, double rmin, double rmax, double *w)
+/* The weights of one domain [lowerb, upperb] of n points of a grid spanning [gmin, gmax]
+ * (Chebyshev_Polynomials.F90:176-200 inside its loop over domains: int_scale takes the whole
+ * grid's gmax^3 - gmin^3, the domain's slope and n). */
+void orc_cheby_weights_domain(int n, double lowerb, double upperb, double gmin, double gmax, double *w)
 {
     const double pi = 3.1415926535897932384626433832795028841972;
     double *th = malloc(sizeof(double) * (size_t)n), *x = malloc(sizeof(double) * (size_t)n);
     cheby_theta(n, th);
     for (int i = 0; i < n; ++i) x[i] = cos(th[i]);
     const double xmin = x[n - 1], xmax = x[0];
-    const double scaling = (rmax - rmin) / (xmax - xmin);
-    const double int_scale = (3 * pi * scaling) / ((rmax * rmax * rmax - rmin * rmin * rmin) * n);
+    const double scaling = (upperb - lowerb) / (xmax - xmin);
+    const double int_scale = (3 * pi * scaling) / ((gmax * gmax * gmax - gmin * gmin * gmin) * n);
     for (int i = 0; i < n; ++i) {
-        const double r = (x[i] - xmin) * scaling + rmin;
+        const double r = (x[i] - xmin) * scaling + lowerb;
         w[i] = int_scale * (r * r) * sqrt(1.0 - x[i] * x[i]);
     }
     w[0] *= 0.5;
     w[n - 1] *= 0.5;
     free(th);
     free(x);
+}
+
+void orc_cheby_weights(int n, double rmin, double rmax, double *w)
+{
+    orc_cheby_weights_domain(n, rmin, rmax, rmin, rmax, w);
 }
 
 /* Legendre_Polynomials.F90:332-373 as stored (Real*16 coloc, gl_weights): the quad

@@ -88,6 +88,7 @@ void orc_permute_3b2b(int nt, int lmax, const int *mvals, int nm, int nr, int nf
 /* One-domain Chebyshev zeros grid: grid on [rmin,rmax] (descending), tables. */
 void orc_cheby_grid(int n, double rmin, double rmax, double *grid);
 void orc_cheby_tables(int n, double *ceven, double *codd);
+void orc_cheby_weights_domain(int n, double lowerb, double upperb, double gmin, double gmax, double *w);
 void orc_cheby_weights(int n, double rmin, double rmax, double *w);
 void orc_gauss_legendre_binary128(int n, void *x16, void *w16);
 void orc_cheby_to_spectral(int n, long ncols, const double *ceven, const double *codd,

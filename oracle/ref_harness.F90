@@ -160,22 +160,63 @@ Contains
         DeAllocate(s)
     End Subroutine ref_pts
 
-    ! One-domain Chebyshev grid on [rmin, rmax] with n points (zeros grid).
+    ! One-domain Chebyshev grid on [rmin, rmax] with n points (zeros grid).  dealias_by is always
+    ! passed, 0 (the default rda) here: Initialize_Cheby_Grid's custom_dealiasing is a local with
+    ! an initializer, hence saved between calls, so a call without the argument after one with
+    ! it would read the absent argument.
     Subroutine ref_cheby_init(n, rmin, rmax, grid, weights) bind(C, name='ref_cheby_init')
         Integer(C_INT), Value :: n
         Real(C_DOUBLE), Value :: rmin, rmax
         Real(C_DOUBLE), Intent(Out) :: grid(n), weights(n)
-        Integer :: npoly(1)
+        Integer :: npoly(1), db(1)
         Real*8 :: bounds(2)
         Type(Cheby_Grid) :: fresh
         npoly(1) = n
         bounds(1) = rmin
         bounds(2) = rmax
+        db(1) = 0
         h_grid = fresh
-        Call h_grid%init(grid, weights, 1, npoly, bounds)
+        Call h_grid%init(grid, weights, 1, npoly, bounds, dealias_by=db)
         h_grid_ready = .true.
         h_ncheb = n
     End Subroutine ref_cheby_init
+
+    ! Cheby_Grid%init with nd domains: npoly(nd), bounds(nd + 1) ascending, dealias_by(nd) when
+    ! has_db /= 0, else zeros (the default rda; passed always, as in ref_cheby_init); ntotal =
+    ! sum(npoly) radii into grid and weights.
+    Subroutine ref_cheby_init_domains(nd, npoly, bounds, has_db, dealias_by, ntotal, grid, weights) &
+            bind(C, name='ref_cheby_init_domains')
+        Integer(C_INT), Value :: nd, has_db, ntotal
+        Integer(C_INT), Intent(In) :: npoly(nd), dealias_by(nd)
+        Real(C_DOUBLE), Intent(In) :: bounds(nd + 1)
+        Real(C_DOUBLE), Intent(Out) :: grid(ntotal), weights(ntotal)
+        Integer :: np(nd), db(nd)
+        Real*8 :: bd(nd + 1)
+        Type(Cheby_Grid) :: fresh
+        np = npoly
+        db = dealias_by
+        bd = bounds
+        If (has_db == 0) db = 0
+        h_grid = fresh
+        Call h_grid%init(grid, weights, nd, np, bd, dealias_by=db)
+        h_grid_ready = .true.
+        h_ncheb = ntotal
+    End Subroutine ref_cheby_init_domains
+
+    ! the grid's deriv_scaling(1:ntotal) and rda(1:domain_count)
+    Subroutine ref_cheby_grid_arrays(dscale, rda) bind(C, name='ref_cheby_grid_arrays')
+        Real(C_DOUBLE), Intent(Out) :: dscale(*)
+        Integer(C_INT), Intent(Out) :: rda(*)
+        dscale(1:h_grid%ntotal) = h_grid%deriv_scaling(1:h_grid%ntotal)
+        rda(1:h_grid%domain_count) = h_grid%rda(1:h_grid%domain_count)
+    End Subroutine ref_cheby_grid_arrays
+
+    ! dealias_buffer4d on a (n, ncols, 1, 1) buffer, in place
+    Subroutine ref_cheby_dealias(ncols, buf) bind(C, name='ref_cheby_dealias')
+        Integer(C_INT), Value :: ncols
+        Real(C_DOUBLE), Intent(InOut) :: buf(h_ncheb, ncols, 1, 1)
+        Call h_grid%dealias_buffer(buf)
+    End Subroutine ref_cheby_dealias
 
     Subroutine ref_cheby_tables(ceven, codd) bind(C, name='ref_cheby_tables')
         Real(C_DOUBLE), Intent(Out) :: ceven(*), codd(*)

@@ -63,6 +63,7 @@ SIGNATURES = {
     "raysht_batch_set_schedule": (_I, [_P, _I, _I]),
     "raysht_graph_destroy": (None, [_P]),
     "raysht_cheb_create": (_I, [ctypes.POINTER(_P), _I, _D, _D, _P]),
+    "raysht_cheb_create_domains": (_I, [ctypes.POINTER(_P), _I, _P, _P, _P, _P]),
     "raysht_cheb_destroy": (None, [_P]),
     "raysht_cheb_table": (_I, [_P, _I, _P]),
     "raysht_cheb_grid_data": (_I, [_P, _P, _P, _P, _P]),

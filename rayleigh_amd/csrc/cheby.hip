@@ -202,10 +202,11 @@ __device__ __forceinline__ void load_table_regs(const double *__restrict__ t, in
     }
 }
 
-// From_Spectral4D with K = coefficients (KB * 4 >= k4), rows = radii i < n_x.
-template <int KB, int NW, int TC>
+// From_Spectral4D with K = coefficients (KB * 4 >= k4), rows = radii i < n_x.  Columns are
+// n apart, or ld apart (SD: one domain's rows of a grid of several).
+template <int KB, int NW, int TC, bool SD>
 __global__ __launch_bounds__(64 * NW) void cheb_from_spectral_reg_kernel(int n, int n_x, int n_even, int k4, int ip,
-                                                                     long ncols, const double *__restrict__ t_e,
+                                                                     long ncols, long ld, const double *__restrict__ t_e,
                                                                      const double *__restrict__ t_o,
                                                                      const double *__restrict__ c, double *__restrict__ f)
 {
@@ -226,12 +227,23 @@ __global__ __launch_bounds__(64 * NW) void cheb_from_spectral_reg_kernel(int n, 
     constexpr int kU = (kMaxPer + nthr - 1) / nthr;
     double pre[kU];
     auto fetch = [&](long tile) {
-        const double *src = c + tile * TC * (long)n;
-        const long left = (ncols - tile * TC) * (long)n;
+        if constexpr (SD) {
+            const double *src = c + tile * TC * ld;
+            const long left = ncols - tile * TC;  // columns
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int idx = tid + u * nthr;
-            pre[u] = (idx < per && idx < left) ? src[idx] : 0.0;
+            for (int u = 0; u < kU; ++u) {
+                const int idx = tid + u * nthr;
+                const int col = small_div(idx, inv_n), q = idx - col * n;
+                pre[u] = (idx < per && col < left) ? src[col * ld + q] : 0.0;
+            }
+        } else {
+            const double *src = c + tile * TC * (long)n;
+            const long left = (ncols - tile * TC) * (long)n;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int idx = tid + u * nthr;
+                pre[u] = (idx < per && idx < left) ? src[idx] : 0.0;
+            }
         }
     };
     long tile = blockIdx.x;
@@ -275,7 +287,7 @@ __global__ __launch_bounds__(64 * NW) void cheb_from_spectral_reg_kernel(int n, 
                     const long col = tile * TC + cl;
                     if (col < ncols) {
                         const double e = acc_e[cb][v], o = acc_o[cb][v], c0h = ce[cl * LD] / 2.0;
-                        double *fc = f + col * n;
+                        double *fc = f + col * (SD ? ld : (long)n);
                         if (odd_n && i == n_x - 1) {
                             fc[i] = (e * 2.0 + o * 2.0) - c0h;
                         } else {
@@ -288,10 +300,12 @@ __global__ __launch_bounds__(64 * NW) void cheb_from_spectral_reg_kernel(int n, 
     }
 }
 
-// To_Spectral4D with K = radii i < n_x (KB * 4 >= ix4), rows = coefficient pairs k.
-template <int KB, int NW, int TC>
+// To_Spectral4D with K = radii i < n_x (KB * 4 >= ix4), rows = coefficient pairs k; columns
+// n or (SD) ld apart.
+template <int KB, int NW, int TC, bool SD>
 __global__ __launch_bounds__(64 * NW) void cheb_to_spectral_reg_kernel(int n, int n_x, int n_even, int n_odd, int ix4,
-                                                                   int kp, long ncols, const double *__restrict__ tt_e,
+                                                                   int kp, long ncols, long ld,
+                                                                   const double *__restrict__ tt_e,
                                                                    const double *__restrict__ tt_o,
                                                                    const double *__restrict__ f, double *__restrict__ c)
 {
@@ -307,19 +321,20 @@ __global__ __launch_bounds__(64 * NW) void cheb_to_spectral_reg_kernel(int n, in
     const float inv_nx = 1.0f / (float)n_x;
     const long ntiles = (ncols + TC - 1) / TC;
     const int per = TC * n_x;  // folded pairs per tile
+    const long cs = SD ? ld : (long)n;
     constexpr int kMaxPer = TC * 4 * KB;  // per = 32 n_x <= 32 * 4 KB
     constexpr int kU = (kMaxPer + nthr - 1) / nthr;
     double pa[kU], pb[kU];
     auto fetch = [&](long tile) {
-        const double *src = f + tile * TC * (long)n;
+        const double *src = f + tile * TC * cs;
         const long cols_left = ncols - tile * TC;
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int idx = tid + u * nthr;
             const int col = small_div(idx, inv_nx), i = idx - col * n_x;
             const bool ok = idx < per && col < cols_left;
-            pa[u] = ok ? src[(long)col * n + i] : 0.0;
-            pb[u] = ok ? src[(long)col * n + (n - 1 - i)] : 0.0;
+            pa[u] = ok ? src[(long)col * cs + i] : 0.0;
+            pb[u] = ok ? src[(long)col * cs + (n - 1 - i)] : 0.0;
         }
     };
     const double alpha = 2.0 / (double)n;
@@ -360,20 +375,20 @@ __global__ __launch_bounds__(64 * NW) void cheb_to_spectral_reg_kernel(int n, in
             for (int v = 0; v < 4; ++v) {
                 const long col = tile * TC + 16 * cb + lg + 4 * v;
                 if (col < ncols) {
-                    if (k < n_even) c[col * n + 2 * k] = alpha * acc_e[cb][v];
-                    if (k < n_odd) c[col * n + 2 * k + 1] = alpha * acc_o[cb][v];
+                    if (k < n_even) c[col * cs + 2 * k] = alpha * acc_e[cb][v];
+                    if (k < n_odd) c[col * cs + 2 * k + 1] = alpha * acc_o[cb][v];
                 }
             }
     }
 }
 
-__global__ void cheb_dealias_kernel(int n, int first, long ncols, double *__restrict__ c)
+__global__ void cheb_dealias_kernel(int n, int first, long ncols, long ld, double *__restrict__ c)
 {
     const int width = n - first;
     const long total = ncols * width;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
         const long col = i / width;
-        c[col * n + first + (int)(i % width)] = 0.0;
+        c[col * ld + first + (int)(i % width)] = 0.0;
     }
 }
 
@@ -390,7 +405,7 @@ __global__ void cheb_dealias_kernel(int n, int first, long ncols, double *__rest
 // separately, as the reference's compiled recursion does.
 constexpr int kDerivCols = 256, kDerivChunk = 16, kDerivMaxOrder = 4;
 
-__global__ __launch_bounds__(kDerivCols) void cheb_deriv_kernel(int n, long ncols, int dorder, double scale,
+__global__ __launch_bounds__(kDerivCols) void cheb_deriv_kernel(int n, long ld, long ncols, int dorder, double scale,
                                                                 const double *__restrict__ c, double *__restrict__ d)
 {
 #pragma clang fp contract(off)
@@ -409,7 +424,7 @@ __global__ __launch_bounds__(kDerivCols) void cheb_deriv_kernel(int n, long ncol
 #pragma unroll
         for (int q = 0; q < kDerivChunk; ++q) {
             const int v = q * kDerivCols + tid, cl = v / kDerivChunk, e = v % kDerivChunk;
-            pf[q] = cl < cols && e < w ? c[(col0 + cl) * n + i0 + e] : 0.0;
+            pf[q] = cl < cols && e < w ? c[(col0 + cl) * ld + i0 + e] : 0.0;
         }
     };
     fetch(((n - 1) / kDerivChunk) * kDerivChunk);
@@ -445,7 +460,7 @@ __global__ __launch_bounds__(kDerivCols) void cheb_deriv_kernel(int n, long ncol
 #pragma unroll 4
         for (int q = 0; q < kDerivChunk; ++q) {
             const int v = q * kDerivCols + tid, cl = v / kDerivChunk, e = v % kDerivChunk;
-            if (cl < cols && e < w) d[(col0 + cl) * n + i0 + e] = tile[cl][e];
+            if (cl < cols && e < w) d[(col0 + cl) * ld + i0 + e] = tile[cl][e];
         }
         __syncthreads();
     }
@@ -762,20 +777,28 @@ namespace {
 constexpr int kChebRegBlocks = 1024;  // persistent workgroups (about four per CU)
 
 // register-table kernels: (KB, NW) = K steps of 4 held per parity, waves (16 rows each)
-template <int KB, int NW, int TC>
-int launch_cheb_reg(raysht_cheb *cb, bool to, const double *in, double *out, long ncols, hipStream_t st)
+template <int KB, int NW, int TC, bool SD>
+int launch_cheb_reg_sd(raysht_cheb *cb, bool to, const double *in, double *out, long ncols, long ld, hipStream_t st)
 {
     const long ntiles = (ncols + TC - 1) / TC;
     const unsigned blocks = (unsigned)std::min<long>(ntiles, kChebRegBlocks);
     const size_t lds = sizeof(double) * 2 * TC * (4 * KB + 1);
     if (to)
-        hipLaunchKernelGGL((cheb_to_spectral_reg_kernel<KB, NW, TC>), dim3(blocks), dim3(64 * NW), lds, st, cb->n,
-                           cb->n_x, cb->n_even, cb->n_odd, cb->ix4, cb->kp, ncols, cb->d_tt_e, cb->d_tt_o, in, out);
+        hipLaunchKernelGGL((cheb_to_spectral_reg_kernel<KB, NW, TC, SD>), dim3(blocks), dim3(64 * NW), lds, st, cb->n,
+                           cb->n_x, cb->n_even, cb->n_odd, cb->ix4, cb->kp, ncols, ld, cb->d_tt_e, cb->d_tt_o, in,
+                           out);
     else
-        hipLaunchKernelGGL((cheb_from_spectral_reg_kernel<KB, NW, TC>), dim3(blocks), dim3(64 * NW), lds, st, cb->n,
-                           cb->n_x, cb->n_even, cb->k4, cb->ip, ncols, cb->d_t_e, cb->d_t_o, in, out);
+        hipLaunchKernelGGL((cheb_from_spectral_reg_kernel<KB, NW, TC, SD>), dim3(blocks), dim3(64 * NW), lds, st, cb->n,
+                           cb->n_x, cb->n_even, cb->k4, cb->ip, ncols, ld, cb->d_t_e, cb->d_t_o, in, out);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
+}
+
+template <int KB, int NW, int TC>
+int launch_cheb_reg(raysht_cheb *cb, bool to, const double *in, double *out, long ncols, long ld, hipStream_t st)
+{
+    return ld == cb->n ? launch_cheb_reg_sd<KB, NW, TC, false>(cb, to, in, out, ncols, ld, st)
+                       : launch_cheb_reg_sd<KB, NW, TC, true>(cb, to, in, out, ncols, ld, st);
 }
 
 // persistent workgroups: as many as are resident at once (a grid of several rounds leaves
@@ -829,42 +852,48 @@ bool cheb_dct_dispatch(raysht_cheb *cb, bool to, const double *in, double *out, 
     }
 }
 
-// returns false when n is too long for the register tables (> 256)
-bool cheb_reg_dispatch(raysht_cheb *cb, bool to, const double *in, double *out, long ncols, hipStream_t st, int *rc)
+// returns false when n is too long for the register tables (> 256); ld = column stride
+bool cheb_reg_dispatch(raysht_cheb *cb, bool to, const double *in, double *out, long ncols, long ld, hipStream_t st,
+                       int *rc)
 {
     // K steps: to = radii (ix4), from = coefficients (k4); rows: to = kp, from = ip
     const int ksteps = (std::max(cb->ix4, cb->k4) + 3) / 4, rows = std::max(cb->kp, cb->ip);
     if (ksteps <= 8 && rows <= 32)
-        *rc = launch_cheb_reg<8, 2, 32>(cb, to, in, out, ncols, st);
+        *rc = launch_cheb_reg<8, 2, 32>(cb, to, in, out, ncols, ld, st);
     else if (ksteps <= 16 && rows <= 64)
-        *rc = launch_cheb_reg<16, 4, 32>(cb, to, in, out, ncols, st);
+        *rc = launch_cheb_reg<16, 4, 32>(cb, to, in, out, ncols, ld, st);
     else if (ksteps <= 32 && rows <= 128)
-        *rc = launch_cheb_reg<32, 8, 16>(cb, to, in, out, ncols, st);
+        *rc = launch_cheb_reg<32, 8, 16>(cb, to, in, out, ncols, ld, st);
     else
         return false;
     return true;
 }
+
+// a grid of several domains: each domain's table form on its rows of every column
+int cheb_domains_dispatch(raysht_cheb *cb, bool to, const double *in, double *out, long ncols, hipStream_t st)
+{
+    for (size_t h = 0; h < cb->dom.size(); ++h) {
+        int rc = RAYSHT_OK;
+        if (!cheb_reg_dispatch(cb->dom[h], to, in + cb->hoff[h], out + cb->hoff[h], ncols, cb->n, st, &rc))
+            return RAYSHT_EUNSUPPORTED;
+        if (rc != RAYSHT_OK) return rc;
+    }
+    return RAYSHT_OK;
+}
 }  // namespace
 }  // namespace raysht
 
-extern "C" {
-
-int raysht_cheb_create(raysht_cheb **out, int n, double rmin, double rmax, double *grid)
+namespace raysht {
+namespace {
+// One domain of n points: Gen_Colocation_Points' zeros (:233-243, the angle grown by repeated
+// addition), Gen_Tn's parity tables (:251-297), their padded device copies and, where allowed,
+// the DCT form's tables.  The map onto radii is the caller's.
+int cheb_build(raysht_cheb *cb, int n, bool allow_dct)
 {
-    if (!out) return RAYSHT_EINVAL;
-    *out = nullptr;
-    if (n < 2 || n > kChebMaxN || !(rmax > rmin)) return RAYSHT_EINVAL;
-    raysht_cheb *cb = new (std::nothrow) raysht_cheb();
-    if (!cb) return RAYSHT_ENOMEM;
-    if (hipGetDevice(&cb->device) != hipSuccess) {
-        delete cb;
-        return RAYSHT_EHIP;
-    }
     cb->n = n;
     cb->n_odd = n / 2;
     cb->n_even = cb->n_odd + n % 2;
     cb->n_x = cb->n_even;
-    // Gen_Colocation_Points (zeros branch, :233-243): the angle grows by repeated addition
     const double pi = 3.1415926535897932384626433832795028841972;
     std::vector<double> theta(n), x(n);
     const double dtheta = pi / n;
@@ -874,26 +903,8 @@ int raysht_cheb_create(raysht_cheb **out, int n, double rmin, double rmax, doubl
         x[i] = std::cos(arg);
         arg = arg + dtheta;
     }
-    {
-        // Initialize_Cheby_Grid (:161-200): affine map of [x_n, x_1] onto [rmin, rmax]; the
-        // derivative scaling is the reciprocal of the map's slope; the radial integration
-        // weights int_scale r_i^2 sqrt(1 - x_i^2), halved at both ends, with
-        // int_scale = 3 pi slope / ((rmax^3 - rmin^3) n); each rounded in the order written there
-        const double xmin = x[n - 1], xmax = x[0];
-        const double scaling = (rmax - rmin) / (xmax - xmin);
-        std::vector<double> r(n);
-        for (int i = 0; i < n; ++i) r[i] = (x[i] - xmin) * scaling + rmin;
-        if (grid)
-            for (int i = 0; i < n; ++i) grid[i] = r[i];
-        const double int_scale = (3.0 * pi * scaling) / ((rmax * rmax * rmax - rmin * rmin * rmin) * (double)n);
-        cb->weights.resize(n);
-        for (int i = 0; i < n; ++i) cb->weights[i] = int_scale * (r[i] * r[i]) * std::sqrt(1.0 - x[i] * x[i]);
-        cb->weights[0] *= 0.5;
-        cb->weights[n - 1] *= 0.5;
-        cb->deriv_scaling = 1.0 / scaling;
-        cb->x = x;
-        cb->theta = theta;
-    }
+    cb->x = x;
+    cb->theta = theta;
     // Gen_Tn (:251-297): cheby_even(i, k) = T_{2k-2}(x_i), cheby_odd(i, k) = T_{2k-1}(x_i), i <= n_x
     cb->ceven.assign((size_t)cb->n_x * cb->n_even, 0.0);
     cb->codd.assign((size_t)cb->n_x * (cb->n_odd > 0 ? cb->n_odd : 1), 0.0);
@@ -925,7 +936,7 @@ int raysht_cheb_create(raysht_cheb **out, int n, double rmin, double rmax, doubl
     // DCT form: twiddles, e^{-i pi j/(2n)}, and the two correction sequences (delta mirrored
     // as the parity split uses the first half's angles for the second half)
     std::vector<double> dct;
-    if (dct_length(n)) {
+    if (allow_dct && dct_length(n)) {
         std::vector<double> twc(n), tws(n), qc(n), qs(n), delta(n);
         cheb_dct_tables(n, theta.data(), twc.data(), tws.data(), qc.data(), qs.data(), delta.data());
         std::vector<double> dp(n);
@@ -971,21 +982,107 @@ int raysht_cheb_create(raysht_cheb **out, int n, double rmin, double rmax, doubl
         const size_t bytes = u.src->size() * sizeof(double);
         if (hipMalloc(u.dst, bytes) != hipSuccess) {
             *u.dst = nullptr;
-            raysht_cheb_destroy(cb);
             return RAYSHT_ENOMEM;
         }
-        if (hipMemcpy(*u.dst, u.src->data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
-            raysht_cheb_destroy(cb);
-            return RAYSHT_EHIP;
-        }
+        if (hipMemcpy(*u.dst, u.src->data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return RAYSHT_EHIP;
     }
+    return RAYSHT_OK;
+}
+}  // namespace
+}  // namespace raysht
+
+extern "C" {
+
+int raysht_cheb_create_domains(raysht_cheb **out, int ndomains, const int *npoly, const double *bounds,
+                               const int *dealias_by, double *grid)
+{
+    if (!out) return RAYSHT_EINVAL;
+    *out = nullptr;
+    if (ndomains < 1 || !npoly || !bounds) return RAYSHT_EINVAL;
+    long ntotal = 0;
+    for (int i = 0; i < ndomains; ++i) {
+        if (npoly[i] < 2 || !(bounds[i + 1] > bounds[i])) return RAYSHT_EINVAL;
+        // several domains run the register-table kernels, which take up to 256 points
+        if (npoly[i] > (ndomains == 1 ? kChebMaxN : 256)) return RAYSHT_EUNSUPPORTED;
+        ntotal += npoly[i];
+    }
+    if (ntotal > (1L << 20)) return RAYSHT_EUNSUPPORTED;
+    raysht_cheb *cb = new (std::nothrow) raysht_cheb();
+    if (!cb) return RAYSHT_ENOMEM;
+    if (hipGetDevice(&cb->device) != hipSuccess) {
+        delete cb;
+        return RAYSHT_EHIP;
+    }
+    // Initialize_Cheby_Grid (:101-200): the domains and their bounds reversed (outermost
+    // first), rda = (2 n) / 3 + 1 or n - dealias_by + 1, and per domain the affine map of
+    // [x_n, x_1] onto [lowerb, upperb], the derivative scaling (the reciprocal of the map's
+    // slope) and the radial integration weights int_scale r_i^2 sqrt(1 - x_i^2), halved at
+    // the domain's ends, int_scale = 3 pi slope / ((gmax^3 - gmin^3) n); each rounded in the
+    // order written there
+    const double pi = 3.1415926535897932384626433832795028841972;
+    const double gmax = bounds[ndomains], gmin = bounds[0];
+    const double gcube = gmax * gmax * gmax - gmin * gmin * gmin;
+    std::vector<double> r_all, x_all, th_all, w_all, ds_all;
+    int hoff = 0;
+    for (int h = 0; h < ndomains; ++h) {
+        const int src = ndomains - 1 - h, n = npoly[src];
+        const double upperb = bounds[src + 1], lowerb = bounds[src];
+        raysht_cheb *d = cb;
+        if (ndomains > 1) {
+            d = new (std::nothrow) raysht_cheb();
+            if (!d) {
+                raysht_cheb_destroy(cb);
+                return RAYSHT_ENOMEM;
+            }
+            d->device = cb->device;
+            cb->dom.push_back(d);
+        }
+        if (int rc = cheb_build(d, n, ndomains == 1)) {
+            raysht_cheb_destroy(cb);
+            return rc;
+        }
+        int rda = (2 * n) / 3 + 1;
+        if (dealias_by && dealias_by[src] >= 1 && dealias_by[src] < n) rda = n - dealias_by[src] + 1;
+        const double xmin = d->x[n - 1], xmax = d->x[0];
+        const double scaling = (upperb - lowerb) / (xmax - xmin);
+        const double int_scale = (3.0 * pi * scaling) / (gcube * (double)n);
+        d->deriv_scaling = 1.0 / scaling;
+        for (int i = 0; i < n; ++i) {
+            const double r = (d->x[i] - xmin) * scaling + lowerb;
+            r_all.push_back(r);
+            x_all.push_back(d->x[i]);
+            th_all.push_back(d->theta[i]);
+            w_all.push_back(int_scale * (r * r) * std::sqrt(1.0 - d->x[i] * d->x[i]));
+            ds_all.push_back(d->deriv_scaling);
+        }
+        w_all[hoff] *= 0.5;
+        w_all[hoff + n - 1] *= 0.5;
+        cb->hoff.push_back(hoff);
+        cb->rda.push_back(rda);
+        hoff += n;
+    }
+    cb->n = (int)ntotal;
+    cb->x = x_all;
+    cb->theta = th_all;
+    cb->weights = w_all;
+    cb->dscale = ds_all;
+    if (ndomains > 1) cb->form = RAYSHT_CHEB_MATRIX;
+    if (grid)
+        for (long i = 0; i < ntotal; ++i) grid[i] = r_all[i];
     *out = cb;
     return RAYSHT_OK;
+}
+
+int raysht_cheb_create(raysht_cheb **out, int n, double rmin, double rmax, double *grid)
+{
+    const double bounds[2] = {rmin, rmax};
+    return raysht_cheb_create_domains(out, 1, &n, bounds, nullptr, grid);
 }
 
 void raysht_cheb_destroy(raysht_cheb *cb)
 {
     if (!cb) return;
+    for (raysht_cheb *d : cb->dom) raysht_cheb_destroy(d);
     for (double *d : {cb->d_tt_e, cb->d_tt_o, cb->d_t_e, cb->d_t_o, cb->d_dct})
         if (d) (void)hipFree(d);
     delete cb;
@@ -998,7 +1095,7 @@ int raysht_cheb_grid_data(const raysht_cheb *cb, double *x, double *theta, doubl
         if (x) x[i] = cb->x[i];
         if (theta) theta[i] = cb->theta[i];
         if (weights) weights[i] = cb->weights[i];
-        if (deriv_scaling) deriv_scaling[i] = cb->deriv_scaling;
+        if (deriv_scaling) deriv_scaling[i] = cb->dscale[i];
     }
     return RAYSHT_OK;
 }
@@ -1006,6 +1103,7 @@ int raysht_cheb_grid_data(const raysht_cheb *cb, double *x, double *theta, doubl
 int raysht_cheb_table(const raysht_cheb *cb, int which, double *host_out)
 {
     if (!cb || !host_out || which < 0 || which > 1) return RAYSHT_EINVAL;
+    if (!cb->dom.empty()) return RAYSHT_EUNSUPPORTED;  // one table set per domain
     const std::vector<double> &t = which == 0 ? cb->ceven : cb->codd;
     const size_t n = which == 0 ? (size_t)cb->n_x * cb->n_even : (size_t)cb->n_x * cb->n_odd;
     for (size_t i = 0; i < n; ++i) host_out[i] = t[i];
@@ -1022,8 +1120,9 @@ int raysht_cheb_to_spectral(raysht_cheb *cb, const double *f, double *c, long nc
     if (!cb || ncols < 0 || (ncols > 0 && (!f || !c))) return RAYSHT_EINVAL;
     if (ncols == 0) return RAYSHT_OK;
     int rc = RAYSHT_OK;
+    if (!cb->dom.empty()) return cheb_domains_dispatch(cb, true, f, c, ncols, reinterpret_cast<hipStream_t>(stream));
     if (cheb_dct_dispatch(cb, true, f, c, ncols, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
-    if (cheb_reg_dispatch(cb, true, f, c, ncols, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
+    if (cheb_reg_dispatch(cb, true, f, c, ncols, cb->n, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
     const int waves = cheb_waves(cb->n), tc = 16 * waves;
     const long blocks = (ncols + tc - 1) / tc;
     if (blocks > 0x7fffffffL) return RAYSHT_EINVAL;
@@ -1040,8 +1139,9 @@ int raysht_cheb_from_spectral(raysht_cheb *cb, const double *c, double *f, long 
     if (!cb || ncols < 0 || (ncols > 0 && (!f || !c))) return RAYSHT_EINVAL;
     if (ncols == 0) return RAYSHT_OK;
     int rc = RAYSHT_OK;
+    if (!cb->dom.empty()) return cheb_domains_dispatch(cb, false, c, f, ncols, reinterpret_cast<hipStream_t>(stream));
     if (cheb_dct_dispatch(cb, false, c, f, ncols, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
-    if (cheb_reg_dispatch(cb, false, c, f, ncols, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
+    if (cheb_reg_dispatch(cb, false, c, f, ncols, cb->n, reinterpret_cast<hipStream_t>(stream), &rc)) return rc;
     const int waves = cheb_waves(cb->n), tc = 16 * waves;
     const long blocks = (ncols + tc - 1) / tc;
     if (blocks > 0x7fffffffL) return RAYSHT_EINVAL;
@@ -1056,7 +1156,7 @@ int raysht_cheb_from_spectral(raysht_cheb *cb, const double *c, double *f, long 
 int raysht_cheb_set_form(raysht_cheb *cb, int form)
 {
     if (!cb || (form != RAYSHT_CHEB_DCT && form != RAYSHT_CHEB_MATRIX)) return RAYSHT_EINVAL;
-    if (form == RAYSHT_CHEB_DCT && !cb->d_dct) return RAYSHT_EUNSUPPORTED;
+    if (form == RAYSHT_CHEB_DCT && !cb->d_dct) return RAYSHT_EUNSUPPORTED;  // several domains: table form only
     cb->form = form;
     return RAYSHT_OK;
 }
@@ -1072,26 +1172,36 @@ int raysht_cheb_deriv(raysht_cheb *cb, const double *c, double *d, long ncols, i
     if (c < d + n_el && d < c + n_el) return RAYSHT_EINVAL;
     const long blocks = (ncols + kDerivCols - 1) / kDerivCols;
     if (blocks > 0x7fffffffL) return RAYSHT_EUNSUPPORTED;
-    // deriv_scaling**dorder as the compiled Fortran power forms it for small orders
-    const double s = cb->deriv_scaling;
-    const double scale = dorder == 1 ? s : dorder == 2 ? s * s : dorder == 3 ? (s * s) * s : (s * s) * (s * s);
-    hipLaunchKernelGGL(cheb_deriv_kernel, dim3((unsigned)blocks), dim3(kDerivCols), 0,
-                       reinterpret_cast<hipStream_t>(stream), cb->n, ncols, dorder, scale, c, d);
-    RAYSHT_HIP_TRY(hipGetLastError());
+    // one launch per domain (its rows of every column), each with its own scaling
+    const int nd = cb->dom.empty() ? 1 : (int)cb->dom.size();
+    for (int h = 0; h < nd; ++h) {
+        const raysht_cheb *dm = cb->dom.empty() ? cb : cb->dom[h];
+        // deriv_scaling**dorder as the compiled Fortran power forms it for small orders
+        const double s = dm->deriv_scaling;
+        const double scale = dorder == 1 ? s : dorder == 2 ? s * s : dorder == 3 ? (s * s) * s : (s * s) * (s * s);
+        hipLaunchKernelGGL(cheb_deriv_kernel, dim3((unsigned)blocks), dim3(kDerivCols), 0,
+                           reinterpret_cast<hipStream_t>(stream), dm->n, (long)cb->n, ncols, dorder, scale,
+                           c + cb->hoff[h], d + cb->hoff[h]);
+        RAYSHT_HIP_TRY(hipGetLastError());
+    }
     return RAYSHT_OK;
 }
 
 int raysht_cheb_dealias(raysht_cheb *cb, double *c, long ncols, void *stream)
 {
     if (!cb || ncols < 0 || (ncols > 0 && !c)) return RAYSHT_EINVAL;
-    const int rda = (2 * cb->n) / 3 + 1;  // Initialize_Cheby_Grid :128
-    const int first = rda - 1;
-    if (ncols == 0 || first >= cb->n) return RAYSHT_OK;
-    const long total = ncols * (cb->n - first);
-    const long blocks = std::min<long>((total + 255) / 256, 1L << 20);
-    hipLaunchKernelGGL(cheb_dealias_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                       reinterpret_cast<hipStream_t>(stream), cb->n, first, ncols, c);
-    RAYSHT_HIP_TRY(hipGetLastError());
+    if (ncols == 0) return RAYSHT_OK;
+    // dealias_buffer4d (:353-374): rows rda .. npoly (1-based) of every domain zeroed
+    const int nd = cb->dom.empty() ? 1 : (int)cb->dom.size();
+    for (int h = 0; h < nd; ++h) {
+        const int n = cb->dom.empty() ? cb->n : cb->dom[h]->n, first = cb->rda[h] - 1;
+        if (first >= n) continue;
+        const long total = ncols * (n - first);
+        const long blocks = std::min<long>((total + 255) / 256, 1L << 20);
+        hipLaunchKernelGGL(cheb_dealias_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                           reinterpret_cast<hipStream_t>(stream), n, first, ncols, (long)cb->n, c + cb->hoff[h]);
+        RAYSHT_HIP_TRY(hipGetLastError());
+    }
     return RAYSHT_OK;
 }
 

@@ -124,6 +124,14 @@ struct raysht_cheb {
     double *d_dct = nullptr;
     double dct_inv_s = 1.0;  // 1 / the power of two the analysis correction is scaled by
     int form = 0;            // RAYSHT_CHEB_DCT where d_dct exists, else RAYSHT_CHEB_MATRIX
+    // Every grid keeps its domains in the reference's internal order (outermost first): one
+    // domain is the grid itself; several (Initialize_Cheby_Grid with ndomains > 1) are
+    // table-only grids in `dom`, domain h on rows hoff[h] .. hoff[h] + dom[h]->n - 1 of each
+    // column of n = the total length.  rda[h] = the domain's first dealiased row + 1
+    // (1-based, as the reference's rda), dscale = deriv_scaling per row.
+    std::vector<raysht_cheb *> dom;
+    std::vector<int> hoff, rda;
+    std::vector<double> dscale;
 };
 
 namespace raysht {

@@ -148,6 +148,17 @@ Module RaySHT_Math_Layer
             Real(C_DOUBLE), Intent(Out) :: grid(*)
             Integer(C_INT) :: st
         End Function
+        Function c_cheb_create_domains(cb, nd, npoly, bounds, dealias_by, grid) &
+                Bind(C, name='raysht_cheb_create_domains') Result(st)
+            Import :: C_PTR, C_INT, C_DOUBLE
+            Type(C_PTR), Intent(Out) :: cb
+            Integer(C_INT), Value :: nd
+            Integer(C_INT), Intent(In) :: npoly(*)
+            Real(C_DOUBLE), Intent(In) :: bounds(*)
+            Type(C_PTR), Value :: dealias_by
+            Real(C_DOUBLE), Intent(Out) :: grid(*)
+            Integer(C_INT) :: st
+        End Function
         Subroutine c_cheb_destroy(cb) Bind(C, name='raysht_cheb_destroy')
             Import :: C_PTR
             Type(C_PTR), Value :: cb
@@ -609,12 +620,12 @@ Contains
     End Subroutine Cheby_d_by_dr_cp
 
     ! ------------------------------------------------------------------ Cheby_Grid
-    ! Initialize_Cheby_Grid (Chebyshev_Polynomials.F90:65-207) for one domain of npoly(1)
-    ! zeros-grid points between bounds(1) and bounds(2) (ascending, as the caller passes
-    ! them): grid(1:n) receives the radii (descending), integration_weights(1:n) the radial
-    ! integration weights; the components are filled as the reference fills them.  dmax and
-    ! nthread are accepted (the device kernels need neither); more than one domain, or a
-    ! custom dealias_by, stop with a message.
+    ! Initialize_Cheby_Grid (Chebyshev_Polynomials.F90:65-207) for ndomains domains of npoly(i)
+    ! zeros-grid points between bounds(i) and bounds(i + 1) (ascending, as the caller passes
+    ! them) and the optional dealias_by: grid(1:n) receives the radii (descending, n = the total
+    ! of npoly), integration_weights(1:n) the radial integration weights; the components are
+    ! filled as the reference fills them (domains outermost first, as Initialize_Cheby_Grid
+    ! stores them).  dmax and nthread are accepted (the device kernels need neither).
     Subroutine Initialize_Cheby_Grid(self, grid, integration_weights, ndomains, npoly, bounds, dmax, nthread, &
                                      dealias_by, verbose)
         Class(Cheby_Grid) :: self
@@ -622,48 +633,69 @@ Contains
         Real*8, Intent(In) :: bounds(1:)
         Integer, Intent(In) :: ndomains, npoly(1:)
         Integer, Intent(In), Optional :: nthread, dmax
-        Integer, Intent(In), Optional :: dealias_by(1:)
+        Integer, Intent(In), Optional, Target :: dealias_by(1:)
         Logical, Intent(In), Optional :: verbose
-        Integer :: n
+        Integer :: n, i, j, off, db
+        Integer(C_INT), Allocatable, Target :: np_c(:), db_c(:)
+        Real(C_DOUBLE), Allocatable :: bd_c(:)
+        Type(C_PTR) :: dbp
         Real*8, Allocatable, Target :: x(:), th(:), ds(:), g(:), w(:)
-        If (ndomains /= 1) Then
-            Write(6, *) 'RaySHT: Cheby_Grid%Init supports one Chebyshev domain, got ', ndomains
+        Logical :: custom
+        If (ndomains < 1) Then
+            Write(6, *) 'RaySHT: Cheby_Grid%Init needs at least one domain, got ', ndomains
             Error Stop 1
         Endif
-        n = npoly(1)
-        If (present(dealias_by)) Then
-            If (size(dealias_by) >= 1) Then
-                If (dealias_by(1) >= 1 .and. dealias_by(1) < n) Then
-                    Write(6, *) 'RaySHT: Cheby_Grid%Init: custom dealias_by is not supported'
-                    Error Stop 1
-                Endif
-            Endif
-        Endif
+        n = sum(npoly(1:ndomains))
+        custom = .false.
+        If (present(dealias_by)) custom = size(dealias_by) >= ndomains
         If (present(verbose)) Then
-            If (verbose) Write(6, *) 'Ndomains: ', ndomains, ' npoly: ', n, ' bounds: ', bounds(1:2)
+            If (verbose) Write(6, *) 'Ndomains: ', ndomains, ' npoly: ', npoly(1:ndomains), &
+                                     ' bounds: ', bounds(1:ndomains + 1)
         Endif
         Call self%finalize()
+        Allocate(np_c(ndomains), db_c(ndomains), bd_c(ndomains + 1))
+        np_c(:) = int(npoly(1:ndomains), C_INT)
+        bd_c(:) = bounds(1:ndomains + 1)
+        db_c(:) = 0
+        dbp = C_NULL_PTR
+        If (custom) Then
+            db_c(:) = int(dealias_by(1:ndomains), C_INT)
+            dbp = c_loc(db_c)
+        Endif
         Allocate(x(n), th(n), ds(n), g(n), w(n))
-        Call check(c_cheb_create(self%handle, int(n, C_INT), bounds(1), bounds(2), g), 'Cheby_Grid%Init')
+        Call check(c_cheb_create_domains(self%handle, int(ndomains, C_INT), np_c, bd_c, dbp, g), 'Cheby_Grid%Init')
         Call check(c_cheb_grid_data(self%handle, c_loc(x), c_loc(th), c_loc(w), c_loc(ds)), 'Cheby_Grid%Init')
         grid(:) = 0.0d0
         integration_weights(:) = 0.0d0
         grid(1:n) = g(:)
         integration_weights(1:n) = w(:)
-        self%domain_count = 1
-        self%max_npoly = n
+        self%domain_count = ndomains
+        self%max_npoly = maxval(npoly(1:ndomains))
         self%ntotal = n
-        Allocate(self%npoly(1), self%n_x(1), self%rda(1), self%domain_bounds(2), self%n_even(1), self%n_odd(1))
-        self%npoly(1) = n
-        self%rda(1) = (2*n)/3 + 1
-        self%domain_bounds(1) = bounds(2)
-        self%domain_bounds(2) = bounds(1)
-        self%n_odd(1) = n/2
-        self%n_even(1) = n/2 + mod(n, 2)
-        self%n_x(1) = n/2 + mod(n, 2)
-        Allocate(self%x(n, 1), self%theta(n, 1), self%deriv_scaling(n))
-        self%x(:, 1) = x(:)
-        self%theta(:, 1) = th(:)
+        Allocate(self%npoly(ndomains), self%n_x(ndomains), self%rda(ndomains), self%domain_bounds(ndomains + 1), &
+                 self%n_even(ndomains), self%n_odd(ndomains))
+        Allocate(self%x(self%max_npoly, ndomains), self%theta(self%max_npoly, ndomains), self%deriv_scaling(n))
+        self%x(:, :) = 0.0d0
+        self%theta(:, :) = 0.0d0
+        off = 0
+        Do i = 1, ndomains
+            j = ndomains - i + 1
+            self%npoly(i) = npoly(j)
+            self%rda(i) = (2*npoly(j))/3 + 1
+            If (custom) Then
+                db = dealias_by(j)
+                If (db >= 1 .and. db < npoly(j)) self%rda(i) = npoly(j) - db + 1
+            Endif
+            self%n_odd(i) = npoly(j)/2
+            self%n_even(i) = npoly(j)/2 + mod(npoly(j), 2)
+            self%n_x(i) = self%n_even(i)
+            self%x(1:npoly(j), i) = x(off + 1:off + npoly(j))
+            self%theta(1:npoly(j), i) = th(off + 1:off + npoly(j))
+            off = off + npoly(j)
+        Enddo
+        Do i = 1, ndomains + 1
+            self%domain_bounds(i) = bounds(ndomains - i + 2)
+        Enddo
         self%deriv_scaling(:) = ds(:)
     End Subroutine Initialize_Cheby_Grid
 
@@ -684,8 +716,8 @@ Contains
             Write(6, *) 'RaySHT: ', what, ' before Cheby_Grid%Init'
             Error Stop 1
         Endif
-        If (n1 /= self%npoly(1)) Then
-            Write(6, *) 'RaySHT: ', what, ': radial size ', n1, ' does not match the grid (', self%npoly(1), ')'
+        If (n1 /= self%ntotal) Then
+            Write(6, *) 'RaySHT: ', what, ': radial size ', n1, ' does not match the grid (', self%ntotal, ')'
             Error Stop 1
         Endif
     End Subroutine require_grid
@@ -711,12 +743,17 @@ Contains
     End Subroutine From_Spectral4D
 
     ! dealias_buffer4d (Chebyshev_Polynomials.F90:353-374): coefficients rda..npoly of every
-    ! column set to zero, in the caller's (host) array
+    ! domain of every column set to zero, in the caller's (host) array
     Subroutine dealias_buffer4d(self, buffer)
         Class(Cheby_Grid) :: self
         Real*8, Intent(InOut) :: buffer(1:,1:,1:,1:)
+        Integer :: h, off
         Call require_grid(self, size(buffer, 1), 'dealias_buffer')
-        buffer(self%rda(1):self%npoly(1), :, :, :) = 0.0d0
+        off = 0
+        Do h = 1, self%domain_count
+            buffer(off + self%rda(h):off + self%npoly(h), :, :, :) = 0.0d0
+            off = off + self%npoly(h)
+        Enddo
     End Subroutine dealias_buffer4d
 
     ! Cheby_Deriv_Buffer_4D (d_by_dr_cp, :560-640): buffer(:,:,:,dind) receives the dorder-th

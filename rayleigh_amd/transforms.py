@@ -544,7 +544,8 @@ def lu_solve_band(mats, pivots, rhs, rhs_off, nrhs, n: int, kl: int, max_nrhs: i
 
 
 class ChebyGrid:
-    """Cheby_Grid with one domain of n zeros-grid points on [r_min, r_max]."""
+    """Cheby_Grid with one domain of n zeros-grid points on [r_min, r_max], or (ChebyGrid.domains)
+    several domains; n is then the total of every domain's points."""
 
     def __init__(self, n: int, r_min: float, r_max: float):
         self.n = int(n)
@@ -553,6 +554,36 @@ class ChebyGrid:
         _lib.check(_lib.lib().raysht_cheb_create(ctypes.byref(h), self.n, float(r_min), float(r_max),
                                                  self.grid.ctypes.data_as(ctypes.c_void_p)), "raysht_cheb_create")
         self._h = h
+
+    @classmethod
+    def domains(cls, npoly, bounds, dealias_by=None):
+        """Cheby_Grid%init with len(npoly) domains (Chebyshev_Polynomials.F90:65-207): npoly and
+        bounds (len(npoly) + 1) in ascending order of radius, as the reference takes them;
+        dealias_by optional, same order.  The grid holds the domains outermost first."""
+        self = cls.__new__(cls)
+        npl = np.ascontiguousarray(npoly, dtype=np.int32)
+        bd = np.ascontiguousarray(bounds, dtype=np.float64)
+        if bd.size != npl.size + 1:
+            raise ValueError("bounds has one more value than npoly")
+        db = None if dealias_by is None else np.ascontiguousarray(dealias_by, dtype=np.int32)
+        if db is not None and db.size != npl.size:
+            raise ValueError("dealias_by has one value per domain")
+        self.n = int(npl.sum())
+        self.grid = np.zeros(self.n)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().raysht_cheb_create_domains(
+            ctypes.byref(h), int(npl.size), npl.ctypes.data_as(ctypes.c_void_p), bd.ctypes.data_as(ctypes.c_void_p),
+            None if db is None else db.ctypes.data_as(ctypes.c_void_p), self.grid.ctypes.data_as(ctypes.c_void_p)),
+            "raysht_cheb_create_domains")
+        self._h = h
+        return self
+
+    def grid_data(self):
+        """(x, theta, weights, deriv_scaling), n values each"""
+        out = [np.zeros(self.n) for _ in range(4)]
+        _lib.check(_lib.lib().raysht_cheb_grid_data(self._h, *[o.ctypes.data_as(ctypes.c_void_p) for o in out]),
+                   "raysht_cheb_grid_data")
+        return tuple(out)
 
     def __del__(self):
         h = getattr(self, "_h", None)
