@@ -27,7 +27,7 @@ __device__ __forceinline__ double4_t mfma_f64(double a, double b, double4_t c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-constexpr int kChebMaxN = 512;
+constexpr int kChebMaxN = 1024;
 constexpr int kChebKB = 4;  // 16-row output blocks per accumulator group
 
 // Both contractions are small dense products per column (K <= n/2) over a huge
@@ -1112,7 +1112,16 @@ int raysht_cheb_table(const raysht_cheb *cb, int which, double *host_out)
 
 namespace {
 // 16 columns per wave; fewer waves per workgroup for long columns to bound the LDS tile
+// (n > 256: one wave, 2 x 16 x (n / 2 + 1) doubles, 131 KB at n = 1024)
 int cheb_waves(int n) { return n <= 128 ? 4 : (n <= 256 ? 2 : 1); }
+
+// the LDS one workgroup of the grid's device may allocate
+bool lds_fits(const raysht_cheb *cb, size_t bytes)
+{
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, cb->device) != hipSuccess) return false;
+    return bytes <= (size_t)v;
+}
 }  // namespace
 
 int raysht_cheb_to_spectral(raysht_cheb *cb, const double *f, double *c, long ncols, void *stream)
@@ -1127,6 +1136,7 @@ int raysht_cheb_to_spectral(raysht_cheb *cb, const double *f, double *c, long nc
     const long blocks = (ncols + tc - 1) / tc;
     if (blocks > 0x7fffffffL) return RAYSHT_EINVAL;
     const size_t lds = sizeof(double) * 2 * tc * (cb->ix4 + 1);
+    if (!lds_fits(cb, lds)) return RAYSHT_EUNSUPPORTED;
     hipLaunchKernelGGL(cheb_to_spectral_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds,
                        reinterpret_cast<hipStream_t>(stream), cb->n, cb->n_x, cb->n_even, cb->n_odd, cb->ix4, cb->kp,
                        ncols, cb->d_tt_e, cb->d_tt_o, f, c);
@@ -1146,6 +1156,7 @@ int raysht_cheb_from_spectral(raysht_cheb *cb, const double *c, double *f, long 
     const long blocks = (ncols + tc - 1) / tc;
     if (blocks > 0x7fffffffL) return RAYSHT_EINVAL;
     const size_t lds = sizeof(double) * 2 * tc * (cb->k4 + 1);
+    if (!lds_fits(cb, lds)) return RAYSHT_EUNSUPPORTED;
     hipLaunchKernelGGL(cheb_from_spectral_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds,
                        reinterpret_cast<hipStream_t>(stream), cb->n, cb->n_x, cb->n_even, cb->n_odd, cb->k4, cb->ip,
                        ncols, cb->d_t_e, cb->d_t_o, c, f);

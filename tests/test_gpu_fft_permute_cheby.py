@@ -149,6 +149,31 @@ def test_cheby_vs_oracle(ra, n):
     assert np.array_equal(host(d), O.cheby_dealias(n, want))
 
 
+@pytest.mark.parametrize("n", [300, 512, 777, 1024])
+def test_cheby_long_columns_vs_oracle(ra, n):
+    """n > 256: the LDS-staged table form (one wave per 16 columns, up to 131 KB of LDS at
+    n = 1024) against the oracle; the sums' rounding grows with their n / 2 terms."""
+    grid = ra.ChebyGrid(n, 0.35, 1.0)
+    assert np.array_equal(grid.grid, O.cheby_grid(n, 0.35, 1.0))
+    rng = np.random.default_rng(n)
+    ncols = 300
+    f = rng.standard_normal(n * ncols)
+    c = empty(f.size)
+    grid.to_spectral(dev(f), c)
+    want = O.cheby_to_spectral(n, f)
+    tol = 1e-14 * n / 256
+    assert rel_max(host(c), want) < tol
+    f2 = empty(f.size)
+    grid.from_spectral(dev(want), f2)
+    assert rel_max(host(f2), O.cheby_from_spectral(n, want)) < tol
+    d = dev(want)
+    grid.dealias_buffer(d)
+    assert np.array_equal(host(d), O.cheby_dealias(n, want))
+    d2 = empty(f.size)
+    grid.d_by_dr(dev(want), d2, dorder=2)
+    assert np.array_equal(host(d2), O.cheby_deriv(n, 0.35, 1.0, want, 2))
+
+
 DCT_TOL = 3e-14  # against the table form's oracle (the table form's own rounding is ~1e-14)
 
 
