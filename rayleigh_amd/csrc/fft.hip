@@ -498,10 +498,12 @@ constexpr int fft3_threads()
 }
 
 // c2r: z_k = A_k + i B_k (A = X_k + conj X_{N-k}, B = (X_k - conj X_{N-k}) e^{2 pi i k / 2N}),
-// inverse length-N FFT, x_{2n} + i x_{2n+1} = z_n.
+// inverse length-N FFT, x_{2n} + i x_{2n+1} = z_n.  Modes above mmax are taken as zeros
+// without being read (the same doubles as zeros read from memory): after the row
+// transpose of a plan they are the m > l_max its permute need not write.
 template <int N>
 __global__ __launch_bounds__(fft3_threads<N>()) void fft3_c2r_kernel(long howmany, double *__restrict__ x,
-                                                                      const double2 *__restrict__ tw)
+                                                                      const double2 *__restrict__ tw, int mmax)
 {
     constexpr int T = N / kP, LPB = fft3_threads<N>() / T, A = fft3_n8(N), F = fft3_last(N);
     constexpr int R0 = A > 0 ? 8 : F;
@@ -518,8 +520,9 @@ __global__ __launch_bounds__(fft3_threads<N>()) void fft3_c2r_kernel(long howman
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
             const int k = t + T * u + r * (N / R0);
-            const double2 xa = reinterpret_cast<const double2 *>(xl)[k];
-            const double2 xb = reinterpret_cast<const double2 *>(xl)[N - k];
+            const double2 z = make_double2(0.0, 0.0);
+            const double2 xa = k <= mmax ? reinterpret_cast<const double2 *>(xl)[k] : z;
+            const double2 xb = N - k <= mmax ? reinterpret_cast<const double2 *>(xl)[N - k] : z;
             const cplx a = {xa.x, k == 0 ? 0.0 : xa.y};
             const cplx bc = {xb.x, k == 0 ? 0.0 : -xb.y};  // conj X_{N-k}; Im X_N does not enter
             const cplx Ak = cadd(a, bc);
@@ -707,8 +710,9 @@ constexpr int kPT = 64, kPQ = 32;
 
 // 2a -> 3a: p3 line (f, theta, r) gets Re/Im of m at 2m, 2m+1 and zeros elsewhere.
 // Tiles of Q modes (q) x 64 theta: a physical row is written as Q * 8 contiguous bytes.
+// qlim: modes q >= qlim are not written (nphi + 2 writes the whole line, its zeros included)
 template <int Q>
-__global__ __launch_bounds__(256) void permute_2a3a_kernel(int nt, int nr, int nf, int nphi, int lmax,
+__global__ __launch_bounds__(256) void permute_2a3a_kernel(int nt, int nr, int nf, int nphi, int lmax, int qlim,
                                                            const int *__restrict__ mp_of_m,
                                                            const double *__restrict__ p2, double *__restrict__ p3)
 {
@@ -740,7 +744,7 @@ __global__ __launch_bounds__(256) void permute_2a3a_kernel(int nt, int nr, int n
 #pragma unroll
     for (int i = 0; i < kPT / TR; ++i) {
         const int tt = tid / QP + TR * i, t = t0 + tt, q = q0 + 2 * qp;
-        if (t < nt && q < ld) {
+        if (t < nt && q < qlim) {
             const double2 v = any_m ? make_double2(tile[2 * qp][tt], tile[2 * qp + 1][tt]) : make_double2(0.0, 0.0);
             *reinterpret_cast<double2 *>(p3 + (((long)f * nt + t) * nr + r) * ld + q) = v;
         }
@@ -819,14 +823,14 @@ int get_twiddles(int nphi, const double2 **out, bool create = true)
 // the register-resident kernels for n2 = 3 * 2^k, 24 <= n2 <= 3072; returns false for
 // other lengths (handled by the generic LDS Stockham kernels)
 template <int N>
-int launch_fft3(bool c2r, long howmany, double *x, const double2 *tw, int rescale, hipStream_t st)
+int launch_fft3(bool c2r, long howmany, double *x, const double2 *tw, int rescale, hipStream_t st, int mmax)
 {
     static_assert(fft3_ok(N), "length without a register-resident schedule");
     constexpr int threads = fft3_threads<N>(), lpb = threads / (N / kP);
     const long blocks = (howmany + lpb - 1) / lpb;
     if (blocks > 0x7fffffffL) return RAYSHT_EUNSUPPORTED;
     if (c2r)
-        hipLaunchKernelGGL(fft3_c2r_kernel<N>, dim3((unsigned)blocks), dim3(threads), 0, st, howmany, x, tw);
+        hipLaunchKernelGGL(fft3_c2r_kernel<N>, dim3((unsigned)blocks), dim3(threads), 0, st, howmany, x, tw, mmax);
     else
         hipLaunchKernelGGL(fft3_r2c_kernel<N>, dim3((unsigned)blocks), dim3(threads), 0, st, howmany, x, tw,
                            rescale);
@@ -834,18 +838,29 @@ int launch_fft3(bool c2r, long howmany, double *x, const double2 *tw, int rescal
     return RAYSHT_OK;
 }
 
+// mmax: the c2r reads modes 0 .. mmax of each line and takes the rest as zeros
 bool fft3_dispatch(int n2, bool c2r, long howmany, double *x, const double2 *tw, int rescale, hipStream_t st,
-                   int *rc)
+                   int *rc, int mmax = 1 << 30)
 {
     switch (n2) {
-    case 24: *rc = launch_fft3<24>(c2r, howmany, x, tw, rescale, st); return true;
-    case 48: *rc = launch_fft3<48>(c2r, howmany, x, tw, rescale, st); return true;
-    case 96: *rc = launch_fft3<96>(c2r, howmany, x, tw, rescale, st); return true;
-    case 192: *rc = launch_fft3<192>(c2r, howmany, x, tw, rescale, st); return true;
-    case 384: *rc = launch_fft3<384>(c2r, howmany, x, tw, rescale, st); return true;
-    case 768: *rc = launch_fft3<768>(c2r, howmany, x, tw, rescale, st); return true;
-    case 1536: *rc = launch_fft3<1536>(c2r, howmany, x, tw, rescale, st); return true;
-    case 3072: *rc = launch_fft3<3072>(c2r, howmany, x, tw, rescale, st); return true;
+    case 24: *rc = launch_fft3<24>(c2r, howmany, x, tw, rescale, st, mmax); return true;
+    case 48: *rc = launch_fft3<48>(c2r, howmany, x, tw, rescale, st, mmax); return true;
+    case 96: *rc = launch_fft3<96>(c2r, howmany, x, tw, rescale, st, mmax); return true;
+    case 192: *rc = launch_fft3<192>(c2r, howmany, x, tw, rescale, st, mmax); return true;
+    case 384: *rc = launch_fft3<384>(c2r, howmany, x, tw, rescale, st, mmax); return true;
+    case 768: *rc = launch_fft3<768>(c2r, howmany, x, tw, rescale, st, mmax); return true;
+    case 1536: *rc = launch_fft3<1536>(c2r, howmany, x, tw, rescale, st, mmax); return true;
+    case 3072: *rc = launch_fft3<3072>(c2r, howmany, x, tw, rescale, st, mmax); return true;
+    default: return false;
+    }
+}
+
+// whether the c2r of nphi runs the register-resident kernel, which can take the modes
+// above a plan's l_max as zeros (the generic kernel reads every mode)
+bool c2r_band_limited(int nphi)
+{
+    switch (nphi / 2) {
+    case 24: case 48: case 96: case 192: case 384: case 768: case 1536: case 3072: return true;
     default: return false;
     }
 }
@@ -937,11 +952,12 @@ int check_fft_args(int nphi, long howmany, const double *x)
     return RAYSHT_OK;
 }
 
-// the c2r of `howmany` lines with the twiddle table tw of nphi (arguments checked)
-int launch_c2r(int nphi, long howmany, double *x, const double2 *tw, hipStream_t st)
+// the c2r of `howmany` lines with the twiddle table tw of nphi (arguments checked); with
+// mmax, modes above it are zeros the register-resident kernel need not read
+int launch_c2r(int nphi, long howmany, double *x, const double2 *tw, hipStream_t st, int mmax = 1 << 30)
 {
     int rc = RAYSHT_OK;
-    if (fft3_dispatch(nphi / 2, true, howmany, x, tw, 0, st, &rc)) return rc;
+    if (fft3_dispatch(nphi / 2, true, howmany, x, tw, 0, st, &rc, mmax)) return rc;
     int code = 0, nrad = 0;
     fft_schedule(nphi / 2, &code, &nrad);
     const int lines = fft_lines_per_block(nphi / 2);
@@ -954,6 +970,20 @@ int launch_c2r(int nphi, long howmany, double *x, const double2 *tw, hipStream_t
 }
 
 }  // namespace
+
+// 2a -> 3a then c2r for a plan (arguments checked): where the c2r is the register-resident
+// kernel, the permute leaves the modes above l_max unwritten and the c2r takes them as
+// zeros (every line is then written whole by the c2r, with the same doubles)
+int permute_c2r(raysht_plan *p, int nphi, const double *p2, double *p3, const double2 *tw, hipStream_t st)
+{
+    const bool band = c2r_band_limited(nphi);
+    const int qlim = band ? 2 * (p->lmax + 1) : nphi + 2;
+    dim3 grid((p->nt + kPT - 1) / kPT, (qlim + kPQ2a3a - 1) / kPQ2a3a, p->nr * p->nf);
+    hipLaunchKernelGGL(permute_2a3a_kernel<kPQ2a3a>, grid, dim3(256), 0, st, p->nt, p->nr, p->nf, nphi, p->lmax, qlim,
+                       p->d_mp_of_m, p2, p3);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return launch_c2r(nphi, (long)p->nr * p->nt * p->nf, p3, tw, st, band ? p->lmax : 1 << 30);
+}
 
 int fft_prepare(int nphi)
 {
@@ -1033,8 +1063,7 @@ int raysht_spectral_to_physical_scratch(raysht_plan *p, int nphi, const double *
     if ((rc = scratch_address(&work)) != RAYSHT_OK) return rc;
     const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     if ((rc = raysht_legendre_stp(p, s, work, stream)) != RAYSHT_OK) return rc;
-    if ((rc = raysht_permute_2a3a(p, nphi, work, p3, stream)) != RAYSHT_OK) return rc;
-    return launch_c2r(nphi, (long)p->nr * p->nt * p->nf, p3, tw, st);
+    return permute_c2r(p, nphi, work, p3, tw, st);
 }
 
 int raysht_permute_2a3a(raysht_plan *p, int nphi, const double *p2, double *p3, void *stream)
@@ -1043,7 +1072,7 @@ int raysht_permute_2a3a(raysht_plan *p, int nphi, const double *p2, double *p3, 
     if ((long)p->nr * p->nf > 65535) return RAYSHT_EUNSUPPORTED;
     dim3 grid((p->nt + kPT - 1) / kPT, (nphi + 2 + kPQ2a3a - 1) / kPQ2a3a, p->nr * p->nf);
     hipLaunchKernelGGL(permute_2a3a_kernel<kPQ2a3a>, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), p->nt, p->nr,
-                       p->nf, nphi, p->lmax, p->d_mp_of_m, p2, p3);
+                       p->nf, nphi, p->lmax, nphi + 2, p->d_mp_of_m, p2, p3);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
@@ -1065,8 +1094,9 @@ int raysht_reform_to_physical(raysht_plan *p, int nphi, const double *p2, double
     int rc = check_fft_args(nphi, 1, p3);
     if (rc != RAYSHT_OK) return rc;
     if (p->nr < 1 || nphi < 2 * (p->lmax + 1) || (long)p->nr * p->nf > 65535) return RAYSHT_EINVAL;
-    if ((rc = raysht_permute_2a3a(p, nphi, p2, p3, stream)) != RAYSHT_OK) return rc;
-    return raysht_fft_to_physical(nphi, (long)p->nr * p->nt * p->nf, p3, stream);
+    const double2 *tw = nullptr;
+    if ((rc = get_twiddles(nphi, &tw)) != RAYSHT_OK) return rc;
+    return permute_c2r(p, nphi, p2, p3, tw, reinterpret_cast<hipStream_t>(stream));
 }
 
 int raysht_reform_to_spectral(raysht_plan *p, int nphi, double *p3, double *p2, void *stream)
