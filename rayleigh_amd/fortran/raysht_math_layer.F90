@@ -47,6 +47,8 @@ Module RaySHT_Math_Layer
     Public :: Device_Allocate, Device_Free, Device_Upload, Device_Download, Device_Synchronize
     Public :: d_by_dtheta, d_by_sdtheta, d_by_dphi
     Public :: LU_Decompose_full, LU_Solve_full, LU_Decompose_band, LU_Solve_Band
+    Public :: Rlm_Spacea_Hydro_Device, Compute_BandCurlB_Device, Adjust_Emf_Device, Rlm_Spaceb_Hydro_Device
+    Public :: LM_Scale_Device, Zero_Lmax_Device
     Public :: coloc, gl_weights
     ! arithmetic of the Legendre contractions (enum raysht_precision of include/raysht.h)
     Integer, Parameter, Public :: raysht_fp64 = 0, raysht_bf16x3 = 1, raysht_bf16x6 = 2, &
@@ -252,6 +254,46 @@ Module RaySHT_Math_Layer
             Import :: C_PTR, C_INT
             Type(C_PTR), Value :: plan
             Integer(C_INT) :: n
+        End Function
+        Function c_spacea_hydro(plan, buf, nf, fields, orr, orsq, drho, tor, dlnrho, dsdr, stream) &
+                Bind(C, name='raysht_rlm_spacea_hydro') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, buf, orr, orsq, drho, tor, dlnrho, dsdr, stream
+            Integer(C_INT), Value :: nf
+            Integer(C_INT), Intent(In) :: fields(10)
+            Integer(C_INT) :: st
+        End Function
+        Function c_bandcurlb(plan, buf, nf, fields, o2, stream) Bind(C, name='raysht_compute_bandcurlb') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, buf, o2, stream
+            Integer(C_INT), Value :: nf
+            Integer(C_INT), Intent(In) :: fields(6)
+            Integer(C_INT) :: st
+        End Function
+        Function c_adjust_emf(plan, buf, nf, et, ep, stream) Bind(C, name='raysht_adjust_emf') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, buf, stream
+            Integer(C_INT), Value :: nf, et, ep
+            Integer(C_INT) :: st
+        End Function
+        Function c_spaceb_hydro(plan, buf, nf, wv, pv, zv, stream) Bind(C, name='raysht_rlm_spaceb_hydro') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, buf, stream
+            Integer(C_INT), Value :: nf, wv, pv, zv
+            Integer(C_INT) :: st
+        End Function
+        Function c_lm_scale(plan, a, nf_in, fin, b, nf_out, fout, lfac, rfac, stream) &
+                Bind(C, name='raysht_lm_scale') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, a, b, lfac, rfac, stream
+            Integer(C_INT), Value :: nf_in, fin, nf_out, fout
+            Integer(C_INT) :: st
+        End Function
+        Function c_zero_lmax(plan, buf, nf, stream) Bind(C, name='raysht_zero_lmax') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, buf, stream
+            Integer(C_INT), Value :: nf
+            Integer(C_INT) :: st
         End Function
         Function c_stp_dev(plan, s, p2, stream) Bind(C, name='raysht_legendre_stp') Result(st)
             Import :: C_PTR, C_INT
@@ -811,6 +853,75 @@ Contains
         st = C_NULL_PTR
         If (present(stream)) st = stream
     End Function opt_stream
+
+    ! ------------------------------------------------------------------ spectral-space passes
+    ! The passes of Sphere_Hybrid_Space.F90 on a device buffer of nf fields in the packed
+    ! spectral layout of Initialize_Legendre's m values and l_max (data(m:l_max, 1:nr, 1:2,
+    ! 1:nf) per m); field arguments are Rayleigh's 1-based field indices (wvar, dwdr, ...),
+    ! radial profiles device arrays of nr values.
+    ! Velocity_Components + Velocity_Derivatives + d_by_dtheta(tvar, dtdt) (:80,235-355):
+    ! fields = (wvar, dwdr, d2wdr2, zvar, dzdr, tvar, vtheta, vphi, dvtdr, dvpdr); dsdr_over_cp
+    ! given when pseudo-incompressible
+    Subroutine Rlm_Spacea_Hydro_Device(nr, nf, buf, fields, over_rhor, over_rhorsq, drho_term, two_over_r, dlnrho, &
+                                       dsdr_over_cp, stream)
+        Integer, Intent(In) :: nr, nf, fields(10)
+        Type(C_PTR), Intent(In) :: buf, over_rhor, over_rhorsq, drho_term, two_over_r, dlnrho
+        Type(C_PTR), Intent(In), Optional :: dsdr_over_cp, stream
+        Integer(C_INT) :: f0(10)
+        Type(C_PTR) :: ds
+        f0 = int(fields - 1, C_INT)
+        ds = C_NULL_PTR
+        If (present(dsdr_over_cp)) ds = dsdr_over_cp
+        Call check(c_spacea_hydro(plan_for(nr, nf), buf, int(nf, C_INT), f0, over_rhor, over_rhorsq, drho_term, &
+                                  two_over_r, dlnrho, ds, opt_stream(stream)), 'Rlm_Spacea_Hydro_Device')
+    End Subroutine Rlm_Spacea_Hydro_Device
+
+    ! Compute_BandCurlB (:377-460): fields = (cvar, avar, dcdr, dadr, d2cdr2, curlbr)
+    Subroutine Compute_BandCurlB_Device(nr, nf, buf, fields, one_over_rsq, stream)
+        Integer, Intent(In) :: nr, nf, fields(6)
+        Type(C_PTR), Intent(In) :: buf, one_over_rsq
+        Type(C_PTR), Intent(In), Optional :: stream
+        Integer(C_INT) :: f0(6)
+        f0 = int(fields - 1, C_INT)
+        Call check(c_bandcurlb(plan_for(nr, nf), buf, int(nf, C_INT), f0, one_over_rsq, opt_stream(stream)), &
+                   'Compute_BandCurlB_Device')
+    End Subroutine Compute_BandCurlB_Device
+
+    ! Adjust_Emf (:557-602), before rlm_spaceb's final l_max zeroing
+    Subroutine Adjust_Emf_Device(nr, nf, buf, emftheta, emfphi, stream)
+        Integer, Intent(In) :: nr, nf, emftheta, emfphi
+        Type(C_PTR), Intent(In) :: buf
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_adjust_emf(plan_for(nr, nf), buf, int(nf, C_INT), int(emftheta - 1, C_INT), &
+                                int(emfphi - 1, C_INT), opt_stream(stream)), 'Adjust_Emf_Device')
+    End Subroutine Adjust_Emf_Device
+
+    ! rlm_spaceb's hydro part (:127-214), its l = 0 and l_max zeroing included
+    Subroutine Rlm_Spaceb_Hydro_Device(nr, nf, buf, wvar, pvar, zvar, stream)
+        Integer, Intent(In) :: nr, nf, wvar, pvar, zvar
+        Type(C_PTR), Intent(In) :: buf
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_spaceb_hydro(plan_for(nr, nf), buf, int(nf, C_INT), int(wvar - 1, C_INT), int(pvar - 1, C_INT), &
+                                  int(zvar - 1, C_INT), opt_stream(stream)), 'Rlm_Spaceb_Hydro_Device')
+    End Subroutine Rlm_Spaceb_Hydro_Device
+
+    ! out(l, r, imi, fout) = (lfac(l) in(l, r, imi, fin)) rfac(r); lfac (l_max + 1) or rfac
+    ! (nr) may be C_NULL_PTR
+    Subroutine LM_Scale_Device(nr, nf_in, a, fin, nf_out, b, fout, lfac, rfac, stream)
+        Integer, Intent(In) :: nr, nf_in, fin, nf_out, fout
+        Type(C_PTR), Intent(In) :: a, b, lfac, rfac
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_lm_scale(plan_for(nr, nf_in), a, int(nf_in, C_INT), int(fin - 1, C_INT), b, int(nf_out, C_INT), &
+                              int(fout - 1, C_INT), lfac, rfac, opt_stream(stream)), 'LM_Scale_Device')
+    End Subroutine LM_Scale_Device
+
+    ! "Zero out l_max mode" of rlm_spacea / rlm_spaceb on every field
+    Subroutine Zero_Lmax_Device(nr, nf, buf, stream)
+        Integer, Intent(In) :: nr, nf
+        Type(C_PTR), Intent(In) :: buf
+        Type(C_PTR), Intent(In), Optional :: stream
+        Call check(c_zero_lmax(plan_for(nr, nf), buf, int(nf, C_INT), opt_stream(stream)), 'Zero_Lmax_Device')
+    End Subroutine Zero_Lmax_Device
 
     ! ------------------------------------------------------------------ reform (one rank)
     ! Transpose_2a3a (Spherical_Buffer.F90:196-459): p2a(1:ntheta, 1:2 nr nf, 1:n_m) ->

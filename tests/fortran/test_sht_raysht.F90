@@ -37,6 +37,7 @@ Program test_sht_raysht
     Call angular_derivatives(nfail)
     Call implicit_solves(nfail)
     Call cheby_grid_object(nfail)
+    Call spectral_passes(nfail)
     ! the split forms with 6 digits stay within the same tolerance over ten round trips
     Call Set_Legendre_Precision(raysht_int8x6)
     Call white_noise(nfail)
@@ -628,5 +629,100 @@ Contains
         Call Cheby_Finalize()
         Call report('Chebyshev derivatives', d, nfail)
     End Subroutine cheby_derivatives
+
+    ! Adjust_Emf_Device (Sphere_Hybrid_Space.F90:557-602) on a device buffer against the
+    ! routine's own statements over the module's host d_by_sdtheta / d_by_dphi (the same
+    ! doubles), then Zero_Lmax_Device
+    Subroutine spectral_passes(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: nf = 3, et = 1, ep = 3
+        Type(rmcontainer4d), Allocatable :: s(:)
+        Type(rmcontainer3d), Allocatable :: t1(:), t2(:), t3(:)
+        Real*8, Allocatable, Target :: packed(:), back(:)
+        Type(C_PTR) :: dbuf
+        Integer :: mp, m, l, r, imi, f, n, off, total
+        Integer*8 :: nb
+        Real*8 :: d, ol(0:lmax), ell
+        Call alloc_spec(s, nf)
+        Allocate(t1(lmax+1), t2(lmax+1), t3(lmax+1))
+        total = 0
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Allocate(t1(mp)%data(m:lmax, 1:nr, 1:2), t2(mp)%data(m:lmax, 1:nr, 1:2), t3(mp)%data(m:lmax, 1:nr, 1:2))
+            Do f = 1, nf
+                Do imi = 1, 2
+                    Do r = 1, nr
+                        Do l = m, lmax
+                            s(mp)%data(l, r, imi, f) = sin(0.37d0*l + 1.1d0*r + 0.5d0*imi + 2.3d0*f + 0.013d0*m)
+                        Enddo
+                    Enddo
+                Enddo
+            Enddo
+            total = total + size(s(mp)%data)
+        Enddo
+        ol(0) = 1.0d0
+        Do l = 1, lmax
+            ell = l*1.0d0
+            ol(l) = 1.0d0/(ell*(ell+1))
+        Enddo
+        Allocate(packed(total), back(total))
+        Call pack_spec(s, packed)
+        nb = int(total, 8) * 8
+        dbuf = Device_Allocate(nb)
+        Call Device_Upload(dbuf, c_loc(packed), nb)
+        Call Adjust_Emf_Device(nr, nf, dbuf, et, ep)
+        Call Device_Download(c_loc(back), dbuf, nb)
+        ! the routine's statements on the host arrays
+        Call d_by_sdtheta(s, ep, t1)
+        Call d_by_dphi(s, et, t2)
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            t3(mp)%data = s(mp)%data(:, :, :, et)
+            Do imi = 1, 2
+                Do r = 1, nr
+                    s(mp)%data(m:lmax, r, imi, et) = (t1(mp)%data(m:lmax, r, imi) - t2(mp)%data(m:lmax, r, imi))*ol(m:lmax)
+                Enddo
+            Enddo
+        Enddo
+        Call d_by_dphi(s, ep, t2)
+        Do mp = 1, lmax+1
+            s(mp)%data(:, :, :, ep) = t3(mp)%data
+        Enddo
+        Call d_by_sdtheta(s, ep, t1)
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Do imi = 1, 2
+                Do r = 1, nr
+                    s(mp)%data(m:lmax, r, imi, ep) = (t2(mp)%data(m:lmax, r, imi) + t1(mp)%data(m:lmax, r, imi))*ol(m:lmax)
+                Enddo
+            Enddo
+        Enddo
+        Call pack_spec(s, packed)
+        d = maxval(abs(packed - back))
+        Call Zero_Lmax_Device(nr, nf, dbuf)
+        Call Device_Download(c_loc(back), dbuf, nb)
+        off = 0
+        Do mp = 1, lmax+1
+            n = lmax - mvals(mp) + 1
+            Do f = 1, 2 * nr * nf
+                d = max(d, abs(back(off + f * n)))
+            Enddo
+            off = off + n * 2 * nr * nf
+        Enddo
+        Call Device_Free(dbuf)
+        Call report('Adjust_Emf_Device / Zero_Lmax_Device', d, nfail)
+    End Subroutine spectral_passes
+
+    Subroutine pack_spec(s, packed)
+        Type(rmcontainer4d), Intent(In) :: s(:)
+        Real*8, Intent(Out) :: packed(:)
+        Integer :: mp, n, off
+        off = 0
+        Do mp = 1, size(s)
+            n = size(s(mp)%data)
+            packed(off + 1:off + n) = Reshape(s(mp)%data, (/ n /))
+            off = off + n
+        Enddo
+    End Subroutine pack_spec
 
 End Program test_sht_raysht

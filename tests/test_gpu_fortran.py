@@ -1,7 +1,8 @@
 """The Fortran ISO_C_BINDING layer (rayleigh_amd/fortran) driven by a Test_SHT-style
 Fortran program: Legendre_Transform (4-D and 2-D), FFT_To_Physical/Spectral and the
 Chebyshev pair, radial and angular derivatives and the dense and band implicit solves through the
-host-array entry points, on the GPU."""
+host-array entry points, and Adjust_Emf_Device / Zero_Lmax_Device on a device buffer against the
+reference's statements over the host derivatives, on the GPU."""
 import os
 import subprocess
 
@@ -26,4 +27,4 @@ def test_fortran_binding_test_sht():
     # device buffers, the radial reforms, the Real*16 grid, the 2-D pair, Chebyshev, angular
     # derivatives, implicit solves, the Cheby_Grid object), then the white-noise round trips
     # with INT8X6 and BF16X6
-    assert len(lines) == 16 and all("PASSING" in ln for ln in lines), r.stdout
+    assert len(lines) == 17 and all("PASSING" in ln for ln in lines), r.stdout
