@@ -80,11 +80,11 @@ def test_lu_singular_info(ra):
     assert info.cpu().numpy().tolist() == [inf, 0] and inf == 8
 
 
-def test_lu_nan_diagonal_keeps_row(ra):
+@pytest.mark.parametrize("n", [24, 100])  # the unblocked kernel, and the blocked one (n > 32)
+def test_lu_nan_diagonal_keeps_row(ra, n):
     """idamax never ranks an entry above a NaN a_kk, so dgetrf keeps row k as the pivot
     there (and passes over NaNs below the diagonal); the factors then match the oracle."""
     import torch
-    n = 24
     rng = np.random.default_rng(5)
     a = rng.standard_normal((n, n)) + 4 * np.eye(n)
     a[3, 3] = np.nan        # NaN pivot candidate with finite entries below it
