@@ -316,6 +316,14 @@ def cpu_reference_extras(L, mv, nr, nf, nlm, n_lu, ncols, out):
             res[key] = {"ms_full_size": t * (nr / nrs) * 1e3, "cores": 1, "kind": "reference",
                         "sample": f"{nrs} of {nr} radii of a 16-field buffer, scaled (the reference's "
                                   "d_by_dtheta / d_by_dphi and the routines' array statements)"}
+        co = rng.standard_normal(O.spectral_size(L, mv, nrs, 9))
+        rr = [rng.uniform(0.5, 2.0, nrs) for _ in range(3)]
+        t0 = time.perf_counter()
+        R.hybrid_output(buf, 16, co, 9, [11, 12, 13, 14, 2, 6], [3, 10, 11, 12, 0, 3, 5, 6, 1], *rr)
+        t = time.perf_counter() - t0
+        res["hybrid_output"] = {"ms_full_size": t * (nr / nrs) * 1e3, "cores": 1, "kind": "reference",
+                                "sample": f"{nrs} of {nr} radii, magnetic: Hybrid_Output_Initial's magnetic "
+                                          "part and Hybrid_Output_Final up to its transform, scaled"}
         R.angular_init(L, mv, nr)
     # the phi FFTs: numpy's pocketfft (FFTW, the reference's library, is not available) on
     # a sample of lines, one core, scaled to every (r, theta, field) line
@@ -563,7 +571,31 @@ def stack_bench(ra, steps: int, warmup: int, workload: str = "stack"):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / steps
         out["rlm_spacea"][name] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9, "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
-    del b16
+    # the output iteration's passes (Hybrid_Output_Initial / _Final up to their transforms) from
+    # the 16-field buffer into a 9-field co-buffer and 2 work fields: the initial pass reads 4
+    # fields and writes 4 (64 B per coefficient of a field); the final pass, magnetic, reads 4
+    # of s, the 2 work fields and dbrdr_cb and writes 5 (96 B), hydro only 1 and 1 (16 B)
+    co = torch.randn(9 * per_field, dtype=torch.float64, device="cuda")
+    tmp = torch.randn(2 * per_field, dtype=torch.float64, device="cuda")
+    o_r, o_rsq, t_r = prof["over_rhor"], prof["over_rhorsq"], prof["two_over_r"]
+    vi, vf = [11, 12, 13, 14, 2, 6], [3, 10, 11, 12, 0, 3, 5, 6, 1]
+    out["output_iteration"] = {}
+    for name, fn, nb in (("initial", lambda: plan.hybrid_output_initial(b16, co, tmp, vi, o_rsq), 64.0 * per_field),
+                         ("final", lambda: plan.hybrid_output_final(b16, co, vf, tmp, o_r, o_rsq, t_r),
+                          96.0 * per_field),
+                         ("final_hydro", lambda: plan.hybrid_output_final(b16, co, vf), 16.0 * per_field)):
+        for _ in range(max(warmup, 1)):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        out["output_iteration"][name] = {"ms": ms, "gbps": nb / (ms * 1e-3) / 1e9,
+                                         "frac_hbm_peak": nb / (ms * 1e-3) / 8e12}
+    del b16, co, tmp
     # one field scaled by l(l+1) and a radial profile (rlm_spacea's l_l_plus1 * f * Over_RhoRSQ):
     # 16 B per coefficient
     lfac = torch.from_numpy(ra.l_l_plus1(L)).cuda()

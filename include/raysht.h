@@ -204,6 +204,31 @@ int raysht_compute_bandcurlb(raysht_plan *plan, double *buf, int nf, const int *
  * It reads and writes only those two fields, so calling it before raysht_rlm_spaceb_hydro (whose
  * last step zeroes l_max) gives rlm_spaceb's magnetic order.  Results are the reference's doubles. */
 int raysht_adjust_emf(raysht_plan *plan, double *buf, int nf, int emftheta, int emfphi, void *stream);
+/* Hybrid_Output_Initial's magnetic part (Physics/Sphere_Hybrid_Space.F90:525-541) in one pass
+ * from `s` (nf fields) into the output co-buffer `co` (nf_co fields) and `tmp` (2 fields, the
+ * reference's ftemp3 and ftemp4): co[avar_cb] = (l(l+1) s[avar]) one_over_rsq, co[dbrdr_cb] =
+ * s[dcdr], tmp[0] = s[d2cdr2], tmp[1] = s[dadr].  fields (host array of 6 0-based fields) =
+ * avar, dcdr, d2cdr2, dadr (distinct, of s), avar_cb, dbrdr_cb (distinct, of co); s, co and tmp
+ * must not overlap.  one_over_rsq = OneOverRSquared (device, N_r values).  Results are the
+ * reference's doubles. */
+int raysht_hybrid_output_initial(raysht_plan *plan, const double *s, int nf, double *co, int nf_co, double *tmp,
+                                 const int *fields, const double *one_over_rsq, void *stream);
+/* Hybrid_Output_Final (Physics/Sphere_Hybrid_Space.F90:543-552) up to its Legendre transform,
+ * in one pass: every co field's l_max zeroed, then Hydro_Output_Derivatives (:224-233)
+ * co[dpdt_cb] = d_by_dtheta s[pvar] and, when `tmp` (raysht_hybrid_output_initial's work fields)
+ * is not NULL, BField_Derivatives (:437-498):
+ *   co[dbtdr_cb] = (d_by_dtheta tmp[0] + d_by_dphi tmp[1]) one_over_r - s[btheta] one_over_rsq,
+ *   co[dbpdr_cb] = (d_by_dphi tmp[0] - d_by_dtheta tmp[1]) one_over_r - s[bphi] one_over_rsq,
+ *   co[dbrdr_cb] = (l(l+1) co[dbrdr_cb]) one_over_rsq - s[br] two_over_r,
+ *   co[dbrdt_cb] = d_by_dtheta s[br].
+ * fields (host array of 9 0-based fields) = pvar, br, btheta, bphi (of s), dpdt_cb, dbtdr_cb,
+ * dbpdr_cb, dbrdr_cb, dbrdt_cb (of co); with tmp NULL only pvar and dpdt_cb are read and the
+ * profiles may be NULL, otherwise each group is distinct and the profiles (device, N_r values:
+ * One_Over_R, OneOverRSquared, Two_Over_R) are required.  s must not overlap co or tmp.
+ * Results are the doubles of the reference's sequence. */
+int raysht_hybrid_output_final(raysht_plan *plan, const double *s, int nf, double *co, int nf_co, const double *tmp,
+                               const int *fields, const double *one_over_r, const double *one_over_rsq,
+                               const double *two_over_r, void *stream);
 /* The l(l+1) and radial scalings of rlm_spacea / rlm_spaceb (Velocity_Components,
  * Velocity_Derivatives, Compute_BandCurlB: Sphere_Hybrid_Space.F90:244,324,353,367,373) on one
  * field: out(l, r, imi) = (lfac(l) in(l, r, imi)) rfac(r), rounded in that order; lfac

@@ -75,6 +75,8 @@ def oracle_lib():
         lib.orc_rlm_spacea_hydro.argtypes = [_I, _P, _I, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P]
         lib.orc_compute_bandcurlb.argtypes = [_I, _P, _I, _I, _P, _I, _P, _P]
         lib.orc_adjust_emf.argtypes = [_I, _P, _I, _I, _P, _I, _I, _I]
+        lib.orc_hybrid_output_initial.argtypes = [_I, _P, _I, _I, _P, _I, _P, _I, _P, _P, _P]
+        lib.orc_hybrid_output_final.argtypes = [_I, _P, _I, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P]
         lib.orc_lu_decompose.restype = _I
         lib.orc_lu_decompose.argtypes = [_I, _P, _P]
         lib.orc_lu_solve.argtypes = [_I, _P, _P, _I, _P]
@@ -130,6 +132,7 @@ def ref_lib():
         lib.ref_rlm_spacea_hydro.argtypes = [_I, _P, _I, _P, _P, _P, _P, _P, _P, _P]
         lib.ref_compute_bandcurlb.argtypes = [_I, _P, _P, _P]
         lib.ref_adjust_emf.argtypes = [_I, _I, _I, _P]
+        lib.ref_hybrid_output.argtypes = [_I, _I, _P, _P, _I, _P, _P, _P, _P, _P]
         _ref = lib
     return _ref
 
@@ -346,6 +349,26 @@ def adjust_emf(lmax, mvals, nr, buf, nf, et, ep):
     b = np.ascontiguousarray(buf, dtype=np.float64).copy()
     oracle_lib().orc_adjust_emf(lmax, _ptr(mv), len(mv), nr, _ptr(b), nf, et, ep)
     return b
+
+
+def hybrid_output(lmax, mvals, nr, s, nf, co, nf_co, vi, vf, one_over_r, one_over_rsq, two_over_r, magnetism=True):
+    """Sphere_Hybrid_Space.F90:224-233,437-552: Hybrid_Output_Initial's magnetic part (into the
+    co-buffer and the two work fields) then Hybrid_Output_Final up to its transform, on copies;
+    vi = 0-based (avar, dcdr, d2cdr2, dadr, avar_cb, dbrdr_cb), vf = 0-based (pvar, br, btheta,
+    bphi, dpdt_cb, dbtdr_cb, dbpdr_cb, dbrdr_cb, dbrdt_cb).  Returns the new co-buffer."""
+    mv = np.ascontiguousarray(mvals, dtype=np.int32)
+    sb = np.ascontiguousarray(s, dtype=np.float64)
+    c = np.ascontiguousarray(co, dtype=np.float64).copy()
+    a_i = np.ascontiguousarray(vi, dtype=np.int32)
+    a_f = np.ascontiguousarray(vf, dtype=np.int32)
+    rr = [np.ascontiguousarray(x, dtype=np.float64) for x in (one_over_r, one_over_rsq, two_over_r)]
+    tmp = np.zeros(spectral_size(lmax, mv, nr, 2))
+    if magnetism:
+        oracle_lib().orc_hybrid_output_initial(lmax, _ptr(mv), len(mv), nr, _ptr(sb), nf, _ptr(c), nf_co, _ptr(tmp),
+                                               _ptr(a_i), _ptr(rr[1]))
+    oracle_lib().orc_hybrid_output_final(lmax, _ptr(mv), len(mv), nr, _ptr(sb), nf, _ptr(c), nf_co,
+                                         _ptr(tmp) if magnetism else None, _ptr(a_f), *[_ptr(x) for x in rr])
+    return c
 
 
 def lm_scale(lmax, mvals, nr, a, nf_in, fin, lfac=None, rfac=None, out=None, nf_out=1, fout=0):
@@ -753,6 +776,17 @@ class Reference:
         b = np.ascontiguousarray(buf, dtype=np.float64).copy()
         self.lib.ref_adjust_emf(nf, et + 1, ep + 1, _ptr(b))
         return b
+
+    def hybrid_output(self, s, nf, co, nf_co, vi, vf, one_over_r, one_over_rsq, two_over_r, magnetism=True):
+        """the reference's output-iteration steps (fields 0-based, as hybrid_output above)"""
+        sb = np.ascontiguousarray(s, dtype=np.float64).copy()
+        c = np.ascontiguousarray(co, dtype=np.float64).copy()
+        a_i = np.ascontiguousarray(np.asarray(vi) + 1, dtype=np.int32)
+        a_f = np.ascontiguousarray(np.asarray(vf) + 1, dtype=np.int32)
+        rr = [np.ascontiguousarray(x, dtype=np.float64) for x in (one_over_r, one_over_rsq, two_over_r)]
+        self.lib.ref_hybrid_output(nf, nf_co, _ptr(a_i), _ptr(a_f), int(magnetism), *[_ptr(x) for x in rr], _ptr(sb),
+                                   _ptr(c))
+        return c
 
     def cheby_deriv(self, c: np.ndarray, dorder: int, nobuffer: bool = False) -> np.ndarray:
         c = np.ascontiguousarray(c, dtype=np.float64)

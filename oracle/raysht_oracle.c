@@ -1020,6 +1020,21 @@ static void st_sub_scaled(const slab *g, double *buf, int nf, int f, int v, cons
     }
 }
 
+/* field f of buffer a = itself - field u of buffer b (nb fields) times rfac(r) */
+static void st_sub_scaled_from(const slab *g, double *a, int na, int f, double *b, int nb, int u, const double *rfac)
+{
+    for (int mp = 0; mp < g->nm; ++mp) {
+        const int m = g->mvals[mp], nl = g->lmax - m + 1;
+        double *o = fld(g, a, na, f, mp), *x = fld(g, b, nb, u, mp);
+        for (int imi = 0; imi < 2; ++imi)
+            for (int r = 0; r < g->nr; ++r)
+                for (int i = 0; i < nl; ++i) {
+                    const long e = ((long)imi * g->nr + r) * nl + i;
+                    o[e] = o[e] - x[e] * rfac[r];
+                }
+    }
+}
+
 /* one-field slab t times rfac(r) into field f   (SBUFFA(IDX2,f) = ftemp1(mp)%data(IDX2)*rfac(r)) */
 static void st_store_scaled(const slab *g, double *buf, int nf, int f, const double *t, const double *rfac)
 {
@@ -1187,6 +1202,71 @@ void orc_adjust_emf(int lmax, const int *mvals, int nm, int nr, double *buf, int
     free(t1);
     free(t2);
     free(t3);
+}
+
+/* Sphere_Hybrid_Space.F90:525-541 Hybrid_Output_Initial, magnetic part: from the spectral buffer
+ * s (nf fields) into the output co-buffer co (nf_co fields) and the two-field work buffer tmp
+ * (the reference's ftemp3, ftemp4): co[acb] = l_l_plus1 s[avar] one_over_rsq, co[dbrcb] = s[dcdr],
+ * tmp[0] = s[d2cdr2], tmp[1] = s[dadr].  v = 0-based (avar, dcdr, d2cdr2, dadr, avar_cb,
+ * dbrdr_cb). */
+void orc_hybrid_output_initial(int lmax, const int *mvals, int nm, int nr, const double *s, int nf, double *co,
+                               int nf_co, double *tmp, const int *v, const double *one_over_rsq)
+{
+    const slab g = {lmax, nm, nr, mvals};
+    for (int mp = 0; mp < nm; ++mp) {
+        const int m = mvals[mp], nl = lmax - m + 1;
+        double *S = (double *)s;
+        const double *A = fld(&g, S, nf, v[0], mp), *DC = fld(&g, S, nf, v[1], mp);
+        const double *D2C = fld(&g, S, nf, v[2], mp), *DA = fld(&g, S, nf, v[3], mp);
+        double *ACB = fld(&g, co, nf_co, v[4], mp), *DBR = fld(&g, co, nf_co, v[5], mp);
+        double *T3 = fld(&g, tmp, 2, 0, mp), *T4 = fld(&g, tmp, 2, 1, mp);
+        for (int imi = 0; imi < 2; ++imi)
+            for (int r = 0; r < nr; ++r)
+                for (int l = m; l <= lmax; ++l) {
+                    const long e = ((long)imi * nr + r) * nl + l - m;
+                    ACB[e] = l_l_plus1(l) * A[e] * one_over_rsq[r];
+                    DBR[e] = DC[e];
+                    T3[e] = D2C[e];
+                    T4[e] = DA[e];
+                }
+    }
+}
+
+/* Sphere_Hybrid_Space.F90:543-552 Hybrid_Output_Final up to its Legendre transform: co's l_max
+ * zeroed; Hydro_Output_Derivatives (:224-233) co[dpdt_cb] = d_by_dtheta s[pvar]; with tmp (the
+ * magnetic case) BField_Derivatives (:437-498):
+ *   t1 = d_by_dtheta tmp[0] + d_by_dphi tmp[1]; co[dbtdr_cb] = t1 one_over_r - s[btheta] one_over_rsq;
+ *   t1 = d_by_dphi tmp[0] - d_by_dtheta tmp[1]; co[dbpdr_cb] = t1 one_over_r - s[bphi] one_over_rsq;
+ *   co[dbrdr_cb] = l_l_plus1 co[dbrdr_cb] one_over_rsq - s[br] two_over_r;
+ *   co[dbrdt_cb] = d_by_dtheta s[br].
+ * v = 0-based (pvar, br, btheta, bphi, dpdt_cb, dbtdr_cb, dbpdr_cb, dbrdr_cb, dbrdt_cb). */
+void orc_hybrid_output_final(int lmax, const int *mvals, int nm, int nr, const double *s, int nf, double *co,
+                             int nf_co, const double *tmp, const int *v, const double *one_over_r,
+                             const double *one_over_rsq, const double *two_over_r)
+{
+    const slab g = {lmax, nm, nr, mvals};
+    const long fsz = field_block(lmax, mvals, nm, nr, 1);
+    double *S = (double *)s, *Tm = (double *)tmp;
+    orc_zero_lmax(lmax, mvals, nm, nr, co, nf_co);
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, S, nf, v[0], co, nf_co, v[4]);
+    if (!tmp) return;
+    double *t1 = (double *)malloc(sizeof(double) * (fsz + 1));
+    double *t2 = (double *)malloc(sizeof(double) * (fsz + 1));
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, Tm, 2, 0, t1, 1, 0);
+    orc_angular_dphi(lmax, mvals, nm, nr, Tm, 2, 1, t2, 1, 0);
+    st_combine(fsz, t1, t2, +1);
+    st_store_scaled(&g, co, nf_co, v[5], t1, one_over_r);
+    st_sub_scaled_from(&g, co, nf_co, v[5], S, nf, v[2], one_over_rsq);
+    orc_angular_dphi(lmax, mvals, nm, nr, Tm, 2, 0, t1, 1, 0);
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, Tm, 2, 1, t2, 1, 0);
+    st_combine(fsz, t1, t2, -1);
+    st_store_scaled(&g, co, nf_co, v[6], t1, one_over_r);
+    st_sub_scaled_from(&g, co, nf_co, v[6], S, nf, v[3], one_over_rsq);
+    st_lscale(&g, co, nf_co, v[7], v[7], one_over_rsq);
+    st_sub_scaled_from(&g, co, nf_co, v[7], S, nf, v[1], two_over_r);
+    orc_angular_dtheta(0, lmax, mvals, nm, nr, S, nf, v[1], co, nf_co, v[8]);
+    free(t1);
+    free(t2);
 }
 
 /* LU_Decompose_band (Linear_Solve.F90:1041-1055): LAPACK dgbtrf on an n x n band matrix in

@@ -73,6 +73,11 @@ void orc_rlm_spacea_hydro(int lmax, const int *mvals, int nm, int nr, double *bu
                           const double *over_rhor, const double *over_rhorsq, const double *drho_term,
                           const double *two_over_r, const double *dlnrho, const double *dsdr_over_cp);
 void orc_adjust_emf(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, int et, int ep);
+void orc_hybrid_output_initial(int lmax, const int *mvals, int nm, int nr, const double *s, int nf, double *co,
+                               int nf_co, double *tmp, const int *v, const double *one_over_rsq);
+void orc_hybrid_output_final(int lmax, const int *mvals, int nm, int nr, const double *s, int nf, double *co,
+                             int nf_co, const double *tmp, const int *v, const double *one_over_r,
+                             const double *one_over_rsq, const double *two_over_r);
 void orc_compute_bandcurlb(int lmax, const int *mvals, int nm, int nr, double *buf, int nf, const int *v,
                            const double *one_over_rsq);
 int orc_lu_decompose(int n, double *a, int *piv);

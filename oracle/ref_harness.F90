@@ -668,4 +668,96 @@ Contains
         DeAllocate(a, ftemp1, ftemp2, ftemp3, ol)
     End Subroutine ref_adjust_emf
 
+    ! Hybrid_Output_Initial's magnetic part and Hybrid_Output_Final up to its transform
+    ! (Physics/Sphere_Hybrid_Space.F90:224-233,437-552) through the reference's d_by_dtheta /
+    ! d_by_dphi (their 3-D and 4-D forms) and the same array statements: s (nf fields), the
+    ! co-buffer co (nf_co fields) and the work fields ftemp3, ftemp4 (tmp, two fields) packed;
+    ! vi = 1-based (avar, dcdr, d2cdr2, dadr, avar_cb, dbrdr_cb); vf = 1-based (pvar, br,
+    ! btheta, bphi, dpdt_cb, dbtdr_cb, dbpdr_cb, dbrdr_cb, dbrdt_cb); mag = 0: the hydro part only.
+    Subroutine ref_hybrid_output(nf, nf_co, vi, vf, mag, one_over_r, one_over_rsq, two_over_r, s, co) &
+            bind(C, name='ref_hybrid_output')
+        Integer(C_INT), Value :: nf, nf_co, mag
+        Integer(C_INT), Intent(In) :: vi(6), vf(9)
+        Real(C_DOUBLE), Intent(In) :: one_over_r(*), one_over_rsq(*), two_over_r(*)
+        Real(C_DOUBLE), Intent(InOut) :: s(*), co(*)
+        Type(rmcontainer4d), Allocatable :: a(:), c(:)
+        Type(rmcontainer3d), Allocatable :: ftemp1(:), ftemp2(:), ftemp3(:), ftemp4(:)
+        Real*8, Allocatable :: l_l_plus1(:)
+        Real*8 :: ell
+        Integer :: mp, m, l, r, imi, lm
+        lm = h_ang_lmax
+        Allocate(l_l_plus1(0:lm))
+        Do l = 0, lm
+            ell = l*1.0d0
+            l_l_plus1(l) = ell*(ell+1)
+        Enddo
+        Call unpack4d(s, nf, a)
+        Call unpack4d(co, nf_co, c)
+        Call alloc3d(ftemp1)
+        Call alloc3d(ftemp2)
+        Call alloc3d(ftemp3)
+        Call alloc3d(ftemp4)
+        ! Hybrid_Output_Initial
+        If (mag /= 0) Then
+            Do mp = 1, h_ang_nm
+                m = h_ang_mvals(mp)
+                Do imi = 1, 2
+                    Do r = 1, h_ang_nr
+                        c(mp)%data(m:lm,r,imi,vi(5)) = l_l_plus1(m:lm)*a(mp)%data(m:lm,r,imi,vi(1))*one_over_rsq(r)
+                    Enddo
+                Enddo
+                c(mp)%data(:,:,:,vi(6)) = a(mp)%data(:,:,:,vi(2))
+                ftemp3(mp)%data = a(mp)%data(:,:,:,vi(3))
+                ftemp4(mp)%data = a(mp)%data(:,:,:,vi(4))
+            Enddo
+        Endif
+        ! Hybrid_Output_Final
+        Do mp = 1, h_ang_nm
+            c(mp)%data(lm,:,:,:) = 0.0d0
+        Enddo
+        Call d_by_dtheta(a, vf(1), ftemp1)
+        Do mp = 1, h_ang_nm
+            c(mp)%data(:,:,:,vf(5)) = ftemp1(mp)%data
+        Enddo
+        If (mag /= 0) Then
+            Call d_by_dtheta(ftemp3, ftemp1)
+            Call d_by_dphi(ftemp4, ftemp2)
+            Do mp = 1, h_ang_nm
+                ftemp1(mp)%data = ftemp1(mp)%data + ftemp2(mp)%data
+                Do r = 1, h_ang_nr
+                    c(mp)%data(:,r,:,vf(6)) = ftemp1(mp)%data(:,r,:)*one_over_r(r)
+                Enddo
+                Do r = 1, h_ang_nr
+                    c(mp)%data(:,r,:,vf(6)) = c(mp)%data(:,r,:,vf(6)) - a(mp)%data(:,r,:,vf(3))*one_over_rsq(r)
+                Enddo
+            Enddo
+            Call d_by_dphi(ftemp3, ftemp1)
+            Call d_by_dtheta(ftemp4, ftemp2)
+            Do mp = 1, h_ang_nm
+                m = h_ang_mvals(mp)
+                ftemp1(mp)%data = ftemp1(mp)%data - ftemp2(mp)%data
+                Do r = 1, h_ang_nr
+                    c(mp)%data(:,r,:,vf(7)) = ftemp1(mp)%data(:,r,:)*one_over_r(r)
+                Enddo
+                Do r = 1, h_ang_nr
+                    c(mp)%data(:,r,:,vf(7)) = c(mp)%data(:,r,:,vf(7)) - a(mp)%data(:,r,:,vf(4))*one_over_rsq(r)
+                Enddo
+                Do imi = 1, 2
+                    Do r = 1, h_ang_nr
+                        c(mp)%data(m:lm,r,imi,vf(8)) = l_l_plus1(m:lm)*c(mp)%data(m:lm,r,imi,vf(8))*one_over_rsq(r)
+                    Enddo
+                Enddo
+                Do r = 1, h_ang_nr
+                    c(mp)%data(:,r,:,vf(8)) = c(mp)%data(:,r,:,vf(8)) - a(mp)%data(:,r,:,vf(2))*two_over_r(r)
+                Enddo
+            Enddo
+            Call d_by_dtheta(a, vf(2), ftemp1)
+            Do mp = 1, h_ang_nm
+                c(mp)%data(:,:,:,vf(9)) = ftemp1(mp)%data
+            Enddo
+        Endif
+        Call pack4d(c, nf_co, co)
+        DeAllocate(a, c, ftemp1, ftemp2, ftemp3, ftemp4, l_l_plus1)
+    End Subroutine ref_hybrid_output
+
 End Module raysht_ref_harness

@@ -87,6 +87,8 @@ SIGNATURES = {
     "raysht_rlm_spacea_hydro": (_I, [_P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "raysht_compute_bandcurlb": (_I, [_P, _P, _I, _P, _P, _P]),
     "raysht_adjust_emf": (_I, [_P, _P, _I, _I, _I, _P]),
+    "raysht_hybrid_output_initial": (_I, [_P, _P, _I, _P, _I, _P, _P, _P, _P]),
+    "raysht_hybrid_output_final": (_I, [_P, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P]),
     "raysht_lm_scale": (_I, [_P, _P, _I, _I, _P, _I, _I, _P, _P, _P]),
     "raysht_zero_lmax": (_I, [_P, _P, _I, _P]),
     "raysht_lu_decompose_full": (_I, [_I, _I, _P, _P, _P, _P]),

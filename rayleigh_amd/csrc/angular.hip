@@ -505,6 +505,134 @@ __global__ __launch_bounds__(kAThreads) void adjust_emf_kernel(int lmax, int nr,
     }
 }
 
+// Hybrid_Output_Initial, magnetic part (Sphere_Hybrid_Space.F90:525-541): from s into the
+// output co-buffer and the two work fields (the reference's ftemp3, ftemp4), element by element:
+// co[avar_cb] = (L s[avar]) one_over_rsq, co[dbrdr_cb] = s[dcdr], tmp0 = s[d2cdr2], tmp1 = s[dadr].
+// One wave per (r, imi) l-column.
+struct output_initial_fields {
+    int avar, dcdr, d2cdr2, dadr, avar_cb, dbrdr_cb;
+};
+__global__ __launch_bounds__(kAThreads) void hybrid_output_initial_kernel(
+    int lmax, int nr, const int *__restrict__ mvals, const int *__restrict__ lm_base, const double *__restrict__ s,
+    int nf, double *__restrict__ co, int nf_co, double *__restrict__ tmp, output_initial_fields f,
+    const double *__restrict__ one_over_rsq)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * (kAThreads / 64) + (threadIdx.x >> 6);  // column r + nr imi
+    if (k >= 2 * nr) return;
+    const int mp = blockIdx.y, m = mvals[mp], nl = lmax - m + 1;
+    const long lb = lm_base[mp], fsz = 2L * nl * nr, col = (long)k * nl;
+    const double *sb = s + lb * 2L * nr * nf + col;
+    double *cb = co + lb * 2L * nr * nf_co + col, *t = tmp + lb * 2L * nr * 2 + col;
+    const double o2 = one_over_rsq[k >= nr ? k - nr : k];
+    for (int i = lane; i < nl; i += 64) {
+        cb[f.avar_cb * fsz + i] = l_l_plus1(m + i) * sb[f.avar * fsz + i] * o2;
+        cb[f.dbrdr_cb * fsz + i] = sb[f.dcdr * fsz + i];
+        t[i] = sb[f.d2cdr2 * fsz + i];
+        t[fsz + i] = sb[f.dadr * fsz + i];
+    }
+}
+
+// Hybrid_Output_Final up to its transform (:543-552): co's l_max zeroed, Hydro_Output_Derivatives
+// (:224-233) co[dpdt_cb] = D s[pvar], and with the work fields (magnetism) BField_Derivatives
+// (:437-498), with D = d_by_dtheta, P = d_by_dphi and the radial factors of radius r:
+//   co[dbtdr_cb] = (D tmp0 + P tmp1) one_over_r - s[btheta] one_over_rsq;
+//   co[dbpdr_cb] = (P tmp0 - D tmp1) one_over_r - s[bphi] one_over_rsq;
+//   co[dbrdr_cb] = (L co[dbrdr_cb]) one_over_rsq - s[br] two_over_r   (l_max of co[dbrdr_cb] read as 0);
+//   co[dbrdt_cb] = D s[br].
+// Rounded where the reference rounds; one wave per radius of one m-block (both imi), l
+// neighbours as in rlm_spacea_hydro_kernel.
+struct output_final_fields {
+    int pvar, br, btheta, bphi, dpdt_cb, dbtdr_cb, dbpdr_cb, dbrdr_cb, dbrdt_cb;
+};
+__global__ __launch_bounds__(kAThreads) void hybrid_output_final_kernel(
+    int lmax, int nr, const int *__restrict__ mvals, const int *__restrict__ lm_base, const double2 *__restrict__ dcoef,
+    const double *__restrict__ s, int nf, double *__restrict__ co, int nf_co, const double *__restrict__ tmp,
+    output_final_fields f, const double *__restrict__ one_over_r, const double *__restrict__ one_over_rsq,
+    const double *__restrict__ two_over_r)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * (kAThreads / 64) + (threadIdx.x >> 6);
+    if (r >= nr) return;  // whole waves
+    const int mp = blockIdx.y, m = mvals[mp], nl = lmax - m + 1;
+    const long lb = lm_base[mp], half = (long)nl * nr, fsz = 2 * half;
+    const double *sb = s + lb * 2L * nr * nf + (long)r * nl;
+    double *cb = co + lb * 2L * nr * nf_co + (long)r * nl;
+    const double *P = sb + f.pvar * fsz, *Br = sb + f.br * fsz, *Bt = sb + f.btheta * fsz, *Bp = sb + f.bphi * fsz;
+    const bool mag = tmp != nullptr;
+    const double *T3 = mag ? tmp + lb * 2L * nr * 2 + (long)r * nl : nullptr, *T4 = mag ? T3 + fsz : nullptr;
+    const double2 *c = dcoef + lb;
+    const double dm = (double)m, orr = one_over_r[r], o2 = one_over_rsq[r], tor = two_over_r[r];
+    double k_pr = 0.0, k_pi = 0.0, k_br = 0.0, k_bi = 0.0, k_3r = 0.0, k_3i = 0.0, k_4r = 0.0, k_4i = 0.0;
+    for (int base = 0; base < nl; base += 64) {
+        const int i = base + lane, l = m + i;
+        const bool in = i < nl;
+        double pr = 0.0, pi = 0.0, brr = 0.0, bri = 0.0, t3r = 0.0, t3i = 0.0, t4r = 0.0, t4i = 0.0;
+        double2 ci = make_double2(0.0, 0.0);
+        if (in) {
+            pr = P[i];
+            pi = P[half + i];
+            ci = c[i];
+            if (mag) {
+                brr = Br[i];
+                bri = Br[half + i];
+                t3r = T3[i];
+                t3i = T3[half + i];
+                t4r = T4[i];
+                t4i = T4[half + i];
+            }
+        }
+        double n_pr = 0.0, n_pi = 0.0, n_br = 0.0, n_bi = 0.0, n_3r = 0.0, n_3i = 0.0, n_4r = 0.0, n_4i = 0.0;
+        if (lane == 63 && i + 1 < nl) {
+            n_pr = P[i + 1];
+            n_pi = P[half + i + 1];
+            if (mag) {
+                n_br = Br[i + 1];
+                n_bi = Br[half + i + 1];
+                n_3r = T3[i + 1];
+                n_3i = T3[half + i + 1];
+                n_4r = T4[i + 1];
+                n_4i = T4[half + i + 1];
+            }
+        }
+        const lnb b_pr = l_neighbours(pr, n_pr, k_pr, lane), b_pi = l_neighbours(pi, n_pi, k_pi, lane);
+        const lnb b_br = l_neighbours(brr, n_br, k_br, lane), b_bi = l_neighbours(bri, n_bi, k_bi, lane);
+        const lnb b_3r = l_neighbours(t3r, n_3r, k_3r, lane), b_3i = l_neighbours(t3i, n_3i, k_3i, lane);
+        const lnb b_4r = l_neighbours(t4r, n_4r, k_4r, lane), b_4i = l_neighbours(t4i, n_4i, k_4i, lane);
+        if (!in) continue;
+        if (i == nl - 1)  // co's l_max zeroed first; the fields computed below overwrite theirs
+            for (int g = 0; g < nf_co; ++g) {
+                cb[g * fsz + i] = 0.0;
+                cb[g * fsz + half + i] = 0.0;
+            }
+        double *Dp = cb + f.dpdt_cb * fsz;
+        Dp[i] = dth_at(b_pr, ci, i, nl);
+        Dp[half + i] = dth_at(b_pi, ci, i, nl);
+        if (!mag) continue;
+        double *Dt = cb + f.dbtdr_cb * fsz, *Dq = cb + f.dbpdr_cb * fsz, *Dr = cb + f.dbrdr_cb * fsz;
+        double *Dd = cb + f.dbrdt_cb * fsz;
+        const double dr_r = l == lmax ? 0.0 : Dr[i], dr_i = l == lmax ? 0.0 : Dr[half + i];
+        double v;
+        v = (dth_at(b_3r, ci, i, nl) + -(dm * t4i)) * orr;
+        Dt[i] = v - Bt[i] * o2;
+        v = (dth_at(b_3i, ci, i, nl) + dm * t4r) * orr;
+        Dt[half + i] = v - Bt[half + i] * o2;
+        v = (-(dm * t3i) - dth_at(b_4r, ci, i, nl)) * orr;
+        Dq[i] = v - Bp[i] * o2;
+        v = (dm * t3r - dth_at(b_4i, ci, i, nl)) * orr;
+        Dq[half + i] = v - Bp[half + i] * o2;
+        const double L = l_l_plus1(l);
+        v = L * dr_r * o2;
+        Dr[i] = v - brr * tor;
+        v = L * dr_i * o2;
+        Dr[half + i] = v - bri * tor;
+        Dd[i] = dth_at(b_br, ci, i, nl);
+        Dd[half + i] = dth_at(b_bi, ci, i, nl);
+    }
+}
+
 // out(l, r, imi) = (lfac(l) in(l, r, imi)) rfac(r) for one field (either factor may be
 // absent); in place allowed.  The products round in this order, as the reference's
 // l_l_plus1(m:l_max)*SBUFFA(IDX2, f)*Over_RhoRSQ(r) statements round them.  One wave per
@@ -655,6 +783,60 @@ int launch_adjust_emf(raysht_plan *p, double *buf, int nf, int et, int ep, hipSt
     return RAYSHT_OK;
 }
 
+// s and co (and tmp) must not overlap: the passes read s while writing co
+bool disjoint(const void *a, long na, const void *b, long nb)
+{
+    const char *x = static_cast<const char *>(a), *y = static_cast<const char *>(b);
+    return x + na * 8 <= y || y + nb * 8 <= x;
+}
+
+int launch_output_initial(raysht_plan *p, const double *s, int nf, double *co, int nf_co, double *tmp,
+                          const int *fields, const double *one_over_rsq, hipStream_t st)
+{
+    if (!p || !s || !co || !tmp || !fields || !one_over_rsq) return RAYSHT_EINVAL;
+    if (int rc = check_spectral(p)) return rc;
+    const long per_field = p->s_off[p->nm] / p->nf;
+    const int vs[4] = {fields[0], fields[1], fields[2], fields[3]}, vc[2] = {fields[4], fields[5]};
+    if (nf < 4 || nf_co < 2 || !distinct_fields(vs, 4, nf) || !distinct_fields(vc, 2, nf_co)) return RAYSHT_EINVAL;
+    if (!disjoint(s, per_field * nf, co, per_field * nf_co) || !disjoint(s, per_field * nf, tmp, per_field * 2) ||
+        !disjoint(co, per_field * nf_co, tmp, per_field * 2))
+        return RAYSHT_EINVAL;
+    const output_initial_fields f{fields[0], fields[1], fields[2], fields[3], fields[4], fields[5]};
+    const unsigned bx = (unsigned)((2L * p->nr + kAThreads / 64 - 1) / (kAThreads / 64));
+    hipLaunchKernelGGL(hybrid_output_initial_kernel, dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr,
+                       p->d_mvals, p->d_lm_base, s, nf, co, nf_co, tmp, f, one_over_rsq);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
+int launch_output_final(raysht_plan *p, const double *s, int nf, double *co, int nf_co, const double *tmp,
+                        const int *fields, const double *one_over_r, const double *one_over_rsq,
+                        const double *two_over_r, hipStream_t st)
+{
+    if (!p || !s || !co || !fields) return RAYSHT_EINVAL;
+    if (tmp && (!one_over_r || !one_over_rsq || !two_over_r)) return RAYSHT_EINVAL;
+    if (int rc = check_spectral(p)) return rc;
+    const long per_field = p->s_off[p->nm] / p->nf;
+    const int vs[4] = {fields[0], fields[1], fields[2], fields[3]};
+    const int vc[5] = {fields[4], fields[5], fields[6], fields[7], fields[8]};
+    if (nf < 1 || nf_co < 1) return RAYSHT_EINVAL;
+    if (tmp ? (!distinct_fields(vs, 4, nf) || !distinct_fields(vc, 5, nf_co))
+            : (vs[0] < 0 || vs[0] >= nf || vc[0] < 0 || vc[0] >= nf_co))
+        return RAYSHT_EINVAL;
+    if (!disjoint(s, per_field * nf, co, per_field * nf_co) || (tmp && !disjoint(co, per_field * nf_co, tmp, per_field * 2)))
+        return RAYSHT_EINVAL;
+    // the hydro-only form reads and writes one field each: the other indices are unused
+    const output_final_fields f{fields[0], tmp ? fields[1] : fields[0], tmp ? fields[2] : fields[0],
+                                tmp ? fields[3] : fields[0], fields[4], tmp ? fields[5] : fields[4],
+                                tmp ? fields[6] : fields[4], tmp ? fields[7] : fields[4], tmp ? fields[8] : fields[4]};
+    const unsigned bx = (unsigned)((p->nr + kAThreads / 64 - 1) / (kAThreads / 64));
+    hipLaunchKernelGGL(hybrid_output_final_kernel, dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr,
+                       p->d_mvals, p->d_lm_base, p->d_acoef, s, nf, co, nf_co, tmp, f,
+                       tmp ? one_over_r : s, tmp ? one_over_rsq : s, tmp ? two_over_r : s);
+    RAYSHT_HIP_TRY(hipGetLastError());
+    return RAYSHT_OK;
+}
+
 int launch_lm_scale(raysht_plan *p, const double *in, int nf_in, int fin, double *out, int nf_out, int fout,
                     const double *lfac, const double *rfac, hipStream_t st)
 {
@@ -749,6 +931,20 @@ int raysht_rlm_spacea_hydro(raysht_plan *p, double *buf, int nf, const int *fiel
 {
     return launch_spacea_hydro(p, buf, nf, fields, over_rhor, over_rhorsq, drho_term, two_over_r, dlnrho,
                                dsdr_over_cp, reinterpret_cast<hipStream_t>(stream));
+}
+
+int raysht_hybrid_output_initial(raysht_plan *p, const double *s, int nf, double *co, int nf_co, double *tmp,
+                                 const int *fields, const double *one_over_rsq, void *stream)
+{
+    return launch_output_initial(p, s, nf, co, nf_co, tmp, fields, one_over_rsq, reinterpret_cast<hipStream_t>(stream));
+}
+
+int raysht_hybrid_output_final(raysht_plan *p, const double *s, int nf, double *co, int nf_co, const double *tmp,
+                               const int *fields, const double *one_over_r, const double *one_over_rsq,
+                               const double *two_over_r, void *stream)
+{
+    return launch_output_final(p, s, nf, co, nf_co, tmp, fields, one_over_r, one_over_rsq, two_over_r,
+                               reinterpret_cast<hipStream_t>(stream));
 }
 
 int raysht_adjust_emf(raysht_plan *p, double *buf, int nf, int emftheta, int emfphi, void *stream)

@@ -222,6 +222,46 @@ class LegendrePlan:
         _lib.check(_lib.lib().raysht_adjust_emf(self._h, _dptr(buf, "buf"), nf, int(emftheta), int(emfphi),
                                                 _stream(stream)), "raysht_adjust_emf")
 
+    def hybrid_output_initial(self, s, co, tmp, fields, one_over_rsq, stream=None):
+        """Sphere_Hybrid_Space.F90:525-541 Hybrid_Output_Initial, magnetic part: from `s` into
+        the output co-buffer `co` and the two work fields `tmp` (whole fields each; none
+        overlapping).  fields = 0-based (avar, dcdr, d2cdr2, dadr) of s and (avar_cb, dbrdr_cb)
+        of co: co[avar_cb] = l(l+1) s[avar] / r^2, co[dbrdr_cb] = s[dcdr], tmp = s[d2cdr2],
+        s[dadr].  one_over_rsq: float64 device, n_r values."""
+        nf, nf_co, nt = self._whole_fields(s, co, tmp)
+        v = np.ascontiguousarray(fields, dtype=np.int32)
+        if v.size != 6 or nt != 2:
+            raise ValueError("fields = avar, dcdr, d2cdr2, dadr, avar_cb, dbrdr_cb; tmp holds 2 fields")
+        _lib.check(_lib.lib().raysht_hybrid_output_initial(
+            self._h, _dptr(s, "s"), nf, _dptr(co, "co"), nf_co, _dptr(tmp, "tmp"), v.ctypes.data_as(ctypes.c_void_p),
+            _dptr(one_over_rsq, "one_over_rsq", self.n_r), _stream(stream)), "raysht_hybrid_output_initial")
+
+    def hybrid_output_final(self, s, co, fields, tmp=None, one_over_r=None, one_over_rsq=None, two_over_r=None,
+                            stream=None):
+        """Sphere_Hybrid_Space.F90:543-552 Hybrid_Output_Final up to its transform: l_max of
+        every co field zeroed, co[dpdt_cb] = d_by_dtheta s[pvar] and, with `tmp` (the work
+        fields of hybrid_output_initial), BField_Derivatives (:437-498) into dbtdr_cb, dbpdr_cb,
+        dbrdr_cb (which must hold dcdr) and dbrdt_cb.  fields = 0-based (pvar, br, btheta, bphi)
+        of s and (dpdt_cb, dbtdr_cb, dbpdr_cb, dbrdr_cb, dbrdt_cb) of co; the radial profiles
+        (float64 device, n_r values) are required with tmp."""
+        nf, nf_co = self._whole_fields(s, co)
+        v = np.ascontiguousarray(fields, dtype=np.int32)
+        if v.size != 9:
+            raise ValueError("fields = pvar, br, btheta, bphi, dpdt_cb, dbtdr_cb, dbpdr_cb, dbrdr_cb, dbrdt_cb")
+        prof = [None, None, None]
+        t = None
+        if tmp is not None:
+            if self._whole_fields(tmp) != [2]:
+                raise ValueError("tmp holds 2 fields")
+            if one_over_r is None or one_over_rsq is None or two_over_r is None:
+                raise ValueError("one_over_r, one_over_rsq and two_over_r are required with tmp")
+            t = _dptr(tmp, "tmp")
+            prof = [_dptr(x, k, self.n_r) for x, k in ((one_over_r, "one_over_r"), (one_over_rsq, "one_over_rsq"),
+                                                        (two_over_r, "two_over_r"))]
+        _lib.check(_lib.lib().raysht_hybrid_output_final(
+            self._h, _dptr(s, "s"), nf, _dptr(co, "co"), nf_co, t, v.ctypes.data_as(ctypes.c_void_p), *prof,
+            _stream(stream)), "raysht_hybrid_output_final")
+
     def lm_scale(self, a, fin: int, out, fout: int, lfac=None, rfac=None, stream=None):
         """out(l, r, imi) = (lfac(l) a(l, r, imi)) rfac(r) from field fin of `a` into field fout of
         `out` (the l(l+1) / radial scalings of rlm_spacea and rlm_spaceb); lfac (l_max + 1

@@ -127,6 +127,17 @@ def rlm_spacea_case(name, lmax, mvals, nr, nf, hydro, magnetic, emf, pseudo, see
                hydro_out=R.rlm_spacea_hydro(buf, nf, hydro, prof),
                magnetic_out=R.compute_bandcurlb(buf, nf, magnetic, one_over_rsq),
                emf=np.array(emf, np.int32), emf_out=R.adjust_emf(buf, nf, *emf))
+    # the output-iteration steps (Hybrid_Output_Initial / _Final) into a seeded co-buffer of 7
+    # fields, with and without magnetism; vi, vf 0-based as oracle.hybrid_output takes them
+    nf_co = 7
+    co = rng.standard_normal(O.spectral_size(lmax, mv, nr, nf_co))
+    ho_prof = [rng.uniform(0.5, 2.0, nr) for _ in range(3)]
+    vi, vf = (magnetic[1], magnetic[2], magnetic[4], magnetic[3], 2, 6), (hydro[3], magnetic[0], magnetic[1],
+                                                                          magnetic[2], 0, 3, 5, 6, 1)
+    rec.update(ho_co=co, ho_vi=np.array(vi, np.int32), ho_vf=np.array(vf, np.int32), ho_one_over_r=ho_prof[0],
+               ho_one_over_rsq=ho_prof[1], ho_two_over_r=ho_prof[2],
+               ho_out=R.hybrid_output(buf, nf, co, nf_co, vi, vf, *ho_prof, magnetism=True),
+               ho_out_hydro=R.hybrid_output(buf, nf, co, nf_co, vi, vf, *ho_prof, magnetism=False))
     for k in O.HYDRO_PROFILES:
         if prof[k] is not None:
             rec[k] = prof[k]

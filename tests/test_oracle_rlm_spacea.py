@@ -135,3 +135,69 @@ def test_spacea_linear_and_other_fields_untouched():
         assert np.allclose(fn(a + b), ra + rb, rtol=0, atol=1e-14 * scale)
         keep = ~np.isin(field, roles)
         assert np.array_equal(ra[keep], a[keep])
+
+
+def _ho_args(g):
+    return (int(g["lmax"]), g["mvals"], int(g["nr"]), g["buf"], int(g["nf"]), g["ho_co"], g["ho_co"].size //
+            (g["buf"].size // int(g["nf"])), g["ho_vi"], g["ho_vf"], g["ho_one_over_r"], g["ho_one_over_rsq"],
+            g["ho_two_over_r"])
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_hybrid_output_vs_reference_fixture(golden, name):
+    """Hybrid_Output_Initial's magnetic part then Hybrid_Output_Final up to its transform
+    (Sphere_Hybrid_Space.F90:224-233,437-552), with and without magnetism."""
+    g = golden(name)
+    assert np.array_equal(O.hybrid_output(*_ho_args(g), magnetism=True), g["ho_out"])
+    assert np.array_equal(O.hybrid_output(*_ho_args(g), magnetism=False), g["ho_out_hydro"])
+
+
+@pytest.mark.parametrize("which,swap", [("vi", (1, 2)), ("vi", (2, 3)), ("vf", (2, 3)), ("vf", (5, 6))])
+def test_planted_output_role_exchange_fails(golden, which, swap):
+    g = golden(FIXTURES[1])
+    args = list(_ho_args(g))
+    k = 7 if which == "vi" else 8
+    v = list(args[k])
+    v[swap[0]], v[swap[1]] = v[swap[1]], v[swap[0]]
+    args[k] = v
+    assert not np.array_equal(O.hybrid_output(*args, magnetism=True), g["ho_out"])
+
+
+def test_hybrid_output_hydro_is_dtheta_and_lmax_zeroing():
+    """Without magnetism the pass is the l_max zeroing of every co field and d_by_dtheta of
+    pvar into dpdt_cb (both pinned on their own against the reference); with it, the fields
+    outside the roles are only l_max-zeroed, and the pass is linear in (s, co) jointly."""
+    L, mv, nr, nf, nf_co = 19, O.m_order_balanced(19), 3, 9, 8
+    rng = np.random.default_rng(21)
+    s = rng.standard_normal(O.spectral_size(L, mv, nr, nf))
+    co = rng.standard_normal(O.spectral_size(L, mv, nr, nf_co))
+    prof = [rng.uniform(0.5, 2.0, nr) for _ in range(3)]
+    vi, vf = (4, 5, 6, 7, 1, 6), (2, 3, 4, 5, 0, 2, 3, 6, 7)
+    out = O.hybrid_output(L, mv, nr, s, nf, co, nf_co, vi, vf, *prof, magnetism=False)
+    want = O.zero_lmax(L, mv, nr, co, nf_co)
+    O.angular_deriv("dtheta", L, mv, nr, s, nf, vf[0], out=want, nf_out=nf_co, fout=vf[4])
+    assert np.array_equal(out, want)
+    out = O.hybrid_output(L, mv, nr, s, nf, co, nf_co, vi, vf, *prof, magnetism=True)
+    field = _field_of(L, mv, nr, nf_co)
+    keep = ~np.isin(field, [vi[4], *vf[4:]])
+    assert np.array_equal(out[keep], O.zero_lmax(L, mv, nr, co, nf_co)[keep])
+    assert np.array_equal(O.hybrid_output(L, mv, nr, 2.0 * s, nf, 2.0 * co, nf_co, vi, vf, *prof), 2.0 * out)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("L,mv,nr", [(0, [0], 2), (1, [1, 0], 3), (17, [17, 2, 9, 0], 2), (33, "balanced", 2)])
+def test_hybrid_output_vs_compiled_reference(L, mv, nr):
+    mv = O.m_order_balanced(L) if isinstance(mv, str) else np.array(mv, np.int32)
+    nf, nf_co = 10, 9
+    rng = np.random.default_rng(L * 7 + nr)
+    R = O.Reference()
+    R.angular_init(L, mv, nr)
+    s = rng.standard_normal(O.spectral_size(L, mv, nr, nf))
+    co = rng.standard_normal(O.spectral_size(L, mv, nr, nf_co))
+    prof = [rng.uniform(0.5, 2.0, nr) for _ in range(3)]
+    ps, pc = rng.permutation(nf), rng.permutation(nf_co)
+    vi = (ps[0], ps[1], ps[2], ps[3], pc[0], pc[1])
+    vf = (ps[4], ps[5], ps[6], ps[7], pc[2], pc[3], pc[4], pc[1], pc[5])
+    for mag in (True, False):
+        assert np.array_equal(O.hybrid_output(L, mv, nr, s, nf, co, nf_co, vi, vf, *prof, magnetism=mag),
+                              R.hybrid_output(s, nf, co, nf_co, vi, vf, *prof, magnetism=mag)), mag
