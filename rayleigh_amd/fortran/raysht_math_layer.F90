@@ -48,6 +48,7 @@ Module RaySHT_Math_Layer
     Public :: d_by_dtheta, d_by_sdtheta, d_by_dphi
     Public :: LU_Decompose_full, LU_Solve_full, LU_Decompose_band, LU_Solve_Band
     Public :: Rlm_Spacea_Hydro_Device, Compute_BandCurlB_Device, Adjust_Emf_Device, Rlm_Spaceb_Hydro_Device
+    Public :: Hybrid_Output_Initial_Device, Hybrid_Output_Final_Device
     Public :: LM_Scale_Device, Zero_Lmax_Device
     Public :: coloc, gl_weights
     ! arithmetic of the Legendre contractions (enum raysht_precision of include/raysht.h)
@@ -274,6 +275,22 @@ Module RaySHT_Math_Layer
             Import :: C_PTR, C_INT
             Type(C_PTR), Value :: plan, buf, stream
             Integer(C_INT), Value :: nf, et, ep
+            Integer(C_INT) :: st
+        End Function
+        Function c_output_initial(plan, s, nf, co, nf_co, tmp, fields, o2, stream) &
+                Bind(C, name='raysht_hybrid_output_initial') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, s, co, tmp, o2, stream
+            Integer(C_INT), Value :: nf, nf_co
+            Integer(C_INT), Intent(In) :: fields(6)
+            Integer(C_INT) :: st
+        End Function
+        Function c_output_final(plan, s, nf, co, nf_co, tmp, fields, orr, o2, tor, stream) &
+                Bind(C, name='raysht_hybrid_output_final') Result(st)
+            Import :: C_PTR, C_INT
+            Type(C_PTR), Value :: plan, s, co, tmp, orr, o2, tor, stream
+            Integer(C_INT), Value :: nf, nf_co
+            Integer(C_INT), Intent(In) :: fields(9)
             Integer(C_INT) :: st
         End Function
         Function c_spaceb_hydro(plan, buf, nf, wv, pv, zv, stream) Bind(C, name='raysht_rlm_spaceb_hydro') Result(st)
@@ -895,6 +912,43 @@ Contains
         Call check(c_adjust_emf(plan_for(nr, nf), buf, int(nf, C_INT), int(emftheta - 1, C_INT), &
                                 int(emfphi - 1, C_INT), opt_stream(stream)), 'Adjust_Emf_Device')
     End Subroutine Adjust_Emf_Device
+
+    ! Hybrid_Output_Initial's magnetic part (:525-541) from s (nf fields) into the co-buffer (nf_co
+    ! fields) and tmp (2 fields, the routine's ftemp3 / ftemp4, kept for the final pass):
+    ! fields = (avar, dcdr, d2cdr2, dadr) of s, (avar_cb, dbrdr_cb) of co
+    Subroutine Hybrid_Output_Initial_Device(nr, nf, s, nf_co, co, tmp, fields, one_over_rsq, stream)
+        Integer, Intent(In) :: nr, nf, nf_co, fields(6)
+        Type(C_PTR), Intent(In) :: s, co, tmp, one_over_rsq
+        Type(C_PTR), Intent(In), Optional :: stream
+        Integer(C_INT) :: f0(6)
+        f0 = int(fields - 1, C_INT)
+        Call check(c_output_initial(plan_for(nr, nf), s, int(nf, C_INT), co, int(nf_co, C_INT), tmp, f0, &
+                                    one_over_rsq, opt_stream(stream)), 'Hybrid_Output_Initial_Device')
+    End Subroutine Hybrid_Output_Initial_Device
+
+    ! Hybrid_Output_Final (:543-552) up to its transform: l_max of every co field zeroed,
+    ! Hydro_Output_Derivatives (:224-233) and, with tmp (magnetism), BField_Derivatives (:437-498);
+    ! fields = (pvar, br, btheta, bphi) of s, (dpdt_cb, dbtdr_cb, dbpdr_cb, dbrdr_cb, dbrdt_cb) of co;
+    ! the profiles (One_Over_R, OneOverRSquared, Two_Over_R) are required with tmp
+    Subroutine Hybrid_Output_Final_Device(nr, nf, s, nf_co, co, fields, tmp, one_over_r, one_over_rsq, two_over_r, &
+                                          stream)
+        Integer, Intent(In) :: nr, nf, nf_co, fields(9)
+        Type(C_PTR), Intent(In) :: s, co
+        Type(C_PTR), Intent(In), Optional :: tmp, one_over_r, one_over_rsq, two_over_r, stream
+        Integer(C_INT) :: f0(9)
+        Type(C_PTR) :: t, p1, p2, p3
+        f0 = int(fields - 1, C_INT)
+        t = C_NULL_PTR
+        p1 = C_NULL_PTR
+        p2 = C_NULL_PTR
+        p3 = C_NULL_PTR
+        If (present(tmp)) t = tmp
+        If (present(one_over_r)) p1 = one_over_r
+        If (present(one_over_rsq)) p2 = one_over_rsq
+        If (present(two_over_r)) p3 = two_over_r
+        Call check(c_output_final(plan_for(nr, nf), s, int(nf, C_INT), co, int(nf_co, C_INT), t, f0, p1, p2, p3, &
+                                  opt_stream(stream)), 'Hybrid_Output_Final_Device')
+    End Subroutine Hybrid_Output_Final_Device
 
     ! rlm_spaceb's hydro part (:127-214), its l = 0 and l_max zeroing included
     Subroutine Rlm_Spaceb_Hydro_Device(nr, nf, buf, wvar, pvar, zvar, stream)

@@ -38,6 +38,7 @@ Program test_sht_raysht
     Call implicit_solves(nfail)
     Call cheby_grid_object(nfail)
     Call spectral_passes(nfail)
+    Call output_final_hydro(nfail)
     ! the split forms with 6 digits stay within the same tolerance over ten round trips
     Call Set_Legendre_Precision(raysht_int8x6)
     Call white_noise(nfail)
@@ -712,6 +713,57 @@ Contains
         Call Device_Free(dbuf)
         Call report('Adjust_Emf_Device / Zero_Lmax_Device', d, nfail)
     End Subroutine spectral_passes
+
+    ! Hybrid_Output_Final_Device without magnetism against its statements on the host arrays:
+    ! every co field's l_max zeroed, then co(dpdt_cb) = d_by_dtheta s(pvar)
+    Subroutine output_final_hydro(nfail)
+        Integer, Intent(InOut) :: nfail
+        Integer, Parameter :: nf = 3, pvar = 2, dpdt_cb = 3
+        Type(rmcontainer4d), Allocatable :: s(:), co(:)
+        Type(rmcontainer3d), Allocatable :: t1(:)
+        Real*8, Allocatable, Target :: packed(:), back(:)
+        Type(C_PTR) :: ds, dco
+        Integer :: mp, m, l, r, imi, f, total
+        Integer*8 :: nb
+        Call alloc_spec(s, nf)
+        Call alloc_spec(co, nf)
+        Allocate(t1(lmax+1))
+        total = 0
+        Do mp = 1, lmax+1
+            m = mvals(mp)
+            Allocate(t1(mp)%data(m:lmax, 1:nr, 1:2))
+            Do f = 1, nf
+                Do imi = 1, 2
+                    Do r = 1, nr
+                        Do l = m, lmax
+                            s(mp)%data(l, r, imi, f) = sin(0.41d0*l + 0.9d0*r + 0.5d0*imi + 1.7d0*f + 0.019d0*m)
+                            co(mp)%data(l, r, imi, f) = cos(0.23d0*l + 1.3d0*r + 0.7d0*imi + 0.6d0*f + 0.011d0*m)
+                        Enddo
+                    Enddo
+                Enddo
+            Enddo
+            total = total + size(s(mp)%data)
+        Enddo
+        Allocate(packed(total), back(total))
+        nb = int(total, 8) * 8
+        ds = Device_Allocate(nb)
+        dco = Device_Allocate(nb)
+        Call pack_spec(s, packed)
+        Call Device_Upload(ds, c_loc(packed), nb)
+        Call pack_spec(co, packed)
+        Call Device_Upload(dco, c_loc(packed), nb)
+        Call Hybrid_Output_Final_Device(nr, nf, ds, nf, dco, (/pvar, 1, 1, 1, dpdt_cb, 1, 1, 1, 1/))
+        Call Device_Download(c_loc(back), dco, nb)
+        Call d_by_dtheta(s, pvar, t1)
+        Do mp = 1, lmax+1
+            co(mp)%data(lmax, :, :, :) = 0.0d0
+            co(mp)%data(:, :, :, dpdt_cb) = t1(mp)%data
+        Enddo
+        Call pack_spec(co, packed)
+        Call Device_Free(ds)
+        Call Device_Free(dco)
+        Call report('Hybrid_Output_Final_Device (hydro)', maxval(abs(packed - back)), nfail)
+    End Subroutine output_final_hydro
 
     Subroutine pack_spec(s, packed)
         Type(rmcontainer4d), Intent(In) :: s(:)

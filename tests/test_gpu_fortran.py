@@ -27,4 +27,4 @@ def test_fortran_binding_test_sht():
     # device buffers, the radial reforms, the Real*16 grid, the 2-D pair, Chebyshev, angular
     # derivatives, implicit solves, the Cheby_Grid object), then the white-noise round trips
     # with INT8X6 and BF16X6
-    assert len(lines) == 17 and all("PASSING" in ln for ln in lines), r.stdout
+    assert len(lines) == 18 and all("PASSING" in ln for ln in lines), r.stdout
