@@ -10,7 +10,8 @@
 // rounded separately, as the reference's compiled loops round them, so the results are its
 // doubles.  The same file holds the passes built from these derivatives, each one kernel
 // whose waves own whole radii: rlm_spaceb's combination and its Adjust_Emf after PtS,
-// rlm_spacea's velocity steps and Compute_BandCurlB before StP, and the l(l+1) / radial
+// rlm_spacea's velocity steps and Compute_BandCurlB before StP, the output iterations'
+// Hybrid_Output_Initial / _Final into the output co-buffer, and the l(l+1) / radial
 // scalings of the spectral fields.
 #include <algorithm>
 #include <cmath>
@@ -546,6 +547,8 @@ __global__ __launch_bounds__(kAThreads) void hybrid_output_initial_kernel(
 struct output_final_fields {
     int pvar, br, btheta, bphi, dpdt_cb, dbtdr_cb, dbpdr_cb, dbrdr_cb, dbrdt_cb;
 };
+// (MAG: tmp given; the hydro-only instance needs far fewer registers, so more waves stay resident)
+template <bool MAG>
 __global__ __launch_bounds__(kAThreads) void hybrid_output_final_kernel(
     int lmax, int nr, const int *__restrict__ mvals, const int *__restrict__ lm_base, const double2 *__restrict__ dcoef,
     const double *__restrict__ s, int nf, double *__restrict__ co, int nf_co, const double *__restrict__ tmp,
@@ -561,7 +564,7 @@ __global__ __launch_bounds__(kAThreads) void hybrid_output_final_kernel(
     const double *sb = s + lb * 2L * nr * nf + (long)r * nl;
     double *cb = co + lb * 2L * nr * nf_co + (long)r * nl;
     const double *P = sb + f.pvar * fsz, *Br = sb + f.br * fsz, *Bt = sb + f.btheta * fsz, *Bp = sb + f.bphi * fsz;
-    const bool mag = tmp != nullptr;
+    constexpr bool mag = MAG;
     const double *T3 = mag ? tmp + lb * 2L * nr * 2 + (long)r * nl : nullptr, *T4 = mag ? T3 + fsz : nullptr;
     const double2 *c = dcoef + lb;
     const double dm = (double)m, orr = one_over_r[r], o2 = one_over_rsq[r], tor = two_over_r[r];
@@ -830,9 +833,11 @@ int launch_output_final(raysht_plan *p, const double *s, int nf, double *co, int
                                 tmp ? fields[3] : fields[0], fields[4], tmp ? fields[5] : fields[4],
                                 tmp ? fields[6] : fields[4], tmp ? fields[7] : fields[4], tmp ? fields[8] : fields[4]};
     const unsigned bx = (unsigned)((p->nr + kAThreads / 64 - 1) / (kAThreads / 64));
-    hipLaunchKernelGGL(hybrid_output_final_kernel, dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr,
-                       p->d_mvals, p->d_lm_base, p->d_acoef, s, nf, co, nf_co, tmp, f,
-                       tmp ? one_over_r : s, tmp ? one_over_rsq : s, tmp ? two_over_r : s);
+    // the hydro-only instance reads no profile: s stands in for them
+    hipLaunchKernelGGL(tmp ? hybrid_output_final_kernel<true> : hybrid_output_final_kernel<false>,
+                       dim3(bx, (unsigned)p->nm), dim3(kAThreads), 0, st, p->lmax, p->nr, p->d_mvals, p->d_lm_base,
+                       p->d_acoef, s, nf, co, nf_co, tmp, f, tmp ? one_over_r : s, tmp ? one_over_rsq : s,
+                       tmp ? two_over_r : s);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
