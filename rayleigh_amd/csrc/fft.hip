@@ -602,7 +602,7 @@ __global__ __launch_bounds__(fft3_threads<N>()) void fft3_r2c_kernel(long howman
 // X_{N-k} in place in its line's LDS buffer, and the workgroup then writes the m <= l_max
 // coefficients of its TB lines to the hybrid rows p2[mp][c][theta], TB consecutive theta of
 // every mode q = 2m + imi at a time.  The padded grid is read once and the separate pass
-// that read it back for 3b -> 2b is gone (1.26 ms against 0.94 + 0.67 ms at l_max = 255,
+// that read it back for 3b -> 2b is gone (1.16 ms against 0.94 + 0.67 ms at l_max = 255,
 // N_r = 128, 8 fields); the X arithmetic is fft3_r2c_kernel's own, so the results are
 // identical.  (The mirror fusion for c2r, gathering the hybrid rows through an LDS tile,
 // measured slower than the 2a -> 3a kernel plus c2r: one wave per 31 KB workgroup left
@@ -692,11 +692,24 @@ __global__ __launch_bounds__(hyb_lines<N>() * (N / kP), 2) void fft3_r2c_hyb_ker
     const int nq = 2 * (a.lmax + 1);
     const long nrhs = 2L * a.nr * a.nf;
     const double *xs = hyb_lds;  // line i's buffer at i STRIDE complex
-    for (int idx = tid; idx < nq * TB; idx += NTH) {
-        const int q = idx / TB, i = idx - q * TB, th2 = t0 + i, mp = a.mp_of_m[q >> 1];
-        if (mp >= 0 && th2 < a.nt)
-            p2[((long)mp * nrhs + r + (long)(q & 1) * a.nr + 2L * a.nr * f) * a.nt + th2] =
-                xs[2 * ((long)i * STRIDE + fft3_pad(q >> 1)) + (q & 1)];
+    // eight stores per thread at a time, their m-block indices loaded together first, so the
+    // loop waits for one round of index loads per eight stores rather than one per store
+    constexpr int U = 8;
+    const int nidx = nq * TB;
+    for (int base = tid; base < nidx; base += U * NTH) {
+        int mpv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int idx = base + u * NTH;
+            mpv[u] = idx < nidx ? a.mp_of_m[(idx / TB) >> 1] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int idx = base + u * NTH, q = idx / TB, i = idx - q * TB, th2 = t0 + i;
+            if (mpv[u] >= 0 && th2 < a.nt)
+                p2[((long)mpv[u] * nrhs + r + (long)(q & 1) * a.nr + 2L * a.nr * f) * a.nt + th2] =
+                    xs[2 * ((long)i * STRIDE + fft3_pad(q >> 1)) + (q & 1)];
+        }
     }
 }
 
