@@ -536,11 +536,37 @@ __global__ __launch_bounds__(kLuThreads) void band_lu_decompose_kernel(int n, in
                 for (int i = 1 + tid; i <= km; i += kLuThreads) AB(kv + 1 + i, j) = r * AB(kv + 1 + i, j);
                 for (int c = 1 + tid; c <= ju - j; c += kLuThreads) s_y[c] = AB(kv + 1 - c, j + c);
                 __syncthreads();
-                const int w = ju - j;
-                for (int e = tid; e < km * w; e += kLuThreads) {
-                    const int i = 1 + e % km, c = 1 + e / km;
-                    const double y = s_y[c];
-                    if (y != 0.0) AB(kv + 1 + i - c, j + c) = AB(kv + 1 + i - c, j + c) + AB(kv + 1 + i, j) * (-y);
+                // element e = (i - 1) + km (c - 1), e = tid, tid + kLuThreads, ...: i and c stepped
+                // without divisions, four elements per thread at a time with their operands (row
+                // j and column j, which the update does not write) read before any is written
+                const int w = ju - j, tot = km * w, di = kLuThreads % km, dc = kLuThreads / km;
+                int i = 1 + tid % km, c = 1 + tid / km;
+                constexpr int U = 4;
+                for (int e = tid; e < tot; e += U * kLuThreads) {
+                    int iu[U], cu[U];
+                    bool on[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        on[u] = e + u * kLuThreads < tot;
+                        iu[u] = on[u] ? i : 1;
+                        cu[u] = on[u] ? c : 1;
+                        i += di;
+                        c += dc;
+                        if (i > km) {
+                            i -= km;
+                            ++c;
+                        }
+                    }
+                    double y[U], a[U], x[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        y[u] = s_y[cu[u]];
+                        a[u] = AB(kv + 1 + iu[u] - cu[u], j + cu[u]);
+                        x[u] = AB(kv + 1 + iu[u], j);
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (on[u] && y[u] != 0.0) AB(kv + 1 + iu[u] - cu[u], j + cu[u]) = a[u] + x[u] * (-y[u]);
                 }
                 __syncthreads();
             }
@@ -549,6 +575,161 @@ __global__ __launch_bounds__(kLuThreads) void band_lu_decompose_kernel(int n, in
         }
     }
     if (tid == 0) infos[blockIdx.x] = info;
+#undef AB
+}
+
+// The same factorisation with its active window in LDS.  Step j of dgbtf2 reads and writes
+// only matrix rows j .. j + kl and columns j .. j + kv (the pivot search, the interchange
+// over columns j .. ju <= j + kv, the multipliers and the rank-1 update), so that window,
+// (kl + 1) x (kv + 1) values, lives in an LDS ring (matrix row % (kl + 1), column % (kv + 1),
+// column-major) while the factorisation walks down the band: after step j, row j (U) and
+// column j (L) are final and go to the band storage, and step j + 1 brings in row j + kl + 1
+// (plain band entries) and column j + kv + 1 (rows j + 1 .. j + kl: dgbtf2's zeroed fill-in,
+// so zeros are written, not read).  The arithmetic, its order and the pivot choice are the
+// global-memory kernel's, statement for statement, so the factors and pivots are the same
+// doubles; the difference is that the rank-1 updates no longer read and write the band
+// storage through the caches on every step.  Used where the window and the reduction
+// buffers fit one workgroup's LDS.
+__global__ __launch_bounds__(kLuThreads) void band_lu_decompose_lds_kernel(int n, int kl, int ku, int ldab,
+                                                                           double *__restrict__ mats,
+                                                                           int *__restrict__ pivots,
+                                                                           int *__restrict__ infos)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double win[];
+    __shared__ double s_val[kLuThreads];
+    __shared__ int s_idx[kLuThreads];
+    const int tid = threadIdx.x;
+    double *ab = mats + (long)blockIdx.x * ldab * n;
+    int *piv = pivots + (long)blockIdx.x * n;
+    const int kv = ku + kl, R = kl + 1, C = kv + 1;
+#define AB(i, j) ab[(long)((j) - 1) * ldab + ((i) - 1)]
+#define W(row, col) win[((col) % C) * R + ((row) % R)]
+    // the first window, rows 1 .. min(R, n) x columns 1 .. min(C, n); entries more than ku
+    // above the diagonal are fill-in, zero at the start
+    {
+        const int nrw = min(R, n), ncl = min(C, n);
+        for (int e = tid; e < nrw * ncl; e += kLuThreads) {
+            const int row = 1 + e % nrw, col = 1 + e / nrw;
+            W(row, col) = col - row > ku ? 0.0 : AB(kv + 1 + row - col, col);
+        }
+    }
+    __syncthreads();
+    int info = 0, ju = 1;
+    for (int j = 1; j <= n; ++j) {
+        if (j > 1) {
+            // row j + kl (columns j .. j + kv) and column j + kv (rows j .. j + kl - 1) into the
+            // slots of row and column j - 1, written out at the end of the previous step
+            const int rn = j + kl, cn = j + kv;
+            if (rn <= n)
+                for (int col = j + tid; col <= min(cn, n); col += kLuThreads) W(rn, col) = AB(kv + 1 + rn - col, col);
+            if (cn <= n)
+                for (int row = j + tid; row <= min(rn - 1, n); row += kLuThreads) W(row, cn) = 0.0;
+            __syncthreads();
+        }
+        const int km = min(kl, n - j);
+        double best = -1.0;
+        int bi = 1 << 30;
+        for (int i = 1 + tid; i <= km + 1; i += kLuThreads) {
+            const double v = fabs(W(j + i - 1, j));
+            if (v > best) {
+                best = v;
+                bi = i;
+            }
+        }
+        // the largest |a| with the lowest index among equals: a total order on the candidates
+        // (NaNs never enter), so the shuffle and cross-wave order cannot change the choice
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double v = __shfl_xor(best, off);
+            const int q = __shfl_xor(bi, off);
+            if (v > best || (v == best && q < bi)) {
+                best = v;
+                bi = q;
+            }
+        }
+        if ((tid & 63) == 0) {
+            s_val[tid >> 6] = best;
+            s_idx[tid >> 6] = bi;
+        }
+        __syncthreads();
+        best = s_val[0];
+        bi = s_idx[0];
+#pragma unroll
+        for (int w = 1; w < kLuThreads / 64; ++w)
+            if (s_val[w] > best || (s_val[w] == best && s_idx[w] < bi)) {
+                best = s_val[w];
+                bi = s_idx[w];
+            }
+        // idamax: a NaN first entry is kept (nothing compares larger); later NaNs are passed over
+        const int jp = (bi > km + 1 || isnan(W(j, j))) ? 1 : bi;
+        const double pv = W(j + jp - 1, j);
+        __syncthreads();  // every thread has the pivot before the interchange moves it
+        if (tid == 0) piv[j - 1] = jp + j - 1;
+        if (pv != 0.0) {
+            ju = max(ju, min(j + ku + jp - 1, n));
+            if (jp != 1)
+                for (int c = tid; c <= ju - j; c += kLuThreads) {
+                    const double t = W(j + jp - 1, j + c);
+                    W(j + jp - 1, j + c) = W(j, j + c);
+                    W(j, j + c) = t;
+                }
+            __syncthreads();
+            if (km > 0) {
+                const double r = 1.0 / W(j, j);
+                for (int i = 1 + tid; i <= km; i += kLuThreads) W(j + i, j) = r * W(j + i, j);
+                __syncthreads();
+                // element e = (i - 1) + km (c - 1) of the update, e = tid, tid + kLuThreads, ...:
+                // i, c and their ring slots stepped by kLuThreads elements without divisions
+                const int w = ju - j, di = kLuThreads % km, dc = (kLuThreads / km) % C;
+                int ii = tid % km, rs = (j + 1 + ii) % R, cs = (j + 1 + tid / km) % C;
+                const int js = j % R, jc = (j % C) * R, tot = km * w;
+                // four elements per thread at a time: their operands (row j and column j are
+                // not written by the update) are read before any of the four is written
+                constexpr int U = 4;
+                for (int e = tid; e < tot; e += U * kLuThreads) {
+                    int rsu[U], csu[U];
+                    bool on[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        on[u] = e + u * kLuThreads < tot;
+                        rsu[u] = on[u] ? rs : 0;
+                        csu[u] = on[u] ? cs : 0;
+                        ii += di;
+                        rs += di;
+                        if (rs >= R) rs -= R;
+                        cs += dc;
+                        if (ii >= km) {
+                            ii -= km;
+                            rs -= km;
+                            if (rs < 0) rs += R;
+                            ++cs;
+                        }
+                        if (cs >= C) cs -= C;
+                    }
+                    double y[U], a[U], x[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        y[u] = win[csu[u] * R + js];
+                        a[u] = win[csu[u] * R + rsu[u]];
+                        x[u] = win[jc + rsu[u]];
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (on[u] && y[u] != 0.0) win[csu[u] * R + rsu[u]] = a[u] + x[u] * (-y[u]);
+                }
+                __syncthreads();
+            }
+        } else if (info == 0) {
+            info = j;
+        }
+        // row j (columns j .. j + kv) and column j (rows j + 1 .. j + kl) are final
+        for (int col = j + tid; col <= min(j + kv, n); col += kLuThreads) AB(kv + 1 + j - col, col) = W(j, col);
+        for (int row = j + 1 + tid; row <= min(j + kl, n); row += kLuThreads) AB(kv + 1 + row - j, j) = W(row, j);
+        __syncthreads();
+    }
+    if (tid == 0) infos[blockIdx.x] = info;
+#undef W
 #undef AB
 }
 
@@ -762,8 +943,17 @@ int raysht_lu_decompose_band(int n, int kl, int ku, int nbatch, double *mats, in
     if (nbatch > 0 && (!mats || !pivots || !info)) return RAYSHT_EINVAL;
     if (kl + ku > kBandMaxKv) return RAYSHT_EUNSUPPORTED;
     if (nbatch == 0) return RAYSHT_OK;
-    hipLaunchKernelGGL(band_lu_decompose_kernel, dim3((unsigned)nbatch), dim3(kLuThreads), 0,
-                       reinterpret_cast<hipStream_t>(stream), n, kl, ku, ldab, mats, pivots, info);
+    // the LDS-window kernel where its window and reduction buffers fit one workgroup's LDS
+    const size_t win = sizeof(double) * (size_t)(kl + 1) * (kl + ku + 1);
+    int dev = 0, lds_max = 0;
+    RAYSHT_HIP_TRY(hipGetDevice(&dev));
+    RAYSHT_HIP_TRY(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (win + kLuThreads * (sizeof(double) + sizeof(int)) <= (size_t)lds_max)
+        hipLaunchKernelGGL(band_lu_decompose_lds_kernel, dim3((unsigned)nbatch), dim3(kLuThreads), win,
+                           reinterpret_cast<hipStream_t>(stream), n, kl, ku, ldab, mats, pivots, info);
+    else
+        hipLaunchKernelGGL(band_lu_decompose_kernel, dim3((unsigned)nbatch), dim3(kLuThreads), 0,
+                           reinterpret_cast<hipStream_t>(stream), n, kl, ku, ldab, mats, pivots, info);
     RAYSHT_HIP_TRY(hipGetLastError());
     return RAYSHT_OK;
 }
